@@ -1,0 +1,272 @@
+"""ctypes declarations of the C ABI (include/ff/ff_abi.h) and thin wrappers.
+
+Device buffers are torch tensors on the GPU (PyTorch is only used for device
+memory and streams); every compute call goes through libff_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+
+FF_OK = 0
+FF_MAT_FLOATS = 16
+FF_MAT_MIX, FF_MAT_DIFFUSE, FF_MAT_CONDUCTOR, FF_MAT_DIELECTRIC = 0, 1, 2, 3
+FF_RENDER_USE_GRAPH = 1
+
+NODE_DTYPE = np.dtype([("bmin", "<f4", 3), ("bmax", "<f4", 3), ("offset", "<i4"),
+                       ("n_prims", "<u2"), ("axis", "u1"), ("pad", "u1")])
+assert NODE_DTYPE.itemsize == 32
+
+
+class FFError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"ff status {status}: {msg}")
+        self.status = status
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("n_tris", C.c_int32), ("positions", C.c_void_p), ("normals", C.c_void_p),
+                ("uvs", C.c_void_p), ("material_ids", C.c_void_p), ("n_materials", C.c_int32),
+                ("materials", C.c_void_p), ("n_textures", C.c_int32), ("textures", C.c_void_p),
+                ("tex_w", C.c_void_p), ("tex_h", C.c_void_p), ("tex_c", C.c_void_p),
+                ("max_prims_in_node", C.c_int32)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("inv_view_proj", C.c_float * 16), ("lens_radius", C.c_float),
+                ("focal_distance", C.c_float), ("exposure", C.c_float), ("pad", C.c_float)]
+
+
+class RenderReq(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32),
+                ("max_depth", C.c_int32), ("seed", C.c_uint32), ("spp_batch", C.c_int32),
+                ("sample_offset", C.c_int32), ("flags", C.c_int32)]
+
+
+class RayStats(C.Structure):
+    _fields_ = [("rays_primary", C.c_double), ("rays_extension", C.c_double),
+                ("rays_shadow", C.c_double), ("paths", C.c_double),
+                ("nodes_tested", C.c_double), ("tris_tested", C.c_double)]
+
+
+_P = C.c_void_p
+_PROTOS = {
+    "ff_last_error": (C.c_char_p, []),
+    "ff_abi_version": (C.c_int32, []),
+    "ff_context_create": (C.c_int, [C.c_int32, C.POINTER(_P)]),
+    "ff_context_destroy": (C.c_int, [_P]),
+    "ff_bvh_build_host": (C.c_int, [_P, C.c_int32, C.c_int32, _P, _P, C.POINTER(C.c_int32),
+                                    C.POINTER(C.c_int32)]),
+    "ff_scene_create": (C.c_int, [_P, C.POINTER(SceneDesc), C.POINTER(_P)]),
+    "ff_scene_destroy": (C.c_int, [_P]),
+    "ff_scene_info": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                C.POINTER(C.c_int32)]),
+    "ff_scene_get_bvh": (C.c_int, [_P, _P, _P]),
+    "ff_envmap_create": (C.c_int, [_P, _P, C.c_int32, C.c_int32, C.POINTER(_P)]),
+    "ff_envmap_read": (C.c_int, [_P, _P, _P, _P, _P, _P]),
+    "ff_envmap_destroy": (C.c_int, [_P]),
+    "ff_trace_closest": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_int32, _P]),
+    "ff_trace_any": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
+    "ff_generate_rays": (C.c_int, [_P, C.POINTER(Camera), C.c_int32, C.c_int32, C.c_uint32,
+                                   C.c_int32, _P, _P]),
+    "ff_render": (C.c_int, [_P, _P, _P, C.POINTER(Camera), C.POINTER(RenderReq), _P, _P]),
+    "ff_last_render_stats": (C.c_int, [_P, C.POINTER(RayStats)]),
+    "ff_film_resolve": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_float, _P, _P, _P]),
+}
+
+EXPORTED = sorted(_PROTOS)
+_L: Optional[C.CDLL] = None
+
+
+def load(path: str) -> C.CDLL:
+    global _L
+    L = C.CDLL(path)
+    for name, (res, args) in _PROTOS.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _L = L
+    return L
+
+
+def _lib() -> C.CDLL:
+    if _L is None:
+        from . import lib
+        lib()
+    return _L
+
+
+def check(status: int) -> None:
+    if status != FF_OK:
+        raise FFError(status, _lib().ff_last_error().decode())
+
+
+def _ptr(a) -> int:
+    """Address of a numpy array or torch tensor."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()
+
+
+def bvh_build_host(positions: np.ndarray, max_prims: int = 8):
+    """Host SAH BVH (no GPU): returns (nodes structured array, ordered, max_depth)."""
+    pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 9)
+    n = pos.shape[0]
+    nodes = np.zeros(max(2 * n - 1, 1), dtype=NODE_DTYPE)
+    ordered = np.zeros(n, dtype=np.int32)
+    nn, md = C.c_int32(), C.c_int32()
+    check(_lib().ff_bvh_build_host(pos.ctypes.data, n, max_prims, nodes.ctypes.data,
+                                   ordered.ctypes.data, C.byref(nn), C.byref(md)))
+    return nodes[: nn.value].copy(), ordered, md.value
+
+
+def make_camera(inv_vp_colmajor: Sequence[float], lens_radius=0.0, focal=1.0, exposure=1.0) -> Camera:
+    cam = Camera()
+    for i, v in enumerate(np.asarray(inv_vp_colmajor, dtype=np.float32).reshape(16)):
+        cam.inv_view_proj[i] = float(v)
+    cam.lens_radius = float(lens_radius)
+    cam.focal_distance = float(focal)
+    cam.exposure = float(exposure)
+    return cam
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = _P()
+        check(_lib().ff_context_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            check(_lib().ff_context_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    """Device scene built from host arrays (positions/normals/uvs [n,3,...])."""
+
+    def __init__(self, ctx: Context, positions, normals, uvs, material_ids, materials,
+                 textures: Sequence[np.ndarray] = (), max_prims: int = 8):
+        self.ctx = ctx
+        self._keep = []
+        pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 9)
+        nrm = np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 9)
+        uv = None if uvs is None else np.ascontiguousarray(uvs, dtype=np.float32).reshape(-1, 6)
+        mid = np.ascontiguousarray(material_ids, dtype=np.int32).reshape(-1)
+        mats = np.ascontiguousarray(materials, dtype=np.float32).reshape(-1, FF_MAT_FLOATS)
+        texs = [np.ascontiguousarray(t, dtype=np.float32) for t in textures]
+        d = SceneDesc()
+        d.n_tris = pos.shape[0]
+        d.positions, d.normals = pos.ctypes.data, nrm.ctypes.data
+        d.uvs = None if uv is None else uv.ctypes.data
+        d.material_ids = mid.ctypes.data
+        d.n_materials = mats.shape[0]
+        d.materials = mats.ctypes.data
+        d.n_textures = len(texs)
+        if texs:
+            ptrs = (C.c_void_p * len(texs))(*[t.ctypes.data for t in texs])
+            tw = np.array([t.shape[1] for t in texs], dtype=np.int32)
+            th = np.array([t.shape[0] for t in texs], dtype=np.int32)
+            tc = np.array([t.shape[2] if t.ndim == 3 else 1 for t in texs], dtype=np.int32)
+            d.textures = C.cast(ptrs, C.c_void_p)
+            d.tex_w, d.tex_h, d.tex_c = tw.ctypes.data, th.ctypes.data, tc.ctypes.data
+            self._keep += [ptrs, tw, th, tc]
+        d.max_prims_in_node = max_prims
+        self._keep += [pos, nrm, uv, mid, mats, texs]
+        h = _P()
+        check(_lib().ff_scene_create(ctx.h, C.byref(d), C.byref(h)))
+        self.h = h
+        self._keep = []
+        nt, nn, md = C.c_int32(), C.c_int32(), C.c_int32()
+        check(_lib().ff_scene_info(h, C.byref(nt), C.byref(nn), C.byref(md)))
+        self.n_tris, self.n_nodes, self.max_depth = nt.value, nn.value, md.value
+
+    def bvh(self):
+        nodes = np.zeros(self.n_nodes, dtype=NODE_DTYPE)
+        ordered = np.zeros(self.n_tris, dtype=np.int32)
+        check(_lib().ff_scene_get_bvh(self.h, nodes.ctypes.data, ordered.ctypes.data))
+        return nodes, ordered
+
+    def trace_closest(self, rays_dev, hits_dev, two_sided=False, stream=None):
+        n = rays_dev.shape[0]
+        check(_lib().ff_trace_closest(self.h, _ptr(rays_dev), n, _ptr(hits_dev), int(two_sided), stream))
+
+    def trace_any(self, rays_dev, occ_dev, stream=None):
+        n = rays_dev.shape[0]
+        check(_lib().ff_trace_any(self.h, _ptr(rays_dev), n, _ptr(occ_dev), stream))
+
+    def close(self):
+        if self.h:
+            check(_lib().ff_scene_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class EnvMap:
+    def __init__(self, ctx: Context, rgb: np.ndarray):
+        a = np.ascontiguousarray(rgb, dtype=np.float32)
+        h_, w_ = a.shape[0], a.shape[1]
+        h = _P()
+        check(_lib().ff_envmap_create(ctx.h, a.ctypes.data, w_, h_, C.byref(h)))
+        self.h, self.w, self.hgt = h, w_, h_
+
+    def read(self):
+        my = np.zeros(self.hgt, np.float64)
+        mp = np.zeros(self.hgt, np.float64)
+        conds = np.zeros((self.hgt, self.w), np.float64)
+        pdf = np.zeros((self.hgt, self.w), np.float32)
+        denom = np.zeros(1, np.float64)
+        check(_lib().ff_envmap_read(self.h, my.ctypes.data, mp.ctypes.data, conds.ctypes.data,
+                                    pdf.ctypes.data, denom.ctypes.data))
+        return my, mp, conds, pdf, float(denom[0])
+
+    def close(self):
+        if self.h:
+            check(_lib().ff_envmap_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def render(ctx: Context, scene: Scene, env: EnvMap, cam: Camera, width: int, height: int,
+           spp: int, max_depth: int, seed: int, film_dev, spp_batch: int = 1,
+           sample_offset: int = 0, flags: int = 0, stream=None) -> None:
+    r = RenderReq(width, height, spp, max_depth, seed, spp_batch, sample_offset, flags)
+    check(_lib().ff_render(ctx.h, scene.h, env.h, C.byref(cam), C.byref(r), _ptr(film_dev), stream))
+
+
+def last_render_stats(ctx: Context) -> RayStats:
+    s = RayStats()
+    check(_lib().ff_last_render_stats(ctx.h, C.byref(s)))
+    return s
+
+
+def generate_rays(ctx: Context, cam: Camera, width: int, height: int, seed: int, sample: int,
+                  rays_dev, stream=None) -> None:
+    check(_lib().ff_generate_rays(ctx.h, C.byref(cam), width, height, seed, sample, _ptr(rays_dev), stream))
+
+
+def film_resolve(film_dev, width: int, height: int, exposure: float, avg_dev=None, rgba_dev=None,
+                 stream=None) -> None:
+    check(_lib().ff_film_resolve(_ptr(film_dev), width, height, exposure, _ptr(avg_dev),
+                                 _ptr(rgba_dev), stream))

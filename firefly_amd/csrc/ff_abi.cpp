@@ -1,0 +1,491 @@
+// ff_abi.cpp -- implementation of the C ABI in include/ff/ff_abi.h.
+//
+// Host side of the wavefront tracer: scene upload (host SAH BVH, triangles
+// reordered into BVH-slot order), the FP64 environment CDF build, and the
+// per-request bounce loop that replaces PathTracer::draw ->
+// wavefront_pathtrace (CRT/PathTracer.cpp:59-79, CRT/kernel.cu:658-676).
+// Errors are reported as ff_status plus a thread-local message; nothing here
+// exits the process (the reference's checkCudaErrors exits, CRT/CudaHelpers.cpp:3-11).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ff/ff_abi.h"
+#include "ff_internal.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+ff_status fail(ff_status st, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return st;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(e_ == hipErrorOutOfMemory ? FF_ERR_ALLOC : FF_ERR_HIP, "%s: %s (%s:%d)", \
+                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                     \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc((void**)p, sizeof(T) * count);
+}
+template <typename T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+struct PathStore {
+    size_t cap = 0;
+    ff_paths p{};
+    void release() {
+        dfree(p.ray); dfree(p.hit); dfree(p.beta); dfree(p.Lp); dfree(p.fext); dfree(p.nee);
+        dfree(p.sray); dfree(p.vis); dfree(p.rngb); dfree(p.q_ext[0]); dfree(p.q_ext[1]);
+        for (int m = 0; m < 4; m++) dfree(p.q_mat[m]);
+        dfree(p.q_shadow); dfree(p.layers);
+        cap = 0;
+    }
+};
+
+}  // namespace
+
+struct ff_context {
+    int device = 0;
+    PathStore paths;
+    uint32_t* counters = nullptr;
+    unsigned long long* stats = nullptr;
+    int32_t* ovf = nullptr;
+    size_t ovf_entries = 0;  // stack entries per grid thread
+    hipStream_t last_stream = nullptr;
+    bool have_stats = false;
+};
+
+struct ff_scene {
+    ff_context* ctx = nullptr;
+    int32_t n_tris = 0, n_mats = 0, max_depth = 0;
+    std::vector<ffh::BvhNode> nodes;
+    std::vector<int32_t> ordered;
+    float4* d_nodes = nullptr;
+    float* d_tris = nullptr;
+    float* d_nrm = nullptr;
+    float* d_uv = nullptr;
+    int32_t* d_mat = nullptr;
+    float* d_mats = nullptr;
+    std::vector<float*> d_tex;
+    ff_tex_desc* d_tex_desc = nullptr;
+    int32_t n_tex = 0;
+    bool has_type[4] = {false, false, false, false};
+    ff_dev_scene view() const {
+        ff_dev_scene s;
+        s.nodes = d_nodes; s.tris = d_tris; s.nrm = d_nrm; s.uv = d_uv; s.mat = d_mat;
+        s.mats = d_mats; s.tex = d_tex_desc; s.n_nodes = (int32_t)nodes.size(); s.n_tris = n_tris;
+        s.n_mats = n_mats; s.n_tex = n_tex; s.max_depth = max_depth;
+        return s;
+    }
+};
+
+struct ff_envmap {
+    int32_t w = 0, h = 0;
+    float* rgb = nullptr;
+    float* lum = nullptr;
+    double* row_sum = nullptr;
+    double* denom = nullptr;
+    double* marg_p = nullptr;
+    double* marg_y = nullptr;
+    double* conds = nullptr;
+    float* pdf = nullptr;
+    ff_dev_env view() const {
+        ff_dev_env e;
+        e.rgb = rgb; e.marg_y = marg_y; e.conds = conds; e.pdf = pdf; e.w = w; e.h = h;
+        return e;
+    }
+};
+
+namespace {
+
+ff_status ensure_paths(ff_context* c, size_t P) {
+    if (c->paths.cap >= P) return FF_OK;
+    c->paths.release();
+    ff_paths& p = c->paths.p;
+    HIP_TRY(dalloc(&p.ray, 2 * P));
+    HIP_TRY(dalloc(&p.hit, P));
+    HIP_TRY(dalloc(&p.beta, P));
+    HIP_TRY(dalloc(&p.Lp, P));
+    HIP_TRY(dalloc(&p.fext, P));
+    HIP_TRY(dalloc(&p.nee, 3 * P));
+    HIP_TRY(dalloc(&p.sray, 6 * P));
+    HIP_TRY(dalloc(&p.vis, 3 * P));
+    HIP_TRY(dalloc(&p.rngb, P));
+    HIP_TRY(dalloc(&p.q_ext[0], P));
+    HIP_TRY(dalloc(&p.q_ext[1], P));
+    for (int m = 0; m < 4; m++) HIP_TRY(dalloc(&p.q_mat[m], P));
+    HIP_TRY(dalloc(&p.q_shadow, 3 * P));
+    HIP_TRY(dalloc(&p.layers, P));
+    c->paths.cap = P;
+    return FF_OK;
+}
+
+// traversal stack overflow area: entries beyond FF_LDS_STACK per grid thread
+ff_status ensure_ovf(ff_context* c, int32_t max_depth) {
+    size_t need = max_depth > FF_LDS_STACK ? (size_t)(max_depth - FF_LDS_STACK + 1) : 1;
+    if (c->ovf && c->ovf_entries >= need) return FF_OK;
+    dfree(c->ovf);
+    HIP_TRY(dalloc(&c->ovf, need * (size_t)FF_TRAV_GRID * FF_BLOCK));
+    c->ovf_entries = need;
+    return FF_OK;
+}
+
+ff_dev_camera dev_camera(const ff_camera* cam) {
+    ff_dev_camera d;
+    std::memcpy(d.inv_vp, cam->inv_view_proj, sizeof(d.inv_vp));
+    d.lens_radius = cam->lens_radius;
+    d.focal = cam->focal_distance;
+    d.exposure = cam->exposure;
+    d.pad = 0.f;
+    return d;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ff_last_error(void) { return g_err.c_str(); }
+int32_t ff_abi_version(void) { return FF_ABI_VERSION; }
+
+ff_status ff_context_create(int32_t device, ff_context** out) {
+    if (!out) return fail(FF_ERR_INVALID, "ff_context_create: out is NULL");
+    *out = nullptr;
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(FF_ERR_INVALID, "ff_context_create: device %d of %d", device, n);
+    HIP_TRY(hipSetDevice(device));
+    ff_context* c = new ff_context();
+    c->device = device;
+    if (dalloc(&c->counters, C_COUNT) != hipSuccess || dalloc(&c->stats, S_COUNT) != hipSuccess) {
+        dfree(c->counters);
+        dfree(c->stats);
+        delete c;
+        return fail(FF_ERR_ALLOC, "ff_context_create: counter allocation failed");
+    }
+    *out = c;
+    return FF_OK;
+}
+
+ff_status ff_context_destroy(ff_context* c) {
+    if (!c) return fail(FF_ERR_INVALID, "ff_context_destroy: NULL context");
+    (void)hipSetDevice(c->device);
+    (void)hipDeviceSynchronize();
+    c->paths.release();
+    dfree(c->counters);
+    dfree(c->stats);
+    dfree(c->ovf);
+    delete c;
+    return FF_OK;
+}
+
+ff_status ff_bvh_build_host(const float* positions, int32_t n_tris, int32_t max_prims, ff_bvh_node* nodes_out,
+                            int32_t* ordered_out, int32_t* n_nodes_out, int32_t* max_depth_out) {
+    if (!positions || n_tris <= 0 || max_prims <= 0)
+        return fail(FF_ERR_INVALID, "ff_bvh_build_host: need positions, n_tris > 0, max_prims > 0");
+    std::vector<ffh::BvhNode> nodes;
+    std::vector<int32_t> ordered;
+    ffh::build_bvh_sah(positions, n_tris, max_prims, nodes, ordered);
+    if (nodes_out) std::memcpy(nodes_out, nodes.data(), nodes.size() * sizeof(ffh::BvhNode));
+    if (ordered_out) std::memcpy(ordered_out, ordered.data(), ordered.size() * sizeof(int32_t));
+    if (n_nodes_out) *n_nodes_out = (int32_t)nodes.size();
+    if (max_depth_out) *max_depth_out = ffh::bvh_max_depth(nodes);
+    return FF_OK;
+}
+
+ff_status ff_scene_create(ff_context* ctx, const ff_scene_desc* d, ff_scene** out) {
+    if (!ctx || !d || !out) return fail(FF_ERR_INVALID, "ff_scene_create: NULL argument");
+    *out = nullptr;
+    if (d->n_tris <= 0 || !d->positions || !d->normals || !d->material_ids)
+        return fail(FF_ERR_INVALID, "ff_scene_create: empty scene or missing positions/normals/material ids");
+    if (d->n_materials <= 0 || !d->materials) return fail(FF_ERR_INVALID, "ff_scene_create: no materials");
+    if (d->n_tris > (1 << 30)) return fail(FF_ERR_CAPACITY, "ff_scene_create: %d triangles", d->n_tris);
+    for (int32_t i = 0; i < d->n_tris; i++)
+        if (d->material_ids[i] < 0 || d->material_ids[i] >= d->n_materials)
+            return fail(FF_ERR_INVALID, "ff_scene_create: triangle %d material %d out of range", i, d->material_ids[i]);
+    for (int32_t m = 0; m < d->n_materials; m++) {
+        const float* M = d->materials + (size_t)m * FF_MAT_FLOATS;
+        int type = (int)M[0];
+        if (type < 0 || type > 3) return fail(FF_ERR_INVALID, "ff_scene_create: material %d type %d", m, type);
+        for (int k = 10; k <= 11; k++) {
+            int t = (int)M[k];
+            if (t >= d->n_textures || t < -1)
+                return fail(FF_ERR_INVALID, "ff_scene_create: material %d texture %d out of range", m, t);
+            if (t >= 0 && d->tex_c[t] != (k == 10 ? 3 : 1))
+                return fail(FF_ERR_INVALID, "ff_scene_create: material %d texture %d has %d channels", m, t, d->tex_c[t]);
+        }
+    }
+    for (int32_t t = 0; t < d->n_textures; t++)
+        if (!d->textures || !d->textures[t] || d->tex_w[t] <= 0 || d->tex_h[t] <= 0 ||
+            (d->tex_c[t] != 1 && d->tex_c[t] != 3))
+            return fail(FF_ERR_INVALID, "ff_scene_create: texture %d invalid", t);
+    HIP_TRY(hipSetDevice(ctx->device));
+
+    ff_scene* s = new ff_scene();
+    s->ctx = ctx;
+    s->n_tris = d->n_tris;
+    s->n_mats = d->n_materials;
+    int32_t max_prims = d->max_prims_in_node > 0 ? d->max_prims_in_node : 8;
+    ffh::build_bvh_sah(d->positions, d->n_tris, max_prims, s->nodes, s->ordered);
+    s->max_depth = ffh::bvh_max_depth(s->nodes);
+    size_t n = (size_t)d->n_tris;
+    // reorder_primitives (CRT/kernel.cu:184-191,701-712) done on the host while
+    // packing the positions-only records v0 | e1 | e2.
+    std::vector<float> tris(9 * n), nrm(9 * n), uv(6 * n, 0.f);
+    std::vector<int32_t> mat(n);
+    for (size_t i = 0; i < n; i++) {
+        size_t src = (size_t)s->ordered[i];
+        const float* p = d->positions + src * 9;
+        float* t = &tris[i * 9];
+        for (int k = 0; k < 3; k++) {
+            t[k] = p[k];
+            t[3 + k] = p[3 + k] - p[k];
+            t[6 + k] = p[6 + k] - p[k];
+        }
+        std::memcpy(&nrm[i * 9], d->normals + src * 9, 36);
+        if (d->uvs) std::memcpy(&uv[i * 6], d->uvs + src * 6, 24);
+        mat[i] = d->material_ids[src];
+    }
+    for (int32_t m = 0; m < d->n_materials; m++) s->has_type[(int)d->materials[(size_t)m * FF_MAT_FLOATS]] = true;
+    auto cleanup = [&](ff_status st) { ff_scene_destroy(s); return st; };
+    if (dalloc(&s->d_nodes, 2 * s->nodes.size()) != hipSuccess || dalloc(&s->d_tris, 9 * n) != hipSuccess ||
+        dalloc(&s->d_nrm, 9 * n) != hipSuccess || dalloc(&s->d_uv, 6 * n) != hipSuccess ||
+        dalloc(&s->d_mat, n) != hipSuccess || dalloc(&s->d_mats, (size_t)FF_MAT_FLOATS * d->n_materials) != hipSuccess)
+        return cleanup(fail(FF_ERR_ALLOC, "ff_scene_create: device allocation failed"));
+    hipError_t e = hipMemcpy(s->d_nodes, s->nodes.data(), s->nodes.size() * 32, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->d_tris, tris.data(), 36 * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->d_nrm, nrm.data(), 36 * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->d_uv, uv.data(), 24 * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->d_mat, mat.data(), 4 * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(s->d_mats, d->materials, sizeof(float) * FF_MAT_FLOATS * d->n_materials, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return cleanup(fail(FF_ERR_HIP, "ff_scene_create: upload: %s", hipGetErrorString(e)));
+    s->n_tex = d->n_textures;
+    std::vector<ff_tex_desc> descs((size_t)std::max(1, d->n_textures));
+    for (int32_t t = 0; t < d->n_textures; t++) {
+        size_t cnt = (size_t)d->tex_w[t] * d->tex_h[t] * d->tex_c[t];
+        float* p = nullptr;
+        if (dalloc(&p, cnt) != hipSuccess) return cleanup(fail(FF_ERR_ALLOC, "ff_scene_create: texture alloc"));
+        s->d_tex.push_back(p);
+        if (hipMemcpy(p, d->textures[t], cnt * 4, hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(FF_ERR_HIP, "ff_scene_create: texture upload"));
+        descs[t].data = p; descs[t].w = d->tex_w[t]; descs[t].h = d->tex_h[t]; descs[t].c = d->tex_c[t]; descs[t].pad = 0;
+    }
+    if (dalloc(&s->d_tex_desc, descs.size()) != hipSuccess ||
+        hipMemcpy(s->d_tex_desc, descs.data(), descs.size() * sizeof(ff_tex_desc), hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(FF_ERR_ALLOC, "ff_scene_create: texture table"));
+    ff_status st = ensure_ovf(ctx, s->max_depth);
+    if (st != FF_OK) return cleanup(st);
+    *out = s;
+    return FF_OK;
+}
+
+ff_status ff_scene_destroy(ff_scene* s) {
+    if (!s) return fail(FF_ERR_INVALID, "ff_scene_destroy: NULL scene");
+    (void)hipDeviceSynchronize();
+    dfree(s->d_nodes); dfree(s->d_tris); dfree(s->d_nrm); dfree(s->d_uv); dfree(s->d_mat); dfree(s->d_mats);
+    for (auto& p : s->d_tex) dfree(p);
+    dfree(s->d_tex_desc);
+    delete s;
+    return FF_OK;
+}
+
+ff_status ff_scene_info(const ff_scene* s, int32_t* n_tris, int32_t* n_nodes, int32_t* max_depth) {
+    if (!s) return fail(FF_ERR_INVALID, "ff_scene_info: NULL scene");
+    if (n_tris) *n_tris = s->n_tris;
+    if (n_nodes) *n_nodes = (int32_t)s->nodes.size();
+    if (max_depth) *max_depth = s->max_depth;
+    return FF_OK;
+}
+
+ff_status ff_scene_get_bvh(const ff_scene* s, ff_bvh_node* nodes_out, int32_t* ordered_out) {
+    if (!s) return fail(FF_ERR_INVALID, "ff_scene_get_bvh: NULL scene");
+    if (nodes_out) std::memcpy(nodes_out, s->nodes.data(), s->nodes.size() * 32);
+    if (ordered_out) std::memcpy(ordered_out, s->ordered.data(), s->ordered.size() * 4);
+    return FF_OK;
+}
+
+ff_status ff_envmap_create(ff_context* ctx, const float* rgb, int32_t w, int32_t h, ff_envmap** out) {
+    if (!ctx || !rgb || !out) return fail(FF_ERR_INVALID, "ff_envmap_create: NULL argument");
+    *out = nullptr;
+    if (w < 2 || h < 2) return fail(FF_ERR_INVALID, "ff_envmap_create: map must be at least 2x2 (got %dx%d)", w, h);
+    double total = 0.0;
+    for (size_t i = 0; i < (size_t)w * h * 3; i++) {
+        if (!(rgb[i] >= 0.f) || rgb[i] > 3.0e38f)
+            return fail(FF_ERR_INVALID, "ff_envmap_create: texel value %zu is negative or not finite", i / 3);
+        total += rgb[i];
+    }
+    if (total == 0.0) return fail(FF_ERR_INVALID, "ff_envmap_create: all-zero map has no sampling distribution");
+    HIP_TRY(hipSetDevice(ctx->device));
+    ff_envmap* e = new ff_envmap();
+    e->w = w;
+    e->h = h;
+    size_t wh = (size_t)w * h;
+    if (dalloc(&e->rgb, 3 * wh) != hipSuccess || dalloc(&e->lum, wh) != hipSuccess ||
+        dalloc(&e->row_sum, h) != hipSuccess || dalloc(&e->denom, 1) != hipSuccess ||
+        dalloc(&e->marg_p, h) != hipSuccess || dalloc(&e->marg_y, h) != hipSuccess ||
+        dalloc(&e->conds, wh) != hipSuccess || dalloc(&e->pdf, wh) != hipSuccess) {
+        ff_envmap_destroy(e);
+        return fail(FF_ERR_ALLOC, "ff_envmap_create: device allocation failed");
+    }
+    hipError_t err = hipMemcpy(e->rgb, rgb, 12 * wh, hipMemcpyHostToDevice);
+    if (err == hipSuccess)
+        err = ffk::launch_env_build(e->rgb, w, h, e->lum, e->row_sum, e->denom, e->marg_p, e->marg_y, e->conds,
+                                    e->pdf, nullptr);
+    if (err == hipSuccess) err = hipDeviceSynchronize();
+    if (err != hipSuccess) {
+        ff_envmap_destroy(e);
+        return fail(FF_ERR_HIP, "ff_envmap_create: CDF build: %s", hipGetErrorString(err));
+    }
+    *out = e;
+    return FF_OK;
+}
+
+ff_status ff_envmap_read(const ff_envmap* e, double* marginal_y, double* marginal_p, double* conds, float* pdf,
+                         double* denom) {
+    if (!e) return fail(FF_ERR_INVALID, "ff_envmap_read: NULL env");
+    size_t wh = (size_t)e->w * e->h;
+    if (marginal_y) HIP_TRY(hipMemcpy(marginal_y, e->marg_y, 8 * (size_t)e->h, hipMemcpyDeviceToHost));
+    if (marginal_p) HIP_TRY(hipMemcpy(marginal_p, e->marg_p, 8 * (size_t)e->h, hipMemcpyDeviceToHost));
+    if (conds) HIP_TRY(hipMemcpy(conds, e->conds, 8 * wh, hipMemcpyDeviceToHost));
+    if (pdf) HIP_TRY(hipMemcpy(pdf, e->pdf, 4 * wh, hipMemcpyDeviceToHost));
+    if (denom) HIP_TRY(hipMemcpy(denom, e->denom, 8, hipMemcpyDeviceToHost));
+    return FF_OK;
+}
+
+ff_status ff_envmap_destroy(ff_envmap* e) {
+    if (!e) return fail(FF_ERR_INVALID, "ff_envmap_destroy: NULL env");
+    (void)hipDeviceSynchronize();
+    dfree(e->rgb); dfree(e->lum); dfree(e->row_sum); dfree(e->denom); dfree(e->marg_p); dfree(e->marg_y);
+    dfree(e->conds); dfree(e->pdf);
+    delete e;
+    return FF_OK;
+}
+
+ff_status ff_trace_closest(ff_scene* s, const float* rays, uint32_t n, float* hits, int32_t two_sided,
+                           void* stream) {
+    if (!s || (!rays && n) || (!hits && n)) return fail(FF_ERR_INVALID, "ff_trace_closest: NULL argument");
+    if (n == 0) return FF_OK;
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    HIP_TRY(ffk::launch_trace_closest(s->view(), (const float4*)rays, n, (float4*)hits, two_sided, s->ctx->ovf,
+                                      (hipStream_t)stream));
+    return FF_OK;
+}
+
+ff_status ff_trace_any(ff_scene* s, const float* rays, uint32_t n, uint8_t* occluded, void* stream) {
+    if (!s || (!rays && n) || (!occluded && n)) return fail(FF_ERR_INVALID, "ff_trace_any: NULL argument");
+    if (n == 0) return FF_OK;
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    HIP_TRY(ffk::launch_trace_any(s->view(), (const float4*)rays, n, occluded, s->ctx->ovf, (hipStream_t)stream));
+    return FF_OK;
+}
+
+ff_status ff_generate_rays(ff_context* ctx, const ff_camera* cam, int32_t width, int32_t height, uint32_t seed,
+                           int32_t sample, float* rays, void* stream) {
+    if (!ctx || !cam || !rays) return fail(FF_ERR_INVALID, "ff_generate_rays: NULL argument");
+    if (width <= 0 || height <= 0 || sample < 0) return fail(FF_ERR_INVALID, "ff_generate_rays: bad size");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(ffk::launch_gen_rays(dev_camera(cam), width, height, seed, (uint32_t)sample, (float4*)rays,
+                                 (hipStream_t)stream));
+    return FF_OK;
+}
+
+ff_status ff_render(ff_context* ctx, ff_scene* s, ff_envmap* env, const ff_camera* cam, const ff_render_req* r,
+                    float* film, void* stream) {
+    if (!ctx || !s || !env || !cam || !r || !film) return fail(FF_ERR_INVALID, "ff_render: NULL argument");
+    if (s->ctx != ctx) return fail(FF_ERR_INVALID, "ff_render: scene belongs to another context");
+    if (r->width <= 0 || r->height <= 0 || r->spp <= 0 || r->max_depth < 1 || r->max_depth > 250 ||
+        r->sample_offset < 0)
+        return fail(FF_ERR_INVALID, "ff_render: invalid request (%dx%d, spp %d, depth %d)", r->width, r->height,
+                    r->spp, r->max_depth);
+    uint64_t WH = (uint64_t)r->width * (uint64_t)r->height;
+    int32_t K = r->spp_batch > 0 ? std::min(r->spp_batch, r->spp) : 1;
+    uint64_t P = WH * (uint64_t)K;
+    if (P > (1ull << 30)) return fail(FF_ERR_CAPACITY, "ff_render: %llu paths in flight exceed 2^30", (unsigned long long)P);
+    HIP_TRY(hipSetDevice(ctx->device));
+    ff_status st = ensure_paths(ctx, (size_t)P);
+    if (st != FF_OK) return st;
+    st = ensure_ovf(ctx, s->max_depth);
+    if (st != FF_OK) return st;
+    hipStream_t hs = (hipStream_t)stream;
+    ff_paths p = ctx->paths.p;
+    p.counters = ctx->counters;
+    p.stats = ctx->stats;
+    p.ovf = ctx->ovf;
+    ff_dev_scene S = s->view();
+    ff_dev_env E = env->view();
+    ff_dev_camera C = dev_camera(cam);
+    HIP_TRY(ffk::launch_fill4(p.layers, (uint32_t)P, hs));
+    HIP_TRY(ffk::launch_fill_u64(p.stats, S_COUNT, hs));
+    const uint32_t all = (1u << C_COUNT) - 1u;
+    for (int32_t b = 0; (int64_t)b * K < r->spp; b++) {
+        int32_t k_act = std::min<int32_t>(K, r->spp - b * K);
+        uint32_t n_act = (uint32_t)(WH * (uint64_t)k_act);
+        HIP_TRY(ffk::launch_reset(p.counters, all, hs));
+        HIP_TRY(ffk::launch_generate(C, r->width, r->height, n_act, (uint32_t)(r->sample_offset + b * K), r->seed,
+                                     p, 0, hs));
+        int cur = 0;
+        for (int32_t it = 1; it <= r->max_depth + 1; it++) {
+            HIP_TRY(ffk::launch_extend(S, p, cur, hs));
+            HIP_TRY(ffk::launch_shadow(S, p, hs));
+            uint32_t mask = (1u << (cur == 0 ? C_EXT1 : C_EXT0)) | (1u << C_SHADOW) | (0xfu << C_MAT0);
+            HIP_TRY(ffk::launch_reset(p.counters, mask, hs));
+            HIP_TRY(ffk::launch_logic(S, E, p, cur, (uint32_t)r->max_depth, n_act, hs));
+            if (it <= r->max_depth) {
+                for (int m = 0; m < 4; m++)
+                    if (s->has_type[m]) HIP_TRY(ffk::launch_shade(m, S, E, p, 1 - cur, n_act, hs));
+            }
+            cur = 1 - cur;
+        }
+    }
+    HIP_TRY(ffk::launch_resolve_layers(p.layers, (uint32_t)K, (uint32_t)WH, (float4*)film, (float)r->spp, hs));
+    ctx->last_stream = hs;
+    ctx->have_stats = true;
+    return FF_OK;
+}
+
+ff_status ff_last_render_stats(ff_context* ctx, ff_ray_stats* out) {
+    if (!ctx || !out) return fail(FF_ERR_INVALID, "ff_last_render_stats: NULL argument");
+    if (!ctx->have_stats) return fail(FF_ERR_INVALID, "ff_last_render_stats: no render on this context");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->last_stream));
+    unsigned long long h[S_COUNT];
+    HIP_TRY(hipMemcpy(h, ctx->stats, sizeof(h), hipMemcpyDeviceToHost));
+    std::memset(out, 0, sizeof(*out));
+    out->rays_primary = (double)h[S_PRIMARY];
+    out->rays_extension = (double)h[S_EXTENSION];
+    out->rays_shadow = (double)h[S_SHADOW];
+    out->paths = (double)h[S_PATHS];
+    return FF_OK;
+}
+
+ff_status ff_film_resolve(const float* film, int32_t width, int32_t height, float exposure, float* avg_rgb,
+                          uint8_t* rgba8, void* stream) {
+    if (!film || width <= 0 || height <= 0) return fail(FF_ERR_INVALID, "ff_film_resolve: bad arguments");
+    HIP_TRY(ffk::launch_film_resolve((const float4*)film, (uint32_t)((uint64_t)width * height), exposure, avg_rgb,
+                                     rgba8, (hipStream_t)stream));
+    return FF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,521 @@
+// ff_device.h -- device math, RNG, BSDF lobes, environment light and BVH
+// traversal for the gfx950 wavefront path tracer.
+//
+// Every routine restates a reference routine (JakeKurtz/Firefly,
+// CUDA-RayTracer/ = CRT/) cited beside it.  The whole library is compiled
+// with -ffp-contract=off so each float/double operation is rounded on its own
+// as the reference's arithmetic is written; the traversal results are
+// therefore bit-identical to a host evaluation of the same expressions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ff_layout.h"
+
+namespace ffd {
+
+// ------------------------------------------------------------ constants --
+// Float constants as the reference declares them (CRT/dMath.cuh:78-128).
+constexpr float F_PI = 3.14159265358979323846f;
+constexpr float F_2PI = 6.28318530717958647692f;
+constexpr float F_PI_2 = 1.57079632679489661923f;
+constexpr float F_PI_4 = 0.785398163397448309615f;
+constexpr float F_1_PI = 0.318309886183790671538f;
+constexpr float F_1_2PI = 0.159154943091895335769f;
+constexpr float K_EPS = 1e-6f;
+constexpr float K_HUGE = 1e32f;
+
+// RNG dimensions: bounce 0 is the camera, bounce L the L-th path vertex.
+enum : uint32_t {
+    DIM_JX = 0, DIM_JY = 1, DIM_LX = 2, DIM_LY = 3,
+    DIM_RR = 0, DIM_EX = 1, DIM_EY = 2, DIM_S0 = 3, DIM_S1 = 4,
+    DIM_D0 = 5, DIM_D1 = 6, DIM_LOBE = 7, DIM_DIEL = 8
+};
+
+// Conservative distance pruning added to the reference traversal (which
+// tests every box the ray's line crosses).  Order of visits is unchanged, so
+// the first closest candidate is the same triangle the reference returns.
+constexpr float PRUNE_REL = 0x1p-12f;
+constexpr float PRUNE_ABS = 1e-6f;
+constexpr float PRUNE_BEHIND = 1e-4f;
+
+// Transcendentals are evaluated in double and rounded to float: the CUDA
+// math library of the reference is closed, and this makes the float results
+// reproducible between host and device (they differ only when a double result
+// lies within an ulp of a float rounding boundary).
+__device__ __forceinline__ float f_sin(float x) { return (float)sin((double)x); }
+__device__ __forceinline__ float f_cos(float x) { return (float)cos((double)x); }
+__device__ __forceinline__ float f_atan(float x) { return (float)atan((double)x); }
+__device__ __forceinline__ float f_atan2(float y, float x) { return (float)atan2((double)y, (double)x); }
+__device__ __forceinline__ float f_asin(float x) { return (float)asin((double)x); }
+__device__ __forceinline__ float f_acos(float x) { return (float)acos((double)x); }
+
+// --------------------------------------------------------------- vectors --
+struct f3 { float x, y, z; };
+__device__ __forceinline__ f3 V3(float a, float b, float c) { f3 r; r.x = a; r.y = b; r.z = c; return r; }
+__device__ __forceinline__ f3 vadd(f3 a, f3 b) { return V3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 vsub(f3 a, f3 b) { return V3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 vmul(f3 a, f3 b) { return V3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ f3 vscl(f3 a, float s) { return V3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ f3 vdiv(f3 a, float s) { return V3(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ f3 vneg(f3 a) { return V3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ float vdot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 vcross(f3 a, f3 b) {
+    return V3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+// normalize with 1/sqrtf (the reference's host rsqrtf, CRT/dMath.cuh:70-73,1300-1304)
+__device__ __forceinline__ f3 vnorm(f3 v) { float inv = 1.0f / sqrtf(vdot(v, v)); return vscl(v, inv); }
+__device__ __forceinline__ bool vzero(f3 a) { return a.x == 0.f && a.y == 0.f && a.z == 0.f; }
+__device__ __forceinline__ f3 vreflect(f3 i, f3 n) { return vsub(i, vscl(vscl(n, 2.0f), vdot(n, i))); }
+__device__ __forceinline__ f3 vlerp(f3 a, f3 b, float t) { return vadd(a, vscl(vsub(b, a), t)); }
+__device__ __forceinline__ f3 xyz(float4 v) { return V3(v.x, v.y, v.z); }
+__device__ __forceinline__ float4 f4(f3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+__device__ __forceinline__ f3 ld3(const float* p) { return V3(p[0], p[1], p[2]); }
+
+// ------------------------------------------------------------------- RNG --
+// Counter RNG keyed by (seed, pixel, sample, bounce, dim); replaces the
+// clock()-seeded wang_hash of CRT/dRandom.cu:20-27.  Mixer: "lowbias32"
+// (C. Wellons, hash-prospector, public domain).
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+    return x;
+}
+__device__ __forceinline__ uint32_t rng_base(uint32_t seed, uint32_t pixel, uint32_t sample) {
+    uint32_t h = mix32(seed ^ 0x5bd1e995U);
+    h = mix32(h ^ pixel);
+    h = mix32(h + sample * 0x9E3779B9U);
+    return h;
+}
+__device__ __forceinline__ float rng(uint32_t base, uint32_t bounce, uint32_t dim) {
+    uint32_t h = mix32(mix32(base ^ ((bounce << 8) | dim)));
+    return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
+
+// ------------------------------------------------------------- textures --
+// Software bilinear filter, wrap addressing on normalized coordinates
+// (replaces tex2DLod on linear/wrap texture objects, CRT/dTexture.cu:251-255).
+__device__ __forceinline__ void bilinear(const float* __restrict__ tex, int w, int h, int c,
+                                         float u, float v, float* out) {
+    float uu = u - floorf(u), vv = v - floorf(v);
+    float x = uu * (float)w - 0.5f, y = vv * (float)h - 0.5f;
+    float fx = floorf(x), fy = floorf(y);
+    float a = x - fx, b = y - fy;
+    int i0 = (int)fx, j0 = (int)fy;
+    i0 = ((i0 % w) + w) % w;
+    j0 = ((j0 % h) + h) % h;
+    int i1 = (i0 + 1) % w, j1 = (j0 + 1) % h;
+    float w00 = (1.f - a) * (1.f - b), w10 = a * (1.f - b), w01 = (1.f - a) * b, w11 = a * b;
+    for (int k = 0; k < c; k++) {
+        float t00 = tex[((size_t)j0 * w + i0) * c + k], t10 = tex[((size_t)j0 * w + i1) * c + k];
+        float t01 = tex[((size_t)j1 * w + i0) * c + k], t11 = tex[((size_t)j1 * w + i1) * c + k];
+        out[k] = ((w00 * t00 + w10 * t10) + w01 * t01) + w11 * t11;
+    }
+}
+// luminance (CRT/dMath.cuh:1503-1507): double expression returned as float
+__device__ __forceinline__ float lum_of(const float* s) {
+    return (float)(0.299 * (double)s[0] + 0.587 * (double)s[1] + 0.114 * (double)s[2]);
+}
+
+// ------------------------------------------------------------ env light --
+// sample_spherical_direction / sample_spherical_map (CRT/dMath.cuh:1483-1501)
+__device__ __forceinline__ f3 sph_dir(float u, float v) {
+    float theta = F_2PI * u, phi = F_PI * v;
+    float x = f_cos(theta) * f_sin(phi), y = f_cos(phi), z = f_sin(theta) * f_sin(phi);
+    return V3(-x, -y, -z);
+}
+__device__ __forceinline__ void sph_map(f3 d, float& u, float& v) {
+    u = f_atan2(d.z, d.x) * F_1_2PI + 0.5f;
+    v = f_asin(d.y) * F_1_PI + 0.5f;
+}
+// upper_bound with the trailing list[left <= val] adjustment (CRT/Helpers.cu:16-31)
+__device__ __forceinline__ int upper_bound_ref(const double* __restrict__ list, int size, float val) {
+    int left = 0, right = size;
+    while (left < right) {
+        int middle = (right - left) / 2 + left;
+        if ((double)val >= list[middle]) left = middle + 1;
+        else right = middle;
+    }
+    if (left < size && list[((float)left <= val) ? 1 : 0] != 0.0) left++;
+    return left;
+}
+
+struct EnvView {
+    const float* rgb;      // [h][w][3]
+    const double* marg_y;  // [h]
+    const double* conds;   // [h][w]
+    const float* pdf;      // [h][w]
+    int w, h;
+};
+// dEnvironmentLight::L (CRT/dEnvironmentLight.cu:66-76)
+__device__ __forceinline__ f3 env_L(const EnvView& e, f3 wi) {
+    float u, v, o[3];
+    sph_map(wi, u, v);
+    bilinear(e.rgb, e.w, e.h, 3, u, v, o);
+    return ld3(o);
+}
+// dEnvironmentLight::get_pdf(isect, wi) (CRT/dEnvironmentLight.cu:83-103)
+__device__ __forceinline__ float env_pdf(const EnvView& e, f3 wi) {
+    float u, v;
+    sph_map(wi, u, v);
+    int px = (int)(u * (float)(e.w - 1)), py = (int)(v * (float)(e.h - 1));
+    px = px < 0 ? 0 : (px > e.w - 1 ? e.w - 1 : px);
+    py = py < 0 ? 0 : (py > e.h - 1 ? e.h - 1 : py);
+    float z = wi.z < -1.f ? -1.f : (wi.z > 1.f ? 1.f : wi.z);
+    return e.pdf[(size_t)py * e.w + px] / f_sin(f_acos(z));
+}
+// dEnvironmentLight::get_direction (CRT/dEnvironmentLight.cu:41-58);
+// indices clamped at 0 where the reference would index out of range.
+__device__ __forceinline__ f3 env_sample(const EnvView& e, f3 P, float ex, float ey) {
+    int y = (int)((float)upper_bound_ref(e.marg_y, e.h, ey) - 1.f);
+    if (y < 0) y = 0;
+    int x = (int)((float)upper_bound_ref(e.conds + (size_t)y * e.w, e.w, ex) - 1.f);
+    if (x < 0) x = 0;
+    f3 dir = sph_dir((float)x / (float)e.w, (float)y / (float)e.h);
+    f3 spt = vadd(V3(0.f, 0.f, 0.f), vscl(dir, 10000.f));
+    return vnorm(vsub(spt, P));
+}
+
+// -------------------------------------------------------------- camera --
+// dCamera::gen_ray (CRT/dCamera.cu:10-36) with Matrix4x4*float4
+// (CRT/dMatrix.cuh:61-69) and ConcentricSampleDisk (CRT/dRandom.cu:104-126).
+__device__ __forceinline__ void gen_ray(const ff_dev_camera& cam, int width, int height, int px,
+                                        int py, uint32_t base, f3& o, f3& d) {
+    float nx = (float)(2.0f * (((double)px + 0.5) / (double)width) - 1.0f);
+    float ny = (float)(1.0f - 2.0f * (((double)py + 0.5) / (double)height));
+    const float* m = cam.inv_vp;
+    float a[4] = {nx, ny, -1.f, 1.f}, b[4] = {nx, ny, 1.f, 1.f}, pa[4], pb[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        pa[r] = m[0 * 4 + r] * a[0] + m[1 * 4 + r] * a[1] + m[2 * 4 + r] * a[2] + m[3 * 4 + r] * a[3];
+        pb[r] = m[0 * 4 + r] * b[0] + m[1 * 4 + r] * b[1] + m[2 * 4 + r] * b[2] + m[3 * 4 + r] * b[3];
+    }
+    f3 pn = V3(pa[0] / pa[3], pa[1] / pa[3], pa[2] / pa[3]);
+    f3 pf = V3(pb[0] / pb[3], pb[1] / pb[3], pb[2] / pb[3]);
+    float sx = rng(base, 0, DIM_JX) * 0.005f, sy = rng(base, 0, DIM_JY) * 0.005f;
+    o = vadd(pn, V3(sx, sy, 0.f));
+    d = vnorm(vsub(pf, pn));
+    if (cam.lens_radius > 0.f) {
+        f3 pfoc = vadd(o, vscl(d, cam.focal));
+        float u0 = rng(base, 0, DIM_LX), u1 = rng(base, 0, DIM_LY);
+        float ox = 2.f * u0 - 1.f, oy = 2.f * u1 - 1.f, lx = 0.f, ly = 0.f;
+        if (!(ox == 0 && oy == 0)) {
+            float theta, r;
+            if (fabsf(ox) > fabsf(oy)) { r = ox; theta = F_PI_4 * (oy / ox); }
+            else { r = oy; theta = F_PI_2 - F_PI_4 * (ox / oy); }
+            lx = r * f_cos(theta);
+            ly = r * f_sin(theta);
+        }
+        o = vadd(o, V3(lx * cam.lens_radius, ly * cam.lens_radius, 0.f));
+        d = vnorm(vsub(pfoc, o));
+    }
+}
+
+// ---------------------------------------------------------------- BSDFs --
+__device__ __forceinline__ float pow5(float x) { float x2 = x * x; return (x2 * x2) * x; }
+// fresnel (CRT/dMaterial.cu:90-92)
+__device__ __forceinline__ f3 fresnel_s(f3 f0, f3 h, f3 wo) {
+    float p = pow5(1.f - fmaxf(0.f, vdot(h, wo)));
+    return vadd(f0, vscl(V3(1.f - f0.x, 1.f - f0.y, 1.f - f0.z), p));
+}
+// power_heuristic with nf = ng = 1 (CRT/dMaterial.cu:114-119)
+__device__ __forceinline__ double power_heur(double f, double g) { return (f * f) / (f * f + g * g); }
+// ggxtr_ndf (CRT/dMaterial.cu:120-125)
+__device__ __forceinline__ double ggx_ndf(f3 n, f3 h, float r) {
+    double a2 = (double)(r * r * r * r);
+    float nh = fmaxf(0.0f, vdot(n, h));
+    double nh2 = (double)(nh * nh);
+    double q = nh2 * (a2 - 1.0) + 1.0;
+    return a2 / ((double)F_PI * (q * q));
+}
+// geo_atten (CRT/dMaterial.cu:126-137)
+__device__ __forceinline__ double geo_att(f3 wi, f3 wo, f3 n, float r) {
+    float rp = r + 1.f;
+    double k = (double)((rp * rp) / 8.f);
+    double nl = (double)fmaxf(vdot(n, wi), 0.0f), nv = (double)fmaxf(vdot(n, wo), 0.0f);
+    double g1 = nl / (nl * (1.0 - k) + k), g2 = nv / (nv * (1.0 - k) + k);
+    return g1 * g2;
+}
+// spec_get_pdf (CRT/dMaterial.cu:255-265)
+__device__ __forceinline__ float spec_pdf(f3 n, f3 wi, f3 wo, float r) {
+    f3 wh = vnorm(vadd(wo, wi));
+    double whn = (double)fmaxf(vdot(wh, n), 0.f), wowh = (double)fmaxf(vdot(wo, wh), 0.f);
+    double D = ggx_ndf(n, wh, r);
+    float den = fmaxf((float)(4.0 * wowh), 0.001f);
+    return (float)((D * whn) / (double)den);
+}
+// spec_f (CRT/dMaterial.cu:266-288)
+__device__ __forceinline__ f3 spec_f(f3 n, f3 wi, f3 wo, float r, float m, f3 alb, f3 fres) {
+    f3 f0 = vlerp(fres, alb, m);
+    f3 wh = vnorm(vadd(wo, wi));
+    double ndwi = (double)fmaxf(vdot(n, wi), 0.f), ndwo = (double)fmaxf(vdot(n, wo), 0.f);
+    double D = ggx_ndf(n, wh, r), G = geo_att(wi, wo, n, r);
+    f3 F = fresnel_s(f0, wh, wi);
+    float den = fmaxf((float)(4.0 * ndwo * ndwi), 0.001f);
+    return vdiv(vscl(F, (float)(D * G)), den);
+}
+// diff_f (CRT/dMaterial.cu:164-178)
+__device__ __forceinline__ f3 diff_f(f3 wi, f3 wo, float m, f3 alb, f3 fres) {
+    f3 f0 = vlerp(fres, alb, m);
+    f3 wh = vnorm(vadd(wo, wi));
+    f3 F = fresnel_s(f0, wh, wi);
+    f3 kd = V3(1.f - F.x, 1.f - F.y, 1.f - F.z);
+    kd = vscl(kd, (float)(1.0 - (double)m));
+    return vscl(vmul(kd, alb), F_1_PI);
+}
+// get_orthogonal_vec (CRT/dMath.cuh:1450-1463)
+__device__ __forceinline__ f3 ortho_axis(f3 in) {
+    if (fabsf(in.x) < 0.57735026919f) return V3(1, 0, 0);
+    if (fabsf(in.y) < 0.57735026919f) return V3(0, 1, 0);
+    return V3(0, 0, 1);
+}
+// diff_sample (CRT/dMaterial.cu:139-159)
+__device__ __forceinline__ f3 diff_sample(f3 N, float e0, float e1) {
+    float sin_t = sqrtf(1.f - e0 * e0);
+    float phi = 2.f * F_PI * e1;
+    float x = sin_t * f_cos(phi), z = sin_t * f_sin(phi);
+    f3 T = vnorm(vcross(N, ortho_axis(N)));
+    f3 B = vnorm(vcross(N, T));
+    return vnorm(vadd(vadd(vscl(T, x), vscl(N, e0)), vscl(B, z)));
+}
+// spec_sample (CRT/dMaterial.cu:229-254)
+__device__ __forceinline__ f3 spec_sample(f3 N, f3 wo, float r, float e0, float e1) {
+    double theta = (double)f_atan(r * r * sqrtf(e0 / (1.f - e0)));
+    double phi = (double)((2 * F_PI) * e1);
+    f3 h = V3((float)(sin(theta) * cos(phi)), (float)cos(theta), (float)(sin(theta) * sin(phi)));
+    f3 T = vnorm(vcross(N, ortho_axis(N)));
+    f3 B = vnorm(vcross(N, T));
+    f3 smp = vadd(vadd(vscl(T, h.x), vscl(N, h.y)), vscl(B, h.z));
+    return vneg(vreflect(wo, vnorm(smp)));
+}
+// Exact Fresnel reflectance of a smooth dielectric (textbook Fresnel
+// equations).  No reference counterpart (the reference has no dielectric).
+__device__ __forceinline__ float fresnel_dielectric(float cos_i, float eta_i, float eta_t) {
+    float sin_i = sqrtf(fmaxf(0.f, 1.f - cos_i * cos_i));
+    float sin_t = eta_i / eta_t * sin_i;
+    if (sin_t >= 1.f) return 1.f;
+    float cos_t = sqrtf(fmaxf(0.f, 1.f - sin_t * sin_t));
+    float rpar = (eta_t * cos_i - eta_i * cos_t) / (eta_t * cos_i + eta_i * cos_t);
+    float rper = (eta_i * cos_i - eta_t * cos_t) / (eta_i * cos_i + eta_t * cos_t);
+    return 0.5f * (rpar * rpar + rper * rper);
+}
+
+// ------------------------------------------------------------ traversal --
+// Bounds3f::hit(ray, invDir, dirIsNeg) (CRT/Bounds3f.h:128-160) on a node
+// held as two float4: (bmin.xyz, bmax.x), (bmax.yz, offset, nprims|axis<<16).
+__device__ __forceinline__ bool slab(float4 a, float4 b, f3 o, f3 inv, bool nx, bool ny, bool nz,
+                                     float& tn, float& tf) {
+    float tmin = ((nx ? a.w : a.x) - o.x) * inv.x;
+    float tmax = ((nx ? a.x : a.w) - o.x) * inv.x;
+    float tymin = ((ny ? b.x : a.y) - o.y) * inv.y;
+    float tymax = ((ny ? a.y : b.x) - o.y) * inv.y;
+    if ((tmin > tymax) || (tymin > tmax)) return false;
+    if (tymin > tmin) tmin = tymin;
+    if (tymax < tmax) tmax = tymax;
+    float tzmin = ((nz ? b.y : a.z) - o.z) * inv.z;
+    float tzmax = ((nz ? a.z : b.y) - o.z) * inv.z;
+    if ((tmin > tzmax) || (tzmin > tmax)) return false;
+    if (tzmin > tmin) tmin = tzmin;
+    if (tzmax < tmax) tmax = tzmax;
+    tn = tmin;
+    tf = tmax;
+    return true;
+}
+
+// Triangle record in BVH-slot order: v0, e1 = v1 - v0, e2 = v2 - v0 (the
+// edges the reference forms per test, CRT/dTriangle.cu:35-36).
+struct Tri { f3 v0, e1, e2; };
+__device__ __forceinline__ Tri load_tri(const float* __restrict__ t, int slot) {
+    const float* p = t + (size_t)slot * 9;
+    Tri r;
+    r.v0 = V3(p[0], p[1], p[2]);
+    r.e1 = V3(p[3], p[4], p[5]);
+    r.e2 = V3(p[6], p[7], p[8]);
+    return r;
+}
+// dTriangle::intersect, TEST_CULL branch (CRT/dTriangle.cu:34-62)
+__device__ __forceinline__ bool mt_cull(const Tri& T, f3 o, f3 d, float& t, float& u, float& v) {
+    f3 pvec = vcross(d, T.e2);
+    float detf = vdot(T.e1, pvec);
+    if (detf < K_EPS) return false;
+    f3 tvec = vsub(o, T.v0);
+    float uf = vdot(tvec, pvec);
+    if (uf < 0.f || uf > detf) return false;
+    f3 qvec = vcross(tvec, T.e1);
+    float vf = vdot(d, qvec);
+    if (vf < 0.f || (uf + vf) > detf) return false;
+    float tf = vdot(T.e2, qvec);
+    double inv_det = 1.0 / (double)detf;
+    t = (float)((double)tf * inv_det);
+    u = (float)((double)uf * inv_det);
+    v = (float)((double)vf * inv_det);
+    return true;
+}
+// dTriangle::intersect, non-culling branch (CRT/dTriangle.cu:63-80)
+__device__ __forceinline__ bool mt_two(const Tri& T, f3 o, f3 d, float& t, float& u, float& v) {
+    f3 pvec = vcross(d, T.e2);
+    double det = (double)vdot(T.e1, pvec);
+    if (det > (double)-K_EPS && det < (double)K_EPS) return false;
+    double inv_det = 1.0 / det;
+    f3 tvec = vsub(o, T.v0);
+    float uf = (float)((double)vdot(tvec, pvec) * inv_det);
+    if (uf < 0.f || uf > 1.f) return false;
+    f3 qvec = vcross(tvec, T.e1);
+    float vf = (float)((double)vdot(d, qvec) * inv_det);
+    if (vf < 0.f || (double)(uf + vf) > 1.0) return false;
+    t = (float)((double)vdot(T.e2, qvec) * inv_det);
+    u = uf;
+    v = vf;
+    return true;
+}
+// dTriangle::shadow_hit (CRT/dTriangle.cu:110-138)
+__device__ __forceinline__ bool mt_shadow(const Tri& T, f3 o, f3 d, float& t) {
+    f3 pvec = vcross(d, T.e2);
+    double det = (double)vdot(T.e1, pvec);
+    if (det > (double)-K_EPS && det < (double)K_EPS) return false;
+    double inv_det = 1.0 / det;
+    f3 tvec = vsub(o, T.v0);
+    double u = (double)vdot(tvec, pvec) * inv_det;
+    if (u < 0.0 || u > 1.0) return false;
+    f3 qvec = vcross(tvec, T.e1);
+    double v = (double)vdot(d, qvec) * inv_det;
+    if (v < 0.0 || u + v > 1.0) return false;
+    float tf = (float)((double)vdot(T.e2, qvec) * inv_det);
+    if (tf < 0.f) return false;
+    t = tf;
+    return true;
+}
+
+// Traversal stack: the first FF_LDS_STACK entries of each lane live in LDS
+// ([depth][lane], conflict-free), deeper entries in a per-lane global
+// overflow area ([depth - FF_LDS_STACK][grid thread]).
+struct Stack {
+    int* lds;          // &smem[threadIdx.x]
+    int* ovf;          // &overflow[grid thread]
+    uint32_t lds_stride, ovf_stride;
+    int sp;
+    __device__ __forceinline__ void push(int v) {
+        if (sp < FF_LDS_STACK) lds[sp * lds_stride] = v;
+        else ovf[(size_t)(sp - FF_LDS_STACK) * ovf_stride] = v;
+        sp++;
+    }
+    __device__ __forceinline__ int pop() {
+        sp--;
+        return sp < FF_LDS_STACK ? lds[sp * lds_stride] : ovf[(size_t)(sp - FF_LDS_STACK) * ovf_stride];
+    }
+};
+
+struct Hit { float t, u, v; int id; };
+
+// intersect() closest hit (CRT/dTriangle.cu:140-213): near child first by
+// dirIsNeg[axis]; a candidate replaces the record when 0 <= t < tmin.
+__device__ __forceinline__ Hit closest_hit(const float4* __restrict__ nodes,
+                                           const float* __restrict__ tris, f3 o, f3 d,
+                                           bool two_sided, Stack& st) {
+    Hit h;
+    h.t = K_HUGE; h.u = 0.f; h.v = 0.f; h.id = -1;
+    f3 inv = V3(1.f / d.x, 1.f / d.y, 1.f / d.z);
+    bool nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
+    st.sp = 0;
+    int cur = 0;
+    for (;;) {
+        float4 a = nodes[2 * cur], b = nodes[2 * cur + 1];
+        float tn, tf;
+        bool hit = slab(a, b, o, inv, nx, ny, nz, tn, tf);
+        if (hit) {
+            float lim = h.t + h.t * PRUNE_REL + PRUNE_ABS;
+            if (tn > lim || tf < -PRUNE_BEHIND) hit = false;
+        }
+        if (hit) {
+            uint32_t meta = __float_as_uint(b.w);
+            int off = __float_as_int(b.z);
+            int np = (int)(meta & 0xffffu);
+            if (np > 0) {
+                for (int i = 0; i < np; i++) {
+                    Tri T = load_tri(tris, off + i);
+                    float t = 0.f, u = 0.f, v = 0.f;
+                    bool ok = two_sided ? mt_two(T, o, d, t, u, v) : mt_cull(T, o, d, t, u, v);
+                    if (ok && !(t < 0.f) && t < h.t) { h.t = t; h.u = u; h.v = v; h.id = off + i; }
+                }
+                if (st.sp == 0) break;
+                cur = st.pop();
+            } else {
+                int axis = (int)((meta >> 16) & 0xffu);
+                bool ng = axis == 0 ? nx : (axis == 1 ? ny : nz);
+                if (ng) { st.push(cur + 1); cur = off; }
+                else { st.push(off); cur = cur + 1; }
+            }
+        } else {
+            if (st.sp == 0) break;
+            cur = st.pop();
+        }
+    }
+    return h;
+}
+
+// intersect_shadows with tmin = K_HUGE (CRT/dTriangle.cu:215-259,
+// CRT/dEnvironmentLight.cu:60-64): true when any triangle has 0 <= t < 1e32.
+__device__ __forceinline__ bool any_hit(const float4* __restrict__ nodes,
+                                        const float* __restrict__ tris, f3 o, f3 d, Stack& st) {
+    f3 inv = V3(1 / d.x, 1 / d.y, 1 / d.z);
+    bool nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
+    st.sp = 0;
+    int cur = 0;
+    for (;;) {
+        float4 a = nodes[2 * cur], b = nodes[2 * cur + 1];
+        float tn, tf;
+        bool hit = slab(a, b, o, inv, nx, ny, nz, tn, tf);
+        if (hit && tf < -PRUNE_BEHIND) hit = false;
+        if (hit) {
+            uint32_t meta = __float_as_uint(b.w);
+            int off = __float_as_int(b.z);
+            int np = (int)(meta & 0xffffu);
+            if (np > 0) {
+                for (int i = 0; i < np; i++) {
+                    Tri T = load_tri(tris, off + i);
+                    float t = 0.f;
+                    if (mt_shadow(T, o, d, t) && t < K_HUGE) return true;
+                }
+                if (st.sp == 0) break;
+                cur = st.pop();
+            } else {
+                int axis = (int)((meta >> 16) & 0xffu);
+                bool ng = axis == 0 ? nx : (axis == 1 ? ny : nz);
+                if (ng) { st.push(cur + 1); cur = off; }
+                else { st.push(off); cur = cur + 1; }
+            }
+        } else {
+            if (st.sp == 0) break;
+            cur = st.pop();
+        }
+    }
+    return false;
+}
+
+// ----------------------------------------------------- wave64 utilities --
+// Wave-aggregated queue append: one atomic per wave, lane offsets from the
+// 64-bit ballot (replaces the 32-lane atomicAggInc, CRT/kernel.cu:24-36).
+// Every lane of the wave must call it; returns the slot for lanes with pred.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
+    uint64_t mask = __ballot(pred);
+    uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                              __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+    uint32_t base = 0;
+    if (mask) {
+        int leader = __ffsll((unsigned long long)mask) - 1;
+        if ((int)__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+        base = __shfl(base, leader);
+    }
+    return base + rank;
+}
+// Wave-aggregated 64-bit counter of lanes with pred.
+__device__ __forceinline__ void wave_count(unsigned long long* counter, bool pred) {
+    uint64_t mask = __ballot(pred);
+    uint64_t all = __ballot(1);
+    if (mask) {
+        int leader = __ffsll((unsigned long long)all) - 1;
+        if ((int)__lane_id() == leader) atomicAdd(counter, (unsigned long long)__popcll(mask));
+    }
+}
+
+}  // namespace ffd

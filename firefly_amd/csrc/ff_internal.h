@@ -1,0 +1,57 @@
+// ff_internal.h -- host-side declarations shared by the C ABI implementation,
+// the host BVH builder and the kernel launchers.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "ff_layout.h"
+
+namespace ffk {
+hipError_t launch_generate(const ff_dev_camera& cam, int W, int H, uint32_t n_active,
+                           uint32_t sample_base, uint32_t seed, const ff_paths& P, int out_q,
+                           hipStream_t s);
+hipError_t launch_extend(const ff_dev_scene& S, const ff_paths& P, int q, hipStream_t s);
+hipError_t launch_shadow(const ff_dev_scene& S, const ff_paths& P, hipStream_t s);
+hipError_t launch_reset(uint32_t* counters, uint32_t mask, hipStream_t s);
+hipError_t launch_logic(const ff_dev_scene& S, const ff_dev_env& E, const ff_paths& P, int q,
+                        uint32_t max_depth, uint32_t n_max, hipStream_t s);
+hipError_t launch_shade(int type, const ff_dev_scene& S, const ff_dev_env& E, const ff_paths& P,
+                        int next_q, uint32_t n_max, hipStream_t s);
+hipError_t launch_fill4(float4* p, uint32_t n, hipStream_t s);
+hipError_t launch_fill_u64(unsigned long long* p, uint32_t n, hipStream_t s);
+hipError_t launch_resolve_layers(const float4* layers, uint32_t K, uint32_t WH, float4* film,
+                                 float spp, hipStream_t s);
+hipError_t launch_trace_closest(const ff_dev_scene& S, const float4* rays, uint32_t n,
+                                float4* hits, int two_sided, int32_t* ovf, hipStream_t s);
+hipError_t launch_trace_any(const ff_dev_scene& S, const float4* rays, uint32_t n, uint8_t* occ,
+                            int32_t* ovf, hipStream_t s);
+hipError_t launch_gen_rays(const ff_dev_camera& cam, int W, int H, uint32_t seed,
+                           uint32_t sample, float4* rays, hipStream_t s);
+hipError_t launch_env_build(const float* rgb, int w, int h, float* lum, double* row_sum,
+                            double* denom, double* marg_p, double* marg_y, double* conds,
+                            float* pdf, hipStream_t s);
+hipError_t launch_film_resolve(const float4* film, uint32_t n, float exposure, float* avg,
+                               uint8_t* rgba, hipStream_t s);
+}  // namespace ffk
+
+namespace ffh {
+// 32-byte node as stored in HBM (CRT/BVH.h:63-72 field order).
+struct BvhNode {
+    float bmin[3];
+    float bmax[3];
+    int32_t offset;
+    uint16_t n_prims;
+    uint8_t axis;
+    uint8_t pad;
+};
+static_assert(sizeof(BvhNode) == 32, "BVH node must stay 32 bytes");
+
+// SAH BVH over triangles (positions [n][3][3]); returns nodes in flattened
+// depth-first order and the slot -> triangle permutation.
+void build_bvh_sah(const float* positions, int32_t n, int32_t max_prims,
+                   std::vector<BvhNode>& nodes, std::vector<int32_t>& ordered);
+int32_t bvh_max_depth(const std::vector<BvhNode>& nodes);
+}  // namespace ffh

@@ -1,0 +1,95 @@
+// ff_layout.h -- device data layout shared by the kernels and the host driver.
+//
+// HBM layout (all arrays are structure-of-arrays indexed by path id or BVH
+// slot; float4 fields keep every per-path record a 16-byte aligned load):
+//   nodes   : 2 x float4 per BVH node (32 B, CRT/BVH.h:63-72 field order)
+//   tris    : 9 floats per slot, v0 | e1 | e2 (36 B, positions only)
+//   nrm/uv  : shading attributes per slot, read once per closest hit
+//   paths   : ray (2 x float4), hit, beta+state, Lp, fext, nee[3], sray[3][2],
+//             vis[3], rng base; queues of u32 path ids with device counters.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define FF_BLOCK 256        // threads per workgroup (4 waves of 64)
+#define FF_LDS_STACK 16     // traversal stack entries per lane held in LDS
+#define FF_TRAV_GRID 2048   // workgroups of the persistent traversal kernels
+#define FF_GRID_CAP 4096    // grid-stride cap of the other stage kernels
+
+// path state word (beta.w bits)
+#define ST_LEN_MASK 0xffu
+#define ST_INSIDE (1u << 8)
+#define ST_PREV_DELTA (1u << 9)
+#define ST_NEED_L (1u << 10)
+#define ST_NEED_S (1u << 11)
+#define ST_NEED_D (1u << 12)
+
+// device counters
+#define C_EXT0 0
+#define C_EXT1 1
+#define C_SHADOW 2
+#define C_MAT0 3
+#define C_COUNT 8
+
+// ray statistics
+#define S_PRIMARY 0
+#define S_EXTENSION 1
+#define S_SHADOW 2
+#define S_PATHS 3
+#define S_COUNT 4
+
+#define FF_MAT_FLOATS_DEV 16
+
+#define MAT_MIX 0
+#define MAT_DIFFUSE 1
+#define MAT_CONDUCTOR 2
+#define MAT_DIELECTRIC 3
+
+struct ff_dev_camera {
+    float inv_vp[16];
+    float lens_radius, focal, exposure, pad;
+};
+
+struct ff_tex_desc {
+    const float* data;
+    int32_t w, h, c, pad;
+};
+
+struct ff_dev_scene {
+    const float4* nodes;
+    const float* tris;
+    const float* nrm;
+    const float* uv;
+    const int32_t* mat;
+    const float* mats;
+    const ff_tex_desc* tex;
+    int32_t n_nodes, n_tris, n_mats, n_tex, max_depth;
+};
+
+struct ff_dev_env {
+    const float* rgb;
+    const double* marg_y;
+    const double* conds;
+    const float* pdf;
+    int32_t w, h;
+};
+
+struct ff_paths {
+    float4* ray;     // [P][2]  (o, -) (d, -)
+    float4* hit;     // [P]     (t, u, v, slot)
+    float4* beta;    // [P]     (beta.xyz, state)
+    float4* Lp;      // [P]     (path radiance, -)
+    float4* fext;    // [P]     (f.xyz, pdf)
+    float4* nee;     // [P][3]  light term (w = cosine), spec term, diffuse term
+    float4* sray;    // [P][3][2] shadow rays
+    uint8_t* vis;    // [P][3]
+    uint32_t* rngb;  // [P]
+    uint32_t* q_ext[2];
+    uint32_t* q_mat[4];
+    uint32_t* q_shadow;
+    uint32_t* counters;  // [C_COUNT]
+    float4* layers;      // [P] film layer accumulators (path id = layer*WH + pixel)
+    unsigned long long* stats;  // [S_COUNT]
+    int32_t* ovf;        // traversal stack overflow [depth][FF_TRAV_GRID*FF_BLOCK]
+};

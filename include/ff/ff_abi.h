@@ -1,0 +1,181 @@
+/*
+ * ff_abi.h -- C ABI of firefly_amd, the MI355X (gfx950) wavefront path tracer.
+ *
+ * Plain C: opaque handles, explicit sizes, caller-visible device buffers,
+ * HIP streams passed as void*.  Every call returns an ff_status and never
+ * exits the process; ff_last_error() gives the message of the calling
+ * thread's last failure.  One context per host thread; handles are not
+ * thread-safe.
+ *
+ * Each entry point names the reference interface it replaces
+ * (JakeKurtz/Firefly, CUDA-RayTracer/ = CRT/).
+ */
+#ifndef FF_ABI_H
+#define FF_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FF_ABI_VERSION 1
+
+typedef enum {
+    FF_OK = 0,
+    FF_ERR_INVALID = 1,     /* bad handle, size, pointer or parameter      */
+    FF_ERR_HIP = 2,         /* HIP runtime error (message has the detail)  */
+    FF_ERR_ALLOC = 3,       /* device or host allocation failed            */
+    FF_ERR_UNSUPPORTED = 4, /* request outside what this build supports    */
+    FF_ERR_CAPACITY = 5     /* request exceeds a documented capacity limit */
+} ff_status;
+
+typedef struct ff_context ff_context;
+typedef struct ff_scene ff_scene;
+typedef struct ff_envmap ff_envmap;
+
+/* 32-byte flattened BVH node, same fields as LinearBVHNode (CRT/BVH.h:63-72):
+ * bounds, first primitive (leaf) or second child (interior), primitive count
+ * (0 = interior), split axis. */
+typedef struct {
+    float bmin[3];
+    float bmax[3];
+    int32_t offset;
+    uint16_t n_prims;
+    uint8_t axis;
+    uint8_t pad;
+} ff_bvh_node;
+
+/* Material record (16 floats), fields of dMaterial (CRT/dMaterial.cuh:9-31):
+ * [0] type, [1..3] base colour, [4] roughness, [5] metallic, [6..8] fresnel f0,
+ * [9] ior, [10] albedo texture (-1 none), [11] roughness texture (-1 none). */
+#define FF_MAT_MIX 0        /* reference wf_mat_mix lobe pair (diffuse + GGX)  */
+#define FF_MAT_DIFFUSE 1    /* reference diffuse lobe (wf_mat_diffuse)         */
+#define FF_MAT_CONDUCTOR 2  /* reference GGX lobe (wf_mat_cook), metallic      */
+#define FF_MAT_DIELECTRIC 3 /* smooth dielectric (no reference counterpart)    */
+#define FF_MAT_FLOATS 16
+
+/* Scene description in host memory; copied at ff_scene_create. */
+typedef struct {
+    int32_t n_tris;
+    const float *positions;    /* [n][3][3] */
+    const float *normals;      /* [n][3][3] */
+    const float *uvs;          /* [n][3][2] (may be NULL) */
+    const int32_t *material_ids; /* [n] */
+    int32_t n_materials;
+    const float *materials;    /* [n_materials][16] */
+    int32_t n_textures;
+    const float *const *textures; /* per texture [h][w][c] float */
+    const int32_t *tex_w;
+    const int32_t *tex_h;
+    const int32_t *tex_c;      /* 1 or 3 */
+    int32_t max_prims_in_node; /* SAH leaf size, 8 in the reference (CRT/dScene.cpp:372) */
+} ff_scene_desc;
+
+/* dCamera (CRT/dCamera.cuh:9-22): inverse(proj * view), column-major. */
+typedef struct {
+    float inv_view_proj[16];
+    float lens_radius;
+    float focal_distance;
+    float exposure;
+    float pad;
+} ff_camera;
+
+/* One render request (replaces PathTracer's compile-time MAX_PATH_LENGTH and
+ * the per-frame bounce loop, CRT/PathTracer.h:20-21, CRT/PathTracer.cpp:59-79). */
+typedef struct {
+    int32_t width, height;
+    int32_t spp;          /* samples per pixel rendered by this call          */
+    int32_t max_depth;    /* a path ends when its length exceeds max_depth    */
+    uint32_t seed;
+    int32_t spp_batch;    /* samples of each pixel in flight at once (film layers) */
+    int32_t sample_offset;/* index of the first sample (progressive rendering)*/
+    int32_t flags;        /* FF_RENDER_* */
+} ff_render_req;
+
+#define FF_RENDER_USE_GRAPH 1 /* capture the bounce loop in a hipGraph */
+
+typedef struct {
+    double rays_primary;   /* camera rays (closest hit)          */
+    double rays_extension; /* bounce >= 1 extension rays          */
+    double rays_shadow;    /* any-hit rays (light + BRDF samples) */
+    double paths;          /* completed samples                   */
+    double nodes_tested;   /* extension + shadow box tests (when counted) */
+    double tris_tested;
+} ff_ray_stats;
+
+/* ---- errors / version ---- */
+const char *ff_last_error(void);
+int32_t ff_abi_version(void);
+
+/* ---- context (device selection + reusable wavefront buffers) ----
+ * Replaces PathTracer's allocation of Paths/Queues (CRT/PathTracer.cpp:98-140). */
+ff_status ff_context_create(int32_t device, ff_context **out);
+ff_status ff_context_destroy(ff_context *ctx);
+
+/* ---- host SAH BVH build (no GPU needed) ----
+ * Replaces BVHAccel(info, SplitMethod::SAH, maxPrims) + init_BVH_triangle_info
+ * (CRT/BVH.cpp:52-64, CRT/dScene.cpp:324-379).  nodes_out holds 2n-1 nodes. */
+ff_status ff_bvh_build_host(const float *positions, int32_t n_tris, int32_t max_prims,
+                            ff_bvh_node *nodes_out, int32_t *ordered_out,
+                            int32_t *n_nodes_out, int32_t *max_depth_out);
+
+/* ---- scene: BVH build on the host, triangles reordered and uploaded ----
+ * Replaces dScene::load_scene (CRT/dScene.cpp:60-80) with load_models,
+ * process_mesh and reorder_primitives (CRT/kernel.cu:162-191,701-723). */
+ff_status ff_scene_create(ff_context *ctx, const ff_scene_desc *desc, ff_scene **out);
+ff_status ff_scene_destroy(ff_scene *scene);
+ff_status ff_scene_info(const ff_scene *scene, int32_t *n_tris, int32_t *n_nodes,
+                        int32_t *max_depth);
+ff_status ff_scene_get_bvh(const ff_scene *scene, ff_bvh_node *nodes_out,
+                           int32_t *ordered_out);
+
+/* ---- environment light: FP64 marginal/conditional CDF build on the GPU ----
+ * Replaces add_environment_light and its four CDF kernels
+ * (CRT/kernel.cu:65-149,742-771).  rgb is host [h][w][3] float. */
+ff_status ff_envmap_create(ff_context *ctx, const float *rgb, int32_t w, int32_t h,
+                           ff_envmap **out);
+ff_status ff_envmap_read(const ff_envmap *env, double *marginal_y, double *marginal_p,
+                         double *conds, float *pdf, double *denom);
+ff_status ff_envmap_destroy(ff_envmap *env);
+
+/* ---- standalone wavefront stages on caller device buffers ----
+ * rays: device [n][8] floats (o.x o.y o.z - d.x d.y d.z -).
+ * ff_trace_closest replaces wf_extend/intersect (CRT/kernel.cu:487-523,
+ *   CRT/dTriangle.cu:140-213): hits device [n][4] = (t, u, v, slot as int32);
+ *   slot -1 and t = 1e32 for a miss.  two_sided != 0 uses the non-culling
+ *   test of dTriangle::intersect (CRT/dTriangle.cu:63-80).
+ * ff_trace_any replaces wf_shadow/intersect_shadows (CRT/kernel.cu:525-543,
+ *   CRT/dTriangle.cu:215-259): occluded device [n] bytes. */
+ff_status ff_trace_closest(ff_scene *scene, const float *rays, uint32_t n, float *hits,
+                           int32_t two_sided, void *stream);
+ff_status ff_trace_any(ff_scene *scene, const float *rays, uint32_t n, uint8_t *occluded,
+                       void *stream);
+/* camera-ray generation only (wf_generate + dCamera::gen_ray,
+ * CRT/kernel.cu:457-485, CRT/dCamera.cu:10-36): rays device [h*w][8] for
+ * sample index `sample`. */
+ff_status ff_generate_rays(ff_context *ctx, const ff_camera *cam, int32_t width,
+                           int32_t height, uint32_t seed, int32_t sample, float *rays,
+                           void *stream);
+
+/* ---- full render: plain per-request wavefront ----
+ * Replaces PathTracer::draw -> wavefront_pathtrace (CRT/PathTracer.cpp:59-79,
+ * CRT/kernel.cu:658-676).  film: device [h][w][4]; the call ADDS the radiance
+ * sums of req->spp samples to xyz and req->spp to w (clear it first, as
+ * PathTracer::clear_buffer does, CRT/PathTracer.cpp:150-170). */
+ff_status ff_render(ff_context *ctx, ff_scene *scene, ff_envmap *env, const ff_camera *cam,
+                    const ff_render_req *req, float *film, void *stream);
+/* ray counts of the last ff_render on this context (synchronises its stream). */
+ff_status ff_last_render_stats(ff_context *ctx, ff_ray_stats *out);
+
+/* ---- film resolve (dFilm): average and Reinhard tonemap ----
+ * Replaces the resolve at the end of wf_logic (CRT/kernel.cu:447-454):
+ * avg = sum/n; c = avg*exposure; c/(c+1); rgba = (int)(255*c).  Either output
+ * may be NULL. */
+ff_status ff_film_resolve(const float *film, int32_t width, int32_t height, float exposure,
+                          float *avg_rgb, uint8_t *rgba8, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1148 @@
+/*
+ * ff_oracle.c -- CPU oracle (TEST INFRASTRUCTURE) for the Firefly wavefront
+ * path-tracing hot path.  Plain C restatement of the reference's algorithm;
+ * it is the checker for firefly_amd's HIP kernels, never part of the product.
+ *
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off), see that file.
+ *
+ * What pins each function (tests/test_oracle_*.py):
+ *   or_rng_*              : uniformity / independence statistics (chi^2), range [0,1).
+ *   or_bvh_build          : invariants (every primitive exactly once, child bounds
+ *                           inside parent, leaf size <= max unless centroids are
+ *                           degenerate, SAH split decision re-derived by brute
+ *                           force on small nodes); traversal results identical to
+ *                           brute force over all triangles.
+ *   or_trace_closest/any  : textbook Moller-Trumbore cases with exact answers
+ *                           (tests/golden/mt_cases.json), brute-force agreement on
+ *                           random scenes, culling / parallel / edge cases.
+ *   or_env_build          : constant-map closed form, brute-force FP64 on 8x4 maps,
+ *                           monotone CDF rows ending at 1.
+ *   or_upper_bound        : hand-derived index table incl. the reference quirk.
+ *   or_bilinear           : closed form (corner average, linear ramps, wrap).
+ *   or_gen_ray            : closed-form pinhole unprojection.
+ *   BSDF pieces           : closed forms (Schlick F(0)=f0, GGX normalisation
+ *                           integral = 1, pdf integrals = 1, Fresnel R+T=1, TIR).
+ *   or_render             : white-furnace / constant-environment closed forms and
+ *                           symmetry; no independent pin of the exact quirk
+ *                           arithmetic of the full estimator beyond its parts.
+ */
+#include "ff_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* float constants as the reference declares them (CRT/dMath.cuh:78-128) */
+#define F_PI 3.14159265358979323846f
+#define F_2PI 6.28318530717958647692f
+#define F_PI_2 1.57079632679489661923f
+#define F_PI_4 0.785398163397448309615f
+#define F_1_PI 0.318309886183790671538f
+#define F_1_2PI 0.159154943091895335769f
+#define K_EPS 1e-6f
+#define K_HUGE 1e32f
+
+/* RNG dimension map.  bounce 0 = camera; bounce L = logic + shading at the
+ * L-th vertex. */
+#define DIM_JX 0
+#define DIM_JY 1
+#define DIM_LX 2
+#define DIM_LY 3
+#define DIM_RR 0
+#define DIM_EX 1
+#define DIM_EY 2
+#define DIM_S0 3
+#define DIM_S1 4
+#define DIM_D0 5
+#define DIM_D1 6
+#define DIM_LOBE 7
+#define DIM_DIEL 8
+
+/* pruning slack of the GPU traversal, mirrored by or_count_traversal */
+#define PRUNE_REL 0x1p-12f
+#define PRUNE_ABS 1e-6f
+#define PRUNE_BEHIND 1e-4f
+
+/* Transcendentals are evaluated in double and rounded to float: the CUDA math
+ * library of the reference is closed, and this makes the float results
+ * reproducible between this oracle and the GPU (they differ only when a
+ * double result lies within an ulp of a float rounding boundary). */
+static inline float f_sin(float x) { return (float)sin((double)x); }
+static inline float f_cos(float x) { return (float)cos((double)x); }
+static inline float f_atan(float x) { return (float)atan((double)x); }
+static inline float f_atan2(float y, float x) { return (float)atan2((double)y, (double)x); }
+static inline float f_asin(float x) { return (float)asin((double)x); }
+static inline float f_acos(float x) { return (float)acos((double)x); }
+
+typedef struct { float x, y, z; } v3;
+static inline v3 V3(float a, float b, float c) { v3 r; r.x = a; r.y = b; r.z = c; return r; }
+static inline v3 vadd(v3 a, v3 b) { return V3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 vsub(v3 a, v3 b) { return V3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 vmul(v3 a, v3 b) { return V3(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline v3 vscl(v3 a, float s) { return V3(a.x * s, a.y * s, a.z * s); }
+static inline v3 vdiv(v3 a, float s) { return V3(a.x / s, a.y / s, a.z / s); }
+static inline v3 vneg(v3 a) { return V3(-a.x, -a.y, -a.z); }
+static inline float vdot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline v3 vcross(v3 a, v3 b) {
+    return V3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+/* normalize with the reference's host rsqrtf = 1/sqrtf (CRT/dMath.cuh:70-73,1300-1304) */
+static inline v3 vnorm(v3 v) { float inv = 1.0f / sqrtf(vdot(v, v)); return vscl(v, inv); }
+static inline int vzero(v3 a) { return a.x == 0.f && a.y == 0.f && a.z == 0.f; }
+static inline v3 vld(const float *p) { return V3(p[0], p[1], p[2]); }
+/* CRT/dMath.cuh:1402-1405 */
+static inline v3 vreflect(v3 i, v3 n) { return vsub(i, vscl(vscl(n, 2.0f), vdot(n, i))); }
+/* CUTIL lerp a + t*(b-a) */
+static inline v3 vlerp(v3 a, v3 b, float t) { return vadd(a, vscl(vsub(b, a), t)); }
+
+/* ------------------------------------------------------------------ RNG -- */
+/* Counter RNG replacing the clock()-seeded wang_hash (CRT/dRandom.cu:20-27).
+ * Mixer: "lowbias32" integer hash (C. Wellons, hash-prospector, public domain). */
+static inline uint32_t mix32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+    return x;
+}
+uint32_t or_rng_base(uint32_t seed, uint32_t pixel, uint32_t sample) {
+    uint32_t h = mix32(seed ^ 0x5bd1e995U);
+    h = mix32(h ^ pixel);
+    h = mix32(h + sample * 0x9E3779B9U);
+    return h;
+}
+float or_rng(uint32_t base, uint32_t bounce, uint32_t dim) {
+    uint32_t h = mix32(mix32(base ^ ((bounce << 8) | dim)));
+    return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
+
+/* ------------------------------------------------------------------ BVH -- */
+/* Host fminf/fmaxf of the reference's host build (CRT/dMath.cuh:40-48). */
+static inline float fmin_r(float a, float b) { return a < b ? a : b; }
+static inline float fmax_r(float a, float b) { return a > b ? a : b; }
+
+typedef struct { float mn[3], mx[3]; } bbox;
+
+static bbox bb_empty(void) {
+    bbox b; /* Bounds3f() (CRT/Bounds3f.h:15-20) */
+    for (int k = 0; k < 3; k++) { b.mn[k] = FLT_MAX; b.mx[k] = -FLT_MAX; }
+    return b;
+}
+/* Bounds3f(p1,p2) (CRT/Bounds3f.h:24-28) */
+static bbox bb_make(const float *p1, const float *p2) {
+    bbox b;
+    for (int k = 0; k < 3; k++) { b.mn[k] = fmin_r(p1[k], p2[k]); b.mx[k] = fmax_r(p1[k], p2[k]); }
+    return b;
+}
+/* Union(b, p) and Union(b1, b2) (CRT/BVH.cpp:26-50): min/max, then the ctor again */
+static bbox bb_union_pt(bbox b, const float *p) {
+    float lo[3], hi[3];
+    for (int k = 0; k < 3; k++) { lo[k] = fmin_r(b.mn[k], p[k]); hi[k] = fmax_r(b.mx[k], p[k]); }
+    return bb_make(lo, hi);
+}
+static bbox bb_union(bbox a, bbox b) {
+    float lo[3], hi[3];
+    for (int k = 0; k < 3; k++) { lo[k] = fmin_r(a.mn[k], b.mn[k]); hi[k] = fmax_r(a.mx[k], b.mx[k]); }
+    return bb_make(lo, hi);
+}
+/* surface_area in double of float diagonal terms (CRT/Bounds3f.h:60-64) */
+static double bb_sa(bbox b) {
+    float dx = b.mx[0] - b.mn[0], dy = b.mx[1] - b.mn[1], dz = b.mx[2] - b.mn[2];
+    return 2.0 * (double)(dx * dy + dx * dz + dy * dz);
+}
+static int bb_max_extent(bbox b) { /* CRT/Bounds3f.h:71-79 */
+    float dx = b.mx[0] - b.mn[0], dy = b.mx[1] - b.mn[1], dz = b.mx[2] - b.mn[2];
+    if (dx > dy && dx > dz) return 0;
+    if (dy > dz) return 1;
+    return 2;
+}
+
+typedef struct { int32_t prim; bbox b; float c[3]; } pinfo;
+
+typedef struct {
+    pinfo *pi;
+    or_node *nodes;
+    int32_t n_nodes;
+    int32_t *ordered;
+    int32_t n_ordered;
+    int32_t max_prims;
+} bvh_ctx;
+
+/* bucket index of a centroid (CRT/BVH.cpp:220-232, 270-282) */
+static inline int bucket_of(const bbox *cb, int dim, const float *c) {
+    float o = c[dim] - cb->mn[dim];
+    if (cb->mx[dim] > cb->mn[dim]) o /= cb->mx[dim] - cb->mn[dim];
+    int b = (int)(12.0f * o);
+    if (b == 12) b = 11;
+    return b;
+}
+
+static void make_leaf(bvh_ctx *B, int32_t idx, int start, int end, bbox b) {
+    or_node *nd = &B->nodes[idx];
+    int first = B->n_ordered;
+    for (int i = start; i < end; i++) B->ordered[B->n_ordered++] = B->pi[i].prim;
+    memcpy(nd->bmin, b.mn, 12);
+    memcpy(nd->bmax, b.mx, 12);
+    nd->offset = first;
+    nd->n_prims = (uint16_t)(end - start);
+    nd->axis = 0;
+    nd->pad = 0;
+}
+
+/* recursive_build (CRT/BVH.cpp:81-307), SAH branch; nodes are produced in the
+ * depth-first pre-order that flatten_tree (CRT/BVH.cpp:309-331) assigns. */
+static void build_rec(bvh_ctx *B, int start, int end) {
+    int32_t idx = B->n_nodes++;
+    bbox bounds = bb_empty();
+    for (int i = start; i < end; i++) bounds = bb_union(bounds, B->pi[i].b);
+    int n = end - start;
+    if (n == 1) { make_leaf(B, idx, start, end, bounds); return; }
+    bbox cb = bb_empty();
+    for (int i = start; i < end; i++) cb = bb_union_pt(cb, B->pi[i].c);
+    int dim = bb_max_extent(cb);
+    if (cb.mx[dim] == cb.mn[dim]) { make_leaf(B, idx, start, end, bounds); return; }
+    int mid;
+    if (n <= 2) {
+        /* nth_element on 2 elements == insertion sort by centroid[dim] */
+        mid = (start + end) / 2;
+        if (B->pi[start + 1].c[dim] < B->pi[start].c[dim]) {
+            pinfo t = B->pi[start]; B->pi[start] = B->pi[start + 1]; B->pi[start + 1] = t;
+        }
+    } else {
+        int count[12];
+        bbox bb[12];
+        for (int k = 0; k < 12; k++) { count[k] = 0; bb[k] = bb_empty(); }
+        for (int i = start; i < end; i++) {
+            int b = bucket_of(&cb, dim, B->pi[i].c);
+            count[b]++;
+            bb[b] = bb_union(bb[b], B->pi[i].b);
+        }
+        float cost[11];
+        double sa = bb_sa(bounds);
+        for (int i = 0; i < 11; i++) {
+            bbox b0 = bb_empty(), b1 = bb_empty();
+            int c0 = 0, c1 = 0;
+            for (int j = 0; j <= i; j++) { b0 = bb_union(b0, bb[j]); c0 += count[j]; }
+            for (int j = i + 1; j < 12; j++) { b1 = bb_union(b1, bb[j]); c1 += count[j]; }
+            cost[i] = (float)((double)0.125f + (c0 * bb_sa(b0) + c1 * bb_sa(b1)) / sa);
+        }
+        float min_cost = cost[0];
+        int min_b = 0;
+        for (int i = 1; i < 11; i++)
+            if (cost[i] < min_cost) { min_cost = cost[i]; min_b = i; }
+        float leaf_cost = (float)n;
+        if (n > B->max_prims || min_cost < leaf_cost) {
+            /* two-pointer partition (the classic algorithm behind std::partition
+             * for bidirectional iterators), predicate bucket <= min_b */
+            int first = start, last = end;
+            for (;;) {
+                for (;;) {
+                    if (first == last) goto done;
+                    if (bucket_of(&cb, dim, B->pi[first].c) <= min_b) first++;
+                    else break;
+                }
+                last--;
+                for (;;) {
+                    if (first == last) goto done;
+                    if (!(bucket_of(&cb, dim, B->pi[last].c) <= min_b)) last--;
+                    else break;
+                }
+                { pinfo t = B->pi[first]; B->pi[first] = B->pi[last]; B->pi[last] = t; }
+                first++;
+            }
+        done:
+            mid = first;
+        } else {
+            make_leaf(B, idx, start, end, bounds);
+            return;
+        }
+    }
+    build_rec(B, start, mid);
+    int32_t second = B->n_nodes;
+    build_rec(B, mid, end);
+    or_node *nd = &B->nodes[idx];
+    /* init_interior: bounds = Union(c0->bounds, c1->bounds) (CRT/BVH.h:105-112) */
+    bbox b0, b1;
+    memcpy(b0.mn, B->nodes[idx + 1].bmin, 12);
+    memcpy(b0.mx, B->nodes[idx + 1].bmax, 12);
+    memcpy(b1.mn, B->nodes[second].bmin, 12);
+    memcpy(b1.mx, B->nodes[second].bmax, 12);
+    bbox ib = bb_union(b0, b1);
+    memcpy(nd->bmin, ib.mn, 12);
+    memcpy(nd->bmax, ib.mx, 12);
+    nd->offset = second;
+    nd->n_prims = 0;
+    nd->axis = (uint8_t)dim;
+    nd->pad = 0;
+}
+
+/* BVHAccel(info, SAH, maxPrims) over triangles (CRT/dScene.cpp:324-379).
+ * Triangle bounds Union(Bounds3f(v0,v1),v2); centroid .5*pMin+.5*pMax
+ * (CRT/BVH.h:87-90).  nodes_out needs room for 2n-1 nodes. */
+int32_t or_bvh_build(const float *pos, int32_t n, int32_t max_prims, or_node *nodes_out,
+                     int32_t *ordered_out) {
+    if (n <= 0) return 0;
+    bvh_ctx B;
+    B.pi = (pinfo *)malloc(sizeof(pinfo) * (size_t)n);
+    for (int32_t i = 0; i < n; i++) {
+        const float *t = pos + (size_t)i * 9;
+        bbox b = bb_union_pt(bb_make(t, t + 3), t + 6);
+        B.pi[i].prim = i;
+        B.pi[i].b = b;
+        for (int k = 0; k < 3; k++) B.pi[i].c[k] = 0.5f * b.mn[k] + 0.5f * b.mx[k];
+    }
+    B.nodes = nodes_out;
+    B.n_nodes = 0;
+    B.ordered = ordered_out;
+    B.n_ordered = 0;
+    B.max_prims = max_prims;
+    build_rec(&B, 0, n);
+    free(B.pi);
+    return B.n_nodes;
+}
+
+static int depth_rec(const or_node *nodes, int32_t i) {
+    if (nodes[i].n_prims > 0) return 1;
+    int a = depth_rec(nodes, i + 1), b = depth_rec(nodes, nodes[i].offset);
+    return 1 + (a > b ? a : b);
+}
+int32_t or_bvh_max_depth(const or_node *nodes, int32_t n_nodes) {
+    return n_nodes > 0 ? depth_rec(nodes, 0) : 0;
+}
+
+/* ------------------------------------------------------------ traversal -- */
+/* Bounds3f::hit(ray, invDir, dirIsNeg) (CRT/Bounds3f.h:128-160): the line
+ * test without [0,tmax] clipping.  Returns the final tmin/tmax it computed. */
+static inline int slab(const or_node *nd, const float *o, const float *inv, const int *ng,
+                       float *tn, float *tf) {
+    float tmin = ((ng[0] ? nd->bmax[0] : nd->bmin[0]) - o[0]) * inv[0];
+    float tmax = ((ng[0] ? nd->bmin[0] : nd->bmax[0]) - o[0]) * inv[0];
+    float tymin = ((ng[1] ? nd->bmax[1] : nd->bmin[1]) - o[1]) * inv[1];
+    float tymax = ((ng[1] ? nd->bmin[1] : nd->bmax[1]) - o[1]) * inv[1];
+    if ((tmin > tymax) || (tymin > tmax)) return 0;
+    if (tymin > tmin) tmin = tymin;
+    if (tymax < tmax) tmax = tymax;
+    float tzmin = ((ng[2] ? nd->bmax[2] : nd->bmin[2]) - o[2]) * inv[2];
+    float tzmax = ((ng[2] ? nd->bmin[2] : nd->bmax[2]) - o[2]) * inv[2];
+    if ((tmin > tzmax) || (tzmin > tmax)) return 0;
+    if (tzmin > tmin) tmin = tzmin;
+    if (tzmax < tmax) tmax = tzmax;
+    *tn = tmin;
+    *tf = tmax;
+    return 1;
+}
+
+/* dTriangle::intersect with TEST_CULL (CRT/dTriangle.cu:34-81): float dots,
+ * double det, scaled u/v tests, t,u,v *= 1.0/det in double. */
+static inline int mt_cull(const float *tri, v3 o, v3 d, float *t, float *u, float *v) {
+    v3 v0 = vld(tri), e1 = vsub(vld(tri + 3), v0), e2 = vsub(vld(tri + 6), v0);
+    v3 pvec = vcross(d, e2);
+    double det = (double)vdot(e1, pvec);
+    if (det < (double)K_EPS) return 0;
+    v3 tvec = vsub(o, v0);
+    float uf = vdot(tvec, pvec);
+    if (uf < 0.0 || (double)uf > det) return 0;
+    v3 qvec = vcross(tvec, e1);
+    float vf = vdot(d, qvec);
+    if (vf < 0.0 || (double)(uf + vf) > det) return 0;
+    float tf = vdot(e2, qvec);
+    double inv_det = 1.0 / det;
+    *t = (float)((double)tf * inv_det);
+    *u = (float)((double)uf * inv_det);
+    *v = (float)((double)vf * inv_det);
+    return 1;
+}
+/* non-culling branch of dTriangle::intersect (CRT/dTriangle.cu:63-80) */
+static inline int mt_two(const float *tri, v3 o, v3 d, float *t, float *u, float *v) {
+    v3 v0 = vld(tri), e1 = vsub(vld(tri + 3), v0), e2 = vsub(vld(tri + 6), v0);
+    v3 pvec = vcross(d, e2);
+    double det = (double)vdot(e1, pvec);
+    if (det > (double)-K_EPS && det < (double)K_EPS) return 0;
+    double inv_det = 1.0 / det;
+    v3 tvec = vsub(o, v0);
+    float uf = (float)((double)vdot(tvec, pvec) * inv_det);
+    if (uf < 0.0 || uf > 1.0) return 0;
+    v3 qvec = vcross(tvec, e1);
+    float vf = (float)((double)vdot(d, qvec) * inv_det);
+    if (vf < 0.0 || (double)(uf + vf) > 1.0) return 0;
+    *t = (float)((double)vdot(e2, qvec) * inv_det);
+    *u = uf;
+    *v = vf;
+    return 1;
+}
+/* dTriangle::shadow_hit (CRT/dTriangle.cu:110-138) */
+static inline int mt_shadow(const float *tri, v3 o, v3 d, float *t) {
+    v3 v0 = vld(tri), e1 = vsub(vld(tri + 3), v0), e2 = vsub(vld(tri + 6), v0);
+    v3 pvec = vcross(d, e2);
+    double det = (double)vdot(e1, pvec);
+    if (det > (double)-K_EPS && det < (double)K_EPS) return 0;
+    double inv_det = 1.0 / det;
+    v3 tvec = vsub(o, v0);
+    double u = (double)vdot(tvec, pvec) * inv_det;
+    if (u < 0.0 || u > 1.0) return 0;
+    v3 qvec = vcross(tvec, e1);
+    double v = (double)vdot(d, qvec) * inv_det;
+    if (v < 0.0 || u + v > 1.0) return 0;
+    float tf = (float)((double)vdot(e2, qvec) * inv_det);
+    if (tf < 0) return 0;
+    *t = tf;
+    return 1;
+}
+
+typedef struct { int found; float t, u, v; int32_t id; } hitrec;
+
+#define OR_STACK 1024
+
+/* intersect() closest hit (CRT/dTriangle.cu:140-213): near child first by
+ * dirIsNeg[axis]; a leaf candidate replaces the record when 0 <= t < tmin. */
+static hitrec closest_ref(const or_node *nodes, const float *tris, v3 o, v3 d, int two_sided,
+                          int prune, int64_t *cn, int64_t *ct) {
+    hitrec h;
+    h.found = 0; h.t = K_HUGE; h.u = h.v = 0.f; h.id = -1;
+    float of[3] = {o.x, o.y, o.z};
+    float inv[3] = {1.f / d.x, 1.f / d.y, 1.f / d.z};
+    int ng[3] = {inv[0] < 0, inv[1] < 0, inv[2] < 0};
+    int32_t stack[OR_STACK];
+    int sp = 0;
+    int32_t cur = 0;
+    for (;;) {
+        const or_node *nd = &nodes[cur];
+        float tn, tf;
+        if (cn) (*cn)++;
+        int hit = slab(nd, of, inv, ng, &tn, &tf);
+        if (hit && prune) {
+            float lim = h.t + h.t * PRUNE_REL + PRUNE_ABS;
+            if (tn > lim || tf < -PRUNE_BEHIND) hit = 0;
+        }
+        if (hit) {
+            if (nd->n_prims > 0) {
+                for (int i = 0; i < nd->n_prims; i++) {
+                    int32_t slot = nd->offset + i;
+                    float t = 0.f, u = 0.f, v = 0.f;
+                    if (ct) (*ct)++;
+                    int ok = two_sided ? mt_two(tris + (size_t)slot * 9, o, d, &t, &u, &v)
+                                       : mt_cull(tris + (size_t)slot * 9, o, d, &t, &u, &v);
+                    if (ok && !(t < 0) && t < h.t) {
+                        h.found = 1; h.t = t; h.u = u; h.v = v; h.id = slot;
+                    }
+                }
+                if (sp == 0) break;
+                cur = stack[--sp];
+            } else {
+                if (sp >= OR_STACK) abort();
+                if (ng[nd->axis]) { stack[sp++] = cur + 1; cur = nd->offset; }
+                else { stack[sp++] = nd->offset; cur = cur + 1; }
+            }
+        } else {
+            if (sp == 0) break;
+            cur = stack[--sp];
+        }
+    }
+    return h;
+}
+
+/* intersect_shadows (CRT/dTriangle.cu:215-259) with tmin = K_HUGE
+ * (CRT/dEnvironmentLight.cu:60-64): true when any triangle has 0 <= t < 1e32. */
+static int any_ref(const or_node *nodes, const float *tris, v3 o, v3 d, int prune, int64_t *cn,
+                   int64_t *ct) {
+    float of[3] = {o.x, o.y, o.z};
+    float inv[3] = {1 / d.x, 1 / d.y, 1 / d.z};
+    int ng[3] = {inv[0] < 0, inv[1] < 0, inv[2] < 0};
+    int32_t stack[OR_STACK];
+    int sp = 0;
+    int32_t cur = 0;
+    for (;;) {
+        const or_node *nd = &nodes[cur];
+        float tn, tf;
+        if (cn) (*cn)++;
+        int hit = slab(nd, of, inv, ng, &tn, &tf);
+        if (hit && prune && tf < -PRUNE_BEHIND) hit = 0;
+        if (hit) {
+            if (nd->n_prims > 0) {
+                for (int i = 0; i < nd->n_prims; i++) {
+                    float t = 0.f;
+                    if (ct) (*ct)++;
+                    if (mt_shadow(tris + (size_t)(nd->offset + i) * 9, o, d, &t) && t < K_HUGE)
+                        return 1;
+                }
+                if (sp == 0) break;
+                cur = stack[--sp];
+            } else {
+                if (sp >= OR_STACK) abort();
+                if (ng[nd->axis]) { stack[sp++] = cur + 1; cur = nd->offset; }
+                else { stack[sp++] = nd->offset; cur = cur + 1; }
+            }
+        } else {
+            if (sp == 0) break;
+            cur = stack[--sp];
+        }
+    }
+    return 0;
+}
+
+void or_trace_closest(const or_node *nodes, const float *tris, const float *rays, int32_t n,
+                      int32_t two_sided, float *t, float *u, float *v, int32_t *id) {
+    for (int32_t i = 0; i < n; i++) {
+        const float *r = rays + (size_t)i * 6;
+        hitrec h = closest_ref(nodes, tris, vld(r), vld(r + 3), two_sided, 0, NULL, NULL);
+        t[i] = h.t; u[i] = h.u; v[i] = h.v; id[i] = h.id;
+    }
+}
+
+/* brute force over every triangle slot in order, same test and strict '<' */
+void or_trace_closest_brute(int32_t n_tris, const float *tris, const float *rays, int32_t n,
+                            int32_t two_sided, float *t, int32_t *id) {
+    for (int32_t i = 0; i < n; i++) {
+        const float *r = rays + (size_t)i * 6;
+        v3 o = vld(r), d = vld(r + 3);
+        float best = K_HUGE;
+        int32_t bid = -1;
+        for (int32_t k = 0; k < n_tris; k++) {
+            float tt = 0.f, uu, vv;
+            int ok = two_sided ? mt_two(tris + (size_t)k * 9, o, d, &tt, &uu, &vv)
+                               : mt_cull(tris + (size_t)k * 9, o, d, &tt, &uu, &vv);
+            if (ok && !(tt < 0) && tt < best) { best = tt; bid = k; }
+        }
+        t[i] = best; id[i] = bid;
+    }
+}
+
+void or_trace_any(const or_node *nodes, const float *tris, const float *rays, int32_t n,
+                  uint8_t *occluded) {
+    for (int32_t i = 0; i < n; i++) {
+        const float *r = rays + (size_t)i * 6;
+        occluded[i] = (uint8_t)any_ref(nodes, tris, vld(r), vld(r + 3), 0, NULL, NULL);
+    }
+}
+
+void or_trace_any_brute(int32_t n_tris, const float *tris, const float *rays, int32_t n,
+                        uint8_t *occluded) {
+    for (int32_t i = 0; i < n; i++) {
+        const float *r = rays + (size_t)i * 6;
+        v3 o = vld(r), d = vld(r + 3);
+        uint8_t occ = 0;
+        for (int32_t k = 0; k < n_tris && !occ; k++) {
+            float tt;
+            if (mt_shadow(tris + (size_t)k * 9, o, d, &tt) && tt < K_HUGE) occ = 1;
+        }
+        occluded[i] = occ;
+    }
+}
+
+/* Counting pass for the roofline: boxes and triangles tested per ray by the
+ * traversal the GPU runs (same order; prune=1 adds the GPU's conservative
+ * distance pruning). */
+void or_count_traversal(const or_node *nodes, const float *tris, const float *rays, int32_t n,
+                        int32_t any_hit, int32_t prune, int64_t *nodes_tested,
+                        int64_t *tris_tested) {
+    int64_t cn = 0, ct = 0;
+    for (int32_t i = 0; i < n; i++) {
+        const float *r = rays + (size_t)i * 6;
+        if (any_hit) any_ref(nodes, tris, vld(r), vld(r + 3), prune, &cn, &ct);
+        else closest_ref(nodes, tris, vld(r), vld(r + 3), 0, prune, &cn, &ct);
+    }
+    *nodes_tested = cn;
+    *tris_tested = ct;
+}
+
+/* --------------------------------------------------------- environment -- */
+/* Software bilinear filter with wrap addressing on normalized coordinates
+ * (replaces tex2DLod of a linear/wrap texture, CRT/dTexture.cu:251-255;
+ * hardware filter weights are parity unpinned). */
+void or_bilinear(const float *tex, int32_t w, int32_t h, int32_t c, float u, float v, float *out) {
+    float uu = u - floorf(u), vv = v - floorf(v);
+    float x = uu * (float)w - 0.5f, y = vv * (float)h - 0.5f;
+    float fx = floorf(x), fy = floorf(y);
+    float a = x - fx, b = y - fy;
+    int i0 = (int)fx, j0 = (int)fy;
+    i0 = ((i0 % w) + w) % w;
+    j0 = ((j0 % h) + h) % h;
+    int i1 = (i0 + 1) % w, j1 = (j0 + 1) % h;
+    float w00 = (1.f - a) * (1.f - b), w10 = a * (1.f - b), w01 = (1.f - a) * b, w11 = a * b;
+    for (int k = 0; k < c; k++) {
+        float t00 = tex[((size_t)j0 * w + i0) * c + k], t10 = tex[((size_t)j0 * w + i1) * c + k];
+        float t01 = tex[((size_t)j1 * w + i0) * c + k], t11 = tex[((size_t)j1 * w + i1) * c + k];
+        out[k] = ((w00 * t00 + w10 * t10) + w01 * t01) + w11 * t11;
+    }
+}
+
+/* luminance returns float of a double expression (CRT/dMath.cuh:1503-1507) */
+static inline float lum_of(const float *s) {
+    return (float)(0.299 * s[0] + 0.587 * s[1] + 0.114 * s[2]);
+}
+static inline float lum_at(const float *rgb, int32_t w, int32_t h, float u, float v) {
+    float s[3];
+    or_bilinear(rgb, w, h, 3, u, v, s);
+    return lum_of(s);
+}
+
+/* g_compute_pdf_denom / g_compute_marginal_dist / g_compute_conditional_dist /
+ * g_write_pdf_texture (CRT/kernel.cu:65-149), serial FP64 order of the
+ * reference.  conds is [h][w]. */
+void or_env_build(const float *rgb, int32_t w, int32_t h, double *marginal_y, double *marginal_p,
+                  double *conds, float *pdf, double *denom_out) {
+    float *lum = (float *)malloc(sizeof(float) * (size_t)w * h);
+    for (int32_t j = 0; j < h; j++) {
+        float v = (float)j / (float)h;
+        for (int32_t i = 0; i < w; i++) lum[(size_t)j * w + i] = lum_at(rgb, w, h, (float)i / (float)w, v);
+    }
+    double denom = 0.0;
+    for (int32_t j = 0; j < h; j++) {
+        float v = (float)j / (float)h;
+        double st = (double)f_sin(F_PI * v);
+        for (int32_t i = 0; i < w; i++) denom += (double)lum[(size_t)j * w + i] * st;
+    }
+    for (int32_t j = 0; j < h; j++) {
+        float v = (float)j / (float)h;
+        double st = (double)f_sin(F_PI * v) / denom;
+        marginal_p[j] = 0.0;
+        for (int32_t i = 0; i < w; i++) marginal_p[j] += (double)lum[(size_t)j * w + i] * st;
+        marginal_y[j] = j ? marginal_p[j] + marginal_y[j - 1] : marginal_p[j];
+    }
+    for (int32_t y = 0; y < h; y++) {
+        float v = (float)y / (float)h;
+        double st = (double)f_sin(F_PI * v);
+        double val = st / (denom * marginal_p[y]);
+        double *row = conds + (size_t)y * w;
+        for (int32_t x = 0; x < w; x++) {
+            row[x] = (double)lum[(size_t)y * w + x] * val;
+            if (x) row[x] += row[x - 1];
+        }
+    }
+    float scale = (float)(w * h) / (2.f * F_PI * F_PI);
+    for (int32_t y = 0; y < h; y++) {
+        float v = (float)y / (float)h;
+        double st = (double)f_sin(F_PI * v);
+        for (int32_t x = 0; x < w; x++) {
+            float p = (float)(((double)lum[(size_t)y * w + x] * st) / denom);
+            p *= scale;
+            pdf[(size_t)y * w + x] = p;
+        }
+    }
+    if (denom_out) *denom_out = denom;
+    free(lum);
+}
+
+/* upper_bound with its trailing adjustment list[left <= val] (CRT/Helpers.cu:16-31) */
+int32_t or_upper_bound(const double *list, int32_t size, float val) {
+    int32_t left = 0, right = size;
+    while (left < right) {
+        int32_t middle = (right - left) / 2 + left;
+        if ((double)val >= list[middle]) left = middle + 1;
+        else right = middle;
+    }
+    if (left < size && list[((float)left <= val) ? 1 : 0] != 0.0) left++;
+    return left;
+}
+
+/* sample_spherical_direction (CRT/dMath.cuh:1491-1501) */
+static inline v3 sph_dir(float u, float v) {
+    float theta = F_2PI * u, phi = F_PI * v;
+    float x = f_cos(theta) * f_sin(phi), y = f_cos(phi), z = f_sin(theta) * f_sin(phi);
+    return V3(-x, -y, -z);
+}
+/* sample_spherical_map (CRT/dMath.cuh:1483-1489) */
+static inline void sph_map(v3 d, float *u, float *v) {
+    *u = f_atan2(d.z, d.x) * F_1_2PI + 0.5f;
+    *v = f_asin(d.y) * F_1_PI + 0.5f;
+}
+
+/* inverse-CDF texel pick of dEnvironmentLight::get_direction
+ * (CRT/dEnvironmentLight.cu:41-51); indices clamped at 0 (reference UB). */
+void or_env_sample_dir(const double *marginal_y, const double *conds, int32_t w, int32_t h,
+                       float ex, float ey, int32_t *xo, int32_t *yo, float *dir) {
+    int32_t y = (int32_t)((float)or_upper_bound(marginal_y, h, ey) - 1.f);
+    if (y < 0) y = 0;
+    int32_t x = (int32_t)((float)or_upper_bound(conds + (size_t)y * w, w, ex) - 1.f);
+    if (x < 0) x = 0;
+    v3 d = sph_dir((float)x / (float)w, (float)y / (float)h);
+    *xo = x; *yo = y;
+    dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+}
+
+/* dEnvironmentLight::get_pdf(isect, wi) (CRT/dEnvironmentLight.cu:83-103) */
+float or_env_pdf(const float *pdf, int32_t w, int32_t h, const float *wi) {
+    float u, v;
+    sph_map(V3(wi[0], wi[1], wi[2]), &u, &v);
+    int px = (int)(u * (float)(w - 1)), py = (int)(v * (float)(h - 1));
+    if (px < 0) px = 0;
+    if (px > w - 1) px = w - 1;
+    if (py < 0) py = 0;
+    if (py > h - 1) py = h - 1;
+    float z = wi[2] < -1.f ? -1.f : (wi[2] > 1.f ? 1.f : wi[2]);
+    return pdf[(size_t)py * w + px] / f_sin(f_acos(z));
+}
+
+/* --------------------------------------------------------------- camera -- */
+/* column-major Matrix4x4 * float4 (CRT/dMatrix.cuh:61-69) */
+static inline void mat_mul4(const float *m, const float *v, float *out) {
+    for (int r = 0; r < 4; r++)
+        out[r] = m[0 * 4 + r] * v[0] + m[1 * 4 + r] * v[1] + m[2 * 4 + r] * v[2] + m[3 * 4 + r] * v[3];
+}
+/* ConcentricSampleDisk (CRT/dRandom.cu:104-126) */
+static inline void concentric(float u0, float u1, float *dx, float *dy) {
+    float ox = 2.f * u0 - 1.f, oy = 2.f * u1 - 1.f;
+    if (ox == 0 && oy == 0) { *dx = 0.f; *dy = 0.f; return; }
+    float theta, r;
+    if (fabsf(ox) > fabsf(oy)) { r = ox; theta = F_PI_4 * (oy / ox); }
+    else { r = oy; theta = F_PI_2 - F_PI_4 * (ox / oy); }
+    *dx = r * f_cos(theta);
+    *dy = r * f_sin(theta);
+}
+/* dCamera::gen_ray (CRT/dCamera.cu:10-36); AA jitter and lens draw from the
+ * counter RNG at bounce 0. */
+void or_gen_ray(const or_scene *s, int32_t width, int32_t height, int32_t px, int32_t py,
+                uint32_t base, float *ray) {
+    float nx = (float)(2.0f * (((double)px + 0.5) / (double)width) - 1.0f);
+    float ny = (float)(1.0f - 2.0f * (((double)py + 0.5) / (double)height));
+    float a[4] = {nx, ny, -1.f, 1.f}, b[4] = {nx, ny, 1.f, 1.f}, pa[4], pb[4];
+    mat_mul4(s->cam_inv_vp, a, pa);
+    mat_mul4(s->cam_inv_vp, b, pb);
+    v3 pn = V3(pa[0] / pa[3], pa[1] / pa[3], pa[2] / pa[3]);
+    v3 pf = V3(pb[0] / pb[3], pb[1] / pb[3], pb[2] / pb[3]);
+    float sx = or_rng(base, 0, DIM_JX) * 0.005f, sy = or_rng(base, 0, DIM_JY) * 0.005f;
+    v3 o = vadd(pn, V3(sx, sy, 0.f));
+    v3 d = vnorm(vsub(pf, pn));
+    if (s->lens_radius > 0.f) {
+        v3 pfoc = vadd(o, vscl(d, s->focal));
+        float lx, ly;
+        concentric(or_rng(base, 0, DIM_LX), or_rng(base, 0, DIM_LY), &lx, &ly);
+        o = vadd(o, V3(lx * s->lens_radius, ly * s->lens_radius, 0.f));
+        d = vnorm(vsub(pfoc, o));
+    }
+    ray[0] = o.x; ray[1] = o.y; ray[2] = o.z;
+    ray[3] = d.x; ray[4] = d.y; ray[5] = d.z;
+}
+
+/* ---------------------------------------------------------------- BSDFs -- */
+static inline float pow5(float x) { float x2 = x * x; return (x2 * x2) * x; }
+/* fresnel (CRT/dMaterial.cu:90-92), pow(x,5) as explicit products */
+static inline v3 fresnel_s(v3 f0, v3 h, v3 wo) {
+    float p = pow5(1.f - fmaxf(0.f, vdot(h, wo)));
+    return vadd(f0, vscl(V3(1.f - f0.x, 1.f - f0.y, 1.f - f0.z), p));
+}
+static inline double power_heur(double f, double g) { return (f * f) / (f * f + g * g); }
+/* ggxtr_ndf (CRT/dMaterial.cu:120-125) */
+static double ggx_ndf(v3 n, v3 h, float r) {
+    double a2 = (double)(r * r * r * r);
+    float nh = fmaxf(0.0f, vdot(n, h));
+    double nh2 = (double)(nh * nh);
+    double q = nh2 * (a2 - 1.0) + 1.0;
+    return a2 / ((double)F_PI * (q * q));
+}
+/* geo_atten (CRT/dMaterial.cu:126-137) */
+static double geo_att(v3 wi, v3 wo, v3 n, float r) {
+    float rp = r + 1.f;
+    double k = (double)((rp * rp) / 8.f);
+    double nl = (double)fmaxf(vdot(n, wi), 0.0f), nv = (double)fmaxf(vdot(n, wo), 0.0f);
+    double g1 = nl / (nl * (1.0 - k) + k), g2 = nv / (nv * (1.0 - k) + k);
+    return g1 * g2;
+}
+/* spec_get_pdf (CRT/dMaterial.cu:255-265) */
+static float spec_pdf(v3 n, v3 wi, v3 wo, float r) {
+    v3 wh = vnorm(vadd(wo, wi));
+    double whn = (double)fmaxf(vdot(wh, n), 0.f), wowh = (double)fmaxf(vdot(wo, wh), 0.f);
+    double D = ggx_ndf(n, wh, r);
+    float den = fmaxf((float)(4.0 * wowh), 0.001f);
+    return (float)((D * whn) / (double)den);
+}
+/* spec_f (CRT/dMaterial.cu:266-288) */
+static v3 spec_f(v3 n, v3 wi, v3 wo, float r, float m, v3 alb, v3 fres) {
+    v3 f0 = vlerp(fres, alb, m);
+    v3 wh = vnorm(vadd(wo, wi));
+    double ndwi = (double)fmaxf(vdot(n, wi), 0.f), ndwo = (double)fmaxf(vdot(n, wo), 0.f);
+    double D = ggx_ndf(n, wh, r), G = geo_att(wi, wo, n, r);
+    v3 F = fresnel_s(f0, wh, wi);
+    float den = fmaxf((float)(4.0 * ndwo * ndwi), 0.001f);
+    return vdiv(vscl(F, (float)(D * G)), den);
+}
+/* diff_f (CRT/dMaterial.cu:164-178) */
+static v3 diff_f(v3 wi, v3 wo, float m, v3 alb, v3 fres) {
+    v3 f0 = vlerp(fres, alb, m);
+    v3 wh = vnorm(vadd(wo, wi));
+    v3 F = fresnel_s(f0, wh, wi);
+    v3 kd = V3(1.f - F.x, 1.f - F.y, 1.f - F.z);
+    kd = vscl(kd, (float)(1.0 - (double)m));
+    return vscl(vmul(kd, alb), F_1_PI);
+}
+/* get_orthogonal_vec (CRT/dMath.cuh:1450-1463) */
+static inline v3 ortho_axis(v3 in) {
+    if (fabsf(in.x) < 0.57735026919f) return V3(1, 0, 0);
+    if (fabsf(in.y) < 0.57735026919f) return V3(0, 1, 0);
+    return V3(0, 0, 1);
+}
+/* diff_sample (CRT/dMaterial.cu:139-159): uniform hemisphere around N */
+static v3 diff_sample(v3 N, float e0, float e1) {
+    float sin_t = sqrtf(1.f - e0 * e0);
+    float phi = 2.f * F_PI * e1;
+    float x = sin_t * f_cos(phi), z = sin_t * f_sin(phi);
+    v3 T = vnorm(vcross(N, ortho_axis(N)));
+    v3 B = vnorm(vcross(N, T));
+    return vnorm(vadd(vadd(vscl(T, x), vscl(N, e0)), vscl(B, z)));
+}
+/* spec_sample (CRT/dMaterial.cu:229-254): GGX half vector, reflect wo */
+static v3 spec_sample(v3 N, v3 wo, float r, float e0, float e1) {
+    double theta = (double)f_atan(r * r * sqrtf(e0 / (1.f - e0)));
+    double phi = (double)((2 * F_PI) * e1);
+    v3 h = V3((float)(sin(theta) * cos(phi)), (float)cos(theta), (float)(sin(theta) * sin(phi)));
+    v3 T = vnorm(vcross(N, ortho_axis(N)));
+    v3 B = vnorm(vcross(N, T));
+    v3 smp = vadd(vadd(vscl(T, h.x), vscl(N, h.y)), vscl(B, h.z));
+    return vneg(vreflect(wo, vnorm(smp)));
+}
+/* Exact unpolarised Fresnel reflectance of a smooth dielectric interface
+ * (textbook Fresnel equations).  No reference counterpart: the reference has
+ * no dielectric (only the unused refract helper, CRT/dMaterial.cu:97-113). */
+float or_fresnel_dielectric(float cos_i, float eta_i, float eta_t) {
+    float sin_i = sqrtf(fmaxf(0.f, 1.f - cos_i * cos_i));
+    float sin_t = eta_i / eta_t * sin_i;
+    if (sin_t >= 1.f) return 1.f;
+    float cos_t = sqrtf(fmaxf(0.f, 1.f - sin_t * sin_t));
+    float rpar = (eta_t * cos_i - eta_i * cos_t) / (eta_t * cos_i + eta_i * cos_t);
+    float rper = (eta_i * cos_i - eta_t * cos_t) / (eta_i * cos_i + eta_t * cos_t);
+    return 0.5f * (rpar * rpar + rper * rper);
+}
+
+/* exported wrappers for the BSDF tests */
+double or_ggx_ndf(const float *n, const float *h, float r) { return ggx_ndf(vld(n), vld(h), r); }
+double or_geo_atten(const float *wi, const float *wo, const float *n, float r) {
+    return geo_att(vld(wi), vld(wo), vld(n), r);
+}
+float or_spec_pdf(const float *n, const float *wi, const float *wo, float r) {
+    return spec_pdf(vld(n), vld(wi), vld(wo), r);
+}
+void or_spec_f(const float *n, const float *wi, const float *wo, float r, float m,
+               const float *albedo, const float *fres, float *out) {
+    v3 f = spec_f(vld(n), vld(wi), vld(wo), r, m, vld(albedo), vld(fres));
+    out[0] = f.x; out[1] = f.y; out[2] = f.z;
+}
+void or_diff_f(const float *wi, const float *wo, float m, const float *albedo, const float *fres,
+               float *out) {
+    v3 f = diff_f(vld(wi), vld(wo), m, vld(albedo), vld(fres));
+    out[0] = f.x; out[1] = f.y; out[2] = f.z;
+}
+void or_diff_sample(const float *n, float e0, float e1, float *out) {
+    v3 d = diff_sample(vld(n), e0, e1);
+    out[0] = d.x; out[1] = d.y; out[2] = d.z;
+}
+void or_spec_sample(const float *n, const float *wo, float r, float e0, float e1, float *out) {
+    v3 d = spec_sample(vld(n), vld(wo), r, e0, e1);
+    out[0] = d.x; out[1] = d.y; out[2] = d.z;
+}
+
+/* --------------------------------------------------------------- render -- */
+typedef struct {
+    const or_scene *s;
+    const or_params *p;
+    float *tri, *tnrm, *tuv;   /* BVH-slot order */
+    int32_t *tmat;
+    double *my, *mp, *conds;
+    float *pdf;
+    float *film;
+    int next_row;
+    pthread_mutex_t mu;
+    or_stats st;
+} rctx;
+
+static inline v3 env_L(const rctx *c, v3 wi) {
+    const or_scene *s = c->s;
+    if (!s->env_rgb) return V3(s->env_color[0], s->env_color[1], s->env_color[2]);
+    float u, v, o[3];
+    sph_map(wi, &u, &v);
+    or_bilinear(s->env_rgb, s->env_w, s->env_h, 3, u, v, o);
+    return vld(o);
+}
+static inline float env_pdf_c(const rctx *c, v3 wi) {
+    float w[3] = {wi.x, wi.y, wi.z};
+    return or_env_pdf(c->pdf, c->s->env_w, c->s->env_h, w);
+}
+
+typedef struct { double prim, ext, shd; } pstat;
+
+/* One sample of one pixel through the wavefront state machine of
+ * wf_generate / wf_extend / wf_logic / wf_mat_* / wf_shadow
+ * (CRT/kernel.cu:297-651), with deferred visibility of the BRDF-sampled
+ * MIS rays (same sums as BRDF_L, CRT/dMaterial.cu:363-458). */
+static v3 trace_path(const rctx *c, int32_t px, int32_t py, int32_t sample, pstat *ps) {
+    const or_scene *s = c->s;
+    const or_params *p = c->p;
+    uint32_t pixel = (uint32_t)py * (uint32_t)p->width + (uint32_t)px;
+    uint32_t base = or_rng_base(p->seed, pixel, (uint32_t)sample);
+    float r6[6];
+    or_gen_ray(s, p->width, p->height, px, py, base, r6);
+    v3 o = vld(r6), d = vld(r6 + 3);
+    v3 beta = V3(1.f, 1.f, 1.f), Lp = V3(0.f, 0.f, 0.f);
+    int inside = 0, prev_delta = 1;
+    hitrec h = closest_ref(s->nodes, c->tri, o, d, 0, 0, NULL, NULL);
+    ps->prim += 1;
+    uint32_t len = 1;
+    v3 A = V3(0, 0, 0), Bs = A, Cd = A, fext = A;
+    int visL = 0, visS = 0, visD = 0;
+    float pdf_ext = 0.f, cos_ext = 0.f;
+    const float *M = NULL;
+    for (;;) {
+        /* ---------------- logic (wf_logic) ---------------- */
+        int term = (len > (uint32_t)p->max_depth) || !h.found;
+        v3 P = V3(0, 0, 0), n = P, ng = P;
+        v3 uvv = P;
+        if (h.found) {
+            const float *T = c->tri + (size_t)h.id * 9;
+            const float *N = c->tnrm + (size_t)h.id * 9;
+            const float *U = c->tuv + (size_t)h.id * 6;
+            M = s->mats + (size_t)c->tmat[h.id] * 16;
+            P = vadd(o, vscl(d, h.t));
+            float w0 = (1.f - h.u) - h.v;
+            n = vnorm(vadd(vadd(vscl(vld(N + 3), h.u), vscl(vld(N + 6), h.v)), vscl(vld(N), w0)));
+            uvv.x = (U[2] * h.u + U[4] * h.v) + U[0] * w0;
+            uvv.y = (U[3] * h.u + U[5] * h.v) + U[1] * w0;
+            v3 e1 = vsub(vld(T + 3), vld(T)), e2 = vsub(vld(T + 6), vld(T));
+            ng = vcross(e1, e2);
+            int front = vdot(d, ng) < 0.f;
+            if (inside && (int)M[0] != OR_MAT_DIELECTRIC && !front) term = 1; /* absorbed */
+        }
+        if (!h.found && prev_delta) Lp = vadd(Lp, vmul(beta, env_L(c, d)));
+        if (len > 1) {
+            if (visL) {
+                v3 lb = A;
+                if (visS) lb = vadd(lb, Bs);
+                if (visD) lb = vadd(lb, Cd);
+                Lp = vadd(Lp, vmul(lb, beta));
+            }
+            if (vzero(fext) || pdf_ext == 0.f) { term = 1; goto TERM; }
+            beta = vmul(beta, vdiv(vscl(fext, cos_ext), pdf_ext));
+            if (len > 3) {
+                float q = fmaxf(0.05f, 1.f - beta.y);
+                if (or_rng(base, len, DIM_RR) < q) term = 1;
+                goto TERM;
+            }
+        }
+    TERM:
+        if (term) break;
+
+        /* ---------------- material parameters ---------------- */
+        int type = (int)M[0];
+        v3 alb = V3(M[1], M[2], M[3]);
+        float rough = M[4], metal = M[5];
+        v3 fres = V3(M[6], M[7], M[8]);
+        int at = (int)M[10], rt = (int)M[11];
+        if (at >= 0) {
+            float t3[3];
+            or_bilinear(s->tex[at], s->tex_w[at], s->tex_h[at], 3, uvv.x, uvv.y, t3);
+            alb = vld(t3);
+        }
+        if (rt >= 0) {
+            float t1[3];
+            or_bilinear(s->tex[rt], s->tex_w[rt], s->tex_h[rt], 1, uvv.x, uvv.y, t1);
+            rough = fmaxf(t1[0], 0.01f);
+        }
+        v3 wo = vneg(d);
+        v3 next_o = P, next_d = d;
+        int needL = 0, needS = 0, needD = 0;
+        v3 lo = P, ld = P, so = P, sd = P, dd_o = P, dd_d = P;
+        A = V3(0, 0, 0); Bs = A; Cd = A;
+
+        if (type == OR_MAT_DIELECTRIC) {
+            /* new queue: smooth dielectric (no reference counterpart) */
+            v3 gn = vnorm(ng);
+            float cs = vdot(d, gn);
+            int entering = cs < 0.f;
+            v3 nn = entering ? gn : vneg(gn);
+            float cos_i = fabsf(cs);
+            if (cos_i > 1.f) cos_i = 1.f;
+            float ior = M[9];
+            float eta_i = entering ? 1.f : ior, eta_t = entering ? ior : 1.f;
+            float F = or_fresnel_dielectric(cos_i, eta_i, eta_t);
+            if (or_rng(base, len, DIM_DIEL) < F) {
+                next_d = vnorm(vadd(d, vscl(nn, 2.f * cos_i)));
+                next_o = vadd(P, vscl(nn, 1e-4f));
+            } else {
+                float eta = eta_i / eta_t;
+                float k = 1.f - eta * eta * (1.f - cos_i * cos_i);
+                float cos_t = sqrtf(fmaxf(k, 0.f));
+                next_d = vnorm(vadd(vscl(d, eta), vscl(nn, eta * cos_i - cos_t)));
+                next_o = vsub(P, vscl(nn, 1e-4f));
+                inside = !inside;
+            }
+            fext = V3(1.f, 1.f, 1.f);
+            pdf_ext = 1.f;
+            cos_ext = 1.f;
+            prev_delta = 1;
+        } else {
+            /* env light sample in logic (CRT/kernel.cu:435-439) */
+            float ex = or_rng(base, len, DIM_EX), ey = or_rng(base, len, DIM_EY);
+            int32_t tx, ty;
+            float dir3[3];
+            or_env_sample_dir(c->my, c->conds, s->env_w, s->env_h, ex, ey, &tx, &ty, dir3);
+            v3 spt = vadd(V3(0.f, 0.f, 0.f), vscl(vld(dir3), 10000.f));
+            v3 wi = vnorm(vsub(spt, P));
+            lo = vadd(P, vscl(n, 0.001f));
+            ld = wi;
+            double diff_pdf = (double)(1.f / (2.f * F_PI));
+            int do_spec = (type == OR_MAT_MIX || type == OR_MAT_CONDUCTOR);
+            int do_diff = (type == OR_MAT_MIX || type == OR_MAT_DIFFUSE);
+            /* light-sample term of BRDF_L (CRT/dMaterial.cu:376-402) */
+            v3 Li = env_L(c, wi);
+            float ndwi = vdot(n, wi);
+            if (ndwi > 0.f) {
+                double sp = do_spec ? (double)spec_pdf(n, wi, wo, rough) : 0.0;
+                double lp = (double)env_pdf_c(c, wi);
+                /* lp is infinite where sin(acos(wi.z)) == 0; the term's limit there is 0 */
+                if (lp > 0.f && lp < INFINITY && !vzero(Li)) {
+                    double dwt = power_heur(lp, diff_pdf), swt = power_heur(lp, sp);
+                    if (do_spec) {
+                        v3 f = vscl(spec_f(n, wi, wo, rough, metal, alb, fres), ndwi);
+                        if (!vzero(f)) A = vadd(A, vdiv(vmul(vscl(f, (float)swt), Li), (float)lp));
+                    }
+                    if (do_diff) {
+                        v3 f = vscl(diff_f(wi, wo, metal, alb, fres), ndwi);
+                        if (!vzero(f)) A = vadd(A, vdiv(vmul(vscl(f, (float)dwt), Li), (float)lp));
+                    }
+                }
+            }
+            /* BRDF-sampled MIS terms (CRT/dMaterial.cu:404-448) */
+            v3 wi_s = V3(0, 0, 0), wi_d = wi_s;
+            if (do_spec)
+                wi_s = spec_sample(n, wo, rough, or_rng(base, len, DIM_S0), or_rng(base, len, DIM_S1));
+            if (do_diff) wi_d = diff_sample(n, or_rng(base, len, DIM_D0), or_rng(base, len, DIM_D1));
+            if (do_spec) {
+                float nd = vdot(n, wi_s);
+                if (nd > 0.f) {
+                    double sp = (double)spec_pdf(n, wi_s, wo, rough);
+                    double lp = (double)env_pdf_c(c, wi_s);
+                    double wt = power_heur(sp, lp);
+                    v3 f = vscl(spec_f(n, wi_s, wo, rough, metal, alb, fres), nd);
+                    v3 L2 = env_L(c, wi_s);
+                    if (!vzero(f) && sp > 0.f && !vzero(L2)) {
+                        Bs = vdiv(vscl(vmul(f, L2), (float)wt), (float)sp);
+                        needS = 1;
+                        so = vadd(P, vscl(wi_s, 0.001f));
+                        sd = wi_s;
+                    }
+                }
+            }
+            if (do_diff) {
+                float nd = vdot(n, wi_d);
+                if (nd > 0.f) {
+                    double lp = (double)env_pdf_c(c, wi_d);
+                    double wt = power_heur(diff_pdf, lp);
+                    v3 f = vscl(diff_f(wi_d, wo, metal, alb, fres), nd);
+                    v3 L3 = env_L(c, wi_d);
+                    if (!vzero(f) && diff_pdf > 0.f && !vzero(L3)) {
+                        Cd = vdiv(vscl(vmul(f, L3), (float)wt), (float)diff_pdf);
+                        needD = 1;
+                        /* the mixed material offsets this ray along wi_spec (CRT/dMaterial.cu:431) */
+                        dd_o = vadd(P, vscl(type == OR_MAT_MIX ? wi_s : wi_d, 0.001f));
+                        dd_d = wi_d;
+                    }
+                }
+            }
+            needL = !vzero(A) || needS || needD;
+            /* continuation direction and its f / pdf / cosine */
+            v3 dir;
+            if (type == OR_MAT_MIX) {
+                dir = (or_rng(base, len, DIM_LOBE) < 0.5f) ? wi_d : wi_s;
+                fext = vadd(spec_f(n, dir, wo, rough, metal, alb, fres), diff_f(dir, wo, metal, alb, fres));
+                float spf = spec_pdf(n, dir, wo, rough);
+                float dpf = (float)diff_pdf;
+                pdf_ext = (float)(0.5 * (double)(spf + dpf));
+            } else if (type == OR_MAT_DIFFUSE) {
+                dir = wi_d;
+                fext = diff_f(dir, wo, metal, alb, fres);
+                pdf_ext = (float)diff_pdf;
+            } else {
+                dir = wi_s;
+                fext = spec_f(n, dir, wo, rough, metal, alb, fres);
+                pdf_ext = spec_pdf(n, dir, wo, rough);
+            }
+            cos_ext = fmaxf(0.0f, vdot(n, dir));
+            next_o = P;
+            next_d = dir;
+            prev_delta = 0;
+        }
+        /* ---------------- shadow (wf_shadow) ---------------- */
+        visL = visS = visD = 0;
+        if (needL) { visL = !any_ref(s->nodes, c->tri, lo, ld, 0, NULL, NULL); ps->shd += 1; }
+        if (needS) { visS = !any_ref(s->nodes, c->tri, so, sd, 0, NULL, NULL); ps->shd += 1; }
+        if (needD) { visD = !any_ref(s->nodes, c->tri, dd_o, dd_d, 0, NULL, NULL); ps->shd += 1; }
+        /* ---------------- extend (wf_extend) ---------------- */
+        o = next_o;
+        d = next_d;
+        h = closest_ref(s->nodes, c->tri, o, d, inside, 0, NULL, NULL);
+        ps->ext += 1;
+        len++;
+    }
+    return Lp;
+}
+
+static void *render_worker(void *arg) {
+    rctx *c = (rctx *)arg;
+    const or_params *p = c->p;
+    int K = p->spp_batch > 0 ? p->spp_batch : 1;
+    v3 *layer = (v3 *)malloc(sizeof(v3) * (size_t)K);
+    pstat ps = {0, 0, 0};
+    double paths = 0;
+    for (;;) {
+        pthread_mutex_lock(&c->mu);
+        int row = c->next_row++;
+        pthread_mutex_unlock(&c->mu);
+        if (row >= p->row_end) break;
+        for (int32_t x = 0; x < p->width; x++) {
+            for (int k = 0; k < K; k++) layer[k] = V3(0, 0, 0);
+            for (int32_t smp = 0; smp < p->spp; smp++) {
+                v3 L = trace_path(c, x, row, smp, &ps);
+                layer[smp % K] = vadd(layer[smp % K], L);
+                paths += 1;
+            }
+            v3 sum = layer[0];
+            for (int k = 1; k < K; k++) sum = vadd(sum, layer[k]);
+            float *f = c->film + ((size_t)row * p->width + x) * 4;
+            f[0] = sum.x; f[1] = sum.y; f[2] = sum.z; f[3] = (float)p->spp;
+        }
+    }
+    pthread_mutex_lock(&c->mu);
+    c->st.rays_primary += ps.prim;
+    c->st.rays_extension += ps.ext;
+    c->st.rays_shadow += ps.shd;
+    c->st.paths += paths;
+    pthread_mutex_unlock(&c->mu);
+    free(layer);
+    return NULL;
+}
+
+void or_render(const or_scene *s, const or_params *p, float *film, or_stats *st) {
+    rctx c;
+    memset(&c, 0, sizeof(c));
+    c.s = s;
+    c.p = p;
+    size_t n = (size_t)s->n_tris;
+    c.tri = (float *)malloc(sizeof(float) * 9 * n);
+    c.tnrm = (float *)malloc(sizeof(float) * 9 * n);
+    c.tuv = (float *)malloc(sizeof(float) * 6 * n);
+    c.tmat = (int32_t *)malloc(sizeof(int32_t) * n);
+    for (size_t i = 0; i < n; i++) {
+        int32_t src = s->ordered[i];
+        memcpy(c.tri + i * 9, s->pos + (size_t)src * 9, 36);
+        memcpy(c.tnrm + i * 9, s->nrm + (size_t)src * 9, 36);
+        memcpy(c.tuv + i * 6, s->uv + (size_t)src * 6, 24);
+        c.tmat[i] = s->mat[src];
+    }
+    if (s->env_rgb) {
+        size_t wh = (size_t)s->env_w * s->env_h;
+        c.my = (double *)malloc(sizeof(double) * s->env_h);
+        c.mp = (double *)malloc(sizeof(double) * s->env_h);
+        c.conds = (double *)malloc(sizeof(double) * wh);
+        c.pdf = (float *)malloc(sizeof(float) * wh);
+        or_env_build(s->env_rgb, s->env_w, s->env_h, c.my, c.mp, c.conds, c.pdf, NULL);
+    }
+    c.film = film;
+    c.next_row = p->row_begin;
+    pthread_mutex_init(&c.mu, NULL);
+    int nt = p->n_threads > 0 ? p->n_threads : 1;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nt);
+    for (int i = 0; i < nt; i++) pthread_create(&th[i], NULL, render_worker, &c);
+    for (int i = 0; i < nt; i++) pthread_join(th[i], NULL);
+    free(th);
+    pthread_mutex_destroy(&c.mu);
+    if (st) *st = c.st;
+    free(c.tri); free(c.tnrm); free(c.tuv); free(c.tmat);
+    free(c.my); free(c.mp); free(c.conds); free(c.pdf);
+}

@@ -1,0 +1,136 @@
+/*
+ * ff_oracle.h -- CPU oracle for the Firefly wavefront path-tracing hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference's
+ * algorithm (JakeKurtz/Firefly, CUDA-RayTracer/) used as the checker for the
+ * HIP implementation in firefly_amd/.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it; the product never links or calls it.
+ *
+ * Every function cites the reference file:line it restates.  How each function
+ * is pinned (closed forms, brute force, invariants) is listed in the header of
+ * ff_oracle.c; functions without an independent pin are marked
+ * "parity unpinned" there.
+ */
+#ifndef FF_ORACLE_H
+#define FF_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 32-byte flattened BVH node; same fields as LinearBVHNode (CRT/BVH.h:63-72). */
+typedef struct {
+    float bmin[3];
+    float bmax[3];
+    int32_t offset;   /* leaf: first primitive slot; interior: second child index */
+    uint16_t n_prims; /* 0 => interior */
+    uint8_t axis;
+    uint8_t pad;
+} or_node;
+
+/* Material record: 16 floats.  [0] type (0 mix, 1 diffuse, 2 conductor,
+ * 3 dielectric), [1..3] base colour, [4] roughness, [5] metallic,
+ * [6..8] fresnel f0, [9] ior, [10] albedo texture (-1 none),
+ * [11] roughness texture (-1 none), [12..15] reserved.
+ * Fields follow dMaterial (CRT/dMaterial.cuh:9-31). */
+#define OR_MAT_MIX 0
+#define OR_MAT_DIFFUSE 1
+#define OR_MAT_CONDUCTOR 2
+#define OR_MAT_DIELECTRIC 3
+
+typedef struct {
+    int32_t n_tris;
+    const float *pos;      /* [n][3 verts][3]  original triangle order   */
+    const float *nrm;      /* [n][3][3]                                  */
+    const float *uv;       /* [n][3][2]                                  */
+    const int32_t *mat;    /* [n]                                        */
+    int32_t n_mats;
+    const float *mats;     /* [n_mats][16]                               */
+    int32_t n_tex;
+    const float *const *tex; /* per texture: [h][w][c] float texels     */
+    const int32_t *tex_w, *tex_h, *tex_c;
+    int32_t env_w, env_h;  /* 0 => constant environment colour          */
+    const float *env_rgb;  /* [h][w][3], row j at v = (j+0.5)/h          */
+    float env_color[3];
+    float cam_inv_vp[16];  /* column-major: column c, row r at [c*4+r]   */
+    float lens_radius, focal, exposure;
+    int32_t n_nodes;       /* BVH from or_bvh_build over pos             */
+    const or_node *nodes;
+    const int32_t *ordered;/* BVH slot -> original triangle              */
+} or_scene;
+
+typedef struct {
+    int32_t width, height, spp, max_depth;
+    uint32_t seed;
+    int32_t spp_batch;     /* K: sample s accumulates into film layer s % K */
+    int32_t n_threads;
+    int32_t row_begin, row_end; /* rows rendered; [0,height) for a full frame */
+} or_params;
+
+typedef struct {
+    double rays_primary;   /* camera rays (closest hit)                  */
+    double rays_extension; /* bounce >= 1 extension rays (closest hit)   */
+    double rays_shadow;    /* any-hit rays                               */
+    double paths;          /* completed samples                          */
+} or_stats;
+
+/* ---- RNG (counter based, keyed by seed/pixel/sample/bounce/dim) ---- */
+uint32_t or_rng_base(uint32_t seed, uint32_t pixel, uint32_t sample);
+float or_rng(uint32_t base, uint32_t bounce, uint32_t dim);
+
+/* ---- BVH ---- */
+int32_t or_bvh_build(const float *pos, int32_t n, int32_t max_prims,
+                     or_node *nodes_out, int32_t *ordered_out);
+int32_t or_bvh_max_depth(const or_node *nodes, int32_t n_nodes);
+
+/* ---- traversal: rays are [n][6] (o.xyz, d.xyz); tris are [slot][9] (v0,v1,v2) ---- */
+void or_trace_closest(const or_node *nodes, const float *tris, const float *rays,
+                      int32_t n, int32_t two_sided, float *t, float *u, float *v,
+                      int32_t *id);
+void or_trace_closest_brute(int32_t n_tris, const float *tris, const float *rays,
+                            int32_t n, int32_t two_sided, float *t, int32_t *id);
+void or_trace_any(const or_node *nodes, const float *tris, const float *rays,
+                  int32_t n, uint8_t *occluded);
+void or_trace_any_brute(int32_t n_tris, const float *tris, const float *rays,
+                        int32_t n, uint8_t *occluded);
+void or_count_traversal(const or_node *nodes, const float *tris, const float *rays,
+                        int32_t n, int32_t any_hit, int32_t prune,
+                        int64_t *nodes_tested, int64_t *tris_tested);
+
+/* ---- environment CDF (FP64) and sampling ---- */
+void or_env_build(const float *rgb, int32_t w, int32_t h, double *marginal_y,
+                  double *marginal_p, double *conds, float *pdf, double *denom);
+int32_t or_upper_bound(const double *list, int32_t size, float val);
+void or_env_sample_dir(const double *marginal_y, const double *conds, int32_t w,
+                       int32_t h, float ex, float ey, int32_t *x, int32_t *y,
+                       float *dir);
+void or_bilinear(const float *tex, int32_t w, int32_t h, int32_t c, float u,
+                 float v, float *out);
+float or_env_pdf(const float *pdf, int32_t w, int32_t h, const float *wi);
+
+/* ---- camera ---- */
+void or_gen_ray(const or_scene *s, int32_t width, int32_t height, int32_t px,
+                int32_t py, uint32_t rng_base, float *ray6);
+
+/* ---- BSDF pieces (reference lobes) ---- */
+double or_ggx_ndf(const float *n, const float *h, float r);
+double or_geo_atten(const float *wi, const float *wo, const float *n, float r);
+float or_spec_pdf(const float *n, const float *wi, const float *wo, float r);
+void or_spec_f(const float *n, const float *wi, const float *wo, float r, float m,
+               const float *albedo, const float *fres, float *out);
+void or_diff_f(const float *wi, const float *wo, float m, const float *albedo,
+               const float *fres, float *out);
+void or_diff_sample(const float *n, float e0, float e1, float *out);
+void or_spec_sample(const float *n, const float *wo, float r, float e0, float e1,
+                    float *out);
+float or_fresnel_dielectric(float cos_i, float eta_i, float eta_t);
+
+/* ---- full render: film is [h][w][4] = (sum r, sum g, sum b, samples) ---- */
+void or_render(const or_scene *s, const or_params *p, float *film, or_stats *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,247 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product never uses it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+NODE_DTYPE = np.dtype([("bmin", "<f4", 3), ("bmax", "<f4", 3), ("offset", "<i4"),
+                       ("n_prims", "<u2"), ("axis", "u1"), ("pad", "u1")])
+
+
+class OrScene(C.Structure):
+    _fields_ = [("n_tris", C.c_int32), ("pos", C.c_void_p), ("nrm", C.c_void_p), ("uv", C.c_void_p),
+                ("mat", C.c_void_p), ("n_mats", C.c_int32), ("mats", C.c_void_p),
+                ("n_tex", C.c_int32), ("tex", C.c_void_p), ("tex_w", C.c_void_p),
+                ("tex_h", C.c_void_p), ("tex_c", C.c_void_p), ("env_w", C.c_int32),
+                ("env_h", C.c_int32), ("env_rgb", C.c_void_p), ("env_color", C.c_float * 3),
+                ("cam_inv_vp", C.c_float * 16), ("lens_radius", C.c_float), ("focal", C.c_float),
+                ("exposure", C.c_float), ("n_nodes", C.c_int32), ("nodes", C.c_void_p),
+                ("ordered", C.c_void_p)]
+
+
+class OrParams(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32),
+                ("max_depth", C.c_int32), ("seed", C.c_uint32), ("spp_batch", C.c_int32),
+                ("n_threads", C.c_int32), ("row_begin", C.c_int32), ("row_end", C.c_int32)]
+
+
+class OrStats(C.Structure):
+    _fields_ = [("rays_primary", C.c_double), ("rays_extension", C.c_double),
+                ("rays_shadow", C.c_double), ("paths", C.c_double)]
+
+
+_L = None
+_P = C.c_void_p
+
+
+def lib() -> C.CDLL:
+    global _L
+    if _L is None:
+        if not os.path.exists(LIB_PATH):
+            subprocess.run(["make", "-s", "-C", HERE], check=True)
+        L = C.CDLL(LIB_PATH)
+        protos = {
+            "or_rng_base": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32]),
+            "or_rng": (C.c_float, [C.c_uint32, C.c_uint32, C.c_uint32]),
+            "or_bvh_build": (C.c_int32, [_P, C.c_int32, C.c_int32, _P, _P]),
+            "or_bvh_max_depth": (C.c_int32, [_P, C.c_int32]),
+            "or_trace_closest": (None, [_P, _P, _P, C.c_int32, C.c_int32, _P, _P, _P, _P]),
+            "or_trace_closest_brute": (None, [C.c_int32, _P, _P, C.c_int32, C.c_int32, _P, _P]),
+            "or_trace_any": (None, [_P, _P, _P, C.c_int32, _P]),
+            "or_trace_any_brute": (None, [C.c_int32, _P, _P, C.c_int32, _P]),
+            "or_count_traversal": (None, [_P, _P, _P, C.c_int32, C.c_int32, C.c_int32, _P, _P]),
+            "or_env_build": (None, [_P, C.c_int32, C.c_int32, _P, _P, _P, _P, _P]),
+            "or_upper_bound": (C.c_int32, [_P, C.c_int32, C.c_float]),
+            "or_env_sample_dir": (None, [_P, _P, C.c_int32, C.c_int32, C.c_float, C.c_float, _P, _P, _P]),
+            "or_bilinear": (None, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float, _P]),
+            "or_env_pdf": (C.c_float, [_P, C.c_int32, C.c_int32, _P]),
+            "or_gen_ray": (None, [C.POINTER(OrScene), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                  C.c_uint32, _P]),
+            "or_ggx_ndf": (C.c_double, [_P, _P, C.c_float]),
+            "or_geo_atten": (C.c_double, [_P, _P, _P, C.c_float]),
+            "or_spec_pdf": (C.c_float, [_P, _P, _P, C.c_float]),
+            "or_spec_f": (None, [_P, _P, _P, C.c_float, C.c_float, _P, _P, _P]),
+            "or_diff_f": (None, [_P, _P, C.c_float, _P, _P, _P]),
+            "or_diff_sample": (None, [_P, C.c_float, C.c_float, _P]),
+            "or_spec_sample": (None, [_P, _P, C.c_float, C.c_float, C.c_float, _P]),
+            "or_fresnel_dielectric": (C.c_float, [C.c_float, C.c_float, C.c_float]),
+            "or_render": (None, [C.POINTER(OrScene), C.POINTER(OrParams), _P, C.POINTER(OrStats)]),
+        }
+        for name, (res, args) in protos.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _L = L
+    return _L
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def bvh_build(positions: np.ndarray, max_prims: int = 8):
+    pos = f32(positions).reshape(-1, 9)
+    n = pos.shape[0]
+    nodes = np.zeros(max(2 * n - 1, 1), NODE_DTYPE)
+    ordered = np.zeros(n, np.int32)
+    nn = lib().or_bvh_build(pos.ctypes.data, n, max_prims, nodes.ctypes.data, ordered.ctypes.data)
+    return nodes[:nn].copy(), ordered
+
+
+def bvh_max_depth(nodes) -> int:
+    return lib().or_bvh_max_depth(nodes.ctypes.data, len(nodes))
+
+
+def ordered_tris(positions, ordered):
+    return f32(f32(positions).reshape(-1, 9)[ordered])
+
+
+def _rays6(rays):
+    r = f32(rays)
+    if r.shape[1] == 8:
+        r = f32(np.concatenate([r[:, 0:3], r[:, 4:7]], 1))
+    return r
+
+
+def trace_closest(nodes, tris, rays, two_sided=False):
+    r = _rays6(rays)
+    n = r.shape[0]
+    t, u, v = (np.zeros(n, np.float32) for _ in range(3))
+    i = np.zeros(n, np.int32)
+    lib().or_trace_closest(nodes.ctypes.data, tris.ctypes.data, r.ctypes.data, n, int(two_sided),
+                           t.ctypes.data, u.ctypes.data, v.ctypes.data, i.ctypes.data)
+    return t, u, v, i
+
+
+def trace_closest_brute(tris, rays, two_sided=False):
+    r = _rays6(rays)
+    n = r.shape[0]
+    t = np.zeros(n, np.float32)
+    i = np.zeros(n, np.int32)
+    lib().or_trace_closest_brute(tris.shape[0], tris.ctypes.data, r.ctypes.data, n, int(two_sided),
+                                 t.ctypes.data, i.ctypes.data)
+    return t, i
+
+
+def trace_any(nodes, tris, rays):
+    r = _rays6(rays)
+    o = np.zeros(r.shape[0], np.uint8)
+    lib().or_trace_any(nodes.ctypes.data, tris.ctypes.data, r.ctypes.data, r.shape[0], o.ctypes.data)
+    return o
+
+
+def trace_any_brute(tris, rays):
+    r = _rays6(rays)
+    o = np.zeros(r.shape[0], np.uint8)
+    lib().or_trace_any_brute(tris.shape[0], tris.ctypes.data, r.ctypes.data, r.shape[0], o.ctypes.data)
+    return o
+
+
+def count_traversal(nodes, tris, rays, any_hit=False, prune=True):
+    r = _rays6(rays)
+    cn = np.zeros(1, np.int64)
+    ct = np.zeros(1, np.int64)
+    lib().or_count_traversal(nodes.ctypes.data, tris.ctypes.data, r.ctypes.data, r.shape[0],
+                             int(any_hit), int(prune), cn.ctypes.data, ct.ctypes.data)
+    return int(cn[0]), int(ct[0])
+
+
+def env_build(rgb):
+    a = f32(rgb)
+    h, w = a.shape[0], a.shape[1]
+    my, mp = np.zeros(h), np.zeros(h)
+    conds = np.zeros((h, w))
+    pdf = np.zeros((h, w), np.float32)
+    denom = np.zeros(1)
+    lib().or_env_build(a.ctypes.data, w, h, my.ctypes.data, mp.ctypes.data, conds.ctypes.data,
+                       pdf.ctypes.data, denom.ctypes.data)
+    return my, mp, conds, pdf, float(denom[0])
+
+
+def upper_bound(lst, val) -> int:
+    a = np.ascontiguousarray(lst, dtype=np.float64)
+    return lib().or_upper_bound(a.ctypes.data, len(a), float(val))
+
+
+def bilinear(tex, u, v):
+    t = f32(tex)
+    h, w = t.shape[0], t.shape[1]
+    c = t.shape[2] if t.ndim == 3 else 1
+    out = np.zeros(c, np.float32)
+    lib().or_bilinear(t.ctypes.data, w, h, c, float(u), float(v), out.ctypes.data)
+    return out
+
+
+class PreparedScene:
+    """Oracle view of a SceneData (keeps the arrays alive)."""
+
+    def __init__(self, sd, nodes=None, ordered=None):
+        self.sd = sd
+        self.pos = f32(sd.positions).reshape(-1, 9)
+        self.nrm = f32(sd.normals).reshape(-1, 9)
+        self.uv = f32(sd.uvs).reshape(-1, 6)
+        self.mat = np.ascontiguousarray(sd.material_ids, dtype=np.int32)
+        self.mats = f32(sd.materials).reshape(-1, 16)
+        self.env = f32(sd.env)
+        if nodes is None:
+            nodes, ordered = bvh_build(self.pos)
+        self.nodes, self.ordered = nodes, np.ascontiguousarray(ordered, dtype=np.int32)
+        self.texs = [f32(t) for t in sd.textures]
+        s = OrScene()
+        s.n_tris = self.pos.shape[0]
+        s.pos, s.nrm, s.uv, s.mat = (self.pos.ctypes.data, self.nrm.ctypes.data, self.uv.ctypes.data,
+                                     self.mat.ctypes.data)
+        s.n_mats, s.mats = self.mats.shape[0], self.mats.ctypes.data
+        s.n_tex = len(self.texs)
+        if self.texs:
+            self._tp = (C.c_void_p * len(self.texs))(*[t.ctypes.data for t in self.texs])
+            self._tw = np.array([t.shape[1] for t in self.texs], np.int32)
+            self._th = np.array([t.shape[0] for t in self.texs], np.int32)
+            self._tc = np.array([t.shape[2] if t.ndim == 3 else 1 for t in self.texs], np.int32)
+            s.tex = C.cast(self._tp, C.c_void_p)
+            s.tex_w, s.tex_h, s.tex_c = self._tw.ctypes.data, self._th.ctypes.data, self._tc.ctypes.data
+        s.env_h, s.env_w = self.env.shape[0], self.env.shape[1]
+        s.env_rgb = self.env.ctypes.data
+        for i, x in enumerate(f32(sd.inv_vp).reshape(16)):
+            s.cam_inv_vp[i] = float(x)
+        s.lens_radius, s.focal, s.exposure = sd.lens_radius, sd.focal, sd.exposure
+        s.n_nodes, s.nodes, s.ordered = len(self.nodes), self.nodes.ctypes.data, self.ordered.ctypes.data
+        self.s = s
+
+    def tris_ordered(self):
+        return ordered_tris(self.pos, self.ordered)
+
+    def gen_ray(self, px, py, sample, seed=None, width=None, height=None):
+        w = width or self.sd.width
+        h = height or self.sd.height
+        sd = self.sd.seed if seed is None else seed
+        base = lib().or_rng_base(sd, py * w + px, sample)
+        out = np.zeros(6, np.float32)
+        lib().or_gen_ray(C.byref(self.s), w, h, px, py, base, out.ctypes.data)
+        return out
+
+    def render(self, spp=None, max_depth=None, seed=None, spp_batch=1, n_threads=None,
+               rows=None, width=None, height=None):
+        sd = self.sd
+        p = OrParams()
+        p.width, p.height = width or sd.width, height or sd.height
+        p.spp = spp or sd.spp
+        p.max_depth = max_depth or sd.max_depth
+        p.seed = sd.seed if seed is None else seed
+        p.spp_batch = spp_batch
+        p.n_threads = n_threads or os.cpu_count() or 1
+        p.row_begin, p.row_end = rows if rows else (0, p.height)
+        film = np.zeros((p.height, p.width, 4), np.float32)
+        st = OrStats()
+        lib().or_render(C.byref(self.s), C.byref(p), film.ctypes.data, C.byref(st))
+        return film, st

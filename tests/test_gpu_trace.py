@@ -1,0 +1,136 @@
+"""GPU parity of the extend (closest hit) and shadow (any hit) stages and of
+camera-ray generation against the CPU oracle: bit-exact on identical rays."""
+import numpy as np
+import pytest
+
+from firefly_amd import abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cfg0(gpu, oracle):
+    sd = scenes.spheres_scene()
+    sc = abi.Scene(gpu, sd.positions, sd.normals, sd.uvs, sd.material_ids, sd.materials)
+    nodes, ordered = sc.bvh()
+    ps = oracle.PreparedScene(sd, nodes, ordered)
+    yield sd, sc, ps
+    sc.close()
+
+
+def _closest(sc, rays, two_sided=False):
+    import torch
+    r = torch.from_numpy(np.ascontiguousarray(rays, dtype=np.float32)).cuda()
+    h = torch.empty((len(rays), 4), dtype=torch.float32, device="cuda")
+    sc.trace_closest(r, h, two_sided)
+    torch.cuda.synchronize()
+    return h.cpu().numpy()
+
+
+def _any(sc, rays):
+    import torch
+    r = torch.from_numpy(np.ascontiguousarray(rays, dtype=np.float32)).cuda()
+    o = torch.empty(len(rays), dtype=torch.uint8, device="cuda")
+    sc.trace_any(r, o)
+    torch.cuda.synchronize()
+    return o.cpu().numpy()
+
+
+def _assert_hits_equal(h, ref, label):
+    t, u, v, i = ref
+    gid = h[:, 3].view(np.int32)
+    bad = np.nonzero(gid != i)[0]
+    assert bad.size == 0, f"{label}: {bad.size} triangle id mismatches, first {bad[:5]}"
+    hit = i >= 0
+    for name, g, c in (("t", h[:, 0], t), ("u", h[:, 1], u), ("v", h[:, 2], v)):
+        gb, cb = g[hit].view(np.uint32), c[hit].view(np.uint32)
+        nb = np.count_nonzero(gb != cb)
+        assert nb == 0, f"{label}: {nb} {name} values differ in bits"
+
+
+def test_scene_bvh_equals_oracle_bvh(cfg0, oracle):
+    sd, sc, _ = cfg0
+    n_o, o_o = oracle.bvh_build(sd.positions)
+    nodes, ordered = sc.bvh()
+    assert nodes.tobytes() == n_o.tobytes()
+    assert np.array_equal(ordered, o_o)
+
+
+def test_camera_rays_bit_exact(cfg0, gpu, oracle):
+    import torch
+    sd, _, ps = cfg0
+    W, H = sd.width, sd.height
+    rays = torch.empty((W * H, 8), dtype=torch.float32, device="cuda")
+    cam = abi.make_camera(sd.inv_vp)
+    for sample in (0, 5):
+        abi.generate_rays(gpu, cam, W, H, sd.seed, sample, rays)
+        torch.cuda.synchronize()
+        g = rays.cpu().numpy()
+        rng = np.random.default_rng(sample)
+        for pix in np.concatenate([[0, W - 1, W * H - 1], rng.integers(0, W * H, 400)]):
+            ref = ps.gen_ray(int(pix % W), int(pix // W), sample)
+            got = np.concatenate([g[pix, 0:3], g[pix, 4:7]])
+            assert got.view(np.uint32).tolist() == ref.view(np.uint32).tolist(), f"pixel {pix}"
+
+
+def test_primary_hits_bit_exact(cfg0, gpu, oracle):
+    import torch
+    sd, sc, ps = cfg0
+    W, H = sd.width, sd.height
+    rays = torch.empty((W * H, 8), dtype=torch.float32, device="cuda")
+    abi.generate_rays(gpu, abi.make_camera(sd.inv_vp), W, H, sd.seed, 0, rays)
+    torch.cuda.synchronize()
+    r = rays.cpu().numpy()
+    h = _closest(sc, r)
+    ref = oracle.trace_closest(ps.nodes, ps.tris_ordered(), r)
+    assert np.count_nonzero(ref[3] >= 0) > W * H // 4
+    _assert_hits_equal(h, ref, "primary")
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_incoherent_hits_bit_exact(cfg0, oracle, seed):
+    sd, sc, ps = cfg0
+    r = scenes.random_rays(65536, seed)
+    h = _closest(sc, r)
+    _assert_hits_equal(h, oracle.trace_closest(ps.nodes, ps.tris_ordered(), r), f"incoherent{seed}")
+
+
+def test_two_sided_hits_bit_exact(cfg0, oracle):
+    sd, sc, ps = cfg0
+    r = scenes.random_rays(32768, 3)
+    h = _closest(sc, r, two_sided=True)
+    _assert_hits_equal(h, oracle.trace_closest(ps.nodes, ps.tris_ordered(), r, two_sided=True), "two-sided")
+
+
+def test_any_hit_bit_exact(cfg0, oracle):
+    sd, sc, ps = cfg0
+    r = scenes.random_rays(65536, 4)
+    o = _any(sc, r)
+    ref = oracle.trace_any(ps.nodes, ps.tris_ordered(), r)
+    assert 0 < ref.sum() < len(r)
+    assert np.array_equal(o, ref)
+
+
+def test_rays_from_surfaces_bit_exact(cfg0, oracle):
+    """Rays starting on surfaces (extension-ray origins) exercise the
+    behind-origin pruning next to the origin triangle."""
+    sd, sc, ps = cfg0
+    r = scenes.random_rays(32768, 5)
+    h0 = _closest(sc, r)
+    hit = h0[:, 3].view(np.int32) >= 0
+    o = r[hit, 0:3] + r[hit, 4:7] * h0[hit, 0:1]
+    rng = np.random.default_rng(6)
+    d = rng.normal(size=o.shape)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r2 = np.zeros((o.shape[0], 8), np.float32)
+    r2[:, 0:3] = o
+    r2[:, 4:7] = d
+    _assert_hits_equal(_closest(sc, r2), oracle.trace_closest(ps.nodes, ps.tris_ordered(), r2), "surface")
+    assert np.array_equal(_any(sc, r2), oracle.trace_any(ps.nodes, ps.tris_ordered(), r2))
+
+
+def test_empty_and_single_ray(cfg0, oracle):
+    sd, sc, ps = cfg0
+    sc.trace_closest(np.zeros((0, 8), np.float32), np.zeros((0, 4), np.float32))  # n = 0 is a no-op
+    r = scenes.random_rays(1, 9)
+    _assert_hits_equal(_closest(sc, r), oracle.trace_closest(ps.nodes, ps.tris_ordered(), r), "single")
