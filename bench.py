@@ -1,0 +1,213 @@
+#!/usr/bin/env python3
+"""Benchmark of the gfx950 wavefront path tracer on one MI355X.
+
+Workload (the single-large-frame configuration): 1920x1080, 64 spp, max
+depth 8, displaced icosphere at subdivision 8 (1,310,720 tris) + ground quad,
+three material queues (diffuse / GGX conductor / dielectric), 2048x1024 HDR
+environment with a sigma = 0.5 deg sun lobe of peak 1e4.  Synthetic,
+seeded inputs.  A step is one full frame (all 64 samples of every pixel).
+
+Prints ONE JSON line: value = incoherent Mrays/s (bounce >= 1 extension rays
+plus all any-hit rays, over the whole-frame wall time), with spp/s at 1080p,
+the extend-kernel roofline and the CPU oracle baseline beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+METRIC = "Mrays/s incoherent + spp/s at 1080p; adjoint/fwd time at grad rel-err≤1e-3"
+
+# algorithmic bytes per unit of work: extend ray = 24 (ray) + 16 (hit) +
+# 32 per box tested + 36 per triangle tested; shadow ray = 24 + 4 + same.
+EXT_RAY_BYTES, SHD_RAY_BYTES, NODE_BYTES, TRI_BYTES = 40, 28, 32, 36
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def load_traffic(kernel: str, workload: str):
+    """HBM bytes per launch from a committed rocprof PMC summary, if present."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("traffic_bytes_per_launch")
+
+
+def cpu_baseline(sd, nodes, ordered, budget_s: float, spp: int):
+    """Oracle (CPU restatement) on a bounded sample of the same frame: the
+    full frame at s spp, s chosen from a 1-spp calibration run so the sample
+    costs about budget_s; all host threads.  Scaled to rays/s and spp/s."""
+    from oracle import oracle as O
+    ps = O.PreparedScene(sd, nodes, ordered)
+    threads = len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    ps.render(spp=1, n_threads=threads)
+    t1 = time.perf_counter() - t0
+    s = int(max(1, min(spp, round(budget_s / max(t1, 1e-3)))))
+    t0 = time.perf_counter()
+    film, st = ps.render(spp=s, n_threads=threads)
+    t = time.perf_counter() - t0
+    inc = st.rays_extension + st.rays_shadow
+    tot = inc + st.rays_primary
+    return {
+        "value": round(inc / t / 1e6, 4),
+        "unit": "Mrays/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"full {sd.width}x{sd.height} frame at {s} spp, depth {sd.max_depth} "
+                  f"({int(st.paths)} paths, {t:.1f} s)",
+        "total_mrays_s": round(tot / t / 1e6, 4),
+        "spp_per_s_1080p": round(st.paths / t / (1920 * 1080), 5),
+        "frame_s_extrapolated": round(t / s * spp, 1),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--depth", type=int, default=8)
+    ap.add_argument("--subdiv", type=int, default=8)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--spp-batch", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="run a plain device copy of this many MiB first (PMC calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build runs on a single MI355X); "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        return 2
+
+    import torch
+    from firefly_amd import abi, scenes
+
+    t0 = time.perf_counter()
+    sd = scenes.icosphere_scene(args.width, args.height, args.spp, args.depth, seed=0, subdiv=args.subdiv)
+    log(f"scene: {sd.n_tris} tris, env {sd.env.shape[1]}x{sd.env.shape[0]} ({time.perf_counter() - t0:.1f} s)")
+    ctx = abi.Context(0)
+    t0 = time.perf_counter()
+    scene = abi.Scene(ctx, sd.positions, sd.normals, sd.uvs, sd.material_ids, sd.materials, sd.textures)
+    env = abi.EnvMap(ctx, sd.env)
+    cam = abi.make_camera(sd.inv_vp, sd.lens_radius, sd.focal, sd.exposure)
+    log(f"upload + host BVH ({scene.n_nodes} nodes, depth {scene.max_depth}) + env CDF: "
+        f"{time.perf_counter() - t0:.1f} s")
+    film = torch.zeros((sd.height, sd.width, 4), dtype=torch.float32, device="cuda")
+    if args.copy_mb:
+        src = torch.ones(args.copy_mb * 2 ** 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {args.copy_mb} MiB read + {args.copy_mb} MiB written")
+        del src, dst
+
+    def frame(flags=0):
+        film.zero_()
+        abi.render(ctx, scene, env, cam, sd.width, sd.height, args.spp, args.depth, sd.seed, film,
+                   spp_batch=args.spp_batch, flags=flags)
+
+    for i in range(args.warmup):
+        frame()
+        torch.cuda.synchronize()
+        log(f"warmup frame {i + 1}/{args.warmup} done")
+
+    keys = ("rays_primary", "rays_extension", "rays_shadow", "paths", "ext_nodes", "ext_tris",
+            "shadow_nodes", "shadow_tris", "ms_extend", "ms_shadow", "ms_logic", "ms_shade", "ms_other",
+            "launches_extend", "launches_shadow")
+    tot = dict.fromkeys(keys, 0.0)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        frame(abi.FF_RENDER_TIMING)
+        st = abi.last_render_stats(ctx)  # synchronises the frame
+        for k in keys:
+            tot[k] += getattr(st, k)
+        log(f"timed frame {i + 1}/{args.steps}: {time.perf_counter() - t_start:.2f} s elapsed")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    ms_step = elapsed / args.steps * 1e3
+
+    incoherent = tot["rays_extension"] + tot["rays_shadow"]
+    total_rays = incoherent + tot["rays_primary"]
+    spp_s = args.steps * args.spp / elapsed * (sd.width * sd.height) / (1920 * 1080)
+    ext_bytes = (EXT_RAY_BYTES * (tot["rays_primary"] + tot["rays_extension"]) +
+                 NODE_BYTES * tot["ext_nodes"] + TRI_BYTES * tot["ext_tris"])
+    shd_bytes = SHD_RAY_BYTES * tot["rays_shadow"] + NODE_BYTES * tot["shadow_nodes"] + TRI_BYTES * tot["shadow_tris"]
+    kern = {
+        "k_extend": (ext_bytes, tot["ms_extend"], tot["launches_extend"]),
+        "k_shadow": (shd_bytes, tot["ms_shadow"], tot["launches_shadow"]),
+    }
+    dom = max(kern, key=lambda k: kern[k][1])
+    b, ms, nl = kern[dom]
+    achieved = b / (ms / 1e3) / 1e9
+    workload = f"icosphere{args.subdiv}_{sd.width}x{sd.height}_{args.spp}spp_d{args.depth}"
+    traffic = load_traffic(dom, workload)
+    roofline = {
+        "bound": "hbm",
+        "kernel": dom,
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 5),
+        "traffic": traffic,
+        "algorithmic_bytes_per_launch": round(b / nl),
+        "avg_launch_ms": round(ms / nl, 4),
+        "launches": int(nl),
+    }
+    stages = {k: round(tot[k] / args.steps, 2) for k in ("ms_extend", "ms_shadow", "ms_logic", "ms_shade", "ms_other")}
+    per_ray = {
+        "ext_nodes_per_ray": round(tot["ext_nodes"] / max(1.0, tot["rays_primary"] + tot["rays_extension"]), 3),
+        "ext_tris_per_ray": round(tot["ext_tris"] / max(1.0, tot["rays_primary"] + tot["rays_extension"]), 3),
+        "shadow_nodes_per_ray": round(tot["shadow_nodes"] / max(1.0, tot["rays_shadow"]), 3),
+        "shadow_tris_per_ray": round(tot["shadow_tris"] / max(1.0, tot["rays_shadow"]), 3),
+    }
+    cpu = None
+    if not args.no_cpu:
+        nodes, ordered = scene.bvh()
+        log("cpu baseline (oracle) ...")
+        cpu = cpu_baseline(sd, nodes, ordered, args.cpu_seconds, args.spp)
+        log(f"cpu baseline: {cpu}")
+    out = {
+        "metric": METRIC,
+        "value": round(incoherent / elapsed / 1e6, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": workload, "width": sd.width, "height": sd.height, "spp": args.spp,
+                   "max_depth": args.depth, "tris": sd.n_tris, "env": f"{sd.env.shape[1]}x{sd.env.shape[0]}",
+                   "spp_batch": args.spp_batch},
+        "spp_per_s_1080p": round(spp_s, 3),
+        "total_mrays_s": round(total_rays / elapsed / 1e6, 3),
+        "rays_per_step": {k: int(tot[k] / args.steps) for k in ("rays_primary", "rays_extension", "rays_shadow")},
+        "stage_ms_per_step": stages,
+        "traversal": per_ray,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

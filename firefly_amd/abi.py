@@ -14,6 +14,7 @@ FF_OK = 0
 FF_MAT_FLOATS = 16
 FF_MAT_MIX, FF_MAT_DIFFUSE, FF_MAT_CONDUCTOR, FF_MAT_DIELECTRIC = 0, 1, 2, 3
 FF_RENDER_USE_GRAPH = 1
+FF_RENDER_TIMING = 2
 
 NODE_DTYPE = np.dtype([("bmin", "<f4", 3), ("bmax", "<f4", 3), ("offset", "<i4"),
                        ("n_prims", "<u2"), ("axis", "u1"), ("pad", "u1")])
@@ -46,9 +47,13 @@ class RenderReq(C.Structure):
 
 
 class RayStats(C.Structure):
-    _fields_ = [("rays_primary", C.c_double), ("rays_extension", C.c_double),
-                ("rays_shadow", C.c_double), ("paths", C.c_double),
-                ("nodes_tested", C.c_double), ("tris_tested", C.c_double)]
+    _fields_ = [(n, C.c_double) for n in (
+        "rays_primary", "rays_extension", "rays_shadow", "paths", "ext_nodes", "ext_tris",
+        "shadow_nodes", "shadow_tris", "ms_extend", "ms_shadow", "ms_logic", "ms_shade", "ms_other",
+        "launches_extend", "launches_shadow")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 _P = C.c_void_p
@@ -67,8 +72,8 @@ _PROTOS = {
     "ff_envmap_create": (C.c_int, [_P, _P, C.c_int32, C.c_int32, C.POINTER(_P)]),
     "ff_envmap_read": (C.c_int, [_P, _P, _P, _P, _P, _P]),
     "ff_envmap_destroy": (C.c_int, [_P]),
-    "ff_trace_closest": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_int32, _P]),
-    "ff_trace_any": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
+    "ff_trace_closest": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_int32, _P, _P]),
+    "ff_trace_any": (C.c_int, [_P, _P, C.c_uint32, _P, _P, _P]),
     "ff_generate_rays": (C.c_int, [_P, C.POINTER(Camera), C.c_int32, C.c_int32, C.c_uint32,
                                    C.c_int32, _P, _P]),
     "ff_render": (C.c_int, [_P, _P, _P, C.POINTER(Camera), C.POINTER(RenderReq), _P, _P]),
@@ -198,13 +203,14 @@ class Scene:
         check(_lib().ff_scene_get_bvh(self.h, nodes.ctypes.data, ordered.ctypes.data))
         return nodes, ordered
 
-    def trace_closest(self, rays_dev, hits_dev, two_sided=False, stream=None):
+    def trace_closest(self, rays_dev, hits_dev, two_sided=False, counts_dev=None, stream=None):
         n = rays_dev.shape[0]
-        check(_lib().ff_trace_closest(self.h, _ptr(rays_dev), n, _ptr(hits_dev), int(two_sided), stream))
+        check(_lib().ff_trace_closest(self.h, _ptr(rays_dev), n, _ptr(hits_dev), int(two_sided),
+                                      _ptr(counts_dev), stream))
 
-    def trace_any(self, rays_dev, occ_dev, stream=None):
+    def trace_any(self, rays_dev, occ_dev, counts_dev=None, stream=None):
         n = rays_dev.shape[0]
-        check(_lib().ff_trace_any(self.h, _ptr(rays_dev), n, _ptr(occ_dev), stream))
+        check(_lib().ff_trace_any(self.h, _ptr(rays_dev), n, _ptr(occ_dev), _ptr(counts_dev), stream))
 
     def close(self):
         if self.h:

@@ -66,8 +66,16 @@ struct PathStore {
 
 }  // namespace
 
+struct StageRecord {
+    int stage;  // 0 extend, 1 shadow, 2 logic, 3 shade, 4 other
+    hipEvent_t begin, end;
+};
+
 struct ff_context {
     int device = 0;
+    std::vector<hipEvent_t> event_pool;
+    size_t events_used = 0;
+    std::vector<StageRecord> records;
     PathStore paths;
     uint32_t* counters = nullptr;
     unsigned long long* stats = nullptr;
@@ -193,6 +201,7 @@ ff_status ff_context_destroy(ff_context* c) {
     (void)hipSetDevice(c->device);
     (void)hipDeviceSynchronize();
     c->paths.release();
+    for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
     dfree(c->counters);
     dfree(c->stats);
     dfree(c->ovf);
@@ -384,20 +393,22 @@ ff_status ff_envmap_destroy(ff_envmap* e) {
 }
 
 ff_status ff_trace_closest(ff_scene* s, const float* rays, uint32_t n, float* hits, int32_t two_sided,
-                           void* stream) {
+                           uint64_t* counts, void* stream) {
     if (!s || (!rays && n) || (!hits && n)) return fail(FF_ERR_INVALID, "ff_trace_closest: NULL argument");
     if (n == 0) return FF_OK;
     HIP_TRY(hipSetDevice(s->ctx->device));
     HIP_TRY(ffk::launch_trace_closest(s->view(), (const float4*)rays, n, (float4*)hits, two_sided, s->ctx->ovf,
-                                      (hipStream_t)stream));
+                                      (unsigned long long*)counts, (hipStream_t)stream));
     return FF_OK;
 }
 
-ff_status ff_trace_any(ff_scene* s, const float* rays, uint32_t n, uint8_t* occluded, void* stream) {
+ff_status ff_trace_any(ff_scene* s, const float* rays, uint32_t n, uint8_t* occluded, uint64_t* counts,
+                       void* stream) {
     if (!s || (!rays && n) || (!occluded && n)) return fail(FF_ERR_INVALID, "ff_trace_any: NULL argument");
     if (n == 0) return FF_OK;
     HIP_TRY(hipSetDevice(s->ctx->device));
-    HIP_TRY(ffk::launch_trace_any(s->view(), (const float4*)rays, n, occluded, s->ctx->ovf, (hipStream_t)stream));
+    HIP_TRY(ffk::launch_trace_any(s->view(), (const float4*)rays, n, occluded, s->ctx->ovf,
+                                  (unsigned long long*)counts, (hipStream_t)stream));
     return FF_OK;
 }
 
@@ -436,6 +447,26 @@ ff_status ff_render(ff_context* ctx, ff_scene* s, ff_envmap* env, const ff_camer
     ff_dev_scene S = s->view();
     ff_dev_env E = env->view();
     ff_dev_camera C = dev_camera(cam);
+    const bool timing = (r->flags & FF_RENDER_TIMING) != 0;
+    ctx->records.clear();
+    ctx->events_used = 0;
+    // Records a begin/end event pair around one stage launch when timing.
+    auto timed = [&](int stage, auto&& launch) -> hipError_t {
+        if (!timing) return launch();
+        while (ctx->event_pool.size() < ctx->events_used + 2) {
+            hipEvent_t e;
+            hipError_t ce = hipEventCreate(&e);
+            if (ce != hipSuccess) return ce;
+            ctx->event_pool.push_back(e);
+        }
+        StageRecord rec{stage, ctx->event_pool[ctx->events_used], ctx->event_pool[ctx->events_used + 1]};
+        ctx->events_used += 2;
+        hipError_t e = hipEventRecord(rec.begin, hs);
+        if (e == hipSuccess) e = launch();
+        if (e == hipSuccess) e = hipEventRecord(rec.end, hs);
+        ctx->records.push_back(rec);
+        return e;
+    };
     HIP_TRY(ffk::launch_fill4(p.layers, (uint32_t)P, hs));
     HIP_TRY(ffk::launch_fill_u64(p.stats, S_COUNT, hs));
     const uint32_t all = (1u << C_COUNT) - 1u;
@@ -443,23 +474,28 @@ ff_status ff_render(ff_context* ctx, ff_scene* s, ff_envmap* env, const ff_camer
         int32_t k_act = std::min<int32_t>(K, r->spp - b * K);
         uint32_t n_act = (uint32_t)(WH * (uint64_t)k_act);
         HIP_TRY(ffk::launch_reset(p.counters, all, hs));
-        HIP_TRY(ffk::launch_generate(C, r->width, r->height, n_act, (uint32_t)(r->sample_offset + b * K), r->seed,
-                                     p, 0, hs));
+        HIP_TRY(timed(4, [&] {
+            return ffk::launch_generate(C, r->width, r->height, n_act, (uint32_t)(r->sample_offset + b * K),
+                                        r->seed, p, 0, hs);
+        }));
         int cur = 0;
         for (int32_t it = 1; it <= r->max_depth + 1; it++) {
-            HIP_TRY(ffk::launch_extend(S, p, cur, hs));
-            HIP_TRY(ffk::launch_shadow(S, p, hs));
+            HIP_TRY(timed(0, [&] { return ffk::launch_extend(S, p, cur, hs); }));
+            HIP_TRY(timed(1, [&] { return ffk::launch_shadow(S, p, hs); }));
             uint32_t mask = (1u << (cur == 0 ? C_EXT1 : C_EXT0)) | (1u << C_SHADOW) | (0xfu << C_MAT0);
             HIP_TRY(ffk::launch_reset(p.counters, mask, hs));
-            HIP_TRY(ffk::launch_logic(S, E, p, cur, (uint32_t)r->max_depth, n_act, hs));
+            HIP_TRY(timed(2, [&] { return ffk::launch_logic(S, E, p, cur, (uint32_t)r->max_depth, n_act, hs); }));
             if (it <= r->max_depth) {
                 for (int m = 0; m < 4; m++)
-                    if (s->has_type[m]) HIP_TRY(ffk::launch_shade(m, S, E, p, 1 - cur, n_act, hs));
+                    if (s->has_type[m])
+                        HIP_TRY(timed(3, [&] { return ffk::launch_shade(m, S, E, p, 1 - cur, n_act, hs); }));
             }
             cur = 1 - cur;
         }
     }
-    HIP_TRY(ffk::launch_resolve_layers(p.layers, (uint32_t)K, (uint32_t)WH, (float4*)film, (float)r->spp, hs));
+    HIP_TRY(timed(4, [&] {
+        return ffk::launch_resolve_layers(p.layers, (uint32_t)K, (uint32_t)WH, (float4*)film, (float)r->spp, hs);
+    }));
     ctx->last_stream = hs;
     ctx->have_stats = true;
     return FF_OK;
@@ -477,6 +513,18 @@ ff_status ff_last_render_stats(ff_context* ctx, ff_ray_stats* out) {
     out->rays_extension = (double)h[S_EXTENSION];
     out->rays_shadow = (double)h[S_SHADOW];
     out->paths = (double)h[S_PATHS];
+    out->ext_nodes = (double)h[S_EXT_NODES];
+    out->ext_tris = (double)h[S_EXT_TRIS];
+    out->shadow_nodes = (double)h[S_SHD_NODES];
+    out->shadow_tris = (double)h[S_SHD_TRIS];
+    double* ms[5] = {&out->ms_extend, &out->ms_shadow, &out->ms_logic, &out->ms_shade, &out->ms_other};
+    for (const StageRecord& rec : ctx->records) {
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, rec.begin, rec.end));
+        *ms[rec.stage] += (double)t;
+        if (rec.stage == 0) out->launches_extend += 1;
+        if (rec.stage == 1) out->launches_shadow += 1;
+    }
     return FF_OK;
 }
 

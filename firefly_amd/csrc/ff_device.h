@@ -411,7 +411,8 @@ struct Hit { float t, u, v; int id; };
 // dirIsNeg[axis]; a candidate replaces the record when 0 <= t < tmin.
 __device__ __forceinline__ Hit closest_hit(const float4* __restrict__ nodes,
                                            const float* __restrict__ tris, f3 o, f3 d,
-                                           bool two_sided, Stack& st) {
+                                           bool two_sided, Stack& st, uint32_t& cnt_nodes,
+                                           uint32_t& cnt_tris) {
     Hit h;
     h.t = K_HUGE; h.u = 0.f; h.v = 0.f; h.id = -1;
     f3 inv = V3(1.f / d.x, 1.f / d.y, 1.f / d.z);
@@ -420,6 +421,7 @@ __device__ __forceinline__ Hit closest_hit(const float4* __restrict__ nodes,
     int cur = 0;
     for (;;) {
         float4 a = nodes[2 * cur], b = nodes[2 * cur + 1];
+        cnt_nodes++;
         float tn, tf;
         bool hit = slab(a, b, o, inv, nx, ny, nz, tn, tf);
         if (hit) {
@@ -433,6 +435,7 @@ __device__ __forceinline__ Hit closest_hit(const float4* __restrict__ nodes,
             if (np > 0) {
                 for (int i = 0; i < np; i++) {
                     Tri T = load_tri(tris, off + i);
+                    cnt_tris++;
                     float t = 0.f, u = 0.f, v = 0.f;
                     bool ok = two_sided ? mt_two(T, o, d, t, u, v) : mt_cull(T, o, d, t, u, v);
                     if (ok && !(t < 0.f) && t < h.t) { h.t = t; h.u = u; h.v = v; h.id = off + i; }
@@ -456,13 +459,15 @@ __device__ __forceinline__ Hit closest_hit(const float4* __restrict__ nodes,
 // intersect_shadows with tmin = K_HUGE (CRT/dTriangle.cu:215-259,
 // CRT/dEnvironmentLight.cu:60-64): true when any triangle has 0 <= t < 1e32.
 __device__ __forceinline__ bool any_hit(const float4* __restrict__ nodes,
-                                        const float* __restrict__ tris, f3 o, f3 d, Stack& st) {
+                                        const float* __restrict__ tris, f3 o, f3 d, Stack& st,
+                                        uint32_t& cnt_nodes, uint32_t& cnt_tris) {
     f3 inv = V3(1 / d.x, 1 / d.y, 1 / d.z);
     bool nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
     st.sp = 0;
     int cur = 0;
     for (;;) {
         float4 a = nodes[2 * cur], b = nodes[2 * cur + 1];
+        cnt_nodes++;
         float tn, tf;
         bool hit = slab(a, b, o, inv, nx, ny, nz, tn, tf);
         if (hit && tf < -PRUNE_BEHIND) hit = false;
@@ -473,6 +478,7 @@ __device__ __forceinline__ bool any_hit(const float4* __restrict__ nodes,
             if (np > 0) {
                 for (int i = 0; i < np; i++) {
                     Tri T = load_tri(tris, off + i);
+                    cnt_tris++;
                     float t = 0.f;
                     if (mt_shadow(T, o, d, t) && t < K_HUGE) return true;
                 }
@@ -516,6 +522,15 @@ __device__ __forceinline__ void wave_count(unsigned long long* counter, bool pre
         int leader = __ffsll((unsigned long long)all) - 1;
         if ((int)__lane_id() == leader) atomicAdd(counter, (unsigned long long)__popcll(mask));
     }
+}
+
+// Sum of a per-lane value over a full wave, added once per wave.  Must be
+// called with every lane of the wave active (after the grid-stride loop).
+__device__ __forceinline__ void wave_sum_add(unsigned long long* counter, uint32_t v) {
+    unsigned long long s = v;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (__lane_id() == 0) atomicAdd(counter, s);
 }
 
 }  // namespace ffd

@@ -25,9 +25,10 @@ hipError_t launch_fill_u64(unsigned long long* p, uint32_t n, hipStream_t s);
 hipError_t launch_resolve_layers(const float4* layers, uint32_t K, uint32_t WH, float4* film,
                                  float spp, hipStream_t s);
 hipError_t launch_trace_closest(const ff_dev_scene& S, const float4* rays, uint32_t n,
-                                float4* hits, int two_sided, int32_t* ovf, hipStream_t s);
+                                float4* hits, int two_sided, int32_t* ovf,
+                                unsigned long long* counts, hipStream_t s);
 hipError_t launch_trace_any(const ff_dev_scene& S, const float4* rays, uint32_t n, uint8_t* occ,
-                            int32_t* ovf, hipStream_t s);
+                            int32_t* ovf, unsigned long long* counts, hipStream_t s);
 hipError_t launch_gen_rays(const ff_dev_camera& cam, int W, int H, uint32_t seed,
                            uint32_t sample, float4* rays, hipStream_t s);
 hipError_t launch_env_build(const float* rgb, int w, int h, float* lum, double* row_sum,

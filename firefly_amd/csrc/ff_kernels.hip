@@ -54,13 +54,16 @@ __global__ __launch_bounds__(FF_BLOCK) void k_extend(ff_dev_scene S, ff_paths P,
     st.ovf_stride = FF_TRAV_GRID * FF_BLOCK;
     const uint32_t n = P.counters[q == 0 ? C_EXT0 : C_EXT1];
     const uint32_t* queue = P.q_ext[q];
+    uint32_t cn = 0, ct = 0;
     for (uint32_t i = gtid(); i < n; i += gstride()) {
         uint32_t id = queue[i];
         float4 r0 = P.ray[2 * id], r1 = P.ray[2 * id + 1];
         uint32_t stw = __float_as_uint(P.beta[id].w);
-        Hit h = closest_hit(S.nodes, S.tris, xyz(r0), xyz(r1), (stw & ST_INSIDE) != 0, st);
+        Hit h = closest_hit(S.nodes, S.tris, xyz(r0), xyz(r1), (stw & ST_INSIDE) != 0, st, cn, ct);
         P.hit[id] = make_float4(h.t, h.u, h.v, __int_as_float(h.id));
     }
+    wave_sum_add(&P.stats[S_EXT_NODES], cn);
+    wave_sum_add(&P.stats[S_EXT_TRIS], ct);
 }
 
 // ================================================================== shadow
@@ -73,12 +76,15 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shadow(ff_dev_scene S, ff_paths P)
     st.ovf = P.ovf + gtid();
     st.ovf_stride = FF_TRAV_GRID * FF_BLOCK;
     const uint32_t n = P.counters[C_SHADOW];
+    uint32_t cn = 0, ct = 0;
     for (uint32_t i = gtid(); i < n; i += gstride()) {
         uint32_t s = P.q_shadow[i];
         float4 r0 = P.sray[2 * s], r1 = P.sray[2 * s + 1];
-        bool occ = any_hit(S.nodes, S.tris, xyz(r0), xyz(r1), st);
+        bool occ = any_hit(S.nodes, S.tris, xyz(r0), xyz(r1), st, cn, ct);
         P.vis[s] = occ ? 0 : 1;
     }
+    wave_sum_add(&P.stats[S_SHD_NODES], cn);
+    wave_sum_add(&P.stats[S_SHD_TRIS], ct);
 }
 
 // counters consumed by the previous stages are cleared by a kernel (graph-safe)
@@ -390,30 +396,41 @@ __global__ void k_resolve_layers(const float4* layers, uint32_t K, uint32_t WH, 
 // ======================================================== standalone stages
 __global__ __launch_bounds__(FF_BLOCK) void k_trace_closest(ff_dev_scene S, const float4* rays,
                                                             uint32_t n, float4* hits, int two_sided,
-                                                            int32_t* ovf) {
+                                                            int32_t* ovf, unsigned long long* counts) {
     __shared__ int smem[FF_LDS_STACK * FF_BLOCK];
     Stack st;
     st.lds = &smem[threadIdx.x];
     st.lds_stride = FF_BLOCK;
     st.ovf = ovf + gtid();
     st.ovf_stride = FF_TRAV_GRID * FF_BLOCK;
+    uint32_t cn = 0, ct = 0;
     for (uint32_t i = gtid(); i < n; i += gstride()) {
         float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
-        Hit h = closest_hit(S.nodes, S.tris, xyz(r0), xyz(r1), two_sided != 0, st);
+        Hit h = closest_hit(S.nodes, S.tris, xyz(r0), xyz(r1), two_sided != 0, st, cn, ct);
         hits[i] = make_float4(h.t, h.u, h.v, __int_as_float(h.id));
+    }
+    if (counts) {
+        wave_sum_add(&counts[0], cn);
+        wave_sum_add(&counts[1], ct);
     }
 }
 __global__ __launch_bounds__(FF_BLOCK) void k_trace_any(ff_dev_scene S, const float4* rays, uint32_t n,
-                                                        uint8_t* occ, int32_t* ovf) {
+                                                        uint8_t* occ, int32_t* ovf,
+                                                        unsigned long long* counts) {
     __shared__ int smem[FF_LDS_STACK * FF_BLOCK];
     Stack st;
     st.lds = &smem[threadIdx.x];
     st.lds_stride = FF_BLOCK;
     st.ovf = ovf + gtid();
     st.ovf_stride = FF_TRAV_GRID * FF_BLOCK;
+    uint32_t cn = 0, ct = 0;
     for (uint32_t i = gtid(); i < n; i += gstride()) {
         float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
-        occ[i] = any_hit(S.nodes, S.tris, xyz(r0), xyz(r1), st) ? 1 : 0;
+        occ[i] = any_hit(S.nodes, S.tris, xyz(r0), xyz(r1), st, cn, ct) ? 1 : 0;
+    }
+    if (counts) {
+        wave_sum_add(&counts[0], cn);
+        wave_sum_add(&counts[1], ct);
     }
 }
 __global__ void k_gen_rays(ff_dev_camera cam, int32_t W, int32_t H, uint32_t seed, uint32_t sample,
@@ -619,14 +636,14 @@ hipError_t launch_resolve_layers(const float4* layers, uint32_t K, uint32_t WH, 
     return hipGetLastError();
 }
 hipError_t launch_trace_closest(const ff_dev_scene& S, const float4* rays, uint32_t n, float4* hits,
-                                int two_sided, int32_t* ovf, hipStream_t s) {
+                                int two_sided, int32_t* ovf, unsigned long long* counts, hipStream_t s) {
     hipLaunchKernelGGL(k_trace_closest, dim3(FF_TRAV_GRID), dim3(FF_BLOCK), 0, s, S, rays, n, hits,
-                       two_sided, ovf);
+                       two_sided, ovf, counts);
     return hipGetLastError();
 }
 hipError_t launch_trace_any(const ff_dev_scene& S, const float4* rays, uint32_t n, uint8_t* occ,
-                            int32_t* ovf, hipStream_t s) {
-    hipLaunchKernelGGL(k_trace_any, dim3(FF_TRAV_GRID), dim3(FF_BLOCK), 0, s, S, rays, n, occ, ovf);
+                            int32_t* ovf, unsigned long long* counts, hipStream_t s) {
+    hipLaunchKernelGGL(k_trace_any, dim3(FF_TRAV_GRID), dim3(FF_BLOCK), 0, s, S, rays, n, occ, ovf, counts);
     return hipGetLastError();
 }
 hipError_t launch_gen_rays(const ff_dev_camera& cam, int W, int H, uint32_t seed, uint32_t sample,

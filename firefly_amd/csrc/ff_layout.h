@@ -37,7 +37,11 @@
 #define S_EXTENSION 1
 #define S_SHADOW 2
 #define S_PATHS 3
-#define S_COUNT 4
+#define S_EXT_NODES 4
+#define S_EXT_TRIS 5
+#define S_SHD_NODES 6
+#define S_SHD_TRIS 7
+#define S_COUNT 8
 
 #define FF_MAT_FLOATS_DEV 16
 

@@ -93,15 +93,19 @@ typedef struct {
     int32_t flags;        /* FF_RENDER_* */
 } ff_render_req;
 
-#define FF_RENDER_USE_GRAPH 1 /* capture the bounce loop in a hipGraph */
+#define FF_RENDER_USE_GRAPH 1 /* capture the bounce loop in a hipGraph              */
+#define FF_RENDER_TIMING 2    /* time every stage launch with hipEvents (stats)     */
 
 typedef struct {
-    double rays_primary;   /* camera rays (closest hit)          */
-    double rays_extension; /* bounce >= 1 extension rays          */
-    double rays_shadow;    /* any-hit rays (light + BRDF samples) */
-    double paths;          /* completed samples                   */
-    double nodes_tested;   /* extension + shadow box tests (when counted) */
-    double tris_tested;
+    double rays_primary;   /* camera rays (closest hit)                     */
+    double rays_extension; /* bounce >= 1 extension rays                     */
+    double rays_shadow;    /* any-hit rays (light + BRDF-sampled visibility) */
+    double paths;          /* completed samples                              */
+    double ext_nodes, ext_tris;       /* boxes / triangles tested by extend  */
+    double shadow_nodes, shadow_tris; /* boxes / triangles tested by shadow  */
+    /* FF_RENDER_TIMING only: summed launch durations (ms) and launch counts */
+    double ms_extend, ms_shadow, ms_logic, ms_shade, ms_other;
+    double launches_extend, launches_shadow;
 } ff_ray_stats;
 
 /* ---- errors / version ---- */
@@ -147,10 +151,11 @@ ff_status ff_envmap_destroy(ff_envmap *env);
  *   test of dTriangle::intersect (CRT/dTriangle.cu:63-80).
  * ff_trace_any replaces wf_shadow/intersect_shadows (CRT/kernel.cu:525-543,
  *   CRT/dTriangle.cu:215-259): occluded device [n] bytes. */
+/* counts (device, 2 x uint64, may be NULL) accumulates boxes and triangles tested. */
 ff_status ff_trace_closest(ff_scene *scene, const float *rays, uint32_t n, float *hits,
-                           int32_t two_sided, void *stream);
+                           int32_t two_sided, uint64_t *counts, void *stream);
 ff_status ff_trace_any(ff_scene *scene, const float *rays, uint32_t n, uint8_t *occluded,
-                       void *stream);
+                       uint64_t *counts, void *stream);
 /* camera-ray generation only (wf_generate + dCamera::gen_ray,
  * CRT/kernel.cu:457-485, CRT/dCamera.cu:10-36): rays device [h*w][8] for
  * sample index `sample`. */

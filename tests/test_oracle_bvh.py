@@ -1,0 +1,92 @@
+"""Pins of the SAH BVH build: structural invariants, hand-checkable split
+decisions, and bit-identity of the product's host builder with the oracle."""
+import numpy as np
+import pytest
+
+from firefly_amd import abi, scenes
+
+
+def _check_invariants(nodes, ordered, pos, max_prims=8):
+    pos = np.asarray(pos, np.float32).reshape(-1, 3, 3)
+    n = len(pos)
+    assert sorted(ordered.tolist()) == list(range(n)), "every primitive exactly once"
+    seen = np.zeros(len(nodes), bool)
+    stack = [0]
+    while stack:
+        i = stack.pop()
+        seen[i] = True
+        nd = nodes[i]
+        if nd["n_prims"] > 0:
+            sl = ordered[nd["offset"]: nd["offset"] + nd["n_prims"]]
+            p = pos[sl].reshape(-1, 3)
+            assert np.all(p.min(0) >= nd["bmin"]) and np.all(p.max(0) <= nd["bmax"])
+            if nd["n_prims"] > max_prims:  # only for degenerate centroid extents
+                tb = pos[sl]
+                c = 0.5 * tb.min(1) + 0.5 * tb.max(1)
+                assert np.any(np.ptp(c, axis=0) == 0)
+        else:
+            a, b = i + 1, nd["offset"]
+            assert b > a, "pre-order layout: second child after the first subtree"
+            for ch in (a, b):
+                assert np.all(nodes[ch]["bmin"] >= nd["bmin"]) and np.all(nodes[ch]["bmax"] <= nd["bmax"])
+            lo = np.minimum(nodes[a]["bmin"], nodes[b]["bmin"])
+            hi = np.maximum(nodes[a]["bmax"], nodes[b]["bmax"])
+            assert np.array_equal(lo, nd["bmin"]) and np.array_equal(hi, nd["bmax"])
+            stack += [a, b]
+    assert seen.all(), "every node reachable"
+
+
+def _soup(n, seed, edge=0.05):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-1, 1, (n, 1, 3))
+    return (c + rng.normal(scale=edge, size=(n, 3, 3))).astype(np.float32)
+
+
+@pytest.mark.parametrize("seed,n", [(0, 1), (1, 2), (2, 3), (3, 17), (4, 1000), (5, 20000)])
+def test_bvh_invariants(oracle, seed, n):
+    pos = _soup(n, seed)
+    nodes, ordered = oracle.bvh_build(pos)
+    _check_invariants(nodes, ordered, pos)
+    assert len(nodes) <= 2 * n - 1
+
+
+def test_degenerate_centroids_make_one_leaf(oracle):
+    tri = np.array([[-1, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    pos = np.stack([tri * s for s in (1.0, 2.0, 3.0, 4.0, 5.0, 6.0, 7.0, 8.0, 9.0, 10.0)])
+    pos[:, :, 1] -= pos[:, :, 1].max(1, keepdims=True) / 2  # same centroid for all
+    nodes, ordered = oracle.bvh_build(pos)
+    assert len(nodes) == 1 and nodes[0]["n_prims"] == 10
+
+
+def test_two_primitives_sorted_by_centroid(oracle):
+    """nth_element on two elements orders them by centroid[dim]."""
+    a = np.array([[5, 0, 0], [6, 0, 0], [5, 1, 0]], np.float32)
+    b = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    nodes, ordered = oracle.bvh_build(np.stack([a, b]))
+    assert len(nodes) == 3 and nodes[0]["axis"] == 0
+    assert ordered.tolist() == [1, 0]
+
+
+def test_sah_splits_two_clusters_at_the_gap(oracle):
+    """Two clusters of 8 far apart: the root split separates them, and a
+    cluster of 8 small triangles is split further (leaf cost 8 > SAH cost)."""
+    rng = np.random.default_rng(0)
+    left = rng.uniform(0, 1, (8, 3, 3)) * [1, 1, 1]
+    right = left + [100, 0, 0]
+    pos = np.concatenate([left, right]).astype(np.float32)
+    nodes, ordered = oracle.bvh_build(pos)
+    first_child = nodes[1]
+    second = nodes[nodes[0]["offset"]]
+    assert nodes[0]["axis"] == 0
+    assert first_child["bmax"][0] < 50 and second["bmin"][0] > 50
+
+
+def test_product_bvh_bit_identical_to_oracle(oracle):
+    cases = [scenes.spheres_scene().positions, _soup(5000, 7), _soup(300, 8, edge=0.8),
+             scenes.displaced_icosphere(4, (0, 1, 0), 1.0, 0.05, 0)[0]]
+    for pos in cases:
+        n_o, o_o = oracle.bvh_build(pos)
+        n_p, o_p, depth = abi.bvh_build_host(pos)
+        assert n_p.tobytes() == n_o.tobytes()
+        assert np.array_equal(o_p, o_o)
+        assert depth == oracle.bvh_max_depth(n_o)
