@@ -15,6 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libff_hip.so")
+HOST_LIB = os.path.join(HERE, "libfirefly.so")
+DEMO = os.path.join(HERE, "render_demo")
 
 SOURCES = ["ff_kernels.hip", "ff_abi.cpp", "ff_bvh.cpp"]
 
@@ -61,7 +63,27 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
+
+
+def build_host(verbose: bool = False) -> None:
+    """C++ host interface (libfirefly.so, links libff_hip.so) and the demo."""
+    cc = hipcc()
+    host = os.path.join(HERE, "host")
+    common = ["-O2", "-std=c++17", "-fPIC", "-I", os.path.join(ROOT, "include"), "-I", host,
+              "-Wl,-rpath,$ORIGIN", "-L", HERE]
+    steps = [
+        ([cc] + common + ["-shared", os.path.join(host, "firefly.cpp"), "-lff_hip", "-o", HOST_LIB + ".tmp"],
+         HOST_LIB),
+        ([cc] + common + [os.path.join(host, "render_demo.cpp"), "-lfirefly", "-lff_hip", "-o", DEMO + ".tmp"],
+         DEMO),
+    ]
+    for cmd, out in steps:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(out + ".tmp", out)
 
 
 def build_oracle(verbose: bool = False) -> str:

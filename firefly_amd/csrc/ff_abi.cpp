@@ -90,7 +90,9 @@ struct ff_scene {
     int32_t n_tris = 0, n_mats = 0, max_depth = 0;
     std::vector<ffh::BvhNode> nodes;
     std::vector<int32_t> ordered;
-    float4* d_nodes = nullptr;
+    float4* d_nodes = nullptr;       // two-child records
+    int32_t* d_leaf_count = nullptr;
+    float* d_root = nullptr;
     float* d_tris = nullptr;
     float* d_nrm = nullptr;
     float* d_uv = nullptr;
@@ -102,7 +104,7 @@ struct ff_scene {
     bool has_type[4] = {false, false, false, false};
     ff_dev_scene view() const {
         ff_dev_scene s;
-        s.nodes = d_nodes; s.tris = d_tris; s.nrm = d_nrm; s.uv = d_uv; s.mat = d_mat;
+        s.nodes = d_nodes; s.leaf_count = d_leaf_count; s.root = d_root; s.tris = d_tris; s.nrm = d_nrm; s.uv = d_uv; s.mat = d_mat;
         s.mats = d_mats; s.tex = d_tex_desc; s.n_nodes = (int32_t)nodes.size(); s.n_tris = n_tris;
         s.n_mats = n_mats; s.n_tex = n_tex; s.max_depth = max_depth;
         return s;
@@ -158,6 +160,43 @@ ff_status ensure_ovf(ff_context* c, int32_t max_depth) {
     HIP_TRY(dalloc(&c->ovf, need * (size_t)FF_TRAV_GRID * FF_BLOCK));
     c->ovf_entries = need;
     return FF_OK;
+}
+
+// The reference's pre-order binary BVH re-packed as 64-byte two-child
+// records: record k holds both child boxes of the k-th interior node, the
+// child refs and the split axis.  Leaf refs are ~(first slot << 4 | count),
+// count 15 meaning "read leaf_count[first slot]".
+void pack_two_child(const std::vector<ffh::BvhNode>& nodes, std::vector<float4>& wide,
+                    std::vector<int32_t>& leaf_count, float root[8], size_t n_tris) {
+    std::vector<int32_t> index(nodes.size(), -1);
+    int32_t nw = 0;
+    for (size_t i = 0; i < nodes.size(); i++)
+        if (nodes[i].n_prims == 0) index[i] = nw++;
+    leaf_count.assign(std::max<size_t>(n_tris, 1), 0);
+    auto ref_of = [&](size_t i) -> int32_t {
+        const ffh::BvhNode& nd = nodes[i];
+        if (nd.n_prims == 0) return index[i];
+        int32_t c = nd.n_prims < 15 ? nd.n_prims : 15;
+        if (c == 15) leaf_count[nd.offset] = nd.n_prims;
+        return ~((nd.offset << 4) | c);
+    };
+    wide.assign((size_t)nw * 4, make_float4(0.f, 0.f, 0.f, 0.f));
+    for (size_t i = 0; i < nodes.size(); i++) {
+        if (nodes[i].n_prims != 0) continue;
+        const ffh::BvhNode& A = nodes[i + 1];
+        const ffh::BvhNode& B = nodes[nodes[i].offset];
+        float4* w = &wide[(size_t)index[i] * 4];
+        w[0] = make_float4(A.bmin[0], A.bmin[1], A.bmin[2], A.bmax[0]);
+        w[1] = make_float4(A.bmax[1], A.bmax[2], B.bmin[0], B.bmin[1]);
+        w[2] = make_float4(B.bmin[2], B.bmax[0], B.bmax[1], B.bmax[2]);
+        int32_t m[4] = {ref_of(i + 1), ref_of((size_t)nodes[i].offset), (int32_t)nodes[i].axis, 0};
+        std::memcpy(&w[3], m, 16);
+    }
+    std::memcpy(root, nodes[0].bmin, 12);
+    std::memcpy(root + 3, nodes[0].bmax, 12);
+    int32_t r = ref_of(0);
+    std::memcpy(root + 6, &r, 4);
+    root[7] = 0.f;
 }
 
 ff_dev_camera dev_camera(const ff_camera* cam) {
@@ -229,7 +268,8 @@ ff_status ff_scene_create(ff_context* ctx, const ff_scene_desc* d, ff_scene** ou
     if (d->n_tris <= 0 || !d->positions || !d->normals || !d->material_ids)
         return fail(FF_ERR_INVALID, "ff_scene_create: empty scene or missing positions/normals/material ids");
     if (d->n_materials <= 0 || !d->materials) return fail(FF_ERR_INVALID, "ff_scene_create: no materials");
-    if (d->n_tris > (1 << 30)) return fail(FF_ERR_CAPACITY, "ff_scene_create: %d triangles", d->n_tris);
+    if (d->n_tris >= (1 << 27))
+        return fail(FF_ERR_CAPACITY, "ff_scene_create: %d triangles (limit 2^27 - 1)", d->n_tris);
     for (int32_t i = 0; i < d->n_tris; i++)
         if (d->material_ids[i] < 0 || d->material_ids[i] >= d->n_materials)
             return fail(FF_ERR_INVALID, "ff_scene_create: triangle %d material %d out of range", i, d->material_ids[i]);
@@ -278,11 +318,20 @@ ff_status ff_scene_create(ff_context* ctx, const ff_scene_desc* d, ff_scene** ou
     }
     for (int32_t m = 0; m < d->n_materials; m++) s->has_type[(int)d->materials[(size_t)m * FF_MAT_FLOATS]] = true;
     auto cleanup = [&](ff_status st) { ff_scene_destroy(s); return st; };
-    if (dalloc(&s->d_nodes, 2 * s->nodes.size()) != hipSuccess || dalloc(&s->d_tris, 9 * n) != hipSuccess ||
+    std::vector<float4> wide;
+    std::vector<int32_t> leaf_count;
+    float root[8];
+    pack_two_child(s->nodes, wide, leaf_count, root, n);
+    if (dalloc(&s->d_nodes, wide.size()) != hipSuccess || dalloc(&s->d_leaf_count, leaf_count.size()) != hipSuccess ||
+        dalloc(&s->d_root, 8) != hipSuccess || dalloc(&s->d_tris, 9 * n) != hipSuccess ||
         dalloc(&s->d_nrm, 9 * n) != hipSuccess || dalloc(&s->d_uv, 6 * n) != hipSuccess ||
         dalloc(&s->d_mat, n) != hipSuccess || dalloc(&s->d_mats, (size_t)FF_MAT_FLOATS * d->n_materials) != hipSuccess)
         return cleanup(fail(FF_ERR_ALLOC, "ff_scene_create: device allocation failed"));
-    hipError_t e = hipMemcpy(s->d_nodes, s->nodes.data(), s->nodes.size() * 32, hipMemcpyHostToDevice);
+    hipError_t e = hipSuccess;
+    if (!wide.empty()) e = hipMemcpy(s->d_nodes, wide.data(), wide.size() * sizeof(float4), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(s->d_leaf_count, leaf_count.data(), leaf_count.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->d_root, root, sizeof(root), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->d_tris, tris.data(), 36 * n, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->d_nrm, nrm.data(), 36 * n, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->d_uv, uv.data(), 24 * n, hipMemcpyHostToDevice);
@@ -313,7 +362,7 @@ ff_status ff_scene_create(ff_context* ctx, const ff_scene_desc* d, ff_scene** ou
 ff_status ff_scene_destroy(ff_scene* s) {
     if (!s) return fail(FF_ERR_INVALID, "ff_scene_destroy: NULL scene");
     (void)hipDeviceSynchronize();
-    dfree(s->d_nodes); dfree(s->d_tris); dfree(s->d_nrm); dfree(s->d_uv); dfree(s->d_mat); dfree(s->d_mats);
+    dfree(s->d_nodes); dfree(s->d_leaf_count); dfree(s->d_root); dfree(s->d_tris); dfree(s->d_nrm); dfree(s->d_uv); dfree(s->d_mat); dfree(s->d_mats);
     for (auto& p : s->d_tex) dfree(p);
     dfree(s->d_tex_desc);
     delete s;

@@ -301,19 +301,19 @@ __device__ __forceinline__ float fresnel_dielectric(float cos_i, float eta_i, fl
 }
 
 // ------------------------------------------------------------ traversal --
-// Bounds3f::hit(ray, invDir, dirIsNeg) (CRT/Bounds3f.h:128-160) on a node
-// held as two float4: (bmin.xyz, bmax.x), (bmax.yz, offset, nprims|axis<<16).
-__device__ __forceinline__ bool slab(float4 a, float4 b, f3 o, f3 inv, bool nx, bool ny, bool nz,
-                                     float& tn, float& tf) {
-    float tmin = ((nx ? a.w : a.x) - o.x) * inv.x;
-    float tmax = ((nx ? a.x : a.w) - o.x) * inv.x;
-    float tymin = ((ny ? b.x : a.y) - o.y) * inv.y;
-    float tymax = ((ny ? a.y : b.x) - o.y) * inv.y;
+// Bounds3f::hit(ray, invDir, dirIsNeg) (CRT/Bounds3f.h:128-160) on one box:
+// the line test without [0, tmax] clipping; returns the final tmin/tmax.
+__device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, float hy, float hz, f3 o,
+                                     f3 inv, bool nx, bool ny, bool nz, float& tn, float& tf) {
+    float tmin = ((nx ? hx : lx) - o.x) * inv.x;
+    float tmax = ((nx ? lx : hx) - o.x) * inv.x;
+    float tymin = ((ny ? hy : ly) - o.y) * inv.y;
+    float tymax = ((ny ? ly : hy) - o.y) * inv.y;
     if ((tmin > tymax) || (tymin > tmax)) return false;
     if (tymin > tmin) tmin = tymin;
     if (tymax < tmax) tmax = tymax;
-    float tzmin = ((nz ? b.y : a.z) - o.z) * inv.z;
-    float tzmax = ((nz ? a.z : b.y) - o.z) * inv.z;
+    float tzmin = ((nz ? hz : lz) - o.z) * inv.z;
+    float tzmax = ((nz ? lz : hz) - o.z) * inv.z;
     if ((tmin > tzmax) || (tzmin > tmax)) return false;
     if (tzmin > tmin) tmin = tzmin;
     if (tzmax < tmax) tmax = tzmax;
@@ -407,93 +407,131 @@ struct Stack {
 
 struct Hit { float t, u, v; int id; };
 
+// Device BVH: the reference's binary tree (same nodes, same order) stored as
+// 64-byte two-child records so one fetch tests both children's boxes:
+//   a = (c0.lo.xyz, c0.hi.x)  b = (c0.hi.yz, c1.lo.xy)  c = (c1.lo.z, c1.hi.xyz)
+//   m = (ref0, ref1, axis, -)
+// ref >= 0: interior record index; ref < 0: leaf, ~ref = first slot << 4 |
+// count (count 15 = read leaf_count[first slot]).  The root box is tested
+// first, as the reference tests node 0.
+struct Bvh {
+    const float4* nodes;
+    const int32_t* leaf_count;
+    const float* root;  // lo.xyz, hi.xyz, ref (int bits)
+};
+
+__device__ __forceinline__ void leaf_range(const Bvh& B, int ref, int& off, int& cnt) {
+    int e = ~ref;
+    off = e >> 4;
+    cnt = e & 15;
+    if (cnt == 15) cnt = B.leaf_count[off];
+}
+
 // intersect() closest hit (CRT/dTriangle.cu:140-213): near child first by
-// dirIsNeg[axis]; a candidate replaces the record when 0 <= t < tmin.
-__device__ __forceinline__ Hit closest_hit(const float4* __restrict__ nodes,
-                                           const float* __restrict__ tris, f3 o, f3 d,
+// dirIsNeg[axis], far child pushed; a candidate replaces the record when
+// 0 <= t < tmin.  Children whose boxes miss (or lie beyond the current hit by
+// the conservative slack, or wholly behind the origin) are never visited.
+__device__ __forceinline__ Hit closest_hit(const Bvh& B, const float* __restrict__ tris, f3 o, f3 d,
                                            bool two_sided, Stack& st, uint32_t& cnt_nodes,
                                            uint32_t& cnt_tris) {
     Hit h;
     h.t = K_HUGE; h.u = 0.f; h.v = 0.f; h.id = -1;
     f3 inv = V3(1.f / d.x, 1.f / d.y, 1.f / d.z);
     bool nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
+    float tn, tf;
+    cnt_nodes++;
+    if (!slab(B.root[0], B.root[1], B.root[2], B.root[3], B.root[4], B.root[5], o, inv, nx, ny, nz, tn, tf) ||
+        tf < -PRUNE_BEHIND)
+        return h;
+    int ref = __float_as_int(B.root[6]);
     st.sp = 0;
-    int cur = 0;
     for (;;) {
-        float4 a = nodes[2 * cur], b = nodes[2 * cur + 1];
-        cnt_nodes++;
-        float tn, tf;
-        bool hit = slab(a, b, o, inv, nx, ny, nz, tn, tf);
-        if (hit) {
+        if (ref >= 0) {
+            const float4* n = B.nodes + 4 * (size_t)ref;
+            float4 a = n[0], b = n[1], c = n[2];
+            int4 m = *(const int4*)(n + 3);
+            cnt_nodes += 2;
             float lim = h.t + h.t * PRUNE_REL + PRUNE_ABS;
-            if (tn > lim || tf < -PRUNE_BEHIND) hit = false;
-        }
-        if (hit) {
-            uint32_t meta = __float_as_uint(b.w);
-            int off = __float_as_int(b.z);
-            int np = (int)(meta & 0xffffu);
-            if (np > 0) {
-                for (int i = 0; i < np; i++) {
-                    Tri T = load_tri(tris, off + i);
-                    cnt_tris++;
-                    float t = 0.f, u = 0.f, v = 0.f;
-                    bool ok = two_sided ? mt_two(T, o, d, t, u, v) : mt_cull(T, o, d, t, u, v);
-                    if (ok && !(t < 0.f) && t < h.t) { h.t = t; h.u = u; h.v = v; h.id = off + i; }
-                }
-                if (st.sp == 0) break;
-                cur = st.pop();
-            } else {
-                int axis = (int)((meta >> 16) & 0xffu);
-                bool ng = axis == 0 ? nx : (axis == 1 ? ny : nz);
-                if (ng) { st.push(cur + 1); cur = off; }
-                else { st.push(off); cur = cur + 1; }
+            float tn0, tf0, tn1, tf1;
+            bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, nx, ny, nz, tn0, tf0) &&
+                      !(tn0 > lim || tf0 < -PRUNE_BEHIND);
+            bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, nx, ny, nz, tn1, tf1) &&
+                      !(tn1 > lim || tf1 < -PRUNE_BEHIND);
+            bool neg = m.z == 0 ? nx : (m.z == 1 ? ny : nz);
+            int nref = neg ? m.y : m.x, fref = neg ? m.x : m.y;
+            bool hn = neg ? h1 : h0, hf = neg ? h0 : h1;
+            if (hn) {
+                if (hf) st.push(fref);
+                ref = nref;
+                continue;
+            }
+            if (hf) {
+                ref = fref;
+                continue;
             }
         } else {
-            if (st.sp == 0) break;
-            cur = st.pop();
+            int off, np;
+            leaf_range(B, ref, off, np);
+            for (int i = 0; i < np; i++) {
+                Tri T = load_tri(tris, off + i);
+                cnt_tris++;
+                float t = 0.f, u = 0.f, v = 0.f;
+                bool ok = two_sided ? mt_two(T, o, d, t, u, v) : mt_cull(T, o, d, t, u, v);
+                if (ok && !(t < 0.f) && t < h.t) { h.t = t; h.u = u; h.v = v; h.id = off + i; }
+            }
         }
+        if (st.sp == 0) break;
+        ref = st.pop();
     }
     return h;
 }
 
 // intersect_shadows with tmin = K_HUGE (CRT/dTriangle.cu:215-259,
 // CRT/dEnvironmentLight.cu:60-64): true when any triangle has 0 <= t < 1e32.
-__device__ __forceinline__ bool any_hit(const float4* __restrict__ nodes,
-                                        const float* __restrict__ tris, f3 o, f3 d, Stack& st,
+__device__ __forceinline__ bool any_hit(const Bvh& B, const float* __restrict__ tris, f3 o, f3 d, Stack& st,
                                         uint32_t& cnt_nodes, uint32_t& cnt_tris) {
     f3 inv = V3(1 / d.x, 1 / d.y, 1 / d.z);
     bool nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
+    float tn, tf;
+    cnt_nodes++;
+    if (!slab(B.root[0], B.root[1], B.root[2], B.root[3], B.root[4], B.root[5], o, inv, nx, ny, nz, tn, tf) ||
+        tf < -PRUNE_BEHIND)
+        return false;
+    int ref = __float_as_int(B.root[6]);
     st.sp = 0;
-    int cur = 0;
     for (;;) {
-        float4 a = nodes[2 * cur], b = nodes[2 * cur + 1];
-        cnt_nodes++;
-        float tn, tf;
-        bool hit = slab(a, b, o, inv, nx, ny, nz, tn, tf);
-        if (hit && tf < -PRUNE_BEHIND) hit = false;
-        if (hit) {
-            uint32_t meta = __float_as_uint(b.w);
-            int off = __float_as_int(b.z);
-            int np = (int)(meta & 0xffffu);
-            if (np > 0) {
-                for (int i = 0; i < np; i++) {
-                    Tri T = load_tri(tris, off + i);
-                    cnt_tris++;
-                    float t = 0.f;
-                    if (mt_shadow(T, o, d, t) && t < K_HUGE) return true;
-                }
-                if (st.sp == 0) break;
-                cur = st.pop();
-            } else {
-                int axis = (int)((meta >> 16) & 0xffu);
-                bool ng = axis == 0 ? nx : (axis == 1 ? ny : nz);
-                if (ng) { st.push(cur + 1); cur = off; }
-                else { st.push(off); cur = cur + 1; }
+        if (ref >= 0) {
+            const float4* n = B.nodes + 4 * (size_t)ref;
+            float4 a = n[0], b = n[1], c = n[2];
+            int4 m = *(const int4*)(n + 3);
+            cnt_nodes += 2;
+            float tn0, tf0, tn1, tf1;
+            bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, nx, ny, nz, tn0, tf0) && !(tf0 < -PRUNE_BEHIND);
+            bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, nx, ny, nz, tn1, tf1) && !(tf1 < -PRUNE_BEHIND);
+            bool neg = m.z == 0 ? nx : (m.z == 1 ? ny : nz);
+            int nref = neg ? m.y : m.x, fref = neg ? m.x : m.y;
+            bool hn = neg ? h1 : h0, hf = neg ? h0 : h1;
+            if (hn) {
+                if (hf) st.push(fref);
+                ref = nref;
+                continue;
+            }
+            if (hf) {
+                ref = fref;
+                continue;
             }
         } else {
-            if (st.sp == 0) break;
-            cur = st.pop();
+            int off, np;
+            leaf_range(B, ref, off, np);
+            for (int i = 0; i < np; i++) {
+                Tri T = load_tri(tris, off + i);
+                cnt_tris++;
+                float t = 0.f;
+                if (mt_shadow(T, o, d, t) && t < K_HUGE) return true;
+            }
         }
+        if (st.sp == 0) break;
+        ref = st.pop();
     }
     return false;
 }

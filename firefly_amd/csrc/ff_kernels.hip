@@ -14,6 +14,14 @@ namespace {
 __device__ __forceinline__ uint32_t gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ __forceinline__ uint32_t gstride() { return gridDim.x * blockDim.x; }
 
+__device__ __forceinline__ Bvh bvh_of(const ff_dev_scene& S) {
+    Bvh b;
+    b.nodes = S.nodes;
+    b.leaf_count = S.leaf_count;
+    b.root = S.root;
+    return b;
+}
+
 __device__ __forceinline__ EnvView env_view(const ff_dev_env& e) {
     EnvView v;
     v.rgb = e.rgb; v.marg_y = e.marg_y; v.conds = e.conds; v.pdf = e.pdf; v.w = e.w; v.h = e.h;
@@ -59,7 +67,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_extend(ff_dev_scene S, ff_paths P,
         uint32_t id = queue[i];
         float4 r0 = P.ray[2 * id], r1 = P.ray[2 * id + 1];
         uint32_t stw = __float_as_uint(P.beta[id].w);
-        Hit h = closest_hit(S.nodes, S.tris, xyz(r0), xyz(r1), (stw & ST_INSIDE) != 0, st, cn, ct);
+        Hit h = closest_hit(bvh_of(S), S.tris, xyz(r0), xyz(r1), (stw & ST_INSIDE) != 0, st, cn, ct);
         P.hit[id] = make_float4(h.t, h.u, h.v, __int_as_float(h.id));
     }
     wave_sum_add(&P.stats[S_EXT_NODES], cn);
@@ -80,7 +88,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shadow(ff_dev_scene S, ff_paths P)
     for (uint32_t i = gtid(); i < n; i += gstride()) {
         uint32_t s = P.q_shadow[i];
         float4 r0 = P.sray[2 * s], r1 = P.sray[2 * s + 1];
-        bool occ = any_hit(S.nodes, S.tris, xyz(r0), xyz(r1), st, cn, ct);
+        bool occ = any_hit(bvh_of(S), S.tris, xyz(r0), xyz(r1), st, cn, ct);
         P.vis[s] = occ ? 0 : 1;
     }
     wave_sum_add(&P.stats[S_SHD_NODES], cn);
@@ -406,7 +414,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_trace_closest(ff_dev_scene S, cons
     uint32_t cn = 0, ct = 0;
     for (uint32_t i = gtid(); i < n; i += gstride()) {
         float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
-        Hit h = closest_hit(S.nodes, S.tris, xyz(r0), xyz(r1), two_sided != 0, st, cn, ct);
+        Hit h = closest_hit(bvh_of(S), S.tris, xyz(r0), xyz(r1), two_sided != 0, st, cn, ct);
         hits[i] = make_float4(h.t, h.u, h.v, __int_as_float(h.id));
     }
     if (counts) {
@@ -426,7 +434,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_trace_any(ff_dev_scene S, const fl
     uint32_t cn = 0, ct = 0;
     for (uint32_t i = gtid(); i < n; i += gstride()) {
         float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
-        occ[i] = any_hit(S.nodes, S.tris, xyz(r0), xyz(r1), st, cn, ct) ? 1 : 0;
+        occ[i] = any_hit(bvh_of(S), S.tris, xyz(r0), xyz(r1), st, cn, ct) ? 1 : 0;
     }
     if (counts) {
         wave_sum_add(&counts[0], cn);

@@ -2,7 +2,8 @@
 //
 // HBM layout (all arrays are structure-of-arrays indexed by path id or BVH
 // slot; float4 fields keep every per-path record a 16-byte aligned load):
-//   nodes   : 2 x float4 per BVH node (32 B, CRT/BVH.h:63-72 field order)
+//   nodes   : the reference's binary BVH as 64-byte two-child records
+//             (both child boxes + child refs; 32 B per box tested)
 //   tris    : 9 floats per slot, v0 | e1 | e2 (36 B, positions only)
 //   nrm/uv  : shading attributes per slot, read once per closest hit
 //   paths   : ray (2 x float4), hit, beta+state, Lp, fext, nee[3], sray[3][2],
@@ -61,7 +62,9 @@ struct ff_tex_desc {
 };
 
 struct ff_dev_scene {
-    const float4* nodes;
+    const float4* nodes;       // two-child records, 4 x float4 each
+    const int32_t* leaf_count; // per slot, for leaves of 15 or more primitives
+    const float* root;         // root box lo.xyz hi.xyz, root ref
     const float* tris;
     const float* nrm;
     const float* uv;

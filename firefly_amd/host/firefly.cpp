@@ -1,0 +1,372 @@
+// firefly.cpp -- C++ host interface over the C ABI (see firefly.h).
+#include "firefly.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+
+namespace firefly {
+
+void check(ff_status st, const char* what) {
+    if (st != FF_OK) throw Error(st, std::string(what) + ": " + ff_last_error());
+}
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+vec3 vnorm(vec3 v) {
+    double l = std::sqrt((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z);
+    return vec3((float)(v.x / l), (float)(v.y / l), (float)(v.z / l));
+}
+vec3 vcross(vec3 a, vec3 b) { return vec3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+
+// column-major 4x4 helpers in double: m[c*4 + r]
+void mat_mul(const double* a, const double* b, double* out) {
+    for (int c = 0; c < 4; c++)
+        for (int r = 0; r < 4; r++) {
+            double s = 0;
+            for (int k = 0; k < 4; k++) s += a[k * 4 + r] * b[c * 4 + k];
+            out[c * 4 + r] = s;
+        }
+}
+bool mat_inv(const double* m, double* out) {
+    double a[4][8];
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) {
+            a[r][c] = m[c * 4 + r];
+            a[r][c + 4] = r == c ? 1.0 : 0.0;
+        }
+    for (int col = 0; col < 4; col++) {
+        int piv = col;
+        for (int r = col + 1; r < 4; r++)
+            if (std::fabs(a[r][col]) > std::fabs(a[piv][col])) piv = r;
+        if (a[piv][col] == 0.0) return false;
+        if (piv != col)
+            for (int k = 0; k < 8; k++) std::swap(a[piv][k], a[col][k]);
+        double inv = 1.0 / a[col][col];
+        for (int k = 0; k < 8; k++) a[col][k] *= inv;
+        for (int r = 0; r < 4; r++)
+            if (r != col) {
+                double f = a[r][col];
+                for (int k = 0; k < 8; k++) a[r][k] -= f * a[col][k];
+            }
+    }
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) out[c * 4 + r] = a[r][c + 4];
+    return true;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- Camera --
+void Camera::updateCameraVectors() {
+    double y = yaw * kPi / 180.0, p = pitch * kPi / 180.0;
+    front = vnorm(vec3((float)(std::cos(y) * std::cos(p)), (float)std::sin(p), (float)(std::sin(y) * std::cos(p))));
+    right = vnorm(vcross(front, worldUp));
+    up = vnorm(vcross(right, front));
+    // right-handed lookAt(position, position + front, up)
+    vec3 f = front, s = right, u = up;
+    float m[16] = {s.x, u.x, -f.x, 0.f, s.y, u.y, -f.y, 0.f, s.z, u.z, -f.z, 0.f, 0.f, 0.f, 0.f, 1.f};
+    m[12] = -(s.x * position.x + s.y * position.y + s.z * position.z);
+    m[13] = -(u.x * position.x + u.y * position.y + u.z * position.z);
+    m[14] = f.x * position.x + f.y * position.y + f.z * position.z;
+    std::memcpy(lookat_mat, m, sizeof(m));
+}
+
+void Camera::look_at(vec3 target) {
+    vec3 d = vnorm(vec3(target.x - position.x, target.y - position.y, target.z - position.z));
+    pitch = (float)(std::asin(std::max(-1.f, std::min(1.f, d.y))) * 180.0 / kPi);
+    yaw = (float)(std::atan2(d.z, d.x) * 180.0 / kPi);
+    updateCameraVectors();
+}
+
+void Camera::processMouseMovement(float xoffset, float yoffset, bool constrainPitch) {
+    yaw += xoffset * 0.1f;
+    pitch += yoffset * 0.1f;
+    if (constrainPitch) pitch = std::max(-89.f, std::min(89.f, pitch));
+    updateCameraVectors();
+}
+
+void Camera::processMouseScroll(float yoffset) {
+    zoom = std::max(1.f, std::min(120.f, zoom - yoffset));
+    updateCameraVectors();
+}
+
+void Camera::inv_view_proj(float out[16]) const {
+    double p[16], v[16], pv[16], inv[16];
+    for (int i = 0; i < 16; i++) { p[i] = proj_mat[i]; v[i] = lookat_mat[i]; }
+    mat_mul(p, v, pv);
+    if (!mat_inv(pv, inv)) throw Error(FF_ERR_INVALID, "camera: singular proj * view");
+    for (int i = 0; i < 16; i++) out[i] = (float)inv[i];
+}
+
+PerspectiveCamera::PerspectiveCamera() : aspectRatio(1.6f), yfov(90.f) {
+    znear = 1.f;
+    zfar = 100.f;
+    set_projection();
+    updateCameraVectors();
+}
+PerspectiveCamera::PerspectiveCamera(float a, float fov, float zn, float zf) : aspectRatio(a), yfov(fov) {
+    znear = zn;
+    zfar = zf;
+    set_projection();
+    updateCameraVectors();
+}
+PerspectiveCamera::PerspectiveCamera(vec3 pos, float a, float fov, float zn, float zf) : aspectRatio(a), yfov(fov) {
+    position = pos;
+    znear = zn;
+    zfar = zf;
+    set_projection();
+    updateCameraVectors();
+}
+void PerspectiveCamera::set_projection() {
+    // right-handed perspective, clip z in [-1, 1]; yfov in degrees
+    double f = 1.0 / std::tan(yfov * kPi / 360.0);
+    std::memset(proj_mat, 0, sizeof(proj_mat));
+    proj_mat[0] = (float)(f / aspectRatio);
+    proj_mat[5] = (float)f;
+    proj_mat[10] = (float)(-(zfar + znear) / (double)(zfar - znear));
+    proj_mat[11] = -1.f;
+    proj_mat[14] = (float)(-(2.0 * zfar * znear) / (double)(zfar - znear));
+}
+
+// ------------------------------------------------------ EnvironmentLight --
+EnvironmentLight::EnvironmentLight(vec3 c) : color(c), width(2), height(2), rgb(12) {
+    for (int i = 0; i < 4; i++) { rgb[3 * i] = c.x; rgb[3 * i + 1] = c.y; rgb[3 * i + 2] = c.z; }
+}
+
+EnvironmentLight::EnvironmentLight(int w, int h, std::vector<float> data) : width(w), height(h), rgb(std::move(data)) {
+    if (w <= 0 || h <= 0 || rgb.size() != (size_t)w * h * 3)
+        throw Error(FF_ERR_INVALID, "EnvironmentLight: texel count does not match the size");
+}
+
+// Radiance RGBE reader: header, "-Y h +X w" resolution line, new-style RLE
+// or flat scanlines; value = mantissa * 2^(exponent - 136).
+EnvironmentLight::EnvironmentLight(const std::string& path) : filepath(path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw Error(FF_ERR_INVALID, "EnvironmentLight: cannot open " + path);
+    std::string line;
+    bool fmt = false;
+    while (std::getline(f, line)) {
+        if (line.empty()) break;
+        if (line.rfind("FORMAT=32-bit_rle_rgbe", 0) == 0) fmt = true;
+    }
+    if (!fmt) throw Error(FF_ERR_INVALID, "EnvironmentLight: not a 32-bit_rle_rgbe file: " + path);
+    std::getline(f, line);
+    int w = 0, h = 0;
+    if (std::sscanf(line.c_str(), "-Y %d +X %d", &h, &w) != 2 || w <= 0 || h <= 0)
+        throw Error(FF_ERR_INVALID, "EnvironmentLight: unsupported resolution line: " + line);
+    width = w;
+    height = h;
+    rgb.assign((size_t)w * h * 3, 0.f);
+    std::vector<unsigned char> scan((size_t)w * 4);
+    for (int y = 0; y < h; y++) {
+        unsigned char hd[4];
+        if (!f.read((char*)hd, 4)) throw Error(FF_ERR_INVALID, "EnvironmentLight: truncated file");
+        if (w >= 8 && w < 32768 && hd[0] == 2 && hd[1] == 2 && ((hd[2] << 8) | hd[3]) == w) {
+            for (int ch = 0; ch < 4; ch++) {
+                int x = 0;
+                while (x < w) {
+                    int c = f.get();
+                    if (c == EOF) throw Error(FF_ERR_INVALID, "EnvironmentLight: truncated RLE");
+                    if (c > 128) {
+                        int cnt = c - 128, v = f.get();
+                        if (x + cnt > w) throw Error(FF_ERR_INVALID, "EnvironmentLight: bad RLE run");
+                        for (int k = 0; k < cnt; k++) scan[(size_t)(x++) * 4 + ch] = (unsigned char)v;
+                    } else {
+                        if (c == 0 || x + c > w) throw Error(FF_ERR_INVALID, "EnvironmentLight: bad RLE dump");
+                        for (int k = 0; k < c; k++) scan[(size_t)(x++) * 4 + ch] = (unsigned char)f.get();
+                    }
+                }
+            }
+        } else {
+            std::memcpy(scan.data(), hd, 4);
+            if (!f.read((char*)scan.data() + 4, (std::streamsize)w * 4 - 4))
+                throw Error(FF_ERR_INVALID, "EnvironmentLight: truncated scanline");
+        }
+        int row = h - 1 - y;  // bottom-up, as loaded with vertical flip
+        for (int x = 0; x < w; x++) {
+            const unsigned char* p = &scan[(size_t)x * 4];
+            float s = p[3] ? std::ldexp(1.f, (int)p[3] - 136) : 0.f;
+            float* o = &rgb[((size_t)row * w + x) * 3];
+            o[0] = p[0] * s;
+            o[1] = p[1] * s;
+            o[2] = p[2] * s;
+        }
+    }
+}
+
+// ----------------------------------------------------------------- Scene --
+int Scene::get_nmb_of_triangles() const {
+    int n = 0;
+    for (const Model& m : models)
+        for (const Mesh& me : m.meshes) n += me.get_nmb_of_triangles();
+    return n;
+}
+
+// ---------------------------------------------------------------- dScene --
+dScene::dScene(Scene* s, int device, int max_prims_in_node) : h_scene(s) {
+    if (!s || !s->camera || !s->environment_light) throw Error(FF_ERR_INVALID, "dScene: scene needs a camera and an environment light");
+    // material table (dScene::load_materials, CRT/dScene.cpp:172-214)
+    std::map<std::string, int> index;
+    std::vector<float> mats;
+    auto push_mat = [&](const Material& m) {
+        float r[FF_MAT_FLOATS] = {(float)m.type, m.baseColorFactor.x, m.baseColorFactor.y, m.baseColorFactor.z,
+                                  m.roughnessFactor, m.metallicFactor, m.fresnel.x, m.fresnel.y, m.fresnel.z, m.ior,
+                                  (float)m.baseColorTexture, (float)m.roughnessTexture, 0.f, 0.f, 0.f, 0.f};
+        mats.insert(mats.end(), r, r + FF_MAT_FLOATS);
+    };
+    push_mat(Material{});  // index 0: default material for unknown names
+    for (auto& kv : s->materials_loaded) {
+        index[kv.first] = (int)(mats.size() / FF_MAT_FLOATS);
+        push_mat(kv.second);
+    }
+    // triangles of every mesh in model order (dScene::load_models, CRT/dScene.cpp:82-170)
+    std::vector<float> pos, nrm, uv;
+    std::vector<int32_t> mid;
+    for (const Model& m : s->models)
+        for (const Mesh& me : m.meshes) {
+            auto it = index.find(me.material);
+            int mi = it == index.end() ? 0 : it->second;
+            bool has_n = me.normals.size() == me.positions.size();
+            bool has_t = me.texcoords.size() == me.positions.size();
+            for (size_t k = 0; k + 2 < me.indices.size(); k += 3) {
+                for (int v = 0; v < 3; v++) {
+                    uint32_t vi = me.indices[k + v];
+                    if (vi >= me.positions.size()) throw Error(FF_ERR_INVALID, "dScene: index out of range in " + me.name);
+                    const vec3& p = me.positions[vi];
+                    pos.insert(pos.end(), {p.x, p.y, p.z});
+                    vec3 n = has_n ? me.normals[vi] : vec3(0.f, 1.f, 0.f);
+                    nrm.insert(nrm.end(), {n.x, n.y, n.z});
+                    vec2 t = has_t ? me.texcoords[vi] : vec2{};
+                    uv.insert(uv.end(), {t.x, t.y});
+                }
+                mid.push_back(mi);
+            }
+        }
+    std::vector<const float*> tex_ptr;
+    std::vector<int32_t> tw, th, tc;
+    for (const Texture& t : s->textures_loaded) {
+        tex_ptr.push_back(t.data.data());
+        tw.push_back(t.width);
+        th.push_back(t.height);
+        tc.push_back(t.nrComponents);
+    }
+    ff_scene_desc d{};
+    d.n_tris = (int32_t)mid.size();
+    d.positions = pos.data();
+    d.normals = nrm.data();
+    d.uvs = uv.data();
+    d.material_ids = mid.data();
+    d.n_materials = (int32_t)(mats.size() / FF_MAT_FLOATS);
+    d.materials = mats.data();
+    d.n_textures = (int32_t)tex_ptr.size();
+    d.textures = tex_ptr.data();
+    d.tex_w = tw.data();
+    d.tex_h = th.data();
+    d.tex_c = tc.data();
+    d.max_prims_in_node = max_prims_in_node;
+    check(ff_context_create(device, &ctx), "dScene: context");
+    try {
+        check(ff_scene_create(ctx, &d, &scene), "dScene: scene");
+        const EnvironmentLight* e = s->environment_light;
+        check(ff_envmap_create(ctx, e->get_texels().data(), e->get_tex_width(), e->get_tex_height(), &env),
+              "dScene: environment light");
+        update();
+    } catch (...) {
+        if (env) ff_envmap_destroy(env);
+        if (scene) ff_scene_destroy(scene);
+        ff_context_destroy(ctx);
+        throw;
+    }
+}
+
+dScene::~dScene() {
+    if (env) ff_envmap_destroy(env);
+    if (scene) ff_scene_destroy(scene);
+    if (ctx) ff_context_destroy(ctx);
+}
+
+int dScene::get_nmb_nodes() const {
+    int32_t n = 0;
+    check(ff_scene_info(scene, nullptr, &n, nullptr), "dScene::get_nmb_nodes");
+    return n;
+}
+
+void dScene::update() {
+    const Camera* c = h_scene->camera;
+    c->inv_view_proj(camera.inv_view_proj);
+    camera.lens_radius = c->lens_radius;
+    camera.focal_distance = c->focal_distance;
+    camera.exposure = c->exposure;
+    camera.pad = 0.f;
+}
+
+// ------------------------------------------------------------ PathTracer --
+PathTracer::PathTracer(int w, int h) : width(w), height(h) {
+    if (w <= 0 || h <= 0) throw Error(FF_ERR_INVALID, "PathTracer: bad size");
+    if (hipMalloc((void**)&d_film, sizeof(float) * 4 * (size_t)w * h) != hipSuccess)
+        throw Error(FF_ERR_ALLOC, "PathTracer: film allocation");
+    clear_buffer();
+}
+
+PathTracer::~PathTracer() {
+    if (d_film) (void)hipFree(d_film);
+}
+
+void PathTracer::clear_buffer() {
+    if (hipMemset(d_film, 0, sizeof(float) * 4 * (size_t)width * height) != hipSuccess)
+        throw Error(FF_ERR_HIP, "PathTracer::clear_buffer");
+    n_samples = 0;
+}
+
+void PathTracer::draw(dScene* s) {
+    if (!s) throw Error(FF_ERR_INVALID, "PathTracer::draw: NULL scene");
+    s->update();
+    ff_render_req r{};
+    r.width = width;
+    r.height = height;
+    r.spp = samples_per_draw;
+    r.max_depth = (int32_t)max_path_length;
+    r.seed = seed;
+    r.spp_batch = samples_in_flight;
+    r.sample_offset = n_samples;
+    r.flags = 0;
+    check(ff_render(s->get_context(), s->get_triangles(), s->get_environment_light(), s->get_camera(), &r, d_film,
+                    nullptr),
+          "PathTracer::draw");
+    n_samples += samples_per_draw;
+}
+
+std::vector<float> PathTracer::get_image(dScene* s) const {
+    float* d_avg = nullptr;
+    size_t n = (size_t)width * height;
+    if (hipMalloc((void**)&d_avg, sizeof(float) * 3 * n) != hipSuccess) throw Error(FF_ERR_ALLOC, "get_image");
+    std::vector<float> out(3 * n);
+    ff_status st = ff_film_resolve(d_film, width, height, s->get_camera()->exposure, d_avg, nullptr, nullptr);
+    hipError_t e = st == FF_OK ? hipMemcpy(out.data(), d_avg, sizeof(float) * 3 * n, hipMemcpyDeviceToHost) : hipSuccess;
+    (void)hipFree(d_avg);
+    check(st, "PathTracer::get_image");
+    if (e != hipSuccess) throw Error(FF_ERR_HIP, "PathTracer::get_image: copy");
+    return out;
+}
+
+std::vector<uint8_t> PathTracer::get_rgba8(dScene* s) const {
+    uint8_t* d_rgba = nullptr;
+    size_t n = (size_t)width * height;
+    if (hipMalloc((void**)&d_rgba, 4 * n) != hipSuccess) throw Error(FF_ERR_ALLOC, "get_rgba8");
+    std::vector<uint8_t> out(4 * n);
+    ff_status st = ff_film_resolve(d_film, width, height, s->get_camera()->exposure, nullptr, d_rgba, nullptr);
+    hipError_t e = st == FF_OK ? hipMemcpy(out.data(), d_rgba, 4 * n, hipMemcpyDeviceToHost) : hipSuccess;
+    (void)hipFree(d_rgba);
+    check(st, "PathTracer::get_rgba8");
+    if (e != hipSuccess) throw Error(FF_ERR_HIP, "PathTracer::get_rgba8: copy");
+    return out;
+}
+
+}  // namespace firefly

@@ -1,0 +1,115 @@
+// render_demo.cpp -- renders a scene file through the C++ host interface
+// (Scene -> dScene -> PathTracer::draw, as the reference's main loop drives
+// it, CRT/main.cpp:245-275) and writes the averaged image as a PFM.
+//
+// usage: render_demo <scene.ffs> <out.pfm> <spp> <max_path_length> [samples_per_draw]
+//
+// Scene file (little endian): "FFS1", int32 n_meshes, n_materials, env_w,
+// env_h, width, height, uint32 seed, float eye[3], target[3], yfov_deg,
+// float materials[n_materials][16], float env[env_h][env_w][3], then per mesh
+// int32 material, int32 n_tris, float pos[n][9], nrm[n][9], uv[n][6].
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "firefly.h"
+
+using namespace firefly;
+
+namespace {
+
+template <typename T>
+void rd(std::ifstream& f, T* p, size_t n) {
+    if (!f.read((char*)p, (std::streamsize)(sizeof(T) * n))) throw Error(FF_ERR_INVALID, "scene file truncated");
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 5) {
+        std::fprintf(stderr, "usage: %s scene.ffs out.pfm spp max_path_length [samples_per_draw]\n", argv[0]);
+        return 2;
+    }
+    try {
+        std::ifstream f(argv[1], std::ios::binary);
+        if (!f) throw Error(FF_ERR_INVALID, std::string("cannot open ") + argv[1]);
+        char magic[4];
+        rd(f, magic, 4);
+        if (std::memcmp(magic, "FFS1", 4) != 0) throw Error(FF_ERR_INVALID, "not an FFS1 scene file");
+        int32_t hdr[6];
+        rd(f, hdr, 6);
+        int n_meshes = hdr[0], n_mats = hdr[1], env_w = hdr[2], env_h = hdr[3], width = hdr[4], height = hdr[5];
+        uint32_t seed;
+        rd(f, &seed, 1);
+        float cam[7];
+        rd(f, cam, 7);
+        std::vector<float> mats((size_t)n_mats * 16);
+        rd(f, mats.data(), mats.size());
+        std::vector<float> env((size_t)env_w * env_h * 3);
+        rd(f, env.data(), env.size());
+
+        Scene scene;
+        for (int m = 0; m < n_mats; m++) {
+            const float* r = &mats[(size_t)m * 16];
+            Material mt;
+            mt.name = "material" + std::to_string(1000 + m);
+            mt.type = (int)r[0];
+            mt.baseColorFactor = vec3(r[1], r[2], r[3]);
+            mt.roughnessFactor = r[4];
+            mt.metallicFactor = r[5];
+            mt.fresnel = vec3(r[6], r[7], r[8]);
+            mt.ior = r[9];
+            scene.add_material(mt);
+        }
+        Model model;
+        for (int k = 0; k < n_meshes; k++) {
+            int32_t mi[2];
+            rd(f, mi, 2);
+            int n = mi[1];
+            std::vector<float> pos((size_t)n * 9), nrm((size_t)n * 9), uv((size_t)n * 6);
+            rd(f, pos.data(), pos.size());
+            rd(f, nrm.data(), nrm.size());
+            rd(f, uv.data(), uv.size());
+            Mesh me;
+            me.name = "mesh" + std::to_string(k);
+            me.material = "material" + std::to_string(1000 + mi[0]);
+            for (int t = 0; t < 3 * n; t++) {
+                me.positions.push_back(vec3(pos[3 * t], pos[3 * t + 1], pos[3 * t + 2]));
+                me.normals.push_back(vec3(nrm[3 * t], nrm[3 * t + 1], nrm[3 * t + 2]));
+                me.texcoords.push_back(vec2{uv[2 * t], uv[2 * t + 1]});
+                me.indices.push_back((uint32_t)t);
+            }
+            model.meshes.push_back(std::move(me));
+        }
+        scene.add_model(std::move(model));
+        EnvironmentLight light(env_w, env_h, env);
+        scene.set_environment_light(&light);
+        PerspectiveCamera camera(vec3(cam[0], cam[1], cam[2]), (float)width / (float)height, cam[6], 0.1f, 100.f);
+        camera.look_at(vec3(cam[3], cam[4], cam[5]));
+        scene.set_camera(&camera);
+
+        dScene ds(&scene);
+        PathTracer pt(width, height);
+        pt.seed = seed;
+        pt.max_path_length = (uint32_t)std::atoi(argv[4]);
+        pt.samples_per_draw = argc > 5 ? std::atoi(argv[5]) : 1;
+        int spp = std::atoi(argv[3]);
+        while (pt.samples() < spp) pt.draw(&ds);
+        std::vector<float> img = pt.get_image(&ds);
+
+        FILE* o = std::fopen(argv[2], "wb");
+        if (!o) throw Error(FF_ERR_INVALID, std::string("cannot write ") + argv[2]);
+        std::fprintf(o, "PF\n%d %d\n-1.0\n", width, height);
+        for (int y = height - 1; y >= 0; y--)  // PFM rows run bottom to top
+            std::fwrite(&img[(size_t)y * width * 3], sizeof(float), (size_t)width * 3, o);
+        std::fclose(o);
+        std::printf("rendered %dx%d, %d spp, %d tris, %d BVH nodes\n", width, height, pt.samples(),
+                    scene.get_nmb_of_triangles(), ds.get_nmb_nodes());
+        return 0;
+    } catch (const Error& e) {
+        std::fprintf(stderr, "render_demo: %s (status %d)\n", e.what(), (int)e.status);
+        return 1;
+    }
+}

@@ -34,6 +34,11 @@ class SceneData:
     focal: float = 1.0
     exposure: float = 1.0
     name: str = ""
+    cam_eye: tuple = (0.0, 0.0, 0.0)
+    cam_target: tuple = (0.0, 0.0, -1.0)
+    cam_fovy: float = 45.0
+    mesh_sizes: tuple = ()     # triangles per mesh, in order
+    mesh_materials: tuple = ()  # material index of each mesh
 
     @property
     def n_tris(self) -> int:
@@ -257,9 +262,12 @@ def spheres_scene(width=256, height=256, spp=16, max_depth=4, seed=0, n_spheres=
         parts.append((*uv_sphere((cx, 0.3, cz), r, segments, rings), k + 1))
         mats.append(material(mtype, alb))
     pos, nrm, uv, mid = _concat(parts)
+    eye, target = (0.0, 1.2, 3.5), (0.0, 0.0, 0.0)
     return SceneData(pos, nrm, uv, mid, np.stack(mats), sky_env(512, 256),
-                     inv_view_proj((0.0, 1.2, 3.5), (0.0, 0.0, 0.0), 45.0, width / height),
-                     width, height, spp, max_depth, seed, name="spheres256")
+                     inv_view_proj(eye, target, 45.0, width / height),
+                     width, height, spp, max_depth, seed, name="spheres256", cam_eye=eye, cam_target=target,
+                     cam_fovy=45.0, mesh_sizes=tuple(len(p[0]) for p in parts),
+                     mesh_materials=tuple(p[3] for p in parts))
 
 
 def icosphere_scene(width=1920, height=1080, spp=64, max_depth=8, seed=0, subdiv=8,
@@ -283,9 +291,32 @@ def icosphere_scene(width=1920, height=1080, spp=64, max_depth=8, seed=0, subdiv
         region_mat[r] = len(mats) - 1
     parts = [(*ground_quad(0.0, 20.0), 0), (pos, nrm, uv, region_mat[region])]
     P, N, U, M = _concat(parts)
+    eye, target = (0.0, 1.4, 4.2), (0.0, 1.0, 0.0)
     return SceneData(P, N, U, M, np.stack(mats), sun_env(env_w, env_h, seed),
-                     inv_view_proj((0.0, 1.4, 4.2), (0.0, 1.0, 0.0), 40.0, width / height),
-                     width, height, spp, max_depth, seed, name=f"icosphere{subdiv}")
+                     inv_view_proj(eye, target, 40.0, width / height),
+                     width, height, spp, max_depth, seed, name=f"icosphere{subdiv}", cam_eye=eye,
+                     cam_target=target, cam_fovy=40.0)
+
+
+def write_scene_file(sd: SceneData, path: str) -> None:
+    """Write the FFS1 scene file read by the C++ render_demo (one mesh per
+    material run, in triangle order)."""
+    mid = sd.material_ids
+    cuts = np.flatnonzero(np.diff(mid)) + 1
+    starts = np.concatenate([[0], cuts])
+    ends = np.concatenate([cuts, [len(mid)]])
+    with open(path, "wb") as f:
+        f.write(b"FFS1")
+        np.array([len(starts), len(sd.materials), sd.env.shape[1], sd.env.shape[0], sd.width, sd.height],
+                 np.int32).tofile(f)
+        np.array([sd.seed], np.uint32).tofile(f)
+        np.array(list(sd.cam_eye) + list(sd.cam_target) + [sd.cam_fovy], np.float32).tofile(f)
+        np.ascontiguousarray(sd.materials, np.float32).tofile(f)
+        np.ascontiguousarray(sd.env, np.float32).tofile(f)
+        for a, b in zip(starts, ends):
+            np.array([mid[a], b - a], np.int32).tofile(f)
+            for arr in (sd.positions, sd.normals, sd.uvs):
+                np.ascontiguousarray(arr[a:b], np.float32).tofile(f)
 
 
 def random_rays(n: int, seed: int, lo=-1.5, hi=1.5, y_lo=0.0, y_hi=1.5) -> np.ndarray:

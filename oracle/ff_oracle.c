@@ -529,19 +529,89 @@ void or_trace_any_brute(int32_t n_tris, const float *tris, const float *rays, in
 }
 
 /* Counting pass for the roofline: boxes and triangles tested per ray by the
- * traversal the GPU runs (same order; prune=1 adds the GPU's conservative
- * distance pruning). */
+ * GPU traversal, which tests both child boxes of an interior node together
+ * (same near-first order as intersect(); a child is entered only when its box
+ * is hit and not pruned) and the root box first.  Returns the hits too, so
+ * tests can check the pruned traversal against the unpruned one. */
+static hitrec wide_walk(const or_node *nodes, const float *tris, v3 o, v3 d, int any_hit, int prune,
+                        int64_t *cn, int64_t *ct) {
+    hitrec h;
+    h.found = 0; h.t = K_HUGE; h.u = h.v = 0.f; h.id = -1;
+    float of[3] = {o.x, o.y, o.z};
+    float inv[3];
+    if (any_hit) { inv[0] = 1 / d.x; inv[1] = 1 / d.y; inv[2] = 1 / d.z; }
+    else { inv[0] = 1.f / d.x; inv[1] = 1.f / d.y; inv[2] = 1.f / d.z; }
+    int ng[3] = {inv[0] < 0, inv[1] < 0, inv[2] < 0};
+    float tn, tf;
+    (*cn)++;
+    if (!slab(&nodes[0], of, inv, ng, &tn, &tf) || (prune && tf < -PRUNE_BEHIND)) return h;
+    int32_t stack[OR_STACK];
+    int sp = 0;
+    int32_t cur = 0;
+    for (;;) {
+        const or_node *nd = &nodes[cur];
+        if (nd->n_prims == 0) {
+            int32_t a = cur + 1, b = nd->offset;
+            float tn0, tf0, tn1, tf1;
+            float lim = h.t + h.t * PRUNE_REL + PRUNE_ABS;
+            (*cn) += 2;
+            int h0 = slab(&nodes[a], of, inv, ng, &tn0, &tf0);
+            int h1 = slab(&nodes[b], of, inv, ng, &tn1, &tf1);
+            if (prune) {
+                if (h0 && ((!any_hit && tn0 > lim) || tf0 < -PRUNE_BEHIND)) h0 = 0;
+                if (h1 && ((!any_hit && tn1 > lim) || tf1 < -PRUNE_BEHIND)) h1 = 0;
+            }
+            int neg = ng[nd->axis];
+            int32_t nref = neg ? b : a, fref = neg ? a : b;
+            int hn = neg ? h1 : h0, hf = neg ? h0 : h1;
+            if (hn) {
+                if (hf) { if (sp >= OR_STACK) abort(); stack[sp++] = fref; }
+                cur = nref;
+                continue;
+            }
+            if (hf) { cur = fref; continue; }
+        } else {
+            for (int i = 0; i < nd->n_prims; i++) {
+                int32_t slot = nd->offset + i;
+                float t = 0.f, u = 0.f, v = 0.f;
+                (*ct)++;
+                if (any_hit) {
+                    if (mt_shadow(tris + (size_t)slot * 9, o, d, &t) && t < K_HUGE) {
+                        h.found = 1; h.t = t; h.id = slot;
+                        return h;
+                    }
+                } else if (mt_cull(tris + (size_t)slot * 9, o, d, &t, &u, &v) && !(t < 0) && t < h.t) {
+                    h.found = 1; h.t = t; h.u = u; h.v = v; h.id = slot;
+                }
+            }
+        }
+        if (sp == 0) break;
+        cur = stack[--sp];
+    }
+    return h;
+}
+
 void or_count_traversal(const or_node *nodes, const float *tris, const float *rays, int32_t n,
                         int32_t any_hit, int32_t prune, int64_t *nodes_tested,
                         int64_t *tris_tested) {
     int64_t cn = 0, ct = 0;
     for (int32_t i = 0; i < n; i++) {
         const float *r = rays + (size_t)i * 6;
-        if (any_hit) any_ref(nodes, tris, vld(r), vld(r + 3), prune, &cn, &ct);
-        else closest_ref(nodes, tris, vld(r), vld(r + 3), 0, prune, &cn, &ct);
+        wide_walk(nodes, tris, vld(r), vld(r + 3), any_hit, prune, &cn, &ct);
     }
     *nodes_tested = cn;
     *tris_tested = ct;
+}
+
+void or_trace_walk(const or_node *nodes, const float *tris, const float *rays, int32_t n,
+                   int32_t any_hit, int32_t prune, float *t, int32_t *id) {
+    int64_t cn = 0, ct = 0;
+    for (int32_t i = 0; i < n; i++) {
+        const float *r = rays + (size_t)i * 6;
+        hitrec h = wide_walk(nodes, tris, vld(r), vld(r + 3), any_hit, prune, &cn, &ct);
+        t[i] = h.t;
+        id[i] = h.id;
+    }
 }
 
 /* --------------------------------------------------------- environment -- */

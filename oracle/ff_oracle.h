@@ -98,6 +98,8 @@ void or_trace_any_brute(int32_t n_tris, const float *tris, const float *rays,
 void or_count_traversal(const or_node *nodes, const float *tris, const float *rays,
                         int32_t n, int32_t any_hit, int32_t prune,
                         int64_t *nodes_tested, int64_t *tris_tested);
+void or_trace_walk(const or_node *nodes, const float *tris, const float *rays, int32_t n,
+                   int32_t any_hit, int32_t prune, float *t, int32_t *id);
 
 /* ---- environment CDF (FP64) and sampling ---- */
 void or_env_build(const float *rgb, int32_t w, int32_t h, double *marginal_y,

@@ -134,3 +134,23 @@ def test_empty_and_single_ray(cfg0, oracle):
     sc.trace_closest(np.zeros((0, 8), np.float32), np.zeros((0, 4), np.float32))  # n = 0 is a no-op
     r = scenes.random_rays(1, 9)
     _assert_hits_equal(_closest(sc, r), oracle.trace_closest(ps.nodes, ps.tris_ordered(), r), "single")
+
+
+def test_traversal_counters_match_oracle(cfg0, oracle):
+    """Boxes/triangles tested per ray (the roofline's N_node, N_tri) counted
+    on the GPU equal the oracle's count of the same visit scheme."""
+    import torch
+    sd, sc, ps = cfg0
+    r = scenes.random_rays(20000, 12)
+    rd = torch.from_numpy(r).cuda()
+    cnt = torch.zeros(2, dtype=torch.int64, device="cuda")
+    h = torch.empty((len(r), 4), dtype=torch.float32, device="cuda")
+    sc.trace_closest(rd, h, counts_dev=cnt)
+    torch.cuda.synchronize()
+    assert tuple(cnt.cpu().tolist()) == oracle.count_traversal(ps.nodes, ps.tris_ordered(), r, prune=True)
+    cnt.zero_()
+    o = torch.empty(len(r), dtype=torch.uint8, device="cuda")
+    sc.trace_any(rd, o, counts_dev=cnt)
+    torch.cuda.synchronize()
+    assert tuple(cnt.cpu().tolist()) == oracle.count_traversal(ps.nodes, ps.tris_ordered(), r, any_hit=True,
+                                                               prune=True)

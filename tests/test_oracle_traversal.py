@@ -61,6 +61,22 @@ def test_bvh_traversal_equals_brute_force(oracle, kind):
     assert np.array_equal(o, oracle.trace_any_brute(tris, rays))
 
 
+@pytest.mark.parametrize("seed", [3, 4])
+def test_pruned_two_child_walk_equals_reference_order(oracle, seed):
+    """The GPU's visit scheme (both child boxes per interior node, pruning by
+    the current hit) returns exactly the reference traversal's results."""
+    sd = scenes.spheres_scene()
+    nodes, ordered = oracle.bvh_build(sd.positions)
+    tris = oracle.ordered_tris(sd.positions, ordered)
+    rays = scenes.random_rays(20000, seed)
+    t, u, v, i = oracle.trace_closest(nodes, tris, rays)
+    tw, iw = oracle.trace_walk(nodes, tris, rays, prune=True)
+    assert np.array_equal(i, iw) and np.array_equal(t.view(np.uint32), tw.view(np.uint32))
+    occ = oracle.trace_any(nodes, tris, rays)
+    _, ia = oracle.trace_walk(nodes, tris, rays, any_hit=True, prune=True)
+    assert np.array_equal(occ.astype(bool), ia >= 0)
+
+
 def test_pruned_traversal_tests_fewer_boxes(oracle):
     sd = scenes.spheres_scene()
     nodes, ordered = oracle.bvh_build(sd.positions)
