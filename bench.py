@@ -72,6 +72,90 @@ def cpu_baseline(sd, nodes, ordered, budget_s: float, spp: int):
     }
 
 
+def run_batch(args) -> int:
+    """Batched-requests configuration: 64 seeded requests of uneven
+    resolution, spp, depth and triangle count rendered (a) as one batched
+    wavefront replayed from a hipGraph and (b) as 64 plain sequential frames.
+    Prints one JSON line with both throughputs and their ratio."""
+    import torch
+    from concurrent.futures import ThreadPoolExecutor
+    from firefly_amd import abi, scenes
+
+    t0 = time.perf_counter()
+    specs = scenes.batch_request_specs(args.requests, seed=0)
+    envs = [scenes.sun_env(2048, 1024, k) for k in range(4)]
+    ctx = abi.Context(0)
+
+    def make(spec):
+        sd = scenes.batch_request_scene(spec, envs[spec["index"] % 4])
+        return sd, abi.Scene(ctx, sd.positions, sd.normals, sd.uvs, sd.material_ids, sd.materials)
+
+    with ThreadPoolExecutor(8) as ex:
+        built = list(ex.map(make, specs))
+    env_objs = [abi.EnvMap(ctx, e) for e in envs]
+    log(f"{len(built)} requests: {sum(sd.n_tris for sd, _ in built) / 1e6:.1f} M tris, "
+        f"{sum(sd.width * sd.height for sd, _ in built) / 1e6:.1f} M pixels, "
+        f"{sum(sd.width * sd.height * sd.spp for sd, _ in built) / 1e6:.1f} M paths "
+        f"(setup {time.perf_counter() - t0:.1f} s)")
+    cams = [abi.make_camera(sd.inv_vp) for sd, _ in built]
+    f_seq = [torch.zeros((sd.height, sd.width, 4), dtype=torch.float32, device="cuda") for sd, _ in built]
+    f_bat = [torch.zeros_like(f) for f in f_seq]
+    items = [(sc, env_objs[k % 4], cams[k], sd.width, sd.height, sd.spp, sd.max_depth, sd.seed, f_bat[k])
+             for k, (sd, sc) in enumerate(built)]
+
+    def sequential():
+        for k, (sd, sc) in enumerate(built):
+            abi.render(ctx, sc, env_objs[k % 4], cams[k], sd.width, sd.height, sd.spp, sd.max_depth, sd.seed,
+                       f_seq[k], spp_batch=1)
+
+    def batched():
+        abi.render_batch(ctx, items, flags=abi.FF_RENDER_USE_GRAPH)
+
+    times = {}
+    for name, fn, films in (("sequential", sequential, f_seq), ("batched", batched, f_bat)):
+        for i in range(args.warmup + args.steps):
+            for f in films:
+                f.zero_()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t1
+            if i >= args.warmup:
+                times.setdefault(name, []).append(dt)
+            log(f"{name} run {i + 1}: {dt:.3f} s")
+    st = abi.last_render_stats(ctx)
+    same = all(torch.equal(a, b) for a, b in zip(f_seq, f_bat))
+    inc = st.rays_extension + st.rays_shadow
+    tb = min(times["batched"])
+    ts = min(times["sequential"])
+    out = {
+        "metric": METRIC,
+        "value": round(inc / tb / 1e6, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(tb * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"batch{len(built)}_uneven", "requests": len(built),
+                   "pixels": int(sum(sd.width * sd.height for sd, _ in built)),
+                   "paths": int(sum(sd.width * sd.height * sd.spp for sd, _ in built)),
+                   "tris": int(sum(sd.n_tris for sd, _ in built))},
+        "batched_incoherent_mrays_s": round(inc / tb / 1e6, 3),
+        "sequential_incoherent_mrays_s": round(inc / ts / 1e6, 3),
+        "batched_over_sequential": round(ts / tb, 3),
+        "films_identical": bool(same),
+        "rays": {"primary": st.rays_primary, "extension": st.rays_extension, "shadow": st.rays_shadow},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +169,10 @@ def main() -> int:
     ap.add_argument("--spp-batch", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--variant", type=int, default=-1, help="traversal kernel variant (-1: autotune)")
+    ap.add_argument("--workload", choices=("frame", "batch"), default="frame")
+    ap.add_argument("--requests", type=int, default=64)
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run a plain device copy of this many MiB first (PMC calibration)")
     args = ap.parse_args()
@@ -92,6 +180,9 @@ def main() -> int:
         print(f"bench.py measures one GPU only (this build runs on a single MI355X); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         return 2
+
+    if args.workload == "batch":
+        return run_batch(args)
 
     import torch
     from firefly_amd import abi, scenes
@@ -114,6 +205,17 @@ def main() -> int:
         torch.cuda.synchronize()
         log(f"copy probe: {args.copy_mb} MiB read + {args.copy_mb} MiB written")
         del src, dst
+
+    tuned = None
+    if args.variant >= 0:
+        abi.set_traversal_variant(ctx, args.variant)
+    elif not args.no_autotune:
+        from firefly_amd.autotune import autotune
+        t0 = time.perf_counter()
+        tuned = autotune(ctx, scene, env, cam, sd.width, sd.height, args.depth, sd.seed, probe_spp=2,
+                         batches=(1,))
+        args.spp_batch = tuned["best"]["spp_batch"]
+        log(f"autotune ({time.perf_counter() - t0:.1f} s): {tuned['best']}")
 
     def frame(flags=0):
         film.zero_()
@@ -197,6 +299,7 @@ def main() -> int:
         "config": {"workload": workload, "width": sd.width, "height": sd.height, "spp": args.spp,
                    "max_depth": args.depth, "tris": sd.n_tris, "env": f"{sd.env.shape[1]}x{sd.env.shape[0]}",
                    "spp_batch": args.spp_batch},
+        "launch_shape": (tuned["best"] if tuned else {"variant": args.variant}),
         "spp_per_s_1080p": round(spp_s, 3),
         "total_mrays_s": round(total_rays / elapsed / 1e6, 3),
         "rays_per_step": {k: int(tot[k] / args.steps) for k in ("rays_primary", "rays_extension", "rays_shadow")},

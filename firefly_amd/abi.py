@@ -46,6 +46,12 @@ class RenderReq(C.Structure):
                 ("sample_offset", C.c_int32), ("flags", C.c_int32)]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("env", C.c_void_p), ("camera", Camera), ("width", C.c_int32),
+                ("height", C.c_int32), ("spp", C.c_int32), ("max_depth", C.c_int32), ("seed", C.c_uint32),
+                ("sample_offset", C.c_int32), ("film", C.c_void_p)]
+
+
 class RayStats(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "rays_primary", "rays_extension", "rays_shadow", "paths", "ext_nodes", "ext_tris",
@@ -78,6 +84,10 @@ _PROTOS = {
                                    C.c_int32, _P, _P]),
     "ff_render": (C.c_int, [_P, _P, _P, C.POINTER(Camera), C.POINTER(RenderReq), _P, _P]),
     "ff_last_render_stats": (C.c_int, [_P, C.POINTER(RayStats)]),
+    "ff_render_batch": (C.c_int, [_P, C.POINTER(BatchItem), C.c_int32, C.c_int32, _P]),
+    "ff_traversal_variant_count": (C.c_int32, []),
+    "ff_traversal_variant_info": (C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "ff_set_traversal_variant": (C.c_int, [_P, C.c_int32]),
     "ff_film_resolve": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_float, _P, _P, _P]),
 }
 
@@ -259,6 +269,31 @@ def render(ctx: Context, scene: Scene, env: EnvMap, cam: Camera, width: int, hei
            sample_offset: int = 0, flags: int = 0, stream=None) -> None:
     r = RenderReq(width, height, spp, max_depth, seed, spp_batch, sample_offset, flags)
     check(_lib().ff_render(ctx.h, scene.h, env.h, C.byref(cam), C.byref(r), _ptr(film_dev), stream))
+
+
+def render_batch(ctx: Context, items, flags: int = 0, stream=None) -> None:
+    """items: sequence of (scene, env, camera, width, height, spp, max_depth,
+    seed, film_dev[, sample_offset])."""
+    arr = (BatchItem * len(items))()
+    for k, it in enumerate(items):
+        sc, env, cam, w, h, spp, depth, seed, film = it[:9]
+        off = it[9] if len(it) > 9 else 0
+        arr[k] = BatchItem(sc.h, env.h, cam, w, h, spp, depth, seed, off, _ptr(film))
+    check(_lib().ff_render_batch(ctx.h, arr, len(items), flags, stream))
+
+
+def traversal_variants():
+    """[(variant, block, lds_entries)] of the traversal kernels."""
+    out = []
+    for v in range(_lib().ff_traversal_variant_count()):
+        b, l = C.c_int32(), C.c_int32()
+        check(_lib().ff_traversal_variant_info(v, C.byref(b), C.byref(l)))
+        out.append((v, b.value, l.value))
+    return out
+
+
+def set_traversal_variant(ctx: Context, variant: int) -> None:
+    check(_lib().ff_set_traversal_variant(ctx.h, variant))
 
 
 def last_render_stats(ctx: Context) -> RayStats:

@@ -1,0 +1,58 @@
+"""Autotuner for the wavefront launch shapes.
+
+Times short renders of the actual request for every traversal kernel variant
+(workgroup size x LDS stack entries, see ff_traversal_variant_info) and for a
+few paths-in-flight choices (samples per pixel in flight, spp_batch), then
+keeps the fastest on the context.  Results can be written as JSON and reused.
+"""
+from __future__ import annotations
+
+import json
+import time
+from typing import Dict, Iterable, Optional
+
+from . import abi
+
+
+def _time_render(ctx, scene, env, cam, w, h, spp, depth, seed, film, spp_batch, reps):
+    import torch
+    best = float("inf")
+    for _ in range(reps):
+        film.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        abi.render(ctx, scene, env, cam, w, h, spp, depth, seed, film, spp_batch=spp_batch)
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    return best
+
+
+def autotune(ctx: abi.Context, scene: abi.Scene, env: abi.EnvMap, cam: abi.Camera, width: int, height: int,
+             max_depth: int, seed: int = 0, probe_spp: int = 2, batches: Iterable[int] = (1, 2),
+             reps: int = 2, out_json: Optional[str] = None) -> Dict:
+    """Pick the fastest traversal variant and spp_batch for this request.
+
+    Each candidate renders `probe_spp` samples per pixel (timed, best of
+    `reps`).  The chosen variant is set on `ctx`; returns a dict with the
+    choice and every timing."""
+    import torch
+    film = torch.zeros((height, width, 4), dtype=torch.float32, device="cuda")
+    results = []
+    # warm-up (first launches pay code-object load and allocation)
+    abi.render(ctx, scene, env, cam, width, height, 1, max_depth, seed, film)
+    torch.cuda.synchronize()
+    for v, block, lds in abi.traversal_variants():
+        abi.set_traversal_variant(ctx, v)
+        for k in batches:
+            if k > probe_spp:
+                continue
+            t = _time_render(ctx, scene, env, cam, width, height, probe_spp, max_depth, seed, film, k, reps)
+            results.append({"variant": v, "block": block, "lds_entries": lds, "spp_batch": k, "seconds": t})
+    best = min(results, key=lambda r: r["seconds"])
+    abi.set_traversal_variant(ctx, best["variant"])
+    out = {"best": best, "candidates": results,
+           "request": {"width": width, "height": height, "max_depth": max_depth, "probe_spp": probe_spp}}
+    if out_json:
+        with open(out_json, "w") as f:
+            json.dump(out, f, indent=1)
+    return out

@@ -57,7 +57,7 @@ struct PathStore {
     ff_paths p{};
     void release() {
         dfree(p.ray); dfree(p.hit); dfree(p.beta); dfree(p.Lp); dfree(p.fext); dfree(p.nee);
-        dfree(p.sray); dfree(p.vis); dfree(p.rngb); dfree(p.q_ext[0]); dfree(p.q_ext[1]);
+        dfree(p.sray); dfree(p.vis); dfree(p.rngb); dfree(p.preq); dfree(p.q_ext[0]); dfree(p.q_ext[1]);
         for (int m = 0; m < 4; m++) dfree(p.q_mat[m]);
         dfree(p.q_shadow); dfree(p.layers);
         cap = 0;
@@ -82,7 +82,12 @@ struct ff_context {
     int32_t* ovf = nullptr;
     size_t ovf_entries = 0;  // stack entries per grid thread
     hipStream_t last_stream = nullptr;
+    hipStream_t own_stream = nullptr;  // graph capture when the caller passes the null stream
     bool have_stats = false;
+    uint32_t* bstate = nullptr;
+    ff_dev_request* d_reqs = nullptr;
+    size_t req_cap = 0;
+    int trav_variant = 0;
 };
 
 struct ff_scene {
@@ -143,6 +148,7 @@ ff_status ensure_paths(ff_context* c, size_t P) {
     HIP_TRY(dalloc(&p.sray, 6 * P));
     HIP_TRY(dalloc(&p.vis, 3 * P));
     HIP_TRY(dalloc(&p.rngb, P));
+    HIP_TRY(dalloc(&p.preq, P));
     HIP_TRY(dalloc(&p.q_ext[0], P));
     HIP_TRY(dalloc(&p.q_ext[1], P));
     for (int m = 0; m < 4; m++) HIP_TRY(dalloc(&p.q_mat[m], P));
@@ -152,12 +158,12 @@ ff_status ensure_paths(ff_context* c, size_t P) {
     return FF_OK;
 }
 
-// traversal stack overflow area: entries beyond FF_LDS_STACK per grid thread
+// traversal stack overflow area: entries beyond the smallest LDS stack per grid thread
 ff_status ensure_ovf(ff_context* c, int32_t max_depth) {
-    size_t need = max_depth > FF_LDS_STACK ? (size_t)(max_depth - FF_LDS_STACK + 1) : 1;
+    size_t need = max_depth > FF_LDS_MIN ? (size_t)(max_depth - FF_LDS_MIN + 1) : 1;
     if (c->ovf && c->ovf_entries >= need) return FF_OK;
     dfree(c->ovf);
-    HIP_TRY(dalloc(&c->ovf, need * (size_t)FF_TRAV_GRID * FF_BLOCK));
+    HIP_TRY(dalloc(&c->ovf, need * (size_t)FF_TRAV_THREADS));
     c->ovf_entries = need;
     return FF_OK;
 }
@@ -225,9 +231,11 @@ ff_status ff_context_create(int32_t device, ff_context** out) {
     HIP_TRY(hipSetDevice(device));
     ff_context* c = new ff_context();
     c->device = device;
-    if (dalloc(&c->counters, C_COUNT) != hipSuccess || dalloc(&c->stats, S_COUNT) != hipSuccess) {
+    if (dalloc(&c->counters, C_COUNT) != hipSuccess || dalloc(&c->stats, S_COUNT) != hipSuccess ||
+        dalloc(&c->bstate, B_COUNT) != hipSuccess) {
         dfree(c->counters);
         dfree(c->stats);
+        dfree(c->bstate);
         delete c;
         return fail(FF_ERR_ALLOC, "ff_context_create: counter allocation failed");
     }
@@ -241,6 +249,9 @@ ff_status ff_context_destroy(ff_context* c) {
     (void)hipDeviceSynchronize();
     c->paths.release();
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    dfree(c->bstate);
+    dfree(c->d_reqs);
     dfree(c->counters);
     dfree(c->stats);
     dfree(c->ovf);
@@ -353,8 +364,8 @@ ff_status ff_scene_create(ff_context* ctx, const ff_scene_desc* d, ff_scene** ou
     if (dalloc(&s->d_tex_desc, descs.size()) != hipSuccess ||
         hipMemcpy(s->d_tex_desc, descs.data(), descs.size() * sizeof(ff_tex_desc), hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(FF_ERR_ALLOC, "ff_scene_create: texture table"));
-    ff_status st = ensure_ovf(ctx, s->max_depth);
-    if (st != FF_OK) return cleanup(st);
+    // the context is only read here (its device), so scenes may be created
+    // concurrently from several host threads; stack space is sized per trace
     *out = s;
     return FF_OK;
 }
@@ -446,6 +457,8 @@ ff_status ff_trace_closest(ff_scene* s, const float* rays, uint32_t n, float* hi
     if (!s || (!rays && n) || (!hits && n)) return fail(FF_ERR_INVALID, "ff_trace_closest: NULL argument");
     if (n == 0) return FF_OK;
     HIP_TRY(hipSetDevice(s->ctx->device));
+    ff_status st = ensure_ovf(s->ctx, s->max_depth);
+    if (st != FF_OK) return st;
     HIP_TRY(ffk::launch_trace_closest(s->view(), (const float4*)rays, n, (float4*)hits, two_sided, s->ctx->ovf,
                                       (unsigned long long*)counts, (hipStream_t)stream));
     return FF_OK;
@@ -456,6 +469,8 @@ ff_status ff_trace_any(ff_scene* s, const float* rays, uint32_t n, uint8_t* occl
     if (!s || (!rays && n) || (!occluded && n)) return fail(FF_ERR_INVALID, "ff_trace_any: NULL argument");
     if (n == 0) return FF_OK;
     HIP_TRY(hipSetDevice(s->ctx->device));
+    ff_status st = ensure_ovf(s->ctx, s->max_depth);
+    if (st != FF_OK) return st;
     HIP_TRY(ffk::launch_trace_any(s->view(), (const float4*)rays, n, occluded, s->ctx->ovf,
                                   (unsigned long long*)counts, (hipStream_t)stream));
     return FF_OK;
@@ -471,32 +486,91 @@ ff_status ff_generate_rays(ff_context* ctx, const ff_camera* cam, int32_t width,
     return FF_OK;
 }
 
-ff_status ff_render(ff_context* ctx, ff_scene* s, ff_envmap* env, const ff_camera* cam, const ff_render_req* r,
-                    float* film, void* stream) {
-    if (!ctx || !s || !env || !cam || !r || !film) return fail(FF_ERR_INVALID, "ff_render: NULL argument");
-    if (s->ctx != ctx) return fail(FF_ERR_INVALID, "ff_render: scene belongs to another context");
-    if (r->width <= 0 || r->height <= 0 || r->spp <= 0 || r->max_depth < 1 || r->max_depth > 250 ||
-        r->sample_offset < 0)
-        return fail(FF_ERR_INVALID, "ff_render: invalid request (%dx%d, spp %d, depth %d)", r->width, r->height,
-                    r->spp, r->max_depth);
-    uint64_t WH = (uint64_t)r->width * (uint64_t)r->height;
-    int32_t K = r->spp_batch > 0 ? std::min(r->spp_batch, r->spp) : 1;
-    uint64_t P = WH * (uint64_t)K;
-    if (P > (1ull << 30)) return fail(FF_ERR_CAPACITY, "ff_render: %llu paths in flight exceed 2^30", (unsigned long long)P);
+}  // extern "C"
+
+namespace {
+
+struct HostReq {
+    ff_scene* s;
+    ff_envmap* e;
+    ff_camera cam;
+    int32_t W, H, spp, depth, K, sample_offset;
+    uint32_t seed;
+    float* film;
+};
+
+ff_status check_req(const char* who, ff_context* ctx, const HostReq& q) {
+    if (!q.s || !q.e || !q.film) return fail(FF_ERR_INVALID, "%s: NULL scene, environment or film", who);
+    if (q.s->ctx != ctx) return fail(FF_ERR_INVALID, "%s: scene belongs to another context", who);
+    if (q.W <= 0 || q.H <= 0 || q.spp <= 0 || q.depth < 1 || q.depth > 250 || q.sample_offset < 0 || q.K < 1)
+        return fail(FF_ERR_INVALID, "%s: invalid request (%dx%d, spp %d, depth %d)", who, q.W, q.H, q.spp, q.depth);
+    return FF_OK;
+}
+
+// The wavefront over a table of requests (one frame, or many uneven ones).
+// Per batch b every request renders samples [b*K, b*K + K) of each pixel;
+// the batch's bounce loop runs max depth + 1 iterations of
+// extend -> shadow -> logic -> shade, all stage kernels reading their work
+// counts from device counters (no host synchronisation inside a render).
+ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int32_t flags, hipStream_t hs,
+                          const char* who) {
+    uint64_t n_slots = 0;
+    int32_t max_tree = 0, max_depth = 0;
+    int64_t n_batches = 0;
+    bool has_type[4] = {false, false, false, false};
+    for (const HostReq& q : reqs) {
+        n_slots += (uint64_t)q.W * (uint64_t)q.H * (uint64_t)q.K;
+        max_tree = std::max(max_tree, q.s->max_depth);
+        max_depth = std::max(max_depth, q.depth);
+        n_batches = std::max<int64_t>(n_batches, (q.spp + q.K - 1) / q.K);
+        for (int m = 0; m < 4; m++) has_type[m] = has_type[m] || q.s->has_type[m];
+    }
+    if (n_slots > (1ull << 30))
+        return fail(FF_ERR_CAPACITY, "%s: %llu paths in flight exceed 2^30", who, (unsigned long long)n_slots);
     HIP_TRY(hipSetDevice(ctx->device));
-    ff_status st = ensure_paths(ctx, (size_t)P);
+    ff_status st = ensure_paths(ctx, (size_t)n_slots);
     if (st != FF_OK) return st;
-    st = ensure_ovf(ctx, s->max_depth);
+    st = ensure_ovf(ctx, max_tree);
     if (st != FF_OK) return st;
-    hipStream_t hs = (hipStream_t)stream;
+    const bool timing = (flags & FF_RENDER_TIMING) != 0;
+    const bool graph = (flags & FF_RENDER_USE_GRAPH) != 0 && !timing;
+    if (graph && !hs) {
+        if (!ctx->own_stream) HIP_TRY(hipStreamCreate(&ctx->own_stream));
+        hs = ctx->own_stream;
+    }
     ff_paths p = ctx->paths.p;
+    // request table (uploaded once the stream's earlier work is done with it)
+    std::vector<ff_dev_request> table(reqs.size());
+    uint64_t base = 0;
+    for (size_t r = 0; r < reqs.size(); r++) {
+        const HostReq& q = reqs[r];
+        ff_dev_request& d = table[r];
+        d.S = q.s->view();
+        d.E = q.e->view();
+        d.C = dev_camera(&q.cam);
+        d.film = (float4*)q.film;
+        d.layers = p.layers + base;
+        d.slot_base = (uint32_t)base;
+        d.W = (uint32_t)q.W; d.H = (uint32_t)q.H; d.spp = (uint32_t)q.spp; d.depth = (uint32_t)q.depth;
+        d.K = (uint32_t)q.K; d.seed = q.seed; d.sample_offset = (uint32_t)q.sample_offset;
+        base += (uint64_t)q.W * q.H * q.K;
+    }
+    if (ctx->req_cap < table.size()) {
+        dfree(ctx->d_reqs);
+        HIP_TRY(dalloc(&ctx->d_reqs, table.size()));
+        ctx->req_cap = table.size();
+    }
+    HIP_TRY(hipStreamSynchronize(hs));
+    HIP_TRY(hipMemcpy(ctx->d_reqs, table.data(), table.size() * sizeof(ff_dev_request), hipMemcpyHostToDevice));
     p.counters = ctx->counters;
     p.stats = ctx->stats;
     p.ovf = ctx->ovf;
-    ff_dev_scene S = s->view();
-    ff_dev_env E = env->view();
-    ff_dev_camera C = dev_camera(cam);
-    const bool timing = (r->flags & FF_RENDER_TIMING) != 0;
+    p.bstate = ctx->bstate;
+    p.reqs = ctx->d_reqs;
+    p.n_req = (uint32_t)table.size();
+    p.n_slots = (uint32_t)n_slots;
+    const int var = ctx->trav_variant;
+
     ctx->records.clear();
     ctx->events_used = 0;
     // Records a begin/end event pair around one stage launch when timing.
@@ -516,39 +590,116 @@ ff_status ff_render(ff_context* ctx, ff_scene* s, ff_envmap* env, const ff_camer
         ctx->records.push_back(rec);
         return e;
     };
-    HIP_TRY(ffk::launch_fill4(p.layers, (uint32_t)P, hs));
+    for (size_t r = 0; r < reqs.size(); r++)
+        if (reqs[r].K > 1)
+            HIP_TRY(ffk::launch_fill4(table[r].layers, (uint32_t)((uint64_t)reqs[r].W * reqs[r].H * reqs[r].K), hs));
     HIP_TRY(ffk::launch_fill_u64(p.stats, S_COUNT, hs));
-    const uint32_t all = (1u << C_COUNT) - 1u;
-    for (int32_t b = 0; (int64_t)b * K < r->spp; b++) {
-        int32_t k_act = std::min<int32_t>(K, r->spp - b * K);
-        uint32_t n_act = (uint32_t)(WH * (uint64_t)k_act);
-        HIP_TRY(ffk::launch_reset(p.counters, all, hs));
-        HIP_TRY(timed(4, [&] {
-            return ffk::launch_generate(C, r->width, r->height, n_act, (uint32_t)(r->sample_offset + b * K),
-                                        r->seed, p, 0, hs);
-        }));
+    HIP_TRY(ffk::launch_batch_begin(p, 1, 0xffffffffu, hs));  // first batch_begin makes it 0
+
+    // one batch: identical launches for every batch (the index is on the device)
+    auto batch = [&]() -> ff_status {
+        HIP_TRY(ffk::launch_batch_begin(p, 0, 0, hs));
+        HIP_TRY(timed(4, [&] { return ffk::launch_generate(p, hs); }));
         int cur = 0;
-        for (int32_t it = 1; it <= r->max_depth + 1; it++) {
-            HIP_TRY(timed(0, [&] { return ffk::launch_extend(S, p, cur, hs); }));
-            HIP_TRY(timed(1, [&] { return ffk::launch_shadow(S, p, hs); }));
+        for (int32_t it = 1; it <= max_depth + 1; it++) {
+            HIP_TRY(timed(0, [&] { return ffk::launch_extend(p, cur, var, hs); }));
+            HIP_TRY(timed(1, [&] { return ffk::launch_shadow(p, var, hs); }));
             uint32_t mask = (1u << (cur == 0 ? C_EXT1 : C_EXT0)) | (1u << C_SHADOW) | (0xfu << C_MAT0);
             HIP_TRY(ffk::launch_reset(p.counters, mask, hs));
-            HIP_TRY(timed(2, [&] { return ffk::launch_logic(S, E, p, cur, (uint32_t)r->max_depth, n_act, hs); }));
-            if (it <= r->max_depth) {
+            HIP_TRY(timed(2, [&] { return ffk::launch_logic(p, cur, p.n_slots, hs); }));
+            if (it <= max_depth) {
                 for (int m = 0; m < 4; m++)
-                    if (s->has_type[m])
-                        HIP_TRY(timed(3, [&] { return ffk::launch_shade(m, S, E, p, 1 - cur, n_act, hs); }));
+                    if (has_type[m]) HIP_TRY(timed(3, [&] { return ffk::launch_shade(m, p, 1 - cur, p.n_slots, hs); }));
             }
             cur = 1 - cur;
         }
+        return FF_OK;
+    };
+    if (graph) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        HIP_TRY(hipStreamBeginCapture(hs, hipStreamCaptureModeThreadLocal));
+        ff_status bst = batch();
+        hipError_t ce = hipStreamEndCapture(hs, &g);
+        if (bst != FF_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return bst;
+        }
+        HIP_TRY(ce);
+        hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            HIP_TRY(ie);
+        }
+        hipError_t le = hipSuccess;
+        for (int64_t b = 0; b < n_batches && le == hipSuccess; b++) le = hipGraphLaunch(ge, hs);
+        (void)hipGraphExecDestroy(ge);
+        (void)hipGraphDestroy(g);
+        HIP_TRY(le);
+    } else {
+        for (int64_t b = 0; b < n_batches; b++) {
+            ff_status bst = batch();
+            if (bst != FF_OK) return bst;
+        }
     }
-    HIP_TRY(timed(4, [&] {
-        return ffk::launch_resolve_layers(p.layers, (uint32_t)K, (uint32_t)WH, (float4*)film, (float)r->spp, hs);
-    }));
+    for (size_t r = 0; r < reqs.size(); r++)
+        if (reqs[r].K > 1)
+            HIP_TRY(timed(4, [&] {
+                return ffk::launch_resolve_layers(table[r].layers, (uint32_t)reqs[r].K,
+                                                  (uint32_t)((uint64_t)reqs[r].W * reqs[r].H), (float4*)reqs[r].film, hs);
+            }));
     ctx->last_stream = hs;
     ctx->have_stats = true;
     return FF_OK;
 }
+
+}  // namespace
+
+extern "C" {
+
+ff_status ff_render(ff_context* ctx, ff_scene* s, ff_envmap* env, const ff_camera* cam, const ff_render_req* r,
+                    float* film, void* stream) {
+    if (!ctx || !s || !env || !cam || !r || !film) return fail(FF_ERR_INVALID, "ff_render: NULL argument");
+    HostReq q{s, env, *cam, r->width, r->height, r->spp, r->max_depth,
+              r->spp_batch > 0 ? std::min(r->spp_batch, std::max(r->spp, 1)) : 1, r->sample_offset, r->seed, film};
+    ff_status st = check_req("ff_render", ctx, q);
+    if (st != FF_OK) return st;
+    return render_requests(ctx, {q}, r->flags, (hipStream_t)stream, "ff_render");
+}
+
+ff_status ff_render_batch(ff_context* ctx, const ff_batch_item* items, int32_t n, int32_t flags, void* stream) {
+    if (!ctx || !items || n <= 0) return fail(FF_ERR_INVALID, "ff_render_batch: NULL argument or empty batch");
+    std::vector<HostReq> reqs;
+    reqs.reserve((size_t)n);
+    for (int32_t i = 0; i < n; i++) {
+        const ff_batch_item& it = items[i];
+        HostReq q{it.scene, it.env, it.camera, it.width, it.height, it.spp, it.max_depth, 1, it.sample_offset,
+                  it.seed, it.film};
+        ff_status st = check_req("ff_render_batch", ctx, q);
+        if (st != FF_OK) return st;
+        for (int32_t j = 0; j < i; j++)
+            if (items[j].film == it.film) return fail(FF_ERR_INVALID, "ff_render_batch: items %d and %d share a film", j, i);
+        reqs.push_back(q);
+    }
+    return render_requests(ctx, reqs, flags, (hipStream_t)stream, "ff_render_batch");
+}
+
+ff_status ff_set_traversal_variant(ff_context* ctx, int32_t variant) {
+    if (!ctx || variant < 0 || variant >= ffk::kTravVariants)
+        return fail(FF_ERR_INVALID, "ff_set_traversal_variant: variant %d of %d", variant, ffk::kTravVariants);
+    ctx->trav_variant = variant;
+    return FF_OK;
+}
+
+ff_status ff_traversal_variant_info(int32_t variant, int32_t* block, int32_t* lds_entries) {
+    if (variant < 0 || variant >= ffk::kTravVariants)
+        return fail(FF_ERR_INVALID, "ff_traversal_variant_info: variant %d of %d", variant, ffk::kTravVariants);
+    if (block) *block = ffk::kTravBlock[variant];
+    if (lds_entries) *lds_entries = ffk::kTravLds[variant];
+    return FF_OK;
+}
+
+int32_t ff_traversal_variant_count(void) { return ffk::kTravVariants; }
 
 ff_status ff_last_render_stats(ff_context* ctx, ff_ray_stats* out) {
     if (!ctx || !out) return fail(FF_ERR_INVALID, "ff_last_render_stats: NULL argument");

@@ -386,22 +386,23 @@ __device__ __forceinline__ bool mt_shadow(const Tri& T, f3 o, f3 d, float& t) {
     return true;
 }
 
-// Traversal stack: the first FF_LDS_STACK entries of each lane live in LDS
+// Traversal stack: the first `lds_entries` entries of each lane live in LDS
 // ([depth][lane], conflict-free), deeper entries in a per-lane global
-// overflow area ([depth - FF_LDS_STACK][grid thread]).
+// overflow area ([depth - lds_entries][grid thread]).
 struct Stack {
     int* lds;          // &smem[threadIdx.x]
     int* ovf;          // &overflow[grid thread]
     uint32_t lds_stride, ovf_stride;
+    int lds_entries;
     int sp;
     __device__ __forceinline__ void push(int v) {
-        if (sp < FF_LDS_STACK) lds[sp * lds_stride] = v;
-        else ovf[(size_t)(sp - FF_LDS_STACK) * ovf_stride] = v;
+        if (sp < lds_entries) lds[sp * lds_stride] = v;
+        else ovf[(size_t)(sp - lds_entries) * ovf_stride] = v;
         sp++;
     }
     __device__ __forceinline__ int pop() {
         sp--;
-        return sp < FF_LDS_STACK ? lds[sp * lds_stride] : ovf[(size_t)(sp - FF_LDS_STACK) * ovf_stride];
+        return sp < lds_entries ? lds[sp * lds_stride] : ovf[(size_t)(sp - lds_entries) * ovf_stride];
     }
 };
 

@@ -10,20 +10,19 @@
 #include "ff_layout.h"
 
 namespace ffk {
-hipError_t launch_generate(const ff_dev_camera& cam, int W, int H, uint32_t n_active,
-                           uint32_t sample_base, uint32_t seed, const ff_paths& P, int out_q,
-                           hipStream_t s);
-hipError_t launch_extend(const ff_dev_scene& S, const ff_paths& P, int q, hipStream_t s);
-hipError_t launch_shadow(const ff_dev_scene& S, const ff_paths& P, hipStream_t s);
+extern const int kTravVariants;
+extern const int kTravLds[];
+extern const int kTravBlock[];
+hipError_t launch_batch_begin(const ff_paths& P, int set, uint32_t value, hipStream_t s);
+hipError_t launch_generate(const ff_paths& P, hipStream_t s);
+hipError_t launch_extend(const ff_paths& P, int q, int variant, hipStream_t s);
+hipError_t launch_shadow(const ff_paths& P, int variant, hipStream_t s);
 hipError_t launch_reset(uint32_t* counters, uint32_t mask, hipStream_t s);
-hipError_t launch_logic(const ff_dev_scene& S, const ff_dev_env& E, const ff_paths& P, int q,
-                        uint32_t max_depth, uint32_t n_max, hipStream_t s);
-hipError_t launch_shade(int type, const ff_dev_scene& S, const ff_dev_env& E, const ff_paths& P,
-                        int next_q, uint32_t n_max, hipStream_t s);
+hipError_t launch_logic(const ff_paths& P, int q, uint32_t n_max, hipStream_t s);
+hipError_t launch_shade(int type, const ff_paths& P, int next_q, uint32_t n_max, hipStream_t s);
 hipError_t launch_fill4(float4* p, uint32_t n, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, uint32_t n, hipStream_t s);
-hipError_t launch_resolve_layers(const float4* layers, uint32_t K, uint32_t WH, float4* film,
-                                 float spp, hipStream_t s);
+hipError_t launch_resolve_layers(const float4* layers, uint32_t K, uint32_t WH, float4* film, hipStream_t s);
 hipError_t launch_trace_closest(const ff_dev_scene& S, const float4* rays, uint32_t n,
                                 float4* hits, int two_sided, int32_t* ovf,
                                 unsigned long long* counts, hipStream_t s);

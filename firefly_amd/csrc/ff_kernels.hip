@@ -2,6 +2,12 @@
 // host launchers.  Stage structure follows the reference's wavefront
 // (CRT/kernel.cu:283-676): generate, extend, shadow, logic, material queues,
 // plus the FP64 environment CDF build and the film resolve.
+//
+// Every stage kernel serves a table of render requests: a path slot knows its
+// request (scene, environment, camera, depth, film) through P.preq, so one
+// wavefront renders a single frame or many uneven requests at once.  The
+// batch index lives in device memory (P.bstate) so the per-sample sequence of
+// launches is identical from batch to batch and can be replayed as a graph.
 #include <hip/hip_runtime.h>
 
 #include "ff_device.h"
@@ -28,43 +34,75 @@ __device__ __forceinline__ EnvView env_view(const ff_dev_env& e) {
     return v;
 }
 
-// ================================================================ generate
-// wf_generate + dCamera::gen_ray (CRT/kernel.cu:457-485): every path slot of
-// the batch starts a new sample; slot id = layer * WH + pixel.
-__global__ __launch_bounds__(FF_BLOCK) void k_generate(ff_dev_camera cam, int32_t W, int32_t H,
-                                                       uint32_t n_active, uint32_t sample_base,
-                                                       uint32_t seed, ff_paths P, int out_q) {
-    uint32_t WH = (uint32_t)W * (uint32_t)H;
-    for (uint32_t id = gtid(); id < n_active; id += gstride()) {
-        uint32_t pix = id % WH, layer = id / WH;
-        int px = (int)(pix % (uint32_t)W), py = (int)(pix / (uint32_t)W);
-        uint32_t base = rng_base(seed, pix, sample_base + layer);
-        f3 o, d;
-        gen_ray(cam, W, H, px, py, base, o, d);
-        P.ray[2 * id] = f4(o, 0.f);
-        P.ray[2 * id + 1] = f4(d, 0.f);
-        P.rngb[id] = base;
-        P.beta[id] = make_float4(1.f, 1.f, 1.f, __uint_as_float(ST_PREV_DELTA));
-        P.Lp[id] = make_float4(0.f, 0.f, 0.f, 0.f);
-        P.q_ext[out_q][id] = id;
+// request owning a slot: last r with slot_base <= id
+__device__ __forceinline__ uint32_t find_req(const ff_dev_request* __restrict__ reqs, uint32_t n, uint32_t id) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (reqs[mid].slot_base <= id) lo = mid;
+        else hi = mid;
     }
-    if (gtid() == 0) P.counters[out_q == 0 ? C_EXT0 : C_EXT1] = n_active;
+    return lo;
+}
+
+// ============================================================ batch state
+// Starts a batch: batch index += 1 (or = value), all queue counters cleared.
+__global__ void k_batch_begin(uint32_t* bstate, uint32_t* counters, int set, uint32_t value) {
+    uint32_t i = threadIdx.x;
+    if (i < C_COUNT) counters[i] = 0;
+    if (i == 0) bstate[B_BATCH] = set ? value : bstate[B_BATCH] + 1u;
+}
+
+// ================================================================ generate
+// wf_generate + dCamera::gen_ray (CRT/kernel.cu:457-485): every active slot
+// of the batch starts its next sample and joins the extension queue.
+__global__ __launch_bounds__(FF_BLOCK) void k_generate(ff_paths P) {
+    const uint32_t b = P.bstate[B_BATCH];
+    const uint32_t n_round = (P.n_slots + 63u) & ~63u;
+    for (uint32_t id = gtid(); id < n_round; id += gstride()) {
+        bool act = false;
+        if (id < P.n_slots) {
+            uint32_t r = find_req(P.reqs, P.n_req, id);
+            const ff_dev_request& R = P.reqs[r];
+            uint32_t WH = R.W * R.H, local = id - R.slot_base;
+            uint32_t layer = local / WH, pix = local % WH;
+            int64_t rem = (int64_t)R.spp - (int64_t)b * R.K;
+            act = rem > 0 && (int64_t)layer < rem && layer < R.K;
+            if (act) {
+                uint32_t sample = R.sample_offset + b * R.K + layer;
+                uint32_t base = rng_base(R.seed, pix, sample);
+                f3 o, d;
+                gen_ray(R.C, (int)R.W, (int)R.H, (int)(pix % R.W), (int)(pix / R.W), base, o, d);
+                P.ray[2 * id] = f4(o, 0.f);
+                P.ray[2 * id + 1] = f4(d, 0.f);
+                P.rngb[id] = base;
+                P.preq[id] = r;
+                P.beta[id] = make_float4(1.f, 1.f, 1.f, __uint_as_float(ST_PREV_DELTA));
+                P.Lp[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+        uint32_t slot = wave_append(&P.counters[C_EXT0], act);
+        if (act) P.q_ext[0][slot] = id;
+    }
 }
 
 // ================================================================== extend
 // wf_extend (CRT/kernel.cu:487-523): closest hit of every queued path.
-__global__ __launch_bounds__(FF_BLOCK) void k_extend(ff_dev_scene S, ff_paths P, int q) {
-    __shared__ int smem[FF_LDS_STACK * FF_BLOCK];
+template <int BS, int LDS>
+__global__ __launch_bounds__(BS) void k_extend(ff_paths P, int q) {
+    __shared__ int smem[LDS * BS];
     Stack st;
     st.lds = &smem[threadIdx.x];
-    st.lds_stride = FF_BLOCK;
+    st.lds_stride = BS;
+    st.lds_entries = LDS;
     st.ovf = P.ovf + gtid();
-    st.ovf_stride = FF_TRAV_GRID * FF_BLOCK;
+    st.ovf_stride = FF_TRAV_THREADS;
     const uint32_t n = P.counters[q == 0 ? C_EXT0 : C_EXT1];
     const uint32_t* queue = P.q_ext[q];
     uint32_t cn = 0, ct = 0;
     for (uint32_t i = gtid(); i < n; i += gstride()) {
         uint32_t id = queue[i];
+        const ff_dev_scene& S = P.reqs[P.preq[id]].S;
         float4 r0 = P.ray[2 * id], r1 = P.ray[2 * id + 1];
         uint32_t stw = __float_as_uint(P.beta[id].w);
         Hit h = closest_hit(bvh_of(S), S.tris, xyz(r0), xyz(r1), (stw & ST_INSIDE) != 0, st, cn, ct);
@@ -76,17 +114,20 @@ __global__ __launch_bounds__(FF_BLOCK) void k_extend(ff_dev_scene S, ff_paths P,
 
 // ================================================================== shadow
 // wf_shadow (CRT/kernel.cu:525-543): any-hit of every queued shadow ray.
-__global__ __launch_bounds__(FF_BLOCK) void k_shadow(ff_dev_scene S, ff_paths P) {
-    __shared__ int smem[FF_LDS_STACK * FF_BLOCK];
+template <int BS, int LDS>
+__global__ __launch_bounds__(BS) void k_shadow(ff_paths P) {
+    __shared__ int smem[LDS * BS];
     Stack st;
     st.lds = &smem[threadIdx.x];
-    st.lds_stride = FF_BLOCK;
+    st.lds_stride = BS;
+    st.lds_entries = LDS;
     st.ovf = P.ovf + gtid();
-    st.ovf_stride = FF_TRAV_GRID * FF_BLOCK;
+    st.ovf_stride = FF_TRAV_THREADS;
     const uint32_t n = P.counters[C_SHADOW];
     uint32_t cn = 0, ct = 0;
     for (uint32_t i = gtid(); i < n; i += gstride()) {
         uint32_t s = P.q_shadow[i];
+        const ff_dev_scene& S = P.reqs[P.preq[s / 3]].S;
         float4 r0 = P.sray[2 * s], r1 = P.sray[2 * s + 1];
         bool occ = any_hit(bvh_of(S), S.tris, xyz(r0), xyz(r1), st, cn, ct);
         P.vis[s] = occ ? 0 : 1;
@@ -105,10 +146,8 @@ __global__ void k_reset_counters(uint32_t* c, uint32_t mask) {
 // wf_logic (CRT/kernel.cu:297-455): NEE accumulation gated by the light
 // ray's visibility, throughput update, Russian roulette for len > 3,
 // environment radiance for escaped rays after the camera (or a delta event),
-// termination into the film layer, routing of live paths to material queues.
-__global__ __launch_bounds__(FF_BLOCK) void k_logic(ff_dev_scene S, ff_dev_env Ed, ff_paths P, int q,
-                                                    uint32_t max_depth) {
-    EnvView E = env_view(Ed);
+// termination into the film, routing of live paths to material queues.
+__global__ __launch_bounds__(FF_BLOCK) void k_logic(ff_paths P, int q) {
     const uint32_t n = P.counters[q == 0 ? C_EXT0 : C_EXT1];
     const uint32_t* queue = P.q_ext[q];
     const uint32_t n_round = (n + 63u) & ~63u;
@@ -118,6 +157,8 @@ __global__ __launch_bounds__(FF_BLOCK) void k_logic(ff_dev_scene S, ff_dev_env E
         bool term = false, primary = false;
         uint32_t id = live ? queue[i] : 0u;
         if (live) {
+            const ff_dev_request& R = P.reqs[P.preq[id]];
+            const ff_dev_scene& S = R.S;
             float4 bw = P.beta[id];
             uint32_t stw = __float_as_uint(bw.w);
             uint32_t len = (stw & ST_LEN_MASK) + 1;
@@ -127,7 +168,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_logic(ff_dev_scene S, ff_dev_env E
             float4 hh = P.hit[id];
             int slot = __float_as_int(hh.w);
             bool found = slot >= 0;
-            term = (len > max_depth) || !found;
+            term = (len > R.depth) || !found;
             int mtype = -1;
             f3 d = xyz(P.ray[2 * id + 1]);
             if (found) {
@@ -137,7 +178,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_logic(ff_dev_scene S, ff_dev_env E
                     if (!(vdot(d, vcross(T.e1, T.e2)) < 0.f)) term = true;  // absorbed
                 }
             }
-            if (!found && (stw & ST_PREV_DELTA)) Lp = vadd(Lp, vmul(beta, env_L(E, d)));
+            if (!found && (stw & ST_PREV_DELTA)) Lp = vadd(Lp, vmul(beta, env_L(env_view(R.E), d)));
             if (len > 1) {
                 bool visL = (stw & ST_NEED_L) && P.vis[3 * id + 0];
                 if (visL) {
@@ -160,8 +201,10 @@ __global__ __launch_bounds__(FF_BLOCK) void k_logic(ff_dev_scene S, ff_dev_env E
                 }
             }
             if (term) {
-                float4 L = P.layers[id];
-                P.layers[id] = make_float4(L.x + Lp.x, L.y + Lp.y, L.z + Lp.z, L.w + 1.f);
+                uint32_t local = id - R.slot_base;
+                float4* acc = R.K > 1 ? &R.layers[local] : &R.film[local % (R.W * R.H)];
+                float4 L = *acc;
+                *acc = make_float4(L.x + Lp.x, L.y + Lp.y, L.z + Lp.z, L.w + 1.f);
             } else {
                 uint32_t nst = (stw & (ST_INSIDE | ST_PREV_DELTA)) | len;
                 P.beta[id] = f4(beta, __uint_as_float(nst));
@@ -187,9 +230,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_logic(ff_dev_scene S, ff_dev_env E
 // rays of BRDF_L's BRDF-sampled MIS terms (CRT/dMaterial.cu:404-448) are
 // deferred to the shadow stage with their contributions kept per path.
 template <int TYPE>
-__global__ __launch_bounds__(FF_BLOCK) void k_shade(ff_dev_scene S, ff_dev_env Ed, ff_paths P,
-                                                    int next_q) {
-    EnvView E = env_view(Ed);
+__global__ __launch_bounds__(FF_BLOCK) void k_shade(ff_paths P, int next_q) {
     const uint32_t n = P.counters[C_MAT0 + TYPE];
     const uint32_t* queue = P.q_mat[TYPE];
     const uint32_t n_round = (n + 63u) & ~63u;
@@ -199,6 +240,9 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade(ff_dev_scene S, ff_dev_env E
         uint32_t id = live ? queue[i] : 0u;
         bool needL = false, needS = false, needD = false;
         if (live) {
+            const ff_dev_request& R = P.reqs[P.preq[id]];
+            const ff_dev_scene& S = R.S;
+            EnvView E = env_view(R.E);
             float4 bw = P.beta[id];
             uint32_t stw = __float_as_uint(bw.w);
             uint32_t len = stw & ST_LEN_MASK;
@@ -388,29 +432,30 @@ __global__ void k_fill_u64(unsigned long long* p, uint32_t n) {
     if (i < n) p[i] = 0ull;
 }
 // film[pixel] += sum over layers k (in k order) of layers[k*WH + pixel]
-__global__ void k_resolve_layers(const float4* layers, uint32_t K, uint32_t WH, float4* film,
-                                 float spp) {
+__global__ void k_resolve_layers(const float4* layers, uint32_t K, uint32_t WH, float4* film) {
     for (uint32_t p = gtid(); p < WH; p += gstride()) {
         float4 s = layers[p];
         for (uint32_t k = 1; k < K; k++) {
             float4 l = layers[(size_t)k * WH + p];
-            s.x = s.x + l.x; s.y = s.y + l.y; s.z = s.z + l.z;
+            s.x = s.x + l.x; s.y = s.y + l.y; s.z = s.z + l.z; s.w = s.w + l.w;
         }
         float4 f = film[p];
-        film[p] = make_float4(f.x + s.x, f.y + s.y, f.z + s.z, f.w + spp);
+        film[p] = make_float4(f.x + s.x, f.y + s.y, f.z + s.z, f.w + s.w);
     }
 }
 
 // ======================================================== standalone stages
-__global__ __launch_bounds__(FF_BLOCK) void k_trace_closest(ff_dev_scene S, const float4* rays,
-                                                            uint32_t n, float4* hits, int two_sided,
-                                                            int32_t* ovf, unsigned long long* counts) {
-    __shared__ int smem[FF_LDS_STACK * FF_BLOCK];
+template <int BS, int LDS>
+__global__ __launch_bounds__(BS) void k_trace_closest(ff_dev_scene S, const float4* rays, uint32_t n,
+                                                      float4* hits, int two_sided, int32_t* ovf,
+                                                      unsigned long long* counts) {
+    __shared__ int smem[LDS * BS];
     Stack st;
     st.lds = &smem[threadIdx.x];
-    st.lds_stride = FF_BLOCK;
+    st.lds_stride = BS;
+    st.lds_entries = LDS;
     st.ovf = ovf + gtid();
-    st.ovf_stride = FF_TRAV_GRID * FF_BLOCK;
+    st.ovf_stride = FF_TRAV_THREADS;
     uint32_t cn = 0, ct = 0;
     for (uint32_t i = gtid(); i < n; i += gstride()) {
         float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
@@ -422,15 +467,16 @@ __global__ __launch_bounds__(FF_BLOCK) void k_trace_closest(ff_dev_scene S, cons
         wave_sum_add(&counts[1], ct);
     }
 }
-__global__ __launch_bounds__(FF_BLOCK) void k_trace_any(ff_dev_scene S, const float4* rays, uint32_t n,
-                                                        uint8_t* occ, int32_t* ovf,
-                                                        unsigned long long* counts) {
-    __shared__ int smem[FF_LDS_STACK * FF_BLOCK];
+template <int BS, int LDS>
+__global__ __launch_bounds__(BS) void k_trace_any(ff_dev_scene S, const float4* rays, uint32_t n, uint8_t* occ,
+                                                  int32_t* ovf, unsigned long long* counts) {
+    __shared__ int smem[LDS * BS];
     Stack st;
     st.lds = &smem[threadIdx.x];
-    st.lds_stride = FF_BLOCK;
+    st.lds_stride = BS;
+    st.lds_entries = LDS;
     st.ovf = ovf + gtid();
-    st.ovf_stride = FF_TRAV_GRID * FF_BLOCK;
+    st.ovf_stride = FF_TRAV_THREADS;
     uint32_t cn = 0, ct = 0;
     for (uint32_t i = gtid(); i < n; i += gstride()) {
         float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
@@ -480,8 +526,7 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 }
 
 // one workgroup per row: row_sum[j] = sum_i lum(i,j) * sin_theta_j
-__global__ __launch_bounds__(FF_BLOCK) void k_env_rows(const float* lum, int32_t w, int32_t h,
-                                                       double* row_sum) {
+__global__ __launch_bounds__(FF_BLOCK) void k_env_rows(const float* lum, int32_t w, int32_t h, double* row_sum) {
     __shared__ double red[FF_BLOCK];
     int j = blockIdx.x;
     float v = (float)j / (float)h;
@@ -493,9 +538,9 @@ __global__ __launch_bounds__(FF_BLOCK) void k_env_rows(const float* lum, int32_t
 }
 
 // single workgroup: denom, marginal_p, marginal_y (blocked inclusive scan)
-__global__ __launch_bounds__(FF_BLOCK) void k_env_marginal(const float* lum, const double* row_sum,
-                                                           int32_t w, int32_t h, double* denom_out,
-                                                           double* marg_p, double* marg_y) {
+__global__ __launch_bounds__(FF_BLOCK) void k_env_marginal(const float* lum, const double* row_sum, int32_t w,
+                                                           int32_t h, double* denom_out, double* marg_p,
+                                                           double* marg_y) {
     __shared__ double red[FF_BLOCK];
     double acc = 0.0;
     for (int j = threadIdx.x; j < h; j += FF_BLOCK) acc += row_sum[j];
@@ -528,9 +573,8 @@ __global__ __launch_bounds__(FF_BLOCK) void k_env_marginal(const float* lum, con
 
 // one workgroup per row: conditional CDF (blocked scan) and pdf texels
 __global__ __launch_bounds__(FF_BLOCK) void k_env_conditional(const float* lum, int32_t w, int32_t h,
-                                                              const double* denom_p,
-                                                              const double* marg_p, double* conds,
-                                                              float* pdf) {
+                                                              const double* denom_p, const double* marg_p,
+                                                              double* conds, float* pdf) {
     __shared__ double red[FF_BLOCK];
     int y = blockIdx.x;
     double denom = *denom_p;
@@ -563,8 +607,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_env_conditional(const float* lum, 
 // ============================================================ film resolve
 // Resolve of wf_logic (CRT/kernel.cu:447-454): average, exposure, Reinhard,
 // (int)(255*c) per channel.
-__global__ void k_film_resolve(const float4* film, uint32_t n, float exposure, float* avg,
-                               uint8_t* rgba) {
+__global__ void k_film_resolve(const float4* film, uint32_t n, float exposure, float* avg, uint8_t* rgba) {
     for (uint32_t p = gtid(); p < n; p += gstride()) {
         float4 f = film[p];
         float cnt = f.w;
@@ -588,44 +631,62 @@ inline uint32_t grid_for(uint64_t n, uint32_t cap) {
     return (uint32_t)(g < cap ? g : cap);
 }
 
+// traversal variants (block size, LDS stack entries) available to the autotuner
+template <int BS, int LDS>
+hipError_t trav_launch(int which, const ff_paths& P, int q, hipStream_t s) {
+    dim3 g(FF_TRAV_THREADS / BS), b(BS);
+    if (which == 0) hipLaunchKernelGGL((k_extend<BS, LDS>), g, b, 0, s, P, q);
+    else hipLaunchKernelGGL((k_shadow<BS, LDS>), g, b, 0, s, P);
+    return hipGetLastError();
+}
+
+hipError_t trav_dispatch(int which, int variant, const ff_paths& P, int q, hipStream_t s) {
+    switch (variant) {
+        case 1: return trav_launch<128, 16>(which, P, q, s);
+        case 2: return trav_launch<256, 8>(which, P, q, s);
+        case 3: return trav_launch<256, 24>(which, P, q, s);
+        case 4: return trav_launch<128, 24>(which, P, q, s);
+        case 5: return trav_launch<64, 32>(which, P, q, s);
+        default: return trav_launch<256, 16>(which, P, q, s);
+    }
+}
+
 }  // namespace
 
 // ===================================================================== host
 namespace ffk {
 
-hipError_t launch_generate(const ff_dev_camera& cam, int W, int H, uint32_t n_active,
-                           uint32_t sample_base, uint32_t seed, const ff_paths& P, int out_q,
-                           hipStream_t s) {
-    hipLaunchKernelGGL(k_generate, dim3(grid_for(n_active, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, cam, W, H,
-                       n_active, sample_base, seed, P, out_q);
+const int kTravVariants = 6;
+const int kTravLds[kTravVariants] = {16, 16, 8, 24, 24, 32};
+const int kTravBlock[kTravVariants] = {256, 128, 256, 256, 128, 64};
+
+hipError_t launch_batch_begin(const ff_paths& P, int set, uint32_t value, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(64), 0, s, P.bstate, P.counters, set, value);
     return hipGetLastError();
 }
-hipError_t launch_extend(const ff_dev_scene& S, const ff_paths& P, int q, hipStream_t s) {
-    hipLaunchKernelGGL(k_extend, dim3(FF_TRAV_GRID), dim3(FF_BLOCK), 0, s, S, P, q);
+hipError_t launch_generate(const ff_paths& P, hipStream_t s) {
+    hipLaunchKernelGGL(k_generate, dim3(grid_for(P.n_slots, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, P);
     return hipGetLastError();
 }
-hipError_t launch_shadow(const ff_dev_scene& S, const ff_paths& P, hipStream_t s) {
-    hipLaunchKernelGGL(k_shadow, dim3(FF_TRAV_GRID), dim3(FF_BLOCK), 0, s, S, P);
-    return hipGetLastError();
+hipError_t launch_extend(const ff_paths& P, int q, int variant, hipStream_t s) {
+    return trav_dispatch(0, variant, P, q, s);
 }
+hipError_t launch_shadow(const ff_paths& P, int variant, hipStream_t s) { return trav_dispatch(1, variant, P, 0, s); }
 hipError_t launch_reset(uint32_t* counters, uint32_t mask, hipStream_t s) {
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, counters, mask);
     return hipGetLastError();
 }
-hipError_t launch_logic(const ff_dev_scene& S, const ff_dev_env& E, const ff_paths& P, int q,
-                        uint32_t max_depth, uint32_t n_max, hipStream_t s) {
-    hipLaunchKernelGGL(k_logic, dim3(grid_for(n_max, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, S, E, P, q,
-                       max_depth);
+hipError_t launch_logic(const ff_paths& P, int q, uint32_t n_max, hipStream_t s) {
+    hipLaunchKernelGGL(k_logic, dim3(grid_for(n_max, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, P, q);
     return hipGetLastError();
 }
-hipError_t launch_shade(int type, const ff_dev_scene& S, const ff_dev_env& E, const ff_paths& P,
-                        int next_q, uint32_t n_max, hipStream_t s) {
+hipError_t launch_shade(int type, const ff_paths& P, int next_q, uint32_t n_max, hipStream_t s) {
     dim3 g(grid_for(n_max, FF_GRID_CAP)), b(FF_BLOCK);
     switch (type) {
-        case MAT_MIX: hipLaunchKernelGGL(k_shade<MAT_MIX>, g, b, 0, s, S, E, P, next_q); break;
-        case MAT_DIFFUSE: hipLaunchKernelGGL(k_shade<MAT_DIFFUSE>, g, b, 0, s, S, E, P, next_q); break;
-        case MAT_CONDUCTOR: hipLaunchKernelGGL(k_shade<MAT_CONDUCTOR>, g, b, 0, s, S, E, P, next_q); break;
-        default: hipLaunchKernelGGL(k_shade<MAT_DIELECTRIC>, g, b, 0, s, S, E, P, next_q); break;
+        case MAT_MIX: hipLaunchKernelGGL(k_shade<MAT_MIX>, g, b, 0, s, P, next_q); break;
+        case MAT_DIFFUSE: hipLaunchKernelGGL(k_shade<MAT_DIFFUSE>, g, b, 0, s, P, next_q); break;
+        case MAT_CONDUCTOR: hipLaunchKernelGGL(k_shade<MAT_CONDUCTOR>, g, b, 0, s, P, next_q); break;
+        default: hipLaunchKernelGGL(k_shade<MAT_DIELECTRIC>, g, b, 0, s, P, next_q); break;
     }
     return hipGetLastError();
 }
@@ -637,44 +698,42 @@ hipError_t launch_fill_u64(unsigned long long* p, uint32_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, p, n);
     return hipGetLastError();
 }
-hipError_t launch_resolve_layers(const float4* layers, uint32_t K, uint32_t WH, float4* film, float spp,
-                                 hipStream_t s) {
-    hipLaunchKernelGGL(k_resolve_layers, dim3(grid_for(WH, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, layers, K,
-                       WH, film, spp);
+hipError_t launch_resolve_layers(const float4* layers, uint32_t K, uint32_t WH, float4* film, hipStream_t s) {
+    hipLaunchKernelGGL(k_resolve_layers, dim3(grid_for(WH, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, layers, K, WH,
+                       film);
     return hipGetLastError();
 }
 hipError_t launch_trace_closest(const ff_dev_scene& S, const float4* rays, uint32_t n, float4* hits,
                                 int two_sided, int32_t* ovf, unsigned long long* counts, hipStream_t s) {
-    hipLaunchKernelGGL(k_trace_closest, dim3(FF_TRAV_GRID), dim3(FF_BLOCK), 0, s, S, rays, n, hits,
-                       two_sided, ovf, counts);
+    hipLaunchKernelGGL((k_trace_closest<256, 16>), dim3(FF_TRAV_THREADS / 256), dim3(256), 0, s, S, rays, n,
+                       hits, two_sided, ovf, counts);
     return hipGetLastError();
 }
-hipError_t launch_trace_any(const ff_dev_scene& S, const float4* rays, uint32_t n, uint8_t* occ,
-                            int32_t* ovf, unsigned long long* counts, hipStream_t s) {
-    hipLaunchKernelGGL(k_trace_any, dim3(FF_TRAV_GRID), dim3(FF_BLOCK), 0, s, S, rays, n, occ, ovf, counts);
+hipError_t launch_trace_any(const ff_dev_scene& S, const float4* rays, uint32_t n, uint8_t* occ, int32_t* ovf,
+                            unsigned long long* counts, hipStream_t s) {
+    hipLaunchKernelGGL((k_trace_any<256, 16>), dim3(FF_TRAV_THREADS / 256), dim3(256), 0, s, S, rays, n, occ,
+                       ovf, counts);
     return hipGetLastError();
 }
-hipError_t launch_gen_rays(const ff_dev_camera& cam, int W, int H, uint32_t seed, uint32_t sample,
-                           float4* rays, hipStream_t s) {
-    hipLaunchKernelGGL(k_gen_rays, dim3(grid_for((uint64_t)W * H, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, cam,
-                       W, H, seed, sample, rays);
+hipError_t launch_gen_rays(const ff_dev_camera& cam, int W, int H, uint32_t seed, uint32_t sample, float4* rays,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_gen_rays, dim3(grid_for((uint64_t)W * H, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, cam, W, H,
+                       seed, sample, rays);
     return hipGetLastError();
 }
 hipError_t launch_env_build(const float* rgb, int w, int h, float* lum, double* row_sum, double* denom,
                             double* marg_p, double* marg_y, double* conds, float* pdf, hipStream_t s) {
-    hipLaunchKernelGGL(k_env_lum, dim3(grid_for((uint64_t)w * h, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, rgb, w,
-                       h, lum);
+    hipLaunchKernelGGL(k_env_lum, dim3(grid_for((uint64_t)w * h, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, rgb, w, h,
+                       lum);
     hipLaunchKernelGGL(k_env_rows, dim3(h), dim3(FF_BLOCK), 0, s, lum, w, h, row_sum);
-    hipLaunchKernelGGL(k_env_marginal, dim3(1), dim3(FF_BLOCK), 0, s, lum, row_sum, w, h, denom, marg_p,
-                       marg_y);
-    hipLaunchKernelGGL(k_env_conditional, dim3(h), dim3(FF_BLOCK), 0, s, lum, w, h, denom, marg_p, conds,
-                       pdf);
+    hipLaunchKernelGGL(k_env_marginal, dim3(1), dim3(FF_BLOCK), 0, s, lum, row_sum, w, h, denom, marg_p, marg_y);
+    hipLaunchKernelGGL(k_env_conditional, dim3(h), dim3(FF_BLOCK), 0, s, lum, w, h, denom, marg_p, conds, pdf);
     return hipGetLastError();
 }
 hipError_t launch_film_resolve(const float4* film, uint32_t n, float exposure, float* avg, uint8_t* rgba,
                                hipStream_t s) {
-    hipLaunchKernelGGL(k_film_resolve, dim3(grid_for(n, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, film, n,
-                       exposure, avg, rgba);
+    hipLaunchKernelGGL(k_film_resolve, dim3(grid_for(n, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, film, n, exposure,
+                       avg, rgba);
     return hipGetLastError();
 }
 

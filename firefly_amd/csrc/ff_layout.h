@@ -13,10 +13,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define FF_BLOCK 256        // threads per workgroup (4 waves of 64)
-#define FF_LDS_STACK 16     // traversal stack entries per lane held in LDS
-#define FF_TRAV_GRID 2048   // workgroups of the persistent traversal kernels
-#define FF_GRID_CAP 4096    // grid-stride cap of the other stage kernels
+#define FF_BLOCK 256          // threads per workgroup of the stage kernels (4 waves of 64)
+#define FF_TRAV_THREADS 524288 // threads of the persistent traversal grids (any block size)
+#define FF_LDS_MIN 8          // smallest LDS stack of a traversal variant (sizes the overflow area)
+#define FF_GRID_CAP 4096      // grid-stride cap of the other stage kernels
 
 // path state word (beta.w bits)
 #define ST_LEN_MASK 0xffu
@@ -82,6 +82,23 @@ struct ff_dev_env {
     int32_t w, h;
 };
 
+// One render request: what a path slot needs to know about its frame.
+// Slots [slot_base, slot_base + W*H*K) belong to the request; slot - slot_base
+// = layer * W*H + pixel.  K == 1 accumulates straight into the film (one
+// writer per pixel); K > 1 accumulates into layers resolved at the end.
+struct ff_dev_request {
+    ff_dev_scene S;
+    ff_dev_env E;
+    ff_dev_camera C;
+    float4* film;      // [H][W] (sums, count)
+    float4* layers;    // [K][H*W] when K > 1
+    uint32_t slot_base, W, H, spp, depth, K, seed, sample_offset;
+};
+
+// device-resident batch state (graph-replay safe): [0] batch index
+#define B_BATCH 0
+#define B_COUNT 4
+
 struct ff_paths {
     float4* ray;     // [P][2]  (o, -) (d, -)
     float4* hit;     // [P]     (t, u, v, slot)
@@ -92,6 +109,7 @@ struct ff_paths {
     float4* sray;    // [P][3][2] shadow rays
     uint8_t* vis;    // [P][3]
     uint32_t* rngb;  // [P]
+    uint32_t* preq;  // [P]     request of the slot
     uint32_t* q_ext[2];
     uint32_t* q_mat[4];
     uint32_t* q_shadow;
@@ -99,4 +117,8 @@ struct ff_paths {
     float4* layers;      // [P] film layer accumulators (path id = layer*WH + pixel)
     unsigned long long* stats;  // [S_COUNT]
     int32_t* ovf;        // traversal stack overflow [depth][FF_TRAV_GRID*FF_BLOCK]
+    uint32_t* bstate;    // [B_COUNT]
+    const ff_dev_request* reqs;
+    uint32_t n_req;
+    uint32_t n_slots;    // total slots over all requests
 };

@@ -298,6 +298,66 @@ def icosphere_scene(width=1920, height=1080, spp=64, max_depth=8, seed=0, subdiv
                      cam_target=target, cam_fovy=40.0)
 
 
+def soup_in_cube(n: int, seed: int, edge: float = 0.01) -> tuple:
+    """n triangles with edges ~`edge`, uniform in the unit cube, random facing."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.0, 1.0, (n, 1, 3))
+    pos = c + rng.normal(scale=edge / np.sqrt(2), size=(n, 3, 3))
+    fn = np.cross(pos[:, 1] - pos[:, 0], pos[:, 2] - pos[:, 0])
+    fn /= np.maximum(np.linalg.norm(fn, axis=1, keepdims=True), 1e-12)
+    nrm = np.repeat(fn[:, None, :], 3, 1)
+    uv = rng.uniform(0, 1, (n, 3, 2))
+    return pos.astype(np.float32), nrm.astype(np.float32), uv.astype(np.float32)
+
+
+def batch_request_specs(n: int = 64, seed: int = 0):
+    """Seeded uneven requests: side 2^U[7,10] (square), spp U{1..32}, depth
+    U{2..12}, triangle count log-uniform in [1e4, 2e6]."""
+    rng = np.random.default_rng(seed + 1000)
+    out = []
+    for i in range(n):
+        side = int(round(2.0 ** rng.uniform(7.0, 10.0)))
+        out.append({"index": i, "width": side, "height": side, "spp": int(rng.integers(1, 33)),
+                    "max_depth": int(rng.integers(2, 13)),
+                    "tris": int(round(10.0 ** rng.uniform(4.0, np.log10(2e6)))), "seed": seed * 1000 + i})
+    return out
+
+
+def batch_request_scene(spec: dict, env: np.ndarray) -> SceneData:
+    """One request's scene: a displaced icosphere (largest subdivision under
+    70% of the triangle budget) in the unit cube plus a triangle soup with
+    edge ~0.01 making up the rest; materials as in the single-frame scene."""
+    n = spec["tris"]
+    sub = 2
+    while 20 * 4 ** (sub + 1) <= 0.7 * n:
+        sub += 1
+    rng = np.random.default_rng(spec["seed"])
+    pos, nrm, uv, region = displaced_icosphere(sub, (0.5, 0.5, 0.5), 0.3, 0.05, spec["seed"])
+    mats = [material(FF_MAT_DIFFUSE, rng.uniform(0.1, 0.9, 3))]  # soup
+    region_mat = np.zeros(20, np.int32)
+    for r in range(20):
+        kind = r % 3
+        if kind == 0:
+            mats.append(material(FF_MAT_DIFFUSE, rng.uniform(0.1, 0.9, 3)))
+        elif kind == 1:
+            mats.append(material(FF_MAT_CONDUCTOR, rng.uniform(0.5, 1.0, 3), roughness=rng.uniform(0.05, 0.5),
+                                 metallic=1.0))
+        else:
+            mats.append(material(FF_MAT_DIELECTRIC, (1, 1, 1), ior=1.5))
+        region_mat[r] = len(mats) - 1
+    parts = [(pos, nrm, uv, region_mat[region])]
+    n_soup = n - len(pos)
+    if n_soup > 0:
+        parts.append((*soup_in_cube(n_soup, spec["seed"] + 1), 0))
+    P, N, U, M = _concat(parts)
+    az, el = rng.uniform(0, 2 * np.pi), rng.uniform(0.1, 0.8)
+    eye = (0.5 + 1.6 * np.cos(el) * np.cos(az), 0.5 + 1.6 * np.sin(el), 0.5 + 1.6 * np.cos(el) * np.sin(az))
+    w, h = spec["width"], spec["height"]
+    return SceneData(P, N, U, M, np.stack(mats), env, inv_view_proj(eye, (0.5, 0.5, 0.5), 45.0, w / h), w, h,
+                     spec["spp"], spec["max_depth"], spec["seed"], name=f"request{spec['index']}", cam_eye=eye,
+                     cam_target=(0.5, 0.5, 0.5), cam_fovy=45.0)
+
+
 def write_scene_file(sd: SceneData, path: str) -> None:
     """Write the FFS1 scene file read by the C++ render_demo (one mesh per
     material run, in triangle order)."""

@@ -5,7 +5,8 @@
  * HIP streams passed as void*.  Every call returns an ff_status and never
  * exits the process; ff_last_error() gives the message of the calling
  * thread's last failure.  One context per host thread; handles are not
- * thread-safe.
+ * thread-safe, except that ff_scene_create only reads its context and may
+ * run concurrently on one context from several host threads.
  *
  * Each entry point names the reference interface it replaces
  * (JakeKurtz/Firefly, CUDA-RayTracer/ = CRT/).
@@ -170,6 +171,33 @@ ff_status ff_generate_rays(ff_context *ctx, const ff_camera *cam, int32_t width,
  * PathTracer::clear_buffer does, CRT/PathTracer.cpp:150-170). */
 ff_status ff_render(ff_context *ctx, ff_scene *scene, ff_envmap *env, const ff_camera *cam,
                     const ff_render_req *req, float *film, void *stream);
+/* ---- batched requests: many scene/camera requests of uneven resolution,
+ * spp and depth in one wavefront (one path slot per pixel of every request,
+ * so each film pixel has a single writer; a request's film equals what
+ * ff_render produces for it with spp_batch = 1).  All scenes must belong to
+ * ctx; films must be distinct device [h][w][4] buffers (ADDED to, like
+ * ff_render).  flags: FF_RENDER_USE_GRAPH captures one batch of launches in a
+ * hipGraph and replays it once per sample index.  No reference counterpart:
+ * the reference renders one frame per draw (CRT/PathTracer.cpp:59-79). */
+typedef struct {
+    ff_scene *scene;
+    ff_envmap *env;
+    ff_camera camera;
+    int32_t width, height, spp, max_depth;
+    uint32_t seed;
+    int32_t sample_offset;
+    float *film;
+} ff_batch_item;
+ff_status ff_render_batch(ff_context *ctx, const ff_batch_item *items, int32_t n, int32_t flags,
+                          void *stream);
+
+/* ---- traversal kernel variants (workgroup size, LDS stack entries), chosen
+ * per context by the autotuner (firefly_amd/autotune.py); variant 0 is the
+ * default.  No reference counterpart (the reference has one fixed shape). */
+int32_t ff_traversal_variant_count(void);
+ff_status ff_traversal_variant_info(int32_t variant, int32_t *block, int32_t *lds_entries);
+ff_status ff_set_traversal_variant(ff_context *ctx, int32_t variant);
+
 /* ray counts of the last ff_render on this context (synchronises its stream). */
 ff_status ff_last_render_stats(ff_context *ctx, ff_ray_stats *out);
 

@@ -1,0 +1,129 @@
+"""Batched requests: several scenes/cameras of uneven resolution, spp and depth
+rendered in one wavefront (optionally replayed from a hipGraph) give each
+request exactly the film a plain per-request render gives it."""
+import numpy as np
+import pytest
+
+from firefly_amd import abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests():
+    specs = [  # (seed, width, height, spp, depth, material)
+        (0, 64, 48, 3, 2, abi.FF_MAT_MIX),
+        (1, 40, 40, 5, 5, abi.FF_MAT_DIFFUSE),
+        (2, 96, 32, 1, 3, abi.FF_MAT_CONDUCTOR),
+        (3, 24, 36, 4, 7, abi.FF_MAT_DIELECTRIC),
+    ]
+    out = []
+    for seed, w, h, spp, depth, mt in specs:
+        sd = scenes.spheres_scene(width=w, height=h, spp=spp, max_depth=depth, seed=seed, segments=16, rings=8)
+        mats = sd.materials.copy()
+        mats[1:, 0] = mt
+        if mt == abi.FF_MAT_CONDUCTOR:
+            mats[1:, 5] = 1.0
+            mats[1:, 4] = 0.3
+        sd.materials = mats
+        out.append(sd)
+    return out
+
+
+@pytest.mark.parametrize("flags", [0, abi.FF_RENDER_USE_GRAPH])
+def test_batch_equals_individual_renders(gpu, flags):
+    import torch
+    reqs = _requests()
+    objs = []
+    for sd in reqs:
+        sc = abi.Scene(gpu, sd.positions, sd.normals, sd.uvs, sd.material_ids, sd.materials)
+        env = abi.EnvMap(gpu, sd.env)
+        objs.append((sd, sc, env, abi.make_camera(sd.inv_vp)))
+    singles = []
+    for sd, sc, env, cam in objs:
+        f = torch.zeros((sd.height, sd.width, 4), dtype=torch.float32, device="cuda")
+        abi.render(gpu, sc, env, cam, sd.width, sd.height, sd.spp, sd.max_depth, sd.seed, f, spp_batch=1)
+        singles.append(f)
+    torch.cuda.synchronize()
+    films = [torch.zeros((sd.height, sd.width, 4), dtype=torch.float32, device="cuda") for sd, *_ in objs]
+    abi.render_batch(gpu, [(sc, env, cam, sd.width, sd.height, sd.spp, sd.max_depth, sd.seed, f)
+                           for (sd, sc, env, cam), f in zip(objs, films)], flags=flags)
+    torch.cuda.synchronize()
+    for (sd, *_), f, s in zip(objs, films, singles):
+        a, b = f.cpu().numpy(), s.cpu().numpy()
+        assert np.all(a[..., 3] == sd.spp)
+        assert np.array_equal(a, b), f"request seed {sd.seed}: batched film differs"
+    st = abi.last_render_stats(gpu)
+    assert st.paths == sum(sd.width * sd.height * sd.spp for sd, *_ in objs)
+    for _, sc, env, _ in objs:
+        sc.close()
+        env.close()
+
+
+def test_graph_render_equals_plain_render(gpu):
+    """FF_RENDER_USE_GRAPH on a single request replays the captured batch."""
+    import torch
+    sd = scenes.spheres_scene(width=64, height=40, spp=6, max_depth=4)
+    sc = abi.Scene(gpu, sd.positions, sd.normals, sd.uvs, sd.material_ids, sd.materials)
+    env = abi.EnvMap(gpu, sd.env)
+    cam = abi.make_camera(sd.inv_vp)
+    a = torch.zeros((sd.height, sd.width, 4), dtype=torch.float32, device="cuda")
+    b = torch.zeros_like(a)
+    abi.render(gpu, sc, env, cam, sd.width, sd.height, 6, 4, 0, a, spp_batch=2)
+    abi.render(gpu, sc, env, cam, sd.width, sd.height, 6, 4, 0, b, spp_batch=2, flags=abi.FF_RENDER_USE_GRAPH)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+
+
+def test_traversal_variants_agree(gpu):
+    """Every (workgroup, LDS stack) traversal variant gives the same image."""
+    import torch
+    sd = scenes.spheres_scene(width=48, height=32, spp=2, max_depth=3)
+    sc = abi.Scene(gpu, sd.positions, sd.normals, sd.uvs, sd.material_ids, sd.materials)
+    env = abi.EnvMap(gpu, sd.env)
+    cam = abi.make_camera(sd.inv_vp)
+    ref = None
+    for v, block, lds in abi.traversal_variants():
+        abi.set_traversal_variant(gpu, v)
+        f = torch.zeros((sd.height, sd.width, 4), dtype=torch.float32, device="cuda")
+        abi.render(gpu, sc, env, cam, sd.width, sd.height, 2, 3, 0, f)
+        torch.cuda.synchronize()
+        if ref is None:
+            ref = f
+        assert torch.equal(f, ref), f"variant {v} (block {block}, lds {lds})"
+    abi.set_traversal_variant(gpu, 0)
+
+
+def test_concurrent_scene_creation_deep_trees(gpu):
+    """Scenes built from several host threads on one context, with BVHs deeper
+    than the default LDS traversal stack (so the overflow area is used), render
+    the same films batched (graph) as one at a time."""
+    import torch
+    from concurrent.futures import ThreadPoolExecutor
+    env = scenes.sun_env(256, 128, 0)
+    specs = [{"index": i, "width": 48 + 8 * i, "height": 40, "spp": 1 + i % 3, "max_depth": 3 + i % 4,
+              "tris": t, "seed": 70 + i} for i, t in enumerate((12000, 60000, 250000, 30000))]
+    sds = [scenes.batch_request_scene(s, env) for s in specs]
+
+    def make(sd):
+        return abi.Scene(gpu, sd.positions, sd.normals, sd.uvs, sd.material_ids, sd.materials)
+
+    with ThreadPoolExecutor(4) as ex:
+        scs = list(ex.map(make, sds))
+    depths = [abi.bvh_build_host(sd.positions)[2] for sd in sds]
+    assert max(depths) > 16  # deeper than the default 16-entry LDS stack
+    e = abi.EnvMap(gpu, env)
+    cams = [abi.make_camera(sd.inv_vp) for sd in sds]
+    singles, films = [], []
+    for sd, sc, cam in zip(sds, scs, cams):
+        f = torch.zeros((sd.height, sd.width, 4), dtype=torch.float32, device="cuda")
+        abi.render(gpu, sc, e, cam, sd.width, sd.height, sd.spp, sd.max_depth, sd.seed, f)
+        singles.append(f)
+        films.append(torch.zeros_like(f))
+    abi.render_batch(gpu, [(sc, e, cam, sd.width, sd.height, sd.spp, sd.max_depth, sd.seed, f)
+                           for sd, sc, cam, f in zip(sds, scs, cams, films)], flags=abi.FF_RENDER_USE_GRAPH)
+    torch.cuda.synchronize()
+    for sd, a, b in zip(sds, films, singles):
+        assert torch.equal(a, b), sd.name
+    for sc in scs:
+        sc.close()
+    e.close()
