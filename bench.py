@@ -212,9 +212,12 @@ def main() -> int:
     elif not args.no_autotune:
         from firefly_amd.autotune import autotune
         t0 = time.perf_counter()
-        tuned = autotune(ctx, scene, env, cam, sd.width, sd.height, args.depth, sd.seed, probe_spp=2,
-                         batches=(1,))
+        tuned = autotune(ctx, scene, env, cam, sd.width, sd.height, args.depth, sd.seed, probe_spp=4,
+                         batches=(1, 2, 4))
         args.spp_batch = tuned["best"]["spp_batch"]
+        for c in tuned["candidates"]:
+            log(f"  variant {c['variant']} (block {c['block']}, lds {c['lds_entries']}, refill {c['refill']}) "
+                f"spp_batch {c['spp_batch']}: {c['seconds'] * 1e3:.1f} ms")
         log(f"autotune ({time.perf_counter() - t0:.1f} s): {tuned['best']}")
 
     def frame(flags=0):

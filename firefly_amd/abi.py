@@ -86,7 +86,8 @@ _PROTOS = {
     "ff_last_render_stats": (C.c_int, [_P, C.POINTER(RayStats)]),
     "ff_render_batch": (C.c_int, [_P, C.POINTER(BatchItem), C.c_int32, C.c_int32, _P]),
     "ff_traversal_variant_count": (C.c_int32, []),
-    "ff_traversal_variant_info": (C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "ff_traversal_variant_info": (C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                            C.POINTER(C.c_int32)]),
     "ff_set_traversal_variant": (C.c_int, [_P, C.c_int32]),
     "ff_film_resolve": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_float, _P, _P, _P]),
 }
@@ -283,12 +284,12 @@ def render_batch(ctx: Context, items, flags: int = 0, stream=None) -> None:
 
 
 def traversal_variants():
-    """[(variant, block, lds_entries)] of the traversal kernels."""
+    """[(variant, block, lds_entries, refill)] of the traversal kernels."""
     out = []
     for v in range(_lib().ff_traversal_variant_count()):
-        b, l = C.c_int32(), C.c_int32()
-        check(_lib().ff_traversal_variant_info(v, C.byref(b), C.byref(l)))
-        out.append((v, b.value, l.value))
+        b, l, r = C.c_int32(), C.c_int32(), C.c_int32()
+        check(_lib().ff_traversal_variant_info(v, C.byref(b), C.byref(l), C.byref(r)))
+        out.append((v, b.value, l.value, r.value))
     return out
 
 

@@ -1,7 +1,8 @@
 """Autotuner for the wavefront launch shapes.
 
 Times short renders of the actual request for every traversal kernel variant
-(workgroup size x LDS stack entries, see ff_traversal_variant_info) and for a
+(workgroup size x LDS stack entries x refill threshold, see
+ff_traversal_variant_info) and for a
 few paths-in-flight choices (samples per pixel in flight, spp_batch), then
 keeps the fastest on the context.  Results can be written as JSON and reused.
 """
@@ -41,13 +42,14 @@ def autotune(ctx: abi.Context, scene: abi.Scene, env: abi.EnvMap, cam: abi.Camer
     # warm-up (first launches pay code-object load and allocation)
     abi.render(ctx, scene, env, cam, width, height, 1, max_depth, seed, film)
     torch.cuda.synchronize()
-    for v, block, lds in abi.traversal_variants():
+    for v, block, lds, refill in abi.traversal_variants():
         abi.set_traversal_variant(ctx, v)
         for k in batches:
             if k > probe_spp:
                 continue
             t = _time_render(ctx, scene, env, cam, width, height, probe_spp, max_depth, seed, film, k, reps)
-            results.append({"variant": v, "block": block, "lds_entries": lds, "spp_batch": k, "seconds": t})
+            results.append({"variant": v, "block": block, "lds_entries": lds, "refill": refill, "spp_batch": k,
+                            "seconds": t})
     best = min(results, key=lambda r: r["seconds"])
     abi.set_traversal_variant(ctx, best["variant"])
     out = {"best": best, "candidates": results,

@@ -459,7 +459,8 @@ ff_status ff_trace_closest(ff_scene* s, const float* rays, uint32_t n, float* hi
     HIP_TRY(hipSetDevice(s->ctx->device));
     ff_status st = ensure_ovf(s->ctx, s->max_depth);
     if (st != FF_OK) return st;
-    HIP_TRY(ffk::launch_trace_closest(s->view(), (const float4*)rays, n, (float4*)hits, two_sided, s->ctx->ovf,
+    HIP_TRY(ffk::launch_trace_closest(s->view(), (const float4*)rays, n, (float4*)hits, two_sided, s->ctx->counters,
+                                      s->ctx->ovf,
                                       (unsigned long long*)counts, (hipStream_t)stream));
     return FF_OK;
 }
@@ -471,7 +472,7 @@ ff_status ff_trace_any(ff_scene* s, const float* rays, uint32_t n, uint8_t* occl
     HIP_TRY(hipSetDevice(s->ctx->device));
     ff_status st = ensure_ovf(s->ctx, s->max_depth);
     if (st != FF_OK) return st;
-    HIP_TRY(ffk::launch_trace_any(s->view(), (const float4*)rays, n, occluded, s->ctx->ovf,
+    HIP_TRY(ffk::launch_trace_any(s->view(), (const float4*)rays, n, occluded, s->ctx->counters, s->ctx->ovf,
                                   (unsigned long long*)counts, (hipStream_t)stream));
     return FF_OK;
 }
@@ -538,6 +539,7 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
         if (!ctx->own_stream) HIP_TRY(hipStreamCreate(&ctx->own_stream));
         hs = ctx->own_stream;
     }
+    HIP_TRY(ffk::prepare_traversal(ctx->trav_variant));
     ff_paths p = ctx->paths.p;
     // request table (uploaded once the stream's earlier work is done with it)
     std::vector<ff_dev_request> table(reqs.size());
@@ -604,7 +606,8 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
         for (int32_t it = 1; it <= max_depth + 1; it++) {
             HIP_TRY(timed(0, [&] { return ffk::launch_extend(p, cur, var, hs); }));
             HIP_TRY(timed(1, [&] { return ffk::launch_shadow(p, var, hs); }));
-            uint32_t mask = (1u << (cur == 0 ? C_EXT1 : C_EXT0)) | (1u << C_SHADOW) | (0xfu << C_MAT0);
+            uint32_t mask = (1u << (cur == 0 ? C_EXT1 : C_EXT0)) | (1u << C_SHADOW) | (0xfu << C_MAT0) |
+                            (1u << C_FETCH_EXT) | (1u << C_FETCH_SHD);
             HIP_TRY(ffk::launch_reset(p.counters, mask, hs));
             HIP_TRY(timed(2, [&] { return ffk::launch_logic(p, cur, p.n_slots, hs); }));
             if (it <= max_depth) {
@@ -691,11 +694,12 @@ ff_status ff_set_traversal_variant(ff_context* ctx, int32_t variant) {
     return FF_OK;
 }
 
-ff_status ff_traversal_variant_info(int32_t variant, int32_t* block, int32_t* lds_entries) {
+ff_status ff_traversal_variant_info(int32_t variant, int32_t* block, int32_t* lds_entries, int32_t* refill) {
     if (variant < 0 || variant >= ffk::kTravVariants)
         return fail(FF_ERR_INVALID, "ff_traversal_variant_info: variant %d of %d", variant, ffk::kTravVariants);
     if (block) *block = ffk::kTravBlock[variant];
     if (lds_entries) *lds_entries = ffk::kTravLds[variant];
+    if (refill) *refill = ffk::kTravRefill[variant];
     return FF_OK;
 }
 

@@ -428,113 +428,143 @@ __device__ __forceinline__ void leaf_range(const Bvh& B, int ref, int& off, int&
     if (cnt == 15) cnt = B.leaf_count[off];
 }
 
-// intersect() closest hit (CRT/dTriangle.cu:140-213): near child first by
-// dirIsNeg[axis], far child pushed; a candidate replaces the record when
+// Per-lane traversal state: the ray, its reciprocal direction and octant,
+// and the next node reference to visit.
+struct TravRay {
+    f3 o, d, inv;
+    bool nx, ny, nz;
+    int ref;
+};
+
+// Tests the root box (the reference tests node 0 first); false when the ray
+// misses the scene or the scene lies wholly behind the origin.
+__device__ __forceinline__ bool trav_begin(const Bvh& B, f3 o, f3 d, TravRay& R, Stack& st,
+                                           uint32_t& cnt_nodes) {
+    R.o = o;
+    R.d = d;
+    R.inv = V3(1.f / d.x, 1.f / d.y, 1.f / d.z);
+    R.nx = R.inv.x < 0;
+    R.ny = R.inv.y < 0;
+    R.nz = R.inv.z < 0;
+    float tn, tf;
+    cnt_nodes++;
+    if (!slab(B.root[0], B.root[1], B.root[2], B.root[3], B.root[4], B.root[5], o, R.inv, R.nx, R.ny, R.nz, tn,
+              tf) ||
+        tf < -PRUNE_BEHIND)
+        return false;
+    R.ref = __float_as_int(B.root[6]);
+    st.sp = 0;
+    return true;
+}
+
+// One interior record: both child boxes are tested; the near child (by
+// dirIsNeg[axis]) is visited next and the far one pushed when both are hit.
+// Returns false when neither child is entered (the caller pops).
+template <bool PRUNE>
+__device__ __forceinline__ bool visit_interior(const Bvh& B, TravRay& R, float t_best, Stack& st,
+                                               uint32_t& cnt_nodes) {
+    const float4* n = B.nodes + 4 * (size_t)R.ref;
+    float4 a = n[0], b = n[1], c = n[2];
+    int4 m = *(const int4*)(n + 3);
+    cnt_nodes += 2;
+    float tn0, tf0, tn1, tf1;
+    bool h0, h1;
+    if (PRUNE) {
+        float lim = t_best + t_best * PRUNE_REL + PRUNE_ABS;
+        h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, R.o, R.inv, R.nx, R.ny, R.nz, tn0, tf0) &&
+             !(tn0 > lim || tf0 < -PRUNE_BEHIND);
+        h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, R.o, R.inv, R.nx, R.ny, R.nz, tn1, tf1) &&
+             !(tn1 > lim || tf1 < -PRUNE_BEHIND);
+    } else {
+        h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, R.o, R.inv, R.nx, R.ny, R.nz, tn0, tf0) && !(tf0 < -PRUNE_BEHIND);
+        h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, R.o, R.inv, R.nx, R.ny, R.nz, tn1, tf1) && !(tf1 < -PRUNE_BEHIND);
+    }
+    bool neg = m.z == 0 ? R.nx : (m.z == 1 ? R.ny : R.nz);
+    int nref = neg ? m.y : m.x, fref = neg ? m.x : m.y;
+    bool hn = neg ? h1 : h0, hf = neg ? h0 : h1;
+    if (hn) {
+        if (hf) st.push(fref);
+        R.ref = nref;
+        return true;
+    }
+    if (hf) {
+        R.ref = fref;
+        return true;
+    }
+    return false;
+}
+
+// intersect() closest hit (CRT/dTriangle.cu:140-213), one step: an interior
+// record or a leaf's triangles.  A candidate replaces the record when
 // 0 <= t < tmin.  Children whose boxes miss (or lie beyond the current hit by
 // the conservative slack, or wholly behind the origin) are never visited.
+// Returns false once the traversal is complete.
+__device__ __forceinline__ bool closest_step(const Bvh& B, const float* __restrict__ tris, TravRay& R, Hit& h,
+                                             bool two_sided, Stack& st, uint32_t& cnt_nodes,
+                                             uint32_t& cnt_tris) {
+    if (R.ref >= 0) {
+        if (visit_interior<true>(B, R, h.t, st, cnt_nodes)) return true;
+    } else {
+        int off, np;
+        leaf_range(B, R.ref, off, np);
+        for (int i = 0; i < np; i++) {
+            Tri T = load_tri(tris, off + i);
+            cnt_tris++;
+            float t = 0.f, u = 0.f, v = 0.f;
+            bool ok = two_sided ? mt_two(T, R.o, R.d, t, u, v) : mt_cull(T, R.o, R.d, t, u, v);
+            if (ok && !(t < 0.f) && t < h.t) { h.t = t; h.u = u; h.v = v; h.id = off + i; }
+        }
+    }
+    if (st.sp == 0) return false;
+    R.ref = st.pop();
+    return true;
+}
+
+__device__ __forceinline__ Hit no_hit() {
+    Hit h;
+    h.t = K_HUGE; h.u = 0.f; h.v = 0.f; h.id = -1;
+    return h;
+}
+
 __device__ __forceinline__ Hit closest_hit(const Bvh& B, const float* __restrict__ tris, f3 o, f3 d,
                                            bool two_sided, Stack& st, uint32_t& cnt_nodes,
                                            uint32_t& cnt_tris) {
-    Hit h;
-    h.t = K_HUGE; h.u = 0.f; h.v = 0.f; h.id = -1;
-    f3 inv = V3(1.f / d.x, 1.f / d.y, 1.f / d.z);
-    bool nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
-    float tn, tf;
-    cnt_nodes++;
-    if (!slab(B.root[0], B.root[1], B.root[2], B.root[3], B.root[4], B.root[5], o, inv, nx, ny, nz, tn, tf) ||
-        tf < -PRUNE_BEHIND)
-        return h;
-    int ref = __float_as_int(B.root[6]);
-    st.sp = 0;
-    for (;;) {
-        if (ref >= 0) {
-            const float4* n = B.nodes + 4 * (size_t)ref;
-            float4 a = n[0], b = n[1], c = n[2];
-            int4 m = *(const int4*)(n + 3);
-            cnt_nodes += 2;
-            float lim = h.t + h.t * PRUNE_REL + PRUNE_ABS;
-            float tn0, tf0, tn1, tf1;
-            bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, nx, ny, nz, tn0, tf0) &&
-                      !(tn0 > lim || tf0 < -PRUNE_BEHIND);
-            bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, nx, ny, nz, tn1, tf1) &&
-                      !(tn1 > lim || tf1 < -PRUNE_BEHIND);
-            bool neg = m.z == 0 ? nx : (m.z == 1 ? ny : nz);
-            int nref = neg ? m.y : m.x, fref = neg ? m.x : m.y;
-            bool hn = neg ? h1 : h0, hf = neg ? h0 : h1;
-            if (hn) {
-                if (hf) st.push(fref);
-                ref = nref;
-                continue;
-            }
-            if (hf) {
-                ref = fref;
-                continue;
-            }
-        } else {
-            int off, np;
-            leaf_range(B, ref, off, np);
-            for (int i = 0; i < np; i++) {
-                Tri T = load_tri(tris, off + i);
-                cnt_tris++;
-                float t = 0.f, u = 0.f, v = 0.f;
-                bool ok = two_sided ? mt_two(T, o, d, t, u, v) : mt_cull(T, o, d, t, u, v);
-                if (ok && !(t < 0.f) && t < h.t) { h.t = t; h.u = u; h.v = v; h.id = off + i; }
-            }
-        }
-        if (st.sp == 0) break;
-        ref = st.pop();
-    }
+    Hit h = no_hit();
+    TravRay R;
+    if (!trav_begin(B, o, d, R, st, cnt_nodes)) return h;
+    while (closest_step(B, tris, R, h, two_sided, st, cnt_nodes, cnt_tris)) {}
     return h;
 }
 
 // intersect_shadows with tmin = K_HUGE (CRT/dTriangle.cu:215-259,
-// CRT/dEnvironmentLight.cu:60-64): true when any triangle has 0 <= t < 1e32.
+// CRT/dEnvironmentLight.cu:60-64), one step.  Returns 0 while work remains,
+// 1 when a triangle with 0 <= t < 1e32 is found, 2 when the ray is unoccluded.
+__device__ __forceinline__ int any_step(const Bvh& B, const float* __restrict__ tris, TravRay& R, Stack& st,
+                                        uint32_t& cnt_nodes, uint32_t& cnt_tris) {
+    if (R.ref >= 0) {
+        if (visit_interior<false>(B, R, 0.f, st, cnt_nodes)) return 0;
+    } else {
+        int off, np;
+        leaf_range(B, R.ref, off, np);
+        for (int i = 0; i < np; i++) {
+            Tri T = load_tri(tris, off + i);
+            cnt_tris++;
+            float t = 0.f;
+            if (mt_shadow(T, R.o, R.d, t) && t < K_HUGE) return 1;
+        }
+    }
+    if (st.sp == 0) return 2;
+    R.ref = st.pop();
+    return 0;
+}
+
 __device__ __forceinline__ bool any_hit(const Bvh& B, const float* __restrict__ tris, f3 o, f3 d, Stack& st,
                                         uint32_t& cnt_nodes, uint32_t& cnt_tris) {
-    f3 inv = V3(1 / d.x, 1 / d.y, 1 / d.z);
-    bool nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
-    float tn, tf;
-    cnt_nodes++;
-    if (!slab(B.root[0], B.root[1], B.root[2], B.root[3], B.root[4], B.root[5], o, inv, nx, ny, nz, tn, tf) ||
-        tf < -PRUNE_BEHIND)
-        return false;
-    int ref = __float_as_int(B.root[6]);
-    st.sp = 0;
-    for (;;) {
-        if (ref >= 0) {
-            const float4* n = B.nodes + 4 * (size_t)ref;
-            float4 a = n[0], b = n[1], c = n[2];
-            int4 m = *(const int4*)(n + 3);
-            cnt_nodes += 2;
-            float tn0, tf0, tn1, tf1;
-            bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, nx, ny, nz, tn0, tf0) && !(tf0 < -PRUNE_BEHIND);
-            bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, nx, ny, nz, tn1, tf1) && !(tf1 < -PRUNE_BEHIND);
-            bool neg = m.z == 0 ? nx : (m.z == 1 ? ny : nz);
-            int nref = neg ? m.y : m.x, fref = neg ? m.x : m.y;
-            bool hn = neg ? h1 : h0, hf = neg ? h0 : h1;
-            if (hn) {
-                if (hf) st.push(fref);
-                ref = nref;
-                continue;
-            }
-            if (hf) {
-                ref = fref;
-                continue;
-            }
-        } else {
-            int off, np;
-            leaf_range(B, ref, off, np);
-            for (int i = 0; i < np; i++) {
-                Tri T = load_tri(tris, off + i);
-                cnt_tris++;
-                float t = 0.f;
-                if (mt_shadow(T, o, d, t) && t < K_HUGE) return true;
-            }
-        }
-        if (st.sp == 0) break;
-        ref = st.pop();
-    }
-    return false;
+    TravRay R;
+    if (!trav_begin(B, o, d, R, st, cnt_nodes)) return false;
+    int r;
+    while ((r = any_step(B, tris, R, st, cnt_nodes, cnt_tris)) == 0) {}
+    return r == 1;
 }
 
 // ----------------------------------------------------- wave64 utilities --

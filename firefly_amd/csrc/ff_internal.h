@@ -13,6 +13,9 @@ namespace ffk {
 extern const int kTravVariants;
 extern const int kTravLds[];
 extern const int kTravBlock[];
+extern const int kTravRefill[];
+// computes the grid sizes of a traversal variant (call before a graph capture)
+hipError_t prepare_traversal(int variant);
 hipError_t launch_batch_begin(const ff_paths& P, int set, uint32_t value, hipStream_t s);
 hipError_t launch_generate(const ff_paths& P, hipStream_t s);
 hipError_t launch_extend(const ff_paths& P, int q, int variant, hipStream_t s);
@@ -24,10 +27,10 @@ hipError_t launch_fill4(float4* p, uint32_t n, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, uint32_t n, hipStream_t s);
 hipError_t launch_resolve_layers(const float4* layers, uint32_t K, uint32_t WH, float4* film, hipStream_t s);
 hipError_t launch_trace_closest(const ff_dev_scene& S, const float4* rays, uint32_t n,
-                                float4* hits, int two_sided, int32_t* ovf,
+                                float4* hits, int two_sided, uint32_t* counters, int32_t* ovf,
                                 unsigned long long* counts, hipStream_t s);
 hipError_t launch_trace_any(const ff_dev_scene& S, const float4* rays, uint32_t n, uint8_t* occ,
-                            int32_t* ovf, unsigned long long* counts, hipStream_t s);
+                            uint32_t* counters, int32_t* ovf, unsigned long long* counts, hipStream_t s);
 hipError_t launch_gen_rays(const ff_dev_camera& cam, int W, int H, uint32_t seed,
                            uint32_t sample, float4* rays, hipStream_t s);
 hipError_t launch_env_build(const float* rgb, int w, int h, float* lum, double* row_sum,

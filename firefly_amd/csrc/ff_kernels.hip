@@ -10,6 +10,10 @@
 // launches is identical from batch to batch and can be replayed as a graph.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "ff_device.h"
 #include "ff_internal.h"
 
@@ -86,54 +90,144 @@ __global__ __launch_bounds__(FF_BLOCK) void k_generate(ff_paths P) {
     }
 }
 
-// ================================================================== extend
-// wf_extend (CRT/kernel.cu:487-523): closest hit of every queued path.
-template <int BS, int LDS>
-__global__ __launch_bounds__(BS) void k_extend(ff_paths P, int q) {
-    __shared__ int smem[LDS * BS];
-    Stack st;
-    st.lds = &smem[threadIdx.x];
-    st.lds_stride = BS;
-    st.lds_entries = LDS;
-    st.ovf = P.ovf + gtid();
-    st.ovf_stride = FF_TRAV_THREADS;
-    const uint32_t n = P.counters[q == 0 ? C_EXT0 : C_EXT1];
-    const uint32_t* queue = P.q_ext[q];
-    uint32_t cn = 0, ct = 0;
-    for (uint32_t i = gtid(); i < n; i += gstride()) {
-        uint32_t id = queue[i];
+// =============================================================== traversal
+// Ray sources of the persistent traversal kernels: load() binds queue entry k
+// to a ray and its scene, hit()/occluded() write the result.
+struct ExtSrc {  // wf_extend (CRT/kernel.cu:487-523): closest hit of every queued path
+    ff_paths P;
+    const uint32_t* queue;
+    __device__ __forceinline__ void load(uint32_t k, uint32_t& id, Bvh& B, const float*& tris, f3& o, f3& d,
+                                         bool& two) const {
+        id = queue[k];
         const ff_dev_scene& S = P.reqs[P.preq[id]].S;
-        float4 r0 = P.ray[2 * id], r1 = P.ray[2 * id + 1];
-        uint32_t stw = __float_as_uint(P.beta[id].w);
-        Hit h = closest_hit(bvh_of(S), S.tris, xyz(r0), xyz(r1), (stw & ST_INSIDE) != 0, st, cn, ct);
+        B = bvh_of(S);
+        tris = S.tris;
+        o = xyz(P.ray[2 * id]);
+        d = xyz(P.ray[2 * id + 1]);
+        two = (__float_as_uint(P.beta[id].w) & ST_INSIDE) != 0;
+    }
+    __device__ __forceinline__ void hit(uint32_t id, const Hit& h) const {
         P.hit[id] = make_float4(h.t, h.u, h.v, __int_as_float(h.id));
     }
-    wave_sum_add(&P.stats[S_EXT_NODES], cn);
-    wave_sum_add(&P.stats[S_EXT_TRIS], ct);
-}
+    __device__ __forceinline__ void occluded(uint32_t, bool) const {}
+};
+struct ShadowSrc {  // wf_shadow (CRT/kernel.cu:525-543): any hit of every queued shadow ray
+    ff_paths P;
+    __device__ __forceinline__ void load(uint32_t k, uint32_t& s, Bvh& B, const float*& tris, f3& o, f3& d,
+                                         bool& two) const {
+        s = P.q_shadow[k];
+        const ff_dev_scene& S = P.reqs[P.preq[s / 3]].S;
+        B = bvh_of(S);
+        tris = S.tris;
+        o = xyz(P.sray[2 * s]);
+        d = xyz(P.sray[2 * s + 1]);
+        two = false;
+    }
+    __device__ __forceinline__ void hit(uint32_t, const Hit&) const {}
+    __device__ __forceinline__ void occluded(uint32_t s, bool occ) const { P.vis[s] = occ ? 0 : 1; }
+};
+struct RaySrc {  // a plain ray array (ff_trace_closest / ff_trace_any)
+    ff_dev_scene S;
+    const float4* rays;
+    float4* hits;
+    uint8_t* occ;
+    int two_sided;
+    __device__ __forceinline__ void load(uint32_t k, uint32_t& id, Bvh& B, const float*& tris, f3& o, f3& d,
+                                         bool& two) const {
+        id = k;
+        B = bvh_of(S);
+        tris = S.tris;
+        o = xyz(rays[2 * k]);
+        d = xyz(rays[2 * k + 1]);
+        two = two_sided != 0;
+    }
+    __device__ __forceinline__ void hit(uint32_t id, const Hit& h) const {
+        hits[id] = make_float4(h.t, h.u, h.v, __int_as_float(h.id));
+    }
+    __device__ __forceinline__ void occluded(uint32_t id, bool o) const { occ[id] = o ? 1 : 0; }
+};
 
-// ================================================================== shadow
-// wf_shadow (CRT/kernel.cu:525-543): any-hit of every queued shadow ray.
-template <int BS, int LDS>
-__global__ __launch_bounds__(BS) void k_shadow(ff_paths P) {
-    __shared__ int smem[LDS * BS];
+// Persistent traversal.  The grid holds only the lanes the chip keeps
+// resident; a lane whose ray is finished takes the next queue entry from a
+// shared counter (per wave: one ballot and one atomic for all idle lanes), so
+// a wave no longer lasts as long as its slowest ray and no second round of
+// waves is needed.  A wave refills once REFILL of its lanes are idle (or all
+// are), trading idle lanes against refill stalls.  Every wave leaves the loop
+// once the counter has passed n and its own rays are finished.
+template <int BS, int LDS, int REFILL, bool ANY, class Src>
+__device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint32_t* fetch, int32_t* ovf,
+                                                unsigned long long* cnt, int* smem) {
     Stack st;
     st.lds = &smem[threadIdx.x];
     st.lds_stride = BS;
     st.lds_entries = LDS;
-    st.ovf = P.ovf + gtid();
+    st.ovf = ovf + gtid();
     st.ovf_stride = FF_TRAV_THREADS;
-    const uint32_t n = P.counters[C_SHADOW];
-    uint32_t cn = 0, ct = 0;
-    for (uint32_t i = gtid(); i < n; i += gstride()) {
-        uint32_t s = P.q_shadow[i];
-        const ff_dev_scene& S = P.reqs[P.preq[s / 3]].S;
-        float4 r0 = P.sray[2 * s], r1 = P.sray[2 * s + 1];
-        bool occ = any_hit(bvh_of(S), S.tris, xyz(r0), xyz(r1), st, cn, ct);
-        P.vis[s] = occ ? 0 : 1;
+    uint32_t cn = 0, ct = 0, id = 0;
+    bool busy = false, more = true, two = false;
+    Bvh B;
+    B.nodes = nullptr; B.leaf_count = nullptr; B.root = nullptr;
+    const float* tris = nullptr;
+    TravRay R;
+    Hit h = no_hit();
+    for (;;) {
+        uint64_t idle = __ballot(!busy);
+        uint32_t n_idle = (uint32_t)__popcll(idle);
+        if (more && n_idle >= (uint32_t)REFILL) {
+            uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+            int leader = __ffsll((unsigned long long)idle) - 1;
+            uint32_t base = 0;
+            if ((int)__lane_id() == leader) base = atomicAdd(fetch, n_idle);
+            base = __shfl(base, leader);
+            if (base + n_idle >= n) more = false;
+            if (!busy && base + rank < n) {
+                f3 o, d;
+                src.load(base + rank, id, B, tris, o, d, two);
+                h = no_hit();
+                busy = trav_begin(B, o, d, R, st, cn);
+                if (!busy) {
+                    if (ANY) src.occluded(id, false);
+                    else src.hit(id, h);
+                }
+            }
+        }
+        if (__ballot(busy) == 0) {
+            if (!more) break;
+            continue;
+        }
+        if (busy) {
+            if (ANY) {
+                int r = any_step(B, tris, R, st, cn, ct);
+                if (r != 0) {
+                    src.occluded(id, r == 1);
+                    busy = false;
+                }
+            } else if (!closest_step(B, tris, R, h, two, st, cn, ct)) {
+                src.hit(id, h);
+                busy = false;
+            }
+        }
     }
-    wave_sum_add(&P.stats[S_SHD_NODES], cn);
-    wave_sum_add(&P.stats[S_SHD_TRIS], ct);
+    if (cnt) {
+        wave_sum_add(&cnt[0], cn);
+        wave_sum_add(&cnt[1], ct);
+    }
+}
+
+template <int BS, int LDS, int RF>
+__global__ __launch_bounds__(BS) void k_extend(ff_paths P, int q) {
+    __shared__ int smem[LDS * BS];
+    ExtSrc src{P, P.q_ext[q]};
+    trav_persistent<BS, LDS, RF, false>(src, P.counters[q == 0 ? C_EXT0 : C_EXT1], &P.counters[C_FETCH_EXT], P.ovf,
+                                        &P.stats[S_EXT_NODES], smem);
+}
+template <int BS, int LDS, int RF>
+__global__ __launch_bounds__(BS) void k_shadow(ff_paths P) {
+    __shared__ int smem[LDS * BS];
+    ShadowSrc src{P};
+    trav_persistent<BS, LDS, RF, true>(src, P.counters[C_SHADOW], &P.counters[C_FETCH_SHD], P.ovf,
+                                       &P.stats[S_SHD_NODES], smem);
 }
 
 // counters consumed by the previous stages are cleared by a kernel (graph-safe)
@@ -445,47 +539,20 @@ __global__ void k_resolve_layers(const float4* layers, uint32_t K, uint32_t WH, 
 }
 
 // ======================================================== standalone stages
-template <int BS, int LDS>
+template <int BS, int LDS, int RF>
 __global__ __launch_bounds__(BS) void k_trace_closest(ff_dev_scene S, const float4* rays, uint32_t n,
-                                                      float4* hits, int two_sided, int32_t* ovf,
+                                                      float4* hits, int two_sided, uint32_t* fetch, int32_t* ovf,
                                                       unsigned long long* counts) {
     __shared__ int smem[LDS * BS];
-    Stack st;
-    st.lds = &smem[threadIdx.x];
-    st.lds_stride = BS;
-    st.lds_entries = LDS;
-    st.ovf = ovf + gtid();
-    st.ovf_stride = FF_TRAV_THREADS;
-    uint32_t cn = 0, ct = 0;
-    for (uint32_t i = gtid(); i < n; i += gstride()) {
-        float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
-        Hit h = closest_hit(bvh_of(S), S.tris, xyz(r0), xyz(r1), two_sided != 0, st, cn, ct);
-        hits[i] = make_float4(h.t, h.u, h.v, __int_as_float(h.id));
-    }
-    if (counts) {
-        wave_sum_add(&counts[0], cn);
-        wave_sum_add(&counts[1], ct);
-    }
+    RaySrc src{S, rays, hits, nullptr, two_sided};
+    trav_persistent<BS, LDS, RF, false>(src, n, fetch, ovf, counts, smem);
 }
-template <int BS, int LDS>
+template <int BS, int LDS, int RF>
 __global__ __launch_bounds__(BS) void k_trace_any(ff_dev_scene S, const float4* rays, uint32_t n, uint8_t* occ,
-                                                  int32_t* ovf, unsigned long long* counts) {
+                                                  uint32_t* fetch, int32_t* ovf, unsigned long long* counts) {
     __shared__ int smem[LDS * BS];
-    Stack st;
-    st.lds = &smem[threadIdx.x];
-    st.lds_stride = BS;
-    st.lds_entries = LDS;
-    st.ovf = ovf + gtid();
-    st.ovf_stride = FF_TRAV_THREADS;
-    uint32_t cn = 0, ct = 0;
-    for (uint32_t i = gtid(); i < n; i += gstride()) {
-        float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
-        occ[i] = any_hit(bvh_of(S), S.tris, xyz(r0), xyz(r1), st, cn, ct) ? 1 : 0;
-    }
-    if (counts) {
-        wave_sum_add(&counts[0], cn);
-        wave_sum_add(&counts[1], ct);
-    }
+    RaySrc src{S, rays, nullptr, occ, 0};
+    trav_persistent<BS, LDS, RF, true>(src, n, fetch, ovf, counts, smem);
 }
 __global__ void k_gen_rays(ff_dev_camera cam, int32_t W, int32_t H, uint32_t seed, uint32_t sample,
                            float4* rays) {
@@ -631,23 +698,61 @@ inline uint32_t grid_for(uint64_t n, uint32_t cap) {
     return (uint32_t)(g < cap ? g : cap);
 }
 
-// traversal variants (block size, LDS stack entries) available to the autotuner
-template <int BS, int LDS>
+// Workgroups of a traversal kernel that the current device keeps resident:
+// CUs x blocks per CU at the kernel's occupancy, capped so every lane has its
+// overflow stack column.  Cached per (kernel, device).
+template <class K>
+uint32_t resident_grid(K kernel, int bs) {
+    static std::mutex mu;
+    static std::map<std::pair<const void*, int>, uint32_t> cache;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lock(mu);
+    auto key = std::make_pair((const void*)kernel, dev);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int nb = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, bs, 0) != hipSuccess || nb < 1) nb = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
+    uint64_t g = (uint64_t)nb * (uint64_t)cus, cap = FF_TRAV_THREADS / bs;
+    if (g > cap) g = cap;
+    cache[key] = (uint32_t)g;
+    return (uint32_t)g;
+}
+
+// traversal variants (block size, LDS stack entries, refill threshold)
+#define FF_TRAV_VARIANT_LIST(X) \
+    X(0, 256, 16, 32)           \
+    X(1, 256, 16, 16)           \
+    X(2, 256, 16, 64)           \
+    X(3, 128, 16, 32)           \
+    X(4, 256, 8, 32)            \
+    X(5, 256, 24, 32)           \
+    X(6, 64, 32, 32)            \
+    X(7, 128, 24, 32)
+
+template <int BS, int LDS, int RF>
 hipError_t trav_launch(int which, const ff_paths& P, int q, hipStream_t s) {
-    dim3 g(FF_TRAV_THREADS / BS), b(BS);
-    if (which == 0) hipLaunchKernelGGL((k_extend<BS, LDS>), g, b, 0, s, P, q);
-    else hipLaunchKernelGGL((k_shadow<BS, LDS>), g, b, 0, s, P);
+    if (which == 0) {
+        uint32_t g = resident_grid(k_extend<BS, LDS, RF>, BS);
+        hipLaunchKernelGGL((k_extend<BS, LDS, RF>), dim3(g), dim3(BS), 0, s, P, q);
+    } else if (which == 1) {
+        uint32_t g = resident_grid(k_shadow<BS, LDS, RF>, BS);
+        hipLaunchKernelGGL((k_shadow<BS, LDS, RF>), dim3(g), dim3(BS), 0, s, P);
+    } else {  // grid sizes only (before a graph capture)
+        (void)resident_grid(k_extend<BS, LDS, RF>, BS);
+        (void)resident_grid(k_shadow<BS, LDS, RF>, BS);
+    }
     return hipGetLastError();
 }
 
 hipError_t trav_dispatch(int which, int variant, const ff_paths& P, int q, hipStream_t s) {
     switch (variant) {
-        case 1: return trav_launch<128, 16>(which, P, q, s);
-        case 2: return trav_launch<256, 8>(which, P, q, s);
-        case 3: return trav_launch<256, 24>(which, P, q, s);
-        case 4: return trav_launch<128, 24>(which, P, q, s);
-        case 5: return trav_launch<64, 32>(which, P, q, s);
-        default: return trav_launch<256, 16>(which, P, q, s);
+#define FF_TRAV_CASE(v, bs, lds, rf) \
+    case v: return trav_launch<bs, lds, rf>(which, P, q, s);
+        FF_TRAV_VARIANT_LIST(FF_TRAV_CASE)
+#undef FF_TRAV_CASE
+        default: return hipErrorInvalidValue;
     }
 }
 
@@ -656,9 +761,16 @@ hipError_t trav_dispatch(int which, int variant, const ff_paths& P, int q, hipSt
 // ===================================================================== host
 namespace ffk {
 
-const int kTravVariants = 6;
-const int kTravLds[kTravVariants] = {16, 16, 8, 24, 24, 32};
-const int kTravBlock[kTravVariants] = {256, 128, 256, 256, 128, 64};
+const int kTravVariants = 8;
+#define FF_TRAV_BS(v, bs, lds, rf) bs,
+#define FF_TRAV_LDS(v, bs, lds, rf) lds,
+#define FF_TRAV_RF(v, bs, lds, rf) rf,
+const int kTravBlock[kTravVariants] = {FF_TRAV_VARIANT_LIST(FF_TRAV_BS)};
+const int kTravLds[kTravVariants] = {FF_TRAV_VARIANT_LIST(FF_TRAV_LDS)};
+const int kTravRefill[kTravVariants] = {FF_TRAV_VARIANT_LIST(FF_TRAV_RF)};
+#undef FF_TRAV_BS
+#undef FF_TRAV_LDS
+#undef FF_TRAV_RF
 
 hipError_t launch_batch_begin(const ff_paths& P, int set, uint32_t value, hipStream_t s) {
     hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(64), 0, s, P.bstate, P.counters, set, value);
@@ -672,6 +784,10 @@ hipError_t launch_extend(const ff_paths& P, int q, int variant, hipStream_t s) {
     return trav_dispatch(0, variant, P, q, s);
 }
 hipError_t launch_shadow(const ff_paths& P, int variant, hipStream_t s) { return trav_dispatch(1, variant, P, 0, s); }
+hipError_t prepare_traversal(int variant) {
+    ff_paths none{};
+    return trav_dispatch(2, variant, none, 0, nullptr);
+}
 hipError_t launch_reset(uint32_t* counters, uint32_t mask, hipStream_t s) {
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, counters, mask);
     return hipGetLastError();
@@ -704,15 +820,20 @@ hipError_t launch_resolve_layers(const float4* layers, uint32_t K, uint32_t WH, 
     return hipGetLastError();
 }
 hipError_t launch_trace_closest(const ff_dev_scene& S, const float4* rays, uint32_t n, float4* hits,
-                                int two_sided, int32_t* ovf, unsigned long long* counts, hipStream_t s) {
-    hipLaunchKernelGGL((k_trace_closest<256, 16>), dim3(FF_TRAV_THREADS / 256), dim3(256), 0, s, S, rays, n,
-                       hits, two_sided, ovf, counts);
+                                int two_sided, uint32_t* counters, int32_t* ovf, unsigned long long* counts,
+                                hipStream_t s) {
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, counters, 1u << C_FETCH_TRACE);
+    uint32_t g = resident_grid(k_trace_closest<256, 16, 32>, 256);
+    hipLaunchKernelGGL((k_trace_closest<256, 16, 32>), dim3(g), dim3(256), 0, s, S, rays, n, hits, two_sided,
+                       counters + C_FETCH_TRACE, ovf, counts);
     return hipGetLastError();
 }
-hipError_t launch_trace_any(const ff_dev_scene& S, const float4* rays, uint32_t n, uint8_t* occ, int32_t* ovf,
-                            unsigned long long* counts, hipStream_t s) {
-    hipLaunchKernelGGL((k_trace_any<256, 16>), dim3(FF_TRAV_THREADS / 256), dim3(256), 0, s, S, rays, n, occ,
-                       ovf, counts);
+hipError_t launch_trace_any(const ff_dev_scene& S, const float4* rays, uint32_t n, uint8_t* occ, uint32_t* counters,
+                            int32_t* ovf, unsigned long long* counts, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, counters, 1u << C_FETCH_TRACE);
+    uint32_t g = resident_grid(k_trace_any<256, 16, 32>, 256);
+    hipLaunchKernelGGL((k_trace_any<256, 16, 32>), dim3(g), dim3(256), 0, s, S, rays, n, occ,
+                       counters + C_FETCH_TRACE, ovf, counts);
     return hipGetLastError();
 }
 hipError_t launch_gen_rays(const ff_dev_camera& cam, int W, int H, uint32_t seed, uint32_t sample, float4* rays,

@@ -31,7 +31,10 @@
 #define C_EXT1 1
 #define C_SHADOW 2
 #define C_MAT0 3
-#define C_COUNT 8
+#define C_FETCH_EXT 7    // persistent traversal work counters
+#define C_FETCH_SHD 8
+#define C_FETCH_TRACE 9
+#define C_COUNT 16
 
 // ray statistics
 #define S_PRIMARY 0

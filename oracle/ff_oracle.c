@@ -43,6 +43,7 @@
 #define F_1_2PI 0.159154943091895335769f
 #define K_EPS 1e-6f
 #define K_HUGE 1e32f
+#define D_PI 3.14159265358979323846
 
 /* RNG dimension map.  bounce 0 = camera; bounce L = logic + shading at the
  * L-th vertex. */
@@ -928,11 +929,45 @@ static inline float env_pdf_c(const rctx *c, v3 wi) {
 
 typedef struct { double prim, ext, shd; } pstat;
 
+/* Path tape for the differentiable render: the events of one path in the
+ * order their radiance is added.  An escape adds beta * Le(d); a vertex adds
+ * beta * (its visibility-gated NEE terms) and then multiplies beta by
+ * f(dir) * cos / pdf.  Everything but f and Le (directions, cosines, pdfs,
+ * MIS weights, visibilities, gating) is recorded from the float forward and
+ * held fixed: the differentiable estimator detaches its sampling. */
+#define EV_ESCAPE 0
+#define EV_VERTEX 1
+typedef struct {
+    int kind, type, at, rt;
+    float u, v;                        /* surface texture coordinates */
+    float alb[3], rough, metal, fres[3]; /* material constants (untextured values) */
+    float n[3], wo[3];
+    int a_spec, a_diff, s_on, d_on, ext;
+    float wi[3], ndwi;                 /* light sample (escape: the escaping direction) */
+    double swt, dwt, lp;
+    float wi_s[3], nd_s;
+    double wt_s, sp_s;
+    float wi_d[3], nd_d;
+    double wt_d, dp;
+    float dir[3], cos_ext, pdf_ext;    /* continuation */
+} or_event;
+typedef struct { or_event *ev; int n, cap; } tape_t;
+static or_event *tape_push(tape_t *t) {
+    if (t->n == t->cap) {
+        t->cap = t->cap ? 2 * t->cap : 16;
+        t->ev = (or_event *)realloc(t->ev, sizeof(or_event) * (size_t)t->cap);
+    }
+    or_event *e = &t->ev[t->n++];
+    memset(e, 0, sizeof(*e));
+    return e;
+}
+static void put3(float *dst, v3 a) { dst[0] = a.x; dst[1] = a.y; dst[2] = a.z; }
+
 /* One sample of one pixel through the wavefront state machine of
  * wf_generate / wf_extend / wf_logic / wf_mat_* / wf_shadow
  * (CRT/kernel.cu:297-651), with deferred visibility of the BRDF-sampled
  * MIS rays (same sums as BRDF_L, CRT/dMaterial.cu:363-458). */
-static v3 trace_path(const rctx *c, int32_t px, int32_t py, int32_t sample, pstat *ps) {
+static v3 trace_path(const rctx *c, int32_t px, int32_t py, int32_t sample, pstat *ps, tape_t *tape) {
     const or_scene *s = c->s;
     const or_params *p = c->p;
     uint32_t pixel = (uint32_t)py * (uint32_t)p->width + (uint32_t)px;
@@ -969,7 +1004,14 @@ static v3 trace_path(const rctx *c, int32_t px, int32_t py, int32_t sample, psta
             int front = vdot(d, ng) < 0.f;
             if (inside && (int)M[0] != OR_MAT_DIELECTRIC && !front) term = 1; /* absorbed */
         }
-        if (!h.found && prev_delta) Lp = vadd(Lp, vmul(beta, env_L(c, d)));
+        if (!h.found && prev_delta) {
+            Lp = vadd(Lp, vmul(beta, env_L(c, d)));
+            if (tape) {
+                or_event *e = tape_push(tape);
+                e->kind = EV_ESCAPE;
+                put3(e->wi, d);
+            }
+        }
         if (len > 1) {
             if (visL) {
                 v3 lb = A;
@@ -1052,19 +1094,23 @@ static v3 trace_path(const rctx *c, int32_t px, int32_t py, int32_t sample, psta
             /* light-sample term of BRDF_L (CRT/dMaterial.cu:376-402) */
             v3 Li = env_L(c, wi);
             float ndwi = vdot(n, wi);
+            int a_spec = 0, a_diff = 0;
+            double t_swt = 0, t_dwt = 0, t_lp = 0, t_wt_s = 0, t_sp_s = 0, t_wt_d = 0;
+            float t_nd_s = 0, t_nd_d = 0;
             if (ndwi > 0.f) {
                 double sp = do_spec ? (double)spec_pdf(n, wi, wo, rough) : 0.0;
                 double lp = (double)env_pdf_c(c, wi);
                 /* lp is infinite where sin(acos(wi.z)) == 0; the term's limit there is 0 */
                 if (lp > 0.f && lp < INFINITY && !vzero(Li)) {
                     double dwt = power_heur(lp, diff_pdf), swt = power_heur(lp, sp);
+                    t_swt = swt; t_dwt = dwt; t_lp = lp;
                     if (do_spec) {
                         v3 f = vscl(spec_f(n, wi, wo, rough, metal, alb, fres), ndwi);
-                        if (!vzero(f)) A = vadd(A, vdiv(vmul(vscl(f, (float)swt), Li), (float)lp));
+                        if (!vzero(f)) { A = vadd(A, vdiv(vmul(vscl(f, (float)swt), Li), (float)lp)); a_spec = 1; }
                     }
                     if (do_diff) {
                         v3 f = vscl(diff_f(wi, wo, metal, alb, fres), ndwi);
-                        if (!vzero(f)) A = vadd(A, vdiv(vmul(vscl(f, (float)dwt), Li), (float)lp));
+                        if (!vzero(f)) { A = vadd(A, vdiv(vmul(vscl(f, (float)dwt), Li), (float)lp)); a_diff = 1; }
                     }
                 }
             }
@@ -1084,6 +1130,7 @@ static v3 trace_path(const rctx *c, int32_t px, int32_t py, int32_t sample, psta
                     if (!vzero(f) && sp > 0.f && !vzero(L2)) {
                         Bs = vdiv(vscl(vmul(f, L2), (float)wt), (float)sp);
                         needS = 1;
+                        t_nd_s = nd; t_wt_s = wt; t_sp_s = sp;
                         so = vadd(P, vscl(wi_s, 0.001f));
                         sd = wi_s;
                     }
@@ -1099,6 +1146,7 @@ static v3 trace_path(const rctx *c, int32_t px, int32_t py, int32_t sample, psta
                     if (!vzero(f) && diff_pdf > 0.f && !vzero(L3)) {
                         Cd = vdiv(vscl(vmul(f, L3), (float)wt), (float)diff_pdf);
                         needD = 1;
+                        t_nd_d = nd; t_wt_d = wt;
                         /* the mixed material offsets this ray along wi_spec (CRT/dMaterial.cu:431) */
                         dd_o = vadd(P, vscl(type == OR_MAT_MIX ? wi_s : wi_d, 0.001f));
                         dd_d = wi_d;
@@ -1127,12 +1175,35 @@ static v3 trace_path(const rctx *c, int32_t px, int32_t py, int32_t sample, psta
             next_o = P;
             next_d = dir;
             prev_delta = 0;
+            if (tape) {
+                or_event *e = tape_push(tape);
+                e->kind = EV_VERTEX;
+                e->type = type; e->at = at; e->rt = rt;
+                e->u = uvv.x; e->v = uvv.y;
+                e->alb[0] = M[1]; e->alb[1] = M[2]; e->alb[2] = M[3];
+                e->rough = M[4]; e->metal = metal;
+                put3(e->fres, fres);
+                put3(e->n, n); put3(e->wo, wo);
+                /* visibility is filled in after the shadow stage below */
+                e->a_spec = a_spec; e->a_diff = a_diff;
+                put3(e->wi, wi); e->ndwi = ndwi; e->swt = t_swt; e->dwt = t_dwt; e->lp = t_lp;
+                e->s_on = needS; put3(e->wi_s, wi_s); e->nd_s = t_nd_s; e->wt_s = t_wt_s; e->sp_s = t_sp_s;
+                e->d_on = needD; put3(e->wi_d, wi_d); e->nd_d = t_nd_d; e->wt_d = t_wt_d; e->dp = diff_pdf;
+                e->ext = !(vzero(fext) || pdf_ext == 0.f);
+                put3(e->dir, dir); e->cos_ext = cos_ext; e->pdf_ext = pdf_ext;
+            }
         }
         /* ---------------- shadow (wf_shadow) ---------------- */
         visL = visS = visD = 0;
         if (needL) { visL = !any_ref(s->nodes, c->tri, lo, ld, 0, NULL, NULL); ps->shd += 1; }
         if (needS) { visS = !any_ref(s->nodes, c->tri, so, sd, 0, NULL, NULL); ps->shd += 1; }
         if (needD) { visD = !any_ref(s->nodes, c->tri, dd_o, dd_d, 0, NULL, NULL); ps->shd += 1; }
+        if (tape && type != OR_MAT_DIELECTRIC) {
+            or_event *e = &tape->ev[tape->n - 1];
+            e->a_spec &= visL; e->a_diff &= visL;
+            e->s_on = e->s_on && visL && visS;
+            e->d_on = e->d_on && visL && visD;
+        }
         /* ---------------- extend (wf_extend) ---------------- */
         o = next_o;
         d = next_d;
@@ -1158,7 +1229,7 @@ static void *render_worker(void *arg) {
         for (int32_t x = 0; x < p->width; x++) {
             for (int k = 0; k < K; k++) layer[k] = V3(0, 0, 0);
             for (int32_t smp = 0; smp < p->spp; smp++) {
-                v3 L = trace_path(c, x, row, smp, &ps);
+                v3 L = trace_path(c, x, row, smp, &ps, NULL);
                 layer[smp % K] = vadd(layer[smp % K], L);
                 paths += 1;
             }
@@ -1215,4 +1286,456 @@ void or_render(const or_scene *s, const or_params *p, float *film, or_stats *st)
     if (st) *st = c.st;
     free(c.tri); free(c.tnrm); free(c.tuv); free(c.tmat);
     free(c.my); free(c.mp); free(c.conds); free(c.pdf);
+}
+
+/* ===================================================== differentiable render
+ * The differentiable estimator of one path is the tape above evaluated with
+ * parameters theta: albedo / roughness textures (through f) and environment
+ * texels (through Le), everything else fixed at the recorded forward.
+ * or_diff_eval evaluates it in FP64 (finite differences use this);
+ * or_diff_adjoint differentiates it exactly in FP64 by reverse accumulation
+ * over the tape:
+ *   L = sum_k beta_k * g_k,   beta_{k+1} = beta_k * t_k,
+ *   gbar_k = delta * beta_k,  tbar_k = betabar_{k+1} * beta_k,
+ *   betabar_k = delta * g_k + betabar_{k+1} * t_k.
+ * Parity unpinned by the reference (it has no differentiation); pinned by
+ * central finite differences of or_diff_eval (tests/test_oracle_adjoint.py). */
+
+/* bilinear taps of or_bilinear (same indices, weights in FP64) */
+static void taps_of(int32_t w, int32_t h, float u, float v, size_t idx[4], double wt[4]) {
+    float uu = u - floorf(u), vv = v - floorf(v);
+    float x = uu * (float)w - 0.5f, y = vv * (float)h - 0.5f;
+    float fx = floorf(x), fy = floorf(y);
+    float a = x - fx, b = y - fy;
+    int i0 = (int)fx, j0 = (int)fy;
+    i0 = ((i0 % w) + w) % w;
+    j0 = ((j0 % h) + h) % h;
+    int i1 = (i0 + 1) % w, j1 = (j0 + 1) % h;
+    idx[0] = (size_t)j0 * w + i0; idx[1] = (size_t)j0 * w + i1;
+    idx[2] = (size_t)j1 * w + i0; idx[3] = (size_t)j1 * w + i1;
+    double A = a, B = b;
+    wt[0] = (1.0 - A) * (1.0 - B); wt[1] = A * (1.0 - B); wt[2] = (1.0 - A) * B; wt[3] = A * B;
+}
+static void lookup_d(const float *tex, int32_t w, int32_t h, int32_t c, float u, float v, double *out) {
+    size_t idx[4];
+    double wt[4];
+    taps_of(w, h, u, v, idx, wt);
+    for (int k = 0; k < c; k++) {
+        double s = 0;
+        for (int q = 0; q < 4; q++) s += wt[q] * (double)tex[idx[q] * c + k];
+        out[k] = s;
+    }
+}
+static void env_uv(const float *dir, float *u, float *v) { sph_map(V3(dir[0], dir[1], dir[2]), u, v); }
+
+typedef struct { double f[3], da[3], dr[3]; } lobe_d;
+static double dot_d(const float *a, const double *b) { return (double)a[0] * b[0] + (double)a[1] * b[1] + (double)a[2] * b[2]; }
+static void half_d(const float *wo, const float *wi, double *h) {
+    double x = (double)wo[0] + wi[0], y = (double)wo[1] + wi[1], z = (double)wo[2] + wi[2];
+    double l = sqrt(x * x + y * y + z * z);
+    h[0] = x / l; h[1] = y / l; h[2] = z / l;
+}
+static double ddot(const float *a, const float *b) { return (double)a[0] * b[0] + (double)a[1] * b[1] + (double)a[2] * b[2]; }
+
+/* spec_f (CRT/dMaterial.cu:266-288) in FP64 with d/dalbedo (per channel) and d/droughness */
+static void spec_lobe_d(const or_event *e, const float *wi, double r, const double *alb, lobe_d *o) {
+    double h[3];
+    half_d(e->wo, wi, h);
+    double ndwi = fmax(ddot(e->n, wi), 0.0), ndwo = fmax(ddot(e->n, e->wo), 0.0);
+    double nh = fmax(0.0, dot_d(e->n, h));
+    double a2 = r * r * r * r, nh2 = nh * nh;
+    double q = nh2 * (a2 - 1.0) + 1.0;
+    double D = a2 / (D_PI * q * q);
+    double dD = 4.0 * r * r * r * (q - 2.0 * a2 * nh2) / (D_PI * q * q * q);
+    double k = (r + 1.0) * (r + 1.0) / 8.0, dk = (r + 1.0) / 4.0;
+    double d1 = ndwi * (1.0 - k) + k, d2 = ndwo * (1.0 - k) + k;
+    double g1 = ndwi / d1, g2 = ndwo / d2;
+    double dg1 = -ndwi * (1.0 - ndwi) / (d1 * d1), dg2 = -ndwo * (1.0 - ndwo) / (d2 * d2);
+    double G = g1 * g2, dG = (dg1 * g2 + g1 * dg2) * dk;
+    double p = pow(1.0 - fmax(0.0, dot_d(wi, h)), 5.0);
+    double den = fmax(4.0 * ndwo * ndwi, 0.001);
+    for (int c = 0; c < 3; c++) {
+        double f0 = (double)e->fres[c] + (alb[c] - (double)e->fres[c]) * e->metal;
+        double F = f0 + (1.0 - f0) * p;
+        o->f[c] = F * D * G / den;
+        o->da[c] = e->metal * (1.0 - p) * D * G / den;
+        o->dr[c] = F * (dD * G + D * dG) / den;
+    }
+}
+/* diff_f (CRT/dMaterial.cu:164-178) in FP64 with d/dalbedo */
+static void diff_lobe_d(const or_event *e, const float *wi, const double *alb, lobe_d *o) {
+    double h[3];
+    half_d(e->wo, wi, h);
+    double p = pow(1.0 - fmax(0.0, dot_d(wi, h)), 5.0);
+    double m = e->metal;
+    for (int c = 0; c < 3; c++) {
+        double f0 = (double)e->fres[c] + (alb[c] - (double)e->fres[c]) * m;
+        double F = f0 + (1.0 - f0) * p;
+        o->f[c] = (1.0 - F) * (1.0 - m) * alb[c] / D_PI;
+        o->da[c] = (1.0 - m) / D_PI * ((1.0 - F) - alb[c] * m * (1.0 - p));
+        o->dr[c] = 0.0;
+    }
+}
+static void lobe_add(lobe_d *a, const lobe_d *b) {
+    for (int c = 0; c < 3; c++) { a->f[c] += b->f[c]; a->da[c] += b->da[c]; a->dr[c] += b->dr[c]; }
+}
+
+typedef struct {
+    const or_scene *s;
+    const float *const *tex;   /* evaluation textures (same shapes as the scene's) */
+    const float *env;          /* evaluation environment [h][w][3] */
+} dparams;
+
+static void mat_at(const dparams *P, const or_event *e, double alb[3], double *rough, int *rough_live) {
+    const or_scene *s = P->s;
+    alb[0] = e->alb[0]; alb[1] = e->alb[1]; alb[2] = e->alb[2];
+    *rough = e->rough;
+    *rough_live = 0;
+    if (e->at >= 0) lookup_d(P->tex[e->at], s->tex_w[e->at], s->tex_h[e->at], 3, e->u, e->v, alb);
+    if (e->rt >= 0) {
+        double t;
+        lookup_d(P->tex[e->rt], s->tex_w[e->rt], s->tex_h[e->rt], 1, e->u, e->v, &t);
+        *rough_live = t > 0.01;
+        *rough = t > 0.01 ? t : 0.01;
+    }
+}
+static void env_d(const dparams *P, const float *dir, double out[3]) {
+    float u, v;
+    env_uv(dir, &u, &v);
+    lookup_d(P->env, P->s->env_w, P->s->env_h, 3, u, v, out);
+}
+
+/* per-event quantities of the evaluation: NEE radiance g, transmission t, and
+ * the partial derivatives the adjoint needs */
+typedef struct {
+    double g[3], t[3];
+    double g_da[3], g_dr[3], t_da[3], t_dr[3]; /* d g_c / d albedo_c, d g_c / d rough, ... */
+    double kA[3], kS[3], kD[3];                /* d g_c / d Le_c at wi, wi_s, wi_d */
+    int rough_live;
+} evq;
+
+static void event_eval(const dparams *P, const or_event *e, evq *q) {
+    memset(q, 0, sizeof(*q));
+    if (e->kind == EV_ESCAPE) {
+        env_d(P, e->wi, q->g);
+        for (int c = 0; c < 3; c++) q->kA[c] = 1.0;
+        return;
+    }
+    double alb[3], r;
+    mat_at(P, e, alb, &r, &q->rough_live);
+    if (e->a_spec || e->a_diff) {
+        double Li[3];
+        env_d(P, e->wi, Li);
+        lobe_d l;
+        if (e->a_spec) {
+            spec_lobe_d(e, e->wi, r, alb, &l);
+            double s = (double)e->ndwi * e->swt / e->lp;
+            for (int c = 0; c < 3; c++) {
+                q->g[c] += l.f[c] * s * Li[c]; q->kA[c] += l.f[c] * s;
+                q->g_da[c] += l.da[c] * s * Li[c]; q->g_dr[c] += l.dr[c] * s * Li[c];
+            }
+        }
+        if (e->a_diff) {
+            diff_lobe_d(e, e->wi, alb, &l);
+            double s = (double)e->ndwi * e->dwt / e->lp;
+            for (int c = 0; c < 3; c++) {
+                q->g[c] += l.f[c] * s * Li[c]; q->kA[c] += l.f[c] * s;
+                q->g_da[c] += l.da[c] * s * Li[c];
+            }
+        }
+    }
+    if (e->s_on) {
+        double L2[3];
+        env_d(P, e->wi_s, L2);
+        lobe_d l;
+        spec_lobe_d(e, e->wi_s, r, alb, &l);
+        double s = (double)e->nd_s * e->wt_s / e->sp_s;
+        for (int c = 0; c < 3; c++) {
+            q->g[c] += l.f[c] * s * L2[c]; q->kS[c] = l.f[c] * s;
+            q->g_da[c] += l.da[c] * s * L2[c]; q->g_dr[c] += l.dr[c] * s * L2[c];
+        }
+    }
+    if (e->d_on) {
+        double L3[3];
+        env_d(P, e->wi_d, L3);
+        lobe_d l;
+        diff_lobe_d(e, e->wi_d, alb, &l);
+        double s = (double)e->nd_d * e->wt_d / e->dp;
+        for (int c = 0; c < 3; c++) {
+            q->g[c] += l.f[c] * s * L3[c]; q->kD[c] = l.f[c] * s;
+            q->g_da[c] += l.da[c] * s * L3[c];
+        }
+    }
+    if (e->ext) {
+        lobe_d f, l;
+        memset(&f, 0, sizeof(f));
+        if (e->type == OR_MAT_MIX || e->type == OR_MAT_CONDUCTOR) { spec_lobe_d(e, e->dir, r, alb, &l); lobe_add(&f, &l); }
+        if (e->type == OR_MAT_MIX || e->type == OR_MAT_DIFFUSE) { diff_lobe_d(e, e->dir, alb, &l); lobe_add(&f, &l); }
+        double s = (double)e->cos_ext / (double)e->pdf_ext;
+        for (int c = 0; c < 3; c++) { q->t[c] = f.f[c] * s; q->t_da[c] = f.da[c] * s; q->t_dr[c] = f.dr[c] * s; }
+    }
+}
+
+static double tape_eval(const dparams *P, const tape_t *T, double L[3]) {
+    double beta[3] = {1, 1, 1};
+    L[0] = L[1] = L[2] = 0;
+    for (int k = 0; k < T->n; k++) {
+        evq q;
+        event_eval(P, &T->ev[k], &q);
+        for (int c = 0; c < 3; c++) L[c] += beta[c] * q.g[c];
+        if (T->ev[k].kind == EV_VERTEX && T->ev[k].ext)
+            for (int c = 0; c < 3; c++) beta[c] *= q.t[c];
+    }
+    return L[0] + L[1] + L[2];
+}
+
+typedef struct {
+    double *g_env;     /* [h][w][3] */
+    double **g_tex;    /* per texture [h][w][c] (NULL: not wanted) */
+} dgrads;
+
+static void scatter_env(const or_scene *s, double *g, const float *dir, const double *val) {
+    if (!g) return;
+    float u, v;
+    size_t idx[4];
+    double wt[4];
+    env_uv(dir, &u, &v);
+    taps_of(s->env_w, s->env_h, u, v, idx, wt);
+    for (int q = 0; q < 4; q++)
+        for (int c = 0; c < 3; c++) g[idx[q] * 3 + c] += wt[q] * val[c];
+}
+static void scatter_tex(const or_scene *s, double *g, int t, float u, float v, const double *val, int ch) {
+    if (!g) return;
+    size_t idx[4];
+    double wt[4];
+    taps_of(s->tex_w[t], s->tex_h[t], u, v, idx, wt);
+    for (int q = 0; q < 4; q++)
+        for (int c = 0; c < ch; c++) g[idx[q] * ch + c] += wt[q] * val[c];
+}
+
+static void tape_adjoint(const dparams *P, const tape_t *T, const double delta[3], dgrads *G) {
+    int n = T->n;
+    if (n == 0) return;
+    evq *Q = (evq *)malloc(sizeof(evq) * (size_t)n);
+    double(*B)[3] = (double(*)[3])malloc(sizeof(double) * 3 * (size_t)(n + 1));
+    B[0][0] = B[0][1] = B[0][2] = 1;
+    for (int k = 0; k < n; k++) {
+        event_eval(P, &T->ev[k], &Q[k]);
+        int ext = T->ev[k].kind == EV_VERTEX && T->ev[k].ext;
+        for (int c = 0; c < 3; c++) B[k + 1][c] = ext ? B[k][c] * Q[k].t[c] : B[k][c];
+    }
+    double bbar[3] = {0, 0, 0};
+    for (int k = n - 1; k >= 0; k--) {
+        const or_event *e = &T->ev[k];
+        const evq *q = &Q[k];
+        int ext = e->kind == EV_VERTEX && e->ext;
+        double gbar[3], tbar[3];
+        for (int c = 0; c < 3; c++) {
+            gbar[c] = delta[c] * B[k][c];
+            tbar[c] = ext ? bbar[c] * B[k][c] : 0.0;
+            bbar[c] = delta[c] * q->g[c] + (ext ? bbar[c] * q->t[c] : 0.0);
+        }
+        if (e->kind == EV_ESCAPE) {
+            scatter_env(P->s, G->g_env, e->wi, gbar);
+            continue;
+        }
+        double v[3];
+        for (int c = 0; c < 3; c++) v[c] = gbar[c] * q->kA[c];
+        if (e->a_spec || e->a_diff) scatter_env(P->s, G->g_env, e->wi, v);
+        for (int c = 0; c < 3; c++) v[c] = gbar[c] * q->kS[c];
+        if (e->s_on) scatter_env(P->s, G->g_env, e->wi_s, v);
+        for (int c = 0; c < 3; c++) v[c] = gbar[c] * q->kD[c];
+        if (e->d_on) scatter_env(P->s, G->g_env, e->wi_d, v);
+        if (e->at >= 0 && G->g_tex) {
+            for (int c = 0; c < 3; c++) v[c] = gbar[c] * q->g_da[c] + tbar[c] * q->t_da[c];
+            scatter_tex(P->s, G->g_tex[e->at], e->at, e->u, e->v, v, 3);
+        }
+        if (e->rt >= 0 && G->g_tex && q->rough_live) {
+            double dr = 0;
+            for (int c = 0; c < 3; c++) dr += gbar[c] * q->g_dr[c] + tbar[c] * q->t_dr[c];
+            scatter_tex(P->s, G->g_tex[e->rt], e->rt, e->u, e->v, &dr, 1);
+        }
+    }
+    free(Q);
+    free(B);
+}
+
+/* ---- threaded drivers ---- */
+typedef struct {
+    rctx *c;
+    const dparams *P;
+    const float *d_image;  /* [H][W][3] d loss / d (mean image) */
+    double *image;         /* [H][W][3] mean of the FP64 estimator (may be NULL) */
+    tape_t *tapes;         /* [H][W][spp] recorded tapes (or NULL) */
+    int record;
+    size_t env_n;
+    double *const *tex_g;  /* final gradients */
+    double *env_g;
+    pthread_mutex_t mu;
+} dctx;
+
+static void *diff_worker(void *arg) {
+    dctx *D = (dctx *)arg;
+    rctx *c = D->c;
+    const or_params *p = c->p;
+    const or_scene *s = c->s;
+    dgrads G = {NULL, NULL};
+    if (D->d_image) {
+        if (D->env_g) G.g_env = (double *)calloc(D->env_n * 3, sizeof(double));
+        if (D->tex_g) {
+            G.g_tex = (double **)calloc((size_t)s->n_tex, sizeof(double *));
+            for (int t = 0; t < s->n_tex; t++)
+                if (D->tex_g[t])
+                    G.g_tex[t] = (double *)calloc((size_t)s->tex_w[t] * s->tex_h[t] * s->tex_c[t], sizeof(double));
+        }
+    }
+    pstat ps = {0, 0, 0};
+    tape_t scratch = {NULL, 0, 0};
+    for (;;) {
+        pthread_mutex_lock(&c->mu);
+        int row = c->next_row++;
+        pthread_mutex_unlock(&c->mu);
+        if (row >= p->row_end) break;
+        for (int32_t x = 0; x < p->width; x++) {
+            size_t pix = (size_t)row * p->width + x;
+            double acc[3] = {0, 0, 0};
+            for (int32_t smp = 0; smp < p->spp; smp++) {
+                tape_t *T = D->record ? &D->tapes[pix * p->spp + smp] : &scratch;
+                T->n = 0;
+                trace_path(c, x, row, smp, &ps, T);
+                if (D->image) {
+                    double L[3];
+                    tape_eval(D->P, T, L);
+                    for (int k = 0; k < 3; k++) acc[k] += L[k];
+                }
+                if (D->d_image) {
+                    double dl[3];
+                    for (int k = 0; k < 3; k++) dl[k] = (double)D->d_image[pix * 3 + k] / (double)p->spp;
+                    tape_adjoint(D->P, T, dl, &G);
+                }
+            }
+            if (D->image)
+                for (int k = 0; k < 3; k++) D->image[pix * 3 + k] = acc[k] / (double)p->spp;
+        }
+    }
+    free(scratch.ev);
+    if (D->d_image) {
+        pthread_mutex_lock(&D->mu);
+        if (G.g_env)
+            for (size_t i = 0; i < D->env_n * 3; i++) D->env_g[i] += G.g_env[i];
+        if (G.g_tex)
+            for (int t = 0; t < s->n_tex; t++)
+                if (G.g_tex[t]) {
+                    size_t m = (size_t)s->tex_w[t] * s->tex_h[t] * s->tex_c[t];
+                    for (size_t i = 0; i < m; i++) D->tex_g[t][i] += G.g_tex[t][i];
+                    free(G.g_tex[t]);
+                }
+        pthread_mutex_unlock(&D->mu);
+        free(G.g_env);
+        free(G.g_tex);
+    }
+    return NULL;
+}
+
+static void rctx_init(rctx *c, const or_scene *s, const or_params *p) {
+    memset(c, 0, sizeof(*c));
+    c->s = s;
+    c->p = p;
+    size_t n = (size_t)s->n_tris;
+    c->tri = (float *)malloc(sizeof(float) * 9 * n);
+    c->tnrm = (float *)malloc(sizeof(float) * 9 * n);
+    c->tuv = (float *)malloc(sizeof(float) * 6 * n);
+    c->tmat = (int32_t *)malloc(sizeof(int32_t) * n);
+    for (size_t i = 0; i < n; i++) {
+        int32_t src = s->ordered[i];
+        memcpy(c->tri + i * 9, s->pos + (size_t)src * 9, 36);
+        memcpy(c->tnrm + i * 9, s->nrm + (size_t)src * 9, 36);
+        memcpy(c->tuv + i * 6, s->uv + (size_t)src * 6, 24);
+        c->tmat[i] = s->mat[src];
+    }
+    if (s->env_rgb) {
+        size_t wh = (size_t)s->env_w * s->env_h;
+        c->my = (double *)malloc(sizeof(double) * s->env_h);
+        c->mp = (double *)malloc(sizeof(double) * s->env_h);
+        c->conds = (double *)malloc(sizeof(double) * wh);
+        c->pdf = (float *)malloc(sizeof(float) * wh);
+        or_env_build(s->env_rgb, s->env_w, s->env_h, c->my, c->mp, c->conds, c->pdf, NULL);
+    }
+    c->next_row = p->row_begin;
+    pthread_mutex_init(&c->mu, NULL);
+}
+static void rctx_free(rctx *c) {
+    pthread_mutex_destroy(&c->mu);
+    free(c->tri); free(c->tnrm); free(c->tuv); free(c->tmat);
+    free(c->my); free(c->mp); free(c->conds); free(c->pdf);
+}
+static void run_diff(dctx *D) {
+    const or_params *p = D->c->p;
+    pthread_mutex_init(&D->mu, NULL);
+    int nt = p->n_threads > 0 ? p->n_threads : 1;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nt);
+    for (int i = 0; i < nt; i++) pthread_create(&th[i], NULL, diff_worker, D);
+    for (int i = 0; i < nt; i++) pthread_join(th[i], NULL);
+    free(th);
+    pthread_mutex_destroy(&D->mu);
+}
+
+void or_diff_adjoint(const or_scene *s, const or_params *p, const float *d_image, double *image,
+                     double *g_env, double *const *g_tex) {
+    rctx c;
+    rctx_init(&c, s, p);
+    dparams P = {s, s->tex, s->env_rgb};
+    dctx D;
+    memset(&D, 0, sizeof(D));
+    D.c = &c; D.P = &P; D.d_image = d_image; D.image = image;
+    D.env_n = (size_t)s->env_w * s->env_h;
+    D.env_g = g_env; D.tex_g = g_tex;
+    run_diff(&D);
+    rctx_free(&c);
+}
+
+struct or_tapes {
+    int32_t width, height, spp;
+    tape_t *t;
+};
+or_tapes *or_tapes_record(const or_scene *s, const or_params *p) {
+    rctx c;
+    rctx_init(&c, s, p);
+    dparams P = {s, s->tex, s->env_rgb};
+    or_tapes *T = (or_tapes *)calloc(1, sizeof(or_tapes));
+    T->width = p->width; T->height = p->height; T->spp = p->spp;
+    T->t = (tape_t *)calloc((size_t)p->width * p->height * p->spp, sizeof(tape_t));
+    dctx D;
+    memset(&D, 0, sizeof(D));
+    D.c = &c; D.P = &P; D.tapes = T->t; D.record = 1;
+    run_diff(&D);
+    rctx_free(&c);
+    return T;
+}
+void or_tapes_free(or_tapes *T) {
+    if (!T) return;
+    size_t n = (size_t)T->width * T->height * T->spp;
+    for (size_t i = 0; i < n; i++) free(T->t[i].ev);
+    free(T->t);
+    free(T);
+}
+/* FP64 mean image of the recorded paths evaluated with the given textures and
+ * environment (shapes as in the scene), rows [row_begin, row_end) */
+void or_tapes_eval(const or_tapes *T, const or_scene *s, const float *const *tex, const float *env,
+                   int32_t row_begin, int32_t row_end, double *image) {
+    dparams P = {s, tex, env};
+    for (int32_t y = row_begin; y < row_end; y++)
+        for (int32_t x = 0; x < T->width; x++) {
+            size_t pix = (size_t)y * T->width + x;
+            double acc[3] = {0, 0, 0}, L[3];
+            for (int32_t k = 0; k < T->spp; k++) {
+                tape_eval(&P, &T->t[pix * T->spp + k], L);
+                for (int c = 0; c < 3; c++) acc[c] += L[c];
+            }
+            for (int c = 0; c < 3; c++) image[pix * 3 + c] = acc[c] / (double)T->spp;
+        }
+}
+/* number of events of path (pixel, sample) */
+int32_t or_tapes_len(const or_tapes *T, int32_t pixel, int32_t sample) {
+    return T->t[(size_t)pixel * T->spp + sample].n;
 }

@@ -132,6 +132,20 @@ float or_fresnel_dielectric(float cos_i, float eta_i, float eta_t);
 /* ---- full render: film is [h][w][4] = (sum r, sum g, sum b, samples) ---- */
 void or_render(const or_scene *s, const or_params *p, float *film, or_stats *st);
 
+/* ---- differentiable render (FP64): detached-sampling estimator of each path
+ * as a function of the albedo / roughness textures and environment texels ----
+ * or_diff_adjoint: d_image [H][W][3] = d loss / d (mean image); gradients are
+ * ADDED to g_env [env_h][env_w][3] and g_tex[t] [h][w][c] (NULL: skipped);
+ * image (may be NULL) receives the FP64 mean image. */
+void or_diff_adjoint(const or_scene *s, const or_params *p, const float *d_image, double *image,
+                     double *g_env, double *const *g_tex);
+typedef struct or_tapes or_tapes;
+or_tapes *or_tapes_record(const or_scene *s, const or_params *p);
+void or_tapes_eval(const or_tapes *t, const or_scene *s, const float *const *tex, const float *env,
+                   int32_t row_begin, int32_t row_end, double *image);
+int32_t or_tapes_len(const or_tapes *t, int32_t pixel, int32_t sample);
+void or_tapes_free(or_tapes *t);
+
 #ifdef __cplusplus
 }
 #endif
