@@ -126,6 +126,25 @@ def run_batch(args) -> int:
             log(f"{name} run {i + 1}: {dt:.3f} s")
     st = abi.last_render_stats(ctx)
     same = all(torch.equal(a, b) for a, b in zip(f_seq, f_bat))
+    mismatch = []
+    if not same:
+        # where the films differ, and whether a plain render repeats itself
+        f_rep = [torch.zeros_like(f) for f in f_seq]
+        for k, (sd, sc) in enumerate(built):
+            abi.render(ctx, sc, env_objs[k % 4], cams[k], sd.width, sd.height, sd.spp, sd.max_depth, sd.seed,
+                       f_rep[k], spp_batch=1)
+        torch.cuda.synchronize()
+        for k, (sd, _) in enumerate(built):
+            a, b, c = f_seq[k], f_bat[k], f_rep[k]
+            if not torch.equal(a, b):
+                d = (a - b).abs()
+                mismatch.append({"request": k, "tris": sd.n_tris, "spp": sd.spp, "depth": sd.max_depth,
+                                 "side": sd.width, "pixels": int((d.amax(-1) > 0).sum()),
+                                 "max_abs": float(d.max()), "film_mean": float(a[..., :3].mean()),
+                                 "count_diff": int((a[..., 3] != b[..., 3]).sum()),
+                                 "sequential_repeats": bool(torch.equal(a, c)),
+                                 "batched_equals_repeat": bool(torch.equal(b, c))})
+        log(f"film mismatches: {len(mismatch)} requests; first: {mismatch[:6]}")
     inc = st.rays_extension + st.rays_shadow
     tb = min(times["batched"])
     ts = min(times["sequential"])
@@ -150,6 +169,7 @@ def run_batch(args) -> int:
         "sequential_incoherent_mrays_s": round(inc / ts / 1e6, 3),
         "batched_over_sequential": round(ts / tb, 3),
         "films_identical": bool(same),
+        "film_mismatch": mismatch[:8],
         "rays": {"primary": st.rays_primary, "extension": st.rays_extension, "shadow": st.rays_shadow},
     }
     print(json.dumps(out), flush=True)

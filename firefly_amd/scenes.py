@@ -298,6 +298,64 @@ def icosphere_scene(width=1920, height=1080, spp=64, max_depth=8, seed=0, subdiv
                      cam_target=target, cam_fovy=40.0)
 
 
+def textured_spheres_scene(width=512, height=512, spp=16, max_depth=4, seed=0, tex_size=512, env_w=1024,
+                           env_h=512, segments=32, rings=16) -> SceneData:
+    """Differentiable single view: the 8-sphere scene with the spheres'
+    albedo read from one tex_size^2 RGB texture of U[0.2, 0.8] texels
+    (texture 0) under an env_w x env_h sky."""
+    sd = spheres_scene(width, height, spp, max_depth, seed, segments=segments, rings=rings)
+    rng = np.random.default_rng(seed + 101)
+    sd.textures = [rng.uniform(0.2, 0.8, (tex_size, tex_size, 3)).astype(np.float32)]
+    mats = sd.materials.copy()
+    mats[1:, 10] = 0
+    sd.materials = mats
+    sd.env = sky_env(env_w, env_h)
+    sd.name = f"tspheres{width}"
+    return sd
+
+
+def perturbed(sd: SceneData, seed: int = 1, sigma: float = 0.05):
+    """Parameters of the target image: textures and environment plus
+    N(0, sigma) noise (textures clipped to [0.01, 1])."""
+    rng = np.random.default_rng(seed)
+    tex = [np.clip(t + rng.normal(0.0, sigma, t.shape), 0.01, 1.0).astype(np.float32) for t in sd.textures]
+    env = np.maximum(sd.env + rng.normal(0.0, sigma, sd.env.shape), 0.0).astype(np.float32)
+    return tex, env
+
+
+def view_directions(n: int):
+    """n Fibonacci-spiral directions over the upper hemisphere (y >= 0.1)."""
+    k = np.arange(n) + 0.5
+    y = 0.1 + 0.85 * k / n
+    phi = k * np.pi * (3.0 - np.sqrt(5.0))
+    r = np.sqrt(1.0 - y * y)
+    return np.stack([r * np.cos(phi), y, r * np.sin(phi)], 1)
+
+
+def multiview_scene(views=32, side=256, spp=8, max_depth=6, seed=0, subdiv=8, tex_size=1024,
+                    env_w=2048, env_h=1024, radius=4.0):
+    """Differentiable multi-view: the displaced icosphere (20*4^subdiv tris)
+    and ground quad, the sphere on one mixed GGX/diffuse material whose
+    roughness (U[0.05, 0.6], texture 1) and RGB albedo (texture 0) are
+    tex_size^2 textures, seen from `views` cameras on a sphere of `radius`
+    around it.  Returns (SceneData of view 0, [inverse view-projection of
+    every view])."""
+    rng = np.random.default_rng(seed + 202)
+    pos, nrm, uv, _ = displaced_icosphere(subdiv, (0.0, 1.1, 0.0), 1.0, 0.05, seed)
+    mats = [material(FF_MAT_DIFFUSE, (0.5, 0.5, 0.5)),
+            material(FF_MAT_MIX, (0.5, 0.5, 0.5), roughness=0.3, metallic=0.25, albedo_tex=0, rough_tex=1)]
+    P, N, U, M = _concat([(*ground_quad(0.0, 20.0), 0), (pos, nrm, uv, 1)])
+    textures = [rng.uniform(0.2, 0.8, (tex_size, tex_size, 3)).astype(np.float32),
+                rng.uniform(0.05, 0.6, (tex_size, tex_size, 1)).astype(np.float32)]
+    target = np.array([0.0, 1.1, 0.0])
+    vps = [inv_view_proj(tuple(target + radius * d), tuple(target), 40.0, 1.0) for d in view_directions(views)]
+    eye = tuple(target + radius * view_directions(views)[0])
+    sd = SceneData(P, N, U, M, np.stack(mats), sun_env(env_w, env_h, seed), vps[0], side, side, spp, max_depth,
+                   seed, textures=textures, name=f"multiview{views}", cam_eye=eye, cam_target=tuple(target),
+                   cam_fovy=40.0)
+    return sd, vps
+
+
 def soup_in_cube(n: int, seed: int, edge: float = 0.01) -> tuple:
     """n triangles with edges ~`edge`, uniform in the unit cube, random facing."""
     rng = np.random.default_rng(seed)

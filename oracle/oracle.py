@@ -77,6 +77,11 @@ def lib() -> C.CDLL:
             "or_spec_sample": (None, [_P, _P, C.c_float, C.c_float, C.c_float, _P]),
             "or_fresnel_dielectric": (C.c_float, [C.c_float, C.c_float, C.c_float]),
             "or_render": (None, [C.POINTER(OrScene), C.POINTER(OrParams), _P, C.POINTER(OrStats)]),
+            "or_diff_adjoint": (None, [C.POINTER(OrScene), C.POINTER(OrParams), _P, _P, _P, _P]),
+            "or_tapes_record": (_P, [C.POINTER(OrScene), C.POINTER(OrParams)]),
+            "or_tapes_eval": (None, [_P, C.POINTER(OrScene), _P, _P, C.c_int32, C.c_int32, _P]),
+            "or_tapes_len": (C.c_int32, [_P, C.c_int32, C.c_int32]),
+            "or_tapes_free": (None, [_P]),
         }
         for name, (res, args) in protos.items():
             fn = getattr(L, name)
@@ -242,6 +247,44 @@ class PreparedScene:
         lib().or_gen_ray(C.byref(self.s), w, h, px, py, base, out.ctypes.data)
         return out
 
+    def _params(self, spp, max_depth, seed, n_threads, width=None, height=None):
+        sd = self.sd
+        p = OrParams()
+        p.width, p.height = width or sd.width, height or sd.height
+        p.spp = spp or sd.spp
+        p.max_depth = max_depth or sd.max_depth
+        p.seed = sd.seed if seed is None else seed
+        p.spp_batch = 1
+        p.n_threads = n_threads or os.cpu_count() or 1
+        p.row_begin, p.row_end = 0, p.height
+        return p
+
+    def adjoint(self, d_image, spp=None, max_depth=None, seed=None, n_threads=None, env_grad=True,
+                tex_grads=None):
+        """FP64 adjoint of the detached-sampling estimator.  d_image [H][W][3] is
+        d loss / d (mean image).  Returns (image [H][W][3] float64 mean of the
+        FP64 estimator, g_env [eh][ew][3] or None, {texture: gradient})."""
+        p = self._params(spp, max_depth, seed, n_threads)
+        d = f32(d_image).reshape(p.height, p.width, 3)
+        img = np.zeros((p.height, p.width, 3), np.float64)
+        g_env = np.zeros(self.env.shape[:2] + (3,), np.float64) if env_grad else None
+        tex_grads = list(range(len(self.texs))) if tex_grads is None else list(tex_grads)
+        gt = {t: np.zeros(self.texs[t].shape, np.float64) for t in tex_grads}
+        ptrs = None
+        if self.texs:
+            ptrs = (C.c_void_p * len(self.texs))(*[gt[t].ctypes.data if t in gt else None
+                                                    for t in range(len(self.texs))])
+        lib().or_diff_adjoint(C.byref(self.s), C.byref(p), d.ctypes.data, img.ctypes.data,
+                              g_env.ctypes.data if g_env is not None else None,
+                              C.cast(ptrs, C.c_void_p) if ptrs is not None else None)
+        return img, g_env, gt
+
+    def record_tapes(self, spp=None, max_depth=None, seed=None, n_threads=None):
+        """Records every path of the float forward; .eval(textures, env)
+        evaluates the FP64 detached-sampling estimator for other parameters."""
+        p = self._params(spp, max_depth, seed, n_threads)
+        return Tapes(self, lib().or_tapes_record(C.byref(self.s), C.byref(p)), p.width, p.height, p.spp)
+
     def render(self, spp=None, max_depth=None, seed=None, spp_batch=1, n_threads=None,
                rows=None, width=None, height=None):
         sd = self.sd
@@ -257,3 +300,37 @@ class PreparedScene:
         st = OrStats()
         lib().or_render(C.byref(self.s), C.byref(p), film.ctypes.data, C.byref(st))
         return film, st
+
+
+class Tapes:
+    """Recorded paths of the float forward (oracle path tapes)."""
+
+    def __init__(self, ps, handle, width, height, spp):
+        self.ps, self.h, self.width, self.height, self.spp = ps, handle, width, height, spp
+
+    def eval(self, textures=None, env=None, rows=None):
+        """FP64 mean image [H][W][3] of the recorded paths with the given
+        textures (list, shapes as the scene's) and environment."""
+        ps = self.ps
+        texs = [f32(t) for t in (textures if textures is not None else ps.texs)]
+        e = f32(env if env is not None else ps.env)
+        ptrs = (C.c_void_p * max(1, len(texs)))(*[t.ctypes.data for t in texs]) if texs else None
+        r0, r1 = rows if rows else (0, self.height)
+        img = np.zeros((self.height, self.width, 3), np.float64)
+        lib().or_tapes_eval(self.h, C.byref(ps.s), C.cast(ptrs, C.c_void_p) if ptrs is not None else None,
+                            e.ctypes.data, r0, r1, img.ctypes.data)
+        return img
+
+    def length(self, pixel, sample):
+        return lib().or_tapes_len(self.h, pixel, sample)
+
+    def close(self):
+        if self.h:
+            lib().or_tapes_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,107 @@
+"""Pins of the FP64 adjoint of the oracle's differentiable estimator.
+
+The estimator of a path is its recorded forward (directions, cosines, pdfs,
+MIS weights, visibility and gating held fixed: detached sampling) evaluated
+with albedo / roughness textures (through the BSDF values) and environment
+texels (through Le).  Pins:
+  * at the forward's own parameters the FP64 estimator is the float render;
+  * Euler's theorem: the estimator is homogeneous of degree 1 in the
+    environment texels, so sum(env * dE/denv) = E exactly;
+  * central finite differences (h = 1e-3) of an L2 loss on random touched
+    texels of the albedo texture, the roughness texture and the environment:
+    the plain central difference within 1e-3 relative, and its Richardson
+    extrapolation with h/2 (truncation O(h^4)) within 2e-5 relative.
+"""
+import numpy as np
+import pytest
+
+from firefly_amd import abi, scenes
+from oracle import oracle as O
+
+
+def _small_textured(spp=4, size=32):
+    return scenes.textured_spheres_scene(width=size, height=size, spp=spp, max_depth=4, tex_size=32, env_w=64,
+                                         env_h=32, segments=16, rings=8)
+
+
+def _small_rough():
+    """Icosphere with albedo + roughness textures (multi-view material), one view."""
+    sd, vps = scenes.multiview_scene(views=2, side=24, spp=4, max_depth=4, subdiv=2, tex_size=16, env_w=64,
+                                     env_h=32)
+    sd.env = scenes.sky_env(64, 32)
+    return sd
+
+
+def test_fp64_estimator_is_the_forward():
+    sd = _small_textured()
+    ps = O.PreparedScene(sd)
+    film, _ = ps.render()
+    img, _, _ = ps.adjoint(np.zeros((sd.height, sd.width, 3), np.float32))
+    mean = film[..., :3] / film[..., 3:]
+    assert np.abs(img - mean).max() <= 1e-6 * max(1.0, np.abs(mean).max())
+
+
+def test_env_gradient_euler_identity():
+    sd = _small_textured()
+    ps = O.PreparedScene(sd)
+    rng = np.random.default_rng(3)
+    d = rng.normal(size=(sd.height, sd.width, 3)).astype(np.float32)
+    img, g_env, _ = ps.adjoint(d)
+    lhs = float((g_env * sd.env.astype(np.float64)).sum())
+    rhs = float((d.astype(np.float64) * img).sum())
+    assert abs(lhs - rhs) <= 1e-9 * np.abs(d.astype(np.float64) * img).sum()
+
+
+def _fd_check(sd, params, n_pick, h=1e-3, seed=5):
+    """params: list of ('env'|texture index) to perturb."""
+    ps = O.PreparedScene(sd)
+    tapes = ps.record_tapes()
+    tex0 = [t.astype(np.float32) for t in sd.textures]
+    env0 = sd.env.astype(np.float32)
+    T_tex, T_env = scenes.perturbed(sd, seed=1)
+    target = tapes.eval(T_tex, T_env)
+    I0 = tapes.eval(tex0, env0)
+    N = I0.size
+
+    def loss(tex, env):
+        return float(((tapes.eval(tex, env) - target) ** 2).sum() / N)
+
+    d = (2.0 * (I0 - target) / N).astype(np.float32)
+    _, g_env, g_tex = ps.adjoint(d)
+    rng = np.random.default_rng(seed)
+    checked = 0
+    for which in params:
+        g = g_env if which == "env" else g_tex[which]
+        touched = np.argwhere(np.abs(g) > 1e-3 * np.abs(g).max())
+        pick = touched[rng.choice(len(touched), size=min(n_pick, len(touched)), replace=False)]
+        for idx in map(tuple, pick):
+
+            def central(step):
+                tp = [t.copy() for t in tex0]
+                tm = [t.copy() for t in tex0]
+                ep, em = env0.copy(), env0.copy()
+                arr_p, arr_m = (ep, em) if which == "env" else (tp[which], tm[which])
+                base = float(arr_p[idx])
+                arr_p[idx] = base + step
+                arr_m[idx] = base - step
+                hh = (float(arr_p[idx]) - float(arr_m[idx])) / 2  # the float32 step actually taken
+                return (loss(tp, ep) - loss(tm, em)) / (2 * hh)
+
+            ad = float(g[idx])
+            floor = 1e-9 * np.abs(g).max()
+            f1 = central(h)
+            assert abs(f1 - ad) <= 1e-3 * abs(ad) + floor, (which, idx, f1, ad)
+            f2 = central(h / 2)
+            rich = (4.0 * f2 - f1) / 3.0
+            assert abs(rich - ad) <= 2e-5 * abs(ad) + floor, (which, idx, rich, ad)
+            checked += 1
+    return checked
+
+
+def test_finite_differences_albedo_and_env():
+    assert _fd_check(_small_textured(), [0, "env"], 128) == 256
+
+
+def test_finite_differences_roughness():
+    sd = _small_rough()
+    assert _fd_check(sd, [1, 0], 48) >= 64
