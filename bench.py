@@ -1,15 +1,19 @@
 #!/usr/bin/env python3
 """Benchmark of the gfx950 wavefront path tracer on one MI355X.
 
-Workload (the single-large-frame configuration): 1920x1080, 64 spp, max
-depth 8, displaced icosphere at subdivision 8 (1,310,720 tris) + ground quad,
-three material queues (diffuse / GGX conductor / dielectric), 2048x1024 HDR
-environment with a sigma = 0.5 deg sun lobe of peak 1e4.  Synthetic,
+Headline workload (the single-large-frame configuration): 1920x1080, 64 spp,
+max depth 8, displaced icosphere at subdivision 8 (1,310,720 tris) + ground
+quad, three material queues (diffuse / GGX conductor / dielectric), 2048x1024
+HDR environment with a sigma = 0.5 deg sun lobe of peak 1e4.  Synthetic,
 seeded inputs.  A step is one full frame (all 64 samples of every pixel).
 
 Prints ONE JSON line: value = incoherent Mrays/s (bounce >= 1 extension rays
 plus all any-hit rays, over the whole-frame wall time), with spp/s at 1080p,
-the extend-kernel roofline and the CPU oracle baseline beside it.
+the extend-kernel roofline and the CPU oracle baseline beside it.  The
+default run (--workload all) adds, in the same line, the batched-requests
+workload (64 uneven requests, batched vs sequential), the differentiable
+single view (forward + adjoint time ratio, gradient error against the FP64
+CPU adjoint) and the batched differentiable multi-view workload.
 """
 from __future__ import annotations
 
@@ -72,7 +76,7 @@ def cpu_baseline(sd, nodes, ordered, budget_s: float, spp: int):
     }
 
 
-def run_batch(args) -> int:
+def run_batch(args) -> dict:
     """Batched-requests configuration: 64 seeded requests of uneven
     resolution, spp, depth and triangle count rendered (a) as one batched
     wavefront replayed from a hipGraph and (b) as 64 plain sequential frames.
@@ -172,8 +176,226 @@ def run_batch(args) -> int:
         "film_mismatch": mismatch[:8],
         "rays": {"primary": st.rays_primary, "extension": st.rays_extension, "shadow": st.rays_shadow},
     }
-    print(json.dumps(out), flush=True)
-    return 0
+    for sc in built:
+        sc[1].close()
+    for e in env_objs:
+        e.close()
+    ctx.close()
+    return out
+
+
+def _rel(a, b) -> float:
+    import numpy as np
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _timed(fn, reps: int) -> float:
+    """Best wall time of fn() over reps runs (synchronised)."""
+    import torch
+    best = float("inf")
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    return best
+
+
+def run_adjoint(args) -> dict:
+    """Differentiable single view: 512x512, 16 spp, depth 4, the 8-sphere
+    scene with the spheres' albedo from a 512x512 RGB texture, 1024x512 sky.
+    Loss = mean squared error against the image of parameters perturbed by
+    N(0, 0.05) (seed 1).  Times the forward (with path storage) and the
+    path-replay adjoint (FP32 atomics, and the deterministic two-pass
+    reduction); gradient error against the FP64 CPU adjoint of the same
+    paths on all host threads."""
+    import numpy as np
+    import torch
+    from firefly_amd import abi, scenes
+
+    sd = scenes.textured_spheres_scene(512, 512, 16, 4, seed=0, tex_size=512, env_w=1024, env_h=512)
+    ctx = abi.Context(0)
+    scene = abi.Scene(ctx, sd.positions, sd.normals, sd.uvs, sd.material_ids, sd.materials, sd.textures)
+    env = abi.EnvMap(ctx, sd.env)
+    cam = abi.make_camera(sd.inv_vp)
+    K = args.adj_spp_batch
+    W, H, S, D = sd.width, sd.height, sd.spp, sd.max_depth
+    film = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    paths = torch.zeros((S, H, W, 4), dtype=torch.float32, device="cuda")
+    # target image: perturbed texture and environment
+    t_tex, t_env = scenes.perturbed(sd, seed=1)
+    scene.set_texture(0, t_tex[0])
+    env.update(t_env)
+    abi.render(ctx, scene, env, cam, W, H, S, D, sd.seed, film, spp_batch=K)
+    torch.cuda.synchronize()
+    target = (film[..., :3] / film[..., 3:]).clone()
+    scene.set_texture(0, sd.textures[0])
+    env.update(sd.env)
+
+    def fwd():
+        film.zero_()
+        abi.render_paths(ctx, scene, env, cam, W, H, S, D, sd.seed, film, paths, spp_batch=K)
+
+    fwd()
+    t_fwd = _timed(fwd, args.steps + 1)
+    st_f = abi.last_render_stats(ctx)
+    img = film[..., :3] / float(S)
+    d4 = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    d4[..., :3] = 2.0 * (img - target) / img.numel()
+    g_env = torch.zeros(sd.env.shape, dtype=torch.float32, device="cuda")
+    g_tex = torch.zeros(sd.textures[0].shape, dtype=torch.float32, device="cuda")
+
+    def adj(flags=0):
+        g_env.zero_()
+        g_tex.zero_()
+        abi.adjoint(ctx, scene, env, cam, W, H, S, D, sd.seed, paths, d4, env_grad=g_env, tex_grads=[g_tex],
+                    spp_batch=K, flags=flags)
+
+    adj()
+    t_adj = _timed(adj, args.steps + 1)
+    st_a = abi.last_render_stats(ctx)
+    ge, gt = g_env.cpu().numpy(), g_tex.cpu().numpy()
+    adj(abi.FF_ADJOINT_DETERMINISTIC)
+    t_det = _timed(lambda: adj(abi.FF_ADJOINT_DETERMINISTIC), 2)
+    de1, dt1 = g_env.cpu().numpy(), g_tex.cpu().numpy()
+    adj(abi.FF_ADJOINT_DETERMINISTIC)
+    reproducible = bool(np.array_equal(de1, g_env.cpu().numpy()) and np.array_equal(dt1, g_tex.cpu().numpy()))
+    out = {
+        "workload": "tspheres_512x512_16spp_d4_tex512_env1024x512",
+        "forward_ms": round(t_fwd * 1e3, 3),
+        "adjoint_ms": round(t_adj * 1e3, 3),
+        "adjoint_deterministic_ms": round(t_det * 1e3, 3),
+        "fwd_plus_adj_over_fwd": round((t_fwd + t_adj) / t_fwd, 3),
+        "forward_mrays_s": round((st_f.rays_primary + st_f.rays_extension + st_f.rays_shadow) / t_fwd / 1e6, 2),
+        "adjoint_mrays_s": round((st_a.rays_primary + st_a.rays_extension + st_a.rays_shadow) / t_adj / 1e6, 2),
+        "deterministic_reproducible": reproducible,
+        "spp_batch": K,
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = len(os.sched_getaffinity(0))
+        ps = O.PreparedScene(sd, *scene.bvh())
+        t0 = time.perf_counter()
+        _, r_env, r_tex = ps.adjoint(d4[..., :3].cpu().numpy(), n_threads=threads)
+        t_cpu = time.perf_counter() - t0
+        out["grad_rel_err_env"] = _rel(ge, r_env)
+        out["grad_rel_err_albedo"] = _rel(gt, r_tex[0])
+        out["grad_rel_err_deterministic_albedo"] = _rel(dt1, r_tex[0])
+        out["cpu_fp64_adjoint_s"] = round(t_cpu, 2)
+        out["cpu_threads"] = threads
+    scene.close()
+    env.close()
+    ctx.close()
+    return out
+
+
+def run_multiview(args) -> dict:
+    """Batched differentiable multi-view: 32 views (radius 4, upper-hemisphere
+    Fibonacci directions) of the displaced icosphere (subdivision 8), 256x256,
+    8 spp, depth 6; gradients w.r.t. a 1024^2 roughness texture and a 1024^2
+    RGB albedo texture.  One batched forward + one batched adjoint (graph
+    replay) against 32 separate forward + adjoint runs; gradient error of two
+    views against the FP64 CPU adjoint."""
+    import numpy as np
+    import torch
+    from firefly_amd import abi, scenes
+
+    sd, vps = scenes.multiview_scene(args.views, 256, 8, 6, seed=0, subdiv=args.subdiv, tex_size=1024)
+    ctx = abi.Context(0)
+    scene = abi.Scene(ctx, sd.positions, sd.normals, sd.uvs, sd.material_ids, sd.materials, sd.textures)
+    env = abi.EnvMap(ctx, sd.env)
+    cams = [abi.make_camera(v) for v in vps]
+    W, H, S, D, V = sd.width, sd.height, sd.spp, sd.max_depth, len(vps)
+    films = [torch.zeros((H, W, 4), dtype=torch.float32, device="cuda") for _ in range(V)]
+    paths = [torch.zeros((S, H, W, 4), dtype=torch.float32, device="cuda") for _ in range(V)]
+    items = [(scene, env, c, W, H, S, D, sd.seed, f) for c, f in zip(cams, films)]
+    t_tex, _ = scenes.perturbed(sd, seed=1)
+    for t in range(2):
+        scene.set_texture(t, t_tex[t])
+    abi.render_batch(ctx, items, flags=abi.FF_RENDER_USE_GRAPH)
+    torch.cuda.synchronize()
+    targets = [(f[..., :3] / f[..., 3:]).clone() for f in films]
+    for t in range(2):
+        scene.set_texture(t, sd.textures[t])
+
+    def fwd_batch():
+        for f in films:
+            f.zero_()
+        abi.render_batch(ctx, items, path_radiance=paths, flags=abi.FF_RENDER_USE_GRAPH)
+
+    fwd_batch()
+    n_total = 3 * W * H * V
+    ds = []
+    for f, tg in zip(films, targets):
+        d = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+        d[..., :3] = 2.0 * (f[..., :3] / float(S) - tg) / n_total
+        ds.append(d)
+    g_alb = torch.zeros(sd.textures[0].shape, dtype=torch.float32, device="cuda")
+    g_rough = torch.zeros(sd.textures[1].shape, dtype=torch.float32, device="cuda")
+
+    def adj_batch(flags=abi.FF_RENDER_USE_GRAPH):
+        g_alb.zero_()
+        g_rough.zero_()
+        abi.adjoint_batch(ctx, items, paths, ds, tex_grads=[[g_alb, g_rough]] * V, flags=flags)
+
+    def separate():
+        g_alb.zero_()
+        g_rough.zero_()
+        for k in range(V):
+            films[k].zero_()
+            abi.render_paths(ctx, scene, env, cams[k], W, H, S, D, sd.seed, films[k], paths[k])
+            abi.adjoint(ctx, scene, env, cams[k], W, H, S, D, sd.seed, paths[k], ds[k], tex_grads=[g_alb, g_rough])
+
+    t_fb = _timed(fwd_batch, args.steps + 1)
+    st_f = abi.last_render_stats(ctx)
+    adj_batch()
+    t_ab = _timed(adj_batch, args.steps + 1)
+    st_a = abi.last_render_stats(ctx)
+    ga, gr = g_alb.cpu().numpy(), g_rough.cpu().numpy()
+    t_det = _timed(lambda: adj_batch(abi.FF_RENDER_USE_GRAPH | abi.FF_ADJOINT_DETERMINISTIC), 2)
+    separate()
+    t_sep = _timed(separate, 2)
+    rays_f = st_f.rays_primary + st_f.rays_extension + st_f.rays_shadow
+    rays_a = st_a.rays_primary + st_a.rays_extension + st_a.rays_shadow
+    out = {
+        "workload": f"multiview{V}_icosphere{args.subdiv}_256x256_8spp_d6_tex1024",
+        "batched_forward_ms": round(t_fb * 1e3, 2),
+        "batched_adjoint_ms": round(t_ab * 1e3, 2),
+        "batched_adjoint_deterministic_ms": round(t_det * 1e3, 2),
+        "separate_fwd_adj_ms": round(t_sep * 1e3, 2),
+        "batched_over_separate": round(t_sep / (t_fb + t_ab), 3),
+        "fwd_plus_adj_over_fwd": round((t_fb + t_ab) / t_fb, 3),
+        "forward_mrays_s": round(rays_f / t_fb / 1e6, 2),
+        "adjoint_mrays_s": round(rays_a / t_ab / 1e6, 2),
+        "tris": sd.n_tris,
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = len(os.sched_getaffinity(0))
+        nodes, ordered = scene.bvh()
+        errs_a, errs_r = [], []
+        t0 = time.perf_counter()
+        for k in range(2):
+            sdk = scenes.SceneData(**{**sd.__dict__, "inv_vp": vps[k]})
+            ps = O.PreparedScene(sdk, nodes, ordered)
+            _, _, r_tex = ps.adjoint(ds[k][..., :3].cpu().numpy(), n_threads=threads, env_grad=False)
+            ga_k = torch.zeros_like(g_alb)
+            gr_k = torch.zeros_like(g_rough)
+            abi.adjoint(ctx, scene, env, cams[k], W, H, S, D, sd.seed, paths[k], ds[k], tex_grads=[ga_k, gr_k])
+            torch.cuda.synchronize()
+            errs_a.append(_rel(ga_k.cpu().numpy(), r_tex[0]))
+            errs_r.append(_rel(gr_k.cpu().numpy(), r_tex[1]))
+        out["grad_rel_err_albedo"] = max(errs_a)
+        out["grad_rel_err_roughness"] = max(errs_r)
+        out["grad_check"] = f"views 0-1 against the FP64 CPU adjoint ({threads} threads, " \
+                            f"{time.perf_counter() - t0:.1f} s)"
+    scene.close()
+    env.close()
+    ctx.close()
+    return out
 
 
 def main() -> int:
@@ -190,7 +412,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variant", type=int, default=-1, help="traversal kernel variant (-1: autotune)")
-    ap.add_argument("--workload", choices=("frame", "batch"), default="frame")
+    ap.add_argument("--workload", choices=("all", "frame", "batch", "adjoint", "multiview"), default="all")
+    ap.add_argument("--adj-spp-batch", type=int, default=16)
+    ap.add_argument("--views", type=int, default=32)
     ap.add_argument("--requests", type=int, default=64)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -201,9 +425,34 @@ def main() -> int:
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         return 2
 
-    if args.workload == "batch":
-        return run_batch(args)
+    if args.workload in ("batch", "adjoint", "multiview"):
+        fn = {"batch": run_batch, "adjoint": run_adjoint, "multiview": run_multiview}[args.workload]
+        out = fn(args)
+        out.update({"metric": METRIC, "n_gpus": 1})
+        print(json.dumps(out), flush=True)
+        return 0
+    out = run_frame(args)
+    if args.workload == "all":
+        import torch
+        torch.cuda.empty_cache()
+        for name, fn in (("adjoint", run_adjoint), ("multiview", run_multiview), ("batch", run_batch)):
+            log(f"workload {name} ...")
+            t0 = time.perf_counter()
+            sub = dict(args.__dict__)
+            a = argparse.Namespace(**sub)
+            a.steps, a.warmup = 1, 1
+            r = fn(a)
+            for k in ("metric", "n_gpus", "steps", "warmup", "higher_is_better", "scaling", "vs_baseline", "dtype",
+                      "data", "unit", "film_mismatch"):
+                r.pop(k, None)
+            out[name] = r
+            log(f"workload {name}: {time.perf_counter() - t0:.1f} s: {json.dumps(r)[:400]}")
+            torch.cuda.empty_cache()
+    print(json.dumps(out), flush=True)
+    return 0
 
+
+def run_frame(args) -> dict:
     import torch
     from firefly_amd import abi, scenes
 
@@ -331,8 +580,10 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
-    return 0
+    scene.close()
+    env.close()
+    ctx.close()
+    return out
 
 
 if __name__ == "__main__":

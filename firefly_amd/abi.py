@@ -15,6 +15,7 @@ FF_MAT_FLOATS = 16
 FF_MAT_MIX, FF_MAT_DIFFUSE, FF_MAT_CONDUCTOR, FF_MAT_DIELECTRIC = 0, 1, 2, 3
 FF_RENDER_USE_GRAPH = 1
 FF_RENDER_TIMING = 2
+FF_ADJOINT_DETERMINISTIC = 4
 
 NODE_DTYPE = np.dtype([("bmin", "<f4", 3), ("bmax", "<f4", 3), ("offset", "<i4"),
                        ("n_prims", "<u2"), ("axis", "u1"), ("pad", "u1")])
@@ -46,10 +47,15 @@ class RenderReq(C.Structure):
                 ("sample_offset", C.c_int32), ("flags", C.c_int32)]
 
 
+class Grads(C.Structure):
+    _fields_ = [("env", C.c_void_p), ("n_textures", C.c_int32), ("textures", C.c_void_p)]
+
+
 class BatchItem(C.Structure):
     _fields_ = [("scene", C.c_void_p), ("env", C.c_void_p), ("camera", Camera), ("width", C.c_int32),
                 ("height", C.c_int32), ("spp", C.c_int32), ("max_depth", C.c_int32), ("seed", C.c_uint32),
-                ("sample_offset", C.c_int32), ("film", C.c_void_p)]
+                ("sample_offset", C.c_int32), ("film", C.c_void_p), ("path_radiance", C.c_void_p),
+                ("d_image", C.c_void_p), ("grads", C.c_void_p)]
 
 
 class RayStats(C.Structure):
@@ -85,6 +91,11 @@ _PROTOS = {
     "ff_render": (C.c_int, [_P, _P, _P, C.POINTER(Camera), C.POINTER(RenderReq), _P, _P]),
     "ff_last_render_stats": (C.c_int, [_P, C.POINTER(RayStats)]),
     "ff_render_batch": (C.c_int, [_P, C.POINTER(BatchItem), C.c_int32, C.c_int32, _P]),
+    "ff_envmap_update": (C.c_int, [_P, _P, _P]),
+    "ff_scene_set_texture": (C.c_int, [_P, C.c_int32, _P, _P]),
+    "ff_render_paths": (C.c_int, [_P, _P, _P, C.POINTER(Camera), C.POINTER(RenderReq), _P, _P, _P]),
+    "ff_adjoint": (C.c_int, [_P, _P, _P, C.POINTER(Camera), C.POINTER(RenderReq), _P, _P, C.POINTER(Grads), _P]),
+    "ff_adjoint_batch": (C.c_int, [_P, C.POINTER(BatchItem), C.c_int32, C.c_int32, _P]),
     "ff_traversal_variant_count": (C.c_int32, []),
     "ff_traversal_variant_info": (C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                             C.POINTER(C.c_int32)]),
@@ -223,6 +234,12 @@ class Scene:
         n = rays_dev.shape[0]
         check(_lib().ff_trace_any(self.h, _ptr(rays_dev), n, _ptr(occ_dev), _ptr(counts_dev), stream))
 
+    def set_texture(self, t: int, data, stream=None):
+        """New texels for texture t (host array or device tensor, same shape)."""
+        if isinstance(data, np.ndarray):
+            data = np.ascontiguousarray(data, dtype=np.float32)
+        check(_lib().ff_scene_set_texture(self.h, t, _ptr(data), stream))
+
     def close(self):
         if self.h:
             check(_lib().ff_scene_destroy(self.h))
@@ -253,6 +270,12 @@ class EnvMap:
                                     pdf.ctypes.data, denom.ctypes.data))
         return my, mp, conds, pdf, float(denom[0])
 
+    def update(self, rgb, stream=None):
+        """New texels (host array or device tensor [h][w][3]); CDFs rebuilt."""
+        if isinstance(rgb, np.ndarray):
+            rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+        check(_lib().ff_envmap_update(self.h, _ptr(rgb), stream))
+
     def close(self):
         if self.h:
             check(_lib().ff_envmap_destroy(self.h))
@@ -272,15 +295,67 @@ def render(ctx: Context, scene: Scene, env: EnvMap, cam: Camera, width: int, hei
     check(_lib().ff_render(ctx.h, scene.h, env.h, C.byref(cam), C.byref(r), _ptr(film_dev), stream))
 
 
-def render_batch(ctx: Context, items, flags: int = 0, stream=None) -> None:
+def render_batch(ctx: Context, items, flags: int = 0, stream=None, path_radiance=None) -> None:
     """items: sequence of (scene, env, camera, width, height, spp, max_depth,
-    seed, film_dev[, sample_offset])."""
+    seed, film_dev[, sample_offset]); path_radiance: optional per-item
+    [spp][h][w][4] device buffers for the adjoint."""
     arr = (BatchItem * len(items))()
     for k, it in enumerate(items):
         sc, env, cam, w, h, spp, depth, seed, film = it[:9]
         off = it[9] if len(it) > 9 else 0
-        arr[k] = BatchItem(sc.h, env.h, cam, w, h, spp, depth, seed, off, _ptr(film))
+        pr = _ptr(path_radiance[k]) if path_radiance is not None else None
+        arr[k] = BatchItem(sc.h, env.h, cam, w, h, spp, depth, seed, off, _ptr(film), pr, None, None)
     check(_lib().ff_render_batch(ctx.h, arr, len(items), flags, stream))
+
+
+def make_grads(env_grad=None, tex_grads=None):
+    """ff_grads for device buffers (torch tensors); tex_grads: list indexed by
+    scene texture (None entries skipped).  Returns (Grads, keep-alive)."""
+    tex_grads = list(tex_grads or [])
+    arr = (C.c_void_p * max(1, len(tex_grads)))(*[_ptr(t) for t in tex_grads]) if tex_grads else None
+    g = Grads(_ptr(env_grad), len(tex_grads), C.cast(arr, C.c_void_p) if arr is not None else None)
+    return g, arr
+
+
+def render_paths(ctx: Context, scene: Scene, env: EnvMap, cam: Camera, width: int, height: int, spp: int,
+                 max_depth: int, seed: int, film_dev, path_radiance, spp_batch: int = 1, sample_offset: int = 0,
+                 flags: int = 0, stream=None) -> None:
+    """ff_render that also stores every path's radiance ([spp][h][w][4])."""
+    r = RenderReq(width, height, spp, max_depth, seed, spp_batch, sample_offset, flags)
+    check(_lib().ff_render_paths(ctx.h, scene.h, env.h, C.byref(cam), C.byref(r), _ptr(film_dev),
+                                 _ptr(path_radiance), stream))
+
+
+def adjoint(ctx: Context, scene: Scene, env: EnvMap, cam: Camera, width: int, height: int, spp: int,
+            max_depth: int, seed: int, path_radiance, d_image, env_grad=None, tex_grads=None, spp_batch: int = 1,
+            sample_offset: int = 0, flags: int = 0, stream=None) -> None:
+    """Path-replay adjoint: ADDS d loss / d env texel and d loss / d texel of
+    each scene texture to env_grad / tex_grads[t] (device buffers), given
+    d_image = d loss / d mean image ([h][w][4], xyz used)."""
+    r = RenderReq(width, height, spp, max_depth, seed, spp_batch, sample_offset, flags)
+    g, keep = make_grads(env_grad, tex_grads)
+    check(_lib().ff_adjoint(ctx.h, scene.h, env.h, C.byref(cam), C.byref(r), _ptr(path_radiance), _ptr(d_image),
+                            C.byref(g), stream))
+    del keep
+
+
+def adjoint_batch(ctx: Context, items, path_radiance, d_images, env_grads=None, tex_grads=None, flags: int = 0,
+                  stream=None) -> None:
+    """Batched adjoint of render_batch items; env_grads / tex_grads are per
+    item (items may share buffers)."""
+    n = len(items)
+    arr = (BatchItem * n)()
+    keep = []
+    for k, it in enumerate(items):
+        sc, env, cam, w, h, spp, depth, seed = it[:8]
+        off = it[9] if len(it) > 9 else 0
+        g, a = make_grads(env_grads[k] if env_grads is not None else None,
+                          tex_grads[k] if tex_grads is not None else None)
+        keep.append((g, a))
+        arr[k] = BatchItem(sc.h, env.h, cam, w, h, spp, depth, seed, off, None, _ptr(path_radiance[k]),
+                           _ptr(d_images[k]), C.cast(C.pointer(g), C.c_void_p))
+    check(_lib().ff_adjoint_batch(ctx.h, arr, n, flags, stream))
+    del keep
 
 
 def traversal_variants():

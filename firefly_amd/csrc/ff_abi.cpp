@@ -59,7 +59,7 @@ struct PathStore {
         dfree(p.ray); dfree(p.hit); dfree(p.beta); dfree(p.Lp); dfree(p.fext); dfree(p.nee);
         dfree(p.sray); dfree(p.vis); dfree(p.rngb); dfree(p.preq); dfree(p.q_ext[0]); dfree(p.q_ext[1]);
         for (int m = 0; m < 4; m++) dfree(p.q_mat[m]);
-        dfree(p.q_shadow); dfree(p.layers);
+        dfree(p.q_shadow); dfree(p.layers); dfree(p.dlt);
         cap = 0;
     }
 };
@@ -88,6 +88,13 @@ struct ff_context {
     ff_dev_request* d_reqs = nullptr;
     size_t req_cap = 0;
     int trav_variant = 0;
+    // adjoint: gradient target tables and the deterministic-reduction arena
+    ff_dev_gtarget* d_gt = nullptr;
+    size_t gt_cap = 0;
+    uint32_t* g_max = nullptr;
+    uint32_t* g_cnt = nullptr;
+    unsigned long long* g_fx = nullptr;
+    size_t g_cap = 0;
 };
 
 struct ff_scene {
@@ -104,6 +111,7 @@ struct ff_scene {
     int32_t* d_mat = nullptr;
     float* d_mats = nullptr;
     std::vector<float*> d_tex;
+    std::vector<ff_tex_desc> tex_desc;  // host copy (sizes)
     ff_tex_desc* d_tex_desc = nullptr;
     int32_t n_tex = 0;
     bool has_type[4] = {false, false, false, false};
@@ -117,6 +125,7 @@ struct ff_scene {
 };
 
 struct ff_envmap {
+    int device = 0;
     int32_t w = 0, h = 0;
     float* rgb = nullptr;
     float* lum = nullptr;
@@ -154,6 +163,7 @@ ff_status ensure_paths(ff_context* c, size_t P) {
     for (int m = 0; m < 4; m++) HIP_TRY(dalloc(&p.q_mat[m], P));
     HIP_TRY(dalloc(&p.q_shadow, 3 * P));
     HIP_TRY(dalloc(&p.layers, P));
+    HIP_TRY(dalloc(&p.dlt, P));
     c->paths.cap = P;
     return FF_OK;
 }
@@ -255,6 +265,10 @@ ff_status ff_context_destroy(ff_context* c) {
     dfree(c->counters);
     dfree(c->stats);
     dfree(c->ovf);
+    dfree(c->d_gt);
+    dfree(c->g_max);
+    dfree(c->g_cnt);
+    dfree(c->g_fx);
     delete c;
     return FF_OK;
 }
@@ -361,6 +375,7 @@ ff_status ff_scene_create(ff_context* ctx, const ff_scene_desc* d, ff_scene** ou
             return cleanup(fail(FF_ERR_HIP, "ff_scene_create: texture upload"));
         descs[t].data = p; descs[t].w = d->tex_w[t]; descs[t].h = d->tex_h[t]; descs[t].c = d->tex_c[t]; descs[t].pad = 0;
     }
+    s->tex_desc = descs;
     if (dalloc(&s->d_tex_desc, descs.size()) != hipSuccess ||
         hipMemcpy(s->d_tex_desc, descs.data(), descs.size() * sizeof(ff_tex_desc), hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(FF_ERR_ALLOC, "ff_scene_create: texture table"));
@@ -408,6 +423,7 @@ ff_status ff_envmap_create(ff_context* ctx, const float* rgb, int32_t w, int32_t
     if (total == 0.0) return fail(FF_ERR_INVALID, "ff_envmap_create: all-zero map has no sampling distribution");
     HIP_TRY(hipSetDevice(ctx->device));
     ff_envmap* e = new ff_envmap();
+    e->device = ctx->device;
     e->w = w;
     e->h = h;
     size_t wh = (size_t)w * h;
@@ -428,6 +444,27 @@ ff_status ff_envmap_create(ff_context* ctx, const float* rgb, int32_t w, int32_t
         return fail(FF_ERR_HIP, "ff_envmap_create: CDF build: %s", hipGetErrorString(err));
     }
     *out = e;
+    return FF_OK;
+}
+
+ff_status ff_envmap_update(ff_envmap* e, const float* rgb, void* stream) {
+    if (!e || !rgb) return fail(FF_ERR_INVALID, "ff_envmap_update: NULL argument");
+    HIP_TRY(hipSetDevice(e->device));
+    size_t wh = (size_t)e->w * e->h;
+    hipStream_t hs = (hipStream_t)stream;
+    HIP_TRY(hipMemcpyAsync(e->rgb, rgb, 12 * wh, hipMemcpyDefault, hs));
+    HIP_TRY(ffk::launch_env_build(e->rgb, e->w, e->h, e->lum, e->row_sum, e->denom, e->marg_p, e->marg_y, e->conds,
+                                  e->pdf, hs));
+    return FF_OK;
+}
+
+ff_status ff_scene_set_texture(ff_scene* s, int32_t t, const float* data, void* stream) {
+    if (!s || !data) return fail(FF_ERR_INVALID, "ff_scene_set_texture: NULL argument");
+    if (t < 0 || t >= s->n_tex) return fail(FF_ERR_INVALID, "ff_scene_set_texture: texture %d of %d", t, s->n_tex);
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    const ff_tex_desc& d = s->tex_desc[(size_t)t];
+    HIP_TRY(hipMemcpyAsync(s->d_tex[(size_t)t], data, sizeof(float) * (size_t)d.w * d.h * d.c, hipMemcpyDefault,
+                           (hipStream_t)stream));
     return FF_OK;
 }
 
@@ -498,10 +535,23 @@ struct HostReq {
     int32_t W, H, spp, depth, K, sample_offset;
     uint32_t seed;
     float* film;
+    uint32_t mode = 0;               // RQ_*
+    float* path_L = nullptr;         // [spp][H][W][4]
+    const float* d_img = nullptr;    // adjoint: [H][W][4]
+    const ff_grads* grads = nullptr; // adjoint
 };
 
 ff_status check_req(const char* who, ff_context* ctx, const HostReq& q) {
-    if (!q.s || !q.e || !q.film) return fail(FF_ERR_INVALID, "%s: NULL scene, environment or film", who);
+    if (!q.s || !q.e || (!q.film && !(q.mode & RQ_ADJOINT)))
+        return fail(FF_ERR_INVALID, "%s: NULL scene, environment or film", who);
+    if ((q.mode & (RQ_STORE_PATHS | RQ_ADJOINT)) && !q.path_L)
+        return fail(FF_ERR_INVALID, "%s: NULL path radiance buffer", who);
+    if ((q.mode & RQ_ADJOINT) && (!q.d_img || !q.grads))
+        return fail(FF_ERR_INVALID, "%s: NULL d_image or gradient targets", who);
+    if ((q.mode & RQ_ADJOINT) && (q.grads->n_textures < 0 || q.grads->n_textures > q.s->n_tex ||
+                                  (q.grads->n_textures > 0 && !q.grads->textures)))
+        return fail(FF_ERR_INVALID, "%s: %d gradient textures for a scene with %d", who, q.grads->n_textures,
+                    q.s->n_tex);
     if (q.s->ctx != ctx) return fail(FF_ERR_INVALID, "%s: scene belongs to another context", who);
     if (q.W <= 0 || q.H <= 0 || q.spp <= 0 || q.depth < 1 || q.depth > 250 || q.sample_offset < 0 || q.K < 1)
         return fail(FF_ERR_INVALID, "%s: invalid request (%dx%d, spp %d, depth %d)", who, q.W, q.H, q.spp, q.depth);
@@ -541,6 +591,67 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
     }
     HIP_TRY(ffk::prepare_traversal(ctx->trav_variant));
     ff_paths p = ctx->paths.p;
+    const bool adjoint = (reqs[0].mode & RQ_ADJOINT) != 0;
+    for (const HostReq& q : reqs)
+        if (((q.mode & RQ_ADJOINT) != 0) != adjoint)
+            return fail(FF_ERR_INVALID, "%s: forward and adjoint requests cannot share a call", who);
+    const bool deterministic = adjoint && (flags & FF_ADJOINT_DETERMINISTIC) != 0;
+
+    // gradient targets: [1 + n_tex] per request; one arena slot range per
+    // distinct caller buffer (requests may share a texture's gradient)
+    struct Target { float* buf; uint64_t off, n; };
+    std::vector<Target> targets;
+    std::vector<ff_dev_gtarget> gt;
+    std::vector<size_t> gt_base(reqs.size(), 0);
+    uint64_t arena = 0;
+    if (adjoint) {
+        auto target_of = [&](float* buf, uint64_t n, ff_dev_gtarget& out) -> ff_status {
+            out.buf = buf;
+            out.off = 0;
+            if (!buf) return FF_OK;
+            for (const Target& t : targets)
+                if (t.buf == buf) {
+                    if (t.n != n) return fail(FF_ERR_INVALID, "%s: one gradient buffer bound to targets of %llu and %llu elements",
+                                              who, (unsigned long long)t.n, (unsigned long long)n);
+                    out.off = t.off;
+                    return FF_OK;
+                }
+            targets.push_back({buf, arena, n});
+            out.off = arena;
+            arena += n;
+            return FF_OK;
+        };
+        for (size_t r = 0; r < reqs.size(); r++) {
+            const HostReq& q = reqs[r];
+            gt_base[r] = gt.size();
+            ff_dev_gtarget e{};
+            ff_status st = target_of(q.grads->env, (uint64_t)q.e->w * q.e->h * 3, e);
+            if (st != FF_OK) return st;
+            gt.push_back(e);
+            for (int32_t t = 0; t < q.s->n_tex; t++) {
+                ff_dev_gtarget g{};
+                float* buf = t < q.grads->n_textures ? q.grads->textures[t] : nullptr;
+                const ff_tex_desc& d = q.s->tex_desc[(size_t)t];
+                st = target_of(buf, (uint64_t)d.w * d.h * d.c, g);
+                if (st != FF_OK) return st;
+                gt.push_back(g);
+            }
+        }
+        if (ctx->gt_cap < gt.size()) {
+            dfree(ctx->d_gt);
+            HIP_TRY(dalloc(&ctx->d_gt, gt.size()));
+            ctx->gt_cap = gt.size();
+        }
+        if (deterministic && ctx->g_cap < arena) {
+            dfree(ctx->g_max); dfree(ctx->g_cnt); dfree(ctx->g_fx);
+            ctx->g_cap = 0;
+            HIP_TRY(dalloc(&ctx->g_max, arena));
+            HIP_TRY(dalloc(&ctx->g_cnt, arena));
+            HIP_TRY(dalloc(&ctx->g_fx, arena));
+            ctx->g_cap = arena;
+        }
+    }
+
     // request table (uploaded once the stream's earlier work is done with it)
     std::vector<ff_dev_request> table(reqs.size());
     uint64_t base = 0;
@@ -555,6 +666,11 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
         d.slot_base = (uint32_t)base;
         d.W = (uint32_t)q.W; d.H = (uint32_t)q.H; d.spp = (uint32_t)q.spp; d.depth = (uint32_t)q.depth;
         d.K = (uint32_t)q.K; d.seed = q.seed; d.sample_offset = (uint32_t)q.sample_offset;
+        d.mode = q.mode;
+        d.inv_spp = 1.f / (float)q.spp;
+        d.path_L = (float4*)q.path_L;
+        d.d_img = (const float4*)q.d_img;
+        d.gt = adjoint ? ctx->d_gt + gt_base[r] : nullptr;
         base += (uint64_t)q.W * q.H * q.K;
     }
     if (ctx->req_cap < table.size()) {
@@ -564,6 +680,8 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
     }
     HIP_TRY(hipStreamSynchronize(hs));
     HIP_TRY(hipMemcpy(ctx->d_reqs, table.data(), table.size() * sizeof(ff_dev_request), hipMemcpyHostToDevice));
+    if (adjoint && !gt.empty())
+        HIP_TRY(hipMemcpy(ctx->d_gt, gt.data(), gt.size() * sizeof(ff_dev_gtarget), hipMemcpyHostToDevice));
     p.counters = ctx->counters;
     p.stats = ctx->stats;
     p.ovf = ctx->ovf;
@@ -571,6 +689,10 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
     p.reqs = ctx->d_reqs;
     p.n_req = (uint32_t)table.size();
     p.n_slots = (uint32_t)n_slots;
+    p.g_max = ctx->g_max;
+    p.g_cnt = ctx->g_cnt;
+    p.g_fx = ctx->g_fx;
+    p.gmode = GM_FLOAT;
     const int var = ctx->trav_variant;
 
     ctx->records.clear();
@@ -592,13 +714,13 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
         ctx->records.push_back(rec);
         return e;
     };
-    for (size_t r = 0; r < reqs.size(); r++)
-        if (reqs[r].K > 1)
-            HIP_TRY(ffk::launch_fill4(table[r].layers, (uint32_t)((uint64_t)reqs[r].W * reqs[r].H * reqs[r].K), hs));
+    if (!adjoint)
+        for (size_t r = 0; r < reqs.size(); r++)
+            if (reqs[r].K > 1)
+                HIP_TRY(ffk::launch_fill4(table[r].layers, (uint32_t)((uint64_t)reqs[r].W * reqs[r].H * reqs[r].K), hs));
     HIP_TRY(ffk::launch_fill_u64(p.stats, S_COUNT, hs));
-    HIP_TRY(ffk::launch_batch_begin(p, 1, 0xffffffffu, hs));  // first batch_begin makes it 0
 
-    // one batch: identical launches for every batch (the index is on the device)
+    // one batch of the forward: identical launches for every batch (the index is on the device)
     auto batch = [&]() -> ff_status {
         HIP_TRY(ffk::launch_batch_begin(p, 0, 0, hs));
         HIP_TRY(timed(4, [&] { return ffk::launch_generate(p, hs); }));
@@ -618,39 +740,86 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
         }
         return FF_OK;
     };
-    if (graph) {
-        hipGraph_t g = nullptr;
-        hipGraphExec_t ge = nullptr;
-        HIP_TRY(hipStreamBeginCapture(hs, hipStreamCaptureModeThreadLocal));
-        ff_status bst = batch();
-        hipError_t ce = hipStreamEndCapture(hs, &g);
-        if (bst != FF_OK) {
-            if (g) (void)hipGraphDestroy(g);
-            return bst;
+    // one batch of the adjoint replay: a vertex's shadow rays are traced
+    // between its NEE replay and its gradient kernel
+    auto batch_adj = [&]() -> ff_status {
+        HIP_TRY(ffk::launch_batch_begin(p, 0, 0, hs));
+        HIP_TRY(timed(4, [&] { return ffk::launch_generate(p, hs); }));
+        int cur = 0;
+        for (int32_t it = 1; it <= max_depth + 1; it++) {
+            HIP_TRY(timed(0, [&] { return ffk::launch_extend(p, cur, var, hs); }));
+            uint32_t mask = (1u << (cur == 0 ? C_EXT1 : C_EXT0)) | (1u << C_SHADOW) | (0xfu << C_MAT0) |
+                            (1u << C_FETCH_EXT) | (1u << C_FETCH_SHD);
+            HIP_TRY(ffk::launch_reset(p.counters, mask, hs));
+            HIP_TRY(timed(2, [&] { return ffk::launch_logic_adj(p, cur, p.n_slots, hs); }));
+            if (it <= max_depth) {
+                for (int m = 0; m < 3; m++)
+                    if (has_type[m]) HIP_TRY(timed(3, [&] { return ffk::launch_shade_nee(m, p, p.n_slots, hs); }));
+                HIP_TRY(timed(1, [&] { return ffk::launch_shadow(p, var, hs); }));
+                for (int m = 0; m < 4; m++)
+                    if (has_type[m])
+                        HIP_TRY(timed(3, [&] { return ffk::launch_shade_adj(m, p, 1 - cur, p.n_slots, hs); }));
+            }
+            cur = 1 - cur;
         }
-        HIP_TRY(ce);
-        hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-        if (ie != hipSuccess) {
+        return FF_OK;
+    };
+    auto run = [&]() -> ff_status {
+        HIP_TRY(ffk::launch_batch_begin(p, 1, 0xffffffffu, hs));  // the first batch_begin makes it 0
+        if (graph) {
+            hipGraph_t g = nullptr;
+            hipGraphExec_t ge = nullptr;
+            HIP_TRY(hipStreamBeginCapture(hs, hipStreamCaptureModeThreadLocal));
+            ff_status bst = adjoint ? batch_adj() : batch();
+            hipError_t ce = hipStreamEndCapture(hs, &g);
+            if (bst != FF_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return bst;
+            }
+            HIP_TRY(ce);
+            hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            if (ie != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                HIP_TRY(ie);
+            }
+            hipError_t le = hipSuccess;
+            for (int64_t b = 0; b < n_batches && le == hipSuccess; b++) le = hipGraphLaunch(ge, hs);
+            (void)hipGraphExecDestroy(ge);
             (void)hipGraphDestroy(g);
-            HIP_TRY(ie);
+            HIP_TRY(le);
+        } else {
+            for (int64_t b = 0; b < n_batches; b++) {
+                ff_status bst = adjoint ? batch_adj() : batch();
+                if (bst != FF_OK) return bst;
+            }
         }
-        hipError_t le = hipSuccess;
-        for (int64_t b = 0; b < n_batches && le == hipSuccess; b++) le = hipGraphLaunch(ge, hs);
-        (void)hipGraphExecDestroy(ge);
-        (void)hipGraphDestroy(g);
-        HIP_TRY(le);
+        return FF_OK;
+    };
+    if (!deterministic) {
+        ff_status st = run();
+        if (st != FF_OK) return st;
     } else {
-        for (int64_t b = 0; b < n_batches; b++) {
-            ff_status bst = batch();
-            if (bst != FF_OK) return bst;
-        }
+        // pass 1 bounds every gradient element's terms; pass 2 sums them as
+        // int64 fixed point at per-element scales (order-independent sums)
+        HIP_TRY(ffk::launch_fill_u32(p.g_max, arena, hs));
+        HIP_TRY(ffk::launch_fill_u32(p.g_cnt, arena, hs));
+        HIP_TRY(ffk::launch_fill_u32((uint32_t*)p.g_fx, 2 * arena, hs));
+        p.gmode = GM_BOUND;
+        ff_status st = run();
+        if (st != FF_OK) return st;
+        p.gmode = GM_FIXED;
+        st = run();
+        if (st != FF_OK) return st;
+        for (const Target& t : targets)
+            HIP_TRY(ffk::launch_fixed_to_float(p.g_fx + t.off, p.g_max + t.off, p.g_cnt + t.off, t.n, t.buf, hs));
     }
-    for (size_t r = 0; r < reqs.size(); r++)
-        if (reqs[r].K > 1)
-            HIP_TRY(timed(4, [&] {
-                return ffk::launch_resolve_layers(table[r].layers, (uint32_t)reqs[r].K,
-                                                  (uint32_t)((uint64_t)reqs[r].W * reqs[r].H), (float4*)reqs[r].film, hs);
-            }));
+    if (!adjoint)
+        for (size_t r = 0; r < reqs.size(); r++)
+            if (reqs[r].K > 1)
+                HIP_TRY(timed(4, [&] {
+                    return ffk::launch_resolve_layers(table[r].layers, (uint32_t)reqs[r].K,
+                                                      (uint32_t)((uint64_t)reqs[r].W * reqs[r].H), (float4*)reqs[r].film, hs);
+                }));
     ctx->last_stream = hs;
     ctx->have_stats = true;
     return FF_OK;
@@ -678,6 +847,10 @@ ff_status ff_render_batch(ff_context* ctx, const ff_batch_item* items, int32_t n
         const ff_batch_item& it = items[i];
         HostReq q{it.scene, it.env, it.camera, it.width, it.height, it.spp, it.max_depth, 1, it.sample_offset,
                   it.seed, it.film};
+        if (it.path_radiance) {
+            q.mode = RQ_STORE_PATHS;
+            q.path_L = it.path_radiance;
+        }
         ff_status st = check_req("ff_render_batch", ctx, q);
         if (st != FF_OK) return st;
         for (int32_t j = 0; j < i; j++)
@@ -685,6 +858,54 @@ ff_status ff_render_batch(ff_context* ctx, const ff_batch_item* items, int32_t n
         reqs.push_back(q);
     }
     return render_requests(ctx, reqs, flags, (hipStream_t)stream, "ff_render_batch");
+}
+
+ff_status ff_render_paths(ff_context* ctx, ff_scene* s, ff_envmap* env, const ff_camera* cam,
+                          const ff_render_req* r, float* film, float* path_radiance, void* stream) {
+    if (!ctx || !s || !env || !cam || !r || !film || !path_radiance)
+        return fail(FF_ERR_INVALID, "ff_render_paths: NULL argument");
+    HostReq q{s, env, *cam, r->width, r->height, r->spp, r->max_depth,
+              r->spp_batch > 0 ? std::min(r->spp_batch, std::max(r->spp, 1)) : 1, r->sample_offset, r->seed, film};
+    q.mode = RQ_STORE_PATHS;
+    q.path_L = path_radiance;
+    ff_status st = check_req("ff_render_paths", ctx, q);
+    if (st != FF_OK) return st;
+    return render_requests(ctx, {q}, r->flags, (hipStream_t)stream, "ff_render_paths");
+}
+
+ff_status ff_adjoint(ff_context* ctx, ff_scene* s, ff_envmap* env, const ff_camera* cam, const ff_render_req* r,
+                     const float* path_radiance, const float* d_image, const ff_grads* grads, void* stream) {
+    if (!ctx || !s || !env || !cam || !r || !path_radiance || !d_image || !grads)
+        return fail(FF_ERR_INVALID, "ff_adjoint: NULL argument");
+    HostReq q{s, env, *cam, r->width, r->height, r->spp, r->max_depth,
+              r->spp_batch > 0 ? std::min(r->spp_batch, std::max(r->spp, 1)) : 1, r->sample_offset, r->seed,
+              nullptr};
+    q.mode = RQ_ADJOINT;
+    q.path_L = (float*)path_radiance;
+    q.d_img = d_image;
+    q.grads = grads;
+    ff_status st = check_req("ff_adjoint", ctx, q);
+    if (st != FF_OK) return st;
+    return render_requests(ctx, {q}, r->flags, (hipStream_t)stream, "ff_adjoint");
+}
+
+ff_status ff_adjoint_batch(ff_context* ctx, const ff_batch_item* items, int32_t n, int32_t flags, void* stream) {
+    if (!ctx || !items || n <= 0) return fail(FF_ERR_INVALID, "ff_adjoint_batch: NULL argument or empty batch");
+    std::vector<HostReq> reqs;
+    reqs.reserve((size_t)n);
+    for (int32_t i = 0; i < n; i++) {
+        const ff_batch_item& it = items[i];
+        HostReq q{it.scene, it.env, it.camera, it.width, it.height, it.spp, it.max_depth, 1, it.sample_offset,
+                  it.seed, nullptr};
+        q.mode = RQ_ADJOINT;
+        q.path_L = it.path_radiance;
+        q.d_img = it.d_image;
+        q.grads = it.grads;
+        ff_status st = check_req("ff_adjoint_batch", ctx, q);
+        if (st != FF_OK) return st;
+        reqs.push_back(q);
+    }
+    return render_requests(ctx, reqs, flags, (hipStream_t)stream, "ff_adjoint_batch");
 }
 
 ff_status ff_set_traversal_variant(ff_context* ctx, int32_t variant) {

@@ -300,6 +300,87 @@ __device__ __forceinline__ float fresnel_dielectric(float cos_i, float eta_i, fl
     return 0.5f * (rpar * rpar + rper * rper);
 }
 
+// ------------------------------------------------------------- adjoint --
+// Partial derivatives of the BSDF values for the path-replay adjoint.  The
+// sampling (directions, pdfs, MIS weights) is detached, so only the values
+// of spec_f / diff_f depend on albedo and roughness.  No reference
+// counterpart (the reference is not differentiable).
+struct LobeGrad {
+    f3 da;  // d f_c / d albedo_c
+    f3 dr;  // d f_c / d roughness
+};
+// spec_f = F * D * G / den with F = f0 + (1 - f0) p, f0 = fres + (albedo - fres) m,
+// D = a2 / (pi q^2), a2 = r^4, q = nh^2 (a2 - 1) + 1, G = g(n.wi) g(n.wo),
+// g(x) = x / (x (1 - k) + k), k = (r + 1)^2 / 8.
+__device__ __forceinline__ LobeGrad spec_grad(f3 n, f3 wi, f3 wo, float r, float m, f3 alb, f3 fres) {
+    // D and G in FP64 as ggx_ndf / geo_att evaluate them (q loses all float
+    // precision near the lobe peak at low roughness)
+    f3 f0 = vlerp(fres, alb, m);
+    f3 wh = vnorm(vadd(wo, wi));
+    double ndwi = (double)fmaxf(vdot(n, wi), 0.f), ndwo = (double)fmaxf(vdot(n, wo), 0.f);
+    float nh = fmaxf(0.f, vdot(n, wh));
+    double rd = (double)r;
+    double a2 = (double)(r * r * r * r), nh2 = (double)(nh * nh);
+    double q = nh2 * (a2 - 1.0) + 1.0;
+    double D = a2 / ((double)F_PI * (q * q));
+    double dD = 4.0 * rd * rd * rd * (q - 2.0 * a2 * nh2) / ((double)F_PI * q * q * q);
+    float rp = r + 1.f;
+    double k = (double)((rp * rp) / 8.f), dk = 0.25 * (rd + 1.0);
+    double d1 = ndwi * (1.0 - k) + k, d2 = ndwo * (1.0 - k) + k;
+    double g1 = ndwi / d1, g2 = ndwo / d2;
+    double dg1 = -ndwi * (1.0 - ndwi) / (d1 * d1), dg2 = -ndwo * (1.0 - ndwo) / (d2 * d2);
+    double G = g1 * g2, dG = (dg1 * g2 + g1 * dg2) * dk;
+    float p = pow5(1.f - fmaxf(0.f, vdot(wh, wi)));
+    float den = fmaxf((float)(4.0 * ndwo * ndwi), 0.001f);
+    f3 F = vadd(f0, vscl(V3(1.f - f0.x, 1.f - f0.y, 1.f - f0.z), p));
+    LobeGrad g;
+    float sa = (float)((double)(m * (1.f - p)) * D * G / (double)den);
+    g.da = V3(sa, sa, sa);
+    g.dr = vscl(F, (float)((dD * G + D * dG) / (double)den));
+    return g;
+}
+// diff_f = (1 - F) (1 - m) albedo / pi
+__device__ __forceinline__ f3 diff_grad(f3 wi, f3 wo, float m, f3 alb, f3 fres) {
+    f3 f0 = vlerp(fres, alb, m);
+    f3 wh = vnorm(vadd(wo, wi));
+    float p = pow5(1.f - fmaxf(0.f, vdot(wh, wi)));
+    f3 F = vadd(f0, vscl(V3(1.f - f0.x, 1.f - f0.y, 1.f - f0.z), p));
+    float c = (1.f - m) * F_1_PI, s = m * (1.f - p);
+    return V3(c * ((1.f - F.x) - alb.x * s), c * ((1.f - F.y) - alb.y * s), c * ((1.f - F.z) - alb.z * s));
+}
+
+// The four texels and weights bilinear() reads at (u, v) (same arithmetic).
+struct Taps {
+    uint32_t i[4];
+    float w[4];
+};
+__device__ __forceinline__ Taps taps_of(int w, int h, float u, float v) {
+    float uu = u - floorf(u), vv = v - floorf(v);
+    float x = uu * (float)w - 0.5f, y = vv * (float)h - 0.5f;
+    float fx = floorf(x), fy = floorf(y);
+    float a = x - fx, b = y - fy;
+    int i0 = (int)fx, j0 = (int)fy;
+    i0 = ((i0 % w) + w) % w;
+    j0 = ((j0 % h) + h) % h;
+    int i1 = (i0 + 1) % w, j1 = (j0 + 1) % h;
+    Taps t;
+    t.i[0] = (uint32_t)(j0 * w + i0); t.i[1] = (uint32_t)(j0 * w + i1);
+    t.i[2] = (uint32_t)(j1 * w + i0); t.i[3] = (uint32_t)(j1 * w + i1);
+    t.w[0] = (1.f - a) * (1.f - b); t.w[1] = a * (1.f - b); t.w[2] = (1.f - a) * b; t.w[3] = a * b;
+    return t;
+}
+
+// Scale exponent of one fixed-point gradient element: every partial sum of
+// its terms is bounded by count * max|term| < 2^k, so terms are kept in
+// units of 2^-(61 - k) and the int64 sum cannot overflow.
+__device__ __forceinline__ int fixed_exponent(uint32_t max_bits, uint32_t count) {
+    double b = (double)__uint_as_float(max_bits) * (double)count;
+    if (!(b > 0.0)) return 0;
+    int k;
+    (void)frexp(b, &k);
+    return 61 - k;
+}
+
 // ------------------------------------------------------------ traversal --
 // Bounds3f::hit(ray, invDir, dirIsNeg) (CRT/Bounds3f.h:128-160) on one box:
 // the line test without [0, tmax] clipping; returns the final tmin/tmax.

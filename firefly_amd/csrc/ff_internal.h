@@ -23,7 +23,13 @@ hipError_t launch_shadow(const ff_paths& P, int variant, hipStream_t s);
 hipError_t launch_reset(uint32_t* counters, uint32_t mask, hipStream_t s);
 hipError_t launch_logic(const ff_paths& P, int q, uint32_t n_max, hipStream_t s);
 hipError_t launch_shade(int type, const ff_paths& P, int next_q, uint32_t n_max, hipStream_t s);
+hipError_t launch_logic_adj(const ff_paths& P, int q, uint32_t n_max, hipStream_t s);
+hipError_t launch_shade_nee(int type, const ff_paths& P, uint32_t n_max, hipStream_t s);
+hipError_t launch_shade_adj(int type, const ff_paths& P, int next_q, uint32_t n_max, hipStream_t s);
+hipError_t launch_fixed_to_float(const unsigned long long* fx, const uint32_t* mx, const uint32_t* cnt, uint64_t n,
+                                 float* out, hipStream_t s);
 hipError_t launch_fill4(float4* p, uint32_t n, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, uint32_t n, hipStream_t s);
 hipError_t launch_resolve_layers(const float4* layers, uint32_t K, uint32_t WH, float4* film, hipStream_t s);
 hipError_t launch_trace_closest(const ff_dev_scene& S, const float4* rays, uint32_t n,

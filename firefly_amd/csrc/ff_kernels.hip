@@ -82,7 +82,15 @@ __global__ __launch_bounds__(FF_BLOCK) void k_generate(ff_paths P) {
                 P.rngb[id] = base;
                 P.preq[id] = r;
                 P.beta[id] = make_float4(1.f, 1.f, 1.f, __uint_as_float(ST_PREV_DELTA));
-                P.Lp[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (R.mode & RQ_ADJOINT) {
+                    // replay: L_rem starts as the stored radiance of this path
+                    uint32_t sl = b * R.K + layer;
+                    P.Lp[id] = R.path_L[(size_t)sl * WH + pix];
+                    float4 g = R.d_img[pix];
+                    P.dlt[id] = make_float4(g.x * R.inv_spp, g.y * R.inv_spp, g.z * R.inv_spp, 0.f);
+                } else {
+                    P.Lp[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+                }
             }
         }
         uint32_t slot = wave_append(&P.counters[C_EXT0], act);
@@ -295,10 +303,14 @@ __global__ __launch_bounds__(FF_BLOCK) void k_logic(ff_paths P, int q) {
                 }
             }
             if (term) {
-                uint32_t local = id - R.slot_base;
-                float4* acc = R.K > 1 ? &R.layers[local] : &R.film[local % (R.W * R.H)];
+                uint32_t local = id - R.slot_base, WH = R.W * R.H;
+                float4* acc = R.K > 1 ? &R.layers[local] : &R.film[local % WH];
                 float4 L = *acc;
                 *acc = make_float4(L.x + Lp.x, L.y + Lp.y, L.z + Lp.z, L.w + 1.f);
+                if (R.mode & RQ_STORE_PATHS) {
+                    uint32_t sl = P.bstate[B_BATCH] * R.K + local / WH;
+                    R.path_L[(size_t)sl * WH + local % WH] = f4(Lp, 0.f);
+                }
             } else {
                 uint32_t nst = (stw & (ST_INSIDE | ST_PREV_DELTA)) | len;
                 P.beta[id] = f4(beta, __uint_as_float(nst));
@@ -318,11 +330,238 @@ __global__ __launch_bounds__(FF_BLOCK) void k_logic(ff_paths P, int q) {
 }
 
 // ======================================================= material queues
+// Everything the shading of one vertex produces.  With GRAD the partial
+// derivatives of its terms are produced too (path-replay adjoint).
+struct ShadeOut {
+    f3 A, Bs, Cd;       // light-sample term and BRDF-sampled spec / diffuse terms (before visibility)
+    f3 wi, wi_s, wi_d;  // their directions
+    f3 lo, so, dd_o;    // shadow ray origins
+    bool needL, needS, needD;
+    f3 fext;
+    float pdf_ext, cos_ext;
+    f3 next_o, next_d;
+    uint32_t nst;       // state of the continuation
+    // GRAD only
+    bool a_on, rough_live;
+    int at, rt;
+    float tu, tv;
+    f3 dA_alb, dB_alb, dC_alb, dA_r, dB_r;  // d term_c / d albedo_c, d term_c / d roughness
+    f3 kA, kB, kC;                         // d term_c / d Le_c
+    f3 dext_alb, dext_r;
+};
+
 // Shading of one material class (wf_mat_mix / wf_mat_diffuse / wf_mat_cook,
 // CRT/kernel.cu:545-651, lobes of CRT/dMaterial.cu).  The environment light
 // sample of wf_logic (CRT/kernel.cu:435-439) is drawn here; the visibility
 // rays of BRDF_L's BRDF-sampled MIS terms (CRT/dMaterial.cu:404-448) are
 // deferred to the shadow stage with their contributions kept per path.
+template <int TYPE, bool GRAD>
+__device__ __forceinline__ void shade_core(const ff_paths& P, uint32_t id, ShadeOut& so) {
+    const ff_dev_request& R = P.reqs[P.preq[id]];
+    const ff_dev_scene& S = R.S;
+    EnvView E = env_view(R.E);
+    float4 bw = P.beta[id];
+    uint32_t stw = __float_as_uint(bw.w);
+    uint32_t len = stw & ST_LEN_MASK;
+    bool inside = (stw & ST_INSIDE) != 0;
+    float4 hh = P.hit[id];
+    int slot = __float_as_int(hh.w);
+    float4 r0 = P.ray[2 * id], r1 = P.ray[2 * id + 1];
+    f3 o = xyz(r0), d = xyz(r1);
+    uint32_t base = P.rngb[id];
+    f3 Pp = vadd(o, vscl(d, hh.x));
+    const float* N = S.nrm + (size_t)slot * 9;
+    float w0 = (1.f - hh.y) - hh.z;
+    f3 nrm = vnorm(vadd(vadd(vscl(ld3(N + 3), hh.y), vscl(ld3(N + 6), hh.z)), vscl(ld3(N), w0)));
+    const float* U = S.uv + (size_t)slot * 6;
+    float tu = (U[2] * hh.y + U[4] * hh.z) + U[0] * w0;
+    float tv = (U[3] * hh.y + U[5] * hh.z) + U[1] * w0;
+    const float* M = S.mats + (size_t)S.mat[slot] * FF_MAT_FLOATS_DEV;
+    f3 alb = V3(M[1], M[2], M[3]);
+    float rough = M[4], metal = M[5];
+    f3 fres = V3(M[6], M[7], M[8]);
+    int at = (int)M[10], rt = (int)M[11];
+    bool rough_live = false;
+    if (at >= 0) {
+        ff_tex_desc td = S.tex[at];
+        float t3[3];
+        bilinear(td.data, td.w, td.h, 3, tu, tv, t3);
+        alb = ld3(t3);
+    }
+    if (rt >= 0) {
+        ff_tex_desc td = S.tex[rt];
+        float t1[1];
+        bilinear(td.data, td.w, td.h, 1, tu, tv, t1);
+        rough = fmaxf(t1[0], 0.01f);
+        rough_live = t1[0] > 0.01f;
+    }
+    f3 wo = vneg(d);
+    f3 zero = V3(0, 0, 0);
+    so.A = zero; so.Bs = zero; so.Cd = zero;
+    so.wi = zero; so.wi_s = zero; so.wi_d = zero;
+    so.lo = Pp; so.so = Pp; so.dd_o = Pp;
+    so.needL = so.needS = so.needD = false;
+    if (GRAD) {
+        so.a_on = false;
+        so.rough_live = rough_live;
+        so.at = at; so.rt = rt; so.tu = tu; so.tv = tv;
+        so.dA_alb = zero; so.dB_alb = zero; so.dC_alb = zero; so.dA_r = zero; so.dB_r = zero;
+        so.kA = zero; so.kB = zero; so.kC = zero; so.dext_alb = zero; so.dext_r = zero;
+    }
+    uint32_t nst = (stw & (ST_INSIDE | ST_PREV_DELTA | ST_LEN_MASK));
+    if (TYPE == MAT_DIELECTRIC) {
+        Tri T = load_tri(S.tris, slot);
+        f3 gn = vnorm(vcross(T.e1, T.e2));
+        float cs = vdot(d, gn);
+        bool entering = cs < 0.f;
+        f3 nn = entering ? gn : vneg(gn);
+        float cos_i = fabsf(cs);
+        if (cos_i > 1.f) cos_i = 1.f;
+        float ior = M[9];
+        float eta_i = entering ? 1.f : ior, eta_t = entering ? ior : 1.f;
+        float F = fresnel_dielectric(cos_i, eta_i, eta_t);
+        if (rng(base, len, DIM_DIEL) < F) {
+            so.next_d = vnorm(vadd(d, vscl(nn, 2.f * cos_i)));
+            so.next_o = vadd(Pp, vscl(nn, 1e-4f));
+        } else {
+            float eta = eta_i / eta_t;
+            float k = 1.f - eta * eta * (1.f - cos_i * cos_i);
+            float cos_t = sqrtf(fmaxf(k, 0.f));
+            so.next_d = vnorm(vadd(vscl(d, eta), vscl(nn, eta * cos_i - cos_t)));
+            so.next_o = vsub(Pp, vscl(nn, 1e-4f));
+            inside = !inside;
+        }
+        so.fext = V3(1.f, 1.f, 1.f);
+        so.pdf_ext = 1.f;
+        so.cos_ext = 1.f;
+        so.nst = (nst & ST_LEN_MASK) | ST_PREV_DELTA | (inside ? ST_INSIDE : 0u);
+        return;
+    }
+    constexpr bool do_spec = (TYPE == MAT_MIX || TYPE == MAT_CONDUCTOR);
+    constexpr bool do_diff = (TYPE == MAT_MIX || TYPE == MAT_DIFFUSE);
+    f3 wi = env_sample(E, Pp, rng(base, len, DIM_EX), rng(base, len, DIM_EY));
+    so.wi = wi;
+    double diff_pdf = (double)(1.f / (2.f * F_PI));
+    // light-sample term of BRDF_L (CRT/dMaterial.cu:376-402)
+    f3 Li = env_L(E, wi);
+    float ndwi = vdot(nrm, wi);
+    if (ndwi > 0.f) {
+        double sp = do_spec ? (double)spec_pdf(nrm, wi, wo, rough) : 0.0;
+        double lp = (double)env_pdf(E, wi);
+        // lp is infinite where sin(acos(wi.z)) == 0; the term's limit there is 0
+        if (lp > 0.f && lp < (double)INFINITY && !vzero(Li)) {
+            double dwt = power_heur(lp, diff_pdf), swt = power_heur(lp, sp);
+            if (do_spec) {
+                f3 f = vscl(spec_f(nrm, wi, wo, rough, metal, alb, fres), ndwi);
+                if (!vzero(f)) {
+                    so.A = vadd(so.A, vdiv(vmul(vscl(f, (float)swt), Li), (float)lp));
+                    if (GRAD) {
+                        LobeGrad g = spec_grad(nrm, wi, wo, rough, metal, alb, fres);
+                        float s = ndwi * (float)swt / (float)lp;
+                        so.a_on = true;
+                        so.kA = vadd(so.kA, vscl(f, (float)swt / (float)lp));
+                        so.dA_alb = vadd(so.dA_alb, vmul(vscl(g.da, s), Li));
+                        so.dA_r = vadd(so.dA_r, vmul(vscl(g.dr, s), Li));
+                    }
+                }
+            }
+            if (do_diff) {
+                f3 f = vscl(diff_f(wi, wo, metal, alb, fres), ndwi);
+                if (!vzero(f)) {
+                    so.A = vadd(so.A, vdiv(vmul(vscl(f, (float)dwt), Li), (float)lp));
+                    if (GRAD) {
+                        f3 da = diff_grad(wi, wo, metal, alb, fres);
+                        float s = ndwi * (float)dwt / (float)lp;
+                        so.a_on = true;
+                        so.kA = vadd(so.kA, vscl(f, (float)dwt / (float)lp));
+                        so.dA_alb = vadd(so.dA_alb, vmul(vscl(da, s), Li));
+                    }
+                }
+            }
+        }
+    }
+    // BRDF-sampled MIS terms (CRT/dMaterial.cu:404-448)
+    f3 wi_s = zero, wi_d = zero;
+    if (do_spec) wi_s = spec_sample(nrm, wo, rough, rng(base, len, DIM_S0), rng(base, len, DIM_S1));
+    if (do_diff) wi_d = diff_sample(nrm, rng(base, len, DIM_D0), rng(base, len, DIM_D1));
+    so.wi_s = wi_s;
+    so.wi_d = wi_d;
+    if (do_spec) {
+        float nd = vdot(nrm, wi_s);
+        if (nd > 0.f) {
+            double sp = (double)spec_pdf(nrm, wi_s, wo, rough);
+            double lp = (double)env_pdf(E, wi_s);
+            double wt = power_heur(sp, lp);
+            f3 f = vscl(spec_f(nrm, wi_s, wo, rough, metal, alb, fres), nd);
+            f3 L2 = env_L(E, wi_s);
+            if (!vzero(f) && sp > 0.f && !vzero(L2)) {
+                so.Bs = vdiv(vscl(vmul(f, L2), (float)wt), (float)sp);
+                so.needS = true;
+                so.so = vadd(Pp, vscl(wi_s, 0.001f));
+                if (GRAD) {
+                    LobeGrad g = spec_grad(nrm, wi_s, wo, rough, metal, alb, fres);
+                    float s = nd * (float)wt / (float)sp;
+                    so.kB = vscl(f, (float)wt / (float)sp);
+                    so.dB_alb = vmul(vscl(g.da, s), L2);
+                    so.dB_r = vmul(vscl(g.dr, s), L2);
+                }
+            }
+        }
+    }
+    if (do_diff) {
+        float nd = vdot(nrm, wi_d);
+        if (nd > 0.f) {
+            double lp = (double)env_pdf(E, wi_d);
+            double wt = power_heur(diff_pdf, lp);
+            f3 f = vscl(diff_f(wi_d, wo, metal, alb, fres), nd);
+            f3 L3 = env_L(E, wi_d);
+            if (!vzero(f) && diff_pdf > 0.f && !vzero(L3)) {
+                so.Cd = vdiv(vscl(vmul(f, L3), (float)wt), (float)diff_pdf);
+                so.needD = true;
+                // the mixed material offsets this ray along wi_spec (CRT/dMaterial.cu:431)
+                so.dd_o = vadd(Pp, vscl(TYPE == MAT_MIX ? wi_s : wi_d, 0.001f));
+                if (GRAD) {
+                    f3 da = diff_grad(wi_d, wo, metal, alb, fres);
+                    float s = nd * (float)wt / (float)diff_pdf;
+                    so.kC = vscl(f, (float)wt / (float)diff_pdf);
+                    so.dC_alb = vmul(vscl(da, s), L3);
+                }
+            }
+        }
+    }
+    so.needL = !vzero(so.A) || so.needS || so.needD;
+    so.lo = vadd(Pp, vscl(nrm, 0.001f));
+    f3 dir;
+    if (TYPE == MAT_MIX) {
+        dir = (rng(base, len, DIM_LOBE) < 0.5f) ? wi_d : wi_s;
+        so.fext = vadd(spec_f(nrm, dir, wo, rough, metal, alb, fres), diff_f(dir, wo, metal, alb, fres));
+        float spf = spec_pdf(nrm, dir, wo, rough);
+        float dpf = (float)diff_pdf;
+        so.pdf_ext = (float)(0.5 * (double)(spf + dpf));
+    } else if (TYPE == MAT_DIFFUSE) {
+        dir = wi_d;
+        so.fext = diff_f(dir, wo, metal, alb, fres);
+        so.pdf_ext = (float)diff_pdf;
+    } else {
+        dir = wi_s;
+        so.fext = spec_f(nrm, dir, wo, rough, metal, alb, fres);
+        so.pdf_ext = spec_pdf(nrm, dir, wo, rough);
+    }
+    if (GRAD) {
+        if (do_spec) {
+            LobeGrad g = spec_grad(nrm, dir, wo, rough, metal, alb, fres);
+            so.dext_alb = g.da;
+            so.dext_r = g.dr;
+        }
+        if (do_diff) so.dext_alb = vadd(so.dext_alb, diff_grad(dir, wo, metal, alb, fres));
+    }
+    so.cos_ext = fmaxf(0.0f, vdot(nrm, dir));
+    so.next_o = Pp;
+    so.next_d = dir;
+    so.nst = (nst & (ST_LEN_MASK | ST_INSIDE)) | (so.needL ? ST_NEED_L : 0u) | (so.needS ? ST_NEED_S : 0u) |
+             (so.needD ? ST_NEED_D : 0u);
+}
+
 template <int TYPE>
 __global__ __launch_bounds__(FF_BLOCK) void k_shade(ff_paths P, int next_q) {
     const uint32_t n = P.counters[C_MAT0 + TYPE];
@@ -334,174 +573,28 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade(ff_paths P, int next_q) {
         uint32_t id = live ? queue[i] : 0u;
         bool needL = false, needS = false, needD = false;
         if (live) {
-            const ff_dev_request& R = P.reqs[P.preq[id]];
-            const ff_dev_scene& S = R.S;
-            EnvView E = env_view(R.E);
-            float4 bw = P.beta[id];
-            uint32_t stw = __float_as_uint(bw.w);
-            uint32_t len = stw & ST_LEN_MASK;
-            bool inside = (stw & ST_INSIDE) != 0;
-            float4 hh = P.hit[id];
-            int slot = __float_as_int(hh.w);
-            float4 r0 = P.ray[2 * id], r1 = P.ray[2 * id + 1];
-            f3 o = xyz(r0), d = xyz(r1);
-            uint32_t base = P.rngb[id];
-            f3 Pp = vadd(o, vscl(d, hh.x));
-            const float* N = S.nrm + (size_t)slot * 9;
-            float w0 = (1.f - hh.y) - hh.z;
-            f3 nrm = vnorm(vadd(vadd(vscl(ld3(N + 3), hh.y), vscl(ld3(N + 6), hh.z)), vscl(ld3(N), w0)));
-            const float* U = S.uv + (size_t)slot * 6;
-            float tu = (U[2] * hh.y + U[4] * hh.z) + U[0] * w0;
-            float tv = (U[3] * hh.y + U[5] * hh.z) + U[1] * w0;
-            const float* M = S.mats + (size_t)S.mat[slot] * FF_MAT_FLOATS_DEV;
-            f3 alb = V3(M[1], M[2], M[3]);
-            float rough = M[4], metal = M[5];
-            f3 fres = V3(M[6], M[7], M[8]);
-            int at = (int)M[10], rt = (int)M[11];
-            if (at >= 0) {
-                ff_tex_desc td = S.tex[at];
-                float t3[3];
-                bilinear(td.data, td.w, td.h, 3, tu, tv, t3);
-                alb = ld3(t3);
+            ShadeOut so;
+            shade_core<TYPE, false>(P, id, so);
+            needL = so.needL; needS = so.needS; needD = so.needD;
+            if (needL) {
+                P.sray[6 * id + 0] = f4(so.lo, 0.f);
+                P.sray[6 * id + 1] = f4(so.wi, 0.f);
             }
-            if (rt >= 0) {
-                ff_tex_desc td = S.tex[rt];
-                float t1[1];
-                bilinear(td.data, td.w, td.h, 1, tu, tv, t1);
-                rough = fmaxf(t1[0], 0.01f);
+            if (needS) {
+                P.sray[6 * id + 2] = f4(so.so, 0.f);
+                P.sray[6 * id + 3] = f4(so.wi_s, 0.f);
             }
-            f3 wo = vneg(d);
-            f3 next_o, next_d, fext;
-            float pdf_ext, cos_ext;
-            f3 A = V3(0, 0, 0), Bs = A, Cd = A;
-            uint32_t nst = (stw & (ST_INSIDE | ST_PREV_DELTA | ST_LEN_MASK));
-            if (TYPE == MAT_DIELECTRIC) {
-                Tri T = load_tri(S.tris, slot);
-                f3 gn = vnorm(vcross(T.e1, T.e2));
-                float cs = vdot(d, gn);
-                bool entering = cs < 0.f;
-                f3 nn = entering ? gn : vneg(gn);
-                float cos_i = fabsf(cs);
-                if (cos_i > 1.f) cos_i = 1.f;
-                float ior = M[9];
-                float eta_i = entering ? 1.f : ior, eta_t = entering ? ior : 1.f;
-                float F = fresnel_dielectric(cos_i, eta_i, eta_t);
-                if (rng(base, len, DIM_DIEL) < F) {
-                    next_d = vnorm(vadd(d, vscl(nn, 2.f * cos_i)));
-                    next_o = vadd(Pp, vscl(nn, 1e-4f));
-                } else {
-                    float eta = eta_i / eta_t;
-                    float k = 1.f - eta * eta * (1.f - cos_i * cos_i);
-                    float cos_t = sqrtf(fmaxf(k, 0.f));
-                    next_d = vnorm(vadd(vscl(d, eta), vscl(nn, eta * cos_i - cos_t)));
-                    next_o = vsub(Pp, vscl(nn, 1e-4f));
-                    inside = !inside;
-                }
-                fext = V3(1.f, 1.f, 1.f);
-                pdf_ext = 1.f;
-                cos_ext = 1.f;
-                nst = (nst & ST_LEN_MASK) | ST_PREV_DELTA | (inside ? ST_INSIDE : 0u);
-            } else {
-                constexpr bool do_spec = (TYPE == MAT_MIX || TYPE == MAT_CONDUCTOR);
-                constexpr bool do_diff = (TYPE == MAT_MIX || TYPE == MAT_DIFFUSE);
-                f3 wi = env_sample(E, Pp, rng(base, len, DIM_EX), rng(base, len, DIM_EY));
-                double diff_pdf = (double)(1.f / (2.f * F_PI));
-                // light-sample term of BRDF_L (CRT/dMaterial.cu:376-402)
-                f3 Li = env_L(E, wi);
-                float ndwi = vdot(nrm, wi);
-                if (ndwi > 0.f) {
-                    double sp = do_spec ? (double)spec_pdf(nrm, wi, wo, rough) : 0.0;
-                    double lp = (double)env_pdf(E, wi);
-                    // lp is infinite where sin(acos(wi.z)) == 0; the term's limit there is 0
-                    if (lp > 0.f && lp < (double)INFINITY && !vzero(Li)) {
-                        double dwt = power_heur(lp, diff_pdf), swt = power_heur(lp, sp);
-                        if (do_spec) {
-                            f3 f = vscl(spec_f(nrm, wi, wo, rough, metal, alb, fres), ndwi);
-                            if (!vzero(f)) A = vadd(A, vdiv(vmul(vscl(f, (float)swt), Li), (float)lp));
-                        }
-                        if (do_diff) {
-                            f3 f = vscl(diff_f(wi, wo, metal, alb, fres), ndwi);
-                            if (!vzero(f)) A = vadd(A, vdiv(vmul(vscl(f, (float)dwt), Li), (float)lp));
-                        }
-                    }
-                }
-                // BRDF-sampled MIS terms (CRT/dMaterial.cu:404-448)
-                f3 wi_s = V3(0, 0, 0), wi_d = wi_s;
-                if (do_spec) wi_s = spec_sample(nrm, wo, rough, rng(base, len, DIM_S0), rng(base, len, DIM_S1));
-                if (do_diff) wi_d = diff_sample(nrm, rng(base, len, DIM_D0), rng(base, len, DIM_D1));
-                f3 so = Pp, dd_o = Pp;
-                if (do_spec) {
-                    float nd = vdot(nrm, wi_s);
-                    if (nd > 0.f) {
-                        double sp = (double)spec_pdf(nrm, wi_s, wo, rough);
-                        double lp = (double)env_pdf(E, wi_s);
-                        double wt = power_heur(sp, lp);
-                        f3 f = vscl(spec_f(nrm, wi_s, wo, rough, metal, alb, fres), nd);
-                        f3 L2 = env_L(E, wi_s);
-                        if (!vzero(f) && sp > 0.f && !vzero(L2)) {
-                            Bs = vdiv(vscl(vmul(f, L2), (float)wt), (float)sp);
-                            needS = true;
-                            so = vadd(Pp, vscl(wi_s, 0.001f));
-                        }
-                    }
-                }
-                if (do_diff) {
-                    float nd = vdot(nrm, wi_d);
-                    if (nd > 0.f) {
-                        double lp = (double)env_pdf(E, wi_d);
-                        double wt = power_heur(diff_pdf, lp);
-                        f3 f = vscl(diff_f(wi_d, wo, metal, alb, fres), nd);
-                        f3 L3 = env_L(E, wi_d);
-                        if (!vzero(f) && diff_pdf > 0.f && !vzero(L3)) {
-                            Cd = vdiv(vscl(vmul(f, L3), (float)wt), (float)diff_pdf);
-                            needD = true;
-                            // the mixed material offsets this ray along wi_spec (CRT/dMaterial.cu:431)
-                            dd_o = vadd(Pp, vscl(TYPE == MAT_MIX ? wi_s : wi_d, 0.001f));
-                        }
-                    }
-                }
-                needL = !vzero(A) || needS || needD;
-                f3 dir;
-                if (TYPE == MAT_MIX) {
-                    dir = (rng(base, len, DIM_LOBE) < 0.5f) ? wi_d : wi_s;
-                    fext = vadd(spec_f(nrm, dir, wo, rough, metal, alb, fres), diff_f(dir, wo, metal, alb, fres));
-                    float spf = spec_pdf(nrm, dir, wo, rough);
-                    float dpf = (float)diff_pdf;
-                    pdf_ext = (float)(0.5 * (double)(spf + dpf));
-                } else if (TYPE == MAT_DIFFUSE) {
-                    dir = wi_d;
-                    fext = diff_f(dir, wo, metal, alb, fres);
-                    pdf_ext = (float)diff_pdf;
-                } else {
-                    dir = wi_s;
-                    fext = spec_f(nrm, dir, wo, rough, metal, alb, fres);
-                    pdf_ext = spec_pdf(nrm, dir, wo, rough);
-                }
-                cos_ext = fmaxf(0.0f, vdot(nrm, dir));
-                next_o = Pp;
-                next_d = dir;
-                nst = (nst & (ST_LEN_MASK | ST_INSIDE)) | (needL ? ST_NEED_L : 0u) |
-                      (needS ? ST_NEED_S : 0u) | (needD ? ST_NEED_D : 0u);
-                if (needL) {
-                    P.sray[6 * id + 0] = f4(vadd(Pp, vscl(nrm, 0.001f)), 0.f);
-                    P.sray[6 * id + 1] = f4(wi, 0.f);
-                }
-                if (needS) {
-                    P.sray[6 * id + 2] = f4(so, 0.f);
-                    P.sray[6 * id + 3] = f4(wi_s, 0.f);
-                }
-                if (needD) {
-                    P.sray[6 * id + 4] = f4(dd_o, 0.f);
-                    P.sray[6 * id + 5] = f4(wi_d, 0.f);
-                }
+            if (needD) {
+                P.sray[6 * id + 4] = f4(so.dd_o, 0.f);
+                P.sray[6 * id + 5] = f4(so.wi_d, 0.f);
             }
-            P.nee[3 * id] = f4(A, cos_ext);
-            P.nee[3 * id + 1] = f4(Bs, 0.f);
-            P.nee[3 * id + 2] = f4(Cd, 0.f);
-            P.fext[id] = f4(fext, pdf_ext);
-            P.ray[2 * id] = f4(next_o, 0.f);
-            P.ray[2 * id + 1] = f4(next_d, 0.f);
-            P.beta[id].w = __uint_as_float(nst);
+            P.nee[3 * id] = f4(so.A, so.cos_ext);
+            P.nee[3 * id + 1] = f4(so.Bs, 0.f);
+            P.nee[3 * id + 2] = f4(so.Cd, 0.f);
+            P.fext[id] = f4(so.fext, so.pdf_ext);
+            P.ray[2 * id] = f4(so.next_o, 0.f);
+            P.ray[2 * id + 1] = f4(so.next_d, 0.f);
+            P.beta[id].w = __uint_as_float(so.nst);
         }
         uint32_t es = wave_append(ext_counter, live);
         if (live) P.q_ext[next_q][es] = id;
@@ -517,9 +610,252 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade(ff_paths P, int next_q) {
     }
 }
 
+// ================================================== path-replay adjoint
+// The adjoint replays every stored forward path with the same random
+// numbers (same camera ray, same hits, same samples) and, at each event,
+// moves d loss / d radiance onto the parameters the event's radiance depends
+// on: Le through environment texels, f through albedo / roughness texels.
+// L_rem (in Lp) starts as the path's stored radiance and loses each term as
+// it is replayed, so at a vertex it is the radiance that flows through the
+// vertex's continuation; its derivative w.r.t. f is L_rem * (df/dtheta) / f
+// (path replay backpropagation).  Sampling is detached: directions, pdfs,
+// MIS weights and Russian roulette are those of the forward.
+
+// One term of d loss / d parameter into gradient target element i.
+__device__ __forceinline__ void grad_add(const ff_paths& P, const ff_dev_gtarget& T, uint32_t i, float v) {
+    if (v == 0.f) return;
+    if (P.gmode == GM_FLOAT) {
+        unsafeAtomicAdd(T.buf + i, v);
+    } else if (P.gmode == GM_BOUND) {
+        uint64_t a = T.off + i;
+        atomicMax(&P.g_max[a], __float_as_uint(fabsf(v)));
+        atomicAdd(&P.g_cnt[a], 1u);
+    } else {
+        uint64_t a = T.off + i;
+        int e = fixed_exponent(P.g_max[a], P.g_cnt[a]);
+        long long qv = __double2ll_rn(ldexp((double)v, e));
+        atomicAdd(&P.g_fx[a], (unsigned long long)qv);
+    }
+}
+// d loss / d Le(dir) = val (per channel) onto the environment texels
+__device__ __forceinline__ void grad_env(const ff_paths& P, const ff_dev_request& R, f3 dir, f3 val) {
+    const ff_dev_gtarget& T = R.gt[GT_ENV];
+    if (!T.buf) return;
+    float u, v;
+    sph_map(dir, u, v);
+    Taps tp = taps_of(R.E.w, R.E.h, u, v);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        grad_add(P, T, 3 * tp.i[k] + 0, tp.w[k] * val.x);
+        grad_add(P, T, 3 * tp.i[k] + 1, tp.w[k] * val.y);
+        grad_add(P, T, 3 * tp.i[k] + 2, tp.w[k] * val.z);
+    }
+}
+// d loss / d (texture t value at (u, v)) = val[0..c) onto its texels
+template <int C>
+__device__ __forceinline__ void grad_tex(const ff_paths& P, const ff_dev_request& R, int t, float u, float v,
+                                         const float* val) {
+    const ff_dev_gtarget& T = R.gt[1 + t];
+    if (!T.buf) return;
+    const ff_tex_desc& td = R.S.tex[t];
+    Taps tp = taps_of(td.w, td.h, u, v);
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int c = 0; c < C; c++) grad_add(P, T, C * tp.i[k] + c, tp.w[k] * val[c]);
+}
+
+// wf_logic of the replay: same termination, Russian roulette and routing as
+// k_logic; an escaping delta/camera ray's Le term is replayed.  The vertex's
+// NEE terms and throughput update were replayed by k_shade_adj.
+__global__ __launch_bounds__(FF_BLOCK) void k_logic_adj(ff_paths P, int q) {
+    const uint32_t n = P.counters[q == 0 ? C_EXT0 : C_EXT1];
+    const uint32_t* queue = P.q_ext[q];
+    const uint32_t n_round = (n + 63u) & ~63u;
+    for (uint32_t i = gtid(); i < n_round; i += gstride()) {
+        bool live = i < n;
+        int route = -1;
+        bool term = false, primary = false;
+        uint32_t id = live ? queue[i] : 0u;
+        if (live) {
+            const ff_dev_request& R = P.reqs[P.preq[id]];
+            const ff_dev_scene& S = R.S;
+            float4 bw = P.beta[id];
+            uint32_t stw = __float_as_uint(bw.w);
+            uint32_t len = (stw & ST_LEN_MASK) + 1;
+            primary = len == 1;
+            f3 beta = xyz(bw);
+            f3 Lrem = xyz(P.Lp[id]);
+            float4 hh = P.hit[id];
+            int slot = __float_as_int(hh.w);
+            bool found = slot >= 0;
+            term = (len > R.depth) || !found || (stw & ST_DEAD);
+            int mtype = -1;
+            f3 d = xyz(P.ray[2 * id + 1]);
+            if (found) {
+                mtype = (int)S.mats[(size_t)S.mat[slot] * FF_MAT_FLOATS_DEV];
+                if ((stw & ST_INSIDE) && mtype != MAT_DIELECTRIC) {
+                    Tri T = load_tri(S.tris, slot);
+                    if (!(vdot(d, vcross(T.e1, T.e2)) < 0.f)) term = true;  // absorbed
+                }
+            }
+            if (!found && (stw & ST_PREV_DELTA)) {
+                Lrem = vsub(Lrem, vmul(beta, env_L(env_view(R.E), d)));
+                grad_env(P, R, d, vmul(xyz(P.dlt[id]), beta));
+            }
+            if (!(stw & ST_DEAD) && len > 3) {
+                float qrr = fmaxf(0.05f, 1.f - beta.y);
+                if (rng(P.rngb[id], len, DIM_RR) < qrr) term = true;
+            }
+            if (!term) {
+                uint32_t nst = (stw & (ST_INSIDE | ST_PREV_DELTA)) | len;
+                P.beta[id] = f4(beta, __uint_as_float(nst));
+                P.Lp[id] = f4(Lrem, 0.f);
+                route = mtype;
+            }
+        }
+        wave_count(&P.stats[S_PRIMARY], live && primary);
+        wave_count(&P.stats[S_EXTENSION], live && !primary);
+        wave_count(&P.stats[S_PATHS], live && term);
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            uint32_t slot = wave_append(&P.counters[C_MAT0 + m], route == m);
+            if (route == m) P.q_mat[m][slot] = id;
+        }
+    }
+}
+
+// Replay of a vertex's shadow rays: the same rays k_shade queues.
+template <int TYPE>
+__global__ __launch_bounds__(FF_BLOCK) void k_shade_nee(ff_paths P) {
+    const uint32_t n = P.counters[C_MAT0 + TYPE];
+    const uint32_t* queue = P.q_mat[TYPE];
+    const uint32_t n_round = (n + 63u) & ~63u;
+    for (uint32_t i = gtid(); i < n_round; i += gstride()) {
+        bool live = i < n;
+        uint32_t id = live ? queue[i] : 0u;
+        bool needL = false, needS = false, needD = false;
+        if (live) {
+            ShadeOut so;
+            shade_core<TYPE, false>(P, id, so);
+            needL = so.needL; needS = so.needS; needD = so.needD;
+            if (needL) {
+                P.sray[6 * id + 0] = f4(so.lo, 0.f);
+                P.sray[6 * id + 1] = f4(so.wi, 0.f);
+            }
+            if (needS) {
+                P.sray[6 * id + 2] = f4(so.so, 0.f);
+                P.sray[6 * id + 3] = f4(so.wi_s, 0.f);
+            }
+            if (needD) {
+                P.sray[6 * id + 4] = f4(so.dd_o, 0.f);
+                P.sray[6 * id + 5] = f4(so.wi_d, 0.f);
+            }
+        }
+        wave_count(&P.stats[S_SHADOW], needL);
+        wave_count(&P.stats[S_SHADOW], needS);
+        wave_count(&P.stats[S_SHADOW], needD);
+        uint32_t s0 = wave_append(&P.counters[C_SHADOW], needL);
+        if (needL) P.q_shadow[s0] = 3 * id + 0;
+        uint32_t s1 = wave_append(&P.counters[C_SHADOW], needS);
+        if (needS) P.q_shadow[s1] = 3 * id + 1;
+        uint32_t s2 = wave_append(&P.counters[C_SHADOW], needD);
+        if (needD) P.q_shadow[s2] = 3 * id + 2;
+    }
+}
+
+// Replay of a vertex once its shadow rays are traced: its visibility-gated
+// NEE terms leave L_rem (with their gradients), then the continuation's
+// gradient L_rem * (df/dtheta) / f, the throughput update of wf_logic and the
+// next extension ray.
+template <int TYPE>
+__global__ __launch_bounds__(FF_BLOCK) void k_shade_adj(ff_paths P, int next_q) {
+    const uint32_t n = P.counters[C_MAT0 + TYPE];
+    const uint32_t* queue = P.q_mat[TYPE];
+    const uint32_t n_round = (n + 63u) & ~63u;
+    uint32_t* ext_counter = &P.counters[next_q == 0 ? C_EXT0 : C_EXT1];
+    for (uint32_t i = gtid(); i < n_round; i += gstride()) {
+        bool live = i < n;
+        uint32_t id = live ? queue[i] : 0u;
+        if (live) {
+            const ff_dev_request& R = P.reqs[P.preq[id]];
+            ShadeOut so;
+            shade_core<TYPE, TYPE != MAT_DIELECTRIC>(P, id, so);
+            f3 beta = xyz(P.beta[id]);
+            uint32_t nst = so.nst;
+            if (TYPE != MAT_DIELECTRIC) {
+                f3 Lrem = xyz(P.Lp[id]);
+                f3 dl = xyz(P.dlt[id]);
+                bool visL = so.needL && P.vis[3 * id + 0];
+                bool visS = visL && so.needS && P.vis[3 * id + 1];
+                bool visD = visL && so.needD && P.vis[3 * id + 2];
+                if (visL) {
+                    f3 lb = so.A;
+                    if (visS) lb = vadd(lb, so.Bs);
+                    if (visD) lb = vadd(lb, so.Cd);
+                    Lrem = vsub(Lrem, vmul(lb, beta));
+                    f3 w = vmul(dl, beta);
+                    if (so.at >= 0) {
+                        f3 ga = so.dA_alb;
+                        if (visS) ga = vadd(ga, so.dB_alb);
+                        if (visD) ga = vadd(ga, so.dC_alb);
+                        float g3[3] = {w.x * ga.x, w.y * ga.y, w.z * ga.z};
+                        grad_tex<3>(P, R, so.at, so.tu, so.tv, g3);
+                    }
+                    if (so.rt >= 0 && so.rough_live) {
+                        f3 gr = so.dA_r;
+                        if (visS) gr = vadd(gr, so.dB_r);
+                        float g1 = vdot(w, gr);
+                        grad_tex<1>(P, R, so.rt, so.tu, so.tv, &g1);
+                    }
+                    if (so.a_on) grad_env(P, R, so.wi, vmul(w, so.kA));
+                    if (visS) grad_env(P, R, so.wi_s, vmul(w, so.kB));
+                    if (visD) grad_env(P, R, so.wi_d, vmul(w, so.kC));
+                }
+                bool dead = vzero(so.fext) || so.pdf_ext == 0.f;
+                if (!dead) {
+                    f3 qv = V3(so.fext.x != 0.f ? dl.x * Lrem.x / so.fext.x : 0.f,
+                               so.fext.y != 0.f ? dl.y * Lrem.y / so.fext.y : 0.f,
+                               so.fext.z != 0.f ? dl.z * Lrem.z / so.fext.z : 0.f);
+                    if (so.at >= 0) {
+                        float g3[3] = {qv.x * so.dext_alb.x, qv.y * so.dext_alb.y, qv.z * so.dext_alb.z};
+                        grad_tex<3>(P, R, so.at, so.tu, so.tv, g3);
+                    }
+                    if (so.rt >= 0 && so.rough_live) {
+                        float g1 = vdot(qv, so.dext_r);
+                        grad_tex<1>(P, R, so.rt, so.tu, so.tv, &g1);
+                    }
+                    beta = vmul(beta, vdiv(vscl(so.fext, so.cos_ext), so.pdf_ext));
+                }
+                nst = (nst & (ST_LEN_MASK | ST_INSIDE)) | (dead ? ST_DEAD : 0u);
+                P.Lp[id] = f4(Lrem, 0.f);
+            }
+            P.beta[id] = f4(beta, __uint_as_float(nst));
+            P.ray[2 * id] = f4(so.next_o, 0.f);
+            P.ray[2 * id + 1] = f4(so.next_d, 0.f);
+        }
+        uint32_t es = wave_append(ext_counter, live);
+        if (live) P.q_ext[next_q][es] = id;
+    }
+}
+
+// Adjoint-mode converter: fixed-point sums -> caller's float gradients.
+__global__ void k_fixed_to_float(const unsigned long long* fx, const uint32_t* mx, const uint32_t* cnt,
+                                 uint64_t n, float* out) {
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
+        long long v = (long long)fx[i];
+        if (v == 0) continue;
+        int e = fixed_exponent(mx[i], cnt[i]);
+        out[i] += (float)ldexp((double)v, -e);
+    }
+}
+
 // ============================================================ film layers
 __global__ void k_fill4(float4* p, uint32_t n) {
     for (uint32_t i = gtid(); i < n; i += gstride()) p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+__global__ void k_fill_u32(uint32_t* p, uint64_t n) {
+    for (uint64_t i = gtid(); i < n; i += gstride()) p[i] = 0u;
 }
 __global__ void k_fill_u64(unsigned long long* p, uint32_t n) {
     uint32_t i = threadIdx.x;
@@ -806,8 +1142,41 @@ hipError_t launch_shade(int type, const ff_paths& P, int next_q, uint32_t n_max,
     }
     return hipGetLastError();
 }
+hipError_t launch_logic_adj(const ff_paths& P, int q, uint32_t n_max, hipStream_t s) {
+    hipLaunchKernelGGL(k_logic_adj, dim3(grid_for(n_max, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, P, q);
+    return hipGetLastError();
+}
+hipError_t launch_shade_nee(int type, const ff_paths& P, uint32_t n_max, hipStream_t s) {
+    dim3 g(grid_for(n_max, FF_GRID_CAP)), b(FF_BLOCK);
+    switch (type) {
+        case MAT_MIX: hipLaunchKernelGGL(k_shade_nee<MAT_MIX>, g, b, 0, s, P); break;
+        case MAT_DIFFUSE: hipLaunchKernelGGL(k_shade_nee<MAT_DIFFUSE>, g, b, 0, s, P); break;
+        case MAT_CONDUCTOR: hipLaunchKernelGGL(k_shade_nee<MAT_CONDUCTOR>, g, b, 0, s, P); break;
+        default: return hipSuccess;  // a dielectric vertex has no NEE rays
+    }
+    return hipGetLastError();
+}
+hipError_t launch_shade_adj(int type, const ff_paths& P, int next_q, uint32_t n_max, hipStream_t s) {
+    dim3 g(grid_for(n_max, FF_GRID_CAP)), b(FF_BLOCK);
+    switch (type) {
+        case MAT_MIX: hipLaunchKernelGGL(k_shade_adj<MAT_MIX>, g, b, 0, s, P, next_q); break;
+        case MAT_DIFFUSE: hipLaunchKernelGGL(k_shade_adj<MAT_DIFFUSE>, g, b, 0, s, P, next_q); break;
+        case MAT_CONDUCTOR: hipLaunchKernelGGL(k_shade_adj<MAT_CONDUCTOR>, g, b, 0, s, P, next_q); break;
+        default: hipLaunchKernelGGL(k_shade_adj<MAT_DIELECTRIC>, g, b, 0, s, P, next_q); break;
+    }
+    return hipGetLastError();
+}
+hipError_t launch_fixed_to_float(const unsigned long long* fx, const uint32_t* mx, const uint32_t* cnt, uint64_t n,
+                                 float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_fixed_to_float, dim3(grid_for(n, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, fx, mx, cnt, n, out);
+    return hipGetLastError();
+}
 hipError_t launch_fill4(float4* p, uint32_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_fill4, dim3(grid_for(n, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, p, n);
+    return hipGetLastError();
+}
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, p, n);
     return hipGetLastError();
 }
 hipError_t launch_fill_u64(unsigned long long* p, uint32_t n, hipStream_t s) {

@@ -25,6 +25,7 @@
 #define ST_NEED_L (1u << 10)
 #define ST_NEED_S (1u << 11)
 #define ST_NEED_D (1u << 12)
+#define ST_DEAD (1u << 13)   // adjoint replay: the continuation had zero f or pdf
 
 // device counters
 #define C_EXT0 0
@@ -85,6 +86,23 @@ struct ff_dev_env {
     int32_t w, h;
 };
 
+// Differentiable rendering.  A gradient target is a caller's float buffer
+// (environment texels [h][w][3] or a texture [h][w][c]) plus its offset in
+// the call's arena of per-element bound / fixed-point accumulators.
+struct ff_dev_gtarget {
+    float* buf;
+    uint64_t off;
+};
+#define GT_ENV 0      // target 0: environment; target 1 + t: scene texture t
+// gradient accumulation modes of an adjoint pass
+#define GM_FLOAT 0    // FP32 atomics into the caller's buffers
+#define GM_BOUND 1    // deterministic pass 1: per-element max |term| and term count
+#define GM_FIXED 2    // deterministic pass 2: int64 fixed-point sums at per-element scales
+
+// request modes
+#define RQ_STORE_PATHS 1u  // forward: keep each path's radiance (path_L)
+#define RQ_ADJOINT 2u      // path-replay adjoint of a stored forward
+
 // One render request: what a path slot needs to know about its frame.
 // Slots [slot_base, slot_base + W*H*K) belong to the request; slot - slot_base
 // = layer * W*H + pixel.  K == 1 accumulates straight into the film (one
@@ -96,6 +114,11 @@ struct ff_dev_request {
     float4* film;      // [H][W] (sums, count)
     float4* layers;    // [K][H*W] when K > 1
     uint32_t slot_base, W, H, spp, depth, K, seed, sample_offset;
+    uint32_t mode;     // RQ_*
+    float inv_spp;     // adjoint: 1 / spp (the image is the mean of this call's samples)
+    float4* path_L;    // [spp][H][W] radiance of every path (RQ_STORE_PATHS / RQ_ADJOINT)
+    const float4* d_img;  // adjoint: d loss / d image [H][W] (xyz)
+    const ff_dev_gtarget* gt;  // adjoint: [1 + n_tex] targets (buf null = not wanted)
 };
 
 // device-resident batch state (graph-replay safe): [0] batch index
@@ -124,4 +147,10 @@ struct ff_paths {
     const ff_dev_request* reqs;
     uint32_t n_req;
     uint32_t n_slots;    // total slots over all requests
+    // adjoint replay
+    float4* dlt;         // [P]     d loss / d path radiance (xyz)
+    uint32_t* g_max;     // arena: max |term| bits per gradient element (GM_BOUND)
+    uint32_t* g_cnt;     // arena: number of terms per element (GM_BOUND)
+    unsigned long long* g_fx;  // arena: fixed-point sums (GM_FIXED)
+    int32_t gmode;       // GM_*
 };

@@ -143,6 +143,12 @@ ff_status ff_envmap_create(ff_context *ctx, const float *rgb, int32_t w, int32_t
 ff_status ff_envmap_read(const ff_envmap *env, double *marginal_y, double *marginal_p,
                          double *conds, float *pdf, double *denom);
 ff_status ff_envmap_destroy(ff_envmap *env);
+/* Parameter updates between renders (inverse rendering loops); data may be
+ * host or device memory, copied on `stream`.  ff_envmap_update rebuilds the
+ * CDFs on the device (texels must be finite, non-negative and not all zero).
+ * No reference counterpart: the reference loads its scene once. */
+ff_status ff_envmap_update(ff_envmap *env, const float *rgb, void *stream);
+ff_status ff_scene_set_texture(ff_scene *scene, int32_t texture, const float *data, void *stream);
 
 /* ---- standalone wavefront stages on caller device buffers ----
  * rays: device [n][8] floats (o.x o.y o.z - d.x d.y d.z -).
@@ -179,6 +185,7 @@ ff_status ff_render(ff_context *ctx, ff_scene *scene, ff_envmap *env, const ff_c
  * ff_render).  flags: FF_RENDER_USE_GRAPH captures one batch of launches in a
  * hipGraph and replays it once per sample index.  No reference counterpart:
  * the reference renders one frame per draw (CRT/PathTracer.cpp:59-79). */
+typedef struct ff_grads ff_grads;
 typedef struct {
     ff_scene *scene;
     ff_envmap *env;
@@ -187,9 +194,44 @@ typedef struct {
     uint32_t seed;
     int32_t sample_offset;
     float *film;
+    float *path_radiance;     /* optional: [spp][h][w][4] radiance of every path  */
+    const float *d_image;     /* ff_adjoint_batch: d loss / d image, [h][w][4]    */
+    const ff_grads *grads;    /* ff_adjoint_batch: gradient buffers               */
 } ff_batch_item;
 ff_status ff_render_batch(ff_context *ctx, const ff_batch_item *items, int32_t n, int32_t flags,
                           void *stream);
+
+/* ---- differentiable rendering: path-replay adjoint ----
+ * No reference counterpart (the reference is not differentiable).  The image
+ * of a request is the mean of its spp samples (film.xyz / spp).
+ * ff_render_paths is ff_render that also stores every path's radiance,
+ * path_radiance device [spp][h][w][4] (xyz used).  ff_adjoint replays those
+ * paths with the same random numbers (same request: scene, environment,
+ * camera, size, spp, depth, seed, sample offset; spp_batch may differ) and
+ * ADDS d loss / d parameter to the gradient buffers, given d_image = d loss /
+ * d image, device [h][w][4] (xyz used).  The gradient is that of the
+ * detached-sampling estimator: directions, pdfs, MIS weights and Russian
+ * roulette stay those of the forward; BSDF values (albedo and roughness
+ * textures) and environment radiance (texels) are differentiated.  Terms are
+ * accumulated with FP32 atomics; with FF_ADJOINT_DETERMINISTIC in the flags
+ * the replay runs twice (per-element bounds, then int64 fixed-point sums at
+ * per-element scales) and the result is bit-reproducible.
+ * ff_adjoint_batch does the same for batch items (their path_radiance,
+ * d_image and grads), sharing gradient buffers across items when they point
+ * to the same memory. */
+struct ff_grads {
+    float *env;               /* device [env_h][env_w][3], or NULL             */
+    int32_t n_textures;       /* entries in textures (<= the scene's textures)  */
+    float *const *textures;   /* host array of device [h][w][c] buffers (NULL entries skipped) */
+};
+#define FF_ADJOINT_DETERMINISTIC 4
+ff_status ff_render_paths(ff_context *ctx, ff_scene *scene, ff_envmap *env, const ff_camera *cam,
+                          const ff_render_req *req, float *film, float *path_radiance, void *stream);
+ff_status ff_adjoint(ff_context *ctx, ff_scene *scene, ff_envmap *env, const ff_camera *cam,
+                     const ff_render_req *req, const float *path_radiance, const float *d_image,
+                     const ff_grads *grads, void *stream);
+ff_status ff_adjoint_batch(ff_context *ctx, const ff_batch_item *items, int32_t n, int32_t flags,
+                           void *stream);
 
 /* ---- traversal kernel variants (workgroup size, LDS stack entries, and the
  * number of idle lanes at which a wave of the persistent traversal fetches
