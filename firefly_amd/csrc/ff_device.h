@@ -382,25 +382,43 @@ __device__ __forceinline__ int fixed_exponent(uint32_t max_bits, uint32_t count)
 }
 
 // ------------------------------------------------------------ traversal --
+// Loads through the global address space (global_load instead of flat_load:
+// the scene pointers come from memory, so the compiler cannot infer it).
+template <class T>
+__device__ __forceinline__ T ldg(const T* p) {
+    return *(const __attribute__((address_space(1))) T*)p;
+}
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef int v4i __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ldg4(const float4* p) {
+    v4f v = *(const __attribute__((address_space(1))) v4f*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ int4 ldg4i(const int4* p) {
+    v4i v = *(const __attribute__((address_space(1))) v4i*)p;
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+
 // Bounds3f::hit(ray, invDir, dirIsNeg) (CRT/Bounds3f.h:128-160) on one box:
 // the line test without [0, tmax] clipping; returns the final tmin/tmax.
+// The reference's comparisons are evaluated branch-free (all six plane
+// distances first): same values, same outcome, and no load or branch is
+// left waiting on the x/y test.
 __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, float hy, float hz, f3 o,
                                      f3 inv, bool nx, bool ny, bool nz, float& tn, float& tf) {
     float tmin = ((nx ? hx : lx) - o.x) * inv.x;
     float tmax = ((nx ? lx : hx) - o.x) * inv.x;
     float tymin = ((ny ? hy : ly) - o.y) * inv.y;
     float tymax = ((ny ? ly : hy) - o.y) * inv.y;
-    if ((tmin > tymax) || (tymin > tmax)) return false;
-    if (tymin > tmin) tmin = tymin;
-    if (tymax < tmax) tmax = tymax;
     float tzmin = ((nz ? hz : lz) - o.z) * inv.z;
     float tzmax = ((nz ? lz : hz) - o.z) * inv.z;
-    if ((tmin > tzmax) || (tzmin > tmax)) return false;
-    if (tzmin > tmin) tmin = tzmin;
-    if (tzmax < tmax) tmax = tzmax;
-    tn = tmin;
-    tf = tmax;
-    return true;
+    bool ok_xy = !((tmin > tymax) || (tymin > tmax));
+    float t1 = (tymin > tmin) ? tymin : tmin;
+    float t2 = (tymax < tmax) ? tymax : tmax;
+    bool ok_z = !((t1 > tzmax) || (tzmin > t2));
+    tn = (tzmin > t1) ? tzmin : t1;
+    tf = (tzmax < t2) ? tzmax : t2;
+    return ok_xy & ok_z;
 }
 
 // Triangle record in BVH-slot order: v0, e1 = v1 - v0, e2 = v2 - v0 (the
@@ -409,9 +427,9 @@ struct Tri { f3 v0, e1, e2; };
 __device__ __forceinline__ Tri load_tri(const float* __restrict__ t, int slot) {
     const float* p = t + (size_t)slot * 9;
     Tri r;
-    r.v0 = V3(p[0], p[1], p[2]);
-    r.e1 = V3(p[3], p[4], p[5]);
-    r.e2 = V3(p[6], p[7], p[8]);
+    r.v0 = V3(ldg(p + 0), ldg(p + 1), ldg(p + 2));
+    r.e1 = V3(ldg(p + 3), ldg(p + 4), ldg(p + 5));
+    r.e2 = V3(ldg(p + 6), ldg(p + 7), ldg(p + 8));
     return r;
 }
 // dTriangle::intersect, TEST_CULL branch (CRT/dTriangle.cu:34-62)
@@ -470,9 +488,14 @@ __device__ __forceinline__ bool mt_shadow(const Tri& T, f3 o, f3 d, float& t) {
 // Traversal stack: the first `lds_entries` entries of each lane live in LDS
 // ([depth][lane], conflict-free), deeper entries in a per-lane global
 // overflow area ([depth - lds_entries][grid thread]).
+typedef __attribute__((address_space(3))) int lds_int;
+typedef __attribute__((address_space(1))) int glb_int;
+// Traversal stack: LDS entries first (ds_read / ds_write), then the lane's
+// column of the global overflow area.  The pointers carry their address
+// spaces so no access goes through a generic (flat) pointer.
 struct Stack {
-    int* lds;          // &smem[threadIdx.x]
-    int* ovf;          // &overflow[grid thread]
+    lds_int* lds;      // &smem[threadIdx.x]
+    glb_int* ovf;      // &overflow[grid thread]
     uint32_t lds_stride, ovf_stride;
     int lds_entries;
     int sp;
@@ -483,7 +506,10 @@ struct Stack {
     }
     __device__ __forceinline__ int pop() {
         sp--;
-        return sp < lds_entries ? lds[sp * lds_stride] : ovf[(size_t)(sp - lds_entries) * ovf_stride];
+        int v;
+        if (sp < lds_entries) v = lds[sp * lds_stride];
+        else v = ovf[(size_t)(sp - lds_entries) * ovf_stride];
+        return v;
     }
 };
 
@@ -506,7 +532,7 @@ __device__ __forceinline__ void leaf_range(const Bvh& B, int ref, int& off, int&
     int e = ~ref;
     off = e >> 4;
     cnt = e & 15;
-    if (cnt == 15) cnt = B.leaf_count[off];
+    if (cnt == 15) cnt = ldg(B.leaf_count + off);
 }
 
 // Per-lane traversal state: the ray, its reciprocal direction and octant,
@@ -514,6 +540,7 @@ __device__ __forceinline__ void leaf_range(const Bvh& B, int ref, int& off, int&
 struct TravRay {
     f3 o, d, inv;
     bool nx, ny, nz;
+    uint32_t negm;  // nx | ny << 1 | nz << 2
     int ref;
 };
 
@@ -527,13 +554,13 @@ __device__ __forceinline__ bool trav_begin(const Bvh& B, f3 o, f3 d, TravRay& R,
     R.nx = R.inv.x < 0;
     R.ny = R.inv.y < 0;
     R.nz = R.inv.z < 0;
+    R.negm = (R.nx ? 1u : 0u) | (R.ny ? 2u : 0u) | (R.nz ? 4u : 0u);
     float tn, tf;
     cnt_nodes++;
-    if (!slab(B.root[0], B.root[1], B.root[2], B.root[3], B.root[4], B.root[5], o, R.inv, R.nx, R.ny, R.nz, tn,
-              tf) ||
-        tf < -PRUNE_BEHIND)
+    float4 r0 = ldg4((const float4*)B.root), r1 = ldg4((const float4*)B.root + 1);
+    if (!slab(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, o, R.inv, R.nx, R.ny, R.nz, tn, tf) || tf < -PRUNE_BEHIND)
         return false;
-    R.ref = __float_as_int(B.root[6]);
+    R.ref = __float_as_int(r1.z);
     st.sp = 0;
     return true;
 }
@@ -545,8 +572,8 @@ template <bool PRUNE>
 __device__ __forceinline__ bool visit_interior(const Bvh& B, TravRay& R, float t_best, Stack& st,
                                                uint32_t& cnt_nodes) {
     const float4* n = B.nodes + 4 * (size_t)R.ref;
-    float4 a = n[0], b = n[1], c = n[2];
-    int4 m = *(const int4*)(n + 3);
+    float4 a = ldg4(n), b = ldg4(n + 1), c = ldg4(n + 2);
+    int4 m = ldg4i((const int4*)(n + 3));
     cnt_nodes += 2;
     float tn0, tf0, tn1, tf1;
     bool h0, h1;
@@ -560,7 +587,7 @@ __device__ __forceinline__ bool visit_interior(const Bvh& B, TravRay& R, float t
         h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, R.o, R.inv, R.nx, R.ny, R.nz, tn0, tf0) && !(tf0 < -PRUNE_BEHIND);
         h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, R.o, R.inv, R.nx, R.ny, R.nz, tn1, tf1) && !(tf1 < -PRUNE_BEHIND);
     }
-    bool neg = m.z == 0 ? R.nx : (m.z == 1 ? R.ny : R.nz);
+    bool neg = ((R.negm >> (uint32_t)m.z) & 1u) != 0;  // dirIsNeg[axis]
     int nref = neg ? m.y : m.x, fref = neg ? m.x : m.y;
     bool hn = neg ? h1 : h0, hf = neg ? h0 : h1;
     if (hn) {

@@ -166,10 +166,10 @@ template <int BS, int LDS, int REFILL, bool ANY, class Src>
 __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint32_t* fetch, int32_t* ovf,
                                                 unsigned long long* cnt, int* smem) {
     Stack st;
-    st.lds = &smem[threadIdx.x];
+    st.lds = (lds_int*)&smem[threadIdx.x];
     st.lds_stride = BS;
     st.lds_entries = LDS;
-    st.ovf = ovf + gtid();
+    st.ovf = (glb_int*)(ovf + gtid());
     st.ovf_stride = FF_TRAV_THREADS;
     uint32_t cn = 0, ct = 0, id = 0;
     bool busy = false, more = true, two = false;
