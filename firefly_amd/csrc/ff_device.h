@@ -124,9 +124,11 @@ __device__ __forceinline__ f3 sph_dir(float u, float v) {
     float x = f_cos(theta) * f_sin(phi), y = f_cos(phi), z = f_sin(theta) * f_sin(phi);
     return V3(-x, -y, -z);
 }
+// The asin argument is clamped to [-1, 1]: a normalised float direction can
+// have |y| = 1 + 1 ulp, where the reference's asin gives NaN radiance.
 __device__ __forceinline__ void sph_map(f3 d, float& u, float& v) {
     u = f_atan2(d.z, d.x) * F_1_2PI + 0.5f;
-    v = f_asin(d.y) * F_1_PI + 0.5f;
+    v = f_asin(fminf(fmaxf(d.y, -1.f), 1.f)) * F_1_PI + 0.5f;
 }
 // upper_bound with the trailing list[left <= val] adjustment (CRT/Helpers.cu:16-31)
 __device__ __forceinline__ int upper_bound_ref(const double* __restrict__ list, int size, float val) {

@@ -713,8 +713,18 @@ static inline v3 sph_dir(float u, float v) {
 }
 /* sample_spherical_map (CRT/dMath.cuh:1483-1489) */
 static inline void sph_map(v3 d, float *u, float *v) {
+    /* asin argument clamped to [-1, 1]: |y| can exceed 1 by an ulp after
+     * normalisation, where the reference's asin gives NaN radiance */
     *u = f_atan2(d.z, d.x) * F_1_2PI + 0.5f;
-    *v = f_asin(d.y) * F_1_PI + 0.5f;
+    *v = f_asin(fminf(fmaxf(d.y, -1.f), 1.f)) * F_1_PI + 0.5f;
+}
+
+/* dEnvironmentLight::L (CRT/dEnvironmentLight.cu:66-76): bilinear texel at
+ * the direction's spherical-map coordinates */
+void or_env_L(const float *rgb, int32_t w, int32_t h, const float *dir, float *out) {
+    float u, v;
+    sph_map(V3(dir[0], dir[1], dir[2]), &u, &v);
+    or_bilinear(rgb, w, h, 3, u, v, out);
 }
 
 /* inverse-CDF texel pick of dEnvironmentLight::get_direction
@@ -1329,31 +1339,28 @@ static void lookup_d(const float *tex, int32_t w, int32_t h, int32_t c, float u,
 static void env_uv(const float *dir, float *u, float *v) { sph_map(V3(dir[0], dir[1], dir[2]), u, v); }
 
 typedef struct { double f[3], da[3], dr[3]; } lobe_d;
-static double dot_d(const float *a, const double *b) { return (double)a[0] * b[0] + (double)a[1] * b[1] + (double)a[2] * b[2]; }
-static void half_d(const float *wo, const float *wi, double *h) {
-    double x = (double)wo[0] + wi[0], y = (double)wo[1] + wi[1], z = (double)wo[2] + wi[2];
-    double l = sqrt(x * x + y * y + z * z);
-    h[0] = x / l; h[1] = y / l; h[2] = z / l;
-}
-static double ddot(const float *a, const float *b) { return (double)a[0] * b[0] + (double)a[1] * b[1] + (double)a[2] * b[2]; }
 
-/* spec_f (CRT/dMaterial.cu:266-288) in FP64 with d/dalbedo (per channel) and d/droughness */
-static void spec_lobe_d(const or_event *e, const float *wi, double r, const double *alb, lobe_d *o) {
-    double h[3];
-    half_d(e->wo, wi, h);
-    double ndwi = fmax(ddot(e->n, wi), 0.0), ndwo = fmax(ddot(e->n, e->wo), 0.0);
-    double nh = fmax(0.0, dot_d(e->n, h));
-    double a2 = r * r * r * r, nh2 = nh * nh;
+/* spec_f (CRT/dMaterial.cu:266-288) with d/dalbedo (per channel) and
+ * d/droughness.  The geometric factors (half vector, cosines, the Fresnel
+ * power, the clamped denominator) are detached and taken exactly as the
+ * float forward computes them; the parameter-dependent parts are FP64. */
+static void spec_lobe_d(const or_event *e, const float *wi_, double r, const double *alb, lobe_d *o) {
+    v3 n = vld(e->n), wo = vld(e->wo), wi = vld(wi_);
+    v3 wh = vnorm(vadd(wo, wi));
+    double ndwi = (double)fmaxf(vdot(n, wi), 0.f), ndwo = (double)fmaxf(vdot(n, wo), 0.f);
+    float nh = fmaxf(0.0f, vdot(n, wh));
+    double nh2 = (double)(nh * nh);
+    double a2 = r * r * r * r;
     double q = nh2 * (a2 - 1.0) + 1.0;
-    double D = a2 / (D_PI * q * q);
-    double dD = 4.0 * r * r * r * (q - 2.0 * a2 * nh2) / (D_PI * q * q * q);
+    double D = a2 / ((double)F_PI * (q * q));
+    double dD = 4.0 * r * r * r * (q - 2.0 * a2 * nh2) / ((double)F_PI * q * q * q);
     double k = (r + 1.0) * (r + 1.0) / 8.0, dk = (r + 1.0) / 4.0;
     double d1 = ndwi * (1.0 - k) + k, d2 = ndwo * (1.0 - k) + k;
     double g1 = ndwi / d1, g2 = ndwo / d2;
     double dg1 = -ndwi * (1.0 - ndwi) / (d1 * d1), dg2 = -ndwo * (1.0 - ndwo) / (d2 * d2);
     double G = g1 * g2, dG = (dg1 * g2 + g1 * dg2) * dk;
-    double p = pow(1.0 - fmax(0.0, dot_d(wi, h)), 5.0);
-    double den = fmax(4.0 * ndwo * ndwi, 0.001);
+    double p = (double)pow5(1.f - fmaxf(0.f, vdot(wh, wi)));
+    double den = (double)fmaxf((float)(4.0 * ndwo * ndwi), 0.001f);
     for (int c = 0; c < 3; c++) {
         double f0 = (double)e->fres[c] + (alb[c] - (double)e->fres[c]) * e->metal;
         double F = f0 + (1.0 - f0) * p;
@@ -1362,11 +1369,11 @@ static void spec_lobe_d(const or_event *e, const float *wi, double r, const doub
         o->dr[c] = F * (dD * G + D * dG) / den;
     }
 }
-/* diff_f (CRT/dMaterial.cu:164-178) in FP64 with d/dalbedo */
-static void diff_lobe_d(const or_event *e, const float *wi, const double *alb, lobe_d *o) {
-    double h[3];
-    half_d(e->wo, wi, h);
-    double p = pow(1.0 - fmax(0.0, dot_d(wi, h)), 5.0);
+/* diff_f (CRT/dMaterial.cu:164-178) with d/dalbedo; geometry as the forward */
+static void diff_lobe_d(const or_event *e, const float *wi_, const double *alb, lobe_d *o) {
+    v3 wo = vld(e->wo), wi = vld(wi_);
+    v3 wh = vnorm(vadd(wo, wi));
+    double p = (double)pow5(1.f - fmaxf(0.f, vdot(wh, wi)));
     double m = e->metal;
     for (int c = 0; c < 3; c++) {
         double f0 = (double)e->fres[c] + (alb[c] - (double)e->fres[c]) * m;

@@ -111,6 +111,7 @@ void or_env_sample_dir(const double *marginal_y, const double *conds, int32_t w,
 void or_bilinear(const float *tex, int32_t w, int32_t h, int32_t c, float u,
                  float v, float *out);
 float or_env_pdf(const float *pdf, int32_t w, int32_t h, const float *wi);
+void or_env_L(const float *rgb, int32_t w, int32_t h, const float *dir, float *out);
 
 /* ---- camera ---- */
 void or_gen_ray(const or_scene *s, int32_t width, int32_t height, int32_t px,

@@ -66,6 +66,7 @@ def lib() -> C.CDLL:
             "or_env_sample_dir": (None, [_P, _P, C.c_int32, C.c_int32, C.c_float, C.c_float, _P, _P, _P]),
             "or_bilinear": (None, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float, _P]),
             "or_env_pdf": (C.c_float, [_P, C.c_int32, C.c_int32, _P]),
+            "or_env_L": (None, [_P, C.c_int32, C.c_int32, _P, _P]),
             "or_gen_ray": (None, [C.POINTER(OrScene), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                   C.c_uint32, _P]),
             "or_ggx_ndf": (C.c_double, [_P, _P, C.c_float]),

@@ -127,3 +127,20 @@ def test_concurrent_scene_creation_deep_trees(gpu):
     for sc in scs:
         sc.close()
     e.close()
+
+
+def test_uneven_request_renders_finite(gpu):
+    """Batched request 19 (727^2, 2 spp, depth 4) once rendered a NaN pixel:
+    a GGX sample normalised to |y| = 1 + 1 ulp reached asin in the spherical
+    map.  The lookup clamps the argument now; the film is finite."""
+    import torch
+    spec = scenes.batch_request_specs(64, seed=0)[19]
+    sd = scenes.batch_request_scene(spec, scenes.sun_env(2048, 1024, 3))
+    sc = abi.Scene(gpu, sd.positions, sd.normals, sd.uvs, sd.material_ids, sd.materials)
+    env = abi.EnvMap(gpu, sd.env)
+    f = torch.zeros((sd.height, sd.width, 4), dtype=torch.float32, device="cuda")
+    abi.render(gpu, sc, env, abi.make_camera(sd.inv_vp), sd.width, sd.height, sd.spp, sd.max_depth, sd.seed, f)
+    torch.cuda.synchronize()
+    assert bool(torch.isfinite(f).all())
+    sc.close()
+    env.close()

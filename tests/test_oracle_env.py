@@ -106,3 +106,32 @@ def test_env_pdf_lookup(oracle):
         v = np.arcsin(wi[1]) / np.pi + 0.5
         ref = pdf[int(np.float32(v) * (h - 1)), int(np.float32(u) * (w - 1))] / np.sin(np.arccos(wi[2]))
         np.testing.assert_allclose(got, ref, rtol=1e-5)
+
+
+def test_env_lookup_direction_beyond_unit_y(oracle):
+    """A normalised float direction can have |y| one ulp above 1 (seen on a
+    GGX sample in a batched-request scene); the spherical map clamps the asin
+    argument, so the lookup reads the pole row instead of producing NaN."""
+    w, h = 32, 16
+    rgb = (np.arange(w * h * 3, dtype=np.float32).reshape(h, w, 3) + 1) / (w * h * 3)
+    L = oracle.lib()
+    for y in (np.nextafter(np.float32(-1.0), np.float32(-2.0)), np.nextafter(np.float32(1.0), np.float32(2.0))):
+        wi = np.array([-1.0848e-05, y, 2.73287e-05], np.float32)
+        got = np.zeros(3, np.float32)
+        L.or_env_L(rgb.ctypes.data, w, h, wi.ctypes.data, got.ctypes.data)
+        ref = np.zeros(3, np.float32)
+        wc = wi.copy()
+        wc[1] = np.float32(np.sign(y))
+        L.or_env_L(rgb.ctypes.data, w, h, wc.ctypes.data, ref.ctypes.data)
+        assert np.isfinite(got).all()
+        np.testing.assert_array_equal(got, ref)
+
+
+def test_batched_request_scene_renders_finite(oracle):
+    """The request whose GGX sample produced |y| > 1 renders without NaN."""
+    from oracle import oracle as O
+    spec = scenes.batch_request_specs(64, seed=0)[19]
+    spec = dict(spec, width=64, height=64)
+    sd = scenes.batch_request_scene(spec, scenes.sun_env(256, 128, 3))
+    film, _ = O.PreparedScene(sd).render()
+    assert np.isfinite(film).all()
