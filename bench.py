@@ -38,13 +38,16 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def load_traffic(kernel: str, workload: str):
-    """HBM bytes per launch from a committed rocprof PMC summary, if present."""
+def load_traffic(kernel: str, workload: str, rays_per_launch: float):
+    """HBM bytes per launch of `kernel`: the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, copy-calibrated) gives bytes per
+    ray traced; scaled by this run's rays per launch.  None if absent."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("traffic_bytes_per_launch")
+        per_ray = json.load(f).get("traffic_bytes_per_ray")
+    return None if per_ray is None else round(per_ray * rays_per_launch)
 
 
 def cpu_baseline(sd, nodes, ordered, budget_s: float, spp: int):
@@ -485,7 +488,8 @@ def run_frame(args) -> dict:
                          batches=(1, 2, 4))
         args.spp_batch = tuned["best"]["spp_batch"]
         for c in tuned["candidates"]:
-            log(f"  variant {c['variant']} (block {c['block']}, lds {c['lds_entries']}, refill {c['refill']}) "
+            log(f"  variant {c['variant']} (block {c['block']}, lds {c['lds_entries']}, refill {c['refill']}, "
+                f"ww {c['while_while']}) "
                 f"spp_batch {c['spp_batch']}: {c['seconds'] * 1e3:.1f} ms")
         log(f"autotune ({time.perf_counter() - t0:.1f} s): {tuned['best']}")
 
@@ -529,7 +533,8 @@ def run_frame(args) -> dict:
     b, ms, nl = kern[dom]
     achieved = b / (ms / 1e3) / 1e9
     workload = f"icosphere{args.subdiv}_{sd.width}x{sd.height}_{args.spp}spp_d{args.depth}"
-    traffic = load_traffic(dom, workload)
+    rays_launch = ((tot["rays_primary"] + tot["rays_extension"]) if dom == "k_extend" else tot["rays_shadow"]) / nl
+    traffic = load_traffic(dom, workload, rays_launch)
     roofline = {
         "bound": "hbm",
         "kernel": dom,

@@ -86,7 +86,9 @@ struct ff_context {
     bool have_stats = false;
     uint32_t* bstate = nullptr;
     ff_dev_request* d_reqs = nullptr;
+    uint32_t* d_qbase = nullptr;  // [req_cap + 1]
     size_t req_cap = 0;
+    uint32_t* fetch = nullptr;    // [FETCH_WORDS] segmented traversal work counters
     int trav_variant = 0;
     // adjoint: gradient target tables and the deterministic-reduction arena
     ff_dev_gtarget* d_gt = nullptr;
@@ -241,10 +243,12 @@ ff_status ff_context_create(int32_t device, ff_context** out) {
     HIP_TRY(hipSetDevice(device));
     ff_context* c = new ff_context();
     c->device = device;
-    if (dalloc(&c->counters, C_COUNT) != hipSuccess || dalloc(&c->stats, S_COUNT) != hipSuccess ||
+    if (dalloc(&c->counters, C_COUNT) != hipSuccess || dalloc(&c->stats, S_WORDS) != hipSuccess ||
+        dalloc(&c->fetch, FETCH_WORDS) != hipSuccess || hipMemset(c->fetch, 0, 4 * FETCH_WORDS) != hipSuccess ||
         dalloc(&c->bstate, B_COUNT) != hipSuccess) {
         dfree(c->counters);
         dfree(c->stats);
+        dfree(c->fetch);
         dfree(c->bstate);
         delete c;
         return fail(FF_ERR_ALLOC, "ff_context_create: counter allocation failed");
@@ -262,6 +266,8 @@ ff_status ff_context_destroy(ff_context* c) {
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     dfree(c->bstate);
     dfree(c->d_reqs);
+    dfree(c->d_qbase);
+    dfree(c->fetch);
     dfree(c->counters);
     dfree(c->stats);
     dfree(c->ovf);
@@ -496,7 +502,7 @@ ff_status ff_trace_closest(ff_scene* s, const float* rays, uint32_t n, float* hi
     HIP_TRY(hipSetDevice(s->ctx->device));
     ff_status st = ensure_ovf(s->ctx, s->max_depth);
     if (st != FF_OK) return st;
-    HIP_TRY(ffk::launch_trace_closest(s->view(), (const float4*)rays, n, (float4*)hits, two_sided, s->ctx->counters,
+    HIP_TRY(ffk::launch_trace_closest(s->view(), (const float4*)rays, n, (float4*)hits, two_sided, s->ctx->fetch,
                                       s->ctx->ovf,
                                       (unsigned long long*)counts, (hipStream_t)stream));
     return FF_OK;
@@ -509,7 +515,7 @@ ff_status ff_trace_any(ff_scene* s, const float* rays, uint32_t n, uint8_t* occl
     HIP_TRY(hipSetDevice(s->ctx->device));
     ff_status st = ensure_ovf(s->ctx, s->max_depth);
     if (st != FF_OK) return st;
-    HIP_TRY(ffk::launch_trace_any(s->view(), (const float4*)rays, n, occluded, s->ctx->counters, s->ctx->ovf,
+    HIP_TRY(ffk::launch_trace_any(s->view(), (const float4*)rays, n, occluded, s->ctx->fetch, s->ctx->ovf,
                                   (unsigned long long*)counts, (hipStream_t)stream));
     return FF_OK;
 }
@@ -675,7 +681,10 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
     }
     if (ctx->req_cap < table.size()) {
         dfree(ctx->d_reqs);
+        dfree(ctx->d_qbase);
+        ctx->req_cap = 0;
         HIP_TRY(dalloc(&ctx->d_reqs, table.size()));
+        HIP_TRY(dalloc(&ctx->d_qbase, table.size() + 1));
         ctx->req_cap = table.size();
     }
     HIP_TRY(hipStreamSynchronize(hs));
@@ -683,6 +692,8 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
     if (adjoint && !gt.empty())
         HIP_TRY(hipMemcpy(ctx->d_gt, gt.data(), gt.size() * sizeof(ff_dev_gtarget), hipMemcpyHostToDevice));
     p.counters = ctx->counters;
+    p.fetch = ctx->fetch;
+    p.qbase = ctx->d_qbase;
     p.stats = ctx->stats;
     p.ovf = ctx->ovf;
     p.bstate = ctx->bstate;
@@ -718,7 +729,7 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
         for (size_t r = 0; r < reqs.size(); r++)
             if (reqs[r].K > 1)
                 HIP_TRY(ffk::launch_fill4(table[r].layers, (uint32_t)((uint64_t)reqs[r].W * reqs[r].H * reqs[r].K), hs));
-    HIP_TRY(ffk::launch_fill_u64(p.stats, S_COUNT, hs));
+    HIP_TRY(ffk::launch_fill_u64(p.stats, S_WORDS, hs));
 
     // one batch of the forward: identical launches for every batch (the index is on the device)
     auto batch = [&]() -> ff_status {
@@ -728,9 +739,8 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
         for (int32_t it = 1; it <= max_depth + 1; it++) {
             HIP_TRY(timed(0, [&] { return ffk::launch_extend(p, cur, var, hs); }));
             HIP_TRY(timed(1, [&] { return ffk::launch_shadow(p, var, hs); }));
-            uint32_t mask = (1u << (cur == 0 ? C_EXT1 : C_EXT0)) | (1u << C_SHADOW) | (0xfu << C_MAT0) |
-                            (1u << C_FETCH_EXT) | (1u << C_FETCH_SHD);
-            HIP_TRY(ffk::launch_reset(p.counters, mask, hs));
+            uint32_t mask = (1u << (cur == 0 ? C_EXT1 : C_EXT0)) | (1u << C_SHADOW) | (0xfu << C_MAT0);
+            HIP_TRY(ffk::launch_reset(p.counters, mask, p.fetch, (1u << F_EXT) | (1u << F_SHD), hs));
             HIP_TRY(timed(2, [&] { return ffk::launch_logic(p, cur, p.n_slots, hs); }));
             if (it <= max_depth) {
                 for (int m = 0; m < 4; m++)
@@ -748,9 +758,8 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
         int cur = 0;
         for (int32_t it = 1; it <= max_depth + 1; it++) {
             HIP_TRY(timed(0, [&] { return ffk::launch_extend(p, cur, var, hs); }));
-            uint32_t mask = (1u << (cur == 0 ? C_EXT1 : C_EXT0)) | (1u << C_SHADOW) | (0xfu << C_MAT0) |
-                            (1u << C_FETCH_EXT) | (1u << C_FETCH_SHD);
-            HIP_TRY(ffk::launch_reset(p.counters, mask, hs));
+            uint32_t mask = (1u << (cur == 0 ? C_EXT1 : C_EXT0)) | (1u << C_SHADOW) | (0xfu << C_MAT0);
+            HIP_TRY(ffk::launch_reset(p.counters, mask, p.fetch, (1u << F_EXT) | (1u << F_SHD), hs));
             HIP_TRY(timed(2, [&] { return ffk::launch_logic_adj(p, cur, p.n_slots, hs); }));
             if (it <= max_depth) {
                 for (int m = 0; m < 3; m++)
@@ -915,12 +924,14 @@ ff_status ff_set_traversal_variant(ff_context* ctx, int32_t variant) {
     return FF_OK;
 }
 
-ff_status ff_traversal_variant_info(int32_t variant, int32_t* block, int32_t* lds_entries, int32_t* refill) {
+ff_status ff_traversal_variant_info(int32_t variant, int32_t* block, int32_t* lds_entries, int32_t* refill,
+                                    int32_t* while_while) {
     if (variant < 0 || variant >= ffk::kTravVariants)
         return fail(FF_ERR_INVALID, "ff_traversal_variant_info: variant %d of %d", variant, ffk::kTravVariants);
     if (block) *block = ffk::kTravBlock[variant];
     if (lds_entries) *lds_entries = ffk::kTravLds[variant];
     if (refill) *refill = ffk::kTravRefill[variant];
+    if (while_while) *while_while = ffk::kTravWhileWhile[variant];
     return FF_OK;
 }
 
@@ -931,8 +942,11 @@ ff_status ff_last_render_stats(ff_context* ctx, ff_ray_stats* out) {
     if (!ctx->have_stats) return fail(FF_ERR_INVALID, "ff_last_render_stats: no render on this context");
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipStreamSynchronize(ctx->last_stream));
-    unsigned long long h[S_COUNT];
-    HIP_TRY(hipMemcpy(h, ctx->stats, sizeof(h), hipMemcpyDeviceToHost));
+    std::vector<unsigned long long> rep(S_WORDS);
+    HIP_TRY(hipMemcpy(rep.data(), ctx->stats, sizeof(unsigned long long) * S_WORDS, hipMemcpyDeviceToHost));
+    unsigned long long h[S_COUNT] = {0};
+    for (int r = 0; r < S_REPLICAS; r++)
+        for (int k = 0; k < S_COUNT; k++) h[k] += rep[(size_t)r * S_STRIDE + k];
     std::memset(out, 0, sizeof(*out));
     out->rays_primary = (double)h[S_PRIMARY];
     out->rays_extension = (double)h[S_EXTENSION];

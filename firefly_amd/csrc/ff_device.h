@@ -609,21 +609,40 @@ __device__ __forceinline__ bool visit_interior(const Bvh& B, TravRay& R, float t
 // 0 <= t < tmin.  Children whose boxes miss (or lie beyond the current hit by
 // the conservative slack, or wholly behind the origin) are never visited.
 // Returns false once the traversal is complete.
+// the triangles of the leaf R.ref (closest-hit test of every one)
+__device__ __forceinline__ void leaf_closest(const Bvh& B, const float* __restrict__ tris, const TravRay& R,
+                                             Hit& h, bool two_sided, uint32_t& cnt_tris) {
+    int off, np;
+    leaf_range(B, R.ref, off, np);
+    for (int i = 0; i < np; i++) {
+        Tri T = load_tri(tris, off + i);
+        cnt_tris++;
+        float t = 0.f, u = 0.f, v = 0.f;
+        bool ok = two_sided ? mt_two(T, R.o, R.d, t, u, v) : mt_cull(T, R.o, R.d, t, u, v);
+        if (ok && !(t < 0.f) && t < h.t) { h.t = t; h.u = u; h.v = v; h.id = off + i; }
+    }
+}
+// any-hit test of the leaf R.ref's triangles
+__device__ __forceinline__ bool leaf_any(const Bvh& B, const float* __restrict__ tris, const TravRay& R,
+                                         uint32_t& cnt_tris) {
+    int off, np;
+    leaf_range(B, R.ref, off, np);
+    for (int i = 0; i < np; i++) {
+        Tri T = load_tri(tris, off + i);
+        cnt_tris++;
+        float t = 0.f;
+        if (mt_shadow(T, R.o, R.d, t) && t < K_HUGE) return true;
+    }
+    return false;
+}
+
 __device__ __forceinline__ bool closest_step(const Bvh& B, const float* __restrict__ tris, TravRay& R, Hit& h,
                                              bool two_sided, Stack& st, uint32_t& cnt_nodes,
                                              uint32_t& cnt_tris) {
     if (R.ref >= 0) {
         if (visit_interior<true>(B, R, h.t, st, cnt_nodes)) return true;
     } else {
-        int off, np;
-        leaf_range(B, R.ref, off, np);
-        for (int i = 0; i < np; i++) {
-            Tri T = load_tri(tris, off + i);
-            cnt_tris++;
-            float t = 0.f, u = 0.f, v = 0.f;
-            bool ok = two_sided ? mt_two(T, R.o, R.d, t, u, v) : mt_cull(T, R.o, R.d, t, u, v);
-            if (ok && !(t < 0.f) && t < h.t) { h.t = t; h.u = u; h.v = v; h.id = off + i; }
-        }
+        leaf_closest(B, tris, R, h, two_sided, cnt_tris);
     }
     if (st.sp == 0) return false;
     R.ref = st.pop();
@@ -653,15 +672,8 @@ __device__ __forceinline__ int any_step(const Bvh& B, const float* __restrict__ 
                                         uint32_t& cnt_nodes, uint32_t& cnt_tris) {
     if (R.ref >= 0) {
         if (visit_interior<false>(B, R, 0.f, st, cnt_nodes)) return 0;
-    } else {
-        int off, np;
-        leaf_range(B, R.ref, off, np);
-        for (int i = 0; i < np; i++) {
-            Tri T = load_tri(tris, off + i);
-            cnt_tris++;
-            float t = 0.f;
-            if (mt_shadow(T, R.o, R.d, t) && t < K_HUGE) return 1;
-        }
+    } else if (leaf_any(B, tris, R, cnt_tris)) {
+        return 1;
     }
     if (st.sp == 0) return 2;
     R.ref = st.pop();
@@ -711,5 +723,82 @@ __device__ __forceinline__ void wave_sum_add(unsigned long long* counter, uint32
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
     if (__lane_id() == 0) atomicAdd(counter, s);
 }
+
+// Workgroup-aggregated queue allocation: thread t asks for cnt[q] (0..3)
+// slots of queue q; one atomic per queue per workgroup (device-scope atomics
+// on one address serialise at the memory side, so per-wave atomics on a few
+// queue counters throttle whole kernels).  Every thread of the workgroup must
+// call it the same number of times.  lds: NQ * (BS / 64 + 1) words.
+template <int BS, int NQ>
+__device__ __forceinline__ void block_alloc(uint32_t* counters, const int* qidx, const uint32_t* cnt,
+                                            uint32_t* first, uint32_t* lds) {
+    constexpr int NW = BS / 64;
+    const uint32_t w = threadIdx.x >> 6;
+    uint32_t pre[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        uint64_t b0 = __ballot((cnt[q] & 1u) != 0), b1 = __ballot((cnt[q] & 2u) != 0);
+        uint32_t r0 = __builtin_amdgcn_mbcnt_hi((uint32_t)(b0 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b0, 0u));
+        uint32_t r1 = __builtin_amdgcn_mbcnt_hi((uint32_t)(b1 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b1, 0u));
+        pre[q] = r0 + 2u * r1;
+        if (__lane_id() == 0) lds[q * (NW + 1) + w] = (uint32_t)__popcll(b0) + 2u * (uint32_t)__popcll(b1);
+    }
+    __syncthreads();
+    if (threadIdx.x < NQ) {
+        const int q = threadIdx.x;
+        uint32_t tot = 0;
+        for (int k = 0; k < NW; k++) {
+            uint32_t c = lds[q * (NW + 1) + k];
+            lds[q * (NW + 1) + k] = tot;
+            tot += c;
+        }
+        lds[q * (NW + 1) + NW] = tot ? atomicAdd(&counters[qidx[q]], tot) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; q++) first[q] = lds[q * (NW + 1) + NW] + lds[q * (NW + 1) + w] + pre[q];
+    __syncthreads();
+}
+
+// Segmented work fetch of the persistent traversal: queue [0, n) is split
+// into FETCH_SEGS segments with a counter each; a wave draws from its home
+// segment and moves on when that is exhausted.  Exhaustion is first checked
+// with a plain load (counters only grow, so a stale value is conservative).
+// Wave-uniform: every lane calls it with the same arguments.
+struct SegFetch {
+    uint32_t* ctr;   // FETCH_SEGS counters, FETCH_STRIDE words apart
+    uint32_t n, seg_len, seg, tried;
+    __device__ __forceinline__ void init(uint32_t* c, uint32_t n_, uint32_t home) {
+        ctr = c;
+        n = n_;
+        seg_len = (n + FETCH_SEGS - 1) / FETCH_SEGS;
+        seg = home % FETCH_SEGS;
+        tried = 0;
+    }
+    // Grants up to `want` items (want <= 64): returns the first item and the
+    // number granted (0 when every segment is exhausted: done = true).
+    __device__ __forceinline__ uint32_t take(uint32_t want, uint32_t& got, bool& done) {
+        got = 0;
+        done = false;
+        while (tried < FETCH_SEGS) {
+            uint32_t lo = seg * seg_len;
+            uint32_t len = lo < n ? (n - lo < seg_len ? n - lo : seg_len) : 0u;
+            uint32_t* c = ctr + (size_t)seg * FETCH_STRIDE;
+            if (len > 0 && __builtin_nontemporal_load(c) < len) {
+                uint32_t base = 0;
+                if (__lane_id() == 0) base = atomicAdd(c, want);
+                base = __shfl(base, 0);
+                if (base < len) {
+                    got = len - base < want ? len - base : want;
+                    return lo + base;
+                }
+            }
+            seg = (seg + 1) % FETCH_SEGS;
+            tried++;
+        }
+        done = true;
+        return 0;
+    }
+};
 
 }  // namespace ffd

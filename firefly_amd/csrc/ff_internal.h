@@ -14,13 +14,14 @@ extern const int kTravVariants;
 extern const int kTravLds[];
 extern const int kTravBlock[];
 extern const int kTravRefill[];
+extern const int kTravWhileWhile[];
 // computes the grid sizes of a traversal variant (call before a graph capture)
 hipError_t prepare_traversal(int variant);
 hipError_t launch_batch_begin(const ff_paths& P, int set, uint32_t value, hipStream_t s);
 hipError_t launch_generate(const ff_paths& P, hipStream_t s);
 hipError_t launch_extend(const ff_paths& P, int q, int variant, hipStream_t s);
 hipError_t launch_shadow(const ff_paths& P, int variant, hipStream_t s);
-hipError_t launch_reset(uint32_t* counters, uint32_t mask, hipStream_t s);
+hipError_t launch_reset(uint32_t* counters, uint32_t mask, uint32_t* fetch, uint32_t fetch_mask, hipStream_t s);
 hipError_t launch_logic(const ff_paths& P, int q, uint32_t n_max, hipStream_t s);
 hipError_t launch_shade(int type, const ff_paths& P, int next_q, uint32_t n_max, hipStream_t s);
 hipError_t launch_logic_adj(const ff_paths& P, int q, uint32_t n_max, hipStream_t s);
@@ -33,10 +34,10 @@ hipError_t launch_fill_u32(uint32_t* p, uint64_t n, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, uint32_t n, hipStream_t s);
 hipError_t launch_resolve_layers(const float4* layers, uint32_t K, uint32_t WH, float4* film, hipStream_t s);
 hipError_t launch_trace_closest(const ff_dev_scene& S, const float4* rays, uint32_t n,
-                                float4* hits, int two_sided, uint32_t* counters, int32_t* ovf,
+                                float4* hits, int two_sided, uint32_t* fetch, int32_t* ovf,
                                 unsigned long long* counts, hipStream_t s);
 hipError_t launch_trace_any(const ff_dev_scene& S, const float4* rays, uint32_t n, uint8_t* occ,
-                            uint32_t* counters, int32_t* ovf, unsigned long long* counts, hipStream_t s);
+                            uint32_t* fetch, int32_t* ovf, unsigned long long* counts, hipStream_t s);
 hipError_t launch_gen_rays(const ff_dev_camera& cam, int W, int H, uint32_t seed,
                            uint32_t sample, float4* rays, hipStream_t s);
 hipError_t launch_env_build(const float* rgb, int w, int h, float* lum, double* row_sum,

@@ -38,6 +38,11 @@ __device__ __forceinline__ EnvView env_view(const ff_dev_env& e) {
     return v;
 }
 
+// this workgroup's replica of the ray statistics
+__device__ __forceinline__ unsigned long long* stats_rep(const ff_paths& P) {
+    return P.stats + (size_t)(blockIdx.x % S_REPLICAS) * S_STRIDE;
+}
+
 // request owning a slot: last r with slot_base <= id
 __device__ __forceinline__ uint32_t find_req(const ff_dev_request* __restrict__ reqs, uint32_t n, uint32_t id) {
     uint32_t lo = 0, hi = n;
@@ -50,51 +55,66 @@ __device__ __forceinline__ uint32_t find_req(const ff_dev_request* __restrict__ 
 }
 
 // ============================================================ batch state
-// Starts a batch: batch index += 1 (or = value), all queue counters cleared.
-__global__ void k_batch_begin(uint32_t* bstate, uint32_t* counters, int set, uint32_t value) {
+// Starts a batch: batch index += 1 (or = value), queue and work counters
+// cleared, and the extension queue laid out for the batch's new samples:
+// request r's active slots (its first min(K, spp - b*K) layers, a contiguous
+// slot range) occupy queue entries [qbase[r], qbase[r] + count), so
+// generate places every path without an atomic.
+__global__ void k_batch_begin(ff_paths P, int set, uint32_t value) {
     uint32_t i = threadIdx.x;
-    if (i < C_COUNT) counters[i] = 0;
-    if (i == 0) bstate[B_BATCH] = set ? value : bstate[B_BATCH] + 1u;
+    if (i < C_COUNT) P.counters[i] = 0;
+    for (uint32_t k = i; k < 3 * FETCH_SEGS; k += blockDim.x) P.fetch[(size_t)k * FETCH_STRIDE] = 0;
+    __syncthreads();
+    if (i == 0) {
+        uint32_t b = set ? value : P.bstate[B_BATCH] + 1u;
+        P.bstate[B_BATCH] = b;
+        if (set) return;  // (re)initialisation only
+        uint32_t q = 0;
+        for (uint32_t r = 0; r < P.n_req; r++) {
+            const ff_dev_request& R = P.reqs[r];
+            int64_t rem = (int64_t)R.spp - (int64_t)b * R.K;
+            uint32_t layers = rem <= 0 ? 0u : (rem < (int64_t)R.K ? (uint32_t)rem : R.K);
+            P.qbase[r] = q;
+            q += layers * R.W * R.H;
+        }
+        P.qbase[P.n_req] = q;
+        P.counters[C_EXT0] = q;
+    }
 }
 
 // ================================================================ generate
 // wf_generate + dCamera::gen_ray (CRT/kernel.cu:457-485): every active slot
-// of the batch starts its next sample and joins the extension queue.
+// of the batch starts its next sample; its extension-queue entry is
+// qbase[request] + (slot - slot_base), so the queue is in slot order.
 __global__ __launch_bounds__(FF_BLOCK) void k_generate(ff_paths P) {
     const uint32_t b = P.bstate[B_BATCH];
-    const uint32_t n_round = (P.n_slots + 63u) & ~63u;
-    for (uint32_t id = gtid(); id < n_round; id += gstride()) {
-        bool act = false;
-        if (id < P.n_slots) {
-            uint32_t r = find_req(P.reqs, P.n_req, id);
-            const ff_dev_request& R = P.reqs[r];
-            uint32_t WH = R.W * R.H, local = id - R.slot_base;
-            uint32_t layer = local / WH, pix = local % WH;
-            int64_t rem = (int64_t)R.spp - (int64_t)b * R.K;
-            act = rem > 0 && (int64_t)layer < rem && layer < R.K;
-            if (act) {
-                uint32_t sample = R.sample_offset + b * R.K + layer;
-                uint32_t base = rng_base(R.seed, pix, sample);
-                f3 o, d;
-                gen_ray(R.C, (int)R.W, (int)R.H, (int)(pix % R.W), (int)(pix / R.W), base, o, d);
-                P.ray[2 * id] = f4(o, 0.f);
-                P.ray[2 * id + 1] = f4(d, 0.f);
-                P.rngb[id] = base;
-                P.preq[id] = r;
-                P.beta[id] = make_float4(1.f, 1.f, 1.f, __uint_as_float(ST_PREV_DELTA));
-                if (R.mode & RQ_ADJOINT) {
-                    // replay: L_rem starts as the stored radiance of this path
-                    uint32_t sl = b * R.K + layer;
-                    P.Lp[id] = R.path_L[(size_t)sl * WH + pix];
-                    float4 g = R.d_img[pix];
-                    P.dlt[id] = make_float4(g.x * R.inv_spp, g.y * R.inv_spp, g.z * R.inv_spp, 0.f);
-                } else {
-                    P.Lp[id] = make_float4(0.f, 0.f, 0.f, 0.f);
-                }
-            }
+    for (uint32_t id = gtid(); id < P.n_slots; id += gstride()) {
+        uint32_t r = find_req(P.reqs, P.n_req, id);
+        const ff_dev_request& R = P.reqs[r];
+        uint32_t WH = R.W * R.H, local = id - R.slot_base;
+        uint32_t layer = local / WH, pix = local % WH;
+        int64_t rem = (int64_t)R.spp - (int64_t)b * R.K;
+        bool act = rem > 0 && (int64_t)layer < rem && layer < R.K;
+        if (!act) continue;
+        uint32_t sample = R.sample_offset + b * R.K + layer;
+        uint32_t base = rng_base(R.seed, pix, sample);
+        f3 o, d;
+        gen_ray(R.C, (int)R.W, (int)R.H, (int)(pix % R.W), (int)(pix / R.W), base, o, d);
+        P.ray[2 * id] = f4(o, 0.f);
+        P.ray[2 * id + 1] = f4(d, 0.f);
+        P.rngb[id] = base;
+        P.preq[id] = r;
+        P.beta[id] = make_float4(1.f, 1.f, 1.f, __uint_as_float(ST_PREV_DELTA));
+        if (R.mode & RQ_ADJOINT) {
+            // replay: L_rem starts as the stored radiance of this path
+            uint32_t sl = b * R.K + layer;
+            P.Lp[id] = R.path_L[(size_t)sl * WH + pix];
+            float4 g = R.d_img[pix];
+            P.dlt[id] = make_float4(g.x * R.inv_spp, g.y * R.inv_spp, g.z * R.inv_spp, 0.f);
+        } else {
+            P.Lp[id] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
-        uint32_t slot = wave_append(&P.counters[C_EXT0], act);
-        if (act) P.q_ext[0][slot] = id;
+        P.q_ext[0][P.qbase[r] + local] = id;
     }
 }
 
@@ -156,15 +176,21 @@ struct RaySrc {  // a plain ray array (ff_trace_closest / ff_trace_any)
 };
 
 // Persistent traversal.  The grid holds only the lanes the chip keeps
-// resident; a lane whose ray is finished takes the next queue entry from a
-// shared counter (per wave: one ballot and one atomic for all idle lanes), so
-// a wave no longer lasts as long as its slowest ray and no second round of
-// waves is needed.  A wave refills once REFILL of its lanes are idle (or all
-// are), trading idle lanes against refill stalls.  Every wave leaves the loop
-// once the counter has passed n and its own rays are finished.
-template <int BS, int LDS, int REFILL, bool ANY, class Src>
+// resident; a lane whose ray is finished takes the next queue entry from the
+// segmented work counters (per wave: one ballot and one atomic for all idle
+// lanes), so a wave no longer lasts as long as its slowest ray and no second
+// round of waves is needed.  A wave refills once REFILL of its lanes are idle
+// (or all are), trading idle lanes against refill stalls.  Every wave leaves
+// the loop once all segments are exhausted and its own rays are finished.
+// WW selects the loop shape: false = one step (interior record or leaf) per
+// lane per iteration; true = "while-while": lanes at interior records step
+// until every busy lane has reached a leaf, then all leaves are tested
+// together (fewer divergent switches between the two code paths).
+template <int BS, int LDS, int REFILL, bool ANY, bool WW, class Src>
 __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint32_t* fetch, int32_t* ovf,
                                                 unsigned long long* cnt, int* smem) {
+    SegFetch F;
+    F.init(fetch, n, blockIdx.x * (BS / 64) + (threadIdx.x >> 6));
     Stack st;
     st.lds = (lds_int*)&smem[threadIdx.x];
     st.lds_stride = BS;
@@ -184,14 +210,13 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
         if (more && n_idle >= (uint32_t)REFILL) {
             uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
-            int leader = __ffsll((unsigned long long)idle) - 1;
-            uint32_t base = 0;
-            if ((int)__lane_id() == leader) base = atomicAdd(fetch, n_idle);
-            base = __shfl(base, leader);
-            if (base + n_idle >= n) more = false;
-            if (!busy && base + rank < n) {
+            uint32_t got;
+            bool done;
+            uint32_t first = F.take(n_idle, got, done);
+            if (done) more = false;
+            if (!busy && rank < got) {
                 f3 o, d;
-                src.load(base + rank, id, B, tris, o, d, two);
+                src.load(first + rank, id, B, tris, o, d, two);
                 h = no_hit();
                 busy = trav_begin(B, o, d, R, st, cn);
                 if (!busy) {
@@ -204,7 +229,35 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
             if (!more) break;
             continue;
         }
-        if (busy) {
+        if (WW) {
+            for (;;) {  // interior records until every busy lane is at a leaf
+                bool at_int = busy && R.ref >= 0;
+                if (__ballot(at_int) == 0) break;
+                if (at_int && !visit_interior<!ANY>(B, R, h.t, st, cn)) {
+                    if (st.sp == 0) {
+                        if (ANY) src.occluded(id, false);
+                        else src.hit(id, h);
+                        busy = false;
+                    } else {
+                        R.ref = st.pop();
+                    }
+                }
+            }
+            if (busy) {  // at a leaf
+                bool done;
+                if (ANY) {
+                    bool occ = leaf_any(B, tris, R, ct);
+                    done = occ || st.sp == 0;
+                    if (done) src.occluded(id, occ);
+                } else {
+                    leaf_closest(B, tris, R, h, two, ct);
+                    done = st.sp == 0;
+                    if (done) src.hit(id, h);
+                }
+                if (done) busy = false;
+                else R.ref = st.pop();
+            }
+        } else if (busy) {
             if (ANY) {
                 int r = any_step(B, tris, R, st, cn, ct);
                 if (r != 0) {
@@ -223,25 +276,28 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
     }
 }
 
-template <int BS, int LDS, int RF>
+template <int BS, int LDS, int RF, bool WW>
 __global__ __launch_bounds__(BS) void k_extend(ff_paths P, int q) {
     __shared__ int smem[LDS * BS];
     ExtSrc src{P, P.q_ext[q]};
-    trav_persistent<BS, LDS, RF, false>(src, P.counters[q == 0 ? C_EXT0 : C_EXT1], &P.counters[C_FETCH_EXT], P.ovf,
-                                        &P.stats[S_EXT_NODES], smem);
+    trav_persistent<BS, LDS, RF, false, WW>(src, P.counters[q == 0 ? C_EXT0 : C_EXT1],
+                                            P.fetch + F_EXT * FETCH_SEGS * FETCH_STRIDE, P.ovf,
+                                            stats_rep(P) + S_EXT_NODES, smem);
 }
-template <int BS, int LDS, int RF>
+template <int BS, int LDS, int RF, bool WW>
 __global__ __launch_bounds__(BS) void k_shadow(ff_paths P) {
     __shared__ int smem[LDS * BS];
     ShadowSrc src{P};
-    trav_persistent<BS, LDS, RF, true>(src, P.counters[C_SHADOW], &P.counters[C_FETCH_SHD], P.ovf,
-                                       &P.stats[S_SHD_NODES], smem);
+    trav_persistent<BS, LDS, RF, true, WW>(src, P.counters[C_SHADOW], P.fetch + F_SHD * FETCH_SEGS * FETCH_STRIDE,
+                                           P.ovf, stats_rep(P) + S_SHD_NODES, smem);
 }
 
-// counters consumed by the previous stages are cleared by a kernel (graph-safe)
-__global__ void k_reset_counters(uint32_t* c, uint32_t mask) {
+// counters consumed by the previous stages are cleared by a kernel (graph-safe):
+// queue counters by `mask`, traversal work counters by kind (`fetch_mask`)
+__global__ void k_reset_counters(uint32_t* c, uint32_t mask, uint32_t* fetch, uint32_t fetch_mask) {
     uint32_t i = threadIdx.x;
-    if (i < C_COUNT && ((mask >> i) & 1u)) c[i] = 0;
+    if (c && i < C_COUNT && ((mask >> i) & 1u)) c[i] = 0;
+    if (fetch && i < 3 * FETCH_SEGS && ((fetch_mask >> (i / FETCH_SEGS)) & 1u)) fetch[(size_t)i * FETCH_STRIDE] = 0;
 }
 
 // =================================================================== logic
@@ -250,9 +306,11 @@ __global__ void k_reset_counters(uint32_t* c, uint32_t mask) {
 // environment radiance for escaped rays after the camera (or a delta event),
 // termination into the film, routing of live paths to material queues.
 __global__ __launch_bounds__(FF_BLOCK) void k_logic(ff_paths P, int q) {
+    __shared__ uint32_t lds[4 * (FF_BLOCK / 64 + 1)];
     const uint32_t n = P.counters[q == 0 ? C_EXT0 : C_EXT1];
     const uint32_t* queue = P.q_ext[q];
-    const uint32_t n_round = (n + 63u) & ~63u;
+    const uint32_t n_round = (n + FF_BLOCK - 1) / FF_BLOCK * FF_BLOCK;  // uniform trip count per workgroup
+    uint32_t c_prim = 0, c_ext = 0, c_paths = 0;
     for (uint32_t i = gtid(); i < n_round; i += gstride()) {
         bool live = i < n;
         int route = -1;
@@ -318,15 +376,23 @@ __global__ __launch_bounds__(FF_BLOCK) void k_logic(ff_paths P, int q) {
                 route = mtype;
             }
         }
-        wave_count(&P.stats[S_PRIMARY], live && primary);
-        wave_count(&P.stats[S_EXTENSION], live && !primary);
-        wave_count(&P.stats[S_PATHS], live && term);
+        c_prim += (live && primary) ? 1u : 0u;
+        c_ext += (live && !primary) ? 1u : 0u;
+        c_paths += (live && term) ? 1u : 0u;
+        const int qidx[4] = {C_MAT0, C_MAT0 + 1, C_MAT0 + 2, C_MAT0 + 3};
+        uint32_t cnt[4], first[4];
 #pragma unroll
-        for (int m = 0; m < 4; m++) {
-            uint32_t slot = wave_append(&P.counters[C_MAT0 + m], route == m);
-            if (route == m) P.q_mat[m][slot] = id;
+        for (int m = 0; m < 4; m++) cnt[m] = route == m ? 1u : 0u;
+        block_alloc<FF_BLOCK, 4>(P.counters, qidx, cnt, first, lds);
+        if (route >= 0) {
+            uint32_t f = route == 0 ? first[0] : (route == 1 ? first[1] : (route == 2 ? first[2] : first[3]));
+            P.q_mat[route][f] = id;
         }
     }
+    unsigned long long* sr = stats_rep(P);
+    wave_sum_add(sr + S_PRIMARY, c_prim);
+    wave_sum_add(sr + S_EXTENSION, c_ext);
+    wave_sum_add(sr + S_PATHS, c_paths);
 }
 
 // ======================================================= material queues
@@ -564,10 +630,12 @@ __device__ __forceinline__ void shade_core(const ff_paths& P, uint32_t id, Shade
 
 template <int TYPE>
 __global__ __launch_bounds__(FF_BLOCK) void k_shade(ff_paths P, int next_q) {
+    __shared__ uint32_t lds[2 * (FF_BLOCK / 64 + 1)];
     const uint32_t n = P.counters[C_MAT0 + TYPE];
     const uint32_t* queue = P.q_mat[TYPE];
-    const uint32_t n_round = (n + 63u) & ~63u;
-    uint32_t* ext_counter = &P.counters[next_q == 0 ? C_EXT0 : C_EXT1];
+    const uint32_t n_round = (n + FF_BLOCK - 1) / FF_BLOCK * FF_BLOCK;
+    const int qidx[2] = {next_q == 0 ? C_EXT0 : C_EXT1, C_SHADOW};
+    uint32_t c_shd = 0;
     for (uint32_t i = gtid(); i < n_round; i += gstride()) {
         bool live = i < n;
         uint32_t id = live ? queue[i] : 0u;
@@ -596,18 +664,17 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade(ff_paths P, int next_q) {
             P.ray[2 * id + 1] = f4(so.next_d, 0.f);
             P.beta[id].w = __uint_as_float(so.nst);
         }
-        uint32_t es = wave_append(ext_counter, live);
-        if (live) P.q_ext[next_q][es] = id;
-        wave_count(&P.stats[S_SHADOW], needL);
-        wave_count(&P.stats[S_SHADOW], needS);
-        wave_count(&P.stats[S_SHADOW], needD);
-        uint32_t s0 = wave_append(&P.counters[C_SHADOW], needL);
-        if (needL) P.q_shadow[s0] = 3 * id + 0;
-        uint32_t s1 = wave_append(&P.counters[C_SHADOW], needS);
-        if (needS) P.q_shadow[s1] = 3 * id + 1;
-        uint32_t s2 = wave_append(&P.counters[C_SHADOW], needD);
-        if (needD) P.q_shadow[s2] = 3 * id + 2;
+        uint32_t ns = (needL ? 1u : 0u) + (needS ? 1u : 0u) + (needD ? 1u : 0u);
+        c_shd += ns;
+        uint32_t cnt[2] = {live ? 1u : 0u, ns}, first[2];
+        block_alloc<FF_BLOCK, 2>(P.counters, qidx, cnt, first, lds);
+        if (live) P.q_ext[next_q][first[0]] = id;
+        uint32_t sq = first[1];
+        if (needL) P.q_shadow[sq++] = 3 * id + 0;
+        if (needS) P.q_shadow[sq++] = 3 * id + 1;
+        if (needD) P.q_shadow[sq] = 3 * id + 2;
     }
+    wave_sum_add(stats_rep(P) + S_SHADOW, c_shd);
 }
 
 // ================================================== path-replay adjoint
@@ -669,9 +736,11 @@ __device__ __forceinline__ void grad_tex(const ff_paths& P, const ff_dev_request
 // k_logic; an escaping delta/camera ray's Le term is replayed.  The vertex's
 // NEE terms and throughput update were replayed by k_shade_adj.
 __global__ __launch_bounds__(FF_BLOCK) void k_logic_adj(ff_paths P, int q) {
+    __shared__ uint32_t lds[4 * (FF_BLOCK / 64 + 1)];
     const uint32_t n = P.counters[q == 0 ? C_EXT0 : C_EXT1];
     const uint32_t* queue = P.q_ext[q];
-    const uint32_t n_round = (n + 63u) & ~63u;
+    const uint32_t n_round = (n + FF_BLOCK - 1) / FF_BLOCK * FF_BLOCK;  // uniform trip count per workgroup
+    uint32_t c_prim = 0, c_ext = 0, c_paths = 0;
     for (uint32_t i = gtid(); i < n_round; i += gstride()) {
         bool live = i < n;
         int route = -1;
@@ -714,23 +783,34 @@ __global__ __launch_bounds__(FF_BLOCK) void k_logic_adj(ff_paths P, int q) {
                 route = mtype;
             }
         }
-        wave_count(&P.stats[S_PRIMARY], live && primary);
-        wave_count(&P.stats[S_EXTENSION], live && !primary);
-        wave_count(&P.stats[S_PATHS], live && term);
+        c_prim += (live && primary) ? 1u : 0u;
+        c_ext += (live && !primary) ? 1u : 0u;
+        c_paths += (live && term) ? 1u : 0u;
+        const int qidx[4] = {C_MAT0, C_MAT0 + 1, C_MAT0 + 2, C_MAT0 + 3};
+        uint32_t cnt[4], first[4];
 #pragma unroll
-        for (int m = 0; m < 4; m++) {
-            uint32_t slot = wave_append(&P.counters[C_MAT0 + m], route == m);
-            if (route == m) P.q_mat[m][slot] = id;
+        for (int m = 0; m < 4; m++) cnt[m] = route == m ? 1u : 0u;
+        block_alloc<FF_BLOCK, 4>(P.counters, qidx, cnt, first, lds);
+        if (route >= 0) {
+            uint32_t f = route == 0 ? first[0] : (route == 1 ? first[1] : (route == 2 ? first[2] : first[3]));
+            P.q_mat[route][f] = id;
         }
     }
+    unsigned long long* sr = stats_rep(P);
+    wave_sum_add(sr + S_PRIMARY, c_prim);
+    wave_sum_add(sr + S_EXTENSION, c_ext);
+    wave_sum_add(sr + S_PATHS, c_paths);
 }
 
 // Replay of a vertex's shadow rays: the same rays k_shade queues.
 template <int TYPE>
 __global__ __launch_bounds__(FF_BLOCK) void k_shade_nee(ff_paths P) {
+    __shared__ uint32_t lds[FF_BLOCK / 64 + 1];
     const uint32_t n = P.counters[C_MAT0 + TYPE];
     const uint32_t* queue = P.q_mat[TYPE];
-    const uint32_t n_round = (n + 63u) & ~63u;
+    const uint32_t n_round = (n + FF_BLOCK - 1) / FF_BLOCK * FF_BLOCK;
+    const int qidx[1] = {C_SHADOW};
+    uint32_t c_shd = 0;
     for (uint32_t i = gtid(); i < n_round; i += gstride()) {
         bool live = i < n;
         uint32_t id = live ? queue[i] : 0u;
@@ -752,16 +832,16 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade_nee(ff_paths P) {
                 P.sray[6 * id + 5] = f4(so.wi_d, 0.f);
             }
         }
-        wave_count(&P.stats[S_SHADOW], needL);
-        wave_count(&P.stats[S_SHADOW], needS);
-        wave_count(&P.stats[S_SHADOW], needD);
-        uint32_t s0 = wave_append(&P.counters[C_SHADOW], needL);
-        if (needL) P.q_shadow[s0] = 3 * id + 0;
-        uint32_t s1 = wave_append(&P.counters[C_SHADOW], needS);
-        if (needS) P.q_shadow[s1] = 3 * id + 1;
-        uint32_t s2 = wave_append(&P.counters[C_SHADOW], needD);
-        if (needD) P.q_shadow[s2] = 3 * id + 2;
+        uint32_t ns = (needL ? 1u : 0u) + (needS ? 1u : 0u) + (needD ? 1u : 0u);
+        c_shd += ns;
+        uint32_t cnt[1] = {ns}, first[1];
+        block_alloc<FF_BLOCK, 1>(P.counters, qidx, cnt, first, lds);
+        uint32_t sq = first[0];
+        if (needL) P.q_shadow[sq++] = 3 * id + 0;
+        if (needS) P.q_shadow[sq++] = 3 * id + 1;
+        if (needD) P.q_shadow[sq] = 3 * id + 2;
     }
+    wave_sum_add(stats_rep(P) + S_SHADOW, c_shd);
 }
 
 // Replay of a vertex once its shadow rays are traced: its visibility-gated
@@ -770,10 +850,11 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade_nee(ff_paths P) {
 // next extension ray.
 template <int TYPE>
 __global__ __launch_bounds__(FF_BLOCK) void k_shade_adj(ff_paths P, int next_q) {
+    __shared__ uint32_t lds[FF_BLOCK / 64 + 1];
     const uint32_t n = P.counters[C_MAT0 + TYPE];
     const uint32_t* queue = P.q_mat[TYPE];
-    const uint32_t n_round = (n + 63u) & ~63u;
-    uint32_t* ext_counter = &P.counters[next_q == 0 ? C_EXT0 : C_EXT1];
+    const uint32_t n_round = (n + FF_BLOCK - 1) / FF_BLOCK * FF_BLOCK;
+    const int qidx[1] = {next_q == 0 ? C_EXT0 : C_EXT1};
     for (uint32_t i = gtid(); i < n_round; i += gstride()) {
         bool live = i < n;
         uint32_t id = live ? queue[i] : 0u;
@@ -834,8 +915,9 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade_adj(ff_paths P, int next_q) 
             P.ray[2 * id] = f4(so.next_o, 0.f);
             P.ray[2 * id + 1] = f4(so.next_d, 0.f);
         }
-        uint32_t es = wave_append(ext_counter, live);
-        if (live) P.q_ext[next_q][es] = id;
+        uint32_t cnt[1] = {live ? 1u : 0u}, first[1];
+        block_alloc<FF_BLOCK, 1>(P.counters, qidx, cnt, first, lds);
+        if (live) P.q_ext[next_q][first[0]] = id;
     }
 }
 
@@ -858,8 +940,7 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n) {
     for (uint64_t i = gtid(); i < n; i += gstride()) p[i] = 0u;
 }
 __global__ void k_fill_u64(unsigned long long* p, uint32_t n) {
-    uint32_t i = threadIdx.x;
-    if (i < n) p[i] = 0ull;
+    for (uint32_t i = gtid(); i < n; i += gstride()) p[i] = 0ull;
 }
 // film[pixel] += sum over layers k (in k order) of layers[k*WH + pixel]
 __global__ void k_resolve_layers(const float4* layers, uint32_t K, uint32_t WH, float4* film) {
@@ -881,14 +962,14 @@ __global__ __launch_bounds__(BS) void k_trace_closest(ff_dev_scene S, const floa
                                                       unsigned long long* counts) {
     __shared__ int smem[LDS * BS];
     RaySrc src{S, rays, hits, nullptr, two_sided};
-    trav_persistent<BS, LDS, RF, false>(src, n, fetch, ovf, counts, smem);
+    trav_persistent<BS, LDS, RF, false, false>(src, n, fetch, ovf, counts, smem);
 }
 template <int BS, int LDS, int RF>
 __global__ __launch_bounds__(BS) void k_trace_any(ff_dev_scene S, const float4* rays, uint32_t n, uint8_t* occ,
                                                   uint32_t* fetch, int32_t* ovf, unsigned long long* counts) {
     __shared__ int smem[LDS * BS];
     RaySrc src{S, rays, nullptr, occ, 0};
-    trav_persistent<BS, LDS, RF, true>(src, n, fetch, ovf, counts, smem);
+    trav_persistent<BS, LDS, RF, true, false>(src, n, fetch, ovf, counts, smem);
 }
 __global__ void k_gen_rays(ff_dev_camera cam, int32_t W, int32_t H, uint32_t seed, uint32_t sample,
                            float4* rays) {
@@ -1056,36 +1137,40 @@ uint32_t resident_grid(K kernel, int bs) {
     return (uint32_t)g;
 }
 
-// traversal variants (block size, LDS stack entries, refill threshold)
+// traversal variants (block size, LDS stack entries, refill threshold, loop shape)
 #define FF_TRAV_VARIANT_LIST(X) \
-    X(0, 256, 16, 32)           \
-    X(1, 256, 16, 16)           \
-    X(2, 256, 16, 64)           \
-    X(3, 128, 16, 32)           \
-    X(4, 256, 8, 32)            \
-    X(5, 256, 24, 32)           \
-    X(6, 64, 32, 32)            \
-    X(7, 128, 24, 32)
+    X(0, 256, 16, 32, false)    \
+    X(1, 256, 16, 16, false)    \
+    X(2, 256, 16, 64, false)    \
+    X(3, 128, 16, 32, false)    \
+    X(4, 256, 8, 32, false)     \
+    X(5, 256, 24, 32, false)    \
+    X(6, 64, 32, 32, false)     \
+    X(7, 128, 24, 32, false)    \
+    X(8, 256, 16, 32, true)     \
+    X(9, 64, 32, 32, true)      \
+    X(10, 256, 16, 16, true)    \
+    X(11, 128, 24, 32, true)
 
-template <int BS, int LDS, int RF>
+template <int BS, int LDS, int RF, bool WW>
 hipError_t trav_launch(int which, const ff_paths& P, int q, hipStream_t s) {
     if (which == 0) {
-        uint32_t g = resident_grid(k_extend<BS, LDS, RF>, BS);
-        hipLaunchKernelGGL((k_extend<BS, LDS, RF>), dim3(g), dim3(BS), 0, s, P, q);
+        uint32_t g = resident_grid(k_extend<BS, LDS, RF, WW>, BS);
+        hipLaunchKernelGGL((k_extend<BS, LDS, RF, WW>), dim3(g), dim3(BS), 0, s, P, q);
     } else if (which == 1) {
-        uint32_t g = resident_grid(k_shadow<BS, LDS, RF>, BS);
-        hipLaunchKernelGGL((k_shadow<BS, LDS, RF>), dim3(g), dim3(BS), 0, s, P);
+        uint32_t g = resident_grid(k_shadow<BS, LDS, RF, WW>, BS);
+        hipLaunchKernelGGL((k_shadow<BS, LDS, RF, WW>), dim3(g), dim3(BS), 0, s, P);
     } else {  // grid sizes only (before a graph capture)
-        (void)resident_grid(k_extend<BS, LDS, RF>, BS);
-        (void)resident_grid(k_shadow<BS, LDS, RF>, BS);
+        (void)resident_grid(k_extend<BS, LDS, RF, WW>, BS);
+        (void)resident_grid(k_shadow<BS, LDS, RF, WW>, BS);
     }
     return hipGetLastError();
 }
 
 hipError_t trav_dispatch(int which, int variant, const ff_paths& P, int q, hipStream_t s) {
     switch (variant) {
-#define FF_TRAV_CASE(v, bs, lds, rf) \
-    case v: return trav_launch<bs, lds, rf>(which, P, q, s);
+#define FF_TRAV_CASE(v, bs, lds, rf, ww) \
+    case v: return trav_launch<bs, lds, rf, ww>(which, P, q, s);
         FF_TRAV_VARIANT_LIST(FF_TRAV_CASE)
 #undef FF_TRAV_CASE
         default: return hipErrorInvalidValue;
@@ -1097,19 +1182,22 @@ hipError_t trav_dispatch(int which, int variant, const ff_paths& P, int q, hipSt
 // ===================================================================== host
 namespace ffk {
 
-const int kTravVariants = 8;
-#define FF_TRAV_BS(v, bs, lds, rf) bs,
-#define FF_TRAV_LDS(v, bs, lds, rf) lds,
-#define FF_TRAV_RF(v, bs, lds, rf) rf,
+const int kTravVariants = 12;
+#define FF_TRAV_BS(v, bs, lds, rf, ww) bs,
+#define FF_TRAV_LDS(v, bs, lds, rf, ww) lds,
+#define FF_TRAV_RF(v, bs, lds, rf, ww) rf,
+#define FF_TRAV_WW(v, bs, lds, rf, ww) (ww ? 1 : 0),
 const int kTravBlock[kTravVariants] = {FF_TRAV_VARIANT_LIST(FF_TRAV_BS)};
 const int kTravLds[kTravVariants] = {FF_TRAV_VARIANT_LIST(FF_TRAV_LDS)};
 const int kTravRefill[kTravVariants] = {FF_TRAV_VARIANT_LIST(FF_TRAV_RF)};
+const int kTravWhileWhile[kTravVariants] = {FF_TRAV_VARIANT_LIST(FF_TRAV_WW)};
 #undef FF_TRAV_BS
 #undef FF_TRAV_LDS
 #undef FF_TRAV_RF
+#undef FF_TRAV_WW
 
 hipError_t launch_batch_begin(const ff_paths& P, int set, uint32_t value, hipStream_t s) {
-    hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(64), 0, s, P.bstate, P.counters, set, value);
+    hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(256), 0, s, P, set, value);
     return hipGetLastError();
 }
 hipError_t launch_generate(const ff_paths& P, hipStream_t s) {
@@ -1124,8 +1212,8 @@ hipError_t prepare_traversal(int variant) {
     ff_paths none{};
     return trav_dispatch(2, variant, none, 0, nullptr);
 }
-hipError_t launch_reset(uint32_t* counters, uint32_t mask, hipStream_t s) {
-    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, counters, mask);
+hipError_t launch_reset(uint32_t* counters, uint32_t mask, uint32_t* fetch, uint32_t fetch_mask, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(256), 0, s, counters, mask, fetch, fetch_mask);
     return hipGetLastError();
 }
 hipError_t launch_logic(const ff_paths& P, int q, uint32_t n_max, hipStream_t s) {
@@ -1180,7 +1268,7 @@ hipError_t launch_fill_u32(uint32_t* p, uint64_t n, hipStream_t s) {
     return hipGetLastError();
 }
 hipError_t launch_fill_u64(unsigned long long* p, uint32_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, p, n);
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, p, n);
     return hipGetLastError();
 }
 hipError_t launch_resolve_layers(const float4* layers, uint32_t K, uint32_t WH, float4* film, hipStream_t s) {
@@ -1189,20 +1277,20 @@ hipError_t launch_resolve_layers(const float4* layers, uint32_t K, uint32_t WH, 
     return hipGetLastError();
 }
 hipError_t launch_trace_closest(const ff_dev_scene& S, const float4* rays, uint32_t n, float4* hits,
-                                int two_sided, uint32_t* counters, int32_t* ovf, unsigned long long* counts,
+                                int two_sided, uint32_t* fetch, int32_t* ovf, unsigned long long* counts,
                                 hipStream_t s) {
-    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, counters, 1u << C_FETCH_TRACE);
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(256), 0, s, nullptr, 0u, fetch, 1u << F_TRACE);
     uint32_t g = resident_grid(k_trace_closest<256, 16, 32>, 256);
     hipLaunchKernelGGL((k_trace_closest<256, 16, 32>), dim3(g), dim3(256), 0, s, S, rays, n, hits, two_sided,
-                       counters + C_FETCH_TRACE, ovf, counts);
+                       fetch + F_TRACE * FETCH_SEGS * FETCH_STRIDE, ovf, counts);
     return hipGetLastError();
 }
-hipError_t launch_trace_any(const ff_dev_scene& S, const float4* rays, uint32_t n, uint8_t* occ, uint32_t* counters,
+hipError_t launch_trace_any(const ff_dev_scene& S, const float4* rays, uint32_t n, uint8_t* occ, uint32_t* fetch,
                             int32_t* ovf, unsigned long long* counts, hipStream_t s) {
-    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, counters, 1u << C_FETCH_TRACE);
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(256), 0, s, nullptr, 0u, fetch, 1u << F_TRACE);
     uint32_t g = resident_grid(k_trace_any<256, 16, 32>, 256);
     hipLaunchKernelGGL((k_trace_any<256, 16, 32>), dim3(g), dim3(256), 0, s, S, rays, n, occ,
-                       counters + C_FETCH_TRACE, ovf, counts);
+                       fetch + F_TRACE * FETCH_SEGS * FETCH_STRIDE, ovf, counts);
     return hipGetLastError();
 }
 hipError_t launch_gen_rays(const ff_dev_camera& cam, int W, int H, uint32_t seed, uint32_t sample, float4* rays,

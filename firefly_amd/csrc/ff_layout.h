@@ -32,10 +32,17 @@
 #define C_EXT1 1
 #define C_SHADOW 2
 #define C_MAT0 3
-#define C_FETCH_EXT 7    // persistent traversal work counters
-#define C_FETCH_SHD 8
-#define C_FETCH_TRACE 9
 #define C_COUNT 16
+
+// Work counters of the persistent traversal kernels: the queue is split into
+// FETCH_SEGS segments with one counter each, every counter on its own 128-B
+// line (device-scope atomics on one address serialise at the memory side).
+#define FETCH_SEGS 64
+#define FETCH_STRIDE 32
+#define F_EXT 0
+#define F_SHD 1
+#define F_TRACE 2
+#define FETCH_WORDS (3 * FETCH_SEGS * FETCH_STRIDE)
 
 // ray statistics
 #define S_PRIMARY 0
@@ -47,6 +54,10 @@
 #define S_SHD_NODES 6
 #define S_SHD_TRIS 7
 #define S_COUNT 8
+// ray statistics are replicated per workgroup group on separate lines
+#define S_REPLICAS 64
+#define S_STRIDE 16
+#define S_WORDS (S_REPLICAS * S_STRIDE)
 
 #define FF_MAT_FLOATS_DEV 16
 
@@ -140,8 +151,10 @@ struct ff_paths {
     uint32_t* q_mat[4];
     uint32_t* q_shadow;
     uint32_t* counters;  // [C_COUNT]
+    uint32_t* fetch;     // [FETCH_WORDS] traversal work counters
+    uint32_t* qbase;     // [n_req + 1] first extension-queue slot of each request's new samples
     float4* layers;      // [P] film layer accumulators (path id = layer*WH + pixel)
-    unsigned long long* stats;  // [S_COUNT]
+    unsigned long long* stats;  // [S_REPLICAS][S_STRIDE], S_COUNT used per replica
     int32_t* ovf;        // traversal stack overflow [depth][FF_TRAV_GRID*FF_BLOCK]
     uint32_t* bstate;    // [B_COUNT]
     const ff_dev_request* reqs;

@@ -82,14 +82,14 @@ def test_traversal_variants_agree(gpu):
     env = abi.EnvMap(gpu, sd.env)
     cam = abi.make_camera(sd.inv_vp)
     ref = None
-    for v, block, lds, refill in abi.traversal_variants():
+    for v, block, lds, refill, ww in abi.traversal_variants():
         abi.set_traversal_variant(gpu, v)
         f = torch.zeros((sd.height, sd.width, 4), dtype=torch.float32, device="cuda")
         abi.render(gpu, sc, env, cam, sd.width, sd.height, 2, 3, 0, f)
         torch.cuda.synchronize()
         if ref is None:
             ref = f
-        assert torch.equal(f, ref), f"variant {v} (block {block}, lds {lds}, refill {refill})"
+        assert torch.equal(f, ref), f"variant {v} (block {block}, lds {lds}, refill {refill}, while-while {ww})"
     abi.set_traversal_variant(gpu, 0)
 
 
