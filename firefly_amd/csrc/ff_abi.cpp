@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
+#include <limits>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -105,6 +107,7 @@ struct ff_scene {
     std::vector<ffh::BvhNode> nodes;
     std::vector<int32_t> ordered;
     float4* d_nodes = nullptr;       // two-child records
+    float4* d_nodes4 = nullptr;      // 4-wide records
     int32_t* d_leaf_count = nullptr;
     float* d_root = nullptr;
     float* d_tris = nullptr;
@@ -119,7 +122,7 @@ struct ff_scene {
     bool has_type[4] = {false, false, false, false};
     ff_dev_scene view() const {
         ff_dev_scene s;
-        s.nodes = d_nodes; s.leaf_count = d_leaf_count; s.root = d_root; s.tris = d_tris; s.nrm = d_nrm; s.uv = d_uv; s.mat = d_mat;
+        s.nodes = d_nodes; s.nodes4 = d_nodes4; s.leaf_count = d_leaf_count; s.root = d_root; s.tris = d_tris; s.nrm = d_nrm; s.uv = d_uv; s.mat = d_mat;
         s.mats = d_mats; s.tex = d_tex_desc; s.n_nodes = (int32_t)nodes.size(); s.n_tris = n_tris;
         s.n_mats = n_mats; s.n_tex = n_tex; s.max_depth = max_depth;
         return s;
@@ -214,6 +217,97 @@ void pack_two_child(const std::vector<ffh::BvhNode>& nodes, std::vector<float4>&
     std::memcpy(root + 3, nodes[0].bmax, 12);
     int32_t r = ref_of(0);
     std::memcpy(root + 6, &r, 4);
+    root[7] = 0.f;
+}
+
+// The binary tree collapsed to 4-wide records of 128 bytes.  The record of
+// interior node N holds the boxes of N's grandchildren: slots 0-1 under N's
+// first child, 2-3 under its second (a child that is a leaf keeps its own
+// box in the first slot of its pair and leaves the second empty), plus the
+// three split axes (N, first child, second child) that order them:
+//   f[0..5] = lo.x, lo.y, lo.z, hi.x, hi.y, hi.z of slots 0..3
+//   f[6]    = refs: >= 0 record index, < 0 leaf ~(first slot << 4 | count)
+//   f[7]    = (axis N, axis first child, axis second child, 0)
+// An empty slot has the box (+inf, -inf), which every slab test misses, and
+// the ref of a leaf with no triangles.  Records are numbered in pre-order.
+void pack_four(const std::vector<ffh::BvhNode>& nodes, std::vector<float4>& recs,
+               std::vector<int32_t>& leaf_count, float root[8], size_t n_tris) {
+    leaf_count.assign(std::max<size_t>(n_tris, 1), 0);
+    std::vector<int32_t> index(nodes.size(), -1);
+    std::vector<size_t> order;  // binary node of each record, pre-order
+    if (nodes[0].n_prims == 0) {
+        std::vector<size_t> stack{0};
+        while (!stack.empty()) {
+            size_t i = stack.back();
+            stack.pop_back();
+            index[i] = (int32_t)order.size();
+            order.push_back(i);
+            size_t kids[2] = {i + 1, (size_t)nodes[i].offset};
+            std::vector<size_t> slots;
+            for (size_t c : kids) {
+                if (nodes[c].n_prims == 0) {
+                    slots.push_back(c + 1);
+                    slots.push_back((size_t)nodes[c].offset);
+                } else {
+                    slots.push_back(c);
+                    slots.push_back(SIZE_MAX);
+                }
+            }
+            for (size_t k = slots.size(); k-- > 0;)
+                if (slots[k] != SIZE_MAX && nodes[slots[k]].n_prims == 0) stack.push_back(slots[k]);
+        }
+    }
+    auto leaf_ref = [&](const ffh::BvhNode& nd) -> int32_t {
+        int32_t c = nd.n_prims < 15 ? nd.n_prims : 15;
+        if (c == 15) leaf_count[nd.offset] = nd.n_prims;
+        return ~((nd.offset << 4) | c);
+    };
+    auto ref_of = [&](size_t i) -> int32_t {
+        return nodes[i].n_prims == 0 ? index[i] : leaf_ref(nodes[i]);
+    };
+    const float inf = std::numeric_limits<float>::infinity();
+    recs.assign(order.size() * 8, make_float4(0.f, 0.f, 0.f, 0.f));
+    for (size_t r = 0; r < order.size(); r++) {
+        size_t i = order[r];
+        size_t kids[2] = {i + 1, (size_t)nodes[i].offset};
+        float lo[3][4], hi[3][4];
+        int32_t ref[4], ax[4] = {(int32_t)nodes[i].axis, 0, 0, 0};
+        for (int side = 0; side < 2; side++) {
+            size_t c = kids[side];
+            size_t sl[2];
+            bool two = nodes[c].n_prims == 0;
+            if (two) {
+                sl[0] = c + 1;
+                sl[1] = (size_t)nodes[c].offset;
+                ax[1 + side] = (int32_t)nodes[c].axis;
+            } else {
+                sl[0] = c;
+                sl[1] = SIZE_MAX;
+            }
+            for (int k = 0; k < 2; k++) {
+                int s = 2 * side + k;
+                if (sl[k] == SIZE_MAX) {
+                    for (int a = 0; a < 3; a++) { lo[a][s] = inf; hi[a][s] = -inf; }
+                    ref[s] = -1;  // a leaf of zero triangles
+                } else {
+                    const ffh::BvhNode& nd = nodes[sl[k]];
+                    for (int a = 0; a < 3; a++) { lo[a][s] = nd.bmin[a]; hi[a][s] = nd.bmax[a]; }
+                    ref[s] = ref_of(sl[k]);
+                }
+            }
+        }
+        float4* w = &recs[r * 8];
+        for (int a = 0; a < 3; a++) {
+            w[a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
+            w[3 + a] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
+        }
+        std::memcpy(&w[6], ref, 16);
+        std::memcpy(&w[7], ax, 16);
+    }
+    std::memcpy(root, nodes[0].bmin, 12);
+    std::memcpy(root + 3, nodes[0].bmax, 12);
+    int32_t r0 = ref_of(0);
+    std::memcpy(root + 6, &r0, 4);
     root[7] = 0.f;
 }
 
@@ -353,13 +447,20 @@ ff_status ff_scene_create(ff_context* ctx, const ff_scene_desc* d, ff_scene** ou
     std::vector<int32_t> leaf_count;
     float root[8];
     pack_two_child(s->nodes, wide, leaf_count, root, n);
-    if (dalloc(&s->d_nodes, wide.size()) != hipSuccess || dalloc(&s->d_leaf_count, leaf_count.size()) != hipSuccess ||
+    std::vector<float4> four;
+    std::vector<int32_t> leaf_count4;
+    float root4[8];
+    pack_four(s->nodes, four, leaf_count4, root4, n);
+    if (dalloc(&s->d_nodes, wide.size()) != hipSuccess || dalloc(&s->d_nodes4, four.size()) != hipSuccess ||
+        dalloc(&s->d_leaf_count, leaf_count.size()) != hipSuccess ||
         dalloc(&s->d_root, 8) != hipSuccess || dalloc(&s->d_tris, 9 * n) != hipSuccess ||
         dalloc(&s->d_nrm, 9 * n) != hipSuccess || dalloc(&s->d_uv, 6 * n) != hipSuccess ||
         dalloc(&s->d_mat, n) != hipSuccess || dalloc(&s->d_mats, (size_t)FF_MAT_FLOATS * d->n_materials) != hipSuccess)
         return cleanup(fail(FF_ERR_ALLOC, "ff_scene_create: device allocation failed"));
     hipError_t e = hipSuccess;
     if (!wide.empty()) e = hipMemcpy(s->d_nodes, wide.data(), wide.size() * sizeof(float4), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !four.empty())
+        e = hipMemcpy(s->d_nodes4, four.data(), four.size() * sizeof(float4), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(s->d_leaf_count, leaf_count.data(), leaf_count.size() * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->d_root, root, sizeof(root), hipMemcpyHostToDevice);
@@ -394,7 +495,7 @@ ff_status ff_scene_create(ff_context* ctx, const ff_scene_desc* d, ff_scene** ou
 ff_status ff_scene_destroy(ff_scene* s) {
     if (!s) return fail(FF_ERR_INVALID, "ff_scene_destroy: NULL scene");
     (void)hipDeviceSynchronize();
-    dfree(s->d_nodes); dfree(s->d_leaf_count); dfree(s->d_root); dfree(s->d_tris); dfree(s->d_nrm); dfree(s->d_uv); dfree(s->d_mat); dfree(s->d_mats);
+    dfree(s->d_nodes); dfree(s->d_nodes4); dfree(s->d_leaf_count); dfree(s->d_root); dfree(s->d_tris); dfree(s->d_nrm); dfree(s->d_uv); dfree(s->d_mat); dfree(s->d_mats);
     for (auto& p : s->d_tex) dfree(p);
     dfree(s->d_tex_desc);
     delete s;

@@ -525,7 +525,8 @@ struct Hit { float t, u, v; int id; };
 // count (count 15 = read leaf_count[first slot]).  The root box is tested
 // first, as the reference tests node 0.
 struct Bvh {
-    const float4* nodes;
+    const float4* nodes;   // two-child records
+    const float4* nodes4;  // 4-wide records (same tree, collapsed)
     const int32_t* leaf_count;
     const float* root;  // lo.xyz, hi.xyz, ref (int bits)
 };
@@ -609,6 +610,61 @@ __device__ __forceinline__ bool visit_interior(const Bvh& B, TravRay& R, float t
 // 0 <= t < tmin.  Children whose boxes miss (or lie beyond the current hit by
 // the conservative slack, or wholly behind the origin) are never visited.
 // Returns false once the traversal is complete.
+// One 4-wide record: the four grandchild boxes are tested at once and the
+// hit ones are visited in the binary tree's near-first order (side by the
+// parent's split axis, then each side by its own axis): the first is entered,
+// the others pushed so that they pop in order.  Leaves are visited in the
+// order the binary walk visits them, so the closest hit (ties included) is
+// unchanged.  Empty slots (box +inf..-inf) never hit.
+__device__ __forceinline__ float sel4(const float4& v, int k) {
+    return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
+}
+__device__ __forceinline__ int sel4i(const int4& v, int k) {
+    return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
+}
+template <bool PRUNE>
+__device__ __forceinline__ bool visit_interior4(const Bvh& B, TravRay& R, float t_best, Stack& st,
+                                                uint32_t& cnt_nodes) {
+    const float4* n = B.nodes4 + 8 * (size_t)R.ref;
+    float4 lx = ldg4(n), ly = ldg4(n + 1), lz = ldg4(n + 2);
+    float4 hx = ldg4(n + 3), hy = ldg4(n + 4), hz = ldg4(n + 5);
+    int4 rf = ldg4i((const int4*)(n + 6)), ax = ldg4i((const int4*)(n + 7));
+    float lim = t_best + t_best * PRUNE_REL + PRUNE_ABS;
+    bool hit[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        float tn, tf;
+        bool h = slab(sel4(lx, k), sel4(ly, k), sel4(lz, k), sel4(hx, k), sel4(hy, k), sel4(hz, k), R.o, R.inv,
+                      R.nx, R.ny, R.nz, tn, tf);
+        hit[k] = PRUNE ? (h && !(tn > lim || tf < -PRUNE_BEHIND)) : (h && !(tf < -PRUNE_BEHIND));
+    }
+    cnt_nodes += (rf.x != -1 ? 1u : 0u) + (rf.y != -1 ? 1u : 0u) + (rf.z != -1 ? 1u : 0u) + (rf.w != -1 ? 1u : 0u);
+    bool negN = ((R.negm >> (uint32_t)ax.x) & 1u) != 0;
+    bool negA = ((R.negm >> (uint32_t)ax.y) & 1u) != 0;
+    bool negB = ((R.negm >> (uint32_t)ax.z) & 1u) != 0;
+    int s0 = negN ? 2 : 0, s1 = negN ? 0 : 2;
+    bool f0 = negN ? negB : negA, f1 = negN ? negA : negB;
+    int o0 = s0 + (f0 ? 1 : 0), o1 = s0 + (f0 ? 0 : 1), o2 = s1 + (f1 ? 1 : 0), o3 = s1 + (f1 ? 0 : 1);
+    bool h0 = o0 == 0 ? hit[0] : (o0 == 1 ? hit[1] : (o0 == 2 ? hit[2] : hit[3]));
+    bool h1 = o1 == 0 ? hit[0] : (o1 == 1 ? hit[1] : (o1 == 2 ? hit[2] : hit[3]));
+    bool h2 = o2 == 0 ? hit[0] : (o2 == 1 ? hit[1] : (o2 == 2 ? hit[2] : hit[3]));
+    bool h3 = o3 == 0 ? hit[0] : (o3 == 1 ? hit[1] : (o3 == 2 ? hit[2] : hit[3]));
+    int r0 = sel4i(rf, o0), r1 = sel4i(rf, o1), r2 = sel4i(rf, o2), r3 = sel4i(rf, o3);
+    if (h3 && (h0 || h1 || h2)) st.push(r3);
+    if (h2 && (h0 || h1)) st.push(r2);
+    if (h1 && h0) st.push(r1);
+    if (h0) { R.ref = r0; return true; }
+    if (h1) { R.ref = r1; return true; }
+    if (h2) { R.ref = r2; return true; }
+    if (h3) { R.ref = r3; return true; }
+    return false;
+}
+template <bool WIDE, bool PRUNE>
+__device__ __forceinline__ bool visit_node(const Bvh& B, TravRay& R, float t_best, Stack& st, uint32_t& cnt_nodes) {
+    return WIDE ? visit_interior4<PRUNE>(B, R, t_best, st, cnt_nodes)
+                : visit_interior<PRUNE>(B, R, t_best, st, cnt_nodes);
+}
+
 // the triangles of the leaf R.ref (closest-hit test of every one)
 __device__ __forceinline__ void leaf_closest(const Bvh& B, const float* __restrict__ tris, const TravRay& R,
                                              Hit& h, bool two_sided, uint32_t& cnt_tris) {
@@ -636,11 +692,12 @@ __device__ __forceinline__ bool leaf_any(const Bvh& B, const float* __restrict__
     return false;
 }
 
+template <bool WIDE = false>
 __device__ __forceinline__ bool closest_step(const Bvh& B, const float* __restrict__ tris, TravRay& R, Hit& h,
                                              bool two_sided, Stack& st, uint32_t& cnt_nodes,
                                              uint32_t& cnt_tris) {
     if (R.ref >= 0) {
-        if (visit_interior<true>(B, R, h.t, st, cnt_nodes)) return true;
+        if (visit_node<WIDE, true>(B, R, h.t, st, cnt_nodes)) return true;
     } else {
         leaf_closest(B, tris, R, h, two_sided, cnt_tris);
     }
@@ -668,10 +725,11 @@ __device__ __forceinline__ Hit closest_hit(const Bvh& B, const float* __restrict
 // intersect_shadows with tmin = K_HUGE (CRT/dTriangle.cu:215-259,
 // CRT/dEnvironmentLight.cu:60-64), one step.  Returns 0 while work remains,
 // 1 when a triangle with 0 <= t < 1e32 is found, 2 when the ray is unoccluded.
+template <bool WIDE = false>
 __device__ __forceinline__ int any_step(const Bvh& B, const float* __restrict__ tris, TravRay& R, Stack& st,
                                         uint32_t& cnt_nodes, uint32_t& cnt_tris) {
     if (R.ref >= 0) {
-        if (visit_interior<false>(B, R, 0.f, st, cnt_nodes)) return 0;
+        if (visit_node<WIDE, false>(B, R, 0.f, st, cnt_nodes)) return 0;
     } else if (leaf_any(B, tris, R, cnt_tris)) {
         return 1;
     }
@@ -761,42 +819,63 @@ __device__ __forceinline__ void block_alloc(uint32_t* counters, const int* qidx,
 }
 
 // Segmented work fetch of the persistent traversal: queue [0, n) is split
-// into FETCH_SEGS segments with a counter each; a wave draws from its home
-// segment and moves on when that is exhausted.  Exhaustion is first checked
-// with a plain load (counters only grow, so a stale value is conservative).
+// into FETCH_SEGS segments with a counter each; a wave draws from its current
+// segment and, once that is exhausted, finds the next segment with work left
+// by reading all FETCH_SEGS counters at once (lane k reads counter k with a
+// device-coherent load, one round trip).  It reports `done` only when such a
+// scan finds every segment exhausted.  Counters only grow, so a segment seen
+// exhausted stays exhausted and a wave fails at most FETCH_SEGS times.
 // Wave-uniform: every lane calls it with the same arguments.
+static_assert(FETCH_SEGS == 64, "one segment per lane of the scan");
 struct SegFetch {
     uint32_t* ctr;   // FETCH_SEGS counters, FETCH_STRIDE words apart
-    uint32_t n, seg_len, seg, tried;
+    uint32_t n, seg_len, seg;
     __device__ __forceinline__ void init(uint32_t* c, uint32_t n_, uint32_t home) {
         ctr = c;
         n = n_;
         seg_len = (n + FETCH_SEGS - 1) / FETCH_SEGS;
         seg = home % FETCH_SEGS;
-        tried = 0;
+    }
+    __device__ __forceinline__ uint32_t seg_size(uint32_t s) const {
+        uint32_t lo = s * seg_len;
+        return lo < n ? (n - lo < seg_len ? n - lo : seg_len) : 0u;
     }
     // Grants up to `want` items (want <= 64): returns the first item and the
-    // number granted (0 when every segment is exhausted: done = true).
+    // number granted; got == 0 with done == false means "try again later".
     __device__ __forceinline__ uint32_t take(uint32_t want, uint32_t& got, bool& done) {
         got = 0;
         done = false;
-        while (tried < FETCH_SEGS) {
-            uint32_t lo = seg * seg_len;
-            uint32_t len = lo < n ? (n - lo < seg_len ? n - lo : seg_len) : 0u;
-            uint32_t* c = ctr + (size_t)seg * FETCH_STRIDE;
-            if (len > 0 && __builtin_nontemporal_load(c) < len) {
+        for (int attempt = 0; attempt < 2; attempt++) {
+            uint32_t len = seg_size(seg);
+            if (len > 0) {
                 uint32_t base = 0;
-                if (__lane_id() == 0) base = atomicAdd(c, want);
+                if (__lane_id() == 0) base = atomicAdd(ctr + (size_t)seg * FETCH_STRIDE, want);
                 base = __shfl(base, 0);
                 if (base < len) {
                     got = len - base < want ? len - base : want;
-                    return lo + base;
+                    return seg * seg_len + base;
                 }
             }
-            seg = (seg + 1) % FETCH_SEGS;
-            tried++;
+            // current segment exhausted: scan every counter (lane k -> segment k)
+            uint32_t k = __lane_id();
+            bool avail = false;
+            if (k < FETCH_SEGS) {
+                uint32_t lk = seg_size(k);
+                uint32_t v = lk ? __hip_atomic_load(ctr + (size_t)k * FETCH_STRIDE, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT)
+                                : 0u;
+                avail = lk > 0 && v < lk;
+            }
+            uint64_t m = __ballot(avail);
+            if (m == 0) {
+                done = true;
+                return 0;
+            }
+            // next segment with work, searching upwards from the current one
+            uint32_t sh = (seg + 1) % 64u;
+            uint64_t rot = (m >> sh) | (sh ? (m << (64u - sh)) : 0ull);
+            seg = (sh + (uint32_t)(__ffsll((unsigned long long)rot) - 1)) % 64u;
         }
-        done = true;
         return 0;
     }
 };

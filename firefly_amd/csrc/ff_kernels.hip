@@ -27,6 +27,7 @@ __device__ __forceinline__ uint32_t gstride() { return gridDim.x * blockDim.x; }
 __device__ __forceinline__ Bvh bvh_of(const ff_dev_scene& S) {
     Bvh b;
     b.nodes = S.nodes;
+    b.nodes4 = S.nodes4;
     b.leaf_count = S.leaf_count;
     b.root = S.root;
     return b;

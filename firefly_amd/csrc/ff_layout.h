@@ -78,6 +78,7 @@ struct ff_tex_desc {
 
 struct ff_dev_scene {
     const float4* nodes;       // two-child records, 4 x float4 each
+    const float4* nodes4;      // 4-wide records, 8 x float4 each
     const int32_t* leaf_count; // per slot, for leaves of 15 or more primitives
     const float* root;         // root box lo.xyz hi.xyz, root ref
     const float* tris;
