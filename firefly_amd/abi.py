@@ -98,7 +98,7 @@ _PROTOS = {
     "ff_adjoint_batch": (C.c_int, [_P, C.POINTER(BatchItem), C.c_int32, C.c_int32, _P]),
     "ff_traversal_variant_count": (C.c_int32, []),
     "ff_traversal_variant_info": (C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
-                                            C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+                                            C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "ff_set_traversal_variant": (C.c_int, [_P, C.c_int32]),
     "ff_film_resolve": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_float, _P, _P, _P]),
 }
@@ -359,12 +359,12 @@ def adjoint_batch(ctx: Context, items, path_radiance, d_images, env_grads=None, 
 
 
 def traversal_variants():
-    """[(variant, block, lds_entries, refill, while_while)] of the traversal kernels."""
+    """[(variant, block, lds_entries, refill, while_while, bvh_width)] of the traversal kernels."""
     out = []
     for v in range(_lib().ff_traversal_variant_count()):
-        b, l, r, w = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
-        check(_lib().ff_traversal_variant_info(v, C.byref(b), C.byref(l), C.byref(r), C.byref(w)))
-        out.append((v, b.value, l.value, r.value, w.value))
+        b, l, r, w, k = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        check(_lib().ff_traversal_variant_info(v, C.byref(b), C.byref(l), C.byref(r), C.byref(w), C.byref(k)))
+        out.append((v, b.value, l.value, r.value, w.value, k.value))
     return out
 
 

@@ -42,14 +42,14 @@ def autotune(ctx: abi.Context, scene: abi.Scene, env: abi.EnvMap, cam: abi.Camer
     # warm-up (first launches pay code-object load and allocation)
     abi.render(ctx, scene, env, cam, width, height, 1, max_depth, seed, film)
     torch.cuda.synchronize()
-    for v, block, lds, refill, ww in abi.traversal_variants():
+    for v, block, lds, refill, ww, width_bvh in abi.traversal_variants():
         abi.set_traversal_variant(ctx, v)
         for k in batches:
             if k > probe_spp:
                 continue
             t = _time_render(ctx, scene, env, cam, width, height, probe_spp, max_depth, seed, film, k, reps)
             results.append({"variant": v, "block": block, "lds_entries": lds, "refill": refill,
-                            "while_while": ww, "spp_batch": k, "seconds": t})
+                            "while_while": ww, "bvh_width": width_bvh, "spp_batch": k, "seconds": t})
     best = min(results, key=lambda r: r["seconds"])
     abi.set_traversal_variant(ctx, best["variant"])
     out = {"best": best, "candidates": results,

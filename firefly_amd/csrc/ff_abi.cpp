@@ -174,8 +174,11 @@ ff_status ensure_paths(ff_context* c, size_t P) {
 }
 
 // traversal stack overflow area: entries beyond the smallest LDS stack per grid thread
+// Stack entries a traversal can hold: one per binary level for the two-child
+// records; up to three per 4-wide record (two binary levels) for the others.
 ff_status ensure_ovf(ff_context* c, int32_t max_depth) {
-    size_t need = max_depth > FF_LDS_MIN ? (size_t)(max_depth - FF_LDS_MIN + 1) : 1;
+    int32_t entries = std::max(max_depth, 3 * ((max_depth + 1) / 2));
+    size_t need = entries > FF_LDS_MIN ? (size_t)(entries - FF_LDS_MIN + 1) : 1;
     if (c->ovf && c->ovf_entries >= need) return FF_OK;
     dfree(c->ovf);
     HIP_TRY(dalloc(&c->ovf, need * (size_t)FF_TRAV_THREADS));
@@ -605,7 +608,7 @@ ff_status ff_trace_closest(ff_scene* s, const float* rays, uint32_t n, float* hi
     if (st != FF_OK) return st;
     HIP_TRY(ffk::launch_trace_closest(s->view(), (const float4*)rays, n, (float4*)hits, two_sided, s->ctx->fetch,
                                       s->ctx->ovf,
-                                      (unsigned long long*)counts, (hipStream_t)stream));
+                                      (unsigned long long*)counts, s->ctx->trav_variant, (hipStream_t)stream));
     return FF_OK;
 }
 
@@ -617,7 +620,7 @@ ff_status ff_trace_any(ff_scene* s, const float* rays, uint32_t n, uint8_t* occl
     ff_status st = ensure_ovf(s->ctx, s->max_depth);
     if (st != FF_OK) return st;
     HIP_TRY(ffk::launch_trace_any(s->view(), (const float4*)rays, n, occluded, s->ctx->fetch, s->ctx->ovf,
-                                  (unsigned long long*)counts, (hipStream_t)stream));
+                                  (unsigned long long*)counts, s->ctx->trav_variant, (hipStream_t)stream));
     return FF_OK;
 }
 
@@ -1026,13 +1029,14 @@ ff_status ff_set_traversal_variant(ff_context* ctx, int32_t variant) {
 }
 
 ff_status ff_traversal_variant_info(int32_t variant, int32_t* block, int32_t* lds_entries, int32_t* refill,
-                                    int32_t* while_while) {
+                                    int32_t* while_while, int32_t* bvh_width) {
     if (variant < 0 || variant >= ffk::kTravVariants)
         return fail(FF_ERR_INVALID, "ff_traversal_variant_info: variant %d of %d", variant, ffk::kTravVariants);
     if (block) *block = ffk::kTravBlock[variant];
     if (lds_entries) *lds_entries = ffk::kTravLds[variant];
     if (refill) *refill = ffk::kTravRefill[variant];
     if (while_while) *while_while = ffk::kTravWhileWhile[variant];
+    if (bvh_width) *bvh_width = ffk::kTravWidth[variant];
     return FF_OK;
 }
 

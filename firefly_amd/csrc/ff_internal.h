@@ -15,6 +15,7 @@ extern const int kTravLds[];
 extern const int kTravBlock[];
 extern const int kTravRefill[];
 extern const int kTravWhileWhile[];
+extern const int kTravWidth[];  // BVH record width of the variant (2 or 4)
 // computes the grid sizes of a traversal variant (call before a graph capture)
 hipError_t prepare_traversal(int variant);
 hipError_t launch_batch_begin(const ff_paths& P, int set, uint32_t value, hipStream_t s);
@@ -35,9 +36,10 @@ hipError_t launch_fill_u64(unsigned long long* p, uint32_t n, hipStream_t s);
 hipError_t launch_resolve_layers(const float4* layers, uint32_t K, uint32_t WH, float4* film, hipStream_t s);
 hipError_t launch_trace_closest(const ff_dev_scene& S, const float4* rays, uint32_t n,
                                 float4* hits, int two_sided, uint32_t* fetch, int32_t* ovf,
-                                unsigned long long* counts, hipStream_t s);
+                                unsigned long long* counts, int variant, hipStream_t s);
 hipError_t launch_trace_any(const ff_dev_scene& S, const float4* rays, uint32_t n, uint8_t* occ,
-                            uint32_t* fetch, int32_t* ovf, unsigned long long* counts, hipStream_t s);
+                            uint32_t* fetch, int32_t* ovf, unsigned long long* counts, int variant,
+                            hipStream_t s);
 hipError_t launch_gen_rays(const ff_dev_camera& cam, int W, int H, uint32_t seed,
                            uint32_t sample, float4* rays, hipStream_t s);
 hipError_t launch_env_build(const float* rgb, int w, int h, float* lum, double* row_sum,

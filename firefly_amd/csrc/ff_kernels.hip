@@ -187,7 +187,9 @@ struct RaySrc {  // a plain ray array (ff_trace_closest / ff_trace_any)
 // lane per iteration; true = "while-while": lanes at interior records step
 // until every busy lane has reached a leaf, then all leaves are tested
 // together (fewer divergent switches between the two code paths).
-template <int BS, int LDS, int REFILL, bool ANY, bool WW, class Src>
+// WIDE walks the 4-wide records (half the dependent interior steps, same
+// leaf order) instead of the two-child records.
+template <int BS, int LDS, int REFILL, bool ANY, bool WW, bool WIDE, class Src>
 __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint32_t* fetch, int32_t* ovf,
                                                 unsigned long long* cnt, int* smem) {
     SegFetch F;
@@ -201,7 +203,7 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
     uint32_t cn = 0, ct = 0, id = 0;
     bool busy = false, more = true, two = false;
     Bvh B;
-    B.nodes = nullptr; B.leaf_count = nullptr; B.root = nullptr;
+    B.nodes = nullptr; B.nodes4 = nullptr; B.leaf_count = nullptr; B.root = nullptr;
     const float* tris = nullptr;
     TravRay R;
     Hit h = no_hit();
@@ -234,7 +236,7 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
             for (;;) {  // interior records until every busy lane is at a leaf
                 bool at_int = busy && R.ref >= 0;
                 if (__ballot(at_int) == 0) break;
-                if (at_int && !visit_interior<!ANY>(B, R, h.t, st, cn)) {
+                if (at_int && !visit_node<WIDE, !ANY>(B, R, h.t, st, cn)) {
                     if (st.sp == 0) {
                         if (ANY) src.occluded(id, false);
                         else src.hit(id, h);
@@ -260,12 +262,12 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
             }
         } else if (busy) {
             if (ANY) {
-                int r = any_step(B, tris, R, st, cn, ct);
+                int r = any_step<WIDE>(B, tris, R, st, cn, ct);
                 if (r != 0) {
                     src.occluded(id, r == 1);
                     busy = false;
                 }
-            } else if (!closest_step(B, tris, R, h, two, st, cn, ct)) {
+            } else if (!closest_step<WIDE>(B, tris, R, h, two, st, cn, ct)) {
                 src.hit(id, h);
                 busy = false;
             }
@@ -277,19 +279,19 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
     }
 }
 
-template <int BS, int LDS, int RF, bool WW>
+template <int BS, int LDS, int RF, bool WW, bool WIDE>
 __global__ __launch_bounds__(BS) void k_extend(ff_paths P, int q) {
     __shared__ int smem[LDS * BS];
     ExtSrc src{P, P.q_ext[q]};
-    trav_persistent<BS, LDS, RF, false, WW>(src, P.counters[q == 0 ? C_EXT0 : C_EXT1],
+    trav_persistent<BS, LDS, RF, false, WW, WIDE>(src, P.counters[q == 0 ? C_EXT0 : C_EXT1],
                                             P.fetch + F_EXT * FETCH_SEGS * FETCH_STRIDE, P.ovf,
                                             stats_rep(P) + S_EXT_NODES, smem);
 }
-template <int BS, int LDS, int RF, bool WW>
+template <int BS, int LDS, int RF, bool WW, bool WIDE>
 __global__ __launch_bounds__(BS) void k_shadow(ff_paths P) {
     __shared__ int smem[LDS * BS];
     ShadowSrc src{P};
-    trav_persistent<BS, LDS, RF, true, WW>(src, P.counters[C_SHADOW], P.fetch + F_SHD * FETCH_SEGS * FETCH_STRIDE,
+    trav_persistent<BS, LDS, RF, true, WW, WIDE>(src, P.counters[C_SHADOW], P.fetch + F_SHD * FETCH_SEGS * FETCH_STRIDE,
                                            P.ovf, stats_rep(P) + S_SHD_NODES, smem);
 }
 
@@ -957,20 +959,28 @@ __global__ void k_resolve_layers(const float4* layers, uint32_t K, uint32_t WH, 
 }
 
 // ======================================================== standalone stages
-template <int BS, int LDS, int RF>
-__global__ __launch_bounds__(BS) void k_trace_closest(ff_dev_scene S, const float4* rays, uint32_t n,
-                                                      float4* hits, int two_sided, uint32_t* fetch, int32_t* ovf,
-                                                      unsigned long long* counts) {
+struct TraceArgs {  // one ff_trace_closest / ff_trace_any call
+    ff_dev_scene S;
+    const float4* rays;
+    uint32_t n;
+    float4* hits;
+    uint8_t* occ;
+    int two_sided;
+    uint32_t* fetch;
+    int32_t* ovf;
+    unsigned long long* counts;
+};
+template <int BS, int LDS, int RF, bool WW, bool WIDE>
+__global__ __launch_bounds__(BS) void k_trace_closest(TraceArgs a) {
     __shared__ int smem[LDS * BS];
-    RaySrc src{S, rays, hits, nullptr, two_sided};
-    trav_persistent<BS, LDS, RF, false, false>(src, n, fetch, ovf, counts, smem);
+    RaySrc src{a.S, a.rays, a.hits, nullptr, a.two_sided};
+    trav_persistent<BS, LDS, RF, false, WW, WIDE>(src, a.n, a.fetch, a.ovf, a.counts, smem);
 }
-template <int BS, int LDS, int RF>
-__global__ __launch_bounds__(BS) void k_trace_any(ff_dev_scene S, const float4* rays, uint32_t n, uint8_t* occ,
-                                                  uint32_t* fetch, int32_t* ovf, unsigned long long* counts) {
+template <int BS, int LDS, int RF, bool WW, bool WIDE>
+__global__ __launch_bounds__(BS) void k_trace_any(TraceArgs a) {
     __shared__ int smem[LDS * BS];
-    RaySrc src{S, rays, nullptr, occ, 0};
-    trav_persistent<BS, LDS, RF, true, false>(src, n, fetch, ovf, counts, smem);
+    RaySrc src{a.S, a.rays, nullptr, a.occ, 0};
+    trav_persistent<BS, LDS, RF, true, WW, WIDE>(src, a.n, a.fetch, a.ovf, a.counts, smem);
 }
 __global__ void k_gen_rays(ff_dev_camera cam, int32_t W, int32_t H, uint32_t seed, uint32_t sample,
                            float4* rays) {
@@ -1138,40 +1148,53 @@ uint32_t resident_grid(K kernel, int bs) {
     return (uint32_t)g;
 }
 
-// traversal variants (block size, LDS stack entries, refill threshold, loop shape)
-#define FF_TRAV_VARIANT_LIST(X) \
-    X(0, 256, 16, 32, false)    \
-    X(1, 256, 16, 16, false)    \
-    X(2, 256, 16, 64, false)    \
-    X(3, 128, 16, 32, false)    \
-    X(4, 256, 8, 32, false)     \
-    X(5, 256, 24, 32, false)    \
-    X(6, 64, 32, 32, false)     \
-    X(7, 128, 24, 32, false)    \
-    X(8, 256, 16, 32, true)     \
-    X(9, 64, 32, 32, true)      \
-    X(10, 256, 16, 16, true)    \
-    X(11, 128, 24, 32, true)
+// traversal variants (block size, LDS stack entries, refill threshold, loop
+// shape, BVH record width)
+#define FF_TRAV_VARIANT_LIST(X)      \
+    X(0, 256, 16, 32, false, false)  \
+    X(1, 256, 16, 16, false, false)  \
+    X(2, 256, 16, 64, false, false)  \
+    X(3, 128, 16, 32, false, false)  \
+    X(4, 256, 8, 32, false, false)   \
+    X(5, 256, 24, 32, false, false)  \
+    X(6, 64, 32, 32, false, false)   \
+    X(7, 128, 24, 32, false, false)  \
+    X(8, 256, 16, 32, true, false)   \
+    X(9, 64, 32, 32, true, false)    \
+    X(10, 256, 16, 16, true, false)  \
+    X(11, 128, 24, 32, true, false)  \
+    X(12, 256, 16, 64, false, true)  \
+    X(13, 256, 16, 32, false, true)  \
+    X(14, 64, 32, 32, false, true)   \
+    X(15, 256, 16, 32, true, true)
 
-template <int BS, int LDS, int RF, bool WW>
-hipError_t trav_launch(int which, const ff_paths& P, int q, hipStream_t s) {
+// which: 0 extend, 1 shadow, 2 grid sizes only (before a graph capture),
+// 3 ff_trace_closest, 4 ff_trace_any
+template <int BS, int LDS, int RF, bool WW, bool WIDE>
+hipError_t trav_launch(int which, const ff_paths* P, int q, const TraceArgs* T, hipStream_t s) {
     if (which == 0) {
-        uint32_t g = resident_grid(k_extend<BS, LDS, RF, WW>, BS);
-        hipLaunchKernelGGL((k_extend<BS, LDS, RF, WW>), dim3(g), dim3(BS), 0, s, P, q);
+        uint32_t g = resident_grid(k_extend<BS, LDS, RF, WW, WIDE>, BS);
+        hipLaunchKernelGGL((k_extend<BS, LDS, RF, WW, WIDE>), dim3(g), dim3(BS), 0, s, *P, q);
     } else if (which == 1) {
-        uint32_t g = resident_grid(k_shadow<BS, LDS, RF, WW>, BS);
-        hipLaunchKernelGGL((k_shadow<BS, LDS, RF, WW>), dim3(g), dim3(BS), 0, s, P);
-    } else {  // grid sizes only (before a graph capture)
-        (void)resident_grid(k_extend<BS, LDS, RF, WW>, BS);
-        (void)resident_grid(k_shadow<BS, LDS, RF, WW>, BS);
+        uint32_t g = resident_grid(k_shadow<BS, LDS, RF, WW, WIDE>, BS);
+        hipLaunchKernelGGL((k_shadow<BS, LDS, RF, WW, WIDE>), dim3(g), dim3(BS), 0, s, *P);
+    } else if (which == 3) {
+        uint32_t g = resident_grid(k_trace_closest<BS, LDS, RF, WW, WIDE>, BS);
+        hipLaunchKernelGGL((k_trace_closest<BS, LDS, RF, WW, WIDE>), dim3(g), dim3(BS), 0, s, *T);
+    } else if (which == 4) {
+        uint32_t g = resident_grid(k_trace_any<BS, LDS, RF, WW, WIDE>, BS);
+        hipLaunchKernelGGL((k_trace_any<BS, LDS, RF, WW, WIDE>), dim3(g), dim3(BS), 0, s, *T);
+    } else {
+        (void)resident_grid(k_extend<BS, LDS, RF, WW, WIDE>, BS);
+        (void)resident_grid(k_shadow<BS, LDS, RF, WW, WIDE>, BS);
     }
     return hipGetLastError();
 }
 
-hipError_t trav_dispatch(int which, int variant, const ff_paths& P, int q, hipStream_t s) {
+hipError_t trav_dispatch(int which, int variant, const ff_paths* P, int q, const TraceArgs* T, hipStream_t s) {
     switch (variant) {
-#define FF_TRAV_CASE(v, bs, lds, rf, ww) \
-    case v: return trav_launch<bs, lds, rf, ww>(which, P, q, s);
+#define FF_TRAV_CASE(v, bs, lds, rf, ww, wide) \
+    case v: return trav_launch<bs, lds, rf, ww, wide>(which, P, q, T, s);
         FF_TRAV_VARIANT_LIST(FF_TRAV_CASE)
 #undef FF_TRAV_CASE
         default: return hipErrorInvalidValue;
@@ -1183,19 +1206,24 @@ hipError_t trav_dispatch(int which, int variant, const ff_paths& P, int q, hipSt
 // ===================================================================== host
 namespace ffk {
 
-const int kTravVariants = 12;
-#define FF_TRAV_BS(v, bs, lds, rf, ww) bs,
-#define FF_TRAV_LDS(v, bs, lds, rf, ww) lds,
-#define FF_TRAV_RF(v, bs, lds, rf, ww) rf,
-#define FF_TRAV_WW(v, bs, lds, rf, ww) (ww ? 1 : 0),
-const int kTravBlock[kTravVariants] = {FF_TRAV_VARIANT_LIST(FF_TRAV_BS)};
-const int kTravLds[kTravVariants] = {FF_TRAV_VARIANT_LIST(FF_TRAV_LDS)};
-const int kTravRefill[kTravVariants] = {FF_TRAV_VARIANT_LIST(FF_TRAV_RF)};
-const int kTravWhileWhile[kTravVariants] = {FF_TRAV_VARIANT_LIST(FF_TRAV_WW)};
+#define FF_TRAV_ONE(v, bs, lds, rf, ww, wide) +1
+const int kTravVariants = 0 FF_TRAV_VARIANT_LIST(FF_TRAV_ONE);
+#undef FF_TRAV_ONE
+#define FF_TRAV_BS(v, bs, lds, rf, ww, wide) bs,
+#define FF_TRAV_LDS(v, bs, lds, rf, ww, wide) lds,
+#define FF_TRAV_RF(v, bs, lds, rf, ww, wide) rf,
+#define FF_TRAV_WW(v, bs, lds, rf, ww, wide) (ww ? 1 : 0),
+#define FF_TRAV_WIDTH(v, bs, lds, rf, ww, wide) (wide ? 4 : 2),
+const int kTravBlock[] = {FF_TRAV_VARIANT_LIST(FF_TRAV_BS)};
+const int kTravLds[] = {FF_TRAV_VARIANT_LIST(FF_TRAV_LDS)};
+const int kTravRefill[] = {FF_TRAV_VARIANT_LIST(FF_TRAV_RF)};
+const int kTravWhileWhile[] = {FF_TRAV_VARIANT_LIST(FF_TRAV_WW)};
+const int kTravWidth[] = {FF_TRAV_VARIANT_LIST(FF_TRAV_WIDTH)};
 #undef FF_TRAV_BS
 #undef FF_TRAV_LDS
 #undef FF_TRAV_RF
 #undef FF_TRAV_WW
+#undef FF_TRAV_WIDTH
 
 hipError_t launch_batch_begin(const ff_paths& P, int set, uint32_t value, hipStream_t s) {
     hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(256), 0, s, P, set, value);
@@ -1206,13 +1234,12 @@ hipError_t launch_generate(const ff_paths& P, hipStream_t s) {
     return hipGetLastError();
 }
 hipError_t launch_extend(const ff_paths& P, int q, int variant, hipStream_t s) {
-    return trav_dispatch(0, variant, P, q, s);
+    return trav_dispatch(0, variant, &P, q, nullptr, s);
 }
-hipError_t launch_shadow(const ff_paths& P, int variant, hipStream_t s) { return trav_dispatch(1, variant, P, 0, s); }
-hipError_t prepare_traversal(int variant) {
-    ff_paths none{};
-    return trav_dispatch(2, variant, none, 0, nullptr);
+hipError_t launch_shadow(const ff_paths& P, int variant, hipStream_t s) {
+    return trav_dispatch(1, variant, &P, 0, nullptr, s);
 }
+hipError_t prepare_traversal(int variant) { return trav_dispatch(2, variant, nullptr, 0, nullptr, nullptr); }
 hipError_t launch_reset(uint32_t* counters, uint32_t mask, uint32_t* fetch, uint32_t fetch_mask, hipStream_t s) {
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(256), 0, s, counters, mask, fetch, fetch_mask);
     return hipGetLastError();
@@ -1279,20 +1306,16 @@ hipError_t launch_resolve_layers(const float4* layers, uint32_t K, uint32_t WH, 
 }
 hipError_t launch_trace_closest(const ff_dev_scene& S, const float4* rays, uint32_t n, float4* hits,
                                 int two_sided, uint32_t* fetch, int32_t* ovf, unsigned long long* counts,
-                                hipStream_t s) {
+                                int variant, hipStream_t s) {
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(256), 0, s, nullptr, 0u, fetch, 1u << F_TRACE);
-    uint32_t g = resident_grid(k_trace_closest<256, 16, 32>, 256);
-    hipLaunchKernelGGL((k_trace_closest<256, 16, 32>), dim3(g), dim3(256), 0, s, S, rays, n, hits, two_sided,
-                       fetch + F_TRACE * FETCH_SEGS * FETCH_STRIDE, ovf, counts);
-    return hipGetLastError();
+    TraceArgs T{S, rays, n, hits, nullptr, two_sided, fetch + F_TRACE * FETCH_SEGS * FETCH_STRIDE, ovf, counts};
+    return trav_dispatch(3, variant, nullptr, 0, &T, s);
 }
 hipError_t launch_trace_any(const ff_dev_scene& S, const float4* rays, uint32_t n, uint8_t* occ, uint32_t* fetch,
-                            int32_t* ovf, unsigned long long* counts, hipStream_t s) {
+                            int32_t* ovf, unsigned long long* counts, int variant, hipStream_t s) {
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(256), 0, s, nullptr, 0u, fetch, 1u << F_TRACE);
-    uint32_t g = resident_grid(k_trace_any<256, 16, 32>, 256);
-    hipLaunchKernelGGL((k_trace_any<256, 16, 32>), dim3(g), dim3(256), 0, s, S, rays, n, occ,
-                       fetch + F_TRACE * FETCH_SEGS * FETCH_STRIDE, ovf, counts);
-    return hipGetLastError();
+    TraceArgs T{S, rays, n, nullptr, occ, 0, fetch + F_TRACE * FETCH_SEGS * FETCH_STRIDE, ovf, counts};
+    return trav_dispatch(4, variant, nullptr, 0, &T, s);
 }
 hipError_t launch_gen_rays(const ff_dev_camera& cam, int W, int H, uint32_t seed, uint32_t sample, float4* rays,
                            hipStream_t s) {

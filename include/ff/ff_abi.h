@@ -235,14 +235,18 @@ ff_status ff_adjoint_batch(ff_context *ctx, const ff_batch_item *items, int32_t 
 
 /* ---- traversal kernel variants (workgroup size, LDS stack entries, the
  * number of idle lanes at which a wave of the persistent traversal fetches
- * new rays, and the loop shape: one step per iteration or while-while),
- * chosen per context by the autotuner (firefly_amd/autotune.py);
- * variant 0 is the default.  No reference counterpart (the reference has one
- * fixed shape).  Renders and traces on one context share its work counters
- * and buffers: they must not overlap on different streams. */
+ * new rays, the loop shape: one step per iteration or while-while, and the
+ * BVH record width: 2 = the reference's binary nodes two children at a time,
+ * 4 = the same tree collapsed to 4-wide records, visited in the same leaf
+ * order so hits are identical), chosen per context by the autotuner
+ * (firefly_amd/autotune.py); variant 0 is the default.  The context's variant
+ * applies to its renders and to ff_trace_closest / ff_trace_any.  No
+ * reference counterpart (the reference has one fixed shape).  Renders and
+ * traces on one context share its work counters and buffers: they must not
+ * overlap on different streams. */
 int32_t ff_traversal_variant_count(void);
 ff_status ff_traversal_variant_info(int32_t variant, int32_t *block, int32_t *lds_entries,
-                                    int32_t *refill, int32_t *while_while);
+                                    int32_t *refill, int32_t *while_while, int32_t *bvh_width);
 ff_status ff_set_traversal_variant(ff_context *ctx, int32_t variant);
 
 /* ray counts of the last ff_render on this context (synchronises its stream). */

@@ -592,24 +592,107 @@ static hitrec wide_walk(const or_node *nodes, const float *tris, v3 o, v3 d, int
     return h;
 }
 
+/* The same tree walked four grandchildren at a time (the GPU's 4-wide
+ * records): at an interior node the boxes of its children's children are
+ * tested together (a child that is a leaf stands for itself), and the hit
+ * ones are visited in the binary walk's order: the side by the node's split
+ * axis, then within each side by that child's axis.  The boxes of the two
+ * children themselves are not tested.  Leaves are reached in the binary
+ * walk's order, so hits are the same; the counts differ. */
+static hitrec quad_walk(const or_node *nodes, const float *tris, v3 o, v3 d, int any_hit, int prune,
+                        int64_t *cn, int64_t *ct) {
+    hitrec h;
+    h.found = 0; h.t = K_HUGE; h.u = h.v = 0.f; h.id = -1;
+    float of[3] = {o.x, o.y, o.z};
+    float inv[3];
+    if (any_hit) { inv[0] = 1 / d.x; inv[1] = 1 / d.y; inv[2] = 1 / d.z; }
+    else { inv[0] = 1.f / d.x; inv[1] = 1.f / d.y; inv[2] = 1.f / d.z; }
+    int ng[3] = {inv[0] < 0, inv[1] < 0, inv[2] < 0};
+    float tn, tf;
+    (*cn)++;
+    if (!slab(&nodes[0], of, inv, ng, &tn, &tf) || (prune && tf < -PRUNE_BEHIND)) return h;
+    int32_t stack[3 * OR_STACK];
+    int sp = 0;
+    int32_t cur = 0;
+    for (;;) {
+        const or_node *nd = &nodes[cur];
+        if (nd->n_prims == 0) {
+            int32_t kids[2] = {cur + 1, nd->offset};
+            int32_t order[4];
+            int n_ord = 0;
+            float lim = h.t + h.t * PRUNE_REL + PRUNE_ABS;
+            int first = ng[nd->axis] ? 1 : 0;
+            for (int k = 0; k < 2; k++) {
+                int32_t c = kids[k ^ first];
+                int32_t sl[2];
+                int ns = 1;
+                if (nodes[c].n_prims == 0) {
+                    int flip = ng[nodes[c].axis];
+                    sl[0] = flip ? nodes[c].offset : c + 1;
+                    sl[1] = flip ? c + 1 : nodes[c].offset;
+                    ns = 2;
+                } else {
+                    sl[0] = c;
+                }
+                for (int j = 0; j < ns; j++) {
+                    float t0, t1;
+                    (*cn)++;
+                    int hit = slab(&nodes[sl[j]], of, inv, ng, &t0, &t1);
+                    if (hit && prune && ((!any_hit && t0 > lim) || t1 < -PRUNE_BEHIND)) hit = 0;
+                    if (hit) order[n_ord++] = sl[j];
+                }
+            }
+            if (n_ord > 0) {
+                if (sp + n_ord - 1 > 3 * OR_STACK) abort();
+                for (int k = n_ord - 1; k >= 1; k--) stack[sp++] = order[k];
+                cur = order[0];
+                continue;
+            }
+        } else {
+            for (int i = 0; i < nd->n_prims; i++) {
+                int32_t slot = nd->offset + i;
+                float t = 0.f, u = 0.f, v = 0.f;
+                (*ct)++;
+                if (any_hit) {
+                    if (mt_shadow(tris + (size_t)slot * 9, o, d, &t) && t < K_HUGE) {
+                        h.found = 1; h.t = t; h.id = slot;
+                        return h;
+                    }
+                } else if (mt_cull(tris + (size_t)slot * 9, o, d, &t, &u, &v) && !(t < 0) && t < h.t) {
+                    h.found = 1; h.t = t; h.u = u; h.v = v; h.id = slot;
+                }
+            }
+        }
+        if (sp == 0) break;
+        cur = stack[--sp];
+    }
+    return h;
+}
+
+static hitrec walk_width(const or_node *nodes, const float *tris, v3 o, v3 d, int any_hit, int prune,
+                         int width, int64_t *cn, int64_t *ct) {
+    return width == 4 ? quad_walk(nodes, tris, o, d, any_hit, prune, cn, ct)
+                      : wide_walk(nodes, tris, o, d, any_hit, prune, cn, ct);
+}
+
 void or_count_traversal(const or_node *nodes, const float *tris, const float *rays, int32_t n,
-                        int32_t any_hit, int32_t prune, int64_t *nodes_tested,
+                        int32_t any_hit, int32_t prune, int32_t width, int64_t *nodes_tested,
                         int64_t *tris_tested) {
     int64_t cn = 0, ct = 0;
     for (int32_t i = 0; i < n; i++) {
         const float *r = rays + (size_t)i * 6;
-        wide_walk(nodes, tris, vld(r), vld(r + 3), any_hit, prune, &cn, &ct);
+        walk_width(nodes, tris, vld(r), vld(r + 3), any_hit, prune, width, &cn, &ct);
     }
     *nodes_tested = cn;
     *tris_tested = ct;
 }
 
 void or_trace_walk(const or_node *nodes, const float *tris, const float *rays, int32_t n,
-                   int32_t any_hit, int32_t prune, float *t, int32_t *id) {
+                   int32_t any_hit, int32_t prune, int32_t width, float *t, int32_t *id) {
     int64_t cn = 0, ct = 0;
     for (int32_t i = 0; i < n; i++) {
         const float *r = rays + (size_t)i * 6;
-        hitrec h = wide_walk(nodes, tris, vld(r), vld(r + 3), any_hit, prune, &cn, &ct);
+        hitrec h = walk_width(nodes, tris, vld(r), vld(r + 3), any_hit, prune, width, &cn, &ct);
         t[i] = h.t;
         id[i] = h.id;
     }

@@ -95,11 +95,13 @@ void or_trace_any(const or_node *nodes, const float *tris, const float *rays,
                   int32_t n, uint8_t *occluded);
 void or_trace_any_brute(int32_t n_tris, const float *tris, const float *rays,
                         int32_t n, uint8_t *occluded);
+/* width 2: two child boxes per interior node; width 4: the four grandchild
+ * boxes per interior node (the 4-wide records) */
 void or_count_traversal(const or_node *nodes, const float *tris, const float *rays,
-                        int32_t n, int32_t any_hit, int32_t prune,
+                        int32_t n, int32_t any_hit, int32_t prune, int32_t width,
                         int64_t *nodes_tested, int64_t *tris_tested);
 void or_trace_walk(const or_node *nodes, const float *tris, const float *rays, int32_t n,
-                   int32_t any_hit, int32_t prune, float *t, int32_t *id);
+                   int32_t any_hit, int32_t prune, int32_t width, float *t, int32_t *id);
 
 /* ---- environment CDF (FP64) and sampling ---- */
 void or_env_build(const float *rgb, int32_t w, int32_t h, double *marginal_y,

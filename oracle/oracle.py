@@ -59,8 +59,8 @@ def lib() -> C.CDLL:
             "or_trace_closest_brute": (None, [C.c_int32, _P, _P, C.c_int32, C.c_int32, _P, _P]),
             "or_trace_any": (None, [_P, _P, _P, C.c_int32, _P]),
             "or_trace_any_brute": (None, [C.c_int32, _P, _P, C.c_int32, _P]),
-            "or_count_traversal": (None, [_P, _P, _P, C.c_int32, C.c_int32, C.c_int32, _P, _P]),
-            "or_trace_walk": (None, [_P, _P, _P, C.c_int32, C.c_int32, C.c_int32, _P, _P]),
+            "or_count_traversal": (None, [_P, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P]),
+            "or_trace_walk": (None, [_P, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P]),
             "or_env_build": (None, [_P, C.c_int32, C.c_int32, _P, _P, _P, _P, _P]),
             "or_upper_bound": (C.c_int32, [_P, C.c_int32, C.c_float]),
             "or_env_sample_dir": (None, [_P, _P, C.c_int32, C.c_int32, C.c_float, C.c_float, _P, _P, _P]),
@@ -154,23 +154,25 @@ def trace_any_brute(tris, rays):
     return o
 
 
-def count_traversal(nodes, tris, rays, any_hit=False, prune=True):
+def count_traversal(nodes, tris, rays, any_hit=False, prune=True, width=2):
+    """(boxes, triangles) tested over all rays by the GPU's visit scheme for
+    BVH records of `width` (2: child pairs, 4: grandchild quads)."""
     r = _rays6(rays)
     cn = np.zeros(1, np.int64)
     ct = np.zeros(1, np.int64)
     lib().or_count_traversal(nodes.ctypes.data, tris.ctypes.data, r.ctypes.data, r.shape[0],
-                             int(any_hit), int(prune), cn.ctypes.data, ct.ctypes.data)
+                             int(any_hit), int(prune), int(width), cn.ctypes.data, ct.ctypes.data)
     return int(cn[0]), int(ct[0])
 
 
-def trace_walk(nodes, tris, rays, any_hit=False, prune=True):
-    """Closest/any hit by the two-child visit scheme the GPU uses (with its
-    pruning); returns (t, slot)."""
+def trace_walk(nodes, tris, rays, any_hit=False, prune=True, width=2):
+    """Closest/any hit by the visit scheme the GPU uses for BVH records of
+    `width` (with its pruning); returns (t, slot)."""
     r = _rays6(rays)
     t = np.zeros(r.shape[0], np.float32)
     i = np.zeros(r.shape[0], np.int32)
     lib().or_trace_walk(nodes.ctypes.data, tris.ctypes.data, r.ctypes.data, r.shape[0], int(any_hit),
-                        int(prune), t.ctypes.data, i.ctypes.data)
+                        int(prune), int(width), t.ctypes.data, i.ctypes.data)
     return t, i
 
 

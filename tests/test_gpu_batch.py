@@ -75,21 +75,23 @@ def test_graph_render_equals_plain_render(gpu):
 
 
 def test_traversal_variants_agree(gpu):
-    """Every (workgroup, LDS stack, refill) traversal variant gives the same image."""
+    """Every traversal variant (workgroup, LDS stack, refill, loop shape, BVH
+    record width) gives the same image."""
     import torch
     sd = scenes.spheres_scene(width=48, height=32, spp=2, max_depth=3)
     sc = abi.Scene(gpu, sd.positions, sd.normals, sd.uvs, sd.material_ids, sd.materials)
     env = abi.EnvMap(gpu, sd.env)
     cam = abi.make_camera(sd.inv_vp)
     ref = None
-    for v, block, lds, refill, ww in abi.traversal_variants():
+    for v, block, lds, refill, ww, width in abi.traversal_variants():
         abi.set_traversal_variant(gpu, v)
         f = torch.zeros((sd.height, sd.width, 4), dtype=torch.float32, device="cuda")
         abi.render(gpu, sc, env, cam, sd.width, sd.height, 2, 3, 0, f)
         torch.cuda.synchronize()
         if ref is None:
             ref = f
-        assert torch.equal(f, ref), f"variant {v} (block {block}, lds {lds}, refill {refill}, while-while {ww})"
+        assert torch.equal(f, ref), (f"variant {v} (block {block}, lds {lds}, refill {refill}, while-while {ww}, "
+                                    f"width {width})")
     abi.set_traversal_variant(gpu, 0)
 
 

@@ -8,6 +8,26 @@ from firefly_amd import abi, scenes
 pytestmark = pytest.mark.gpu
 
 
+def _variant_of(kind):
+    """First traversal variant of a kind: two-child records, 4-wide records,
+    4-wide records with the while-while loop."""
+    for v, _, _, _, ww, width in abi.traversal_variants():
+        if (kind == "binary" and width == 2 and not ww) or (kind == "wide" and width == 4 and not ww) or \
+                (kind == "wide_ww" and width == 4 and ww):
+            return v, width
+    raise AssertionError(f"no {kind} traversal variant")
+
+
+@pytest.fixture(params=["binary", "wide", "wide_ww"])
+def width(request, gpu):
+    """Runs the test with the context's traversal variant set to each BVH
+    record format; yields the record width."""
+    v, w = _variant_of(request.param)
+    abi.set_traversal_variant(gpu, v)
+    yield w
+    abi.set_traversal_variant(gpu, 0)
+
+
 @pytest.fixture(scope="module")
 def cfg0(gpu, oracle):
     sd = scenes.spheres_scene()
@@ -88,21 +108,21 @@ def test_primary_hits_bit_exact(cfg0, gpu, oracle):
 
 
 @pytest.mark.parametrize("seed", [1, 2])
-def test_incoherent_hits_bit_exact(cfg0, oracle, seed):
+def test_incoherent_hits_bit_exact(cfg0, oracle, seed, width):
     sd, sc, ps = cfg0
     r = scenes.random_rays(65536, seed)
     h = _closest(sc, r)
     _assert_hits_equal(h, oracle.trace_closest(ps.nodes, ps.tris_ordered(), r), f"incoherent{seed}")
 
 
-def test_two_sided_hits_bit_exact(cfg0, oracle):
+def test_two_sided_hits_bit_exact(cfg0, oracle, width):
     sd, sc, ps = cfg0
     r = scenes.random_rays(32768, 3)
     h = _closest(sc, r, two_sided=True)
     _assert_hits_equal(h, oracle.trace_closest(ps.nodes, ps.tris_ordered(), r, two_sided=True), "two-sided")
 
 
-def test_any_hit_bit_exact(cfg0, oracle):
+def test_any_hit_bit_exact(cfg0, oracle, width):
     sd, sc, ps = cfg0
     r = scenes.random_rays(65536, 4)
     o = _any(sc, r)
@@ -111,7 +131,7 @@ def test_any_hit_bit_exact(cfg0, oracle):
     assert np.array_equal(o, ref)
 
 
-def test_rays_from_surfaces_bit_exact(cfg0, oracle):
+def test_rays_from_surfaces_bit_exact(cfg0, oracle, width):
     """Rays starting on surfaces (extension-ray origins) exercise the
     behind-origin pruning next to the origin triangle."""
     sd, sc, ps = cfg0
@@ -136,9 +156,10 @@ def test_empty_and_single_ray(cfg0, oracle):
     _assert_hits_equal(_closest(sc, r), oracle.trace_closest(ps.nodes, ps.tris_ordered(), r), "single")
 
 
-def test_traversal_counters_match_oracle(cfg0, oracle):
+def test_traversal_counters_match_oracle(cfg0, oracle, width):
     """Boxes/triangles tested per ray (the roofline's N_node, N_tri) counted
-    on the GPU equal the oracle's count of the same visit scheme."""
+    on the GPU equal the oracle's count of the same visit scheme, for each
+    BVH record width."""
     import torch
     sd, sc, ps = cfg0
     r = scenes.random_rays(20000, 12)
@@ -147,10 +168,42 @@ def test_traversal_counters_match_oracle(cfg0, oracle):
     h = torch.empty((len(r), 4), dtype=torch.float32, device="cuda")
     sc.trace_closest(rd, h, counts_dev=cnt)
     torch.cuda.synchronize()
-    assert tuple(cnt.cpu().tolist()) == oracle.count_traversal(ps.nodes, ps.tris_ordered(), r, prune=True)
+    assert tuple(cnt.cpu().tolist()) == oracle.count_traversal(ps.nodes, ps.tris_ordered(), r, prune=True,
+                                                               width=width)
     cnt.zero_()
     o = torch.empty(len(r), dtype=torch.uint8, device="cuda")
     sc.trace_any(rd, o, counts_dev=cnt)
     torch.cuda.synchronize()
     assert tuple(cnt.cpu().tolist()) == oracle.count_traversal(ps.nodes, ps.tris_ordered(), r, any_hit=True,
-                                                               prune=True)
+                                                               prune=True, width=width)
+
+
+@pytest.fixture(scope="module")
+def deep(gpu, oracle):
+    """A deeper tree (icosphere, 81920 triangles): traversal stacks spill from
+    LDS into the overflow area for the narrow LDS variants."""
+    sd = scenes.icosphere_scene(width=8, height=8, subdiv=6)
+    sc = abi.Scene(gpu, sd.positions, sd.normals, sd.uvs, sd.material_ids, sd.materials)
+    nodes, ordered = sc.bvh()
+    ps = oracle.PreparedScene(sd, nodes, ordered)
+    yield sd, sc, ps
+    sc.close()
+
+
+def test_deep_tree_hits_bit_exact_every_variant(deep, gpu, oracle):
+    """Every traversal variant (all block sizes, LDS stack depths, loop
+    shapes and record widths) returns the oracle's hits on a deep tree."""
+    sd, sc, ps = deep
+    r = scenes.random_rays(32768, 21)
+    ref = oracle.trace_closest(ps.nodes, ps.tris_ordered(), r)
+    occ = oracle.trace_any(ps.nodes, ps.tris_ordered(), r)
+    assert np.count_nonzero(ref[3] >= 0) > 1000
+    try:
+        for v, block, lds, refill, ww, width in abi.traversal_variants():
+            abi.set_traversal_variant(gpu, v)
+            label = f"variant {v} (block {block}, lds {lds}, refill {refill}, ww {ww}, width {width})"
+            print(label, flush=True)
+            _assert_hits_equal(_closest(sc, r), ref, label)
+            assert np.array_equal(_any(sc, r), occ), label
+    finally:
+        abi.set_traversal_variant(gpu, 0)

@@ -77,6 +77,44 @@ def test_pruned_two_child_walk_equals_reference_order(oracle, seed):
     assert np.array_equal(occ.astype(bool), ia >= 0)
 
 
+@pytest.mark.parametrize("scene", ["spheres", "icosphere5"])
+def test_four_wide_walk_equals_reference_order(oracle, scene):
+    """The 4-wide visit scheme (grandchild boxes per interior node, pruning by
+    the current hit) returns exactly the reference traversal's results."""
+    if scene == "spheres":
+        pos = scenes.spheres_scene().positions
+    else:
+        pos = scenes.icosphere_scene(width=8, height=8, subdiv=5).positions
+    nodes, ordered = oracle.bvh_build(pos)
+    tris = oracle.ordered_tris(pos, ordered)
+    rays = scenes.random_rays(20000, 7)
+    t, u, v, i = oracle.trace_closest(nodes, tris, rays)
+    tq, iq = oracle.trace_walk(nodes, tris, rays, prune=True, width=4)
+    assert np.array_equal(i, iq) and np.array_equal(t.view(np.uint32), tq.view(np.uint32))
+    assert (i >= 0).sum() > 1000
+    occ = oracle.trace_any(nodes, tris, rays)
+    _, ia = oracle.trace_walk(nodes, tris, rays, any_hit=True, prune=True, width=4)
+    assert np.array_equal(occ.astype(bool), ia >= 0)
+
+
+def test_four_wide_walk_reaches_the_same_leaves(oracle):
+    """Without pruning a leaf is reached iff its box and every ancestor's box
+    is hit (child boxes nest in their parent's, and the slab test is monotone
+    in the plane coordinates), and both schemes reach leaves in the same
+    order: the triangles tested, any-hit's early exit included, are the same
+    for both record widths, while the 4-wide walk tests fewer boxes (the
+    children's own boxes are skipped)."""
+    pos = scenes.icosphere_scene(width=8, height=8, subdiv=5).positions
+    nodes, ordered = oracle.bvh_build(pos)
+    tris = oracle.ordered_tris(pos, ordered)
+    rays = scenes.random_rays(5000, 8)
+    for any_hit in (False, True):
+        n2, t2 = oracle.count_traversal(nodes, tris, rays, any_hit=any_hit, prune=False, width=2)
+        n4, t4 = oracle.count_traversal(nodes, tris, rays, any_hit=any_hit, prune=False, width=4)
+        assert t4 == t2
+        assert n4 < n2
+
+
 def test_pruned_traversal_tests_fewer_boxes(oracle):
     sd = scenes.spheres_scene()
     nodes, ordered = oracle.bvh_build(sd.positions)
