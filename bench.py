@@ -485,12 +485,12 @@ def run_frame(args) -> dict:
         from firefly_amd.autotune import autotune
         t0 = time.perf_counter()
         tuned = autotune(ctx, scene, env, cam, sd.width, sd.height, args.depth, sd.seed, probe_spp=4,
-                         batches=(1, 2, 4))
+                         batches=(1, 2, 4), refine_batches=(8, 16, 32, 64), refine_spp=args.spp)
         args.spp_batch = tuned["best"]["spp_batch"]
         for c in tuned["candidates"]:
             log(f"  variant {c['variant']} (block {c['block']}, lds {c['lds_entries']}, refill {c['refill']}, "
-                f"ww {c['while_while']}) "
-                f"spp_batch {c['spp_batch']}: {c['seconds'] * 1e3:.1f} ms")
+                f"ww {c['while_while']}, width {c['bvh_width']}) "
+                f"spp_batch {c['spp_batch']} at {c.get('probe_spp', 4)} spp: {c['seconds'] * 1e3:.1f} ms")
         log(f"autotune ({time.perf_counter() - t0:.1f} s): {tuned['best']}")
 
     def frame(flags=0):

@@ -30,12 +30,17 @@ def _time_render(ctx, scene, env, cam, w, h, spp, depth, seed, film, spp_batch, 
 
 def autotune(ctx: abi.Context, scene: abi.Scene, env: abi.EnvMap, cam: abi.Camera, width: int, height: int,
              max_depth: int, seed: int = 0, probe_spp: int = 2, batches: Iterable[int] = (1, 2),
-             reps: int = 2, out_json: Optional[str] = None) -> Dict:
+             reps: int = 2, out_json: Optional[str] = None, refine_batches: Iterable[int] = (),
+             refine_spp: int = 0) -> Dict:
     """Pick the fastest traversal variant and spp_batch for this request.
 
     Each candidate renders `probe_spp` samples per pixel (timed, best of
-    `reps`).  The chosen variant is set on `ctx`; returns a dict with the
-    choice and every timing."""
+    `reps`).  With `refine_batches`, a second stage keeps the chosen variant
+    and times those larger spp_batch values (and the first stage's choice)
+    at `refine_spp` samples per pixel: more paths in flight mean fewer
+    traversal launches, each paying the latency of its slowest ray once.
+    The chosen variant is set on `ctx`; returns a dict with the choice and
+    every timing."""
     import torch
     film = torch.zeros((height, width, 4), dtype=torch.float32, device="cuda")
     results = []
@@ -51,9 +56,19 @@ def autotune(ctx: abi.Context, scene: abi.Scene, env: abi.EnvMap, cam: abi.Camer
             results.append({"variant": v, "block": block, "lds_entries": lds, "refill": refill,
                             "while_while": ww, "bvh_width": width_bvh, "spp_batch": k, "seconds": t})
     best = min(results, key=lambda r: r["seconds"])
+    refine = sorted(set(k for k in refine_batches if k <= refine_spp) | {best["spp_batch"]})
+    if refine_spp > 0 and len(refine) > 1:
+        abi.set_traversal_variant(ctx, best["variant"])
+        stage2 = []
+        for k in refine:
+            t = _time_render(ctx, scene, env, cam, width, height, refine_spp, max_depth, seed, film, k, reps)
+            stage2.append(dict(best, spp_batch=k, seconds=t, probe_spp=refine_spp))
+        results += stage2
+        best = min(stage2, key=lambda r: r["seconds"])
     abi.set_traversal_variant(ctx, best["variant"])
     out = {"best": best, "candidates": results,
-           "request": {"width": width, "height": height, "max_depth": max_depth, "probe_spp": probe_spp}}
+           "request": {"width": width, "height": height, "max_depth": max_depth, "probe_spp": probe_spp,
+                       "refine_spp": refine_spp}}
     if out_json:
         with open(out_json, "w") as f:
             json.dump(out, f, indent=1)

@@ -830,18 +830,44 @@ static_assert(FETCH_SEGS == 64, "one segment per lane of the scan");
 struct SegFetch {
     uint32_t* ctr;   // FETCH_SEGS counters, FETCH_STRIDE words apart
     uint32_t n, seg_len, seg;
+    uint32_t res_first, res_left;  // items claimed by this wave and not yet handed out
+    uint32_t chunk;                // items claimed per atomic
     __device__ __forceinline__ void init(uint32_t* c, uint32_t n_, uint32_t home) {
         ctr = c;
         n = n_;
         seg_len = (n + FETCH_SEGS - 1) / FETCH_SEGS;
         seg = home % FETCH_SEGS;
+        res_first = 0;
+        res_left = 0;
+        // a wave claims up to 1/256 of a segment (at most FETCH_CHUNK) per
+        // atomic, so refills of a few lanes rarely wait on the memory-side
+        // atomic, and the last claims of a segment stay small
+        uint32_t c8 = seg_len >> 8;
+        chunk = c8 < FETCH_CHUNK ? c8 : FETCH_CHUNK;
+    }
+    // Hands out up to `want` items from the wave's reservoir, refilling it
+    // from the counters when it is empty.  Same contract as take().
+    __device__ __forceinline__ uint32_t take_chunked(uint32_t want, uint32_t& got, bool& done) {
+        done = false;
+        if (res_left == 0) {
+            uint32_t g;
+            uint32_t f = take(want > chunk ? want : chunk, g, done);
+            res_first = __builtin_amdgcn_readfirstlane(f);
+            res_left = __builtin_amdgcn_readfirstlane(g);
+        }
+        got = res_left < want ? res_left : want;
+        uint32_t first = res_first;
+        res_first += got;
+        res_left -= got;
+        if (res_left > 0) done = false;
+        return first;
     }
     __device__ __forceinline__ uint32_t seg_size(uint32_t s) const {
         uint32_t lo = s * seg_len;
         return lo < n ? (n - lo < seg_len ? n - lo : seg_len) : 0u;
     }
-    // Grants up to `want` items (want <= 64): returns the first item and the
-    // number granted; got == 0 with done == false means "try again later".
+    // Grants up to `want` items: returns the first item and the number
+    // granted; got == 0 with done == false means "try again later".
     __device__ __forceinline__ uint32_t take(uint32_t want, uint32_t& got, bool& done) {
         got = 0;
         done = false;

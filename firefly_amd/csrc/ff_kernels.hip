@@ -183,13 +183,21 @@ struct RaySrc {  // a plain ray array (ff_trace_closest / ff_trace_any)
 // round of waves is needed.  A wave refills once REFILL of its lanes are idle
 // (or all are), trading idle lanes against refill stalls.  Every wave leaves
 // the loop once all segments are exhausted and its own rays are finished.
-// WW selects the loop shape: false = one step (interior record or leaf) per
-// lane per iteration; true = "while-while": lanes at interior records step
+// WW selects the loop shape: 0 = one step (interior record or leaf) per
+// lane per iteration; 1 = "while-while": lanes at interior records step
 // until every busy lane has reached a leaf, then all leaves are tested
-// together (fewer divergent switches between the two code paths).
+// together (fewer divergent switches between the two code paths);
+// 2 = speculative while-while (Aila and Laine, "Understanding the Efficiency
+// of Ray Traversal on GPUs", HPG 2009): a lane that reaches a leaf while it
+// holds none postpones it and keeps walking interior records until every
+// lane holds a leaf or has no interior record left, then the postponed (and
+// current) leaves are tested together.  Each lane still tests its leaves in
+// its visit order, so hits are those of the other shapes; pruning then sees
+// the closest hit of the leaves already tested, so the boxes tested depend
+// on the lanes that share the wave.
 // WIDE walks the 4-wide records (half the dependent interior steps, same
 // leaf order) instead of the two-child records.
-template <int BS, int LDS, int REFILL, bool ANY, bool WW, bool WIDE, class Src>
+template <int BS, int LDS, int REFILL, bool ANY, int WW, bool WIDE, class Src>
 __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint32_t* fetch, int32_t* ovf,
                                                 unsigned long long* cnt, int* smem) {
     SegFetch F;
@@ -202,6 +210,8 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
     st.ovf_stride = FF_TRAV_THREADS;
     uint32_t cn = 0, ct = 0, id = 0;
     bool busy = false, more = true, two = false;
+    bool walk = false;  // WW 2: the lane has records or leaves left to visit
+    int pend = 0;       // WW 2: postponed leaf (0 = none; leaf refs are negative)
     Bvh B;
     B.nodes = nullptr; B.nodes4 = nullptr; B.leaf_count = nullptr; B.root = nullptr;
     const float* tris = nullptr;
@@ -215,13 +225,15 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
             uint32_t got;
             bool done;
-            uint32_t first = F.take(n_idle, got, done);
+            uint32_t first = F.take_chunked(n_idle, got, done);
             if (done) more = false;
             if (!busy && rank < got) {
                 f3 o, d;
                 src.load(first + rank, id, B, tris, o, d, two);
                 h = no_hit();
                 busy = trav_begin(B, o, d, R, st, cn);
+                walk = busy;
+                pend = 0;
                 if (!busy) {
                     if (ANY) src.occluded(id, false);
                     else src.hit(id, h);
@@ -232,7 +244,43 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
             if (!more) break;
             continue;
         }
-        if (WW) {
+        if (WW == 2) {
+            for (;;) {
+                if (busy && walk && R.ref < 0 && pend == 0) {  // postpone the first leaf
+                    pend = R.ref;
+                    if (st.sp == 0) walk = false;
+                    else R.ref = st.pop();
+                }
+                bool at_int = busy && walk && R.ref >= 0;
+                bool seeking = busy && walk && pend == 0;
+                if (__ballot(at_int) == 0 || __ballot(seeking) == 0) break;
+                if (at_int && !visit_node<WIDE, !ANY>(B, R, h.t, st, cn)) {
+                    if (st.sp == 0) walk = false;
+                    else R.ref = st.pop();
+                }
+            }
+            if (busy) {
+                bool occ = false;
+                if (pend != 0) {  // the postponed leaf, then the current one
+                    TravRay L = R;
+                    L.ref = pend;
+                    if (ANY) occ = leaf_any(B, tris, L, ct);
+                    else leaf_closest(B, tris, L, h, two, ct);
+                    pend = 0;
+                }
+                if (!occ && walk && R.ref < 0) {
+                    if (ANY) occ = leaf_any(B, tris, R, ct);
+                    else leaf_closest(B, tris, R, h, two, ct);
+                    if (st.sp == 0) walk = false;
+                    else R.ref = st.pop();
+                }
+                if (occ || !walk) {
+                    if (ANY) src.occluded(id, occ);
+                    else src.hit(id, h);
+                    busy = false;
+                }
+            }
+        } else if (WW == 1) {
             for (;;) {  // interior records until every busy lane is at a leaf
                 bool at_int = busy && R.ref >= 0;
                 if (__ballot(at_int) == 0) break;
@@ -279,7 +327,7 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
     }
 }
 
-template <int BS, int LDS, int RF, bool WW, bool WIDE>
+template <int BS, int LDS, int RF, int WW, bool WIDE>
 __global__ __launch_bounds__(BS) void k_extend(ff_paths P, int q) {
     __shared__ int smem[LDS * BS];
     ExtSrc src{P, P.q_ext[q]};
@@ -287,7 +335,7 @@ __global__ __launch_bounds__(BS) void k_extend(ff_paths P, int q) {
                                             P.fetch + F_EXT * FETCH_SEGS * FETCH_STRIDE, P.ovf,
                                             stats_rep(P) + S_EXT_NODES, smem);
 }
-template <int BS, int LDS, int RF, bool WW, bool WIDE>
+template <int BS, int LDS, int RF, int WW, bool WIDE>
 __global__ __launch_bounds__(BS) void k_shadow(ff_paths P) {
     __shared__ int smem[LDS * BS];
     ShadowSrc src{P};
@@ -970,13 +1018,13 @@ struct TraceArgs {  // one ff_trace_closest / ff_trace_any call
     int32_t* ovf;
     unsigned long long* counts;
 };
-template <int BS, int LDS, int RF, bool WW, bool WIDE>
+template <int BS, int LDS, int RF, int WW, bool WIDE>
 __global__ __launch_bounds__(BS) void k_trace_closest(TraceArgs a) {
     __shared__ int smem[LDS * BS];
     RaySrc src{a.S, a.rays, a.hits, nullptr, a.two_sided};
     trav_persistent<BS, LDS, RF, false, WW, WIDE>(src, a.n, a.fetch, a.ovf, a.counts, smem);
 }
-template <int BS, int LDS, int RF, bool WW, bool WIDE>
+template <int BS, int LDS, int RF, int WW, bool WIDE>
 __global__ __launch_bounds__(BS) void k_trace_any(TraceArgs a) {
     __shared__ int smem[LDS * BS];
     RaySrc src{a.S, a.rays, nullptr, a.occ, 0};
@@ -1149,28 +1197,30 @@ uint32_t resident_grid(K kernel, int bs) {
 }
 
 // traversal variants (block size, LDS stack entries, refill threshold, loop
-// shape, BVH record width)
-#define FF_TRAV_VARIANT_LIST(X)      \
-    X(0, 256, 16, 32, false, false)  \
-    X(1, 256, 16, 16, false, false)  \
-    X(2, 256, 16, 64, false, false)  \
-    X(3, 128, 16, 32, false, false)  \
-    X(4, 256, 8, 32, false, false)   \
-    X(5, 256, 24, 32, false, false)  \
-    X(6, 64, 32, 32, false, false)   \
-    X(7, 128, 24, 32, false, false)  \
-    X(8, 256, 16, 32, true, false)   \
-    X(9, 64, 32, 32, true, false)    \
-    X(10, 256, 16, 16, true, false)  \
-    X(11, 128, 24, 32, true, false)  \
-    X(12, 256, 16, 64, false, true)  \
-    X(13, 256, 16, 32, false, true)  \
-    X(14, 64, 32, 32, false, true)   \
-    X(15, 256, 16, 32, true, true)
+// shape 0/1/2, BVH record width)
+#define FF_TRAV_VARIANT_LIST(X)  \
+    X(0, 256, 16, 32, 0, false)  \
+    X(1, 256, 16, 16, 0, false)  \
+    X(2, 256, 16, 64, 0, false)  \
+    X(3, 128, 16, 32, 0, false)  \
+    X(4, 256, 8, 32, 0, false)   \
+    X(5, 256, 24, 32, 0, false)  \
+    X(6, 64, 32, 32, 0, false)   \
+    X(7, 128, 24, 32, 0, false)  \
+    X(8, 256, 16, 32, 1, false)  \
+    X(9, 64, 32, 32, 1, false)   \
+    X(10, 256, 16, 16, 1, false) \
+    X(11, 128, 24, 32, 1, false) \
+    X(12, 256, 16, 64, 0, true)  \
+    X(13, 256, 16, 32, 0, true)  \
+    X(14, 64, 32, 32, 0, true)   \
+    X(15, 256, 16, 32, 1, true)  \
+    X(16, 256, 16, 64, 2, false) \
+    X(17, 256, 16, 32, 2, false)
 
 // which: 0 extend, 1 shadow, 2 grid sizes only (before a graph capture),
 // 3 ff_trace_closest, 4 ff_trace_any
-template <int BS, int LDS, int RF, bool WW, bool WIDE>
+template <int BS, int LDS, int RF, int WW, bool WIDE>
 hipError_t trav_launch(int which, const ff_paths* P, int q, const TraceArgs* T, hipStream_t s) {
     if (which == 0) {
         uint32_t g = resident_grid(k_extend<BS, LDS, RF, WW, WIDE>, BS);
@@ -1212,7 +1262,7 @@ const int kTravVariants = 0 FF_TRAV_VARIANT_LIST(FF_TRAV_ONE);
 #define FF_TRAV_BS(v, bs, lds, rf, ww, wide) bs,
 #define FF_TRAV_LDS(v, bs, lds, rf, ww, wide) lds,
 #define FF_TRAV_RF(v, bs, lds, rf, ww, wide) rf,
-#define FF_TRAV_WW(v, bs, lds, rf, ww, wide) (ww ? 1 : 0),
+#define FF_TRAV_WW(v, bs, lds, rf, ww, wide) ww,
 #define FF_TRAV_WIDTH(v, bs, lds, rf, ww, wide) (wide ? 4 : 2),
 const int kTravBlock[] = {FF_TRAV_VARIANT_LIST(FF_TRAV_BS)};
 const int kTravLds[] = {FF_TRAV_VARIANT_LIST(FF_TRAV_LDS)};

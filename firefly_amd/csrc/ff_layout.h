@@ -43,6 +43,7 @@
 #define F_SHD 1
 #define F_TRACE 2
 #define FETCH_WORDS (3 * FETCH_SEGS * FETCH_STRIDE)
+#define FETCH_CHUNK 256  // largest claim of one wave from a work counter
 
 // ray statistics
 #define S_PRIMARY 0

@@ -8,23 +8,23 @@ from firefly_amd import abi, scenes
 pytestmark = pytest.mark.gpu
 
 
+KINDS = {"binary": (2, 0), "wide": (4, 0), "wide_ww": (4, 1), "binary_spec": (2, 2)}
+
+
 def _variant_of(kind):
-    """First traversal variant of a kind: two-child records, 4-wide records,
-    4-wide records with the while-while loop."""
+    """First traversal variant of a kind: (BVH record width, loop shape)."""
     for v, _, _, _, ww, width in abi.traversal_variants():
-        if (kind == "binary" and width == 2 and not ww) or (kind == "wide" and width == 4 and not ww) or \
-                (kind == "wide_ww" and width == 4 and ww):
-            return v, width
+        if (width, ww) == KINDS[kind]:
+            return v
     raise AssertionError(f"no {kind} traversal variant")
 
 
-@pytest.fixture(params=["binary", "wide", "wide_ww"])
-def width(request, gpu):
+@pytest.fixture(params=list(KINDS))
+def trav(request, gpu):
     """Runs the test with the context's traversal variant set to each BVH
-    record format; yields the record width."""
-    v, w = _variant_of(request.param)
-    abi.set_traversal_variant(gpu, v)
-    yield w
+    record format and loop shape; yields (record width, loop shape)."""
+    abi.set_traversal_variant(gpu, _variant_of(request.param))
+    yield KINDS[request.param]
     abi.set_traversal_variant(gpu, 0)
 
 
@@ -108,21 +108,21 @@ def test_primary_hits_bit_exact(cfg0, gpu, oracle):
 
 
 @pytest.mark.parametrize("seed", [1, 2])
-def test_incoherent_hits_bit_exact(cfg0, oracle, seed, width):
+def test_incoherent_hits_bit_exact(cfg0, oracle, seed, trav):
     sd, sc, ps = cfg0
     r = scenes.random_rays(65536, seed)
     h = _closest(sc, r)
     _assert_hits_equal(h, oracle.trace_closest(ps.nodes, ps.tris_ordered(), r), f"incoherent{seed}")
 
 
-def test_two_sided_hits_bit_exact(cfg0, oracle, width):
+def test_two_sided_hits_bit_exact(cfg0, oracle, trav):
     sd, sc, ps = cfg0
     r = scenes.random_rays(32768, 3)
     h = _closest(sc, r, two_sided=True)
     _assert_hits_equal(h, oracle.trace_closest(ps.nodes, ps.tris_ordered(), r, two_sided=True), "two-sided")
 
 
-def test_any_hit_bit_exact(cfg0, oracle, width):
+def test_any_hit_bit_exact(cfg0, oracle, trav):
     sd, sc, ps = cfg0
     r = scenes.random_rays(65536, 4)
     o = _any(sc, r)
@@ -131,7 +131,7 @@ def test_any_hit_bit_exact(cfg0, oracle, width):
     assert np.array_equal(o, ref)
 
 
-def test_rays_from_surfaces_bit_exact(cfg0, oracle, width):
+def test_rays_from_surfaces_bit_exact(cfg0, oracle, trav):
     """Rays starting on surfaces (extension-ray origins) exercise the
     behind-origin pruning next to the origin triangle."""
     sd, sc, ps = cfg0
@@ -156,11 +156,15 @@ def test_empty_and_single_ray(cfg0, oracle):
     _assert_hits_equal(_closest(sc, r), oracle.trace_closest(ps.nodes, ps.tris_ordered(), r), "single")
 
 
-def test_traversal_counters_match_oracle(cfg0, oracle, width):
+def test_traversal_counters_match_oracle(cfg0, oracle, trav):
     """Boxes/triangles tested per ray (the roofline's N_node, N_tri) counted
     on the GPU equal the oracle's count of the same visit scheme, for each
-    BVH record width."""
+    BVH record width.  The speculative loop tests leaves later than it
+    reaches them, so its pruning is weaker: its counts lie between the
+    pruned and the unpruned walk's (any-hit: the same triangles, as leaves
+    are still tested in order up to the first occluder)."""
     import torch
+    width, ww = trav
     sd, sc, ps = cfg0
     r = scenes.random_rays(20000, 12)
     rd = torch.from_numpy(r).cuda()
@@ -168,14 +172,23 @@ def test_traversal_counters_match_oracle(cfg0, oracle, width):
     h = torch.empty((len(r), 4), dtype=torch.float32, device="cuda")
     sc.trace_closest(rd, h, counts_dev=cnt)
     torch.cuda.synchronize()
-    assert tuple(cnt.cpu().tolist()) == oracle.count_traversal(ps.nodes, ps.tris_ordered(), r, prune=True,
-                                                               width=width)
+    got = tuple(cnt.cpu().tolist())
+    ref = oracle.count_traversal(ps.nodes, ps.tris_ordered(), r, prune=True, width=width)
+    if ww == 2:
+        hi = oracle.count_traversal(ps.nodes, ps.tris_ordered(), r, prune=False, width=width)
+        assert ref[0] <= got[0] <= hi[0] and ref[1] <= got[1] <= hi[1], (got, ref, hi)
+    else:
+        assert got == ref
     cnt.zero_()
     o = torch.empty(len(r), dtype=torch.uint8, device="cuda")
     sc.trace_any(rd, o, counts_dev=cnt)
     torch.cuda.synchronize()
-    assert tuple(cnt.cpu().tolist()) == oracle.count_traversal(ps.nodes, ps.tris_ordered(), r, any_hit=True,
-                                                               prune=True, width=width)
+    got = tuple(cnt.cpu().tolist())
+    ref = oracle.count_traversal(ps.nodes, ps.tris_ordered(), r, any_hit=True, prune=True, width=width)
+    if ww == 2:
+        assert got[0] >= ref[0] and got[1] == ref[1], (got, ref)
+    else:
+        assert got == ref
 
 
 @pytest.fixture(scope="module")
