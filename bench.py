@@ -38,6 +38,15 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def _shape_of(variant: int, spp_batch: int) -> dict:
+    """Launch shape of a fixed traversal variant (no autotuning)."""
+    for v, block, lds, refill, ww, width in abi.traversal_variants():
+        if v == variant:
+            return {"variant": v, "block": block, "lds_entries": lds, "refill": refill, "while_while": ww,
+                    "bvh_width": width, "spp_batch": spp_batch}
+    return {"variant": variant, "spp_batch": spp_batch}
+
+
 def load_traffic(kernel: str, workload: str, rays_per_launch: float):
     """HBM bytes per launch of `kernel`: the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE per dispatch, copy-calibrated) gives bytes per
@@ -576,7 +585,7 @@ def run_frame(args) -> dict:
         "config": {"workload": workload, "width": sd.width, "height": sd.height, "spp": args.spp,
                    "max_depth": args.depth, "tris": sd.n_tris, "env": f"{sd.env.shape[1]}x{sd.env.shape[0]}",
                    "spp_batch": args.spp_batch},
-        "launch_shape": (tuned["best"] if tuned else {"variant": args.variant}),
+        "launch_shape": (tuned["best"] if tuned else _shape_of(max(args.variant, 0), args.spp_batch)),
         "spp_per_s_1080p": round(spp_s, 3),
         "total_mrays_s": round(total_rays / elapsed / 1e6, 3),
         "rays_per_step": {k: int(tot[k] / args.steps) for k in ("rays_primary", "rays_extension", "rays_shadow")},
