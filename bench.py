@@ -40,6 +40,7 @@ def log(msg: str) -> None:
 
 def _shape_of(variant: int, spp_batch: int) -> dict:
     """Launch shape of a fixed traversal variant (no autotuning)."""
+    from firefly_amd import abi
     for v, block, lds, refill, ww, width in abi.traversal_variants():
         if v == variant:
             return {"variant": v, "block": block, "lds_entries": lds, "refill": refill, "while_while": ww,

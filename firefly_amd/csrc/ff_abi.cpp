@@ -804,6 +804,9 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
     p.reqs = ctx->d_reqs;
     p.n_req = (uint32_t)table.size();
     p.n_slots = (uint32_t)n_slots;
+    p.uni_scene = 1;
+    for (const HostReq& q : reqs)
+        if (q.s != reqs[0].s) p.uni_scene = 0;
     p.g_max = ctx->g_max;
     p.g_cnt = ctx->g_cnt;
     p.g_fx = ctx->g_fx;

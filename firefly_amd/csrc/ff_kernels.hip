@@ -121,16 +121,28 @@ __global__ __launch_bounds__(FF_BLOCK) void k_generate(ff_paths P) {
 
 // =============================================================== traversal
 // Ray sources of the persistent traversal kernels: load() binds queue entry k
-// to a ray and its scene, hit()/occluded() write the result.
+// to a ray and (unless kUniform) its scene, hit()/occluded() write the result.
+// kUniform: every ray of the launch uses one scene, bound once per wave by
+// scene0() (the BVH and triangle pointers then stay in scalar registers and
+// a refill needs no request-table lookup).
+template <bool UNI>
 struct ExtSrc {  // wf_extend (CRT/kernel.cu:487-523): closest hit of every queued path
+    static constexpr bool kUniform = UNI;
     ff_paths P;
     const uint32_t* queue;
+    __device__ __forceinline__ void scene0(Bvh& B, const float*& tris) const {
+        const ff_dev_scene& S = P.reqs[0].S;
+        B = bvh_of(S);
+        tris = S.tris;
+    }
     __device__ __forceinline__ void load(uint32_t k, uint32_t& id, Bvh& B, const float*& tris, f3& o, f3& d,
                                          bool& two) const {
         id = queue[k];
-        const ff_dev_scene& S = P.reqs[P.preq[id]].S;
-        B = bvh_of(S);
-        tris = S.tris;
+        if (!UNI) {
+            const ff_dev_scene& S = P.reqs[P.preq[id]].S;
+            B = bvh_of(S);
+            tris = S.tris;
+        }
         o = xyz(P.ray[2 * id]);
         d = xyz(P.ray[2 * id + 1]);
         two = (__float_as_uint(P.beta[id].w) & ST_INSIDE) != 0;
@@ -140,14 +152,23 @@ struct ExtSrc {  // wf_extend (CRT/kernel.cu:487-523): closest hit of every queu
     }
     __device__ __forceinline__ void occluded(uint32_t, bool) const {}
 };
+template <bool UNI>
 struct ShadowSrc {  // wf_shadow (CRT/kernel.cu:525-543): any hit of every queued shadow ray
+    static constexpr bool kUniform = UNI;
     ff_paths P;
+    __device__ __forceinline__ void scene0(Bvh& B, const float*& tris) const {
+        const ff_dev_scene& S = P.reqs[0].S;
+        B = bvh_of(S);
+        tris = S.tris;
+    }
     __device__ __forceinline__ void load(uint32_t k, uint32_t& s, Bvh& B, const float*& tris, f3& o, f3& d,
                                          bool& two) const {
         s = P.q_shadow[k];
-        const ff_dev_scene& S = P.reqs[P.preq[s / 3]].S;
-        B = bvh_of(S);
-        tris = S.tris;
+        if (!UNI) {
+            const ff_dev_scene& S = P.reqs[P.preq[s / 3]].S;
+            B = bvh_of(S);
+            tris = S.tris;
+        }
         o = xyz(P.sray[2 * s]);
         d = xyz(P.sray[2 * s + 1]);
         two = false;
@@ -156,16 +177,19 @@ struct ShadowSrc {  // wf_shadow (CRT/kernel.cu:525-543): any hit of every queue
     __device__ __forceinline__ void occluded(uint32_t s, bool occ) const { P.vis[s] = occ ? 0 : 1; }
 };
 struct RaySrc {  // a plain ray array (ff_trace_closest / ff_trace_any)
+    static constexpr bool kUniform = true;
     ff_dev_scene S;
     const float4* rays;
     float4* hits;
     uint8_t* occ;
     int two_sided;
-    __device__ __forceinline__ void load(uint32_t k, uint32_t& id, Bvh& B, const float*& tris, f3& o, f3& d,
-                                         bool& two) const {
-        id = k;
+    __device__ __forceinline__ void scene0(Bvh& B, const float*& tris) const {
         B = bvh_of(S);
         tris = S.tris;
+    }
+    __device__ __forceinline__ void load(uint32_t k, uint32_t& id, Bvh&, const float*&, f3& o, f3& d,
+                                         bool& two) const {
+        id = k;
         o = xyz(rays[2 * k]);
         d = xyz(rays[2 * k + 1]);
         two = two_sided != 0;
@@ -215,6 +239,7 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
     Bvh B;
     B.nodes = nullptr; B.nodes4 = nullptr; B.leaf_count = nullptr; B.root = nullptr;
     const float* tris = nullptr;
+    if (Src::kUniform) src.scene0(B, tris);
     TravRay R;
     Hit h = no_hit();
     for (;;) {
@@ -327,18 +352,20 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
     }
 }
 
-template <int BS, int LDS, int RF, int WW, bool WIDE>
-__global__ __launch_bounds__(BS) void k_extend(ff_paths P, int q) {
+// The one-scene extend kernel of the if-if shape fits 7 waves per SIMD
+// (72 VGPRs) without spilling; the others keep the compiler's choice.
+template <int BS, int LDS, int RF, int WW, bool WIDE, bool UNI>
+__global__ __launch_bounds__(BS, (UNI && WW == 0 && !WIDE) ? 7 : 1) void k_extend(ff_paths P, int q) {
     __shared__ int smem[LDS * BS];
-    ExtSrc src{P, P.q_ext[q]};
+    ExtSrc<UNI> src{P, P.q_ext[q]};
     trav_persistent<BS, LDS, RF, false, WW, WIDE>(src, P.counters[q == 0 ? C_EXT0 : C_EXT1],
                                             P.fetch + F_EXT * FETCH_SEGS * FETCH_STRIDE, P.ovf,
                                             stats_rep(P) + S_EXT_NODES, smem);
 }
-template <int BS, int LDS, int RF, int WW, bool WIDE>
+template <int BS, int LDS, int RF, int WW, bool WIDE, bool UNI>
 __global__ __launch_bounds__(BS) void k_shadow(ff_paths P) {
     __shared__ int smem[LDS * BS];
-    ShadowSrc src{P};
+    ShadowSrc<UNI> src{P};
     trav_persistent<BS, LDS, RF, true, WW, WIDE>(src, P.counters[C_SHADOW], P.fetch + F_SHD * FETCH_SEGS * FETCH_STRIDE,
                                            P.ovf, stats_rep(P) + S_SHD_NODES, smem);
 }
@@ -1019,7 +1046,7 @@ struct TraceArgs {  // one ff_trace_closest / ff_trace_any call
     unsigned long long* counts;
 };
 template <int BS, int LDS, int RF, int WW, bool WIDE>
-__global__ __launch_bounds__(BS) void k_trace_closest(TraceArgs a) {
+__global__ __launch_bounds__(BS, (WW == 0 && !WIDE) ? 7 : 1) void k_trace_closest(TraceArgs a) {
     __shared__ int smem[LDS * BS];
     RaySrc src{a.S, a.rays, a.hits, nullptr, a.two_sided};
     trav_persistent<BS, LDS, RF, false, WW, WIDE>(src, a.n, a.fetch, a.ovf, a.counts, smem);
@@ -1222,12 +1249,18 @@ uint32_t resident_grid(K kernel, int bs) {
 // 3 ff_trace_closest, 4 ff_trace_any
 template <int BS, int LDS, int RF, int WW, bool WIDE>
 hipError_t trav_launch(int which, const ff_paths* P, int q, const TraceArgs* T, hipStream_t s) {
-    if (which == 0) {
-        uint32_t g = resident_grid(k_extend<BS, LDS, RF, WW, WIDE>, BS);
-        hipLaunchKernelGGL((k_extend<BS, LDS, RF, WW, WIDE>), dim3(g), dim3(BS), 0, s, *P, q);
+    if (which == 0 && P->uni_scene) {
+        uint32_t g = resident_grid(k_extend<BS, LDS, RF, WW, WIDE, true>, BS);
+        hipLaunchKernelGGL((k_extend<BS, LDS, RF, WW, WIDE, true>), dim3(g), dim3(BS), 0, s, *P, q);
+    } else if (which == 0) {
+        uint32_t g = resident_grid(k_extend<BS, LDS, RF, WW, WIDE, false>, BS);
+        hipLaunchKernelGGL((k_extend<BS, LDS, RF, WW, WIDE, false>), dim3(g), dim3(BS), 0, s, *P, q);
+    } else if (which == 1 && P->uni_scene) {
+        uint32_t g = resident_grid(k_shadow<BS, LDS, RF, WW, WIDE, true>, BS);
+        hipLaunchKernelGGL((k_shadow<BS, LDS, RF, WW, WIDE, true>), dim3(g), dim3(BS), 0, s, *P);
     } else if (which == 1) {
-        uint32_t g = resident_grid(k_shadow<BS, LDS, RF, WW, WIDE>, BS);
-        hipLaunchKernelGGL((k_shadow<BS, LDS, RF, WW, WIDE>), dim3(g), dim3(BS), 0, s, *P);
+        uint32_t g = resident_grid(k_shadow<BS, LDS, RF, WW, WIDE, false>, BS);
+        hipLaunchKernelGGL((k_shadow<BS, LDS, RF, WW, WIDE, false>), dim3(g), dim3(BS), 0, s, *P);
     } else if (which == 3) {
         uint32_t g = resident_grid(k_trace_closest<BS, LDS, RF, WW, WIDE>, BS);
         hipLaunchKernelGGL((k_trace_closest<BS, LDS, RF, WW, WIDE>), dim3(g), dim3(BS), 0, s, *T);
@@ -1235,8 +1268,10 @@ hipError_t trav_launch(int which, const ff_paths* P, int q, const TraceArgs* T, 
         uint32_t g = resident_grid(k_trace_any<BS, LDS, RF, WW, WIDE>, BS);
         hipLaunchKernelGGL((k_trace_any<BS, LDS, RF, WW, WIDE>), dim3(g), dim3(BS), 0, s, *T);
     } else {
-        (void)resident_grid(k_extend<BS, LDS, RF, WW, WIDE>, BS);
-        (void)resident_grid(k_shadow<BS, LDS, RF, WW, WIDE>, BS);
+        (void)resident_grid(k_extend<BS, LDS, RF, WW, WIDE, true>, BS);
+        (void)resident_grid(k_shadow<BS, LDS, RF, WW, WIDE, true>, BS);
+        (void)resident_grid(k_extend<BS, LDS, RF, WW, WIDE, false>, BS);
+        (void)resident_grid(k_shadow<BS, LDS, RF, WW, WIDE, false>, BS);
     }
     return hipGetLastError();
 }

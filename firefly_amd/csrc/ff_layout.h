@@ -162,6 +162,7 @@ struct ff_paths {
     const ff_dev_request* reqs;
     uint32_t n_req;
     uint32_t n_slots;    // total slots over all requests
+    int32_t uni_scene;   // every request of the call renders the same scene
     // adjoint replay
     float4* dlt;         // [P]     d loss / d path radiance (xyz)
     uint32_t* g_max;     // arena: max |term| bits per gradient element (GM_BOUND)
