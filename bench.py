@@ -92,8 +92,10 @@ def cpu_baseline(sd, nodes, ordered, budget_s: float, spp: int):
 def run_batch(args) -> dict:
     """Batched-requests configuration: 64 seeded requests of uneven
     resolution, spp, depth and triangle count rendered (a) as one batched
-    wavefront replayed from a hipGraph and (b) as 64 plain sequential frames.
-    Prints one JSON line with both throughputs and their ratio."""
+    wavefront replayed from a hipGraph, (b) as 64 plain sequential frames
+    (one sample per pixel in flight) and (c) as 64 sequential frames with the
+    batched call's samples in flight (films bit-identical to (a)).  Returns
+    the throughputs and ratios."""
     import torch
     from concurrent.futures import ThreadPoolExecutor
     from firefly_amd import abi, scenes
@@ -116,20 +118,29 @@ def run_batch(args) -> dict:
         f"(setup {time.perf_counter() - t0:.1f} s)")
     cams = [abi.make_camera(sd.inv_vp) for sd, _ in built]
     f_seq = [torch.zeros((sd.height, sd.width, 4), dtype=torch.float32, device="cuda") for sd, _ in built]
+    f_plain = [torch.zeros_like(f) for f in f_seq]
     f_bat = [torch.zeros_like(f) for f in f_seq]
     items = [(sc, env_objs[k % 4], cams[k], sd.width, sd.height, sd.spp, sd.max_depth, sd.seed, f_bat[k])
              for k, (sd, sc) in enumerate(built)]
 
-    def sequential():
+    kb = args.batch_spp_batch  # samples of each pixel in flight in the batched call
+
+    def sequential(films, k_in_flight):
         for k, (sd, sc) in enumerate(built):
             abi.render(ctx, sc, env_objs[k % 4], cams[k], sd.width, sd.height, sd.spp, sd.max_depth, sd.seed,
-                       f_seq[k], spp_batch=1)
+                       films[k], spp_batch=k_in_flight)
 
-    def batched():
-        abi.render_batch(ctx, items, flags=abi.FF_RENDER_USE_GRAPH)
+    def batched(flags=abi.FF_RENDER_USE_GRAPH):
+        abi.render_batch(ctx, items, flags=flags, spp_batch=kb)
 
+    # plain: 64 per-frame wavefronts with one sample per pixel in flight (the
+    # reference's scheme); sequential: the same frames with the batched call's
+    # samples in flight (bit-identical films); batched: one call
     times = {}
-    for name, fn, films in (("sequential", sequential, f_seq), ("batched", batched, f_bat)):
+    runs = (("plain", lambda: sequential(f_plain, 1), f_plain),
+            ("sequential", lambda: sequential(f_seq, kb), f_seq),
+            ("batched", batched, f_bat))
+    for name, fn, films in runs:
         for i in range(args.warmup + args.steps):
             for f in films:
                 f.zero_()
@@ -143,13 +154,19 @@ def run_batch(args) -> dict:
             log(f"{name} run {i + 1}: {dt:.3f} s")
     st = abi.last_render_stats(ctx)
     same = all(torch.equal(a, b) for a, b in zip(f_seq, f_bat))
+    rel_plain = max(float((a - b).abs().max() / b.abs().max().clamp_min(1e-30)) for a, b in zip(f_bat, f_plain))
+    for f in f_bat:
+        f.zero_()
+    batched(abi.FF_RENDER_TIMING)  # one more call, timed per stage
+    stt = abi.last_render_stats(ctx)
+    stage_ms = {k: round(getattr(stt, k), 2) for k in ("ms_extend", "ms_shadow", "ms_logic", "ms_shade", "ms_other")}
     mismatch = []
     if not same:
         # where the films differ, and whether a plain render repeats itself
         f_rep = [torch.zeros_like(f) for f in f_seq]
         for k, (sd, sc) in enumerate(built):
             abi.render(ctx, sc, env_objs[k % 4], cams[k], sd.width, sd.height, sd.spp, sd.max_depth, sd.seed,
-                       f_rep[k], spp_batch=1)
+                       f_rep[k], spp_batch=kb)
         torch.cuda.synchronize()
         for k, (sd, _) in enumerate(built):
             a, b, c = f_seq[k], f_bat[k], f_rep[k]
@@ -165,6 +182,7 @@ def run_batch(args) -> dict:
     inc = st.rays_extension + st.rays_shadow
     tb = min(times["batched"])
     ts = min(times["sequential"])
+    tp = min(times["plain"])
     out = {
         "metric": METRIC,
         "value": round(inc / tb / 1e6, 3),
@@ -181,11 +199,15 @@ def run_batch(args) -> dict:
         "config": {"workload": f"batch{len(built)}_uneven", "requests": len(built),
                    "pixels": int(sum(sd.width * sd.height for sd, _ in built)),
                    "paths": int(sum(sd.width * sd.height * sd.spp for sd, _ in built)),
-                   "tris": int(sum(sd.n_tris for sd, _ in built))},
+                   "tris": int(sum(sd.n_tris for sd, _ in built)), "spp_batch": kb},
         "batched_incoherent_mrays_s": round(inc / tb / 1e6, 3),
+        "sequential_plain_mrays_s": round(inc / tp / 1e6, 3),
         "sequential_incoherent_mrays_s": round(inc / ts / 1e6, 3),
-        "batched_over_sequential": round(ts / tb, 3),
+        "batched_over_sequential": round(tp / tb, 3),
+        "batched_over_sequential_same_spp_batch": round(ts / tb, 3),
         "films_identical": bool(same),
+        "film_max_rel_diff_vs_plain": rel_plain,
+        "batched_stage_ms": stage_ms,
         "film_mismatch": mismatch[:8],
         "rays": {"primary": st.rays_primary, "extension": st.rays_extension, "shadow": st.rays_shadow},
     }
@@ -325,10 +347,11 @@ def run_multiview(args) -> dict:
     films = [torch.zeros((H, W, 4), dtype=torch.float32, device="cuda") for _ in range(V)]
     paths = [torch.zeros((S, H, W, 4), dtype=torch.float32, device="cuda") for _ in range(V)]
     items = [(scene, env, c, W, H, S, D, sd.seed, f) for c, f in zip(cams, films)]
+    kb = args.mv_spp_batch  # samples in flight per pixel, the same for batched and separate runs
     t_tex, _ = scenes.perturbed(sd, seed=1)
     for t in range(2):
         scene.set_texture(t, t_tex[t])
-    abi.render_batch(ctx, items, flags=abi.FF_RENDER_USE_GRAPH)
+    abi.render_batch(ctx, items, flags=abi.FF_RENDER_USE_GRAPH, spp_batch=kb)
     torch.cuda.synchronize()
     targets = [(f[..., :3] / f[..., 3:]).clone() for f in films]
     for t in range(2):
@@ -337,7 +360,7 @@ def run_multiview(args) -> dict:
     def fwd_batch():
         for f in films:
             f.zero_()
-        abi.render_batch(ctx, items, path_radiance=paths, flags=abi.FF_RENDER_USE_GRAPH)
+        abi.render_batch(ctx, items, path_radiance=paths, flags=abi.FF_RENDER_USE_GRAPH, spp_batch=kb)
 
     fwd_batch()
     n_total = 3 * W * H * V
@@ -352,15 +375,16 @@ def run_multiview(args) -> dict:
     def adj_batch(flags=abi.FF_RENDER_USE_GRAPH):
         g_alb.zero_()
         g_rough.zero_()
-        abi.adjoint_batch(ctx, items, paths, ds, tex_grads=[[g_alb, g_rough]] * V, flags=flags)
+        abi.adjoint_batch(ctx, items, paths, ds, tex_grads=[[g_alb, g_rough]] * V, flags=flags, spp_batch=kb)
 
     def separate():
         g_alb.zero_()
         g_rough.zero_()
         for k in range(V):
             films[k].zero_()
-            abi.render_paths(ctx, scene, env, cams[k], W, H, S, D, sd.seed, films[k], paths[k])
-            abi.adjoint(ctx, scene, env, cams[k], W, H, S, D, sd.seed, paths[k], ds[k], tex_grads=[g_alb, g_rough])
+            abi.render_paths(ctx, scene, env, cams[k], W, H, S, D, sd.seed, films[k], paths[k], spp_batch=kb)
+            abi.adjoint(ctx, scene, env, cams[k], W, H, S, D, sd.seed, paths[k], ds[k], tex_grads=[g_alb, g_rough],
+                        spp_batch=kb)
 
     t_fb = _timed(fwd_batch, args.steps + 1)
     st_f = abi.last_render_stats(ctx)
@@ -384,6 +408,7 @@ def run_multiview(args) -> dict:
         "forward_mrays_s": round(rays_f / t_fb / 1e6, 2),
         "adjoint_mrays_s": round(rays_a / t_ab / 1e6, 2),
         "tris": sd.n_tris,
+        "spp_batch": kb,
     }
     if not args.no_cpu:
         from oracle import oracle as O
@@ -429,6 +454,11 @@ def main() -> int:
     ap.add_argument("--adj-spp-batch", type=int, default=16)
     ap.add_argument("--views", type=int, default=32)
     ap.add_argument("--requests", type=int, default=64)
+    ap.add_argument("--mv-spp-batch", type=int, default=8,
+                    help="multiview workload: samples of each pixel in flight (batched and separate runs)")
+    ap.add_argument("--batch-spp-batch", type=int, default=8,
+                    help="batch workload: samples of each pixel in flight (both the batched call and the "
+                         "sequential renders use it, so the films stay bit-identical)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run a plain device copy of this many MiB first (PMC calibration)")

@@ -55,7 +55,7 @@ class BatchItem(C.Structure):
     _fields_ = [("scene", C.c_void_p), ("env", C.c_void_p), ("camera", Camera), ("width", C.c_int32),
                 ("height", C.c_int32), ("spp", C.c_int32), ("max_depth", C.c_int32), ("seed", C.c_uint32),
                 ("sample_offset", C.c_int32), ("film", C.c_void_p), ("path_radiance", C.c_void_p),
-                ("d_image", C.c_void_p), ("grads", C.c_void_p)]
+                ("d_image", C.c_void_p), ("grads", C.c_void_p), ("spp_batch", C.c_int32)]
 
 
 class RayStats(C.Structure):
@@ -295,16 +295,22 @@ def render(ctx: Context, scene: Scene, env: EnvMap, cam: Camera, width: int, hei
     check(_lib().ff_render(ctx.h, scene.h, env.h, C.byref(cam), C.byref(r), _ptr(film_dev), stream))
 
 
-def render_batch(ctx: Context, items, flags: int = 0, stream=None, path_radiance=None) -> None:
+def _per_item(v, n):
+    return list(v) if isinstance(v, (list, tuple)) else [v] * n
+
+
+def render_batch(ctx: Context, items, flags: int = 0, stream=None, path_radiance=None, spp_batch=1) -> None:
     """items: sequence of (scene, env, camera, width, height, spp, max_depth,
     seed, film_dev[, sample_offset]); path_radiance: optional per-item
-    [spp][h][w][4] device buffers for the adjoint."""
+    [spp][h][w][4] device buffers for the adjoint; spp_batch: samples of each
+    pixel in flight, one value or one per item."""
     arr = (BatchItem * len(items))()
+    kb = _per_item(spp_batch, len(items))
     for k, it in enumerate(items):
         sc, env, cam, w, h, spp, depth, seed, film = it[:9]
         off = it[9] if len(it) > 9 else 0
         pr = _ptr(path_radiance[k]) if path_radiance is not None else None
-        arr[k] = BatchItem(sc.h, env.h, cam, w, h, spp, depth, seed, off, _ptr(film), pr, None, None)
+        arr[k] = BatchItem(sc.h, env.h, cam, w, h, spp, depth, seed, off, _ptr(film), pr, None, None, kb[k])
     check(_lib().ff_render_batch(ctx.h, arr, len(items), flags, stream))
 
 
@@ -340,11 +346,12 @@ def adjoint(ctx: Context, scene: Scene, env: EnvMap, cam: Camera, width: int, he
 
 
 def adjoint_batch(ctx: Context, items, path_radiance, d_images, env_grads=None, tex_grads=None, flags: int = 0,
-                  stream=None) -> None:
+                  stream=None, spp_batch=1) -> None:
     """Batched adjoint of render_batch items; env_grads / tex_grads are per
     item (items may share buffers)."""
     n = len(items)
     arr = (BatchItem * n)()
+    kb = _per_item(spp_batch, n)
     keep = []
     for k, it in enumerate(items):
         sc, env, cam, w, h, spp, depth, seed = it[:8]
@@ -353,7 +360,7 @@ def adjoint_batch(ctx: Context, items, path_radiance, d_images, env_grads=None, 
                           tex_grads[k] if tex_grads is not None else None)
         keep.append((g, a))
         arr[k] = BatchItem(sc.h, env.h, cam, w, h, spp, depth, seed, off, None, _ptr(path_radiance[k]),
-                           _ptr(d_images[k]), C.cast(C.pointer(g), C.c_void_p))
+                           _ptr(d_images[k]), C.cast(C.pointer(g), C.c_void_p), kb[k])
     check(_lib().ff_adjoint_batch(ctx.h, arr, n, flags, stream))
     del keep
 

@@ -961,8 +961,9 @@ ff_status ff_render_batch(ff_context* ctx, const ff_batch_item* items, int32_t n
     reqs.reserve((size_t)n);
     for (int32_t i = 0; i < n; i++) {
         const ff_batch_item& it = items[i];
-        HostReq q{it.scene, it.env, it.camera, it.width, it.height, it.spp, it.max_depth, 1, it.sample_offset,
-                  it.seed, it.film};
+        HostReq q{it.scene, it.env, it.camera, it.width, it.height, it.spp, it.max_depth,
+                  it.spp_batch > 0 ? std::min(it.spp_batch, std::max(it.spp, 1)) : 1, it.sample_offset, it.seed,
+                  it.film};
         if (it.path_radiance) {
             q.mode = RQ_STORE_PATHS;
             q.path_L = it.path_radiance;
@@ -1011,8 +1012,9 @@ ff_status ff_adjoint_batch(ff_context* ctx, const ff_batch_item* items, int32_t 
     reqs.reserve((size_t)n);
     for (int32_t i = 0; i < n; i++) {
         const ff_batch_item& it = items[i];
-        HostReq q{it.scene, it.env, it.camera, it.width, it.height, it.spp, it.max_depth, 1, it.sample_offset,
-                  it.seed, nullptr};
+        HostReq q{it.scene, it.env, it.camera, it.width, it.height, it.spp, it.max_depth,
+                  it.spp_batch > 0 ? std::min(it.spp_batch, std::max(it.spp, 1)) : 1, it.sample_offset, it.seed,
+                  nullptr};
         q.mode = RQ_ADJOINT;
         q.path_L = it.path_radiance;
         q.d_img = it.d_image;

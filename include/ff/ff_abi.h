@@ -178,9 +178,9 @@ ff_status ff_generate_rays(ff_context *ctx, const ff_camera *cam, int32_t width,
 ff_status ff_render(ff_context *ctx, ff_scene *scene, ff_envmap *env, const ff_camera *cam,
                     const ff_render_req *req, float *film, void *stream);
 /* ---- batched requests: many scene/camera requests of uneven resolution,
- * spp and depth in one wavefront (one path slot per pixel of every request,
- * so each film pixel has a single writer; a request's film equals what
- * ff_render produces for it with spp_batch = 1).  All scenes must belong to
+ * spp and depth in one wavefront (spp_batch path slots per pixel of every
+ * request, 0 meaning 1; a request's film equals what ff_render produces for
+ * it with the same spp_batch).  All scenes must belong to
  * ctx; films must be distinct device [h][w][4] buffers (ADDED to, like
  * ff_render).  flags: FF_RENDER_USE_GRAPH captures one batch of launches in a
  * hipGraph and replays it once per sample index.  No reference counterpart:
@@ -197,6 +197,7 @@ typedef struct {
     float *path_radiance;     /* optional: [spp][h][w][4] radiance of every path  */
     const float *d_image;     /* ff_adjoint_batch: d loss / d image, [h][w][4]    */
     const ff_grads *grads;    /* ff_adjoint_batch: gradient buffers               */
+    int32_t spp_batch;        /* samples of each pixel in flight (0 = 1)          */
 } ff_batch_item;
 ff_status ff_render_batch(ff_context *ctx, const ff_batch_item *items, int32_t n, int32_t flags,
                           void *stream);

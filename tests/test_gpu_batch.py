@@ -29,9 +29,13 @@ def _requests():
     return out
 
 
-@pytest.mark.parametrize("flags", [0, abi.FF_RENDER_USE_GRAPH])
-def test_batch_equals_individual_renders(gpu, flags):
+@pytest.mark.parametrize("flags,spp_batch", [(0, 1), (abi.FF_RENDER_USE_GRAPH, 1), (0, [2, 4, 1, 3]),
+                                             (abi.FF_RENDER_USE_GRAPH, 3)])
+def test_batch_equals_individual_renders(gpu, flags, spp_batch):
+    """Each request's film is bit-identical to its plain render with the same
+    samples in flight (one value for all items, or one per item)."""
     import torch
+    kb = spp_batch if isinstance(spp_batch, list) else [spp_batch] * 4
     reqs = _requests()
     objs = []
     for sd in reqs:
@@ -39,14 +43,14 @@ def test_batch_equals_individual_renders(gpu, flags):
         env = abi.EnvMap(gpu, sd.env)
         objs.append((sd, sc, env, abi.make_camera(sd.inv_vp)))
     singles = []
-    for sd, sc, env, cam in objs:
+    for (sd, sc, env, cam), k in zip(objs, kb):
         f = torch.zeros((sd.height, sd.width, 4), dtype=torch.float32, device="cuda")
-        abi.render(gpu, sc, env, cam, sd.width, sd.height, sd.spp, sd.max_depth, sd.seed, f, spp_batch=1)
+        abi.render(gpu, sc, env, cam, sd.width, sd.height, sd.spp, sd.max_depth, sd.seed, f, spp_batch=k)
         singles.append(f)
     torch.cuda.synchronize()
     films = [torch.zeros((sd.height, sd.width, 4), dtype=torch.float32, device="cuda") for sd, *_ in objs]
     abi.render_batch(gpu, [(sc, env, cam, sd.width, sd.height, sd.spp, sd.max_depth, sd.seed, f)
-                           for (sd, sc, env, cam), f in zip(objs, films)], flags=flags)
+                           for (sd, sc, env, cam), f in zip(objs, films)], flags=flags, spp_batch=spp_batch)
     torch.cuda.synchronize()
     for (sd, *_), f, s in zip(objs, films, singles):
         a, b = f.cpu().numpy(), s.cpu().numpy()
