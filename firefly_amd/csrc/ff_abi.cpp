@@ -454,6 +454,10 @@ ff_status ff_scene_create(ff_context* ctx, const ff_scene_desc* d, ff_scene** ou
     std::vector<int32_t> leaf_count4;
     float root4[8];
     pack_four(s->nodes, four, leaf_count4, root4, n);
+    // the traversal addresses records by 32-bit byte offsets (64-B and 128-B records)
+    if (wide.size() / 4 >= (size_t(1) << 26) || four.size() / 8 >= (size_t(1) << 25))
+        return cleanup(fail(FF_ERR_CAPACITY, "ff_scene_create: %zu BVH records exceed the 2^25 4-wide / 2^26 "
+                                            "two-child limit of the traversal kernels", wide.size() / 4));
     if (dalloc(&s->d_nodes, wide.size()) != hipSuccess || dalloc(&s->d_nodes4, four.size()) != hipSuccess ||
         dalloc(&s->d_leaf_count, leaf_count.size()) != hipSuccess ||
         dalloc(&s->d_root, 8) != hipSuccess || dalloc(&s->d_tris, 9 * n) != hipSuccess ||
@@ -805,8 +809,11 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
     p.n_req = (uint32_t)table.size();
     p.n_slots = (uint32_t)n_slots;
     p.uni_scene = 1;
-    for (const HostReq& q : reqs)
+    p.uni_env = 1;
+    for (const HostReq& q : reqs) {
         if (q.s != reqs[0].s) p.uni_scene = 0;
+        if (q.e != reqs[0].e) p.uni_env = 0;
+    }
     p.g_max = ctx->g_max;
     p.g_cnt = ctx->g_cnt;
     p.g_fx = ctx->g_fx;

@@ -516,6 +516,8 @@ struct Stack {
 };
 
 struct Hit { float t, u, v; int id; };
+// pruning limit of the current closest hit
+__device__ __forceinline__ float prune_lim(float t) { return t + t * PRUNE_REL + PRUNE_ABS; }
 
 // Device BVH: the reference's binary tree (same nodes, same order) stored as
 // 64-byte two-child records so one fetch tests both children's boxes:
@@ -572,16 +574,17 @@ __device__ __forceinline__ bool trav_begin(const Bvh& B, f3 o, f3 d, TravRay& R,
 // dirIsNeg[axis]) is visited next and the far one pushed when both are hit.
 // Returns false when neither child is entered (the caller pops).
 template <bool PRUNE>
-__device__ __forceinline__ bool visit_interior(const Bvh& B, TravRay& R, float t_best, Stack& st,
+__device__ __forceinline__ bool visit_interior(const Bvh& B, TravRay& R, float lim, Stack& st,
                                                uint32_t& cnt_nodes) {
-    const float4* n = B.nodes + 4 * (size_t)R.ref;
+    // 32-bit byte offset (records < 2^26, checked at scene creation): the
+    // load adds it to the base without 64-bit address arithmetic
+    const float4* n = (const float4*)((const char*)B.nodes + ((uint32_t)R.ref << 6));
     float4 a = ldg4(n), b = ldg4(n + 1), c = ldg4(n + 2);
     int4 m = ldg4i((const int4*)(n + 3));
     cnt_nodes += 2;
     float tn0, tf0, tn1, tf1;
     bool h0, h1;
     if (PRUNE) {
-        float lim = t_best + t_best * PRUNE_REL + PRUNE_ABS;
         h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, R.o, R.inv, R.nx, R.ny, R.nz, tn0, tf0) &&
              !(tn0 > lim || tf0 < -PRUNE_BEHIND);
         h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, R.o, R.inv, R.nx, R.ny, R.nz, tn1, tf1) &&
@@ -623,13 +626,12 @@ __device__ __forceinline__ int sel4i(const int4& v, int k) {
     return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
 }
 template <bool PRUNE>
-__device__ __forceinline__ bool visit_interior4(const Bvh& B, TravRay& R, float t_best, Stack& st,
+__device__ __forceinline__ bool visit_interior4(const Bvh& B, TravRay& R, float lim, Stack& st,
                                                 uint32_t& cnt_nodes) {
-    const float4* n = B.nodes4 + 8 * (size_t)R.ref;
+    const float4* n = (const float4*)((const char*)B.nodes4 + ((uint32_t)R.ref << 7));
     float4 lx = ldg4(n), ly = ldg4(n + 1), lz = ldg4(n + 2);
     float4 hx = ldg4(n + 3), hy = ldg4(n + 4), hz = ldg4(n + 5);
     int4 rf = ldg4i((const int4*)(n + 6)), ax = ldg4i((const int4*)(n + 7));
-    float lim = t_best + t_best * PRUNE_REL + PRUNE_ABS;
     bool hit[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -660,9 +662,9 @@ __device__ __forceinline__ bool visit_interior4(const Bvh& B, TravRay& R, float 
     return false;
 }
 template <bool WIDE, bool PRUNE>
-__device__ __forceinline__ bool visit_node(const Bvh& B, TravRay& R, float t_best, Stack& st, uint32_t& cnt_nodes) {
-    return WIDE ? visit_interior4<PRUNE>(B, R, t_best, st, cnt_nodes)
-                : visit_interior<PRUNE>(B, R, t_best, st, cnt_nodes);
+__device__ __forceinline__ bool visit_node(const Bvh& B, TravRay& R, float lim, Stack& st, uint32_t& cnt_nodes) {
+    return WIDE ? visit_interior4<PRUNE>(B, R, lim, st, cnt_nodes)
+                : visit_interior<PRUNE>(B, R, lim, st, cnt_nodes);
 }
 
 // the triangles of the leaf R.ref (closest-hit test of every one)
@@ -697,7 +699,7 @@ __device__ __forceinline__ bool closest_step(const Bvh& B, const float* __restri
                                              bool two_sided, Stack& st, uint32_t& cnt_nodes,
                                              uint32_t& cnt_tris) {
     if (R.ref >= 0) {
-        if (visit_node<WIDE, true>(B, R, h.t, st, cnt_nodes)) return true;
+        if (visit_node<WIDE, true>(B, R, prune_lim(h.t), st, cnt_nodes)) return true;
     } else {
         leaf_closest(B, tris, R, h, two_sided, cnt_tris);
     }

@@ -33,6 +33,17 @@ __device__ __forceinline__ Bvh bvh_of(const ff_dev_scene& S) {
     return b;
 }
 
+// With UNI every request of the call shades the same scene and environment:
+// they are read from request 0, so their pointers stay in scalar registers.
+template <bool UNI>
+__device__ __forceinline__ const ff_dev_scene& scene_of(const ff_paths& P, const ff_dev_request& R) {
+    return UNI ? P.reqs[0].S : R.S;
+}
+template <bool UNI>
+__device__ __forceinline__ const ff_dev_env& env_of(const ff_paths& P, const ff_dev_request& R) {
+    return UNI ? P.reqs[0].E : R.E;
+}
+
 __device__ __forceinline__ EnvView env_view(const ff_dev_env& e) {
     EnvView v;
     v.rgb = e.rgb; v.marg_y = e.marg_y; v.conds = e.conds; v.pdf = e.pdf; v.w = e.w; v.h = e.h;
@@ -279,7 +290,7 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
                 bool at_int = busy && walk && R.ref >= 0;
                 bool seeking = busy && walk && pend == 0;
                 if (__ballot(at_int) == 0 || __ballot(seeking) == 0) break;
-                if (at_int && !visit_node<WIDE, !ANY>(B, R, h.t, st, cn)) {
+                if (at_int && !visit_node<WIDE, !ANY>(B, R, prune_lim(h.t), st, cn)) {
                     if (st.sp == 0) walk = false;
                     else R.ref = st.pop();
                 }
@@ -309,7 +320,7 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
             for (;;) {  // interior records until every busy lane is at a leaf
                 bool at_int = busy && R.ref >= 0;
                 if (__ballot(at_int) == 0) break;
-                if (at_int && !visit_node<WIDE, !ANY>(B, R, h.t, st, cn)) {
+                if (at_int && !visit_node<WIDE, !ANY>(B, R, prune_lim(h.t), st, cn)) {
                     if (st.sp == 0) {
                         if (ANY) src.occluded(id, false);
                         else src.hit(id, h);
@@ -383,6 +394,7 @@ __global__ void k_reset_counters(uint32_t* c, uint32_t mask, uint32_t* fetch, ui
 // ray's visibility, throughput update, Russian roulette for len > 3,
 // environment radiance for escaped rays after the camera (or a delta event),
 // termination into the film, routing of live paths to material queues.
+template <bool UNI>
 __global__ __launch_bounds__(FF_BLOCK) void k_logic(ff_paths P, int q) {
     __shared__ uint32_t lds[4 * (FF_BLOCK / 64 + 1)];
     const uint32_t n = P.counters[q == 0 ? C_EXT0 : C_EXT1];
@@ -396,7 +408,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_logic(ff_paths P, int q) {
         uint32_t id = live ? queue[i] : 0u;
         if (live) {
             const ff_dev_request& R = P.reqs[P.preq[id]];
-            const ff_dev_scene& S = R.S;
+            const ff_dev_scene& S = scene_of<UNI>(P, R);
             float4 bw = P.beta[id];
             uint32_t stw = __float_as_uint(bw.w);
             uint32_t len = (stw & ST_LEN_MASK) + 1;
@@ -416,7 +428,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_logic(ff_paths P, int q) {
                     if (!(vdot(d, vcross(T.e1, T.e2)) < 0.f)) term = true;  // absorbed
                 }
             }
-            if (!found && (stw & ST_PREV_DELTA)) Lp = vadd(Lp, vmul(beta, env_L(env_view(R.E), d)));
+            if (!found && (stw & ST_PREV_DELTA)) Lp = vadd(Lp, vmul(beta, env_L(env_view(env_of<UNI>(P, R)), d)));
             if (len > 1) {
                 bool visL = (stw & ST_NEED_L) && P.vis[3 * id + 0];
                 if (visL) {
@@ -499,11 +511,11 @@ struct ShadeOut {
 // sample of wf_logic (CRT/kernel.cu:435-439) is drawn here; the visibility
 // rays of BRDF_L's BRDF-sampled MIS terms (CRT/dMaterial.cu:404-448) are
 // deferred to the shadow stage with their contributions kept per path.
-template <int TYPE, bool GRAD>
+template <int TYPE, bool GRAD, bool UNI>
 __device__ __forceinline__ void shade_core(const ff_paths& P, uint32_t id, ShadeOut& so) {
     const ff_dev_request& R = P.reqs[P.preq[id]];
-    const ff_dev_scene& S = R.S;
-    EnvView E = env_view(R.E);
+    const ff_dev_scene& S = scene_of<UNI>(P, R);
+    EnvView E = env_view(env_of<UNI>(P, R));
     float4 bw = P.beta[id];
     uint32_t stw = __float_as_uint(bw.w);
     uint32_t len = stw & ST_LEN_MASK;
@@ -706,7 +718,7 @@ __device__ __forceinline__ void shade_core(const ff_paths& P, uint32_t id, Shade
              (so.needD ? ST_NEED_D : 0u);
 }
 
-template <int TYPE>
+template <int TYPE, bool UNI>
 __global__ __launch_bounds__(FF_BLOCK) void k_shade(ff_paths P, int next_q) {
     __shared__ uint32_t lds[2 * (FF_BLOCK / 64 + 1)];
     const uint32_t n = P.counters[C_MAT0 + TYPE];
@@ -720,7 +732,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade(ff_paths P, int next_q) {
         bool needL = false, needS = false, needD = false;
         if (live) {
             ShadeOut so;
-            shade_core<TYPE, false>(P, id, so);
+            shade_core<TYPE, false, UNI>(P, id, so);
             needL = so.needL; needS = so.needS; needD = so.needD;
             if (needL) {
                 P.sray[6 * id + 0] = f4(so.lo, 0.f);
@@ -813,6 +825,7 @@ __device__ __forceinline__ void grad_tex(const ff_paths& P, const ff_dev_request
 // wf_logic of the replay: same termination, Russian roulette and routing as
 // k_logic; an escaping delta/camera ray's Le term is replayed.  The vertex's
 // NEE terms and throughput update were replayed by k_shade_adj.
+template <bool UNI>
 __global__ __launch_bounds__(FF_BLOCK) void k_logic_adj(ff_paths P, int q) {
     __shared__ uint32_t lds[4 * (FF_BLOCK / 64 + 1)];
     const uint32_t n = P.counters[q == 0 ? C_EXT0 : C_EXT1];
@@ -826,7 +839,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_logic_adj(ff_paths P, int q) {
         uint32_t id = live ? queue[i] : 0u;
         if (live) {
             const ff_dev_request& R = P.reqs[P.preq[id]];
-            const ff_dev_scene& S = R.S;
+            const ff_dev_scene& S = scene_of<UNI>(P, R);
             float4 bw = P.beta[id];
             uint32_t stw = __float_as_uint(bw.w);
             uint32_t len = (stw & ST_LEN_MASK) + 1;
@@ -847,7 +860,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_logic_adj(ff_paths P, int q) {
                 }
             }
             if (!found && (stw & ST_PREV_DELTA)) {
-                Lrem = vsub(Lrem, vmul(beta, env_L(env_view(R.E), d)));
+                Lrem = vsub(Lrem, vmul(beta, env_L(env_view(env_of<UNI>(P, R)), d)));
                 grad_env(P, R, d, vmul(xyz(P.dlt[id]), beta));
             }
             if (!(stw & ST_DEAD) && len > 3) {
@@ -881,7 +894,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_logic_adj(ff_paths P, int q) {
 }
 
 // Replay of a vertex's shadow rays: the same rays k_shade queues.
-template <int TYPE>
+template <int TYPE, bool UNI>
 __global__ __launch_bounds__(FF_BLOCK) void k_shade_nee(ff_paths P) {
     __shared__ uint32_t lds[FF_BLOCK / 64 + 1];
     const uint32_t n = P.counters[C_MAT0 + TYPE];
@@ -895,7 +908,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade_nee(ff_paths P) {
         bool needL = false, needS = false, needD = false;
         if (live) {
             ShadeOut so;
-            shade_core<TYPE, false>(P, id, so);
+            shade_core<TYPE, false, UNI>(P, id, so);
             needL = so.needL; needS = so.needS; needD = so.needD;
             if (needL) {
                 P.sray[6 * id + 0] = f4(so.lo, 0.f);
@@ -926,7 +939,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade_nee(ff_paths P) {
 // NEE terms leave L_rem (with their gradients), then the continuation's
 // gradient L_rem * (df/dtheta) / f, the throughput update of wf_logic and the
 // next extension ray.
-template <int TYPE>
+template <int TYPE, bool UNI>
 __global__ __launch_bounds__(FF_BLOCK) void k_shade_adj(ff_paths P, int next_q) {
     __shared__ uint32_t lds[FF_BLOCK / 64 + 1];
     const uint32_t n = P.counters[C_MAT0 + TYPE];
@@ -939,7 +952,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade_adj(ff_paths P, int next_q) 
         if (live) {
             const ff_dev_request& R = P.reqs[P.preq[id]];
             ShadeOut so;
-            shade_core<TYPE, TYPE != MAT_DIELECTRIC>(P, id, so);
+            shade_core<TYPE, TYPE != MAT_DIELECTRIC, UNI>(P, id, so);
             f3 beta = xyz(P.beta[id]);
             uint32_t nst = so.nst;
             if (TYPE != MAT_DIELECTRIC) {
@@ -1243,7 +1256,9 @@ uint32_t resident_grid(K kernel, int bs) {
     X(14, 64, 32, 32, 0, true)   \
     X(15, 256, 16, 32, 1, true)  \
     X(16, 256, 16, 64, 2, false) \
-    X(17, 256, 16, 32, 2, false)
+    X(17, 256, 16, 32, 2, false) \
+    X(18, 256, 16, 8, 0, false)  \
+    X(19, 256, 24, 16, 0, false)
 
 // which: 0 extend, 1 shadow, 2 grid sizes only (before a graph capture),
 // 3 ff_trace_closest, 4 ff_trace_any
@@ -1310,6 +1325,9 @@ const int kTravWidth[] = {FF_TRAV_VARIANT_LIST(FF_TRAV_WIDTH)};
 #undef FF_TRAV_WW
 #undef FF_TRAV_WIDTH
 
+// every request of the call shades the same scene and environment
+static bool uni_shading(const ff_paths& P) { return P.uni_scene && P.uni_env; }
+
 hipError_t launch_batch_begin(const ff_paths& P, int set, uint32_t value, hipStream_t s) {
     hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(256), 0, s, P, set, value);
     return hipGetLastError();
@@ -1329,43 +1347,65 @@ hipError_t launch_reset(uint32_t* counters, uint32_t mask, uint32_t* fetch, uint
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(256), 0, s, counters, mask, fetch, fetch_mask);
     return hipGetLastError();
 }
-hipError_t launch_logic(const ff_paths& P, int q, uint32_t n_max, hipStream_t s) {
-    hipLaunchKernelGGL(k_logic, dim3(grid_for(n_max, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, P, q);
+template <bool UNI>
+hipError_t logic_launch(const ff_paths& P, int q, uint32_t n_max, hipStream_t s) {
+    hipLaunchKernelGGL(k_logic<UNI>, dim3(grid_for(n_max, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, P, q);
     return hipGetLastError();
 }
-hipError_t launch_shade(int type, const ff_paths& P, int next_q, uint32_t n_max, hipStream_t s) {
+hipError_t launch_logic(const ff_paths& P, int q, uint32_t n_max, hipStream_t s) {
+    return uni_shading(P) ? logic_launch<true>(P, q, n_max, s) : logic_launch<false>(P, q, n_max, s);
+}
+template <bool UNI>
+hipError_t shade_launch(int type, const ff_paths& P, int next_q, uint32_t n_max, hipStream_t s) {
     dim3 g(grid_for(n_max, FF_GRID_CAP)), b(FF_BLOCK);
     switch (type) {
-        case MAT_MIX: hipLaunchKernelGGL(k_shade<MAT_MIX>, g, b, 0, s, P, next_q); break;
-        case MAT_DIFFUSE: hipLaunchKernelGGL(k_shade<MAT_DIFFUSE>, g, b, 0, s, P, next_q); break;
-        case MAT_CONDUCTOR: hipLaunchKernelGGL(k_shade<MAT_CONDUCTOR>, g, b, 0, s, P, next_q); break;
-        default: hipLaunchKernelGGL(k_shade<MAT_DIELECTRIC>, g, b, 0, s, P, next_q); break;
+        case MAT_MIX: hipLaunchKernelGGL((k_shade<MAT_MIX, UNI>), g, b, 0, s, P, next_q); break;
+        case MAT_DIFFUSE: hipLaunchKernelGGL((k_shade<MAT_DIFFUSE, UNI>), g, b, 0, s, P, next_q); break;
+        case MAT_CONDUCTOR: hipLaunchKernelGGL((k_shade<MAT_CONDUCTOR, UNI>), g, b, 0, s, P, next_q); break;
+        default: hipLaunchKernelGGL((k_shade<MAT_DIELECTRIC, UNI>), g, b, 0, s, P, next_q); break;
     }
     return hipGetLastError();
 }
-hipError_t launch_logic_adj(const ff_paths& P, int q, uint32_t n_max, hipStream_t s) {
-    hipLaunchKernelGGL(k_logic_adj, dim3(grid_for(n_max, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, P, q);
+hipError_t launch_shade(int type, const ff_paths& P, int next_q, uint32_t n_max, hipStream_t s) {
+    return uni_shading(P) ? shade_launch<true>(type, P, next_q, n_max, s)
+                          : shade_launch<false>(type, P, next_q, n_max, s);
+}
+template <bool UNI>
+hipError_t logic_adj_launch(const ff_paths& P, int q, uint32_t n_max, hipStream_t s) {
+    hipLaunchKernelGGL(k_logic_adj<UNI>, dim3(grid_for(n_max, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, P, q);
     return hipGetLastError();
 }
-hipError_t launch_shade_nee(int type, const ff_paths& P, uint32_t n_max, hipStream_t s) {
+hipError_t launch_logic_adj(const ff_paths& P, int q, uint32_t n_max, hipStream_t s) {
+    return uni_shading(P) ? logic_adj_launch<true>(P, q, n_max, s) : logic_adj_launch<false>(P, q, n_max, s);
+}
+template <bool UNI>
+hipError_t shade_nee_launch(int type, const ff_paths& P, uint32_t n_max, hipStream_t s) {
     dim3 g(grid_for(n_max, FF_GRID_CAP)), b(FF_BLOCK);
     switch (type) {
-        case MAT_MIX: hipLaunchKernelGGL(k_shade_nee<MAT_MIX>, g, b, 0, s, P); break;
-        case MAT_DIFFUSE: hipLaunchKernelGGL(k_shade_nee<MAT_DIFFUSE>, g, b, 0, s, P); break;
-        case MAT_CONDUCTOR: hipLaunchKernelGGL(k_shade_nee<MAT_CONDUCTOR>, g, b, 0, s, P); break;
+        case MAT_MIX: hipLaunchKernelGGL((k_shade_nee<MAT_MIX, UNI>), g, b, 0, s, P); break;
+        case MAT_DIFFUSE: hipLaunchKernelGGL((k_shade_nee<MAT_DIFFUSE, UNI>), g, b, 0, s, P); break;
+        case MAT_CONDUCTOR: hipLaunchKernelGGL((k_shade_nee<MAT_CONDUCTOR, UNI>), g, b, 0, s, P); break;
         default: return hipSuccess;  // a dielectric vertex has no NEE rays
     }
     return hipGetLastError();
 }
-hipError_t launch_shade_adj(int type, const ff_paths& P, int next_q, uint32_t n_max, hipStream_t s) {
+hipError_t launch_shade_nee(int type, const ff_paths& P, uint32_t n_max, hipStream_t s) {
+    return uni_shading(P) ? shade_nee_launch<true>(type, P, n_max, s) : shade_nee_launch<false>(type, P, n_max, s);
+}
+template <bool UNI>
+hipError_t shade_adj_launch(int type, const ff_paths& P, int next_q, uint32_t n_max, hipStream_t s) {
     dim3 g(grid_for(n_max, FF_GRID_CAP)), b(FF_BLOCK);
     switch (type) {
-        case MAT_MIX: hipLaunchKernelGGL(k_shade_adj<MAT_MIX>, g, b, 0, s, P, next_q); break;
-        case MAT_DIFFUSE: hipLaunchKernelGGL(k_shade_adj<MAT_DIFFUSE>, g, b, 0, s, P, next_q); break;
-        case MAT_CONDUCTOR: hipLaunchKernelGGL(k_shade_adj<MAT_CONDUCTOR>, g, b, 0, s, P, next_q); break;
-        default: hipLaunchKernelGGL(k_shade_adj<MAT_DIELECTRIC>, g, b, 0, s, P, next_q); break;
+        case MAT_MIX: hipLaunchKernelGGL((k_shade_adj<MAT_MIX, UNI>), g, b, 0, s, P, next_q); break;
+        case MAT_DIFFUSE: hipLaunchKernelGGL((k_shade_adj<MAT_DIFFUSE, UNI>), g, b, 0, s, P, next_q); break;
+        case MAT_CONDUCTOR: hipLaunchKernelGGL((k_shade_adj<MAT_CONDUCTOR, UNI>), g, b, 0, s, P, next_q); break;
+        default: hipLaunchKernelGGL((k_shade_adj<MAT_DIELECTRIC, UNI>), g, b, 0, s, P, next_q); break;
     }
     return hipGetLastError();
+}
+hipError_t launch_shade_adj(int type, const ff_paths& P, int next_q, uint32_t n_max, hipStream_t s) {
+    return uni_shading(P) ? shade_adj_launch<true>(type, P, next_q, n_max, s)
+                          : shade_adj_launch<false>(type, P, next_q, n_max, s);
 }
 hipError_t launch_fixed_to_float(const unsigned long long* fx, const uint32_t* mx, const uint32_t* cnt, uint64_t n,
                                  float* out, hipStream_t s) {

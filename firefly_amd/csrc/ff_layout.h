@@ -163,6 +163,7 @@ struct ff_paths {
     uint32_t n_req;
     uint32_t n_slots;    // total slots over all requests
     int32_t uni_scene;   // every request of the call renders the same scene
+    int32_t uni_env;     // every request of the call uses the same environment
     // adjoint replay
     float4* dlt;         // [P]     d loss / d path radiance (xyz)
     uint32_t* g_max;     // arena: max |term| bits per gradient element (GM_BOUND)
