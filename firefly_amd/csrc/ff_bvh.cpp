@@ -16,8 +16,18 @@
 // Min/max use the reference host build's ternary fminf/fmaxf
 // (CRT/dMath.cuh:40-48) and Union's min/max-then-construct sequence
 // (CRT/BVH.cpp:26-50).
+//
+// Parallel build, same output: the two children of a node own disjoint
+// primitive ranges, so subtrees of large nodes are built by separate threads
+// into node lists of their own and spliced in depth-first order afterwards
+// (child offsets and leaf slots relocated).  Every node is computed by the
+// same code on the same range as in a one-thread build, so the node array and
+// slot order are bit-identical.
+#include <algorithm>
 #include <cfloat>
 #include <cstring>
+#include <future>
+#include <thread>
 #include <vector>
 
 #include "ff_internal.h"
@@ -79,15 +89,26 @@ public:
     void build(int start, int end) {
         int32_t idx = (int32_t)nodes_.size();
         nodes_.emplace_back();
-        Box bounds = Box::empty();
+        Box bounds;
+        int dim, mid;
+        if (!split(start, end, bounds, dim, mid)) return leaf(idx, start, end, bounds);
+        build(start, mid);
+        int32_t second = (int32_t)nodes_.size();
+        build(mid, end);
+        interior(nodes_[idx], node_box(idx + 1), node_box(second), second, dim);
+    }
+
+    // Bounds of [start, end) and, unless the range becomes a leaf (false),
+    // the split axis and the partition point (the range is partitioned).
+    bool split(int start, int end, Box& bounds, int& dim, int& mid) {
+        bounds = Box::empty();
         for (int i = start; i < end; i++) bounds = bounds.merged(prims_[i].box);
         int n = end - start;
-        if (n == 1) return leaf(idx, start, end, bounds);
+        if (n == 1) return false;
         Box cb = Box::empty();
         for (int i = start; i < end; i++) cb = cb.merged(prims_[i].c);
-        int dim = cb.widest();
-        if (cb.hi[dim] == cb.lo[dim]) return leaf(idx, start, end, bounds);
-        int mid;
+        dim = cb.widest();
+        if (cb.hi[dim] == cb.lo[dim]) return false;
         if (n <= 2) {
             mid = (start + end) / 2;
             if (prims_[start + 1].c[dim] < prims_[start].c[dim]) std::swap(prims_[start], prims_[start + 1]);
@@ -111,20 +132,35 @@ public:
                 float cost = (float)((double)0.125f + (c0 * b0.area() + c1 * b1.area()) / sa);
                 if (s == 0 || cost < best) { best = cost; best_b = s; }
             }
-            if (!(n > max_prims_ || best < (float)n)) return leaf(idx, start, end, bounds);
+            if (!(n > max_prims_ || best < (float)n)) return false;
             mid = partition(start, end, cb, dim, best_b);
         }
-        build(start, mid);
-        int32_t second = (int32_t)nodes_.size();
-        build(mid, end);
-        Box c0 = node_box(idx + 1), c1 = node_box(second);
+        return true;
+    }
+
+    static void interior(BvhNode& nd, const Box& c0, const Box& c1, int32_t second, int dim) {
         Box ib = c0.merged(c1);
-        BvhNode& nd = nodes_[idx];
         std::memcpy(nd.bmin, ib.lo, 12);
         std::memcpy(nd.bmax, ib.hi, 12);
         nd.offset = second;
         nd.n_prims = 0;
         nd.axis = (uint8_t)dim;
+        nd.pad = 0;
+    }
+    static Box box_of(const BvhNode& nd) {
+        Box b;
+        std::memcpy(b.lo, nd.bmin, 12);
+        std::memcpy(b.hi, nd.bmax, 12);
+        return b;
+    }
+    void leaf(int32_t idx, int start, int end, const Box& b) {
+        BvhNode& nd = nodes_[idx];
+        nd.offset = (int32_t)ordered_.size();
+        for (int i = start; i < end; i++) ordered_.push_back(prims_[i].id);
+        std::memcpy(nd.bmin, b.lo, 12);
+        std::memcpy(nd.bmax, b.hi, 12);
+        nd.n_prims = (uint16_t)(end - start);
+        nd.axis = 0;
         nd.pad = 0;
     }
 
@@ -153,28 +189,63 @@ private:
             ++first;
         }
     }
-    void leaf(int32_t idx, int start, int end, const Box& b) {
-        BvhNode& nd = nodes_[idx];
-        nd.offset = (int32_t)ordered_.size();
-        for (int i = start; i < end; i++) ordered_.push_back(prims_[i].id);
-        std::memcpy(nd.bmin, b.lo, 12);
-        std::memcpy(nd.bmax, b.hi, 12);
-        nd.n_prims = (uint16_t)(end - start);
-        nd.axis = 0;
-        nd.pad = 0;
-    }
-    Box node_box(int32_t i) const {
-        Box b;
-        std::memcpy(b.lo, nodes_[i].bmin, 12);
-        std::memcpy(b.hi, nodes_[i].bmax, 12);
-        return b;
-    }
+    Box node_box(int32_t i) const { return box_of(nodes_[i]); }
 
     std::vector<Prim>& prims_;
     int32_t max_prims_;
     std::vector<BvhNode>& nodes_;
     std::vector<int32_t>& ordered_;
 };
+
+// A subtree built on its own: node indices and leaf slots start at 0.
+struct SubTree {
+    std::vector<BvhNode> nodes;
+    std::vector<int32_t> ordered;
+};
+
+// Appends `t` (built on its own) to `out`, moving its interior offsets and
+// leaf slots to where it lands.
+void splice(SubTree& out, const SubTree& t) {
+    size_t nb = out.nodes.size(), ob = out.ordered.size();
+    out.nodes.resize(nb + t.nodes.size());
+    BvhNode* dst = out.nodes.data() + nb;
+    for (size_t i = 0; i < t.nodes.size(); i++) {
+        BvhNode nd = t.nodes[i];
+        nd.offset += nd.n_prims == 0 ? (int32_t)nb : (int32_t)ob;
+        dst[i] = nd;
+    }
+    out.ordered.insert(out.ordered.end(), t.ordered.begin(), t.ordered.end());
+}
+
+// Builds [start, end) into `out`; large ranges build their first child on
+// another thread (up to `spawn` more levels deep) while this one builds the
+// second.
+void build_parallel(std::vector<Prim>& prims, int32_t max_prims, int start, int end, int spawn, int min_par,
+                    SubTree& out) {
+    if (spawn <= 0 || end - start < min_par) {
+        Builder(prims, max_prims, out.nodes, out.ordered).build(start, end);
+        return;
+    }
+    Builder b(prims, max_prims, out.nodes, out.ordered);
+    out.nodes.emplace_back();
+    Box bounds;
+    int dim, mid;
+    if (!b.split(start, end, bounds, dim, mid)) {
+        b.leaf(0, start, end, bounds);
+        return;
+    }
+    SubTree left, right;
+    left.nodes.reserve((size_t)2 * (mid - start));
+    right.nodes.reserve((size_t)2 * (end - mid));
+    auto task = std::async(std::launch::async, build_parallel, std::ref(prims), max_prims, start, mid, spawn - 1,
+                           min_par, std::ref(left));
+    build_parallel(prims, max_prims, mid, end, spawn - 1, min_par, right);
+    task.get();
+    int32_t second = 1 + (int32_t)left.nodes.size();
+    Builder::interior(out.nodes[0], Builder::box_of(left.nodes[0]), Builder::box_of(right.nodes[0]), second, dim);
+    splice(out, left);
+    splice(out, right);
+}
 
 }  // namespace
 
@@ -184,16 +255,32 @@ void build_bvh_sah(const float* positions, int32_t n, int32_t max_prims, std::ve
     ordered.clear();
     if (n <= 0) return;
     std::vector<Prim> prims((size_t)n);
-    for (int32_t i = 0; i < n; i++) {
-        const float* t = positions + (size_t)i * 9;
-        Box b = Box::make(t, t + 3).merged(t + 6);
-        prims[i].id = i;
-        prims[i].box = b;
-        for (int k = 0; k < 3; k++) prims[i].c[k] = 0.5f * b.lo[k] + 0.5f * b.hi[k];
-    }
-    nodes.reserve((size_t)2 * n);
-    ordered.reserve((size_t)n);
-    Builder(prims, max_prims, nodes, ordered).build(0, n);
+    auto init = [&](int32_t lo, int32_t hi) {
+        for (int32_t i = lo; i < hi; i++) {
+            const float* t = positions + (size_t)i * 9;
+            Box b = Box::make(t, t + 3).merged(t + 6);
+            prims[i].id = i;
+            prims[i].box = b;
+            for (int k = 0; k < 3; k++) prims[i].c[k] = 0.5f * b.lo[k] + 0.5f * b.hi[k];
+        }
+    };
+    unsigned hw = std::thread::hardware_concurrency();
+    int n_init = n < 65536 ? 1 : (int)std::min(hw ? hw : 1u, 16u);
+    std::vector<std::thread> th;
+    for (int k = 1; k < n_init; k++)
+        th.emplace_back(init, (int32_t)((int64_t)n * k / n_init), (int32_t)((int64_t)n * (k + 1) / n_init));
+    init(0, (int32_t)((int64_t)n / n_init));
+    for (auto& x : th) x.join();
+    // subtree threads: up to 4x the cores (32 at most) for balance, none for
+    // ranges under 16384 primitives
+    int spawn = 0;
+    while ((1u << spawn) < 4u * (hw ? hw : 1u) && spawn < 5) spawn++;
+    SubTree t;
+    t.nodes.reserve((size_t)2 * n);
+    t.ordered.reserve((size_t)n);
+    build_parallel(prims, max_prims, 0, n, spawn, 16384, t);
+    nodes.swap(t.nodes);
+    ordered.swap(t.ordered);
 }
 
 int32_t bvh_max_depth(const std::vector<BvhNode>& nodes) {

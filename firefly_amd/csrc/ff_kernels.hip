@@ -364,9 +364,11 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
 }
 
 // The one-scene extend kernel of the if-if shape fits 7 waves per SIMD
-// (72 VGPRs) without spilling; the others keep the compiler's choice.
+// (72 VGPRs) without spilling where its LDS stack allows that many; the
+// others keep the compiler's choice.
 template <int BS, int LDS, int RF, int WW, bool WIDE, bool UNI>
-__global__ __launch_bounds__(BS, (UNI && WW == 0 && !WIDE) ? 7 : 1) void k_extend(ff_paths P, int q) {
+__global__ __launch_bounds__(BS, (UNI && WW == 0 && !WIDE && LDS <= 16 && BS >= 128) ? 7 : 1) void k_extend(
+    ff_paths P, int q) {
     __shared__ int smem[LDS * BS];
     ExtSrc<UNI> src{P, P.q_ext[q]};
     trav_persistent<BS, LDS, RF, false, WW, WIDE>(src, P.counters[q == 0 ? C_EXT0 : C_EXT1],
@@ -1059,7 +1061,8 @@ struct TraceArgs {  // one ff_trace_closest / ff_trace_any call
     unsigned long long* counts;
 };
 template <int BS, int LDS, int RF, int WW, bool WIDE>
-__global__ __launch_bounds__(BS, (WW == 0 && !WIDE) ? 7 : 1) void k_trace_closest(TraceArgs a) {
+__global__ __launch_bounds__(BS, (WW == 0 && !WIDE && LDS <= 16 && BS >= 128) ? 7 : 1) void k_trace_closest(
+    TraceArgs a) {
     __shared__ int smem[LDS * BS];
     RaySrc src{a.S, a.rays, a.hits, nullptr, a.two_sided};
     trav_persistent<BS, LDS, RF, false, WW, WIDE>(src, a.n, a.fetch, a.ovf, a.counts, smem);

@@ -90,3 +90,17 @@ def test_product_bvh_bit_identical_to_oracle(oracle):
         assert n_p.tobytes() == n_o.tobytes()
         assert np.array_equal(o_p, o_o)
         assert depth == oracle.bvh_max_depth(n_o)
+
+
+def test_parallel_product_bvh_bit_identical_to_oracle(oracle):
+    """Meshes large enough that the product builds subtrees on several
+    threads (ranges of 16384 primitives and more) and splices them: the node
+    array and slot order still equal the one-thread oracle build."""
+    ico = scenes.displaced_icosphere(6, (0, 1, 0), 1.0, 0.05, 3)[0].reshape(-1, 9)
+    cases = [ico, np.concatenate([ico, _soup(60000, 9).reshape(-1, 9)]), _soup(70000, 10, edge=0.05)]
+    for pos in cases:
+        n_o, o_o = oracle.bvh_build(pos)
+        n_p, o_p, depth = abi.bvh_build_host(pos)
+        assert len(n_o) > 60000
+        assert n_p.tobytes() == n_o.tobytes()
+        assert np.array_equal(o_p, o_o)
