@@ -147,8 +147,35 @@ struct EnvView {
     const double* marg_y;  // [h]
     const double* conds;   // [h][w]
     const float* pdf;      // [h][w]
+    const uint32_t* guide_m, *guide_c;
     int w, h;
 };
+// upper_bound_ref over a non-decreasing list, starting from its guide table:
+// the guide brackets the first element greater than val (the bracket of
+// val's bucket), so the loop below finds the same index as a search of the
+// whole list, and the trailing adjustment is applied to it unchanged.  Lists
+// that are not monotone (guide entries flagged) and values outside [0, 1)
+// are searched whole, exactly as upper_bound_ref does.
+__device__ __forceinline__ int upper_bound_guided(const double* __restrict__ list, int size,
+                                                  const uint32_t* __restrict__ guide, float val) {
+    int left = 0, right = size;
+    if (val >= 0.f && val < 1.f) {
+        uint32_t g = (uint32_t)(val * (float)ENV_GUIDE);  // exact: ENV_GUIDE is a power of two
+        if (g > ENV_GUIDE - 1) g = ENV_GUIDE - 1;
+        uint32_t a = guide[g], b = guide[g + 1];
+        if (((a | b) & ENV_GUIDE_FULL) == 0) {
+            left = (int)a;
+            right = (int)b;
+        }
+    }
+    while (left < right) {
+        int middle = (right - left) / 2 + left;
+        if ((double)val >= list[middle]) left = middle + 1;
+        else right = middle;
+    }
+    if (left < size && list[((float)left <= val) ? 1 : 0] != 0.0) left++;
+    return left;
+}
 // dEnvironmentLight::L (CRT/dEnvironmentLight.cu:66-76)
 __device__ __forceinline__ f3 env_L(const EnvView& e, f3 wi) {
     float u, v, o[3];
@@ -169,9 +196,10 @@ __device__ __forceinline__ float env_pdf(const EnvView& e, f3 wi) {
 // dEnvironmentLight::get_direction (CRT/dEnvironmentLight.cu:41-58);
 // indices clamped at 0 where the reference would index out of range.
 __device__ __forceinline__ f3 env_sample(const EnvView& e, f3 P, float ex, float ey) {
-    int y = (int)((float)upper_bound_ref(e.marg_y, e.h, ey) - 1.f);
+    int y = (int)((float)upper_bound_guided(e.marg_y, e.h, e.guide_m, ey) - 1.f);
     if (y < 0) y = 0;
-    int x = (int)((float)upper_bound_ref(e.conds + (size_t)y * e.w, e.w, ex) - 1.f);
+    int x = (int)((float)upper_bound_guided(e.conds + (size_t)y * e.w, e.w, e.guide_c + (size_t)y * (ENV_GUIDE + 1),
+                                            ex) - 1.f);
     if (x < 0) x = 0;
     f3 dir = sph_dir((float)x / (float)e.w, (float)y / (float)e.h);
     f3 spt = vadd(V3(0.f, 0.f, 0.f), vscl(dir, 10000.f));

@@ -47,6 +47,7 @@ __device__ __forceinline__ const ff_dev_env& env_of(const ff_paths& P, const ff_
 __device__ __forceinline__ EnvView env_view(const ff_dev_env& e) {
     EnvView v;
     v.rgb = e.rgb; v.marg_y = e.marg_y; v.conds = e.conds; v.pdf = e.pdf; v.w = e.w; v.h = e.h;
+    v.guide_m = e.guide_m; v.guide_c = e.guide_c;
     return v;
 }
 
@@ -1304,6 +1305,36 @@ hipError_t trav_dispatch(int which, int variant, const ff_paths* P, int q, const
     }
 }
 
+// Guide tables of the environment's inverse-CDF searches (block r < h: row
+// r of the conditional CDF, block h: the marginal).  Entry g is the upper
+// bound of g / ENV_GUIDE (first element greater, compared in double as the
+// sampler compares), so the upper bound of any val in [g, g+1) / ENV_GUIDE
+// lies in [entry g, entry g+1].  A list with a NaN or a decreasing step gets
+// flagged entries (searched whole).
+__global__ void k_env_guide(const double* marg_y, const double* conds, int32_t w, int32_t h, uint32_t* guide_m,
+                            uint32_t* guide_c) {
+    const bool marg = (int)blockIdx.x == h;
+    const double* list = marg ? marg_y : conds + (size_t)blockIdx.x * w;
+    const int size = marg ? h : w;
+    uint32_t* out = marg ? guide_m : guide_c + (size_t)blockIdx.x * (ENV_GUIDE + 1);
+    int mine = 0;
+    for (int i = threadIdx.x; i < size; i += blockDim.x) {
+        double v = list[i];
+        if (v != v || (i > 0 && v < list[i - 1])) mine = 1;
+    }
+    const int bad = __syncthreads_or(mine);
+    for (int g = threadIdx.x; g <= ENV_GUIDE; g += blockDim.x) {
+        float val = (float)g / (float)ENV_GUIDE;
+        int left = 0, right = size;
+        while (left < right) {
+            int middle = (right - left) / 2 + left;
+            if ((double)val >= list[middle]) left = middle + 1;
+            else right = middle;
+        }
+        out[g] = bad ? ENV_GUIDE_FULL : (uint32_t)left;
+    }
+}
+
 }  // namespace
 
 // ===================================================================== host
@@ -1452,12 +1483,14 @@ hipError_t launch_gen_rays(const ff_dev_camera& cam, int W, int H, uint32_t seed
     return hipGetLastError();
 }
 hipError_t launch_env_build(const float* rgb, int w, int h, float* lum, double* row_sum, double* denom,
-                            double* marg_p, double* marg_y, double* conds, float* pdf, hipStream_t s) {
+                            double* marg_p, double* marg_y, double* conds, float* pdf, uint32_t* guide_m,
+                            uint32_t* guide_c, hipStream_t s) {
     hipLaunchKernelGGL(k_env_lum, dim3(grid_for((uint64_t)w * h, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, rgb, w, h,
                        lum);
     hipLaunchKernelGGL(k_env_rows, dim3(h), dim3(FF_BLOCK), 0, s, lum, w, h, row_sum);
     hipLaunchKernelGGL(k_env_marginal, dim3(1), dim3(FF_BLOCK), 0, s, lum, row_sum, w, h, denom, marg_p, marg_y);
     hipLaunchKernelGGL(k_env_conditional, dim3(h), dim3(FF_BLOCK), 0, s, lum, w, h, denom, marg_p, conds, pdf);
+    hipLaunchKernelGGL(k_env_guide, dim3(h + 1), dim3(FF_BLOCK), 0, s, marg_y, conds, w, h, guide_m, guide_c);
     return hipGetLastError();
 }
 hipError_t launch_film_resolve(const float4* film, uint32_t n, float exposure, float* avg, uint8_t* rgba,

@@ -91,11 +91,19 @@ struct ff_dev_scene {
     int32_t n_nodes, n_tris, n_mats, n_tex, max_depth;
 };
 
+// Guide tables of the inverse-CDF searches: for bucket g of [0, 1) (width
+// 1 / ENV_GUIDE) the first and last candidate index of the upper bound.
+// Entry with bit 31 set: the list is not monotone (NaN rows), search it whole.
+#define ENV_GUIDE 64
+#define ENV_GUIDE_FULL 0x80000000u
+
 struct ff_dev_env {
     const float* rgb;
     const double* marg_y;
     const double* conds;
     const float* pdf;
+    const uint32_t* guide_m;  // [ENV_GUIDE + 1] over marg_y
+    const uint32_t* guide_c;  // [h][ENV_GUIDE + 1] over the rows of conds
     int32_t w, h;
 };
 
