@@ -94,7 +94,7 @@ struct ff_dev_scene {
 // Guide tables of the inverse-CDF searches: for bucket g of [0, 1) (width
 // 1 / ENV_GUIDE) the first and last candidate index of the upper bound.
 // Entry with bit 31 set: the list is not monotone (NaN rows), search it whole.
-#define ENV_GUIDE 64
+#define ENV_GUIDE 256
 #define ENV_GUIDE_FULL 0x80000000u
 
 struct ff_dev_env {
