@@ -41,15 +41,118 @@ constexpr float PRUNE_ABS = 1e-6f;
 constexpr float PRUNE_BEHIND = 1e-4f;
 
 // Transcendentals are evaluated in double and rounded to float: the CUDA
-// math library of the reference is closed, and this makes the float results
-// reproducible between host and device (they differ only when a double result
-// lies within an ulp of a float rounding boundary).
-__device__ __forceinline__ float f_sin(float x) { return (float)sin((double)x); }
-__device__ __forceinline__ float f_cos(float x) { return (float)cos((double)x); }
-__device__ __forceinline__ float f_atan(float x) { return (float)atan((double)x); }
-__device__ __forceinline__ float f_atan2(float y, float x) { return (float)atan2((double)y, (double)x); }
-__device__ __forceinline__ float f_asin(float x) { return (float)asin((double)x); }
-__device__ __forceinline__ float f_acos(float x) { return (float)acos((double)x); }
+// math library of the reference is closed.  The double routines below are
+// this project's own, and the host oracle evaluates the same operations in
+// the same order, so host and device agree bit for bit; they are within a
+// few ulp of the C library in double, so after rounding to float they nearly
+// always equal the correctly rounded value.  They need far fewer registers
+// than general library routines (the shading kernels' occupancy depends on
+// it); arguments outside their reduced ranges (huge or infinite) go to the
+// library, out of line.
+//
+// sin and cos: Cody-Waite reduction modulo pi/2 with a three-part split of
+// pi/2 (33 + 33 + 53 bits, so k * part is exact for |k| < 2^20), then the
+// Taylor series of sin to r^17 and of cos to r^18 on |r| <= pi/4 (first
+// omitted terms below 1e-19; coefficients 1/n! rounded to double).
+__device__ __noinline__ void sincos_lib(double x, double* s, double* c) {
+    *s = sin(x);
+    *c = cos(x);
+}
+__device__ __forceinline__ void sincos_d(double x, double& s, double& c) {
+    if (!(fabs(x) < 0x1p19)) {
+        sincos_lib(x, &s, &c);
+        return;
+    }
+    if (x == 0.0) {  // keeps the sign of a zero
+        s = x;
+        c = 1.0;
+        return;
+    }
+    double kf = rint(x * 0x1.45f306dc9c883p-1);  // x * 2/pi
+    int q = (int)kf & 3;
+    double r = ((x - kf * 0x1.921fb544p+0) - kf * 0x1.0b4611a6p-34) - kf * 0x1.3198a2e037073p-69;
+    double z = r * r;
+    double ps = 0x1.952c77030ad4ap-49;
+    ps = -0x1.ae7f3e733b81fp-41 + z * ps;
+    ps = 0x1.6124613a86d09p-33 + z * ps;
+    ps = -0x1.ae64567f544e4p-26 + z * ps;
+    ps = 0x1.71de3a556c734p-19 + z * ps;
+    ps = -0x1.a01a01a01a01ap-13 + z * ps;
+    ps = 0x1.1111111111111p-7 + z * ps;
+    ps = -0x1.5555555555555p-3 + z * ps;
+    double pc = -0x1.6827863b97d97p-53;
+    pc = 0x1.ae7f3e733b81fp-45 + z * pc;
+    pc = -0x1.93974a8c07c9dp-37 + z * pc;
+    pc = 0x1.1eed8eff8d898p-29 + z * pc;
+    pc = -0x1.27e4fb7789f5cp-22 + z * pc;
+    pc = 0x1.a01a01a01a01ap-16 + z * pc;
+    pc = -0x1.6c16c16c16c17p-10 + z * pc;
+    pc = 0x1.5555555555555p-5 + z * pc;
+    pc = -0.5 + z * pc;
+    double sr = r + (r * z) * ps, cr = 1.0 + z * pc;
+    s = q == 0 ? sr : (q == 1 ? cr : (q == 2 ? -sr : -cr));
+    c = q == 0 ? cr : (q == 1 ? -sr : (q == 2 ? -cr : sr));
+}
+// atan: odd symmetry; |x| > 1 by atan x = pi/2 - atan(1/x); t > tan(pi/12)
+// by the addition formula atan t = pi/6 + atan((t*sqrt3 - 1)/(t + sqrt3));
+// then the Taylor series to u^27 on |u| <= tan(pi/12) (first omitted term
+// below 1e-18; coefficients (-1)^n/(2n+1) rounded to double).
+__device__ __forceinline__ double atan_d(double x) {
+    if (x != x) return x;
+    double a = fabs(x);
+    bool inv = a > 1.0;
+    double t = inv ? 1.0 / a : a;
+    bool shift = t > 0x1.126145e9ecd56p-2;
+    double u = shift ? (t * 0x1.bb67ae8584caap+0 - 1.0) / (t + 0x1.bb67ae8584caap+0) : t;
+    double z = u * u;
+    double p = -0x1.2f684bda12f68p-5;
+    p = 0x1.47ae147ae147bp-5 + z * p;
+    p = -0x1.642c8590b2164p-5 + z * p;
+    p = 0x1.8618618618618p-5 + z * p;
+    p = -0x1.af286bca1af28p-5 + z * p;
+    p = 0x1.e1e1e1e1e1e1ep-5 + z * p;
+    p = -0x1.1111111111111p-4 + z * p;
+    p = 0x1.3b13b13b13b14p-4 + z * p;
+    p = -0x1.745d1745d1746p-4 + z * p;
+    p = 0x1.c71c71c71c71cp-4 + z * p;
+    p = -0x1.2492492492492p-3 + z * p;
+    p = 0x1.999999999999ap-3 + z * p;
+    p = -0x1.5555555555555p-2 + z * p;
+    double r = u + (u * z) * p;
+    if (shift) r = 0x1.0c152382d7366p-1 + r;
+    if (inv) r = 0x1.921fb54442d18p+0 - r;
+    return copysign(r, x);
+}
+__device__ __noinline__ double atan2_lib(double y, double x) { return atan2(y, x); }
+// atan2 with the C library's conventions for zeros and signs
+__device__ __forceinline__ double atan2_d(double y, double x) {
+    if (y != y || x != x) return x + y;
+    if (isinf(x) || isinf(y)) return atan2_lib(y, x);
+    if (x == 0.0) {
+        if (y == 0.0) return signbit(x) ? copysign(0x1.921fb54442d18p+1, y) : y;
+        return copysign(0x1.921fb54442d18p+0, y);
+    }
+    double r = atan_d(y / x);
+    if (x > 0.0) return r;
+    return signbit(y) ? r - 0x1.921fb54442d18p+1 : r + 0x1.921fb54442d18p+1;
+}
+// asin and acos through atan2 of (x, sqrt((1 - x)(1 + x)))
+__device__ __forceinline__ double asin_d(double x) {
+    if (!(fabs(x) <= 1.0)) return (x - x) / (x - x);
+    return atan2_d(x, sqrt((1.0 - x) * (1.0 + x)));
+}
+__device__ __forceinline__ double acos_d(double x) {
+    if (!(fabs(x) <= 1.0)) return (x - x) / (x - x);
+    return atan2_d(sqrt((1.0 - x) * (1.0 + x)), x);
+}
+__device__ __forceinline__ double sin_d(double x) { double s, c; sincos_d(x, s, c); return s; }
+__device__ __forceinline__ double cos_d(double x) { double s, c; sincos_d(x, s, c); return c; }
+__device__ __forceinline__ float f_sin(float x) { return (float)sin_d((double)x); }
+__device__ __forceinline__ float f_cos(float x) { return (float)cos_d((double)x); }
+__device__ __forceinline__ float f_atan(float x) { return (float)atan_d((double)x); }
+__device__ __forceinline__ float f_atan2(float y, float x) { return (float)atan2_d((double)y, (double)x); }
+__device__ __forceinline__ float f_asin(float x) { return (float)asin_d((double)x); }
+__device__ __forceinline__ float f_acos(float x) { return (float)acos_d((double)x); }
 
 // --------------------------------------------------------------- vectors --
 struct f3 { float x, y, z; };
@@ -312,7 +415,10 @@ __device__ __forceinline__ f3 diff_sample(f3 N, float e0, float e1) {
 __device__ __forceinline__ f3 spec_sample(f3 N, f3 wo, float r, float e0, float e1) {
     double theta = (double)f_atan(r * r * sqrtf(e0 / (1.f - e0)));
     double phi = (double)((2 * F_PI) * e1);
-    f3 h = V3((float)(sin(theta) * cos(phi)), (float)cos(theta), (float)(sin(theta) * sin(phi)));
+    double st, ct, sp, cp;
+    sincos_d(theta, st, ct);
+    sincos_d(phi, sp, cp);
+    f3 h = V3((float)(st * cp), (float)ct, (float)(st * sp));
     f3 T = vnorm(vcross(N, ortho_axis(N)));
     f3 B = vnorm(vcross(N, T));
     f3 smp = vadd(vadd(vscl(T, h.x), vscl(N, h.y)), vscl(B, h.z));

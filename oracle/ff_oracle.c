@@ -70,12 +70,118 @@
  * library of the reference is closed, and this makes the float results
  * reproducible between this oracle and the GPU (they differ only when a
  * double result lies within an ulp of a float rounding boundary). */
-static inline float f_sin(float x) { return (float)sin((double)x); }
-static inline float f_cos(float x) { return (float)cos((double)x); }
-static inline float f_atan(float x) { return (float)atan((double)x); }
-static inline float f_atan2(float y, float x) { return (float)atan2((double)y, (double)x); }
-static inline float f_asin(float x) { return (float)asin((double)x); }
-static inline float f_acos(float x) { return (float)acos((double)x); }
+/* Transcendentals in double, rounded to float, by the same operations in the
+ * same order as the GPU's routines (so host and device agree bit for bit):
+ * sin/cos by Cody-Waite reduction modulo pi/2 (three-part split of pi/2,
+ * 33 + 33 + 53 bits, derived from Machin's formula) and Taylor series to
+ * r^17 / r^18; atan by 1/x for |x| > 1, the addition formula at pi/6 above
+ * tan(pi/12) and the Taylor series to u^27; atan2 with the C library's zero
+ * and sign conventions; asin/acos as atan2 of (x, sqrt((1-x)(1+x))).
+ * Arguments outside the reduced ranges (|x| >= 2^19 for sin/cos, infinities
+ * for atan2) use the C library.  Pinned against the C library in
+ * tests/test_oracle_math.py. */
+void or_sincos_d(double x, double *s, double *c) {
+    if (!(fabs(x) < 0x1p19)) {
+        *s = sin(x);
+        *c = cos(x);
+        return;
+    }
+    if (x == 0.0) {
+        *s = x;
+        *c = 1.0;
+        return;
+    }
+    double kf = rint(x * 0x1.45f306dc9c883p-1);
+    int q = (int)kf & 3;
+    double r = ((x - kf * 0x1.921fb544p+0) - kf * 0x1.0b4611a6p-34) - kf * 0x1.3198a2e037073p-69;
+    double z = r * r;
+    double ps = 0x1.952c77030ad4ap-49;
+    ps = -0x1.ae7f3e733b81fp-41 + z * ps;
+    ps = 0x1.6124613a86d09p-33 + z * ps;
+    ps = -0x1.ae64567f544e4p-26 + z * ps;
+    ps = 0x1.71de3a556c734p-19 + z * ps;
+    ps = -0x1.a01a01a01a01ap-13 + z * ps;
+    ps = 0x1.1111111111111p-7 + z * ps;
+    ps = -0x1.5555555555555p-3 + z * ps;
+    double pc = -0x1.6827863b97d97p-53;
+    pc = 0x1.ae7f3e733b81fp-45 + z * pc;
+    pc = -0x1.93974a8c07c9dp-37 + z * pc;
+    pc = 0x1.1eed8eff8d898p-29 + z * pc;
+    pc = -0x1.27e4fb7789f5cp-22 + z * pc;
+    pc = 0x1.a01a01a01a01ap-16 + z * pc;
+    pc = -0x1.6c16c16c16c17p-10 + z * pc;
+    pc = 0x1.5555555555555p-5 + z * pc;
+    pc = -0.5 + z * pc;
+    double sr = r + (r * z) * ps, cr = 1.0 + z * pc;
+    *s = q == 0 ? sr : (q == 1 ? cr : (q == 2 ? -sr : -cr));
+    *c = q == 0 ? cr : (q == 1 ? -sr : (q == 2 ? -cr : sr));
+}
+double or_atan_d(double x) {
+    if (x != x) return x;
+    double a = fabs(x);
+    int inv = a > 1.0;
+    double t = inv ? 1.0 / a : a;
+    int shift = t > 0x1.126145e9ecd56p-2;
+    double u = shift ? (t * 0x1.bb67ae8584caap+0 - 1.0) / (t + 0x1.bb67ae8584caap+0) : t;
+    double z = u * u;
+    double p = -0x1.2f684bda12f68p-5;
+    p = 0x1.47ae147ae147bp-5 + z * p;
+    p = -0x1.642c8590b2164p-5 + z * p;
+    p = 0x1.8618618618618p-5 + z * p;
+    p = -0x1.af286bca1af28p-5 + z * p;
+    p = 0x1.e1e1e1e1e1e1ep-5 + z * p;
+    p = -0x1.1111111111111p-4 + z * p;
+    p = 0x1.3b13b13b13b14p-4 + z * p;
+    p = -0x1.745d1745d1746p-4 + z * p;
+    p = 0x1.c71c71c71c71cp-4 + z * p;
+    p = -0x1.2492492492492p-3 + z * p;
+    p = 0x1.999999999999ap-3 + z * p;
+    p = -0x1.5555555555555p-2 + z * p;
+    double r = u + (u * z) * p;
+    if (shift) r = 0x1.0c152382d7366p-1 + r;
+    if (inv) r = 0x1.921fb54442d18p+0 - r;
+    return copysign(r, x);
+}
+double or_atan2_d(double y, double x) {
+    if (y != y || x != x) return x + y;
+    if (isinf(x) || isinf(y)) return atan2(y, x);
+    if (x == 0.0) {
+        if (y == 0.0) return signbit(x) ? copysign(0x1.921fb54442d18p+1, y) : y;
+        return copysign(0x1.921fb54442d18p+0, y);
+    }
+    double r = or_atan_d(y / x);
+    if (x > 0.0) return r;
+    return signbit(y) ? r - 0x1.921fb54442d18p+1 : r + 0x1.921fb54442d18p+1;
+}
+double or_asin_d(double x) {
+    if (!(fabs(x) <= 1.0)) return (x - x) / (x - x);
+    return or_atan2_d(x, sqrt((1.0 - x) * (1.0 + x)));
+}
+double or_acos_d(double x) {
+    if (!(fabs(x) <= 1.0)) return (x - x) / (x - x);
+    return or_atan2_d(sqrt((1.0 - x) * (1.0 + x)), x);
+}
+void or_math_array(int32_t fn, const double *x, const double *y, int32_t n, double *out) {
+    for (int32_t i = 0; i < n; i++) {
+        double s, c;
+        switch (fn) {
+            case 0: or_sincos_d(x[i], &s, &c); out[i] = s; break;
+            case 1: or_sincos_d(x[i], &s, &c); out[i] = c; break;
+            case 2: out[i] = or_atan_d(x[i]); break;
+            case 3: out[i] = or_atan2_d(y[i], x[i]); break;
+            case 4: out[i] = or_asin_d(x[i]); break;
+            default: out[i] = or_acos_d(x[i]); break;
+        }
+    }
+}
+static inline double sin_d(double x) { double s, c; or_sincos_d(x, &s, &c); return s; }
+static inline double cos_d(double x) { double s, c; or_sincos_d(x, &s, &c); return c; }
+static inline float f_sin(float x) { return (float)sin_d((double)x); }
+static inline float f_cos(float x) { return (float)cos_d((double)x); }
+static inline float f_atan(float x) { return (float)or_atan_d((double)x); }
+static inline float f_atan2(float y, float x) { return (float)or_atan2_d((double)y, (double)x); }
+static inline float f_asin(float x) { return (float)or_asin_d((double)x); }
+static inline float f_acos(float x) { return (float)or_acos_d((double)x); }
 
 typedef struct { float x, y, z; } v3;
 static inline v3 V3(float a, float b, float c) { v3 r; r.x = a; r.y = b; r.z = c; return r; }
@@ -947,7 +1053,10 @@ static v3 diff_sample(v3 N, float e0, float e1) {
 static v3 spec_sample(v3 N, v3 wo, float r, float e0, float e1) {
     double theta = (double)f_atan(r * r * sqrtf(e0 / (1.f - e0)));
     double phi = (double)((2 * F_PI) * e1);
-    v3 h = V3((float)(sin(theta) * cos(phi)), (float)cos(theta), (float)(sin(theta) * sin(phi)));
+    double st, ct, sp, cp;
+    or_sincos_d(theta, &st, &ct);
+    or_sincos_d(phi, &sp, &cp);
+    v3 h = V3((float)(st * cp), (float)ct, (float)(st * sp));
     v3 T = vnorm(vcross(N, ortho_axis(N)));
     v3 B = vnorm(vcross(N, T));
     v3 smp = vadd(vadd(vscl(T, h.x), vscl(N, h.y)), vscl(B, h.z));

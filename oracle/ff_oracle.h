@@ -95,6 +95,15 @@ void or_trace_any(const or_node *nodes, const float *tris, const float *rays,
                   int32_t n, uint8_t *occluded);
 void or_trace_any_brute(int32_t n_tris, const float *tris, const float *rays,
                         int32_t n, uint8_t *occluded);
+/* the double transcendentals the oracle and the GPU share, element-wise:
+ * fn 0 sin, 1 cos, 2 atan, 3 atan2(y, x), 4 asin, 5 acos (y used by atan2 only) */
+void or_sincos_d(double x, double *s, double *c);
+double or_atan_d(double x);
+double or_atan2_d(double y, double x);
+double or_asin_d(double x);
+double or_acos_d(double x);
+void or_math_array(int32_t fn, const double *x, const double *y, int32_t n, double *out);
+
 /* width 2: two child boxes per interior node; width 4: the four grandchild
  * boxes per interior node (the 4-wide records) */
 void or_count_traversal(const or_node *nodes, const float *tris, const float *rays,

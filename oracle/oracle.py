@@ -59,6 +59,7 @@ def lib() -> C.CDLL:
             "or_trace_closest_brute": (None, [C.c_int32, _P, _P, C.c_int32, C.c_int32, _P, _P]),
             "or_trace_any": (None, [_P, _P, _P, C.c_int32, _P]),
             "or_trace_any_brute": (None, [C.c_int32, _P, _P, C.c_int32, _P]),
+            "or_math_array": (None, [C.c_int32, _P, _P, C.c_int32, _P]),
             "or_count_traversal": (None, [_P, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P]),
             "or_trace_walk": (None, [_P, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P]),
             "or_env_build": (None, [_P, C.c_int32, C.c_int32, _P, _P, _P, _P, _P]),
@@ -152,6 +153,19 @@ def trace_any_brute(tris, rays):
     o = np.zeros(r.shape[0], np.uint8)
     lib().or_trace_any_brute(tris.shape[0], tris.ctypes.data, r.ctypes.data, r.shape[0], o.ctypes.data)
     return o
+
+
+MATH_FNS = {"sin": 0, "cos": 1, "atan": 2, "atan2": 3, "asin": 4, "acos": 5}
+
+
+def math_d(fn, x, y=None):
+    """The oracle's (and the GPU's) double transcendental `fn` element-wise;
+    atan2 takes (y, x)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    yy = np.ascontiguousarray(x if y is None else y, dtype=np.float64)
+    out = np.zeros_like(x)
+    lib().or_math_array(MATH_FNS[fn], x.ctypes.data, yy.ctypes.data, x.size, out.ctypes.data)
+    return out
 
 
 def count_traversal(nodes, tris, rays, any_hit=False, prune=True, width=2):
