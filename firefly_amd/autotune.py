@@ -31,7 +31,7 @@ def _time_render(ctx, scene, env, cam, w, h, spp, depth, seed, film, spp_batch, 
 def autotune(ctx: abi.Context, scene: abi.Scene, env: abi.EnvMap, cam: abi.Camera, width: int, height: int,
              max_depth: int, seed: int = 0, probe_spp: int = 2, batches: Iterable[int] = (1, 2),
              reps: int = 2, out_json: Optional[str] = None, refine_batches: Iterable[int] = (),
-             refine_spp: int = 0) -> Dict:
+             refine_spp: int = 0, finalists: int = 4) -> Dict:
     """Pick the fastest traversal variant and spp_batch for this request.
 
     Each candidate renders `probe_spp` samples per pixel (timed, best of
@@ -39,8 +39,10 @@ def autotune(ctx: abi.Context, scene: abi.Scene, env: abi.EnvMap, cam: abi.Camer
     and times those larger spp_batch values (and the first stage's choice)
     at `refine_spp` samples per pixel: more paths in flight mean fewer
     traversal launches, each paying the latency of its slowest ray once.
-    The chosen variant is set on `ctx`; returns a dict with the choice and
-    every timing."""
+    A third stage then re-times the `finalists` fastest variants of the first
+    stage at the chosen spp_batch and `refine_spp` (the short probes rank
+    close variants unreliably for long renders).  The chosen variant is set
+    on `ctx`; returns a dict with the choice and every timing."""
     import torch
     film = torch.zeros((height, width, 4), dtype=torch.float32, device="cuda")
     results = []
@@ -65,6 +67,21 @@ def autotune(ctx: abi.Context, scene: abi.Scene, env: abi.EnvMap, cam: abi.Camer
             stage2.append(dict(best, spp_batch=k, seconds=t, probe_spp=refine_spp))
         results += stage2
         best = min(stage2, key=lambda r: r["seconds"])
+        ranked, seen = [], set()
+        for r in sorted(results[:len(results) - len(stage2)], key=lambda r: r["seconds"]):
+            if r["variant"] not in seen:
+                seen.add(r["variant"])
+                ranked.append(r)
+        stage3 = [best]
+        for r in ranked[:finalists]:
+            if r["variant"] == best["variant"]:
+                continue
+            abi.set_traversal_variant(ctx, r["variant"])
+            t = _time_render(ctx, scene, env, cam, width, height, refine_spp, max_depth, seed, film,
+                             best["spp_batch"], reps)
+            stage3.append(dict(r, spp_batch=best["spp_batch"], seconds=t, probe_spp=refine_spp))
+        results += stage3[1:]
+        best = min(stage3, key=lambda r: r["seconds"])
     abi.set_traversal_variant(ctx, best["variant"])
     out = {"best": best, "candidates": results,
            "request": {"width": width, "height": height, "max_depth": max_depth, "probe_spp": probe_spp,

@@ -423,7 +423,9 @@ __global__ __launch_bounds__(FF_BLOCK) void k_logic(ff_paths P, int q) {
             bool found = slot >= 0;
             term = (len > R.depth) || !found;
             int mtype = -1;
-            f3 d = xyz(P.ray[2 * id + 1]);
+            // the direction is read only by the two cases that use it
+            f3 d = V3(0.f, 0.f, 0.f);
+            if ((found && (stw & ST_INSIDE)) || (!found && (stw & ST_PREV_DELTA))) d = xyz(P.ray[2 * id + 1]);
             if (found) {
                 mtype = (int)S.mats[(size_t)S.mat[slot] * FF_MAT_FLOATS_DEV];
                 if ((stw & ST_INSIDE) && mtype != MAT_DIELECTRIC) {
@@ -440,13 +442,11 @@ __global__ __launch_bounds__(FF_BLOCK) void k_logic(ff_paths P, int q) {
                     if ((stw & ST_NEED_D) && P.vis[3 * id + 2]) lb = vadd(lb, xyz(P.nee[3 * id + 2]));
                     Lp = vadd(Lp, vmul(lb, beta));
                 }
-                float4 fe = P.fext[id];
-                f3 f = xyz(fe);
-                float cos_ext = P.nee[3 * id].w;
-                if (vzero(f) || fe.w == 0.f) {
+                float4 fe = P.fext[id];  // (f * cos / pdf, 1), or w = 0 where f or pdf is zero
+                if (fe.w == 0.f) {
                     term = true;
                 } else {
-                    beta = vmul(beta, vdiv(vscl(f, cos_ext), fe.w));
+                    beta = vmul(beta, xyz(fe));
                     if (len > 3) {
                         float qrr = fmaxf(0.05f, 1.f - beta.y);
                         if (rng(P.rngb[id], len, DIM_RR) < qrr) term = true;
@@ -749,10 +749,14 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade(ff_paths P, int next_q) {
                 P.sray[6 * id + 4] = f4(so.dd_o, 0.f);
                 P.sray[6 * id + 5] = f4(so.wi_d, 0.f);
             }
-            P.nee[3 * id] = f4(so.A, so.cos_ext);
+            P.nee[3 * id] = f4(so.A, 0.f);
             P.nee[3 * id + 1] = f4(so.Bs, 0.f);
             P.nee[3 * id + 2] = f4(so.Cd, 0.f);
-            P.fext[id] = f4(so.fext, so.pdf_ext);
+            // the throughput factor wf_logic applies, f * cos / pdf, formed here
+            // with the same operations (logic then reads 16 B instead of 32)
+            bool dead = vzero(so.fext) || so.pdf_ext == 0.f;
+            P.fext[id] = dead ? make_float4(0.f, 0.f, 0.f, 0.f)
+                              : f4(vdiv(vscl(so.fext, so.cos_ext), so.pdf_ext), 1.f);
             P.ray[2 * id] = f4(so.next_o, 0.f);
             P.ray[2 * id + 1] = f4(so.next_d, 0.f);
             P.beta[id].w = __uint_as_float(so.nst);
@@ -854,7 +858,9 @@ __global__ __launch_bounds__(FF_BLOCK) void k_logic_adj(ff_paths P, int q) {
             bool found = slot >= 0;
             term = (len > R.depth) || !found || (stw & ST_DEAD);
             int mtype = -1;
-            f3 d = xyz(P.ray[2 * id + 1]);
+            // the direction is read only by the two cases that use it
+            f3 d = V3(0.f, 0.f, 0.f);
+            if ((found && (stw & ST_INSIDE)) || (!found && (stw & ST_PREV_DELTA))) d = xyz(P.ray[2 * id + 1]);
             if (found) {
                 mtype = (int)S.mats[(size_t)S.mat[slot] * FF_MAT_FLOATS_DEV];
                 if ((stw & ST_INSIDE) && mtype != MAT_DIELECTRIC) {

@@ -151,8 +151,8 @@ struct ff_paths {
     float4* hit;     // [P]     (t, u, v, slot)
     float4* beta;    // [P]     (beta.xyz, state)
     float4* Lp;      // [P]     (path radiance, -)
-    float4* fext;    // [P]     (f.xyz, pdf)
-    float4* nee;     // [P][3]  light term (w = cosine), spec term, diffuse term
+    float4* fext;    // [P]     (f * cos / pdf, 1) of the continuation, 0 when f or pdf is zero
+    float4* nee;     // [P][3]  light term, spec term, diffuse term
     float4* sray;    // [P][3][2] shadow rays
     uint8_t* vis;    // [P][3]
     uint32_t* rngb;  // [P]
