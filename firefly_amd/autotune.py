@@ -1,10 +1,10 @@
 """Autotuner for the wavefront launch shapes.
 
-Times short renders of the actual request for every traversal kernel variant
-(workgroup size x LDS stack entries x refill threshold, see
-ff_traversal_variant_info) and for a
-few paths-in-flight choices (samples per pixel in flight, spp_batch), then
-keeps the fastest on the context.  Results can be written as JSON and reused.
+Times renders of the actual request for a few paths-in-flight choices
+(samples per pixel in flight, spp_batch) and for every traversal kernel
+variant (workgroup size, LDS stack entries, refill threshold, loop shape, BVH
+record width; see ff_traversal_variant_info), then keeps the fastest on the
+context.  Results can be written as JSON and reused.
 """
 from __future__ import annotations
 
@@ -31,57 +31,57 @@ def _time_render(ctx, scene, env, cam, w, h, spp, depth, seed, film, spp_batch, 
 def autotune(ctx: abi.Context, scene: abi.Scene, env: abi.EnvMap, cam: abi.Camera, width: int, height: int,
              max_depth: int, seed: int = 0, probe_spp: int = 2, batches: Iterable[int] = (1, 2),
              reps: int = 2, out_json: Optional[str] = None, refine_batches: Iterable[int] = (),
-             refine_spp: int = 0, finalists: int = 4) -> Dict:
+             refine_spp: int = 0) -> Dict:
     """Pick the fastest traversal variant and spp_batch for this request.
 
-    Each candidate renders `probe_spp` samples per pixel (timed, best of
-    `reps`).  With `refine_batches`, a second stage keeps the chosen variant
-    and times those larger spp_batch values (and the first stage's choice)
-    at `refine_spp` samples per pixel: more paths in flight mean fewer
-    traversal launches, each paying the latency of its slowest ray once.
-    A third stage then re-times the `finalists` fastest variants of the first
-    stage at the chosen spp_batch and `refine_spp` (the short probes rank
-    close variants unreliably for long renders).  The chosen variant is set
-    on `ctx`; returns a dict with the choice and every timing."""
+    Without `refine_spp`: every variant renders `probe_spp` samples per pixel
+    with each spp_batch of `batches` (timed, best of `reps`).
+
+    With `refine_spp` (the request's spp): first the spp_batch, timed with
+    variant 0 over `batches` and `refine_batches` at `refine_spp` samples --
+    more paths in flight mean fewer traversal launches, each paying the
+    latency of its slowest ray once; then every variant at that spp_batch
+    (one batch of `min(refine_spp, spp_batch)` samples), since the ranking of
+    variants depends on the launch size.  The chosen variant is set on `ctx`;
+    returns a dict with the choice and every timing."""
     import torch
     film = torch.zeros((height, width, 4), dtype=torch.float32, device="cuda")
     results = []
+    variants = abi.traversal_variants()
     # warm-up (first launches pay code-object load and allocation)
     abi.render(ctx, scene, env, cam, width, height, 1, max_depth, seed, film)
     torch.cuda.synchronize()
-    for v, block, lds, refill, ww, width_bvh in abi.traversal_variants():
-        abi.set_traversal_variant(ctx, v)
-        for k in batches:
-            if k > probe_spp:
-                continue
-            t = _time_render(ctx, scene, env, cam, width, height, probe_spp, max_depth, seed, film, k, reps)
-            results.append({"variant": v, "block": block, "lds_entries": lds, "refill": refill,
-                            "while_while": ww, "bvh_width": width_bvh, "spp_batch": k, "seconds": t})
-    best = min(results, key=lambda r: r["seconds"])
-    refine = sorted(set(k for k in refine_batches if k <= refine_spp) | {best["spp_batch"]})
-    if refine_spp > 0 and len(refine) > 1:
-        abi.set_traversal_variant(ctx, best["variant"])
-        stage2 = []
-        for k in refine:
+
+    def entry(var, k, t, spp):
+        v, block, lds, refill, ww, width_bvh = var
+        return {"variant": v, "block": block, "lds_entries": lds, "refill": refill, "while_while": ww,
+                "bvh_width": width_bvh, "spp_batch": k, "seconds": t, "probe_spp": spp}
+
+    if refine_spp > 0:
+        ks = sorted(set(k for k in list(batches) + list(refine_batches) if k <= refine_spp))
+        abi.set_traversal_variant(ctx, variants[0][0])
+        stage1 = []
+        for k in ks:
             t = _time_render(ctx, scene, env, cam, width, height, refine_spp, max_depth, seed, film, k, reps)
-            stage2.append(dict(best, spp_batch=k, seconds=t, probe_spp=refine_spp))
-        results += stage2
+            stage1.append(entry(variants[0], k, t, refine_spp))
+        k_best = min(stage1, key=lambda r: r["seconds"])["spp_batch"]
+        spp2 = min(refine_spp, k_best)
+        stage2 = []
+        for var in variants:
+            abi.set_traversal_variant(ctx, var[0])
+            t = _time_render(ctx, scene, env, cam, width, height, spp2, max_depth, seed, film, k_best, reps)
+            stage2.append(entry(var, k_best, t, spp2))
+        results = stage1 + stage2
         best = min(stage2, key=lambda r: r["seconds"])
-        ranked, seen = [], set()
-        for r in sorted(results[:len(results) - len(stage2)], key=lambda r: r["seconds"]):
-            if r["variant"] not in seen:
-                seen.add(r["variant"])
-                ranked.append(r)
-        stage3 = [best]
-        for r in ranked[:finalists]:
-            if r["variant"] == best["variant"]:
-                continue
-            abi.set_traversal_variant(ctx, r["variant"])
-            t = _time_render(ctx, scene, env, cam, width, height, refine_spp, max_depth, seed, film,
-                             best["spp_batch"], reps)
-            stage3.append(dict(r, spp_batch=best["spp_batch"], seconds=t, probe_spp=refine_spp))
-        results += stage3[1:]
-        best = min(stage3, key=lambda r: r["seconds"])
+    else:
+        for var in variants:
+            abi.set_traversal_variant(ctx, var[0])
+            for k in batches:
+                if k > probe_spp:
+                    continue
+                t = _time_render(ctx, scene, env, cam, width, height, probe_spp, max_depth, seed, film, k, reps)
+                results.append(entry(var, k, t, probe_spp))
+        best = min(results, key=lambda r: r["seconds"])
     abi.set_traversal_variant(ctx, best["variant"])
     out = {"best": best, "candidates": results,
            "request": {"width": width, "height": height, "max_depth": max_depth, "probe_spp": probe_spp,

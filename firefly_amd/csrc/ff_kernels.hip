@@ -230,7 +230,13 @@ struct RaySrc {  // a plain ray array (ff_trace_closest / ff_trace_any)
 // current) leaves are tested together.  Each lane still tests its leaves in
 // its visit order, so hits are those of the other shapes; pruning then sees
 // the closest hit of the leaves already tested, so the boxes tested depend
-// on the lanes that share the wave.
+// on the lanes that share the wave.  3 = leaf batching: a lane standing on a
+// leaf queues it (two at most) and keeps walking; the queued leaves are
+// tested together once 8 lanes hold one, or when no lane can walk on -- the
+// leaf code then runs for many lanes at once instead of for the few that
+// reach a leaf in any one step.  Leaves are still tested in each lane's visit
+// order.  5, 6, 7 = one queued leaf per lane (fewer registers), tested
+// together once 8, 4 or 16 lanes hold one.
 // WIDE walks the 4-wide records (half the dependent interior steps, same
 // leaf order) instead of the two-child records.
 template <int BS, int LDS, int REFILL, bool ANY, int WW, bool WIDE, class Src>
@@ -246,8 +252,9 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
     st.ovf_stride = FF_TRAV_THREADS;
     uint32_t cn = 0, ct = 0, id = 0;
     bool busy = false, more = true, two = false;
-    bool walk = false;  // WW 2: the lane has records or leaves left to visit
-    int pend = 0;       // WW 2: postponed leaf (0 = none; leaf refs are negative)
+    bool walk = false;  // WW >= 2: the lane has records or leaves left to visit
+    int pend = 0;       // WW >= 2: postponed leaf (0 = none; leaf refs are negative)
+    int pend2 = 0;      // WW >= 3: a second postponed leaf, tested after pend
     Bvh B;
     B.nodes = nullptr; B.nodes4 = nullptr; B.leaf_count = nullptr; B.root = nullptr;
     const float* tris = nullptr;
@@ -271,6 +278,7 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
                 busy = trav_begin(B, o, d, R, st, cn);
                 walk = busy;
                 pend = 0;
+                pend2 = 0;
                 if (!busy) {
                     if (ANY) src.occluded(id, false);
                     else src.hit(id, h);
@@ -316,6 +324,41 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
                     else src.hit(id, h);
                     busy = false;
                 }
+            }
+        } else if (WW >= 3) {
+            // leaf batching: a lane standing on a leaf queues it (two at most)
+            // and walks on; the queued leaves are tested together once
+            // LEAF_T lanes hold one, or when no lane can walk on.  Each lane
+            // tests its leaves in its visit order.
+            constexpr uint32_t LEAF_T = WW == 7 ? 16u : (WW == 6 ? 4u : 8u);
+            constexpr bool TWO = WW == 3;  // shapes 5-7 queue one leaf per lane
+            if (busy && walk && R.ref < 0 && (TWO ? pend2 == 0 : pend == 0)) {
+                if (pend == 0) pend = R.ref;
+                else pend2 = R.ref;
+                if (st.sp == 0) walk = false;
+                else R.ref = st.pop();
+            }
+            if (busy && walk && R.ref >= 0 && !visit_node<WIDE, !ANY>(B, R, prune_lim(h.t), st, cn)) {
+                if (st.sp == 0) walk = false;
+                else R.ref = st.pop();
+            }
+            bool has = busy && pend != 0;
+            uint32_t n_pend = (uint32_t)__popcll(__ballot(has));
+            bool occ = false;
+            if (n_pend >= LEAF_T || (n_pend > 0 && __ballot(busy && walk && R.ref >= 0) == 0)) {
+                if (has) {
+                    TravRay L = R;
+                    L.ref = pend;
+                    if (ANY) occ = leaf_any(B, tris, L, ct);
+                    else leaf_closest(B, tris, L, h, two, ct);
+                    pend = TWO ? pend2 : 0;
+                    pend2 = 0;
+                }
+            }
+            if (busy && (occ || (!walk && pend == 0))) {
+                if (ANY) src.occluded(id, occ);
+                else src.hit(id, h);
+                busy = false;
             }
         } else if (WW == 1) {
             for (;;) {  // interior records until every busy lane is at a leaf
@@ -368,7 +411,7 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
 // (72 VGPRs) without spilling where its LDS stack allows that many; the
 // others keep the compiler's choice.
 template <int BS, int LDS, int RF, int WW, bool WIDE, bool UNI>
-__global__ __launch_bounds__(BS, (UNI && WW == 0 && !WIDE && LDS <= 16 && BS >= 128) ? 7 : 1) void k_extend(
+__global__ __launch_bounds__(BS, (UNI && (WW == 0 || WW >= 5) && !WIDE && LDS <= 16 && BS >= 128) ? 7 : 1) void k_extend(
     ff_paths P, int q) {
     __shared__ int smem[LDS * BS];
     ExtSrc<UNI> src{P, P.q_ext[q]};
@@ -1247,7 +1290,7 @@ uint32_t resident_grid(K kernel, int bs) {
 }
 
 // traversal variants (block size, LDS stack entries, refill threshold, loop
-// shape 0/1/2, BVH record width)
+// shape 0-4, BVH record width)
 #define FF_TRAV_VARIANT_LIST(X)  \
     X(0, 256, 16, 32, 0, false)  \
     X(1, 256, 16, 16, 0, false)  \
@@ -1268,7 +1311,14 @@ uint32_t resident_grid(K kernel, int bs) {
     X(16, 256, 16, 64, 2, false) \
     X(17, 256, 16, 32, 2, false) \
     X(18, 256, 16, 8, 0, false)  \
-    X(19, 256, 24, 16, 0, false)
+    X(19, 256, 24, 16, 0, false) \
+    X(20, 256, 16, 16, 3, false) \
+    X(21, 256, 16, 32, 3, false) \
+    X(22, 256, 16, 16, 5, false) \
+    X(23, 256, 16, 32, 5, false) \
+    X(24, 256, 16, 16, 6, false) \
+    X(25, 256, 16, 16, 7, false) \
+    X(26, 128, 16, 16, 5, false)
 
 // which: 0 extend, 1 shadow, 2 grid sizes only (before a graph capture),
 // 3 ff_trace_closest, 4 ff_trace_any

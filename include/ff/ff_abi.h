@@ -237,7 +237,10 @@ ff_status ff_adjoint_batch(ff_context *ctx, const ff_batch_item *items, int32_t 
 /* ---- traversal kernel variants (workgroup size, LDS stack entries, the
  * number of idle lanes at which a wave of the persistent traversal fetches
  * new rays, the loop shape: 0 = one step per iteration, 1 = while-while,
- * 2 = speculative while-while (leaves postponed while other lanes walk), and the
+ * 2 = speculative while-while (leaves postponed while other lanes walk),
+ * 3, 5, 6, 7 = leaf batching (a lane queues the leaf it reaches and walks on;
+ * queued leaves are tested together once 8, 8, 4 or 16 lanes hold one; shape
+ * 3 queues two leaves per lane, the others one), and the
  * BVH record width: 2 = the reference's binary nodes two children at a time,
  * 4 = the same tree collapsed to 4-wide records, visited in the same leaf
  * order so hits are identical), chosen per context by the autotuner
