@@ -113,7 +113,7 @@ struct ff_scene {
     float* d_tris = nullptr;
     float* d_nrm = nullptr;
     float* d_uv = nullptr;
-    int32_t* d_mat = nullptr;
+    uint32_t* d_mat = nullptr;
     float* d_mats = nullptr;
     std::vector<float*> d_tex;
     std::vector<ff_tex_desc> tex_desc;  // host copy (sizes)
@@ -399,6 +399,8 @@ ff_status ff_scene_create(ff_context* ctx, const ff_scene_desc* d, ff_scene** ou
     if (d->n_tris <= 0 || !d->positions || !d->normals || !d->material_ids)
         return fail(FF_ERR_INVALID, "ff_scene_create: empty scene or missing positions/normals/material ids");
     if (d->n_materials <= 0 || !d->materials) return fail(FF_ERR_INVALID, "ff_scene_create: no materials");
+    if ((uint32_t)d->n_materials > MAT_INDEX_MASK)
+        return fail(FF_ERR_CAPACITY, "ff_scene_create: %d materials (limit 2^30 - 1)", d->n_materials);
     if (d->n_tris >= (1 << 27))
         return fail(FF_ERR_CAPACITY, "ff_scene_create: %d triangles (limit 2^27 - 1)", d->n_tris);
     for (int32_t i = 0; i < d->n_tris; i++)
@@ -433,7 +435,7 @@ ff_status ff_scene_create(ff_context* ctx, const ff_scene_desc* d, ff_scene** ou
     // reorder_primitives (CRT/kernel.cu:184-191,701-712) done on the host while
     // packing the positions-only records v0 | e1 | e2.
     std::vector<float> tris(9 * n), nrm(9 * n), uv(6 * n, 0.f);
-    std::vector<int32_t> mat(n);
+    std::vector<uint32_t> mat(n);
     for (size_t i = 0; i < n; i++) {
         size_t src = (size_t)s->ordered[i];
         const float* p = d->positions + src * 9;
@@ -445,7 +447,8 @@ ff_status ff_scene_create(ff_context* ctx, const ff_scene_desc* d, ff_scene** ou
         }
         std::memcpy(&nrm[i * 9], d->normals + src * 9, 36);
         if (d->uvs) std::memcpy(&uv[i * 6], d->uvs + src * 6, 24);
-        mat[i] = d->material_ids[src];
+        uint32_t m = (uint32_t)d->material_ids[src];
+        mat[i] = m | ((uint32_t)(int)d->materials[(size_t)m * FF_MAT_FLOATS] << MAT_TYPE_SHIFT);
     }
     for (int32_t m = 0; m < d->n_materials; m++) s->has_type[(int)d->materials[(size_t)m * FF_MAT_FLOATS]] = true;
     auto cleanup = [&](ff_status st) { ff_scene_destroy(s); return st; };
@@ -844,9 +847,12 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
         ctx->records.push_back(rec);
         return e;
     };
+    // the first batch stores every layer of a request whose spp covers its
+    // samples in flight (wf_logic writes it without reading); only a request
+    // with fewer samples than layers keeps layers that no path writes
     if (!adjoint)
         for (size_t r = 0; r < reqs.size(); r++)
-            if (reqs[r].K > 1)
+            if (reqs[r].K > 1 && reqs[r].spp < reqs[r].K)
                 HIP_TRY(ffk::launch_fill4(table[r].layers, (uint32_t)((uint64_t)reqs[r].W * reqs[r].H * reqs[r].K), hs));
     HIP_TRY(ffk::launch_fill_u64(p.stats, S_WORDS, hs));
 
@@ -856,11 +862,11 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
         HIP_TRY(timed(4, [&] { return ffk::launch_generate(p, hs); }));
         int cur = 0;
         for (int32_t it = 1; it <= max_depth + 1; it++) {
-            HIP_TRY(timed(0, [&] { return ffk::launch_extend(p, cur, var, hs); }));
+            HIP_TRY(timed(0, [&] { return ffk::launch_extend(p, cur, it == 1 ? EXT_STATE_NONE : EXT_STATE_FEXT, var, hs); }));
             HIP_TRY(timed(1, [&] { return ffk::launch_shadow(p, var, hs); }));
             uint32_t mask = (1u << (cur == 0 ? C_EXT1 : C_EXT0)) | (1u << C_SHADOW) | (0xfu << C_MAT0);
             HIP_TRY(ffk::launch_reset(p.counters, mask, p.fetch, (1u << F_EXT) | (1u << F_SHD), hs));
-            HIP_TRY(timed(2, [&] { return ffk::launch_logic(p, cur, p.n_slots, hs); }));
+            HIP_TRY(timed(2, [&] { return ffk::launch_logic(p, cur, it == 1, p.n_slots, hs); }));
             if (it <= max_depth) {
                 for (int m = 0; m < 4; m++)
                     if (has_type[m]) HIP_TRY(timed(3, [&] { return ffk::launch_shade(m, p, 1 - cur, p.n_slots, hs); }));
@@ -876,7 +882,7 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
         HIP_TRY(timed(4, [&] { return ffk::launch_generate(p, hs); }));
         int cur = 0;
         for (int32_t it = 1; it <= max_depth + 1; it++) {
-            HIP_TRY(timed(0, [&] { return ffk::launch_extend(p, cur, var, hs); }));
+            HIP_TRY(timed(0, [&] { return ffk::launch_extend(p, cur, it == 1 ? EXT_STATE_NONE : EXT_STATE_BETA, var, hs); }));
             uint32_t mask = (1u << (cur == 0 ? C_EXT1 : C_EXT0)) | (1u << C_SHADOW) | (0xfu << C_MAT0);
             HIP_TRY(ffk::launch_reset(p.counters, mask, p.fetch, (1u << F_EXT) | (1u << F_SHD), hs));
             HIP_TRY(timed(2, [&] { return ffk::launch_logic_adj(p, cur, p.n_slots, hs); }));

@@ -157,7 +157,9 @@ struct ExtSrc {  // wf_extend (CRT/kernel.cu:487-523): closest hit of every queu
         }
         o = xyz(P.ray[2 * id]);
         d = xyz(P.ray[2 * id + 1]);
-        two = (__float_as_uint(P.beta[id].w) & ST_INSIDE) != 0;
+        two = false;
+        if (P.ext_state == EXT_STATE_FEXT) two = (__float_as_uint(P.fext[id].w) & ST_INSIDE) != 0;
+        else if (P.ext_state == EXT_STATE_BETA) two = (__float_as_uint(P.beta[id].w) & ST_INSIDE) != 0;
     }
     __device__ __forceinline__ void hit(uint32_t id, const Hit& h) const {
         P.hit[id] = make_float4(h.t, h.u, h.v, __int_as_float(h.id));
@@ -440,89 +442,174 @@ __global__ void k_reset_counters(uint32_t* c, uint32_t mask, uint32_t* fetch, ui
 // ray's visibility, throughput update, Russian roulette for len > 3,
 // environment radiance for escaped rays after the camera (or a delta event),
 // termination into the film, routing of live paths to material queues.
-template <bool UNI>
-__global__ __launch_bounds__(FF_BLOCK) void k_logic(ff_paths P, int q) {
+// Each thread takes two paths of a 512-entry tile (entries t and t + 256, so
+// every load of a wave stays coalesced), and the loads are issued level by
+// level for both before any result is used: queue -> path record -> what the
+// record asks for (material word, direction, visibilities with the NEE terms
+// they gate, throughput factor, RNG base) -> the rare third level (escaped
+// delta rays' environment lookup, inside-absorption triangle).  The NEE terms
+// are read with their visibility bytes rather than after them.  The
+// arithmetic of every path is the one-path wf_logic's, operation for
+// operation.  In the first batch of a samples-in-flight request every slot is
+// written exactly once (each slot holds one path per batch and every path
+// terminates within its batch), so the layer is stored without being read.
+template <bool UNI, int NP>
+__global__ __launch_bounds__(FF_BLOCK, NP == 1 ? 8 : 5) void k_logic(ff_paths P, int q, int primary) {
     __shared__ uint32_t lds[4 * (FF_BLOCK / 64 + 1)];
     const uint32_t n = P.counters[q == 0 ? C_EXT0 : C_EXT1];
     const uint32_t* queue = P.q_ext[q];
-    const uint32_t n_round = (n + FF_BLOCK - 1) / FF_BLOCK * FF_BLOCK;  // uniform trip count per workgroup
+    const bool first_batch = P.bstate[B_BATCH] == 0;
+    constexpr uint32_t TILE = NP * FF_BLOCK;
+    const uint32_t n_round = (n + TILE - 1) / TILE * TILE;  // uniform trip count per workgroup
     uint32_t c_prim = 0, c_ext = 0, c_paths = 0;
-    for (uint32_t i = gtid(); i < n_round; i += gstride()) {
-        bool live = i < n;
-        int route = -1;
-        bool term = false, primary = false;
-        uint32_t id = live ? queue[i] : 0u;
-        if (live) {
-            const ff_dev_request& R = P.reqs[P.preq[id]];
+    for (uint32_t t0 = blockIdx.x * TILE; t0 < n_round; t0 += gridDim.x * TILE) {
+        bool live[NP];
+        uint32_t id[NP];
+#pragma unroll
+        for (int j = 0; j < NP; j++) {
+            uint32_t i = t0 + (uint32_t)j * FF_BLOCK + threadIdx.x;
+            live[j] = i < n;
+            id[j] = live[j] ? queue[i] : 0u;
+        }
+        // level 1: the path record
+        float4 bw[NP], lp4[NP], hh[NP], fe[NP];
+        uint32_t rq[NP];
+#pragma unroll
+        for (int j = 0; j < NP; j++) {
+            bw[j] = P.beta[id[j]];
+            lp4[j] = P.Lp[id[j]];
+            hh[j] = P.hit[id[j]];
+            rq[j] = P.n_req == 1 ? 0u : P.preq[id[j]];
+            // after a shading pass the state word travels in fext.w:
+            // (f * cos / pdf of the continuation, state | ST_DEAD where f or pdf is zero)
+            fe[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (!primary) fe[j] = P.fext[id[j]];
+        }
+        // level 2: what the record asks for
+        uint32_t stw[NP], len[NP], mw[NP], rb[NP];
+        int slot[NP];
+        bool found[NP], need_d[NP];
+        f3 d[NP], nL[NP], nS[NP], nD[NP];
+        uint8_t v0[NP], v1[NP], v2[NP];
+#pragma unroll
+        for (int j = 0; j < NP; j++) {
+            const ff_dev_request& R = P.reqs[rq[j]];
             const ff_dev_scene& S = scene_of<UNI>(P, R);
-            float4 bw = P.beta[id];
-            uint32_t stw = __float_as_uint(bw.w);
-            uint32_t len = (stw & ST_LEN_MASK) + 1;
-            primary = len == 1;
-            f3 beta = xyz(bw);
-            f3 Lp = xyz(P.Lp[id]);
-            float4 hh = P.hit[id];
-            int slot = __float_as_int(hh.w);
-            bool found = slot >= 0;
-            term = (len > R.depth) || !found;
-            int mtype = -1;
-            // the direction is read only by the two cases that use it
-            f3 d = V3(0.f, 0.f, 0.f);
-            if ((found && (stw & ST_INSIDE)) || (!found && (stw & ST_PREV_DELTA))) d = xyz(P.ray[2 * id + 1]);
-            if (found) {
-                mtype = (int)S.mats[(size_t)S.mat[slot] * FF_MAT_FLOATS_DEV];
-                if ((stw & ST_INSIDE) && mtype != MAT_DIELECTRIC) {
-                    Tri T = load_tri(S.tris, slot);
-                    if (!(vdot(d, vcross(T.e1, T.e2)) < 0.f)) term = true;  // absorbed
-                }
+            stw[j] = __float_as_uint(primary ? bw[j].w : fe[j].w);
+            len[j] = (stw[j] & ST_LEN_MASK) + 1;
+            slot[j] = __float_as_int(hh[j].w);
+            found[j] = live[j] && slot[j] >= 0;
+            need_d[j] = live[j] && ((found[j] && (stw[j] & ST_INSIDE)) || (!found[j] && (stw[j] & ST_PREV_DELTA)));
+            mw[j] = 0u;
+            if (found[j]) mw[j] = S.mat[slot[j]];
+            d[j] = V3(0.f, 0.f, 0.f);
+            if (need_d[j]) d[j] = xyz(P.ray[2 * id[j] + 1]);
+            bool ext = live[j] && len[j] > 1;
+            v0[j] = v1[j] = v2[j] = 0;
+            nL[j] = nS[j] = nD[j] = V3(0.f, 0.f, 0.f);
+            if (ext && (stw[j] & ST_NEED_L)) {
+                v0[j] = P.vis[3 * id[j] + 0];
+                nL[j] = xyz(P.nee[3 * id[j]]);
             }
-            if (!found && (stw & ST_PREV_DELTA)) Lp = vadd(Lp, vmul(beta, env_L(env_view(env_of<UNI>(P, R)), d)));
-            if (len > 1) {
-                bool visL = (stw & ST_NEED_L) && P.vis[3 * id + 0];
-                if (visL) {
-                    f3 lb = xyz(P.nee[3 * id]);
-                    if ((stw & ST_NEED_S) && P.vis[3 * id + 1]) lb = vadd(lb, xyz(P.nee[3 * id + 1]));
-                    if ((stw & ST_NEED_D) && P.vis[3 * id + 2]) lb = vadd(lb, xyz(P.nee[3 * id + 2]));
-                    Lp = vadd(Lp, vmul(lb, beta));
+            if (ext && (stw[j] & ST_NEED_S)) {
+                v1[j] = P.vis[3 * id[j] + 1];
+                nS[j] = xyz(P.nee[3 * id[j] + 1]);
+            }
+            if (ext && (stw[j] & ST_NEED_D)) {
+                v2[j] = P.vis[3 * id[j] + 2];
+                nD[j] = xyz(P.nee[3 * id[j] + 2]);
+            }
+            rb[j] = 0u;
+            if (ext && found[j] && len[j] > 3) rb[j] = P.rngb[id[j]];
+        }
+        // the NEE sum of the previous vertex, gated by its visibilities
+        f3 lb[NP];
+        bool visL[NP];
+#pragma unroll
+        for (int j = 0; j < NP; j++) {
+            uint32_t sw = stw[j];
+            visL[j] = live[j] && len[j] > 1 && (sw & ST_NEED_L) && v0[j];
+            lb[j] = nL[j];
+            if ((sw & ST_NEED_S) && v1[j]) lb[j] = vadd(lb[j], nS[j]);
+            if ((sw & ST_NEED_D) && v2[j]) lb[j] = vadd(lb[j], nD[j]);
+        }
+        // level 3 and the per-path arithmetic, in wf_logic's order: an escaped
+        // ray's environment term, then the NEE sum, then (a hit) throughput
+        // and roulette
+        int route[NP];
+#pragma unroll
+        for (int j = 0; j < NP; j++) {
+            route[j] = -1;
+            if (!live[j]) continue;
+            const ff_dev_request& R = P.reqs[rq[j]];
+            const ff_dev_scene& S = scene_of<UNI>(P, R);
+            uint32_t sw = stw[j], ln = len[j];
+            f3 beta = xyz(bw[j]);
+            f3 Lp = xyz(lp4[j]);
+            bool tm = (ln > R.depth) || !found[j];
+            int mtype = -1;
+            if (found[j]) {
+                mtype = (int)(mw[j] >> MAT_TYPE_SHIFT);
+                if ((sw & ST_INSIDE) && mtype != MAT_DIELECTRIC) {
+                    Tri T = load_tri(S.tris, slot[j]);
+                    if (!(vdot(d[j], vcross(T.e1, T.e2)) < 0.f)) tm = true;  // absorbed
                 }
-                float4 fe = P.fext[id];  // (f * cos / pdf, 1), or w = 0 where f or pdf is zero
-                if (fe.w == 0.f) {
-                    term = true;
-                } else {
-                    beta = vmul(beta, xyz(fe));
-                    if (len > 3) {
-                        float qrr = fmaxf(0.05f, 1.f - beta.y);
-                        if (rng(P.rngb[id], len, DIM_RR) < qrr) term = true;
+                if (visL[j]) Lp = vadd(Lp, vmul(lb[j], beta));
+                if (ln > 1) {
+                    if (sw & ST_DEAD) {
+                        tm = true;
+                    } else {
+                        beta = vmul(beta, xyz(fe[j]));
+                        if (ln > 3) {
+                            float qrr = fmaxf(0.05f, 1.f - beta.y);
+                            if (rng(rb[j], ln, DIM_RR) < qrr) tm = true;
+                        }
                     }
                 }
+            } else {
+                if (sw & ST_PREV_DELTA) Lp = vadd(Lp, vmul(beta, env_L(env_view(env_of<UNI>(P, R)), d[j])));
+                if (visL[j]) Lp = vadd(Lp, vmul(lb[j], beta));
             }
-            if (term) {
-                uint32_t local = id - R.slot_base, WH = R.W * R.H;
-                float4* acc = R.K > 1 ? &R.layers[local] : &R.film[local % WH];
-                float4 L = *acc;
-                *acc = make_float4(L.x + Lp.x, L.y + Lp.y, L.z + Lp.z, L.w + 1.f);
+            if (tm) {
+                uint32_t local = id[j] - R.slot_base, WH = R.W * R.H;
+                if (R.K > 1 && first_batch) {
+                    R.layers[local] = make_float4(0.f + Lp.x, 0.f + Lp.y, 0.f + Lp.z, 0.f + 1.f);
+                } else {
+                    float4* acc = R.K > 1 ? &R.layers[local] : &R.film[local % WH];
+                    float4 L = *acc;
+                    *acc = make_float4(L.x + Lp.x, L.y + Lp.y, L.z + Lp.z, L.w + 1.f);
+                }
                 if (R.mode & RQ_STORE_PATHS) {
                     uint32_t sl = P.bstate[B_BATCH] * R.K + local / WH;
                     R.path_L[(size_t)sl * WH + local % WH] = f4(Lp, 0.f);
                 }
             } else {
-                uint32_t nst = (stw & (ST_INSIDE | ST_PREV_DELTA)) | len;
-                P.beta[id] = f4(beta, __uint_as_float(nst));
-                P.Lp[id] = f4(Lp, 0.f);
-                route = mtype;
+                uint32_t nst = (sw & (ST_INSIDE | ST_PREV_DELTA)) | ln;
+                P.beta[id[j]] = f4(beta, __uint_as_float(nst));
+                P.Lp[id[j]] = f4(Lp, 0.f);
+                route[j] = mtype;
             }
+            c_prim += ln == 1 ? 1u : 0u;
+            c_ext += ln == 1 ? 0u : 1u;
+            c_paths += tm ? 1u : 0u;
         }
-        c_prim += (live && primary) ? 1u : 0u;
-        c_ext += (live && !primary) ? 1u : 0u;
-        c_paths += (live && term) ? 1u : 0u;
         const int qidx[4] = {C_MAT0, C_MAT0 + 1, C_MAT0 + 2, C_MAT0 + 3};
         uint32_t cnt[4], first[4];
 #pragma unroll
-        for (int m = 0; m < 4; m++) cnt[m] = route == m ? 1u : 0u;
+        for (int m = 0; m < 4; m++) {
+            cnt[m] = 0u;
+#pragma unroll
+            for (int j = 0; j < NP; j++) cnt[m] += route[j] == m ? 1u : 0u;
+        }
         block_alloc<FF_BLOCK, 4>(P.counters, qidx, cnt, first, lds);
-        if (route >= 0) {
-            uint32_t f = route == 0 ? first[0] : (route == 1 ? first[1] : (route == 2 ? first[2] : first[3]));
-            P.q_mat[route][f] = id;
+#pragma unroll
+        for (int j = 0; j < NP; j++) {
+            if (route[j] >= 0) {
+                uint32_t f = route[j] == 0 ? first[0] : (route[j] == 1 ? first[1] : (route[j] == 2 ? first[2] : first[3]));
+#pragma unroll
+                for (int k = 0; k < j; k++) f += route[k] == route[j] ? 1u : 0u;
+                P.q_mat[route[j]][f] = id[j];
+            }
         }
     }
     unsigned long long* sr = stats_rep(P);
@@ -578,7 +665,7 @@ __device__ __forceinline__ void shade_core(const ff_paths& P, uint32_t id, Shade
     const float* U = S.uv + (size_t)slot * 6;
     float tu = (U[2] * hh.y + U[4] * hh.z) + U[0] * w0;
     float tv = (U[3] * hh.y + U[5] * hh.z) + U[1] * w0;
-    const float* M = S.mats + (size_t)S.mat[slot] * FF_MAT_FLOATS_DEV;
+    const float* M = S.mats + (size_t)(S.mat[slot] & MAT_INDEX_MASK) * FF_MAT_FLOATS_DEV;
     f3 alb = V3(M[1], M[2], M[3]);
     float rough = M[4], metal = M[5];
     f3 fres = V3(M[6], M[7], M[8]);
@@ -797,12 +884,13 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade(ff_paths P, int next_q) {
             P.nee[3 * id + 2] = f4(so.Cd, 0.f);
             // the throughput factor wf_logic applies, f * cos / pdf, formed here
             // with the same operations (logic then reads 16 B instead of 32)
+            // and the continuation's state word rides in its w (the path
+            // record's beta is left to wf_logic)
             bool dead = vzero(so.fext) || so.pdf_ext == 0.f;
-            P.fext[id] = dead ? make_float4(0.f, 0.f, 0.f, 0.f)
-                              : f4(vdiv(vscl(so.fext, so.cos_ext), so.pdf_ext), 1.f);
+            P.fext[id] = dead ? make_float4(0.f, 0.f, 0.f, __uint_as_float(so.nst | ST_DEAD))
+                              : f4(vdiv(vscl(so.fext, so.cos_ext), so.pdf_ext), __uint_as_float(so.nst));
             P.ray[2 * id] = f4(so.next_o, 0.f);
             P.ray[2 * id + 1] = f4(so.next_d, 0.f);
-            P.beta[id].w = __uint_as_float(so.nst);
         }
         uint32_t ns = (needL ? 1u : 0u) + (needS ? 1u : 0u) + (needD ? 1u : 0u);
         c_shd += ns;
@@ -905,7 +993,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_logic_adj(ff_paths P, int q) {
             f3 d = V3(0.f, 0.f, 0.f);
             if ((found && (stw & ST_INSIDE)) || (!found && (stw & ST_PREV_DELTA))) d = xyz(P.ray[2 * id + 1]);
             if (found) {
-                mtype = (int)S.mats[(size_t)S.mat[slot] * FF_MAT_FLOATS_DEV];
+                mtype = (int)(S.mat[slot] >> MAT_TYPE_SHIFT);
                 if ((stw & ST_INSIDE) && mtype != MAT_DIELECTRIC) {
                     Tri T = load_tri(S.tris, slot);
                     if (!(vdot(d, vcross(T.e1, T.e2)) < 0.f)) term = true;  // absorbed
@@ -1426,8 +1514,10 @@ hipError_t launch_generate(const ff_paths& P, hipStream_t s) {
     hipLaunchKernelGGL(k_generate, dim3(grid_for(P.n_slots, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, P);
     return hipGetLastError();
 }
-hipError_t launch_extend(const ff_paths& P, int q, int variant, hipStream_t s) {
-    return trav_dispatch(0, variant, &P, q, nullptr, s);
+hipError_t launch_extend(const ff_paths& P, int q, int state, int variant, hipStream_t s) {
+    ff_paths E = P;
+    E.ext_state = state;
+    return trav_dispatch(0, variant, &E, q, nullptr, s);
 }
 hipError_t launch_shadow(const ff_paths& P, int variant, hipStream_t s) {
     return trav_dispatch(1, variant, &P, 0, nullptr, s);
@@ -1438,12 +1528,12 @@ hipError_t launch_reset(uint32_t* counters, uint32_t mask, uint32_t* fetch, uint
     return hipGetLastError();
 }
 template <bool UNI>
-hipError_t logic_launch(const ff_paths& P, int q, uint32_t n_max, hipStream_t s) {
-    hipLaunchKernelGGL(k_logic<UNI>, dim3(grid_for(n_max, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, P, q);
+hipError_t logic_launch(const ff_paths& P, int q, int primary, uint32_t n_max, hipStream_t s) {
+    hipLaunchKernelGGL((k_logic<UNI, 2>), dim3(grid_for(n_max, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, P, q, primary);
     return hipGetLastError();
 }
-hipError_t launch_logic(const ff_paths& P, int q, uint32_t n_max, hipStream_t s) {
-    return uni_shading(P) ? logic_launch<true>(P, q, n_max, s) : logic_launch<false>(P, q, n_max, s);
+hipError_t launch_logic(const ff_paths& P, int q, int primary, uint32_t n_max, hipStream_t s) {
+    return uni_shading(P) ? logic_launch<true>(P, q, primary, n_max, s) : logic_launch<false>(P, q, primary, n_max, s);
 }
 template <bool UNI>
 hipError_t shade_launch(int type, const ff_paths& P, int next_q, uint32_t n_max, hipStream_t s) {

@@ -18,14 +18,24 @@
 #define FF_LDS_MIN 8          // smallest LDS stack of a traversal variant (sizes the overflow area)
 #define FF_GRID_CAP 4096      // grid-stride cap of the other stage kernels
 
-// path state word (beta.w bits)
+// path state word (beta.w bits; after a shading pass, fext.w)
 #define ST_LEN_MASK 0xffu
 #define ST_INSIDE (1u << 8)
 #define ST_PREV_DELTA (1u << 9)
 #define ST_NEED_L (1u << 10)
 #define ST_NEED_S (1u << 11)
 #define ST_NEED_D (1u << 12)
-#define ST_DEAD (1u << 13)   // adjoint replay: the continuation had zero f or pdf
+#define ST_DEAD (1u << 13)   // the continuation had zero f or pdf
+// Where the extension stage finds a path's state word (its inside flag makes
+// the closest-hit test two-sided): none on camera rays, fext.w after a
+// forward shading pass, beta.w after an adjoint replay pass.
+#define EXT_STATE_NONE 0
+#define EXT_STATE_FEXT 1
+#define EXT_STATE_BETA 2
+// Per-triangle material word: the material's type (0-3) in the top two bits,
+// so wf_logic routes a hit with one load.
+#define MAT_TYPE_SHIFT 30
+#define MAT_INDEX_MASK 0x3fffffffu
 
 // device counters
 #define C_EXT0 0
@@ -85,7 +95,7 @@ struct ff_dev_scene {
     const float* tris;
     const float* nrm;
     const float* uv;
-    const int32_t* mat;
+    const uint32_t* mat;       // per slot: material index | material type << MAT_TYPE_SHIFT
     const float* mats;
     const ff_tex_desc* tex;
     int32_t n_nodes, n_tris, n_mats, n_tex, max_depth;
@@ -151,7 +161,7 @@ struct ff_paths {
     float4* hit;     // [P]     (t, u, v, slot)
     float4* beta;    // [P]     (beta.xyz, state)
     float4* Lp;      // [P]     (path radiance, -)
-    float4* fext;    // [P]     (f * cos / pdf, 1) of the continuation, 0 when f or pdf is zero
+    float4* fext;    // [P]     f * cos / pdf of the continuation (0 when f or pdf is zero), state word
     float4* nee;     // [P][3]  light term, spec term, diffuse term
     float4* sray;    // [P][3][2] shadow rays
     uint8_t* vis;    // [P][3]
@@ -172,6 +182,7 @@ struct ff_paths {
     uint32_t n_slots;    // total slots over all requests
     int32_t uni_scene;   // every request of the call renders the same scene
     int32_t uni_env;     // every request of the call uses the same environment
+    int32_t ext_state;   // where an extension ray's state word is: EXT_STATE_*
     // adjoint replay
     float4* dlt;         // [P]     d loss / d path radiance (xyz)
     uint32_t* g_max;     // arena: max |term| bits per gradient element (GM_BOUND)
