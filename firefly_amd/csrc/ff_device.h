@@ -46,9 +46,8 @@ constexpr float PRUNE_BEHIND = 1e-4f;
 // the same order, so host and device agree bit for bit; they are within a
 // few ulp of the C library in double, so after rounding to float they nearly
 // always equal the correctly rounded value.  They need far fewer registers
-// than general library routines (the shading kernels' occupancy depends on
-// it); arguments outside their reduced ranges (huge or infinite) go to the
-// library, out of line.
+// and instructions than general library routines (one division per atan2);
+// sin/cos arguments of magnitude 2^19 or more go to the library, out of line.
 //
 // sin and cos: Cody-Waite reduction modulo pi/2 with a three-part split of
 // pi/2 (33 + 33 + 53 bits, so k * part is exact for |k| < 2^20), then the
@@ -93,15 +92,11 @@ __device__ __forceinline__ void sincos_d(double x, double& s, double& c) {
     s = q == 0 ? sr : (q == 1 ? cr : (q == 2 ? -sr : -cr));
     c = q == 0 ? cr : (q == 1 ? -sr : (q == 2 ? -cr : sr));
 }
-// atan: odd symmetry; |x| > 1 by atan x = pi/2 - atan(1/x); t > tan(pi/12)
-// by the addition formula atan t = pi/6 + atan((t*sqrt3 - 1)/(t + sqrt3));
-// then the Taylor series to u^27 on |u| <= tan(pi/12) (first omitted term
-// below 1e-18; coefficients (-1)^n/(2n+1) rounded to double).
-__device__ __forceinline__ double atan_d(double x) {
-    if (x != x) return x;
-    double a = fabs(x);
-    bool inv = a > 1.0;
-    double t = inv ? 1.0 / a : a;
+// atan on [0, 1]: above tan(pi/12) the addition formula
+// atan t = pi/6 + atan((t*sqrt3 - 1)/(t + sqrt3)); then the Taylor series to
+// u^27 on |u| <= tan(pi/12) (first omitted term below 1e-18; coefficients
+// (-1)^n/(2n+1) rounded to double).
+__device__ __forceinline__ double atan_unit(double t) {
     bool shift = t > 0x1.126145e9ecd56p-2;
     double u = shift ? (t * 0x1.bb67ae8584caap+0 - 1.0) / (t + 0x1.bb67ae8584caap+0) : t;
     double z = u * u;
@@ -119,22 +114,38 @@ __device__ __forceinline__ double atan_d(double x) {
     p = 0x1.999999999999ap-3 + z * p;
     p = -0x1.5555555555555p-2 + z * p;
     double r = u + (u * z) * p;
-    if (shift) r = 0x1.0c152382d7366p-1 + r;
+    return shift ? 0x1.0c152382d7366p-1 + r : r;
+}
+// atan: odd symmetry; |x| > 1 by atan x = pi/2 - atan(1/x)
+__device__ __forceinline__ double atan_d(double x) {
+    if (x != x) return x;
+    double a = fabs(x);
+    bool inv = a > 1.0;
+    double r = atan_unit(inv ? 1.0 / a : a);
     if (inv) r = 0x1.921fb54442d18p+0 - r;
     return copysign(r, x);
 }
-__device__ __noinline__ double atan2_lib(double y, double x) { return atan2(y, x); }
-// atan2 with the C library's conventions for zeros and signs
+// atan2 with the C library's conventions for zeros, signs and infinities:
+// theta = atan(min/max) of the magnitudes (one division), reflected to
+// pi/2 - theta above the diagonal and to pi - theta for x < 0, signed by y
 __device__ __forceinline__ double atan2_d(double y, double x) {
     if (y != y || x != x) return x + y;
-    if (isinf(x) || isinf(y)) return atan2_lib(y, x);
-    if (x == 0.0) {
-        if (y == 0.0) return signbit(x) ? copysign(0x1.921fb54442d18p+1, y) : y;
+    double a = fabs(y), b = fabs(x);
+    if (isinf(a) || isinf(b)) {
+        double t = isinf(a) ? (isinf(b) ? (x > 0.0 ? 0x1.921fb54442d18p-1 : 0x1.2d97c7f3321d2p+1)
+                                        : 0x1.921fb54442d18p+0)
+                            : (x > 0.0 ? 0.0 : 0x1.921fb54442d18p+1);
+        return copysign(t, y);
+    }
+    if (b == 0.0) {
+        if (a == 0.0) return signbit(x) ? copysign(0x1.921fb54442d18p+1, y) : y;
         return copysign(0x1.921fb54442d18p+0, y);
     }
-    double r = atan_d(y / x);
-    if (x > 0.0) return r;
-    return signbit(y) ? r - 0x1.921fb54442d18p+1 : r + 0x1.921fb54442d18p+1;
+    bool inv = a > b;
+    double r = atan_unit(inv ? b / a : a / b);
+    if (inv) r = 0x1.921fb54442d18p+0 - r;
+    if (signbit(x)) r = 0x1.921fb54442d18p+1 - r;
+    return copysign(r, y);
 }
 // asin and acos through atan2 of (x, sqrt((1 - x)(1 + x)))
 __device__ __forceinline__ double asin_d(double x) {
@@ -196,6 +207,21 @@ __device__ __forceinline__ float rng(uint32_t base, uint32_t bounce, uint32_t di
 }
 
 // ------------------------------------------------------------- textures --
+// Texel columns and rows of a wrap-addressed bilinear tap: with the
+// coordinate's fraction in [0, 1], floor(x) lies in [-1, w - 1], so the
+// wrap is one conditional add (an index out of range, from a NaN
+// coordinate, is taken as 0).
+__device__ __forceinline__ void wrap_texels(float fx, float fy, int w, int h, int& i0, int& j0, int& i1,
+                                            int& j1) {
+    i0 = (int)fx;
+    j0 = (int)fy;
+    if (i0 < 0) i0 += w;
+    if (j0 < 0) j0 += h;
+    if ((unsigned)i0 >= (unsigned)w) i0 = 0;
+    if ((unsigned)j0 >= (unsigned)h) j0 = 0;
+    i1 = i0 + 1 == w ? 0 : i0 + 1;
+    j1 = j0 + 1 == h ? 0 : j0 + 1;
+}
 // Software bilinear filter, wrap addressing on normalized coordinates
 // (replaces tex2DLod on linear/wrap texture objects, CRT/dTexture.cu:251-255).
 __device__ __forceinline__ void bilinear(const float* __restrict__ tex, int w, int h, int c,
@@ -204,10 +230,8 @@ __device__ __forceinline__ void bilinear(const float* __restrict__ tex, int w, i
     float x = uu * (float)w - 0.5f, y = vv * (float)h - 0.5f;
     float fx = floorf(x), fy = floorf(y);
     float a = x - fx, b = y - fy;
-    int i0 = (int)fx, j0 = (int)fy;
-    i0 = ((i0 % w) + w) % w;
-    j0 = ((j0 % h) + h) % h;
-    int i1 = (i0 + 1) % w, j1 = (j0 + 1) % h;
+    int i0, j0, i1, j1;
+    wrap_texels(fx, fy, w, h, i0, j0, i1, j1);
     float w00 = (1.f - a) * (1.f - b), w10 = a * (1.f - b), w01 = (1.f - a) * b, w11 = a * b;
     for (int k = 0; k < c; k++) {
         float t00 = tex[((size_t)j0 * w + i0) * c + k], t10 = tex[((size_t)j0 * w + i1) * c + k];
@@ -495,10 +519,8 @@ __device__ __forceinline__ Taps taps_of(int w, int h, float u, float v) {
     float x = uu * (float)w - 0.5f, y = vv * (float)h - 0.5f;
     float fx = floorf(x), fy = floorf(y);
     float a = x - fx, b = y - fy;
-    int i0 = (int)fx, j0 = (int)fy;
-    i0 = ((i0 % w) + w) % w;
-    j0 = ((j0 % h) + h) % h;
-    int i1 = (i0 + 1) % w, j1 = (j0 + 1) % h;
+    int i0, j0, i1, j1;
+    wrap_texels(fx, fy, w, h, i0, j0, i1, j1);
     Taps t;
     t.i[0] = (uint32_t)(j0 * w + i0); t.i[1] = (uint32_t)(j0 * w + i1);
     t.i[2] = (uint32_t)(j1 * w + i0); t.i[3] = (uint32_t)(j1 * w + i1);

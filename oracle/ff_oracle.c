@@ -74,11 +74,12 @@
  * same order as the GPU's routines (so host and device agree bit for bit):
  * sin/cos by Cody-Waite reduction modulo pi/2 (three-part split of pi/2,
  * 33 + 33 + 53 bits, derived from Machin's formula) and Taylor series to
- * r^17 / r^18; atan by 1/x for |x| > 1, the addition formula at pi/6 above
- * tan(pi/12) and the Taylor series to u^27; atan2 with the C library's zero
- * and sign conventions; asin/acos as atan2 of (x, sqrt((1-x)(1+x))).
- * Arguments outside the reduced ranges (|x| >= 2^19 for sin/cos, infinities
- * for atan2) use the C library.  Pinned against the C library in
+ * r^17 / r^18; atan on [0, 1] by the addition formula at pi/6 above
+ * tan(pi/12) and the Taylor series to u^27, extended by atan x = pi/2 -
+ * atan(1/x); atan2 as atan(min/max) of the magnitudes reflected into its
+ * quadrant, with the C library's zero, sign and infinity conventions;
+ * asin/acos as atan2 of (x, sqrt((1-x)(1+x))).  sin/cos arguments of
+ * magnitude 2^19 or more use the C library.  Pinned against the C library in
  * tests/test_oracle_math.py. */
 void or_sincos_d(double x, double *s, double *c) {
     if (!(fabs(x) < 0x1p19)) {
@@ -116,11 +117,7 @@ void or_sincos_d(double x, double *s, double *c) {
     *s = q == 0 ? sr : (q == 1 ? cr : (q == 2 ? -sr : -cr));
     *c = q == 0 ? cr : (q == 1 ? -sr : (q == 2 ? -cr : sr));
 }
-double or_atan_d(double x) {
-    if (x != x) return x;
-    double a = fabs(x);
-    int inv = a > 1.0;
-    double t = inv ? 1.0 / a : a;
+static double atan_unit(double t) {
     int shift = t > 0x1.126145e9ecd56p-2;
     double u = shift ? (t * 0x1.bb67ae8584caap+0 - 1.0) / (t + 0x1.bb67ae8584caap+0) : t;
     double z = u * u;
@@ -138,20 +135,34 @@ double or_atan_d(double x) {
     p = 0x1.999999999999ap-3 + z * p;
     p = -0x1.5555555555555p-2 + z * p;
     double r = u + (u * z) * p;
-    if (shift) r = 0x1.0c152382d7366p-1 + r;
+    return shift ? 0x1.0c152382d7366p-1 + r : r;
+}
+double or_atan_d(double x) {
+    if (x != x) return x;
+    double a = fabs(x);
+    int inv = a > 1.0;
+    double r = atan_unit(inv ? 1.0 / a : a);
     if (inv) r = 0x1.921fb54442d18p+0 - r;
     return copysign(r, x);
 }
 double or_atan2_d(double y, double x) {
     if (y != y || x != x) return x + y;
-    if (isinf(x) || isinf(y)) return atan2(y, x);
-    if (x == 0.0) {
-        if (y == 0.0) return signbit(x) ? copysign(0x1.921fb54442d18p+1, y) : y;
+    double a = fabs(y), b = fabs(x);
+    if (isinf(a) || isinf(b)) {
+        double t = isinf(a) ? (isinf(b) ? (x > 0.0 ? 0x1.921fb54442d18p-1 : 0x1.2d97c7f3321d2p+1)
+                                        : 0x1.921fb54442d18p+0)
+                            : (x > 0.0 ? 0.0 : 0x1.921fb54442d18p+1);
+        return copysign(t, y);
+    }
+    if (b == 0.0) {
+        if (a == 0.0) return signbit(x) ? copysign(0x1.921fb54442d18p+1, y) : y;
         return copysign(0x1.921fb54442d18p+0, y);
     }
-    double r = or_atan_d(y / x);
-    if (x > 0.0) return r;
-    return signbit(y) ? r - 0x1.921fb54442d18p+1 : r + 0x1.921fb54442d18p+1;
+    int inv = a > b;
+    double r = atan_unit(inv ? b / a : a / b);
+    if (inv) r = 0x1.921fb54442d18p+0 - r;
+    if (signbit(x)) r = 0x1.921fb54442d18p+1 - r;
+    return copysign(r, y);
 }
 double or_asin_d(double x) {
     if (!(fabs(x) <= 1.0)) return (x - x) / (x - x);
@@ -805,6 +816,20 @@ void or_trace_walk(const or_node *nodes, const float *tris, const float *rays, i
 }
 
 /* --------------------------------------------------------- environment -- */
+/* Texel columns and rows of a wrap-addressed bilinear tap.  The coordinate's
+ * fraction lies in [0, 1], so floor(x) lies in [-1, w - 1] and the wrap is one
+ * conditional add (the same result as the remainder (i % w + w) % w there);
+ * an index out of range, from a NaN coordinate, is taken as 0. */
+static void wrap_texels(float fx, float fy, int32_t w, int32_t h, int *i0, int *j0, int *i1, int *j1) {
+    *i0 = isnan(fx) ? 0 : (int)fx;
+    *j0 = isnan(fy) ? 0 : (int)fy;
+    if (*i0 < 0) *i0 += w;
+    if (*j0 < 0) *j0 += h;
+    if ((unsigned)*i0 >= (unsigned)w) *i0 = 0;
+    if ((unsigned)*j0 >= (unsigned)h) *j0 = 0;
+    *i1 = *i0 + 1 == w ? 0 : *i0 + 1;
+    *j1 = *j0 + 1 == h ? 0 : *j0 + 1;
+}
 /* Software bilinear filter with wrap addressing on normalized coordinates
  * (replaces tex2DLod of a linear/wrap texture, CRT/dTexture.cu:251-255;
  * hardware filter weights are parity unpinned). */
@@ -813,10 +838,8 @@ void or_bilinear(const float *tex, int32_t w, int32_t h, int32_t c, float u, flo
     float x = uu * (float)w - 0.5f, y = vv * (float)h - 0.5f;
     float fx = floorf(x), fy = floorf(y);
     float a = x - fx, b = y - fy;
-    int i0 = (int)fx, j0 = (int)fy;
-    i0 = ((i0 % w) + w) % w;
-    j0 = ((j0 % h) + h) % h;
-    int i1 = (i0 + 1) % w, j1 = (j0 + 1) % h;
+    int i0, j0, i1, j1;
+    wrap_texels(fx, fy, w, h, &i0, &j0, &i1, &j1);
     float w00 = (1.f - a) * (1.f - b), w10 = a * (1.f - b), w01 = (1.f - a) * b, w11 = a * b;
     for (int k = 0; k < c; k++) {
         float t00 = tex[((size_t)j0 * w + i0) * c + k], t10 = tex[((size_t)j0 * w + i1) * c + k];
@@ -1509,10 +1532,8 @@ static void taps_of(int32_t w, int32_t h, float u, float v, size_t idx[4], doubl
     float x = uu * (float)w - 0.5f, y = vv * (float)h - 0.5f;
     float fx = floorf(x), fy = floorf(y);
     float a = x - fx, b = y - fy;
-    int i0 = (int)fx, j0 = (int)fy;
-    i0 = ((i0 % w) + w) % w;
-    j0 = ((j0 % h) + h) % h;
-    int i1 = (i0 + 1) % w, j1 = (j0 + 1) % h;
+    int i0, j0, i1, j1;
+    wrap_texels(fx, fy, w, h, &i0, &j0, &i1, &j1);
     idx[0] = (size_t)j0 * w + i0; idx[1] = (size_t)j0 * w + i1;
     idx[2] = (size_t)j1 * w + i0; idx[3] = (size_t)j1 * w + i1;
     double A = a, B = b;
