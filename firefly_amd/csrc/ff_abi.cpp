@@ -13,6 +13,7 @@
 #include <limits>
 #include <cstdarg>
 #include <cstdio>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -403,6 +404,11 @@ ff_status ff_scene_create(ff_context* ctx, const ff_scene_desc* d, ff_scene** ou
         return fail(FF_ERR_CAPACITY, "ff_scene_create: %d materials (limit 2^30 - 1)", d->n_materials);
     if (d->n_tris >= (1 << 27))
         return fail(FF_ERR_CAPACITY, "ff_scene_create: %d triangles (limit 2^27 - 1)", d->n_tris);
+    // finite vertices give boxes with lo <= hi, which the traversal's box test relies on
+    for (size_t k = 0; k < (size_t)d->n_tris * 9; k++)
+        if (!std::isfinite(d->positions[k]))
+            return fail(FF_ERR_INVALID, "ff_scene_create: triangle %d has a non-finite vertex coordinate",
+                        (int)(k / 9));
     for (int32_t i = 0; i < d->n_tris; i++)
         if (d->material_ids[i] < 0 || d->material_ids[i] >= d->n_materials)
             return fail(FF_ERR_INVALID, "ff_scene_create: triangle %d material %d out of range", i, d->material_ids[i]);

@@ -39,6 +39,8 @@ enum : uint32_t {
 constexpr float PRUNE_REL = 0x1p-12f;
 constexpr float PRUNE_ABS = 1e-6f;
 constexpr float PRUNE_BEHIND = 1e-4f;
+// TravRay::negm flag: 1/d.x or o.x is not finite, so a slab's x distance can be NaN
+constexpr uint32_t XNAN_RAY = 8u;
 
 // Transcendentals are evaluated in double and rounded to float: the CUDA
 // math library of the reference is closed.  The double routines below are
@@ -579,6 +581,27 @@ __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, flo
     return ok_xy & ok_z;
 }
 
+// The same test in fewer instructions, for boxes with lo <= hi (every box of
+// a scene with finite vertices) and rays whose x distances cannot be NaN
+// (finite 1/d.x and o.x): entry = max, exit = min of the three slab
+// distances, hit = !(entry > exit).  A NaN y or z distance (0 * inf: origin
+// on a slab plane, zero direction component) drops out of the reference's
+// comparisons and out of fmaxf/fminf alike, so the outcome, entry and exit
+// equal slab()'s.  (A NaN x distance would also hide the y distance from
+// the reference's z comparison; such rays take slab().)
+__device__ __forceinline__ bool slab_box(float lx, float ly, float lz, float hx, float hy, float hz, f3 o,
+                                         f3 inv, bool nx, bool ny, bool nz, float& tn, float& tf) {
+    float x0 = ((nx ? hx : lx) - o.x) * inv.x;
+    float x1 = ((nx ? lx : hx) - o.x) * inv.x;
+    float y0 = ((ny ? hy : ly) - o.y) * inv.y;
+    float y1 = ((ny ? ly : hy) - o.y) * inv.y;
+    float z0 = ((nz ? hz : lz) - o.z) * inv.z;
+    float z1 = ((nz ? lz : hz) - o.z) * inv.z;
+    tn = fmaxf(fmaxf(x0, y0), z0);
+    tf = fminf(fminf(x1, y1), z1);
+    return !(tn > tf);
+}
+
 // Triangle record in BVH-slot order: v0, e1 = v1 - v0, e2 = v2 - v0 (the
 // edges the reference forms per test, CRT/dTriangle.cu:35-36).
 struct Tri { f3 v0, e1, e2; };
@@ -701,7 +724,7 @@ __device__ __forceinline__ void leaf_range(const Bvh& B, int ref, int& off, int&
 struct TravRay {
     f3 o, d, inv;
     bool nx, ny, nz;
-    uint32_t negm;  // nx | ny << 1 | nz << 2
+    uint32_t negm;  // nx | ny << 1 | nz << 2 | XNAN_RAY
     int ref;
 };
 
@@ -715,7 +738,8 @@ __device__ __forceinline__ bool trav_begin(const Bvh& B, f3 o, f3 d, TravRay& R,
     R.nx = R.inv.x < 0;
     R.ny = R.inv.y < 0;
     R.nz = R.inv.z < 0;
-    R.negm = (R.nx ? 1u : 0u) | (R.ny ? 2u : 0u) | (R.nz ? 4u : 0u);
+    R.negm = (R.nx ? 1u : 0u) | (R.ny ? 2u : 0u) | (R.nz ? 4u : 0u) |
+             ((isfinite(R.inv.x) && isfinite(o.x)) ? 0u : XNAN_RAY);
     float tn, tf;
     cnt_nodes++;
     float4 r0 = ldg4((const float4*)B.root), r1 = ldg4((const float4*)B.root + 1);
@@ -740,14 +764,19 @@ __device__ __forceinline__ bool visit_interior(const Bvh& B, TravRay& R, float l
     cnt_nodes += 2;
     float tn0, tf0, tn1, tf1;
     bool h0, h1;
-    if (PRUNE) {
-        h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, R.o, R.inv, R.nx, R.ny, R.nz, tn0, tf0) &&
-             !(tn0 > lim || tf0 < -PRUNE_BEHIND);
-        h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, R.o, R.inv, R.nx, R.ny, R.nz, tn1, tf1) &&
-             !(tn1 > lim || tf1 < -PRUNE_BEHIND);
+    if (__ballot((R.negm & XNAN_RAY) != 0) == 0) {  // wave-uniform: no lane can see a NaN x distance
+        h0 = slab_box(a.x, a.y, a.z, a.w, b.x, b.y, R.o, R.inv, R.nx, R.ny, R.nz, tn0, tf0);
+        h1 = slab_box(b.z, b.w, c.x, c.y, c.z, c.w, R.o, R.inv, R.nx, R.ny, R.nz, tn1, tf1);
     } else {
-        h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, R.o, R.inv, R.nx, R.ny, R.nz, tn0, tf0) && !(tf0 < -PRUNE_BEHIND);
-        h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, R.o, R.inv, R.nx, R.ny, R.nz, tn1, tf1) && !(tf1 < -PRUNE_BEHIND);
+        h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, R.o, R.inv, R.nx, R.ny, R.nz, tn0, tf0);
+        h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, R.o, R.inv, R.nx, R.ny, R.nz, tn1, tf1);
+    }
+    if (PRUNE) {
+        h0 = h0 && !(tn0 > lim || tf0 < -PRUNE_BEHIND);
+        h1 = h1 && !(tn1 > lim || tf1 < -PRUNE_BEHIND);
+    } else {
+        h0 = h0 && !(tf0 < -PRUNE_BEHIND);
+        h1 = h1 && !(tf1 < -PRUNE_BEHIND);
     }
     bool neg = ((R.negm >> (uint32_t)m.z) & 1u) != 0;  // dirIsNeg[axis]
     int nref = neg ? m.y : m.x, fref = neg ? m.x : m.y;

@@ -1,6 +1,7 @@
 """Inputs at the edges of the domain, on the GPU against the oracle:
-non-finite and zero-direction rays, axis-aligned rays, zero-area
-triangles, an all-zero environment, an empty scene and invalid requests.
+non-finite and zero-direction rays, axis-aligned rays, rays in box planes,
+zero-area triangles, non-finite vertices, an all-zero environment, an empty
+scene and invalid requests.
 Every case must finish normally with the oracle's result or a clean error."""
 import numpy as np
 import pytest
@@ -61,6 +62,55 @@ def test_non_finite_zero_and_axis_aligned_rays(gpu, oracle):
     assert np.all(h[[0, 1, 2, 4], 3].view(np.int32) == -1) and not occ[[0, 1, 2, 4]].any()
     assert (h[5:, 3].view(np.int32) >= 0).sum() > 100
     sc.close()
+
+
+def test_rays_in_box_planes(gpu, oracle):
+    """Rays whose origin lies exactly in a BVH box's plane while their
+    direction has a zero component along that axis: the slab distance is
+    0 * inf = NaN there.  Hits, occlusion and the per-ray box and triangle
+    counts equal the oracle's (the reference's comparisons on NaN)."""
+    import torch
+    sd = scenes.spheres_scene(segments=16, rings=8)
+    sc = abi.Scene(gpu, sd.positions, sd.normals, sd.uvs, sd.material_ids, sd.materials)
+    nodes, ordered = sc.bvh()
+    ps = oracle.PreparedScene(sd, nodes, ordered)
+    rng = np.random.default_rng(9)
+    pick = rng.integers(0, len(nodes), 600)
+    rays = np.zeros((600, 8), np.float32)
+    for k, ni in enumerate(pick):
+        lo, hi = nodes[ni]["bmin"], nodes[ni]["bmax"]
+        o = rng.uniform(lo, hi).astype(np.float32)
+        a = rng.integers(0, 3)
+        o[a] = lo[a] if rng.random() < 0.5 else hi[a]   # on the box's plane
+        d = rng.normal(size=3).astype(np.float32)
+        d[a] = 0.0 if rng.random() < 0.5 else -0.0       # parallel to it
+        if rng.random() < 0.3:                            # and axis-aligned
+            d[(a + 1) % 3] = 0.0
+        rays[k, :3], rays[k, 4:7] = o, d / np.linalg.norm(d)
+    h, occ = _trace(sc, rays)
+    _check(h, occ, ps, oracle, rays, "box-plane rays")
+    abi.set_traversal_variant(gpu, 0)  # one step per iteration: counts equal the oracle's walk
+    r = torch.from_numpy(rays).cuda()
+    for any_hit in (False, True):
+        cnt = torch.zeros(2, dtype=torch.int64, device="cuda")
+        if any_hit:
+            sc.trace_any(r, torch.empty(len(rays), dtype=torch.uint8, device="cuda"), counts_dev=cnt)
+        else:
+            sc.trace_closest(r, torch.empty((len(rays), 4), dtype=torch.float32, device="cuda"), counts_dev=cnt)
+        torch.cuda.synchronize()
+        want = oracle.count_traversal(ps.nodes, ps.tris_ordered(), rays, any_hit=any_hit, prune=True)
+        assert tuple(cnt.cpu().tolist()) == tuple(want), (any_hit, cnt, want)
+    sc.close()
+
+
+def test_non_finite_vertices_are_rejected(gpu):
+    sd = scenes.spheres_scene(segments=8, rings=4)
+    for bad in (np.nan, np.inf, -np.inf):
+        pos = sd.positions.copy()
+        pos.reshape(-1)[13] = bad  # triangle 1, second vertex, y
+        with pytest.raises(abi.FFError) as e:
+            abi.Scene(gpu, pos, sd.normals, sd.uvs, sd.material_ids, sd.materials)
+        assert e.value.status == 1 and "non-finite" in str(e.value)
 
 
 def test_zero_area_triangles(gpu, oracle):
