@@ -16,6 +16,7 @@ FF_MAT_MIX, FF_MAT_DIFFUSE, FF_MAT_CONDUCTOR, FF_MAT_DIELECTRIC = 0, 1, 2, 3
 FF_RENDER_USE_GRAPH = 1
 FF_RENDER_TIMING = 2
 FF_ADJOINT_DETERMINISTIC = 4
+FF_BVH_SAH, FF_BVH_LBVH = 0, 1  # host SAH (the reference's tree) / linear BVH built on the GPU
 
 NODE_DTYPE = np.dtype([("bmin", "<f4", 3), ("bmax", "<f4", 3), ("offset", "<i4"),
                        ("n_prims", "<u2"), ("axis", "u1"), ("pad", "u1")])
@@ -33,7 +34,7 @@ class SceneDesc(C.Structure):
                 ("uvs", C.c_void_p), ("material_ids", C.c_void_p), ("n_materials", C.c_int32),
                 ("materials", C.c_void_p), ("n_textures", C.c_int32), ("textures", C.c_void_p),
                 ("tex_w", C.c_void_p), ("tex_h", C.c_void_p), ("tex_c", C.c_void_p),
-                ("max_prims_in_node", C.c_int32)]
+                ("max_prims_in_node", C.c_int32), ("builder", C.c_int32)]
 
 
 class Camera(C.Structure):
@@ -181,10 +182,11 @@ class Context:
 
 
 class Scene:
-    """Device scene built from host arrays (positions/normals/uvs [n,3,...])."""
+    """Device scene built from host arrays (positions/normals/uvs [n,3,...]); the BVH by the
+    reference's host SAH (FF_BVH_SAH) or on the GPU as a linear BVH (FF_BVH_LBVH)."""
 
     def __init__(self, ctx: Context, positions, normals, uvs, material_ids, materials,
-                 textures: Sequence[np.ndarray] = (), max_prims: int = 8):
+                 textures: Sequence[np.ndarray] = (), max_prims: int = 8, builder: int = FF_BVH_SAH):
         self.ctx = ctx
         self._keep = []
         pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 9)
@@ -210,6 +212,7 @@ class Scene:
             d.tex_w, d.tex_h, d.tex_c = tw.ctypes.data, th.ctypes.data, tc.ctypes.data
             self._keep += [ptrs, tw, th, tc]
         d.max_prims_in_node = max_prims
+        d.builder = builder
         self._keep += [pos, nrm, uv, mid, mats, texs]
         h = _P()
         check(_lib().ff_scene_create(ctx.h, C.byref(d), C.byref(h)))

@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libff_hip.so")
 HOST_LIB = os.path.join(HERE, "libfirefly.so")
 DEMO = os.path.join(HERE, "render_demo")
 
-SOURCES = ["ff_kernels.hip", "ff_abi.cpp", "ff_bvh.cpp"]
+SOURCES = ["ff_kernels.hip", "ff_lbvh.hip", "ff_abi.cpp", "ff_bvh.cpp"]
 
 # -ffp-contract=off: every float/double operation rounds on its own, as the
 # reference's arithmetic is written (bit-exact hits against the host oracle).

@@ -400,6 +400,8 @@ ff_status ff_scene_create(ff_context* ctx, const ff_scene_desc* d, ff_scene** ou
     if (d->n_tris <= 0 || !d->positions || !d->normals || !d->material_ids)
         return fail(FF_ERR_INVALID, "ff_scene_create: empty scene or missing positions/normals/material ids");
     if (d->n_materials <= 0 || !d->materials) return fail(FF_ERR_INVALID, "ff_scene_create: no materials");
+    if (d->builder != FF_BVH_SAH && d->builder != FF_BVH_LBVH)
+        return fail(FF_ERR_INVALID, "ff_scene_create: unknown BVH builder %d", d->builder);
     if ((uint32_t)d->n_materials > MAT_INDEX_MASK)
         return fail(FF_ERR_CAPACITY, "ff_scene_create: %d materials (limit 2^30 - 1)", d->n_materials);
     if (d->n_tris >= (1 << 27))
@@ -435,7 +437,15 @@ ff_status ff_scene_create(ff_context* ctx, const ff_scene_desc* d, ff_scene** ou
     s->n_tris = d->n_tris;
     s->n_mats = d->n_materials;
     int32_t max_prims = d->max_prims_in_node > 0 ? d->max_prims_in_node : 8;
-    ffh::build_bvh_sah(d->positions, d->n_tris, max_prims, s->nodes, s->ordered);
+    if (d->builder == FF_BVH_LBVH) {
+        hipError_t be = ffk::build_lbvh(d->positions, d->n_tris, max_prims, s->nodes, s->ordered);
+        if (be != hipSuccess) {
+            ff_scene_destroy(s);
+            return fail(FF_ERR_HIP, "ff_scene_create: GPU BVH build failed: %s", hipGetErrorString(be));
+        }
+    } else {
+        ffh::build_bvh_sah(d->positions, d->n_tris, max_prims, s->nodes, s->ordered);
+    }
     s->max_depth = ffh::bvh_max_depth(s->nodes);
     size_t n = (size_t)d->n_tris;
     // reorder_primitives (CRT/kernel.cu:184-191,701-712) done on the host while

@@ -67,3 +67,10 @@ void build_bvh_sah(const float* positions, int32_t n, int32_t max_prims,
                    std::vector<BvhNode>& nodes, std::vector<int32_t>& ordered);
 int32_t bvh_max_depth(const std::vector<BvhNode>& nodes);
 }  // namespace ffh
+
+namespace ffk {
+// Linear BVH on the GPU (Morton order, Karras hierarchy), in the SAH builder's
+// node format; positions on the host, nodes and permutation returned there.
+hipError_t build_lbvh(const float* positions, int32_t n, int32_t max_prims, std::vector<ffh::BvhNode>& nodes,
+                      std::vector<int32_t>& ordered);
+}  // namespace ffk

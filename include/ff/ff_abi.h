@@ -71,7 +71,17 @@ typedef struct {
     const int32_t *tex_h;
     const int32_t *tex_c;      /* 1 or 3 */
     int32_t max_prims_in_node; /* SAH leaf size, 8 in the reference (CRT/dScene.cpp:372) */
+    int32_t builder;           /* FF_BVH_SAH (0) or FF_BVH_LBVH */
 } ff_scene_desc;
+
+/* BVH builders.  FF_BVH_SAH: the reference's host SAH tree (BVHAccel,
+ * CRT/BVH.cpp:81-307), the default.  FF_BVH_LBVH: a linear BVH built on the
+ * GPU (Morton order, Karras hierarchy; the reference declares HLBVH,
+ * CRT/BVH.h:23, but never builds it), leaves of at most max_prims_in_node
+ * triangles.  No reference counterpart for the tree itself; closest hits are
+ * the same triangles (the closest hit does not depend on the tree). */
+#define FF_BVH_SAH 0
+#define FF_BVH_LBVH 1
 
 /* dCamera (CRT/dCamera.cuh:9-22): inverse(proj * view), column-major. */
 typedef struct {

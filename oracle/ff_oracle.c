@@ -7,6 +7,9 @@
  *
  * What pins each function (tests/test_oracle_*.py):
  *   or_rng_*              : uniformity / independence statistics (chi^2), range [0,1).
+ *   or_lbvh_build         : the GPU linear-BVH builder restated (no reference
+ *                           counterpart): the same invariants, traversal equal to
+ *                           brute force, node-for-node equality with the GPU tree.
  *   or_bvh_build          : invariants (every primitive exactly once, child bounds
  *                           inside parent, leaf size <= max unless centroids are
  *                           degenerate, SAH split decision re-derived by brute
@@ -423,6 +426,188 @@ static int depth_rec(const or_node *nodes, int32_t i) {
     int a = depth_rec(nodes, i + 1), b = depth_rec(nodes, nodes[i].offset);
     return 1 + (a > b ? a : b);
 }
+/* ---- linear BVH (the product's GPU builder, FF_BVH_LBVH, restated) ----
+ * No reference counterpart (CRT/BVH.h:23 declares HLBVH, never built).
+ * Steps as firefly_amd/csrc/ff_lbvh.hip: centroid bounds, 30-bit Morton key
+ * << 32 | triangle index, sort, Karras' hierarchy over the sorted keys
+ * (HPG 2012), subtrees of at most max_prims triangles as leaves, boxes and a
+ * depth-first flattening (first child at +1).  Pinned by tree invariants,
+ * traversal equal to brute force, and node-for-node equality with the GPU. */
+static float lb_mn(float a, float b) { return b < a ? b : a; }
+static float lb_mx(float a, float b) { return b > a ? b : a; }
+static uint32_t lb_f2key(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+static float lb_key2f(uint32_t k) {
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+static void lb_tri_box(const float *pos, int32_t i, float lo[3], float hi[3]) {
+    const float *p = pos + (size_t)i * 9;
+    for (int a = 0; a < 3; a++) {
+        lo[a] = lb_mn(lb_mn(p[a], p[3 + a]), p[6 + a]);
+        hi[a] = lb_mx(lb_mx(p[a], p[3 + a]), p[6 + a]);
+    }
+}
+static uint32_t lb_spread10(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+static uint32_t lb_quant10(float c, float lo, float ext) {
+    float u = ext > 0.f ? (c - lo) / ext : 0.f;
+    float q = u * 1024.f;
+    if (!(q > 0.f)) q = 0.f;
+    if (q > 1023.f) q = 1023.f;
+    return (uint32_t)q;
+}
+static int lb_cmp_u64(const void *a, const void *b) {
+    uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+static int lb_delta(const uint64_t *k, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    return __builtin_clzll(k[i] ^ k[j]);
+}
+typedef struct {
+    const float *pos;
+    const uint64_t *keys;
+    int32_t *cl, *cr, *first, *last; /* internal nodes; child c >= 0 internal, < 0 primitive ~c */
+    uint8_t *axis;
+    int max_prims;
+    or_node *out;
+} lb_ctx;
+static void lb_range_box(const lb_ctx *L, int a, int b, float bx[6]) {
+    lb_tri_box(L->pos, (int32_t)(L->keys[a] & 0xffffffffu), bx, bx + 3);
+    for (int s = a + 1; s <= b; s++) {
+        float lo[3], hi[3];
+        lb_tri_box(L->pos, (int32_t)(L->keys[s] & 0xffffffffu), lo, hi);
+        for (int c = 0; c < 3; c++) {
+            bx[c] = lb_mn(bx[c], lo[c]);
+            bx[3 + c] = lb_mx(bx[3 + c], hi[c]);
+        }
+    }
+}
+/* emits node v (internal >= 0, primitive < 0) at idx; returns its subtree size */
+static int lb_emit(const lb_ctx *L, int v, int idx, float bx[6]) {
+    or_node *r = &L->out[idx];
+    r->pad = 0;
+    int a = v >= 0 ? L->first[v] : ~v, z = v >= 0 ? L->last[v] : ~v;
+    if (v < 0 || z - a + 1 <= L->max_prims) {
+        lb_range_box(L, a, z, bx);
+        r->offset = a;
+        r->n_prims = (uint16_t)(z - a + 1);
+        r->axis = 0;
+    } else {
+        float b0[6], b1[6];
+        int s0 = lb_emit(L, L->cl[v], idx + 1, b0);
+        int s1 = lb_emit(L, L->cr[v], idx + 1 + s0, b1);
+        for (int c = 0; c < 3; c++) {
+            bx[c] = lb_mn(b0[c], b1[c]);
+            bx[3 + c] = lb_mx(b0[3 + c], b1[3 + c]);
+        }
+        r->offset = idx + 1 + s0;
+        r->n_prims = 0;
+        r->axis = L->axis[v];
+        for (int c = 0; c < 3; c++) {
+            r->bmin[c] = bx[c];
+            r->bmax[c] = bx[3 + c];
+        }
+        return 1 + s0 + s1;
+    }
+    for (int c = 0; c < 3; c++) {
+        r->bmin[c] = bx[c];
+        r->bmax[c] = bx[3 + c];
+    }
+    return 1;
+}
+int32_t or_lbvh_build(const float *pos, int32_t n, int32_t max_prims, or_node *nodes_out,
+                      int32_t *ordered_out) {
+    if (n <= 0) return 0;
+    if (max_prims < 1) max_prims = 1;
+    uint32_t kmin[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, kmax[3] = {0, 0, 0};
+    for (int32_t i = 0; i < n; i++) {
+        float lo[3], hi[3];
+        lb_tri_box(pos, i, lo, hi);
+        for (int a = 0; a < 3; a++) {
+            uint32_t k = lb_f2key(0.5f * lo[a] + 0.5f * hi[a]);
+            if (k < kmin[a]) kmin[a] = k;
+            if (k > kmax[a]) kmax[a] = k;
+        }
+    }
+    float clo[3], ext[3];
+    for (int a = 0; a < 3; a++) {
+        clo[a] = lb_key2f(kmin[a]);
+        ext[a] = lb_key2f(kmax[a]) - clo[a];
+    }
+    uint64_t *keys = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)n);
+    for (int32_t i = 0; i < n; i++) {
+        float lo[3], hi[3];
+        lb_tri_box(pos, i, lo, hi);
+        uint32_t q[3];
+        for (int a = 0; a < 3; a++) q[a] = lb_quant10(0.5f * lo[a] + 0.5f * hi[a], clo[a], ext[a]);
+        uint32_t code = (lb_spread10(q[0]) << 2) | (lb_spread10(q[1]) << 1) | lb_spread10(q[2]);
+        keys[i] = ((uint64_t)code << 32) | (uint32_t)i;
+    }
+    qsort(keys, (size_t)n, sizeof(uint64_t), lb_cmp_u64);
+    size_t m = n > 1 ? (size_t)(n - 1) : 1;
+    lb_ctx L;
+    L.pos = pos;
+    L.keys = keys;
+    L.cl = (int32_t *)malloc(4 * m);
+    L.cr = (int32_t *)malloc(4 * m);
+    L.first = (int32_t *)malloc(4 * m);
+    L.last = (int32_t *)malloc(4 * m);
+    L.axis = (uint8_t *)malloc(m);
+    L.max_prims = max_prims;
+    L.out = nodes_out;
+    for (int i = 0; i < n - 1; i++) {
+        int d = lb_delta(keys, n, i, i + 1) > lb_delta(keys, n, i, i - 1) ? 1 : -1;
+        int dmin = lb_delta(keys, n, i, i - d);
+        int lmax = 2;
+        while (lb_delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+        int lo = 0, hi = lmax;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (lb_delta(keys, n, i, i + mid * d) > dmin) lo = mid;
+            else hi = mid;
+        }
+        int j = i + lo * d;
+        int dnode = lb_delta(keys, n, i, j);
+        hi = lo;
+        lo = 0;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (lb_delta(keys, n, i, i + mid * d) > dnode) lo = mid;
+            else hi = mid;
+        }
+        int g = i + lo * d + (d < 0 ? -1 : 0);
+        int first = i < j ? i : j, last = i < j ? j : i;
+        L.cl[i] = first == g ? ~g : g;
+        L.cr[i] = last == g + 1 ? ~(g + 1) : g + 1;
+        L.first[i] = first;
+        L.last[i] = last;
+        int p = 63 - dnode;
+        L.axis[i] = p >= 32 ? (uint8_t)(2 - (p - 32) % 3) : (uint8_t)0;
+    }
+    float bx[6];
+    int32_t n_out = lb_emit(&L, n > 1 ? 0 : ~0, 0, bx);
+    for (int32_t s = 0; s < n; s++) ordered_out[s] = (int32_t)(keys[s] & 0xffffffffu);
+    free(keys);
+    free(L.cl);
+    free(L.cr);
+    free(L.first);
+    free(L.last);
+    free(L.axis);
+    return n_out;
+}
+
 int32_t or_bvh_max_depth(const or_node *nodes, int32_t n_nodes) {
     return n_nodes > 0 ? depth_rec(nodes, 0) : 0;
 }

@@ -104,3 +104,70 @@ def test_parallel_product_bvh_bit_identical_to_oracle(oracle):
         assert len(n_o) > 60000
         assert n_p.tobytes() == n_o.tobytes()
         assert np.array_equal(o_p, o_o)
+
+
+# ---- linear BVH (the GPU builder restated by the oracle) ----
+
+def _leaf_sizes(nodes):
+    return nodes["n_prims"][nodes["n_prims"] > 0]
+
+
+@pytest.mark.parametrize("seed,n,max_prims", [(0, 1, 8), (1, 2, 8), (2, 5, 8), (3, 9, 8), (4, 1000, 8),
+                                              (5, 20000, 4), (6, 3000, 1)])
+def test_lbvh_invariants(oracle, seed, n, max_prims):
+    """Every triangle once, boxes exact unions of their children, pre-order
+    layout, leaves of at most max_prims triangles (Morton ranges of any
+    size split down to that)."""
+    pos = _soup(n, seed)
+    nodes, ordered = oracle.lbvh_build(pos, max_prims)
+    _check_invariants(nodes, ordered, pos, max_prims)
+    assert len(nodes) <= 2 * n - 1 and _leaf_sizes(nodes).max() <= max_prims
+    assert _leaf_sizes(nodes).sum() == n
+
+
+def test_lbvh_identical_and_degenerate_triangles(oracle):
+    """Equal Morton codes (identical triangles, one centroid for all) are
+    split by the triangle index in the key: leaves still hold at most
+    max_prims triangles, in index order."""
+    tri = np.array([[-1, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    pos = np.repeat(tri[None], 37, axis=0)
+    nodes, ordered = oracle.lbvh_build(pos, 4)
+    _check_invariants(nodes, ordered, pos, 4)
+    assert ordered.tolist() == list(range(37)) and _leaf_sizes(nodes).max() <= 4
+
+
+def test_lbvh_morton_order_of_a_grid(oracle):
+    """Triangles centred on a 4x4x4 grid: the slot order is the Morton (Z)
+    order of the cells, x the most significant of each bit triple."""
+    g = np.stack(np.meshgrid(np.arange(4), np.arange(4), np.arange(4), indexing="ij"), -1).reshape(-1, 3)
+    rng = np.random.default_rng(3)
+    perm = rng.permutation(len(g))
+    tri = np.array([[-0.1, -0.1, 0.0], [0.1, -0.1, 0.0], [0.0, 0.1, 0.0]], np.float32)
+    pos = (g[perm][:, None, :].astype(np.float32) + tri[None])
+    pos[:, :, 2] = g[perm][:, None, 2]  # flat in z: the centre is the grid point
+    nodes, ordered = oracle.lbvh_build(pos, 1)
+
+    def morton(c):
+        code = 0
+        for b in range(2):
+            for a, sh in ((0, 2), (1, 1), (2, 0)):
+                code |= ((int(c[a]) >> b) & 1) << (3 * b + sh)
+        return code
+    want = sorted(range(len(g)), key=lambda i: (morton(g[perm][i]), i))
+    assert ordered.tolist() == want
+
+
+def test_lbvh_traversal_equals_brute_force(oracle):
+    """Closest and any hits through the linear BVH are those of a test of
+    every triangle (same t bits; the same triangle wherever t is unique)."""
+    sd = scenes.spheres_scene(segments=16, rings=8)
+    nodes, ordered = oracle.lbvh_build(sd.positions)
+    tris = oracle.ordered_tris(sd.positions, ordered)
+    rays = scenes.random_rays(4000, 21)
+    t, u, v, i = oracle.trace_closest(nodes, tris, rays)
+    tb, ib = oracle.trace_closest_brute(tris, rays)
+    hit = ib >= 0
+    assert np.array_equal(i >= 0, hit)
+    assert np.array_equal(t[hit].view(np.uint32), tb[hit].view(np.uint32))
+    assert np.mean(i[hit] == ib[hit]) > 0.999
+    assert np.array_equal(oracle.trace_any(nodes, tris, rays), oracle.trace_any_brute(tris, rays))
