@@ -450,7 +450,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variant", type=int, default=-1, help="traversal kernel variant (-1: autotune)")
-    ap.add_argument("--workload", choices=("all", "frame", "batch", "adjoint", "multiview"), default="all")
+    ap.add_argument("--workload", choices=("all", "frame", "batch", "adjoint", "multiview", "bvh"), default="all")
     ap.add_argument("--adj-spp-batch", type=int, default=16)
     ap.add_argument("--views", type=int, default=32)
     ap.add_argument("--requests", type=int, default=64)
@@ -468,8 +468,8 @@ def main() -> int:
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         return 2
 
-    if args.workload in ("batch", "adjoint", "multiview"):
-        fn = {"batch": run_batch, "adjoint": run_adjoint, "multiview": run_multiview}[args.workload]
+    if args.workload in ("batch", "adjoint", "multiview", "bvh"):
+        fn = {"batch": run_batch, "adjoint": run_adjoint, "multiview": run_multiview, "bvh": run_bvh}[args.workload]
         out = fn(args)
         out.update({"metric": METRIC, "n_gpus": 1})
         print(json.dumps(out), flush=True)
@@ -478,7 +478,8 @@ def main() -> int:
     if args.workload == "all":
         import torch
         torch.cuda.empty_cache()
-        for name, fn in (("adjoint", run_adjoint), ("multiview", run_multiview), ("batch", run_batch)):
+        for name, fn in (("adjoint", run_adjoint), ("multiview", run_multiview), ("batch", run_batch),
+                         ("bvh_build", run_bvh)):
             log(f"workload {name} ...")
             t0 = time.perf_counter()
             sub = dict(args.__dict__)
@@ -493,6 +494,62 @@ def main() -> int:
             torch.cuda.empty_cache()
     print(json.dumps(out), flush=True)
     return 0
+
+
+def run_bvh(args) -> dict:
+    """BVH construction for the frame's mesh: the reference's host SAH tree
+    (ff_bvh_build_host, all host threads) against the GPU linear BVH
+    (ff_bvh_build_gpu: device time of its kernels, and wall time with the
+    transfers), then scene creation with each builder and the frame traced
+    through each tree at the frame's launch shape."""
+    import torch
+    from firefly_amd import abi, scenes
+
+    sd = scenes.icosphere_scene(args.width, args.height, args.spp, args.depth, seed=0, subdiv=args.subdiv)
+    ctx = abi.Context(0)
+    variant = args.variant if args.variant >= 0 else getattr(args, "tuned_variant", 0)
+    abi.set_traversal_variant(ctx, variant)
+    abi.bvh_build_gpu(ctx, sd.positions)  # warm-up: code objects, allocator
+    t0 = time.perf_counter()
+    _, _, sah_depth = abi.bvh_build_host(sd.positions)
+    sah_s = time.perf_counter() - t0
+    best_wall, best_ms = float("inf"), float("inf")
+    for _ in range(3):
+        t0 = time.perf_counter()
+        nodes, _, lb_depth, ms = abi.bvh_build_gpu(ctx, sd.positions)
+        best_wall, best_ms = min(best_wall, time.perf_counter() - t0), min(best_ms, ms)
+    log(f"bvh: host SAH {sah_s:.3f} s (depth {sah_depth}); GPU LBVH {best_ms:.2f} ms kernels, "
+        f"{best_wall * 1e3:.1f} ms with transfers ({len(nodes)} nodes, depth {lb_depth})")
+    env = abi.EnvMap(ctx, sd.env)
+    cam = abi.make_camera(sd.inv_vp, sd.lens_radius, sd.focal, sd.exposure)
+    film = torch.zeros((sd.height, sd.width, 4), dtype=torch.float32, device="cuda")
+    out = {"tris": sd.n_tris, "host_sah_build_s": round(sah_s, 3), "sah_depth": sah_depth,
+           "gpu_lbvh_build_ms": round(best_ms, 3), "gpu_lbvh_build_wall_ms": round(best_wall * 1e3, 2),
+           "lbvh_depth": lb_depth, "build_speedup": round(sah_s / (best_ms / 1e3), 1),
+           "variant": variant, "spp_batch": args.spp_batch}
+    for name, b in (("sah", abi.FF_BVH_SAH), ("lbvh", abi.FF_BVH_LBVH)):
+        t0 = time.perf_counter()
+        sc = abi.Scene(ctx, sd.positions, sd.normals, sd.uvs, sd.material_ids, sd.materials, sd.textures, builder=b)
+        create_s = time.perf_counter() - t0
+        best = float("inf")
+        for _ in range(2):
+            film.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            abi.render(ctx, sc, env, cam, sd.width, sd.height, args.spp, args.depth, sd.seed, film,
+                       spp_batch=args.spp_batch, flags=abi.FF_RENDER_TIMING)
+            st = abi.last_render_stats(ctx)
+            best = min(best, time.perf_counter() - t0)
+        rays = st.rays_primary + st.rays_extension
+        out[f"{name}_scene_create_s"] = round(create_s, 3)
+        out[f"{name}_frame_ms"] = round(best * 1e3, 2)
+        out[f"{name}_incoherent_mrays_s"] = round((st.rays_extension + st.rays_shadow) / best / 1e6, 1)
+        out[f"{name}_ext_nodes_per_ray"] = round(st.ext_nodes / max(1.0, rays), 2)
+        log(f"bvh {name}: scene create {create_s:.2f} s, frame {best * 1e3:.1f} ms")
+        sc.close()
+    env.close()
+    ctx.close()
+    return out
 
 
 def run_frame(args) -> dict:
@@ -527,6 +584,7 @@ def run_frame(args) -> dict:
         tuned = autotune(ctx, scene, env, cam, sd.width, sd.height, args.depth, sd.seed, probe_spp=4,
                          batches=(1, 2, 4), refine_batches=(8, 16, 32, 64), refine_spp=args.spp)
         args.spp_batch = tuned["best"]["spp_batch"]
+        args.tuned_variant = tuned["best"]["variant"]
         for c in tuned["candidates"]:
             log(f"  variant {c['variant']} (block {c['block']}, lds {c['lds_entries']}, refill {c['refill']}, "
                 f"ww {c['while_while']}, width {c['bvh_width']}) "

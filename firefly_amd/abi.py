@@ -77,6 +77,8 @@ _PROTOS = {
     "ff_context_destroy": (C.c_int, [_P]),
     "ff_bvh_build_host": (C.c_int, [_P, C.c_int32, C.c_int32, _P, _P, C.POINTER(C.c_int32),
                                     C.POINTER(C.c_int32)]),
+    "ff_bvh_build_gpu": (C.c_int, [_P, _P, C.c_int32, C.c_int32, _P, _P, C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_int32), C.POINTER(C.c_float)]),
     "ff_scene_create": (C.c_int, [_P, C.POINTER(SceneDesc), C.POINTER(_P)]),
     "ff_scene_destroy": (C.c_int, [_P]),
     "ff_scene_info": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
@@ -138,6 +140,19 @@ def _ptr(a) -> int:
     if isinstance(a, np.ndarray):
         return a.ctypes.data
     return a.data_ptr()
+
+
+def bvh_build_gpu(ctx: "Context", positions: np.ndarray, max_prims: int = 8):
+    """Linear BVH built on the GPU (the FF_BVH_LBVH builder): (nodes, ordered,
+    max_depth, device ms of the build kernels)."""
+    pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 9)
+    n = pos.shape[0]
+    nodes = np.zeros(max(2 * n - 1, 1), dtype=NODE_DTYPE)
+    ordered = np.zeros(n, dtype=np.int32)
+    nn, md, ms = C.c_int32(), C.c_int32(), C.c_float()
+    check(_lib().ff_bvh_build_gpu(ctx.h, pos.ctypes.data, n, max_prims, nodes.ctypes.data, ordered.ctypes.data,
+                                  C.byref(nn), C.byref(md), C.byref(ms)))
+    return nodes[: nn.value].copy(), ordered, md.value, ms.value
 
 
 def bvh_build_host(positions: np.ndarray, max_prims: int = 8):

@@ -394,6 +394,29 @@ ff_status ff_bvh_build_host(const float* positions, int32_t n_tris, int32_t max_
     return FF_OK;
 }
 
+ff_status ff_bvh_build_gpu(ff_context* ctx, const float* positions, int32_t n_tris, int32_t max_prims,
+                           ff_bvh_node* nodes_out, int32_t* ordered_out, int32_t* n_nodes_out, int32_t* max_depth_out,
+                           float* build_ms_out) {
+    if (!ctx || !positions || n_tris <= 0 || max_prims <= 0)
+        return fail(FF_ERR_INVALID, "ff_bvh_build_gpu: need a context, positions, n_tris > 0, max_prims > 0");
+    if (n_tris >= (1 << 27))
+        return fail(FF_ERR_CAPACITY, "ff_bvh_build_gpu: %d triangles (limit 2^27 - 1)", n_tris);
+    for (size_t k = 0; k < (size_t)n_tris * 9; k++)
+        if (!std::isfinite(positions[k]))
+            return fail(FF_ERR_INVALID, "ff_bvh_build_gpu: triangle %d has a non-finite vertex coordinate", (int)(k / 9));
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::vector<ffh::BvhNode> nodes;
+    std::vector<int32_t> ordered;
+    float ms = 0.f;
+    HIP_TRY(ffk::build_lbvh(positions, n_tris, max_prims, nodes, ordered, &ms));
+    if (nodes_out) std::memcpy(nodes_out, nodes.data(), nodes.size() * sizeof(ffh::BvhNode));
+    if (ordered_out) std::memcpy(ordered_out, ordered.data(), ordered.size() * sizeof(int32_t));
+    if (n_nodes_out) *n_nodes_out = (int32_t)nodes.size();
+    if (max_depth_out) *max_depth_out = ffh::bvh_max_depth(nodes);
+    if (build_ms_out) *build_ms_out = ms;
+    return FF_OK;
+}
+
 ff_status ff_scene_create(ff_context* ctx, const ff_scene_desc* d, ff_scene** out) {
     if (!ctx || !d || !out) return fail(FF_ERR_INVALID, "ff_scene_create: NULL argument");
     *out = nullptr;

@@ -70,7 +70,8 @@ int32_t bvh_max_depth(const std::vector<BvhNode>& nodes);
 
 namespace ffk {
 // Linear BVH on the GPU (Morton order, Karras hierarchy), in the SAH builder's
-// node format; positions on the host, nodes and permutation returned there.
+// node format; positions on the host, nodes and permutation returned there
+// (gpu_ms, if given: device time of the build kernels).
 hipError_t build_lbvh(const float* positions, int32_t n, int32_t max_prims, std::vector<ffh::BvhNode>& nodes,
-                      std::vector<int32_t>& ordered);
+                      std::vector<int32_t>& ordered, float* gpu_ms = nullptr);
 }  // namespace ffk

@@ -309,9 +309,10 @@ hipError_t alloc(X** p, size_t n) {
 
 }  // namespace
 
-// positions on the host [n][3][3]; nodes and ordered come back on the host
+// positions on the host [n][3][3]; nodes and ordered come back on the host;
+// gpu_ms (optional): device time from the bounds pass to the emitted tree
 hipError_t build_lbvh(const float* positions, int32_t n, int32_t max_prims, std::vector<ffh::BvhNode>& nodes,
-                      std::vector<int32_t>& ordered) {
+                      std::vector<int32_t>& ordered, float* gpu_ms) {
     if (n <= 0) return hipErrorInvalidValue;
     if (max_prims < 1) max_prims = 1;
     if (max_prims > 65535) max_prims = 65535;  // a leaf's count is 16 bits
@@ -325,7 +326,10 @@ hipError_t build_lbvh(const float* positions, int32_t n, int32_t max_prims, std:
     ffh::BvhNode* d_nodes = nullptr;
     void* d_tmp = nullptr;
     size_t m = (size_t)n, tmp_bytes = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipError_t e = hipStreamCreate(&s);
+    if (e == hipSuccess && gpu_ms) e = hipEventCreate(&ev0);
+    if (e == hipSuccess && gpu_ms) e = hipEventCreate(&ev1);
     auto ok = [&](hipError_t x) { if (e == hipSuccess) e = x; return e == hipSuccess; };
     if (ok(alloc(&d_pos, 9 * m)) && ok(alloc(&d_keys, m)) && ok(alloc(&d_sorted, m)) && ok(alloc(&d_cb, 6)) &&
         ok(alloc(&d_child, m)) && ok(alloc(&d_range, m)) && ok(alloc(&d_par_int, m)) && ok(alloc(&d_par_leaf, m)) &&
@@ -338,6 +342,7 @@ hipError_t build_lbvh(const float* positions, int32_t n, int32_t max_prims, std:
         ok(hipMalloc(&d_tmp, tmp_bytes ? tmp_bytes : 1))) {
         uint32_t g = (uint32_t)((m + LB_BLOCK - 1) / LB_BLOCK);
         uint32_t gb = g < 4096 ? g : 4096;
+        if (ev0) ok(hipEventRecord(ev0, s));
         hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, s, d_cb);
         hipLaunchKernelGGL(k_centroid_bounds, dim3(gb), dim3(LB_BLOCK), 0, s, d_pos, (uint32_t)n, d_cb);
         hipLaunchKernelGGL(k_morton, dim3(gb), dim3(LB_BLOCK), 0, s, d_pos, (uint32_t)n, d_cb, d_keys);
@@ -350,10 +355,12 @@ hipError_t build_lbvh(const float* positions, int32_t n, int32_t max_prims, std:
             hipLaunchKernelGGL(k_refit, dim3(g), dim3(LB_BLOCK), 0, s, T);
             hipLaunchKernelGGL(k_emit, dim3((uint32_t)((2 * m - 1 + LB_BLOCK - 1) / LB_BLOCK)), dim3(LB_BLOCK), 0, s,
                                T, d_nodes, d_ord);
+            if (ev1) ok(hipEventRecord(ev1, s));
             int32_t n_out = 1;
             if (ok(hipGetLastError()) && n > 1 && (int32_t)(m) > max_prims)
                 ok(hipMemcpyAsync(&n_out, d_isize, sizeof(int32_t), hipMemcpyDeviceToHost, s));
             if (ok(hipStreamSynchronize(s))) {
+                if (ev0) ok(hipEventElapsedTime(gpu_ms, ev0, ev1));
                 nodes.resize((size_t)n_out);
                 ordered.resize(m);
                 if (ok(hipMemcpy(nodes.data(), d_nodes, sizeof(ffh::BvhNode) * (size_t)n_out, hipMemcpyDeviceToHost)))
@@ -365,6 +372,8 @@ hipError_t build_lbvh(const float* positions, int32_t n, int32_t max_prims, std:
                     (void*)d_par_int, (void*)d_par_leaf, (void*)d_axis, (void*)d_ibox, (void*)d_lbox, (void*)d_isize,
                     (void*)d_flag, (void*)d_nodes, (void*)d_ord, d_tmp})
         if (p) (void)hipFree(p);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
     if (s) (void)hipStreamDestroy(s);
     return e;
 }

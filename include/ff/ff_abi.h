@@ -128,6 +128,15 @@ int32_t ff_abi_version(void);
 ff_status ff_context_create(int32_t device, ff_context **out);
 ff_status ff_context_destroy(ff_context *ctx);
 
+/* ---- linear BVH built on the GPU (the FF_BVH_LBVH scene builder) ----
+ * No reference counterpart: the reference declares SplitMethod::HLBVH
+ * (CRT/BVH.h:23) and never builds it.  Same node format and outputs as
+ * ff_bvh_build_host; build_ms_out (may be NULL) receives the device time of
+ * the build kernels (key sort included, host transfers excluded). */
+ff_status ff_bvh_build_gpu(ff_context *ctx, const float *positions, int32_t n_tris, int32_t max_prims,
+                           ff_bvh_node *nodes_out, int32_t *ordered_out, int32_t *n_nodes_out,
+                           int32_t *max_depth_out, float *build_ms_out);
+
 /* ---- host SAH BVH build (no GPU needed) ----
  * Replaces BVHAccel(info, SplitMethod::SAH, maxPrims) + init_BVH_triangle_info
  * (CRT/BVH.cpp:52-64, CRT/dScene.cpp:324-379).  nodes_out holds 2n-1 nodes. */

@@ -38,6 +38,8 @@ def test_null_arguments_are_rejected():
     assert L.ff_trace_any(None, None, 0, None, None, None) == 1
     assert L.ff_film_resolve(None, 0, 0, 1.0, None, None, None) == 1
     assert b"NULL" in L.ff_last_error() or b"bad" in L.ff_last_error()
+    assert L.ff_bvh_build_gpu(None, None, 0, 8, None, None, None, None, None) == 1
+    assert b"ff_bvh_build_gpu" in L.ff_last_error()
     with pytest.raises(abi.FFError) as e:
         abi.check(L.ff_scene_info(None, None, None, None))
     assert e.value.status == 1

@@ -98,6 +98,20 @@ def test_lbvh_render_matches_sah_render(gpu):
     assert same > 0.99 and err < 1e-3
 
 
+def test_standalone_gpu_build_equals_oracle(gpu, oracle):
+    """ff_bvh_build_gpu returns the tree ff_scene_create builds for
+    FF_BVH_LBVH, with the device time of its kernels."""
+    pos = _soup(50000, 11)
+    nodes, ordered, depth, ms = abi.bvh_build_gpu(gpu, pos, 4)
+    want_nodes, want_ordered = oracle.lbvh_build(pos, 4)
+    assert nodes.tobytes() == want_nodes.tobytes() and np.array_equal(ordered, want_ordered)
+    assert depth == oracle.bvh_max_depth(want_nodes) and ms > 0.0
+    bad = pos.copy()
+    bad[5, 1, 2] = np.nan
+    with pytest.raises(abi.FFError):
+        abi.bvh_build_gpu(gpu, bad, 4)
+
+
 def test_unknown_builder_is_rejected(gpu):
     with pytest.raises(abi.FFError) as e:
         _scene(gpu, _soup(10, 1), builder=7)
