@@ -210,7 +210,7 @@ int Scene::get_nmb_of_triangles() const {
 }
 
 // ---------------------------------------------------------------- dScene --
-dScene::dScene(Scene* s, int device, int max_prims_in_node) : h_scene(s) {
+dScene::dScene(Scene* s, int device, int max_prims_in_node, int builder) : h_scene(s) {
     if (!s || !s->camera || !s->environment_light) throw Error(FF_ERR_INVALID, "dScene: scene needs a camera and an environment light");
     // material table (dScene::load_materials, CRT/dScene.cpp:172-214)
     std::map<std::string, int> index;
@@ -271,6 +271,7 @@ dScene::dScene(Scene* s, int device, int max_prims_in_node) : h_scene(s) {
     d.tex_h = th.data();
     d.tex_c = tc.data();
     d.max_prims_in_node = max_prims_in_node;
+    d.builder = builder;
     check(ff_context_create(device, &ctx), "dScene: context");
     try {
         check(ff_scene_create(ctx, &d, &scene), "dScene: scene");

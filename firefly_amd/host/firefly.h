@@ -172,7 +172,8 @@ public:
 // camera.  Owns a device context.
 class dScene {
 public:
-    explicit dScene(Scene* scene, int device = 0, int max_prims_in_node = 8);
+    // builder: FF_BVH_SAH (the reference's host SAH tree) or FF_BVH_LBVH (built on the GPU)
+    explicit dScene(Scene* scene, int device = 0, int max_prims_in_node = 8, int builder = FF_BVH_SAH);
     ~dScene();
     dScene(const dScene&) = delete;
     dScene& operator=(const dScene&) = delete;
