@@ -164,6 +164,12 @@ public:
     void set_environment_light(EnvironmentLight* e) { environment_light = e; }
     void add_model(Model m) { models.push_back(std::move(m)); }
     void add_material(const Material& m) { materials_loaded[m.name] = m; }
+    // Adds the meshes and materials of a binary glTF 2.0 file (.glb), as the
+    // reference's assimp import does (Scene::load, CRT/Scene.cpp:20-221):
+    // vertices placed by their node transforms, v flipped, materials by name
+    // with the loader's forced roughness 1 / metallic 0.  Returns the number
+    // of triangles added; throws Error on a malformed or unsupported file.
+    int load(const std::string& path);
     int get_nmb_of_triangles() const;
 };
 

@@ -2,7 +2,15 @@
 // (Scene -> dScene -> PathTracer::draw, as the reference's main loop drives
 // it, CRT/main.cpp:245-275) and writes the averaged image as a PFM.
 //
-// usage: render_demo <scene.ffs> <out.pfm> <spp> <max_path_length> [samples_per_draw]
+// usage: render_demo <scene.ffs> <out.pfm> <spp> <max_path_length> [samples_per_draw] [model.glb]
+//        render_demo --dump-glb <model.glb> <out.bin>   (no GPU: the loaded meshes and materials)
+//
+// A model.glb argument adds that file's meshes (Scene::load) to the scene's.
+// The dump: int32 n_meshes; per mesh int32 name length, name, int32 material
+// name length, name, int32 n_vertices, n_indices, has_texcoords, float
+// positions[n][3], normals[n][3], (texcoords[n][2]), uint32 indices[]; then
+// int32 n_materials; per material int32 name length, name, float base[3],
+// roughness, metallic.
 //
 // Scene file (little endian): "FFS1", int32 n_meshes, n_materials, env_w,
 // env_h, width, height, uint32 seed, float eye[3], target[3], yfov_deg,
@@ -25,9 +33,55 @@ void rd(std::ifstream& f, T* p, size_t n) {
     if (!f.read((char*)p, (std::streamsize)(sizeof(T) * n))) throw Error(FF_ERR_INVALID, "scene file truncated");
 }
 
+void put_str(FILE* o, const std::string& s) {
+    int32_t n = (int32_t)s.size();
+    std::fwrite(&n, 4, 1, o);
+    std::fwrite(s.data(), 1, s.size(), o);
+}
+
+int dump_glb(const char* glb, const char* out) {
+    Scene scene;
+    int tris = scene.load(glb);
+    FILE* o = std::fopen(out, "wb");
+    if (!o) throw Error(FF_ERR_INVALID, std::string("cannot write ") + out);
+    int32_t nm = 0;
+    for (const Model& m : scene.models) nm += (int32_t)m.meshes.size();
+    std::fwrite(&nm, 4, 1, o);
+    for (const Model& m : scene.models)
+        for (const Mesh& me : m.meshes) {
+            put_str(o, me.name);
+            put_str(o, me.material);
+            int32_t h[3] = {(int32_t)me.positions.size(), (int32_t)me.indices.size(), me.texcoords.empty() ? 0 : 1};
+            std::fwrite(h, 4, 3, o);
+            std::fwrite(me.positions.data(), sizeof(vec3), me.positions.size(), o);
+            std::fwrite(me.normals.data(), sizeof(vec3), me.normals.size(), o);
+            if (h[2]) std::fwrite(me.texcoords.data(), sizeof(vec2), me.texcoords.size(), o);
+            std::fwrite(me.indices.data(), 4, me.indices.size(), o);
+        }
+    int32_t nmat = (int32_t)scene.materials_loaded.size();
+    std::fwrite(&nmat, 4, 1, o);
+    for (const auto& kv : scene.materials_loaded) {
+        put_str(o, kv.first);
+        float r[5] = {kv.second.baseColorFactor.x, kv.second.baseColorFactor.y, kv.second.baseColorFactor.z,
+                      kv.second.roughnessFactor, kv.second.metallicFactor};
+        std::fwrite(r, 4, 5, o);
+    }
+    std::fclose(o);
+    std::printf("loaded %s: %d meshes, %d triangles, %d materials\n", glb, nm, tris, nmat);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
+    if (argc == 4 && std::strcmp(argv[1], "--dump-glb") == 0) {
+        try {
+            return dump_glb(argv[2], argv[3]);
+        } catch (const Error& e) {
+            std::fprintf(stderr, "render_demo: %s (status %d)\n", e.what(), (int)e.status);
+            return 1;
+        }
+    }
     if (argc < 5) {
         std::fprintf(stderr, "usage: %s scene.ffs out.pfm spp max_path_length [samples_per_draw]\n", argv[0]);
         return 2;
@@ -84,6 +138,7 @@ int main(int argc, char** argv) {
             model.meshes.push_back(std::move(me));
         }
         scene.add_model(std::move(model));
+        if (argc > 6) scene.load(argv[6]);
         EnvironmentLight light(env_w, env_h, env);
         scene.set_environment_light(&light);
         PerspectiveCamera camera(vec3(cam[0], cam[1], cam[2]), (float)width / (float)height, cam[6], 0.1f, 100.f);

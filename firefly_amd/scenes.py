@@ -421,8 +421,8 @@ def write_scene_file(sd: SceneData, path: str) -> None:
     material run, in triangle order)."""
     mid = sd.material_ids
     cuts = np.flatnonzero(np.diff(mid)) + 1
-    starts = np.concatenate([[0], cuts])
-    ends = np.concatenate([cuts, [len(mid)]])
+    starts = np.concatenate([[0], cuts]) if len(mid) else np.zeros(0, np.int64)
+    ends = np.concatenate([cuts, [len(mid)]]) if len(mid) else np.zeros(0, np.int64)
     with open(path, "wb") as f:
         f.write(b"FFS1")
         np.array([len(starts), len(sd.materials), sd.env.shape[1], sd.env.shape[0], sd.width, sd.height],

@@ -1,0 +1,306 @@
+"""Scene::load, the binary glTF loader of the C++ host interface (in place of
+the reference's assimp import, CRT/Scene.cpp:20-221), through
+`render_demo --dump-glb` (no GPU): node transforms (TRS and matrix, nested),
+inverse-transpose normals, generated smooth normals, flipped v, indexed and
+unindexed primitives, interleaved buffer views, materials by name with the
+reference loader's forced roughness 1 / metallic 0, and rejection of
+malformed or unsupported files."""
+import json
+import math
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+DEMO = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "firefly_amd", "render_demo")
+
+
+class Glb:
+    """Minimal GLB writer: accessors over one binary buffer."""
+
+    def __init__(self):
+        self.bin = bytearray()
+        self.views, self.accs = [], []
+
+    def _view(self, data: bytes, stride=None):
+        while len(self.bin) % 4:
+            self.bin.append(0)
+        v = {"buffer": 0, "byteOffset": len(self.bin), "byteLength": len(data)}
+        if stride:
+            v["byteStride"] = stride
+        self.bin += data
+        self.views.append(v)
+        return len(self.views) - 1
+
+    def acc(self, arr, ctype, typ, view=None, offset=0, count=None):
+        arr = np.ascontiguousarray(arr)
+        if view is None:
+            view = self._view(arr.tobytes())
+        a = {"bufferView": view, "componentType": ctype, "type": typ,
+             "count": int(count if count is not None else len(arr))}
+        if offset:
+            a["byteOffset"] = offset
+        self.accs.append(a)
+        return len(self.accs) - 1
+
+    def interleaved(self, pos, nrm):
+        data = np.concatenate([pos, nrm], 1).astype(np.float32)
+        v = self._view(data.tobytes(), stride=24)
+        return (self.acc(None, 5126, "VEC3", view=v, count=len(pos)),
+                self.acc(None, 5126, "VEC3", view=v, offset=12, count=len(pos)))
+
+    def write(self, path, doc):
+        doc = dict(doc, asset={"version": "2.0"}, accessors=self.accs, bufferViews=self.views,
+                   buffers=[{"byteLength": len(self.bin)}])
+        js = json.dumps(doc).encode()
+        js += b" " * (-len(js) % 4)
+        b = bytes(self.bin) + b"\0" * (-len(self.bin) % 4)
+        total = 12 + 8 + len(js) + 8 + len(b)
+        with open(path, "wb") as f:
+            f.write(struct.pack("<III", 0x46546C67, 2, total))
+            f.write(struct.pack("<II", len(js), 0x4E4F534A) + js)
+            f.write(struct.pack("<II", len(b), 0x004E4942) + b)
+
+
+def read_dump(path):
+    with open(path, "rb") as f:
+        data = f.read()
+    off = [0]
+
+    def take(fmt):
+        r = struct.unpack_from(fmt, data, off[0])
+        off[0] += struct.calcsize(fmt)
+        return r
+
+    def arr(dt, n):
+        a = np.frombuffer(data, dt, n, off[0])
+        off[0] += a.nbytes
+        return a
+
+    def s():
+        (n,) = take("<i")
+        r = data[off[0]: off[0] + n].decode()
+        off[0] += n
+        return r
+    meshes = []
+    (nm,) = take("<i")
+    for _ in range(nm):
+        name, mat = s(), s()
+        nv, ni, has_uv = take("<iii")
+        m = {"name": name, "material": mat, "pos": arr("<f4", 3 * nv).reshape(-1, 3),
+             "nrm": arr("<f4", 3 * nv).reshape(-1, 3)}
+        m["uv"] = arr("<f4", 2 * nv).reshape(-1, 2) if has_uv else None
+        m["idx"] = arr("<u4", ni)
+        meshes.append(m)
+    mats = {}
+    (nmat,) = take("<i")
+    for _ in range(nmat):
+        name = s()
+        mats[name] = take("<5f")
+    return meshes, mats
+
+
+# ---- the loader's arithmetic restated (same operations, same order) ----
+def mat_mul(a, b):
+    r = [0.0] * 16
+    for c in range(4):
+        for row in range(4):
+            s = 0.0
+            for k in range(4):
+                s += a[k * 4 + row] * b[c * 4 + k]
+            r[c * 4 + row] = s
+    return r
+
+
+def local_of(node):
+    if "matrix" in node:
+        return [float(v) for v in node["matrix"]]
+    t = node.get("translation", [0, 0, 0])
+    x, y, z, w = node.get("rotation", [0, 0, 0, 1])
+    s = node.get("scale", [1, 1, 1])
+    R = [1 - 2 * (y * y + z * z), 2 * (x * y + z * w), 2 * (x * z - y * w),
+         2 * (x * y - z * w), 1 - 2 * (x * x + z * z), 2 * (y * z + x * w),
+         2 * (x * z + y * w), 2 * (y * z - x * w), 1 - 2 * (x * x + y * y)]
+    m = [1.0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 1.0]
+    for c in range(3):
+        for row in range(3):
+            m[c * 4 + row] = R[c * 3 + row] * s[c]
+    for k in range(3):
+        m[12 + k] = float(t[k])
+    return m
+
+
+def place(m, p):
+    out = np.zeros((len(p), 3), np.float32)
+    for v, (x, y, z) in enumerate(p.astype(np.float64)):
+        out[v] = (((m[0] * x + m[4] * y) + m[8] * z) + m[12], ((m[1] * x + m[5] * y) + m[9] * z) + m[13],
+                  ((m[2] * x + m[6] * y) + m[10] * z) + m[14])
+    return out
+
+
+def normal_rows(m):
+    a, b, c, dd, e, f, g, h, i = m[0], m[4], m[8], m[1], m[5], m[9], m[2], m[6], m[10]
+    det = a * (e * i - f * h) - b * (dd * i - f * g) + c * (dd * h - e * g)
+    return [(e * i - f * h) / det, -(dd * i - f * g) / det, (dd * h - e * g) / det,
+            -(b * i - c * h) / det, (a * i - c * g) / det, -(a * h - b * g) / det,
+            (b * f - c * e) / det, -(a * f - c * dd) / det, (a * e - b * dd) / det]
+
+
+def unit(X, Y, Z):
+    ln = math.sqrt((X * X + Y * Y) + Z * Z)
+    return (X / ln, Y / ln, Z / ln) if ln > 0 else (0.0, 1.0, 0.0)
+
+
+def transform_normals(m, n):
+    N = normal_rows(m)
+    out = np.zeros((len(n), 3), np.float32)
+    for v, (x, y, z) in enumerate(n.astype(np.float64)):
+        out[v] = unit((N[0] * x + N[1] * y) + N[2] * z, (N[3] * x + N[4] * y) + N[5] * z,
+                      (N[6] * x + N[7] * y) + N[8] * z)
+    return out
+
+
+def smooth_normals(pos, idx):
+    acc = np.zeros((len(pos), 3))
+    for k in range(0, len(idx) - 2, 3):
+        p0, p1, p2 = (pos[idx[k + q]].astype(np.float64) for q in range(3))
+        ux, uy, uz = p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]
+        vx, vy, vz = p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]
+        f = (uy * vz - uz * vy, uz * vx - ux * vz, ux * vy - uy * vx)
+        for q in range(3):
+            for c in range(3):
+                acc[idx[k + q], c] += f[c]
+    return np.array([unit(*a) for a in acc], np.float32)
+
+
+def _grid(n, seed):
+    rng = np.random.default_rng(seed)
+    xs, zs = np.meshgrid(np.linspace(-1, 1, n), np.linspace(-1, 1, n))
+    pos = np.stack([xs, rng.uniform(-0.1, 0.1, xs.shape), zs], -1).reshape(-1, 3).astype(np.float32)
+    nrm = rng.normal(size=pos.shape).astype(np.float32)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    uv = rng.uniform(0, 1, (len(pos), 2)).astype(np.float32)
+    idx = []
+    for r in range(n - 1):
+        for c in range(n - 1):
+            a = r * n + c
+            idx += [a, a + n, a + 1, a + 1, a + n, a + n + 1]
+    return pos, nrm, uv, np.array(idx)
+
+
+def _run(*args):
+    return subprocess.run([DEMO, *args], capture_output=True, text=True, timeout=60)
+
+
+@pytest.fixture(scope="module")
+def model(tmp_path_factory):
+    assert os.path.exists(DEMO), "render_demo not built"
+    g = Glb()
+    pa, na, ta, ia = _grid(5, 1)
+    pos_a, nrm_a = g.interleaved(pa, na)
+    uv_a = g.acc(ta, 5126, "VEC2")
+    idx_a = g.acc(ia.astype(np.uint16), 5123, "SCALAR")
+    pb, _, _, ib = _grid(4, 2)
+    pos_b = g.acc(pb, 5126, "VEC3")
+    idx_b = g.acc(ib.astype(np.uint32), 5125, "SCALAR")
+    pc = np.random.default_rng(3).uniform(-1, 1, (6, 3)).astype(np.float32)
+    pos_c = g.acc(pc, 5126, "VEC3")
+    q = math.sin(math.pi / 4)
+    mirror = [-2.0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 0.5, 0, 0.25, -1.0, 3.0, 1.0]
+    doc = {
+        "scene": 0,
+        "scenes": [{"nodes": [0, 2, 3]}],
+        "nodes": [{"translation": [1, 2, 3], "scale": [2, 2, 2], "children": [1]},
+                  {"rotation": [0, q, 0, q], "translation": [0.5, 0, 0], "mesh": 0},
+                  {"matrix": mirror, "mesh": 1},
+                  {"mesh": 2}],
+        "meshes": [{"name": "grid", "primitives": [{"attributes": {"POSITION": pos_a, "NORMAL": nrm_a,
+                                                                  "TEXCOORD_0": uv_a},
+                                                    "indices": idx_a, "material": 0}]},
+                   {"name": "bare", "primitives": [{"attributes": {"POSITION": pos_b}, "indices": idx_b}]},
+                   {"primitives": [{"attributes": {"POSITION": pos_c}, "material": 1}]}],
+        "materials": [{"name": "réd", "pbrMetallicRoughness": {"baseColorFactor": [0.8, 0.1, 0.1, 1.0],
+                                                                     "roughnessFactor": 0.3,
+                                                                     "metallicFactor": 0.7}},
+                      {"pbrMetallicRoughness": {"baseColorFactor": [0.2, 0.4, 0.6, 1.0]}}],
+    }
+    d = tmp_path_factory.mktemp("glb")
+    path = str(d / "m.glb")
+    g.write(path, doc)
+    out = str(d / "m.bin")
+    r = _run("--dump-glb", path, out)
+    assert r.returncode == 0, r.stderr
+    return doc, read_dump(out), (pa, na, ta, ia, pb, ib, pc)
+
+
+def test_meshes_placed_by_their_node_transforms(model):
+    doc, (meshes, _), (pa, na, ta, ia, pb, ib, pc) = model
+    assert [m["name"] for m in meshes] == ["grid/0", "bare/0", "mesh2/0"]
+    n0, n1 = doc["nodes"][0], doc["nodes"][1]
+    w_a = mat_mul(mat_mul([1.0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 1.0], local_of(n0)), local_of(n1))
+    a = meshes[0]
+    assert np.array_equal(a["pos"], place(w_a, pa))
+    assert np.array_equal(a["nrm"], transform_normals(w_a, na))
+    assert np.array_equal(a["idx"], ia)
+    assert np.array_equal(a["uv"], np.stack([ta[:, 0], np.float32(1) - ta[:, 1]], 1))
+    # the child's rotation about y by 90 degrees, then the parent's scale 2 and translation
+    assert np.allclose(a["pos"][0], 2 * (np.array([pa[0, 2], pa[0, 1], -pa[0, 0]]) + [0.5, 0, 0]) + [1, 2, 3],
+                       atol=1e-5)
+    w_b = local_of(doc["nodes"][2])
+    b = meshes[1]
+    assert np.array_equal(b["pos"], place(w_b, pb))
+    assert np.array_equal(b["nrm"], smooth_normals(b["pos"], ib)) and b["uv"] is None
+    c = meshes[2]
+    assert np.array_equal(c["pos"], pc) and c["idx"].tolist() == list(range(6))
+    assert np.array_equal(c["nrm"], smooth_normals(pc, np.arange(6)))
+
+
+def test_mirrored_transform_keeps_normals_unit_and_outward(model):
+    _, (meshes, _), _ = model
+    b = meshes[1]
+    assert np.allclose(np.linalg.norm(b["nrm"], axis=1), 1, atol=1e-6)
+    p = b["pos"][b["idx"]].reshape(-1, 3, 3).astype(np.float64)
+    face = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+    assert np.all(np.sum(face * b["nrm"][b["idx"][0::3]], 1) > 0)
+
+
+def test_materials_by_name_with_forced_roughness_and_metallic(model):
+    _, (meshes, mats), _ = model
+    assert [m["material"] for m in meshes] == ["réd", "default", "1"]
+    assert mats["réd"] == pytest.approx((0.8, 0.1, 0.1, 1.0, 0.0))
+    assert mats["1"] == pytest.approx((0.2, 0.4, 0.6, 1.0, 0.0))
+    assert mats["default"] == pytest.approx((1.0, 1.0, 1.0, 1.0, 0.0))
+
+
+def _bad(tmp_path, name, mutate):
+    g = Glb()
+    p = g.acc(np.zeros((3, 3), np.float32), 5126, "VEC3")
+    i = g.acc(np.array([0, 1, 2], np.uint16), 5123, "SCALAR")
+    doc = {"scenes": [{"nodes": [0]}], "nodes": [{"mesh": 0}],
+           "meshes": [{"primitives": [{"attributes": {"POSITION": p}, "indices": i}]}]}
+    mutate(g, doc)
+    path = str(tmp_path / (name + ".glb"))
+    g.write(path, doc)
+    return path
+
+
+@pytest.mark.parametrize("name", ["sparse", "lines", "index_range", "view_range", "truncated", "magic"])
+def test_malformed_files_are_rejected(tmp_path, name):
+    if name == "sparse":
+        path = _bad(tmp_path, name, lambda g, d: g.accs[0].update(sparse={"count": 1}))
+    elif name == "lines":
+        path = _bad(tmp_path, name, lambda g, d: d["meshes"][0]["primitives"][0].update(mode=1))
+    elif name == "index_range":
+        path = _bad(tmp_path, name, lambda g, d: g.acc(np.array([0, 1, 7], np.uint16), 5123, "SCALAR")
+                    and d["meshes"][0]["primitives"][0].update(indices=2))
+    elif name == "view_range":
+        path = _bad(tmp_path, name, lambda g, d: g.accs[0].update(count=1000))
+    else:
+        path = _bad(tmp_path, name, lambda g, d: None)
+        data = open(path, "rb").read()
+        data = data[:len(data) // 2] if name == "truncated" else b"glTx" + data[4:]
+        open(path, "wb").write(data)
+    r = _run("--dump-glb", path, str(tmp_path / "o.bin"))
+    assert r.returncode == 1 and "render_demo:" in r.stderr, (r.returncode, r.stderr)
