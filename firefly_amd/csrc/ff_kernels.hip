@@ -117,16 +117,16 @@ __global__ __launch_bounds__(FF_BLOCK) void k_generate(ff_paths P) {
         P.ray[2 * id + 1] = f4(d, 0.f);
         P.rngb[id] = base;
         P.preq[id] = r;
-        P.beta[id] = make_float4(1.f, 1.f, 1.f, __uint_as_float(ST_PREV_DELTA));
         if (R.mode & RQ_ADJOINT) {
             // replay: L_rem starts as the stored radiance of this path
+            P.beta[id] = make_float4(1.f, 1.f, 1.f, __uint_as_float(ST_PREV_DELTA));
             uint32_t sl = b * R.K + layer;
             P.Lp[id] = R.path_L[(size_t)sl * WH + pix];
             float4 g = R.d_img[pix];
             P.dlt[id] = make_float4(g.x * R.inv_spp, g.y * R.inv_spp, g.z * R.inv_spp, 0.f);
-        } else {
-            P.Lp[id] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
+        // a forward camera path starts at beta = 1, L = 0, state ST_PREV_DELTA:
+        // wf_logic's first pass takes those as constants, so they are not stored
         P.q_ext[0][P.qbase[r] + local] = id;
     }
 }
@@ -476,8 +476,13 @@ __global__ __launch_bounds__(FF_BLOCK, NP == 1 ? 8 : 5) void k_logic(ff_paths P,
         uint32_t rq[NP];
 #pragma unroll
         for (int j = 0; j < NP; j++) {
-            bw[j] = P.beta[id[j]];
-            lp4[j] = P.Lp[id[j]];
+            // a camera path's record is the constant start state (not stored)
+            bw[j] = make_float4(1.f, 1.f, 1.f, __uint_as_float(ST_PREV_DELTA));
+            lp4[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (!primary) {
+                bw[j] = P.beta[id[j]];
+                lp4[j] = P.Lp[id[j]];
+            }
             hh[j] = P.hit[id[j]];
             rq[j] = P.n_req == 1 ? 0u : P.preq[id[j]];
             // after a shading pass the state word travels in fext.w:
