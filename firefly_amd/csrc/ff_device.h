@@ -187,7 +187,28 @@ __device__ __forceinline__ f3 vreflect(f3 i, f3 n) { return vsub(i, vscl(vscl(n,
 __device__ __forceinline__ f3 vlerp(f3 a, f3 b, float t) { return vadd(a, vscl(vsub(b, a), t)); }
 __device__ __forceinline__ f3 xyz(float4 v) { return V3(v.x, v.y, v.z); }
 __device__ __forceinline__ float4 f4(f3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
-__device__ __forceinline__ f3 ld3(const float* p) { return V3(p[0], p[1], p[2]); }
+// Loads through the global address space (global_load instead of flat_load:
+// the scene pointers come from memory, so the compiler cannot infer it).
+template <class T>
+__device__ __forceinline__ T ldg(const T* p) {
+    return *(const __attribute__((address_space(1))) T*)p;
+}
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef int v4i __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ldg4(const float4* p) {
+    v4f v = *(const __attribute__((address_space(1))) v4f*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ int4 ldg4i(const int4* p) {
+    v4i v = *(const __attribute__((address_space(1))) v4i*)p;
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+// a float4 store through the global address space
+__device__ __forceinline__ void stg4(float4* p, float4 v) {
+    v4f w = {v.x, v.y, v.z, v.w};
+    *(__attribute__((address_space(1))) v4f*)p = w;
+}
+__device__ __forceinline__ f3 ld3(const float* p) { return V3(ldg(p), ldg(p + 1), ldg(p + 2)); }
 
 // ------------------------------------------------------------------- RNG --
 // Counter RNG keyed by (seed, pixel, sample, bounce, dim); replaces the
@@ -236,8 +257,8 @@ __device__ __forceinline__ void bilinear(const float* __restrict__ tex, int w, i
     wrap_texels(fx, fy, w, h, i0, j0, i1, j1);
     float w00 = (1.f - a) * (1.f - b), w10 = a * (1.f - b), w01 = (1.f - a) * b, w11 = a * b;
     for (int k = 0; k < c; k++) {
-        float t00 = tex[((size_t)j0 * w + i0) * c + k], t10 = tex[((size_t)j0 * w + i1) * c + k];
-        float t01 = tex[((size_t)j1 * w + i0) * c + k], t11 = tex[((size_t)j1 * w + i1) * c + k];
+        float t00 = ldg(tex + ((size_t)j0 * w + i0) * c + k), t10 = ldg(tex + ((size_t)j0 * w + i1) * c + k);
+        float t01 = ldg(tex + ((size_t)j1 * w + i0) * c + k), t11 = ldg(tex + ((size_t)j1 * w + i1) * c + k);
         out[k] = ((w00 * t00 + w10 * t10) + w01 * t01) + w11 * t11;
     }
 }
@@ -264,10 +285,10 @@ __device__ __forceinline__ int upper_bound_ref(const double* __restrict__ list, 
     int left = 0, right = size;
     while (left < right) {
         int middle = (right - left) / 2 + left;
-        if ((double)val >= list[middle]) left = middle + 1;
+        if ((double)val >= ldg(list + middle)) left = middle + 1;
         else right = middle;
     }
-    if (left < size && list[((float)left <= val) ? 1 : 0] != 0.0) left++;
+    if (left < size && ldg(list + (((float)left <= val) ? 1 : 0)) != 0.0) left++;
     return left;
 }
 
@@ -291,7 +312,7 @@ __device__ __forceinline__ int upper_bound_guided(const double* __restrict__ lis
     if (val >= 0.f && val < 1.f) {
         uint32_t g = (uint32_t)(val * (float)ENV_GUIDE);  // exact: ENV_GUIDE is a power of two
         if (g > ENV_GUIDE - 1) g = ENV_GUIDE - 1;
-        uint32_t a = guide[g], b = guide[g + 1];
+        uint32_t a = ldg(guide + g), b = ldg(guide + g + 1);
         if (((a | b) & ENV_GUIDE_FULL) == 0) {
             left = (int)a;
             right = (int)b;
@@ -299,10 +320,10 @@ __device__ __forceinline__ int upper_bound_guided(const double* __restrict__ lis
     }
     while (left < right) {
         int middle = (right - left) / 2 + left;
-        if ((double)val >= list[middle]) left = middle + 1;
+        if ((double)val >= ldg(list + middle)) left = middle + 1;
         else right = middle;
     }
-    if (left < size && list[((float)left <= val) ? 1 : 0] != 0.0) left++;
+    if (left < size && ldg(list + (((float)left <= val) ? 1 : 0)) != 0.0) left++;
     return left;
 }
 // dEnvironmentLight::L (CRT/dEnvironmentLight.cu:66-76)
@@ -320,7 +341,7 @@ __device__ __forceinline__ float env_pdf(const EnvView& e, f3 wi) {
     px = px < 0 ? 0 : (px > e.w - 1 ? e.w - 1 : px);
     py = py < 0 ? 0 : (py > e.h - 1 ? e.h - 1 : py);
     float z = wi.z < -1.f ? -1.f : (wi.z > 1.f ? 1.f : wi.z);
-    return e.pdf[(size_t)py * e.w + px] / f_sin(f_acos(z));
+    return ldg(e.pdf + (size_t)py * e.w + px) / f_sin(f_acos(z));
 }
 // dEnvironmentLight::get_direction (CRT/dEnvironmentLight.cu:41-58);
 // indices clamped at 0 where the reference would index out of range.
@@ -542,22 +563,6 @@ __device__ __forceinline__ int fixed_exponent(uint32_t max_bits, uint32_t count)
 }
 
 // ------------------------------------------------------------ traversal --
-// Loads through the global address space (global_load instead of flat_load:
-// the scene pointers come from memory, so the compiler cannot infer it).
-template <class T>
-__device__ __forceinline__ T ldg(const T* p) {
-    return *(const __attribute__((address_space(1))) T*)p;
-}
-typedef float v4f __attribute__((ext_vector_type(4)));
-typedef int v4i __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ float4 ldg4(const float4* p) {
-    v4f v = *(const __attribute__((address_space(1))) v4f*)p;
-    return make_float4(v.x, v.y, v.z, v.w);
-}
-__device__ __forceinline__ int4 ldg4i(const int4* p) {
-    v4i v = *(const __attribute__((address_space(1))) v4i*)p;
-    return make_int4(v.x, v.y, v.z, v.w);
-}
 
 // Bounds3f::hit(ray, invDir, dirIsNeg) (CRT/Bounds3f.h:128-160) on one box:
 // the line test without [0, tmax] clipping; returns the final tmin/tmax.

@@ -506,7 +506,7 @@ __global__ __launch_bounds__(FF_BLOCK, NP == 1 ? 8 : 5) void k_logic(ff_paths P,
             found[j] = live[j] && slot[j] >= 0;
             need_d[j] = live[j] && ((found[j] && (stw[j] & ST_INSIDE)) || (!found[j] && (stw[j] & ST_PREV_DELTA)));
             mw[j] = 0u;
-            if (found[j]) mw[j] = S.mat[slot[j]];
+            if (found[j]) mw[j] = ldg(S.mat + slot[j]);
             d[j] = V3(0.f, 0.f, 0.f);
             if (need_d[j]) d[j] = xyz(P.ray[2 * id[j] + 1]);
             bool ext = live[j] && len[j] > 1;
@@ -578,15 +578,15 @@ __global__ __launch_bounds__(FF_BLOCK, NP == 1 ? 8 : 5) void k_logic(ff_paths P,
             if (tm) {
                 uint32_t local = id[j] - R.slot_base, WH = R.W * R.H;
                 if (R.K > 1 && first_batch) {
-                    R.layers[local] = make_float4(0.f + Lp.x, 0.f + Lp.y, 0.f + Lp.z, 0.f + 1.f);
+                    stg4(R.layers + local, make_float4(0.f + Lp.x, 0.f + Lp.y, 0.f + Lp.z, 0.f + 1.f));
                 } else {
                     float4* acc = R.K > 1 ? &R.layers[local] : &R.film[local % WH];
-                    float4 L = *acc;
-                    *acc = make_float4(L.x + Lp.x, L.y + Lp.y, L.z + Lp.z, L.w + 1.f);
+                    float4 L = ldg4(acc);
+                    stg4(acc, make_float4(L.x + Lp.x, L.y + Lp.y, L.z + Lp.z, L.w + 1.f));
                 }
                 if (R.mode & RQ_STORE_PATHS) {
                     uint32_t sl = P.bstate[B_BATCH] * R.K + local / WH;
-                    R.path_L[(size_t)sl * WH + local % WH] = f4(Lp, 0.f);
+                    stg4(R.path_L + (size_t)sl * WH + local % WH, f4(Lp, 0.f));
                 }
             } else {
                 uint32_t nst = (sw & (ST_INSIDE | ST_PREV_DELTA)) | ln;
@@ -668,13 +668,13 @@ __device__ __forceinline__ void shade_core(const ff_paths& P, uint32_t id, Shade
     float w0 = (1.f - hh.y) - hh.z;
     f3 nrm = vnorm(vadd(vadd(vscl(ld3(N + 3), hh.y), vscl(ld3(N + 6), hh.z)), vscl(ld3(N), w0)));
     const float* U = S.uv + (size_t)slot * 6;
-    float tu = (U[2] * hh.y + U[4] * hh.z) + U[0] * w0;
-    float tv = (U[3] * hh.y + U[5] * hh.z) + U[1] * w0;
-    const float* M = S.mats + (size_t)(S.mat[slot] & MAT_INDEX_MASK) * FF_MAT_FLOATS_DEV;
-    f3 alb = V3(M[1], M[2], M[3]);
-    float rough = M[4], metal = M[5];
-    f3 fres = V3(M[6], M[7], M[8]);
-    int at = (int)M[10], rt = (int)M[11];
+    float tu = (ldg(U + 2) * hh.y + ldg(U + 4) * hh.z) + ldg(U) * w0;
+    float tv = (ldg(U + 3) * hh.y + ldg(U + 5) * hh.z) + ldg(U + 1) * w0;
+    const float* M = S.mats + (size_t)(ldg(S.mat + slot) & MAT_INDEX_MASK) * FF_MAT_FLOATS_DEV;
+    f3 alb = V3(ldg(M + 1), ldg(M + 2), ldg(M + 3));
+    float rough = ldg(M + 4), metal = ldg(M + 5);
+    f3 fres = V3(ldg(M + 6), ldg(M + 7), ldg(M + 8));
+    int at = (int)ldg(M + 10), rt = (int)ldg(M + 11);
     bool rough_live = false;
     if (at >= 0) {
         ff_tex_desc td = S.tex[at];
@@ -711,7 +711,7 @@ __device__ __forceinline__ void shade_core(const ff_paths& P, uint32_t id, Shade
         f3 nn = entering ? gn : vneg(gn);
         float cos_i = fabsf(cs);
         if (cos_i > 1.f) cos_i = 1.f;
-        float ior = M[9];
+        float ior = ldg(M + 9);
         float eta_i = entering ? 1.f : ior, eta_t = entering ? ior : 1.f;
         float F = fresnel_dielectric(cos_i, eta_i, eta_t);
         if (rng(base, len, DIM_DIEL) < F) {

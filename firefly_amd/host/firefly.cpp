@@ -370,4 +370,74 @@ std::vector<uint8_t> PathTracer::get_rgba8(dScene* s) const {
     return out;
 }
 
+namespace {
+uint32_t crc_table[256];
+bool crc_ready = false;
+uint32_t crc32(const uint8_t* p, size_t n, uint32_t c = 0xffffffffu) {
+    if (!crc_ready) {
+        for (uint32_t i = 0; i < 256; i++) {
+            uint32_t v = i;
+            for (int k = 0; k < 8; k++) v = (v & 1u) ? 0xedb88320u ^ (v >> 1) : v >> 1;
+            crc_table[i] = v;
+        }
+        crc_ready = true;
+    }
+    for (size_t i = 0; i < n; i++) c = crc_table[(c ^ p[i]) & 0xffu] ^ (c >> 8);
+    return c;
+}
+void put_be32(std::vector<uint8_t>& o, uint32_t v) {
+    for (int s = 24; s >= 0; s -= 8) o.push_back((uint8_t)(v >> s));
+}
+void png_chunk(std::vector<uint8_t>& o, const char* type, const std::vector<uint8_t>& data) {
+    put_be32(o, (uint32_t)data.size());
+    size_t start = o.size();
+    o.insert(o.end(), type, type + 4);
+    o.insert(o.end(), data.begin(), data.end());
+    put_be32(o, crc32(o.data() + start, o.size() - start) ^ 0xffffffffu);
+}
+}  // namespace
+
+void write_png(const std::string& path, const uint8_t* rgba, int width, int height) {
+    if (!rgba || width <= 0 || height <= 0) throw Error(FF_ERR_INVALID, "write_png: empty image");
+    std::vector<uint8_t> raw;  // filter byte 0 + row
+    size_t row = (size_t)width * 4;
+    raw.reserve((row + 1) * (size_t)height);
+    for (int y = 0; y < height; y++) {
+        raw.push_back(0);
+        raw.insert(raw.end(), rgba + (size_t)y * row, rgba + (size_t)(y + 1) * row);
+    }
+    std::vector<uint8_t> z = {0x78, 0x01};  // zlib header: deflate, 32 KiB window, no dictionary
+    uint32_t a = 1, b = 0;
+    for (uint8_t v : raw) {
+        a = (a + v) % 65521u;
+        b = (b + a) % 65521u;
+    }
+    for (size_t off = 0; off < raw.size() || off == 0;) {
+        size_t n = std::min<size_t>(65535, raw.size() - off);
+        bool last = off + n == raw.size();
+        z.push_back(last ? 1 : 0);  // BFINAL, BTYPE = 00 (stored)
+        z.push_back((uint8_t)(n & 0xff));
+        z.push_back((uint8_t)(n >> 8));
+        z.push_back((uint8_t)(~n & 0xff));
+        z.push_back((uint8_t)((~n >> 8) & 0xff));
+        z.insert(z.end(), raw.begin() + (std::ptrdiff_t)off, raw.begin() + (std::ptrdiff_t)(off + n));
+        off += n;
+        if (last) break;
+    }
+    put_be32(z, (b << 16) | a);
+    std::vector<uint8_t> out = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1a, '\n'};
+    std::vector<uint8_t> ihdr;
+    put_be32(ihdr, (uint32_t)width);
+    put_be32(ihdr, (uint32_t)height);
+    ihdr.insert(ihdr.end(), {8, 6, 0, 0, 0});  // 8 bits, RGBA, deflate, filter 0, no interlace
+    png_chunk(out, "IHDR", ihdr);
+    png_chunk(out, "IDAT", z);
+    png_chunk(out, "IEND", {});
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) throw Error(FF_ERR_INVALID, "write_png: cannot write " + path);
+    size_t wr = std::fwrite(out.data(), 1, out.size(), f);
+    std::fclose(f);
+    if (wr != out.size()) throw Error(FF_ERR_INVALID, "write_png: short write to " + path);
+}
+
 }  // namespace firefly

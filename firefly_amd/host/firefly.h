@@ -151,6 +151,13 @@ private:
     std::vector<float> rgb;
 };
 
+// Writes 8-bit RGBA rows (top row first) as a PNG file: colour type 6, no
+// filtering, the zlib stream in stored (uncompressed) deflate blocks, CRC-32
+// per chunk and Adler-32 over the image data (PNG specification, RFC 1950,
+// RFC 1951).  The step after PathTracer::get_rgba8 in place of the
+// reference's GL surface (CRT/kernel.cu:447-454).
+void write_png(const std::string& path, const uint8_t* rgba, int width, int height);
+
 // Scene (CRT/Scene.h:10-47): models, materials, textures, camera, light.
 class Scene {
 public:

@@ -2,7 +2,7 @@
 // (Scene -> dScene -> PathTracer::draw, as the reference's main loop drives
 // it, CRT/main.cpp:245-275) and writes the averaged image as a PFM.
 //
-// usage: render_demo <scene.ffs> <out.pfm> <spp> <max_path_length> [samples_per_draw] [model.glb]
+// usage: render_demo <scene.ffs> <out.pfm|out.png> <spp> <max_path_length> [samples_per_draw] [model.glb]
 //        render_demo --dump-glb <model.glb> <out.bin>   (no GPU: the loaded meshes and materials)
 //
 // A model.glb argument adds that file's meshes (Scene::load) to the scene's.
@@ -152,6 +152,13 @@ int main(int argc, char** argv) {
         pt.samples_per_draw = argc > 5 ? std::atoi(argv[5]) : 1;
         int spp = std::atoi(argv[3]);
         while (pt.samples() < spp) pt.draw(&ds);
+        std::string out_path = argv[2];
+        if (out_path.size() > 4 && out_path.compare(out_path.size() - 4, 4, ".png") == 0) {
+            std::vector<uint8_t> rgba = pt.get_rgba8(&ds);  // the reference's resolve + tone map
+            write_png(out_path, rgba.data(), width, height);
+            std::printf("rendered %dx%d, %d spp into %s\n", width, height, pt.samples(), out_path.c_str());
+            return 0;
+        }
         std::vector<float> img = pt.get_image(&ds);
 
         FILE* o = std::fopen(argv[2], "wb");

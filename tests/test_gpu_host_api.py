@@ -98,3 +98,39 @@ def test_cpp_pathtracer_renders_a_glb_model(gpu, tmp_path):
     assert np.isfinite(img).all() and err < 0.01
     sc.close()
     env.close()
+
+
+def test_png_output_is_the_tone_mapped_film(gpu, tmp_path):
+    """render_demo writes PNG files through the resolve kernel (accum/n *
+    exposure, c/(c+1), (int)(255c)): the PNG parses (chunk CRCs, zlib
+    Adler-32) and its pixels are the tone map of the PFM written from the
+    same frame."""
+    import zlib
+    sd = scenes.spheres_scene(width=72, height=40, spp=4, max_depth=3)
+    ffs = str(tmp_path / "s.ffs")
+    scenes.write_scene_file(sd, ffs)
+    png, pfm = str(tmp_path / "o.png"), str(tmp_path / "o.pfm")
+    for out in (png, pfm):
+        r = subprocess.run([DEMO, ffs, out, "4", "3", "1"], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+    data = open(png, "rb").read()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    off, chunks = 8, {}
+    while off < len(data):
+        n = int.from_bytes(data[off:off + 4], "big")
+        typ, body = data[off + 4:off + 8], data[off + 8:off + 8 + n]
+        assert int.from_bytes(data[off + 8 + n:off + 12 + n], "big") == zlib.crc32(typ + body)
+        chunks[typ] = chunks.get(typ, b"") + body
+        off += 12 + n
+    w, h = int.from_bytes(chunks[b"IHDR"][:4], "big"), int.from_bytes(chunks[b"IHDR"][4:8], "big")
+    assert (w, h) == (sd.width, sd.height) and chunks[b"IHDR"][8:10] == b"\x08\x06"
+    raw = np.frombuffer(zlib.decompress(chunks[b"IDAT"]), np.uint8).reshape(h, 1 + 4 * w)
+    assert np.all(raw[:, 0] == 0)
+    rgba = raw[:, 1:].reshape(h, w, 4)
+    avg = read_pfm(pfm).astype(np.float32)
+    c = avg * np.float32(1.0)
+    c = c / (c + np.float32(1.0))
+    want = (np.float32(255) * c).astype(np.int32).astype(np.uint8)
+    assert np.all(rgba[..., 3] == 255)
+    assert np.abs(rgba[..., :3].astype(int) - want.astype(int)).max() <= 1
+    assert np.mean(rgba[..., :3] == want) > 0.99
