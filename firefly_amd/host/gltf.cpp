@@ -71,6 +71,7 @@ public:
 private:
     const char* p;
     const char* end;
+    int depth = 0;  // nesting bound: a hostile file cannot exhaust the stack
     [[noreturn]] void fail(const char* what) { throw Error(FF_ERR_INVALID, std::string("glTF JSON: ") + what); }
     void ws() {
         while (p < end && (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r')) p++;
@@ -151,9 +152,16 @@ private:
         }
         return o;
     }
+    struct Nest {
+        int& d;
+        explicit Nest(int& x) : d(x) { ++d; }
+        ~Nest() { --d; }
+    };
     JVal value() {
         ws();
         if (p >= end) fail("unexpected end");
+        Nest guard(depth);
+        if (depth > 256) fail("nesting deeper than 256");
         JVal v;
         char c = *p;
         if (c == '{') {

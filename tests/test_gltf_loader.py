@@ -51,10 +51,12 @@ class Glb:
         return (self.acc(None, 5126, "VEC3", view=v, count=len(pos)),
                 self.acc(None, 5126, "VEC3", view=v, offset=12, count=len(pos)))
 
-    def write(self, path, doc):
+    def write(self, path, doc, patch=None):
         doc = dict(doc, asset={"version": "2.0"}, accessors=self.accs, bufferViews=self.views,
                    buffers=[{"byteLength": len(self.bin)}])
         js = json.dumps(doc).encode()
+        if patch:
+            js = patch(js)
         js += b" " * (-len(js) % 4)
         b = bytes(self.bin) + b"\0" * (-len(self.bin) % 4)
         total = 12 + 8 + len(js) + 8 + len(b)
@@ -274,7 +276,7 @@ def test_materials_by_name_with_forced_roughness_and_metallic(model):
     assert mats["default"] == pytest.approx((1.0, 1.0, 1.0, 1.0, 0.0))
 
 
-def _bad(tmp_path, name, mutate):
+def _bad(tmp_path, name, mutate, patch=None):
     g = Glb()
     p = g.acc(np.zeros((3, 3), np.float32), 5126, "VEC3")
     i = g.acc(np.array([0, 1, 2], np.uint16), 5123, "SCALAR")
@@ -282,11 +284,11 @@ def _bad(tmp_path, name, mutate):
            "meshes": [{"primitives": [{"attributes": {"POSITION": p}, "indices": i}]}]}
     mutate(g, doc)
     path = str(tmp_path / (name + ".glb"))
-    g.write(path, doc)
+    g.write(path, doc, patch)
     return path
 
 
-@pytest.mark.parametrize("name", ["sparse", "lines", "index_range", "view_range", "truncated", "magic"])
+@pytest.mark.parametrize("name", ["sparse", "lines", "index_range", "view_range", "truncated", "magic", "deep"])
 def test_malformed_files_are_rejected(tmp_path, name):
     if name == "sparse":
         path = _bad(tmp_path, name, lambda g, d: g.accs[0].update(sparse={"count": 1}))
@@ -297,6 +299,9 @@ def test_malformed_files_are_rejected(tmp_path, name):
                     and d["meshes"][0]["primitives"][0].update(indices=2))
     elif name == "view_range":
         path = _bad(tmp_path, name, lambda g, d: g.accs[0].update(count=1000))
+    elif name == "deep":  # JSON nested far deeper than any glTF file
+        path = _bad(tmp_path, name, lambda g, d: d.update(extras="DEEP"),
+                    lambda js: js.replace(b'"DEEP"', b"[" * 5000 + b"0" + b"]" * 5000))
     else:
         path = _bad(tmp_path, name, lambda g, d: None)
         data = open(path, "rb").read()
