@@ -460,6 +460,9 @@ def main() -> int:
                     help="batch workload: samples of each pixel in flight (both the batched call and the "
                          "sequential renders use it, so the films stay bit-identical)")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tail-variant", type=int, default=-1,
+                    help="with --variant: traversal variant of the bounce iterations from --tail-from on")
+    ap.add_argument("--tail-from", type=int, default=4)
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run a plain device copy of this many MiB first (PMC calibration)")
     args = ap.parse_args()
@@ -578,11 +581,14 @@ def run_frame(args) -> dict:
     tuned = None
     if args.variant >= 0:
         abi.set_traversal_variant(ctx, args.variant)
+        if args.tail_variant >= 0:
+            abi.set_traversal_tail(ctx, args.tail_variant, args.tail_from)
     elif not args.no_autotune:
         from firefly_amd.autotune import autotune
         t0 = time.perf_counter()
         tuned = autotune(ctx, scene, env, cam, sd.width, sd.height, args.depth, sd.seed, probe_spp=4,
-                         batches=(1, 2, 4), refine_batches=(8, 16, 32, 64), refine_spp=args.spp)
+                         batches=(1, 2, 4), refine_batches=(8, 16, 32, 64), refine_spp=args.spp,
+                         tail_candidates=(0, 1, 22, 25, 26), tail_from=(3, 4, 5))
         args.spp_batch = tuned["best"]["spp_batch"]
         args.tuned_variant = tuned["best"]["variant"]
         for c in tuned["candidates"]:
@@ -674,7 +680,9 @@ def run_frame(args) -> dict:
         "config": {"workload": workload, "width": sd.width, "height": sd.height, "spp": args.spp,
                    "max_depth": args.depth, "tris": sd.n_tris, "env": f"{sd.env.shape[1]}x{sd.env.shape[0]}",
                    "spp_batch": args.spp_batch},
-        "launch_shape": (tuned["best"] if tuned else _shape_of(max(args.variant, 0), args.spp_batch)),
+        "launch_shape": (tuned["best"] if tuned else dict(_shape_of(max(args.variant, 0), args.spp_batch),
+                                                           tail_variant=args.tail_variant,
+                                                           tail_from=args.tail_from if args.tail_variant >= 0 else 0)),
         "spp_per_s_1080p": round(spp_s, 3),
         "total_mrays_s": round(total_rays / elapsed / 1e6, 3),
         "rays_per_step": {k: int(tot[k] / args.steps) for k in ("rays_primary", "rays_extension", "rays_shadow")},

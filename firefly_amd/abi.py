@@ -103,6 +103,7 @@ _PROTOS = {
     "ff_traversal_variant_info": (C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                             C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "ff_set_traversal_variant": (C.c_int, [_P, C.c_int32]),
+    "ff_set_traversal_tail": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "ff_film_resolve": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_float, _P, _P, _P]),
 }
 
@@ -395,6 +396,12 @@ def traversal_variants():
 
 def set_traversal_variant(ctx: Context, variant: int) -> None:
     check(_lib().ff_set_traversal_variant(ctx.h, variant))
+
+
+def set_traversal_tail(ctx: Context, variant: int, from_iteration: int = 4) -> None:
+    """Traversal variant of the bounce iterations from `from_iteration` on
+    (-1: the context's variant everywhere)."""
+    check(_lib().ff_set_traversal_tail(ctx.h, variant, from_iteration))
 
 
 def last_render_stats(ctx: Context) -> RayStats:

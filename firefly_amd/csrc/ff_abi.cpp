@@ -93,6 +93,8 @@ struct ff_context {
     size_t req_cap = 0;
     uint32_t* fetch = nullptr;    // [FETCH_WORDS] segmented traversal work counters
     int trav_variant = 0;
+    int trav_tail = -1;      // variant of the late bounce iterations (-1: none)
+    int trav_tail_from = 0;  // first iteration that uses it
     // adjoint: gradient target tables and the deterministic-reduction arena
     ff_dev_gtarget* d_gt = nullptr;
     size_t gt_cap = 0;
@@ -751,6 +753,7 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
         hs = ctx->own_stream;
     }
     HIP_TRY(ffk::prepare_traversal(ctx->trav_variant));
+    if (ctx->trav_tail >= 0) HIP_TRY(ffk::prepare_traversal(ctx->trav_tail));
     ff_paths p = ctx->paths.p;
     const bool adjoint = (reqs[0].mode & RQ_ADJOINT) != 0;
     for (const HostReq& q : reqs)
@@ -865,7 +868,8 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
     p.g_cnt = ctx->g_cnt;
     p.g_fx = ctx->g_fx;
     p.gmode = GM_FLOAT;
-    const int var = ctx->trav_variant;
+    // the context's variant, and from trav_tail_from on the tail variant
+    auto var_at = [&](int32_t it) { return ctx->trav_tail >= 0 && it >= ctx->trav_tail_from ? ctx->trav_tail : ctx->trav_variant; };
 
     ctx->records.clear();
     ctx->events_used = 0;
@@ -901,8 +905,8 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
         HIP_TRY(timed(4, [&] { return ffk::launch_generate(p, hs); }));
         int cur = 0;
         for (int32_t it = 1; it <= max_depth + 1; it++) {
-            HIP_TRY(timed(0, [&] { return ffk::launch_extend(p, cur, it == 1 ? EXT_STATE_NONE : EXT_STATE_FEXT, var, hs); }));
-            HIP_TRY(timed(1, [&] { return ffk::launch_shadow(p, var, hs); }));
+            HIP_TRY(timed(0, [&] { return ffk::launch_extend(p, cur, it == 1 ? EXT_STATE_NONE : EXT_STATE_FEXT, var_at(it), hs); }));
+            HIP_TRY(timed(1, [&] { return ffk::launch_shadow(p, var_at(it), hs); }));
             uint32_t mask = (1u << (cur == 0 ? C_EXT1 : C_EXT0)) | (1u << C_SHADOW) | (0xfu << C_MAT0);
             HIP_TRY(ffk::launch_reset(p.counters, mask, p.fetch, (1u << F_EXT) | (1u << F_SHD), hs));
             HIP_TRY(timed(2, [&] { return ffk::launch_logic(p, cur, it == 1, p.n_slots, hs); }));
@@ -921,14 +925,14 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
         HIP_TRY(timed(4, [&] { return ffk::launch_generate(p, hs); }));
         int cur = 0;
         for (int32_t it = 1; it <= max_depth + 1; it++) {
-            HIP_TRY(timed(0, [&] { return ffk::launch_extend(p, cur, it == 1 ? EXT_STATE_NONE : EXT_STATE_BETA, var, hs); }));
+            HIP_TRY(timed(0, [&] { return ffk::launch_extend(p, cur, it == 1 ? EXT_STATE_NONE : EXT_STATE_BETA, var_at(it), hs); }));
             uint32_t mask = (1u << (cur == 0 ? C_EXT1 : C_EXT0)) | (1u << C_SHADOW) | (0xfu << C_MAT0);
             HIP_TRY(ffk::launch_reset(p.counters, mask, p.fetch, (1u << F_EXT) | (1u << F_SHD), hs));
             HIP_TRY(timed(2, [&] { return ffk::launch_logic_adj(p, cur, p.n_slots, hs); }));
             if (it <= max_depth) {
                 for (int m = 0; m < 3; m++)
                     if (has_type[m]) HIP_TRY(timed(3, [&] { return ffk::launch_shade_nee(m, p, p.n_slots, hs); }));
-                HIP_TRY(timed(1, [&] { return ffk::launch_shadow(p, var, hs); }));
+                HIP_TRY(timed(1, [&] { return ffk::launch_shadow(p, var_at(it), hs); }));
                 for (int m = 0; m < 4; m++)
                     if (has_type[m])
                         HIP_TRY(timed(3, [&] { return ffk::launch_shade_adj(m, p, 1 - cur, p.n_slots, hs); }));
@@ -1087,6 +1091,15 @@ ff_status ff_set_traversal_variant(ff_context* ctx, int32_t variant) {
     if (!ctx || variant < 0 || variant >= ffk::kTravVariants)
         return fail(FF_ERR_INVALID, "ff_set_traversal_variant: variant %d of %d", variant, ffk::kTravVariants);
     ctx->trav_variant = variant;
+    return FF_OK;
+}
+
+ff_status ff_set_traversal_tail(ff_context* ctx, int32_t variant, int32_t from_iteration) {
+    if (!ctx || variant < -1 || variant >= ffk::kTravVariants || from_iteration < 1)
+        return fail(FF_ERR_INVALID, "ff_set_traversal_tail: variant %d (of %d), from iteration %d", variant,
+                    ffk::kTravVariants, from_iteration);
+    ctx->trav_tail = variant;
+    ctx->trav_tail_from = from_iteration;
     return FF_OK;
 }
 

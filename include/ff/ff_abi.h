@@ -272,6 +272,11 @@ int32_t ff_traversal_variant_count(void);
 ff_status ff_traversal_variant_info(int32_t variant, int32_t *block, int32_t *lds_entries,
                                     int32_t *refill, int32_t *while_while, int32_t *bvh_width);
 ff_status ff_set_traversal_variant(ff_context *ctx, int32_t variant);
+/* A second variant for the bounce iterations from `from_iteration` on (1 =
+ * camera rays), whose launches trace the few surviving paths: the shapes that
+ * win on the large first launches lose on small ones.  variant -1 turns it
+ * off (every launch uses the context's variant).  No reference counterpart. */
+ff_status ff_set_traversal_tail(ff_context *ctx, int32_t variant, int32_t from_iteration);
 
 /* ray counts of the last ff_render on this context (synchronises its stream). */
 ff_status ff_last_render_stats(ff_context *ctx, ff_ray_stats *out);

@@ -99,6 +99,34 @@ def test_traversal_variants_agree(gpu):
     abi.set_traversal_variant(gpu, 0)
 
 
+def test_tail_variant_gives_the_same_image(gpu):
+    """A second traversal variant for the late bounces (ff_set_traversal_tail)
+    changes launch shapes only: the film is the one-variant film."""
+    import torch
+    sd = scenes.spheres_scene(width=48, height=32, spp=4, max_depth=6)
+    sc = abi.Scene(gpu, sd.positions, sd.normals, sd.uvs, sd.material_ids, sd.materials)
+    env = abi.EnvMap(gpu, sd.env)
+    cam = abi.make_camera(sd.inv_vp)
+    films = []
+    for tail, start in ((-1, 1), (1, 3), (0, 2), (25, 5)):
+        abi.set_traversal_variant(gpu, 23)
+        abi.set_traversal_tail(gpu, tail, start)
+        f = torch.zeros((sd.height, sd.width, 4), dtype=torch.float32, device="cuda")
+        abi.render(gpu, sc, env, cam, sd.width, sd.height, 4, 6, 0, f, spp_batch=4, flags=abi.FF_RENDER_USE_GRAPH)
+        torch.cuda.synchronize()
+        films.append(f)
+    for f in films[1:]:
+        assert torch.equal(f, films[0])
+    with pytest.raises(abi.FFError):
+        abi.set_traversal_tail(gpu, 999, 3)
+    with pytest.raises(abi.FFError):
+        abi.set_traversal_tail(gpu, 1, 0)
+    abi.set_traversal_tail(gpu, -1, 1)
+    abi.set_traversal_variant(gpu, 0)
+    sc.close()
+    env.close()
+
+
 def test_concurrent_scene_creation_deep_trees(gpu):
     """Scenes built from several host threads on one context, with BVHs deeper
     than the default LDS traversal stack (so the overflow area is used), render
