@@ -24,7 +24,7 @@ def _gpu_available() -> bool:
 @pytest.fixture(scope="session")
 def gpu():
     """Device context for GPU tests; fails loudly (no fallback) if the HIP
-    library is missing on a GPU box."""
+    library is missing on a machine with a GPU."""
     if not _gpu_available():
         pytest.skip("no GPU visible")
     import torch
