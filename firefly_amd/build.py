@@ -74,7 +74,8 @@ def build_host(verbose: bool = False) -> None:
     common = ["-O2", "-std=c++17", "-fPIC", "-I", os.path.join(ROOT, "include"), "-I", host,
               "-Wl,-rpath,$ORIGIN", "-L", HERE]
     steps = [
-        ([cc] + common + ["-shared", os.path.join(host, "firefly.cpp"), os.path.join(host, "gltf.cpp"), "-lff_hip",
+        ([cc] + common + ["-shared", os.path.join(host, "firefly.cpp"), os.path.join(host, "gltf.cpp"),
+                          os.path.join(host, "png.cpp"), "-lff_hip",
                           "-o", HOST_LIB + ".tmp"],
          HOST_LIB),
         ([cc] + common + [os.path.join(host, "render_demo.cpp"), "-lfirefly", "-lff_hip", "-o", DEMO + ".tmp"],

@@ -157,6 +157,11 @@ private:
 // RFC 1951).  The step after PathTracer::get_rgba8 in place of the
 // reference's GL surface (CRT/kernel.cu:447-454).
 void write_png(const std::string& path, const uint8_t* rgba, int width, int height);
+// Decodes a PNG image (bit depth 8 or 16, colour types 0/2/3/4/6, no
+// interlace) into 8-bit samples, top row first; channels: 1-4 as stored
+// (palette images expand to RGB or RGBA).  Throws Error on a malformed or
+// unsupported file.
+void decode_png(const uint8_t* data, size_t n, int& width, int& height, int& channels, std::vector<uint8_t>& pixels);
 
 // Scene (CRT/Scene.h:10-47): models, materials, textures, camera, light.
 class Scene {
@@ -174,8 +179,9 @@ public:
     // Adds the meshes and materials of a binary glTF 2.0 file (.glb), as the
     // reference's assimp import does (Scene::load, CRT/Scene.cpp:20-221):
     // vertices placed by their node transforms, v flipped, materials by name
-    // with the loader's forced roughness 1 / metallic 0.  Returns the number
-    // of triangles added; throws Error on a malformed or unsupported file.
+    // with the loader's forced roughness 1 / metallic 0, embedded PNG base
+    // colour and roughness textures.  Returns the number of triangles added;
+    // throws Error on a malformed or unsupported file.
     int load(const std::string& path);
     int get_nmb_of_triangles() const;
 };

@@ -10,11 +10,15 @@
 //     TEXCOORD_0 with v flipped (aiProcess_FlipUVs), and indices;
 //   * materials by name (or index), base colour from baseColorFactor;
 //     roughness 1 and metallic 0 as the reference's loader forces them
-//     (CRT/Scene.cpp:203-204).
+//     (CRT/Scene.cpp:203-204);
+//   * embedded PNG images (png.cpp) of baseColorTexture (RGB) and of
+//     metallicRoughnessTexture (its green channel, the roughness, which the
+//     shading reads in place of the factor); samples are scaled by 1/255 and
+//     rows stored bottom first, matching the flipped v.
 // Node transforms compose parent * local, the glTF rule (the reference's
 // accumulation multiplies in the other order, CRT/Scene.cpp:88; the two
-// agree for one level of nesting).  Not read: embedded images (they need a
-// PNG/JPEG decoder), cameras and lights (the reference ignores both,
+// agree for one level of nesting).  Not read: JPEG or external images,
+// normal / occlusion / emissive maps, cameras and lights (the reference ignores both,
 // CRT/Scene.cpp:230-258), sparse accessors, skins, morph targets,
 // non-triangle primitives; a file using sparse accessors or other primitive
 // modes is rejected.
@@ -364,6 +368,52 @@ struct Loader {
     Scene& scene;
     Model model;
     int tris = 0;
+    std::vector<std::pair<long, int>> tex_done;  // (image * 2 + roughness?, Scene texture index)
+
+    // glTF texture `ti` as a Scene texture: RGB base colour, or roughness
+    // from the green channel; -1 when the material names none
+    int texture(const JVal* info, bool rough) {
+        if (!info) return -1;
+        const JVal* ti = info->get("index");
+        if (!ti) return -1;
+        const JVal& tj = d.list("textures").at((size_t)ti->num);
+        const JVal* src = tj.get("source");
+        if (!src) throw Error(FF_ERR_UNSUPPORTED, "glTF: texture without a source image");
+        long key = (long)src->num * 2 + (rough ? 1 : 0);
+        for (const auto& kv : tex_done)
+            if (kv.first == key) return kv.second;
+        const JVal& im = d.list("images").at((size_t)src->num);
+        const JVal* bvi = im.get("bufferView");
+        const JVal* mime = im.get("mimeType");
+        if (!bvi || (mime && mime->str != "image/png"))
+            throw Error(FF_ERR_UNSUPPORTED, "glTF: only PNG images embedded in the binary chunk are read");
+        const JVal& bv = d.list("bufferViews").at((size_t)bvi->num);
+        size_t off = (size_t)bv.num_or("byteOffset", 0), len = (size_t)bv.num_or("byteLength", 0);
+        if (off + len > d.bin.size()) throw Error(FF_ERR_INVALID, "glTF: image outside the binary chunk");
+        int w = 0, h = 0, c = 0;
+        std::vector<uint8_t> px;
+        decode_png(d.bin.data() + off, len, w, h, c, px);
+        Texture t;
+        t.filepath = "image" + std::to_string((long)src->num);
+        t.width = w;
+        t.height = h;
+        t.nrComponents = rough ? 1 : 3;
+        t.data.resize((size_t)w * h * (size_t)t.nrComponents);
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) {
+                const uint8_t* s = &px[((size_t)(h - 1 - y) * w + x) * (size_t)c];  // bottom row first
+                float* o = &t.data[((size_t)y * w + x) * (size_t)t.nrComponents];
+                if (rough) {
+                    o[0] = (c >= 3 ? s[1] : s[0]) / 255.f;  // green; gray images: the one sample
+                } else {
+                    for (int k = 0; k < 3; k++) o[k] = (c >= 3 ? s[k] : s[0]) / 255.f;
+                }
+            }
+        scene.textures_loaded.push_back(std::move(t));
+        int idx = (int)scene.textures_loaded.size() - 1;
+        tex_done.emplace_back(key, idx);
+        return idx;
+    }
 
     std::string material_name(int mi) {
         if (mi < 0) {
@@ -382,9 +432,12 @@ struct Loader {
             Material m;
             m.name = name;
             m.baseColorFactor = vec3(1.f, 1.f, 1.f);
-            if (const JVal* pbr = mj.get("pbrMetallicRoughness"))
+            if (const JVal* pbr = mj.get("pbrMetallicRoughness")) {
                 if (const JVal* bc = pbr->get("baseColorFactor"))
                     m.baseColorFactor = vec3((float)bc->at(0).num, (float)bc->at(1).num, (float)bc->at(2).num);
+                m.baseColorTexture = texture(pbr->get("baseColorTexture"), false);
+                m.roughnessTexture = texture(pbr->get("metallicRoughnessTexture"), true);
+            }
             if (const JVal* e = mj.get("emissiveFactor"))
                 m.emissiveColorFactor = vec3((float)e->at(0).num, (float)e->at(1).num, (float)e->at(2).num);
             m.roughnessFactor = 1.f;  // forced, as CRT/Scene.cpp:203-204

@@ -10,7 +10,9 @@
 // name length, name, int32 n_vertices, n_indices, has_texcoords, float
 // positions[n][3], normals[n][3], (texcoords[n][2]), uint32 indices[]; then
 // int32 n_materials; per material int32 name length, name, float base[3],
-// roughness, metallic.
+// roughness, metallic, int32 base colour and roughness texture (-1 none);
+// then int32 n_textures; per texture int32 width, height, channels and
+// float texels[height][width][channels] (bottom row first).
 //
 // Scene file (little endian): "FFS1", int32 n_meshes, n_materials, env_w,
 // env_h, width, height, uint32 seed, float eye[3], target[3], yfov_deg,
@@ -65,6 +67,15 @@ int dump_glb(const char* glb, const char* out) {
         float r[5] = {kv.second.baseColorFactor.x, kv.second.baseColorFactor.y, kv.second.baseColorFactor.z,
                       kv.second.roughnessFactor, kv.second.metallicFactor};
         std::fwrite(r, 4, 5, o);
+        int32_t t[2] = {kv.second.baseColorTexture, kv.second.roughnessTexture};
+        std::fwrite(t, 4, 2, o);
+    }
+    int32_t ntex = (int32_t)scene.textures_loaded.size();
+    std::fwrite(&ntex, 4, 1, o);
+    for (const Texture& t : scene.textures_loaded) {
+        int32_t h[3] = {t.width, t.height, t.nrComponents};
+        std::fwrite(h, 4, 3, o);
+        std::fwrite(t.data.data(), 4, t.data.size(), o);
     }
     std::fclose(o);
     std::printf("loaded %s: %d meshes, %d triangles, %d materials\n", glb, nm, tris, nmat);

@@ -101,6 +101,13 @@ def read_dump(path):
     for _ in range(nmat):
         name = s()
         mats[name] = take("<5f")
+        mats[name + "#tex"] = take("<2i")
+    texs = []
+    (ntex,) = take("<i")
+    for _ in range(ntex):
+        w, h, c = take("<3i")
+        texs.append(arr("<f4", w * h * c).reshape(h, w, c))
+    read_dump.textures = texs
     return meshes, mats
 
 
@@ -271,6 +278,7 @@ def test_mirrored_transform_keeps_normals_unit_and_outward(model):
 def test_materials_by_name_with_forced_roughness_and_metallic(model):
     _, (meshes, mats), _ = model
     assert [m["material"] for m in meshes] == ["réd", "default", "1"]
+    assert mats["réd#tex"] == (-1, -1)
     assert mats["réd"] == pytest.approx((0.8, 0.1, 0.1, 1.0, 0.0))
     assert mats["1"] == pytest.approx((0.2, 0.4, 0.6, 1.0, 0.0))
     assert mats["default"] == pytest.approx((1.0, 1.0, 1.0, 1.0, 0.0))
@@ -309,3 +317,116 @@ def test_malformed_files_are_rejected(tmp_path, name):
         open(path, "wb").write(data)
     r = _run("--dump-glb", path, str(tmp_path / "o.bin"))
     assert r.returncode == 1 and "render_demo:" in r.stderr, (r.returncode, r.stderr)
+
+
+# ---- embedded PNG textures ----
+def png_bytes(img, ctype, filters, zmode, depth=8, palette=None, trns=None, interlace=0, bad_adler=False):
+    """PNG writer with a chosen filter per row and deflate mode ('stored',
+    'fixed', 'dynamic'), so the decoder meets every block type and filter."""
+    import zlib
+    h, w = img.shape[:2]
+    spp = {0: 1, 2: 3, 3: 1, 4: 2, 6: 4}[ctype]
+    bpp = spp * depth // 8
+    rows = img.reshape(h, -1)
+    if depth == 16:
+        rows = rows.astype(">u2").view(np.uint8).reshape(h, -1)
+    raw = bytearray()
+    prev = np.zeros(rows.shape[1], np.int32)
+    for y in range(h):
+        cur = rows[y].astype(np.int32)
+        f = filters[y % len(filters)]
+        left = np.concatenate([np.zeros(bpp, np.int32), cur[:-bpp]])
+        upleft = np.concatenate([np.zeros(bpp, np.int32), prev[:-bpp]])
+        if f == 0:
+            out = cur
+        elif f == 1:
+            out = cur - left
+        elif f == 2:
+            out = cur - prev
+        elif f == 3:
+            out = cur - (left + prev) // 2
+        else:
+            p = left + prev - upleft
+            pa, pb, pc = np.abs(p - left), np.abs(p - prev), np.abs(p - upleft)
+            pred = np.where((pa <= pb) & (pa <= pc), left, np.where(pb <= pc, prev, upleft))
+            out = cur - pred
+        raw.append(f)
+        raw += (out & 0xFF).astype(np.uint8).tobytes()
+        prev = cur
+    if zmode == "stored":
+        z = zlib.compress(bytes(raw), 0)
+    else:
+        co = zlib.compressobj(9, zlib.DEFLATED, 15, 9, zlib.Z_FIXED if zmode == "fixed" else zlib.Z_DEFAULT_STRATEGY)
+        z = co.compress(bytes(raw)) + co.flush()
+    if bad_adler:
+        z = z[:-1] + bytes([z[-1] ^ 1])
+
+    def chunk(t, body):
+        return struct.pack(">I", len(body)) + t + body + struct.pack(">I", zlib.crc32(t + body))
+    out = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, depth, ctype, 0, 0, interlace))
+    if palette is not None:
+        out += chunk(b"PLTE", palette.astype(np.uint8).tobytes())
+    if trns is not None:
+        out += chunk(b"tRNS", trns.astype(np.uint8).tobytes())
+    return out + chunk(b"IDAT", z) + chunk(b"IEND", b"")
+
+
+def _textured_glb(path, base_png, rough_png):
+    g = Glb()
+    p = g.acc(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), 5126, "VEC3")
+    views = [g._view(base_png), g._view(rough_png)]
+    g.write(path, {"scenes": [{"nodes": [0]}], "nodes": [{"mesh": 0}],
+                   "meshes": [{"primitives": [{"attributes": {"POSITION": p}, "material": 0}]}],
+                   "images": [{"bufferView": views[0], "mimeType": "image/png"},
+                              {"bufferView": views[1], "mimeType": "image/png"}],
+                   "textures": [{"source": 0}, {"source": 1}],
+                   "materials": [{"name": "tex", "pbrMetallicRoughness": {
+                       "baseColorTexture": {"index": 0}, "metallicRoughnessTexture": {"index": 1}}}]})
+
+
+@pytest.mark.parametrize("ctype,zmode,depth", [(2, "dynamic", 8), (6, "fixed", 8), (0, "stored", 8),
+                                               (3, "dynamic", 8), (4, "dynamic", 8), (2, "dynamic", 16)])
+def test_png_textures_decoded_exactly(tmp_path, ctype, zmode, depth):
+    """Base colour (RGB) and roughness (green channel) textures from
+    embedded PNGs of every colour type, all five row filters and all three
+    deflate block types: texels = samples / 255, rows bottom first."""
+    rng = np.random.default_rng(ctype * 10 + depth)
+    h, w = 13, 17
+    spp = {0: 1, 2: 3, 3: 1, 4: 2, 6: 4}[ctype]
+    palette = trns = None
+    if ctype == 3:
+        palette = rng.integers(0, 256, (40, 3))
+        trns = rng.integers(0, 256, 12)
+        img = rng.integers(0, 40, (h, w, 1))
+        rgb = palette[img[..., 0]]
+    else:
+        img = rng.integers(0, 256 if depth == 8 else 65536, (h, w, spp))
+        top = img >> 8 if depth == 16 else img
+        rgb = top[..., :3] if spp >= 3 else np.repeat(top[..., :1], 3, -1)
+    dtype = np.uint16 if depth == 16 else np.uint8
+    png = png_bytes(img.astype(dtype), ctype, [0, 1, 2, 3, 4], zmode, depth, palette, trns)
+    path = str(tmp_path / "t.glb")
+    _textured_glb(path, png, png)
+    out = str(tmp_path / "t.bin")
+    r = _run("--dump-glb", path, out)
+    assert r.returncode == 0, r.stderr
+    meshes, mats = read_dump(out)
+    base, rough = read_dump.textures
+    assert mats["tex#tex"] == (0, 1)
+    want = (rgb[::-1].astype(np.float32) / np.float32(255)).astype(np.float32)
+    assert np.array_equal(base, want)
+    assert np.array_equal(rough[..., 0], want[..., 1])
+
+
+@pytest.mark.parametrize("flaw", ["adler", "interlaced", "truncated", "filter"])
+def test_malformed_png_is_rejected(tmp_path, flaw):
+    img = np.random.default_rng(1).integers(0, 256, (8, 8, 3)).astype(np.uint8)
+    png = png_bytes(img, 2, [1, 2], "dynamic", bad_adler=flaw == "adler", interlace=int(flaw == "interlaced"))
+    if flaw == "truncated":
+        png = png[:len(png) // 2]
+    if flaw == "filter":
+        png = png_bytes(img, 2, [7], "stored")
+    path = str(tmp_path / "t.glb")
+    _textured_glb(path, png, png)
+    r = _run("--dump-glb", path, str(tmp_path / "o.bin"))
+    assert r.returncode == 1 and "PNG" in r.stderr, r.stderr
