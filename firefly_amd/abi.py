@@ -34,7 +34,7 @@ class SceneDesc(C.Structure):
                 ("uvs", C.c_void_p), ("material_ids", C.c_void_p), ("n_materials", C.c_int32),
                 ("materials", C.c_void_p), ("n_textures", C.c_int32), ("textures", C.c_void_p),
                 ("tex_w", C.c_void_p), ("tex_h", C.c_void_p), ("tex_c", C.c_void_p),
-                ("max_prims_in_node", C.c_int32), ("builder", C.c_int32)]
+                ("max_prims_in_node", C.c_int32), ("builder", C.c_int32), ("tangents", C.c_void_p)]
 
 
 class Camera(C.Structure):
@@ -202,7 +202,8 @@ class Scene:
     reference's host SAH (FF_BVH_SAH) or on the GPU as a linear BVH (FF_BVH_LBVH)."""
 
     def __init__(self, ctx: Context, positions, normals, uvs, material_ids, materials,
-                 textures: Sequence[np.ndarray] = (), max_prims: int = 8, builder: int = FF_BVH_SAH):
+                 textures: Sequence[np.ndarray] = (), max_prims: int = 8, builder: int = FF_BVH_SAH,
+                 tangents=None):
         self.ctx = ctx
         self._keep = []
         pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 9)
@@ -229,7 +230,11 @@ class Scene:
             self._keep += [ptrs, tw, th, tc]
         d.max_prims_in_node = max_prims
         d.builder = builder
-        self._keep += [pos, nrm, uv, mid, mats, texs]
+        tan = None if tangents is None else np.ascontiguousarray(tangents, dtype=np.float32).reshape(-1, 18)
+        if tan is not None and tan.shape[0] != pos.shape[0]:
+            raise ValueError("tangents: one [3][6] record per triangle")
+        d.tangents = None if tan is None else tan.ctypes.data
+        self._keep += [pos, nrm, uv, mid, mats, texs, tan]
         h = _P()
         check(_lib().ff_scene_create(ctx.h, C.byref(d), C.byref(h)))
         self.h = h

@@ -670,11 +670,13 @@ __device__ __forceinline__ void shade_core(const ff_paths& P, uint32_t id, Shade
     const float* U = S.uv + (size_t)slot * 6;
     float tu = (ldg(U + 2) * hh.y + ldg(U + 4) * hh.z) + ldg(U) * w0;
     float tv = (ldg(U + 3) * hh.y + ldg(U + 5) * hh.z) + ldg(U + 1) * w0;
-    const float* M = S.mats + (size_t)(ldg(S.mat + slot) & MAT_INDEX_MASK) * FF_MAT_FLOATS_DEV;
-    f3 alb = V3(ldg(M + 1), ldg(M + 2), ldg(M + 3));
-    float rough = ldg(M + 4), metal = ldg(M + 5);
-    f3 fres = V3(ldg(M + 6), ldg(M + 7), ldg(M + 8));
-    int at = (int)ldg(M + 10), rt = (int)ldg(M + 11);
+    // the 64-B material record (64-B aligned) as four 16-B loads
+    const float4* M = (const float4*)(S.mats + (size_t)(ldg(S.mat + slot) & MAT_INDEX_MASK) * FF_MAT_FLOATS_DEV);
+    float4 m0 = ldg4(M), m1 = ldg4(M + 1), m2 = ldg4(M + 2), m3 = ldg4(M + 3);
+    f3 alb = V3(m0.y, m0.z, m0.w);
+    float rough = m1.x, metal = m1.y;
+    f3 fres = V3(m1.z, m1.w, m2.x);
+    int at = (int)m2.z, rt = (int)m2.w, mt = (int)m3.x;
     bool rough_live = false;
     if (at >= 0) {
         ff_tex_desc td = S.tex[at];
@@ -688,6 +690,26 @@ __device__ __forceinline__ void shade_core(const ff_paths& P, uint32_t id, Shade
         bilinear(td.data, td.w, td.h, 1, tu, tv, t1);
         rough = fmaxf(t1[0], 0.01f);
         rough_live = t1[0] > 0.01f;
+    }
+    if (mt >= 0) {
+        ff_tex_desc td = S.tex[mt];
+        float t1[1];
+        bilinear(td.data, td.w, td.h, 1, tu, tv, t1);
+        metal = t1[0];
+    }
+    // the light shadow ray leaves along the interpolated normal (CRT/kernel.cu:437);
+    // everything else uses the normal-mapped one (get_normal, CRT/dMaterial.cu:61-87)
+    f3 n_geo = nrm;
+    int nt = (int)m3.y;
+    if (nt >= 0) {
+        const float* T = S.tan + (size_t)slot * 18;
+        f3 tg = vnorm(vadd(vadd(vscl(ld3(T + 6), hh.y), vscl(ld3(T + 12), hh.z)), vscl(ld3(T), w0)));
+        f3 bt = vnorm(vadd(vadd(vscl(ld3(T + 9), hh.y), vscl(ld3(T + 15), hh.z)), vscl(ld3(T + 3), w0)));
+        ff_tex_desc td = S.tex[nt];
+        float t3[3];
+        bilinear(td.data, td.w, td.h, 3, tu, tv, t3);
+        float mx = t3[0] * 2.f - 1.f, my = t3[1] * 2.f - 1.f, mz = t3[2] * 2.f - 1.f;
+        nrm = vnorm(vadd(vadd(vscl(tg, mx), vscl(bt, my)), vscl(nrm, mz)));
     }
     f3 wo = vneg(d);
     f3 zero = V3(0, 0, 0);
@@ -711,7 +733,7 @@ __device__ __forceinline__ void shade_core(const ff_paths& P, uint32_t id, Shade
         f3 nn = entering ? gn : vneg(gn);
         float cos_i = fabsf(cs);
         if (cos_i > 1.f) cos_i = 1.f;
-        float ior = ldg(M + 9);
+        float ior = m2.y;
         float eta_i = entering ? 1.f : ior, eta_t = entering ? ior : 1.f;
         float F = fresnel_dielectric(cos_i, eta_i, eta_t);
         if (rng(base, len, DIM_DIEL) < F) {
@@ -824,7 +846,7 @@ __device__ __forceinline__ void shade_core(const ff_paths& P, uint32_t id, Shade
         }
     }
     so.needL = !vzero(so.A) || so.needS || so.needD;
-    so.lo = vadd(Pp, vscl(nrm, 0.001f));
+    so.lo = vadd(Pp, vscl(n_geo, 0.001f));
     f3 dir;
     if (TYPE == MAT_MIX) {
         dir = (rng(base, len, DIM_LOBE) < 0.5f) ? wi_d : wi_s;

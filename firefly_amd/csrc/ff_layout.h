@@ -95,6 +95,7 @@ struct ff_dev_scene {
     const float* tris;
     const float* nrm;
     const float* uv;
+    const float* tan;          // per slot [3][6] tangent, bitangent (null: no normal textures)
     const uint32_t* mat;       // per slot: material index | material type << MAT_TYPE_SHIFT
     const float* mats;
     const ff_tex_desc* tex;

@@ -218,7 +218,8 @@ dScene::dScene(Scene* s, int device, int max_prims_in_node, int builder) : h_sce
     auto push_mat = [&](const Material& m) {
         float r[FF_MAT_FLOATS] = {(float)m.type, m.baseColorFactor.x, m.baseColorFactor.y, m.baseColorFactor.z,
                                   m.roughnessFactor, m.metallicFactor, m.fresnel.x, m.fresnel.y, m.fresnel.z, m.ior,
-                                  (float)m.baseColorTexture, (float)m.roughnessTexture, 0.f, 0.f, 0.f, 0.f};
+                                  (float)m.baseColorTexture, (float)m.roughnessTexture, (float)m.metallicTexture,
+                                  (float)m.normalTexture, 0.f, 0.f};
         mats.insert(mats.end(), r, r + FF_MAT_FLOATS);
     };
     push_mat(Material{});  // index 0: default material for unknown names
@@ -227,14 +228,19 @@ dScene::dScene(Scene* s, int device, int max_prims_in_node, int builder) : h_sce
         push_mat(kv.second);
     }
     // triangles of every mesh in model order (dScene::load_models, CRT/dScene.cpp:82-170)
-    std::vector<float> pos, nrm, uv;
+    std::vector<float> pos, nrm, uv, tan;
     std::vector<int32_t> mid;
+    bool any_normal_map = false;
+    for (auto& kv : s->materials_loaded) any_normal_map |= kv.second.normalTexture >= 0;
     for (const Model& m : s->models)
         for (const Mesh& me : m.meshes) {
             auto it = index.find(me.material);
             int mi = it == index.end() ? 0 : it->second;
             bool has_n = me.normals.size() == me.positions.size();
             bool has_t = me.texcoords.size() == me.positions.size();
+            bool has_tan = me.tangents.size() == me.positions.size() && me.bitangents.size() == me.positions.size();
+            if (any_normal_map && mi > 0 && mats[(size_t)mi * FF_MAT_FLOATS + 13] >= 0.f && !has_tan)
+                throw Error(FF_ERR_INVALID, "dScene: mesh " + me.name + " has a normal texture but no tangents");
             for (size_t k = 0; k + 2 < me.indices.size(); k += 3) {
                 for (int v = 0; v < 3; v++) {
                     uint32_t vi = me.indices[k + v];
@@ -245,6 +251,11 @@ dScene::dScene(Scene* s, int device, int max_prims_in_node, int builder) : h_sce
                     nrm.insert(nrm.end(), {n.x, n.y, n.z});
                     vec2 t = has_t ? me.texcoords[vi] : vec2{};
                     uv.insert(uv.end(), {t.x, t.y});
+                    if (any_normal_map) {
+                        vec3 a = has_tan ? me.tangents[vi] : vec3(1.f, 0.f, 0.f);
+                        vec3 b = has_tan ? me.bitangents[vi] : vec3(0.f, 0.f, 1.f);
+                        tan.insert(tan.end(), {a.x, a.y, a.z, b.x, b.y, b.z});
+                    }
                 }
                 mid.push_back(mi);
             }
@@ -272,6 +283,7 @@ dScene::dScene(Scene* s, int device, int max_prims_in_node, int builder) : h_sce
     d.tex_c = tc.data();
     d.max_prims_in_node = max_prims_in_node;
     d.builder = builder;
+    d.tangents = any_normal_map ? tan.data() : nullptr;
     check(ff_context_create(device, &ctx), "dScene: context");
     try {
         check(ff_scene_create(ctx, &d, &scene), "dScene: scene");

@@ -60,6 +60,8 @@ struct Material {
     float ior = 1.5f;
     int baseColorTexture = -1;  // index into Scene::textures_loaded
     int roughnessTexture = -1;
+    int metallicTexture = -1;
+    int normalTexture = -1;     // tangent-space normal map (needs mesh tangents)
 };
 
 // Float texture in host memory ([h][w][channels]).
@@ -74,6 +76,7 @@ struct Mesh {
     std::string name;
     std::vector<vec3> positions, normals;
     std::vector<vec2> texcoords;
+    std::vector<vec3> tangents, bitangents;  // per vertex (Vertex, CRT/Mesh.h:16-17), may be empty
     std::vector<uint32_t> indices;
     std::string material;
     int get_nmb_of_triangles() const { return (int)(indices.size() / 3); }

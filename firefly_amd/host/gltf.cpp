@@ -12,13 +12,18 @@
 //     roughness 1 and metallic 0 as the reference's loader forces them
 //     (CRT/Scene.cpp:203-204);
 //   * embedded PNG images (png.cpp) of baseColorTexture (RGB) and of
-//     metallicRoughnessTexture (its green channel, the roughness, which the
-//     shading reads in place of the factor); samples are scaled by 1/255 and
-//     rows stored bottom first, matching the flipped v.
+//     metallicRoughnessTexture (its green channel, the roughness, and its
+//     blue channel, the metalness, which the shading reads in place of the
+//     factors as get_roughness / get_metallic do, CRT/dMaterial.cu:25-59);
+//     samples are scaled by 1/255 and rows stored bottom first, matching the
+//     flipped v; normalTexture (RGB, read by get_normal, CRT/dMaterial.cu:61-87)
+//     with per-vertex tangent frames from TANGENT, or computed from the
+//     texture coordinates when the file has none.
 // Node transforms compose parent * local, the glTF rule (the reference's
 // accumulation multiplies in the other order, CRT/Scene.cpp:88; the two
 // agree for one level of nesting).  Not read: JPEG or external images,
-// normal / occlusion / emissive maps, cameras and lights (the reference ignores both,
+// occlusion / emissive maps (the reference's shading reads neither; its
+// emission is zero, CRT/dScene.cpp:196), cameras and lights (the reference ignores both,
 // CRT/Scene.cpp:230-258), sparse accessors, skins, morph targets,
 // non-triangle primitives; a file using sparse accessors or other primitive
 // modes is rejected.
@@ -372,14 +377,16 @@ struct Loader {
 
     // glTF texture `ti` as a Scene texture: RGB base colour, or roughness
     // from the green channel; -1 when the material names none
-    int texture(const JVal* info, bool rough) {
+    // channel < 0: the RGB texture of a base colour; 1 or 2: a one-channel
+    // texture of the green (roughness) or blue (metalness) samples
+    int texture(const JVal* info, int channel) {
         if (!info) return -1;
         const JVal* ti = info->get("index");
         if (!ti) return -1;
         const JVal& tj = d.list("textures").at((size_t)ti->num);
         const JVal* src = tj.get("source");
         if (!src) throw Error(FF_ERR_UNSUPPORTED, "glTF: texture without a source image");
-        long key = (long)src->num * 2 + (rough ? 1 : 0);
+        long key = (long)src->num * 3 + (channel < 0 ? 0 : channel);
         for (const auto& kv : tex_done)
             if (kv.first == key) return kv.second;
         const JVal& im = d.list("images").at((size_t)src->num);
@@ -397,14 +404,14 @@ struct Loader {
         t.filepath = "image" + std::to_string((long)src->num);
         t.width = w;
         t.height = h;
-        t.nrComponents = rough ? 1 : 3;
+        t.nrComponents = channel < 0 ? 3 : 1;
         t.data.resize((size_t)w * h * (size_t)t.nrComponents);
         for (int y = 0; y < h; y++)
             for (int x = 0; x < w; x++) {
                 const uint8_t* s = &px[((size_t)(h - 1 - y) * w + x) * (size_t)c];  // bottom row first
                 float* o = &t.data[((size_t)y * w + x) * (size_t)t.nrComponents];
-                if (rough) {
-                    o[0] = (c >= 3 ? s[1] : s[0]) / 255.f;  // green; gray images: the one sample
+                if (channel >= 0) {
+                    o[0] = (c >= 3 ? s[channel] : s[0]) / 255.f;  // gray images: the one sample
                 } else {
                     for (int k = 0; k < 3; k++) o[k] = (c >= 3 ? s[k] : s[0]) / 255.f;
                 }
@@ -435,9 +442,11 @@ struct Loader {
             if (const JVal* pbr = mj.get("pbrMetallicRoughness")) {
                 if (const JVal* bc = pbr->get("baseColorFactor"))
                     m.baseColorFactor = vec3((float)bc->at(0).num, (float)bc->at(1).num, (float)bc->at(2).num);
-                m.baseColorTexture = texture(pbr->get("baseColorTexture"), false);
-                m.roughnessTexture = texture(pbr->get("metallicRoughnessTexture"), true);
+                m.baseColorTexture = texture(pbr->get("baseColorTexture"), -1);
+                m.roughnessTexture = texture(pbr->get("metallicRoughnessTexture"), 1);
+                m.metallicTexture = texture(pbr->get("metallicRoughnessTexture"), 2);
             }
+            m.normalTexture = texture(mj.get("normalTexture"), -1);
             if (const JVal* e = mj.get("emissiveFactor"))
                 m.emissiveColorFactor = vec3((float)e->at(0).num, (float)e->at(1).num, (float)e->at(2).num);
             m.roughnessFactor = 1.f;  // forced, as CRT/Scene.cpp:203-204
@@ -533,6 +542,75 @@ struct Loader {
                 for (size_t v = 0; v < nv; v++) {
                     float u = (float)T.v[2 * v], w = (float)T.v[2 * v + 1];
                     me.texcoords[v] = vec2{u, 1.f - w};  // aiProcess_FlipUVs
+                }
+            }
+            const JVal* tgi = at->get("TANGENT");
+            const JVal* nmi = at->get("NORMAL");
+            if (tgi && nmi) {
+                // TANGENT xyz and bitangent cross(NORMAL, xyz) * w (the glTF rule), both
+                // in mesh space, then placed as the reference places them:
+                // normalize(mat * vec4(v, 1)), translation included (CRT/Scene.cpp:143-153)
+                Acc T = read_accessor(d, (int)tgi->num);
+                Acc Nn = read_accessor(d, (int)nmi->num);
+                if (T.comps != 4 || T.count != nv) throw Error(FF_ERR_INVALID, "glTF: TANGENT must be VEC4 per vertex");
+                auto place = [&](double x, double y, double z) {
+                    double X = ((m[0] * x + m[4] * y) + m[8] * z) + m[12], Y = ((m[1] * x + m[5] * y) + m[9] * z) + m[13],
+                           Z = ((m[2] * x + m[6] * y) + m[10] * z) + m[14];
+                    double l = std::sqrt((X * X + Y * Y) + Z * Z);
+                    return l > 0 ? vec3((float)(X / l), (float)(Y / l), (float)(Z / l)) : vec3(1.f, 0.f, 0.f);
+                };
+                me.tangents.resize(nv);
+                me.bitangents.resize(nv);
+                for (size_t v = 0; v < nv; v++) {
+                    double tx = T.v[4 * v], ty = T.v[4 * v + 1], tz = T.v[4 * v + 2], tw = T.v[4 * v + 3];
+                    double nx = Nn.v[3 * v], ny = Nn.v[3 * v + 1], nz = Nn.v[3 * v + 2];
+                    me.tangents[v] = place(tx, ty, tz);
+                    me.bitangents[v] = place((ny * tz - nz * ty) * tw, (nz * tx - nx * tz) * tw, (nx * ty - ny * tx) * tw);
+                }
+            } else if (!me.texcoords.empty() && scene.materials_loaded[me.material].normalTexture >= 0) {
+                // from the texture coordinates (the per-face vectors of assimp's
+                // aiProcess_CalcTangentSpace): tangent w*sy - v*ty, bitangent
+                // w*sx - v*tx, signed by the uv winding, made orthogonal to each
+                // vertex normal, normalised, summed per vertex, normalised.  The
+                // import runs that step before aiProcess_FlipUVs; with v already
+                // flipped here the same bitangent is -(w*sx - v*tx).
+                std::vector<double> at3(3 * nv, 0.0), ab3(3 * nv, 0.0);
+                for (size_t k = 0; k + 2 < me.indices.size(); k += 3) {
+                    uint32_t i0 = me.indices[k], i1 = me.indices[k + 1], i2 = me.indices[k + 2];
+                    const vec3 &p0 = me.positions[i0], &p1 = me.positions[i1], &p2 = me.positions[i2];
+                    double vx = (double)p1.x - p0.x, vy = (double)p1.y - p0.y, vz = (double)p1.z - p0.z;
+                    double wx = (double)p2.x - p0.x, wy = (double)p2.y - p0.y, wz = (double)p2.z - p0.z;
+                    const vec2 &t0 = me.texcoords[i0], &t1 = me.texcoords[i1], &t2 = me.texcoords[i2];
+                    double sx = (double)t1.x - t0.x, sy = (double)t1.y - t0.y;
+                    double tx = (double)t2.x - t0.x, ty = (double)t2.y - t0.y;
+                    double dir = (tx * sy - ty * sx) < 0.0 ? -1.0 : 1.0;
+                    if (sx * ty == sy * tx) { sx = 0.0; sy = 1.0; tx = 1.0; ty = 0.0; }
+                    double T3[3] = {(wx * sy - vx * ty) * dir, (wy * sy - vy * ty) * dir, (wz * sy - vz * ty) * dir};
+                    double B3[3] = {-(wx * sx - vx * tx) * dir, -(wy * sx - vy * tx) * dir, -(wz * sx - vz * tx) * dir};
+                    for (uint32_t vi : {i0, i1, i2}) {
+                        const vec3& n = me.normals[vi];
+                        for (int which = 0; which < 2; which++) {
+                            const double* src = which ? B3 : T3;
+                            double dn = (src[0] * n.x + src[1] * n.y) + src[2] * n.z;
+                            double x = src[0] - n.x * dn, y = src[1] - n.y * dn, z = src[2] - n.z * dn;
+                            double l = std::sqrt((x * x + y * y) + z * z);
+                            if (!(l > 0)) continue;
+                            double* acc = which ? &ab3[3 * vi] : &at3[3 * vi];
+                            acc[0] += x / l;
+                            acc[1] += y / l;
+                            acc[2] += z / l;
+                        }
+                    }
+                }
+                auto unit = [](const double* a, vec3 fallback) {
+                    double l = std::sqrt((a[0] * a[0] + a[1] * a[1]) + a[2] * a[2]);
+                    return l > 0 ? vec3((float)(a[0] / l), (float)(a[1] / l), (float)(a[2] / l)) : fallback;
+                };
+                me.tangents.resize(nv);
+                me.bitangents.resize(nv);
+                for (size_t v = 0; v < nv; v++) {
+                    me.tangents[v] = unit(&at3[3 * v], vec3(1.f, 0.f, 0.f));
+                    me.bitangents[v] = unit(&ab3[3 * v], vec3(0.f, 0.f, 1.f));
                 }
             }
             tris += me.get_nmb_of_triangles();

@@ -7,10 +7,11 @@
 //
 // A model.glb argument adds that file's meshes (Scene::load) to the scene's.
 // The dump: int32 n_meshes; per mesh int32 name length, name, int32 material
-// name length, name, int32 n_vertices, n_indices, has_texcoords, float
-// positions[n][3], normals[n][3], (texcoords[n][2]), uint32 indices[]; then
-// int32 n_materials; per material int32 name length, name, float base[3],
-// roughness, metallic, int32 base colour and roughness texture (-1 none);
+// name length, name, int32 n_vertices, n_indices, has_texcoords, has_tangents,
+// float positions[n][3], normals[n][3], (texcoords[n][2]), (tangents[n][3],
+// bitangents[n][3]), uint32 indices[]; then int32 n_materials; per material
+// int32 name length, name, float base[3], roughness, metallic, int32 base
+// colour, roughness, metallic and normal texture (-1 none);
 // then int32 n_textures; per texture int32 width, height, channels and
 // float texels[height][width][channels] (bottom row first).
 //
@@ -53,11 +54,16 @@ int dump_glb(const char* glb, const char* out) {
         for (const Mesh& me : m.meshes) {
             put_str(o, me.name);
             put_str(o, me.material);
-            int32_t h[3] = {(int32_t)me.positions.size(), (int32_t)me.indices.size(), me.texcoords.empty() ? 0 : 1};
-            std::fwrite(h, 4, 3, o);
+            int32_t h[4] = {(int32_t)me.positions.size(), (int32_t)me.indices.size(), me.texcoords.empty() ? 0 : 1,
+                            me.tangents.empty() ? 0 : 1};
+            std::fwrite(h, 4, 4, o);
             std::fwrite(me.positions.data(), sizeof(vec3), me.positions.size(), o);
             std::fwrite(me.normals.data(), sizeof(vec3), me.normals.size(), o);
             if (h[2]) std::fwrite(me.texcoords.data(), sizeof(vec2), me.texcoords.size(), o);
+            if (h[3]) {
+                std::fwrite(me.tangents.data(), sizeof(vec3), me.tangents.size(), o);
+                std::fwrite(me.bitangents.data(), sizeof(vec3), me.bitangents.size(), o);
+            }
             std::fwrite(me.indices.data(), 4, me.indices.size(), o);
         }
     int32_t nmat = (int32_t)scene.materials_loaded.size();
@@ -67,8 +73,9 @@ int dump_glb(const char* glb, const char* out) {
         float r[5] = {kv.second.baseColorFactor.x, kv.second.baseColorFactor.y, kv.second.baseColorFactor.z,
                       kv.second.roughnessFactor, kv.second.metallicFactor};
         std::fwrite(r, 4, 5, o);
-        int32_t t[2] = {kv.second.baseColorTexture, kv.second.roughnessTexture};
-        std::fwrite(t, 4, 2, o);
+        int32_t t[4] = {kv.second.baseColorTexture, kv.second.roughnessTexture, kv.second.metallicTexture,
+                        kv.second.normalTexture};
+        std::fwrite(t, 4, 4, o);
     }
     int32_t ntex = (int32_t)scene.textures_loaded.size();
     std::fwrite(&ntex, 4, 1, o);

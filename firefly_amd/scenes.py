@@ -39,6 +39,7 @@ class SceneData:
     cam_fovy: float = 45.0
     mesh_sizes: tuple = ()     # triangles per mesh, in order
     mesh_materials: tuple = ()  # material index of each mesh
+    tangents: Optional[np.ndarray] = None  # [n,3,6] tangent, bitangent per vertex (normal textures)
 
     @property
     def n_tris(self) -> int:
@@ -193,9 +194,22 @@ def displaced_icosphere(subdiv: int, center, radius: float, amp: float, seed: in
     return pos, nrm, uv, region
 
 
+def tangent_frames(normals: np.ndarray) -> np.ndarray:
+    """Per-vertex tangent frames [n,3,6] (tangent xyz, bitangent xyz) for
+    normals [n,3,3]: the tangent is the unit vector orthogonal to the normal
+    in the plane of the normal and the world axis least aligned with it;
+    bitangent = normal x tangent."""
+    n = normals.astype(np.float64)
+    ax = np.eye(3)[np.argmin(np.abs(n), axis=-1)]
+    t = ax - n * np.sum(ax * n, axis=-1, keepdims=True)
+    t /= np.linalg.norm(t, axis=-1, keepdims=True)
+    b = np.cross(n, t)
+    return np.concatenate([t, b], axis=-1).astype(np.float32)
+
+
 # ---------------------------------------------------------------- materials --
 def material(mtype: int, albedo=(0.5, 0.5, 0.5), roughness=1.0, metallic=0.0, fresnel=0.04,
-             ior=1.5, albedo_tex=-1, rough_tex=-1) -> np.ndarray:
+             ior=1.5, albedo_tex=-1, rough_tex=-1, metal_tex=-1, normal_tex=-1) -> np.ndarray:
     m = np.zeros(FF_MAT_FLOATS, np.float32)
     m[0] = mtype
     m[1:4] = albedo
@@ -205,6 +219,8 @@ def material(mtype: int, albedo=(0.5, 0.5, 0.5), roughness=1.0, metallic=0.0, fr
     m[9] = ior
     m[10] = albedo_tex
     m[11] = rough_tex
+    m[12] = metal_tex
+    m[13] = normal_tex
     return m
 
 

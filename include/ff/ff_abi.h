@@ -49,7 +49,12 @@ typedef struct {
 
 /* Material record (16 floats), fields of dMaterial (CRT/dMaterial.cuh:9-31):
  * [0] type, [1..3] base colour, [4] roughness, [5] metallic, [6..8] fresnel f0,
- * [9] ior, [10] albedo texture (-1 none), [11] roughness texture (-1 none). */
+ * [9] ior, [10] albedo texture (-1 none, 3 channels), [11] roughness texture
+ * (-1 none, 1 channel), [12] metallic texture (-1 none, 1 channel; the
+ * metallicRoughnessTexture blue / metallicTexture lookups of get_metallic,
+ * CRT/dMaterial.cu:44-59), [13] normal texture (-1 none, 3 channels, tangent
+ * space mapped by 2t-1 as get_normal, CRT/dMaterial.cu:61-87; needs the
+ * scene's tangents), [14..15] reserved. */
 #define FF_MAT_MIX 0        /* reference wf_mat_mix lobe pair (diffuse + GGX)  */
 #define FF_MAT_DIFFUSE 1    /* reference diffuse lobe (wf_mat_diffuse)         */
 #define FF_MAT_CONDUCTOR 2  /* reference GGX lobe (wf_mat_cook), metallic      */
@@ -72,6 +77,9 @@ typedef struct {
     const int32_t *tex_c;      /* 1 or 3 */
     int32_t max_prims_in_node; /* SAH leaf size, 8 in the reference (CRT/dScene.cpp:372) */
     int32_t builder;           /* FF_BVH_SAH (0) or FF_BVH_LBVH */
+    const float *tangents;     /* [n][3][6]: per vertex tangent xyz, bitangent xyz
+                                  (Vertex::tangent/bitangent, CRT/dVertex.cuh:8-9);
+                                  may be NULL when no material has a normal texture */
 } ff_scene_desc;
 
 /* BVH builders.  FF_BVH_SAH: the reference's host SAH tree (BVHAccel,

@@ -1314,7 +1314,7 @@ void or_spec_sample(const float *n, const float *wo, float r, float e0, float e1
 typedef struct {
     const or_scene *s;
     const or_params *p;
-    float *tri, *tnrm, *tuv;   /* BVH-slot order */
+    float *tri, *tnrm, *tuv, *ttan;   /* BVH-slot order (ttan NULL without tangents) */
     int32_t *tmat;
     double *my, *mp, *conds;
     float *pdf;
@@ -1445,7 +1445,7 @@ static v3 trace_path(const rctx *c, int32_t px, int32_t py, int32_t sample, psta
         v3 alb = V3(M[1], M[2], M[3]);
         float rough = M[4], metal = M[5];
         v3 fres = V3(M[6], M[7], M[8]);
-        int at = (int)M[10], rt = (int)M[11];
+        int at = (int)M[10], rt = (int)M[11], mt = (int)M[12];
         if (at >= 0) {
             float t3[3];
             or_bilinear(s->tex[at], s->tex_w[at], s->tex_h[at], 3, uvv.x, uvv.y, t3);
@@ -1455,6 +1455,28 @@ static v3 trace_path(const rctx *c, int32_t px, int32_t py, int32_t sample, psta
             float t1[3];
             or_bilinear(s->tex[rt], s->tex_w[rt], s->tex_h[rt], 1, uvv.x, uvv.y, t1);
             rough = fmaxf(t1[0], 0.01f);
+        }
+        if (mt >= 0) { /* get_metallic, CRT/dMaterial.cu:44-59 (the channel picked at load) */
+            float t1[3];
+            or_bilinear(s->tex[mt], s->tex_w[mt], s->tex_h[mt], 1, uvv.x, uvv.y, t1);
+            metal = t1[0];
+        }
+        /* get_normal (CRT/dMaterial.cu:61-87): interpolated tangent and
+         * bitangent (CRT/dTriangle.cu:95-96), texel mapped by 2t-1, then
+         * TBN * m = T m.x + B m.y + N m.z (CRT/dMatrix.cuh:70-78 with the
+         * rows T, B, N, w = 1); the light shadow ray keeps the interpolated
+         * normal (CRT/kernel.cu:437). */
+        v3 n_geo = n;
+        int nt = (int)M[13];
+        if (nt >= 0) {
+            const float *Tg = c->ttan + (size_t)h.id * 18;
+            float w0 = (1.f - h.u) - h.v;
+            v3 tg = vnorm(vadd(vadd(vscl(vld(Tg + 6), h.u), vscl(vld(Tg + 12), h.v)), vscl(vld(Tg), w0)));
+            v3 bt = vnorm(vadd(vadd(vscl(vld(Tg + 9), h.u), vscl(vld(Tg + 15), h.v)), vscl(vld(Tg + 3), w0)));
+            float t3[3];
+            or_bilinear(s->tex[nt], s->tex_w[nt], s->tex_h[nt], 3, uvv.x, uvv.y, t3);
+            v3 m = V3(t3[0] * 2.f - 1.f, t3[1] * 2.f - 1.f, t3[2] * 2.f - 1.f);
+            n = vnorm(vadd(vadd(vscl(tg, m.x), vscl(bt, m.y)), vscl(n, m.z)));
         }
         v3 wo = vneg(d);
         v3 next_o = P, next_d = d;
@@ -1496,7 +1518,7 @@ static v3 trace_path(const rctx *c, int32_t px, int32_t py, int32_t sample, psta
             or_env_sample_dir(c->my, c->conds, s->env_w, s->env_h, ex, ey, &tx, &ty, dir3);
             v3 spt = vadd(V3(0.f, 0.f, 0.f), vscl(vld(dir3), 10000.f));
             v3 wi = vnorm(vsub(spt, P));
-            lo = vadd(P, vscl(n, 0.001f));
+            lo = vadd(P, vscl(n_geo, 0.001f));
             ld = wi;
             double diff_pdf = (double)(1.f / (2.f * F_PI));
             int do_spec = (type == OR_MAT_MIX || type == OR_MAT_CONDUCTOR);
@@ -1669,11 +1691,13 @@ void or_render(const or_scene *s, const or_params *p, float *film, or_stats *st)
     c.tnrm = (float *)malloc(sizeof(float) * 9 * n);
     c.tuv = (float *)malloc(sizeof(float) * 6 * n);
     c.tmat = (int32_t *)malloc(sizeof(int32_t) * n);
+    c.ttan = s->tan ? (float *)malloc(sizeof(float) * 18 * n) : NULL;
     for (size_t i = 0; i < n; i++) {
         int32_t src = s->ordered[i];
         memcpy(c.tri + i * 9, s->pos + (size_t)src * 9, 36);
         memcpy(c.tnrm + i * 9, s->nrm + (size_t)src * 9, 36);
         memcpy(c.tuv + i * 6, s->uv + (size_t)src * 6, 24);
+        if (s->tan) memcpy(c.ttan + i * 18, s->tan + (size_t)src * 18, 72);
         c.tmat[i] = s->mat[src];
     }
     if (s->env_rgb) {
@@ -1694,7 +1718,7 @@ void or_render(const or_scene *s, const or_params *p, float *film, or_stats *st)
     free(th);
     pthread_mutex_destroy(&c.mu);
     if (st) *st = c.st;
-    free(c.tri); free(c.tnrm); free(c.tuv); free(c.tmat);
+    free(c.tri); free(c.tnrm); free(c.tuv); free(c.ttan); free(c.tmat);
     free(c.my); free(c.mp); free(c.conds); free(c.pdf);
 }
 
@@ -2051,11 +2075,13 @@ static void rctx_init(rctx *c, const or_scene *s, const or_params *p) {
     c->tnrm = (float *)malloc(sizeof(float) * 9 * n);
     c->tuv = (float *)malloc(sizeof(float) * 6 * n);
     c->tmat = (int32_t *)malloc(sizeof(int32_t) * n);
+    c->ttan = s->tan ? (float *)malloc(sizeof(float) * 18 * n) : NULL;
     for (size_t i = 0; i < n; i++) {
         int32_t src = s->ordered[i];
         memcpy(c->tri + i * 9, s->pos + (size_t)src * 9, 36);
         memcpy(c->tnrm + i * 9, s->nrm + (size_t)src * 9, 36);
         memcpy(c->tuv + i * 6, s->uv + (size_t)src * 6, 24);
+        if (s->tan) memcpy(c->ttan + i * 18, s->tan + (size_t)src * 18, 72);
         c->tmat[i] = s->mat[src];
     }
     if (s->env_rgb) {
@@ -2071,7 +2097,7 @@ static void rctx_init(rctx *c, const or_scene *s, const or_params *p) {
 }
 static void rctx_free(rctx *c) {
     pthread_mutex_destroy(&c->mu);
-    free(c->tri); free(c->tnrm); free(c->tuv); free(c->tmat);
+    free(c->tri); free(c->tnrm); free(c->tuv); free(c->ttan); free(c->tmat);
     free(c->my); free(c->mp); free(c->conds); free(c->pdf);
 }
 static void run_diff(dctx *D) {

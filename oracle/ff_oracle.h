@@ -33,7 +33,8 @@ typedef struct {
 /* Material record: 16 floats.  [0] type (0 mix, 1 diffuse, 2 conductor,
  * 3 dielectric), [1..3] base colour, [4] roughness, [5] metallic,
  * [6..8] fresnel f0, [9] ior, [10] albedo texture (-1 none),
- * [11] roughness texture (-1 none), [12..15] reserved.
+ * [11] roughness texture (-1 none), [12] metallic texture (-1 none),
+ * [13] normal texture (-1 none; needs tan), [14..15] reserved.
  * Fields follow dMaterial (CRT/dMaterial.cuh:9-31). */
 #define OR_MAT_MIX 0
 #define OR_MAT_DIFFUSE 1
@@ -59,6 +60,7 @@ typedef struct {
     int32_t n_nodes;       /* BVH from or_bvh_build over pos             */
     const or_node *nodes;
     const int32_t *ordered;/* BVH slot -> original triangle              */
+    const float *tan;      /* [n][3][6] tangent, bitangent per vertex, or NULL */
 } or_scene;
 
 typedef struct {

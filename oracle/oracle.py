@@ -26,7 +26,7 @@ class OrScene(C.Structure):
                 ("env_h", C.c_int32), ("env_rgb", C.c_void_p), ("env_color", C.c_float * 3),
                 ("cam_inv_vp", C.c_float * 16), ("lens_radius", C.c_float), ("focal", C.c_float),
                 ("exposure", C.c_float), ("n_nodes", C.c_int32), ("nodes", C.c_void_p),
-                ("ordered", C.c_void_p)]
+                ("ordered", C.c_void_p), ("tan", C.c_void_p)]
 
 
 class OrParams(C.Structure):
@@ -261,6 +261,8 @@ class PreparedScene:
             s.cam_inv_vp[i] = float(x)
         s.lens_radius, s.focal, s.exposure = sd.lens_radius, sd.focal, sd.exposure
         s.n_nodes, s.nodes, s.ordered = len(self.nodes), self.nodes.ctypes.data, self.ordered.ctypes.data
+        self.tan = None if sd.tangents is None else f32(sd.tangents).reshape(-1, 18)
+        s.tan = None if self.tan is None else self.tan.ctypes.data
         self.s = s
 
     def tris_ordered(self):

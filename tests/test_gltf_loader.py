@@ -90,10 +90,12 @@ def read_dump(path):
     (nm,) = take("<i")
     for _ in range(nm):
         name, mat = s(), s()
-        nv, ni, has_uv = take("<iii")
+        nv, ni, has_uv, has_tan = take("<4i")
         m = {"name": name, "material": mat, "pos": arr("<f4", 3 * nv).reshape(-1, 3),
              "nrm": arr("<f4", 3 * nv).reshape(-1, 3)}
         m["uv"] = arr("<f4", 2 * nv).reshape(-1, 2) if has_uv else None
+        m["tan"] = arr("<f4", 3 * nv).reshape(-1, 3) if has_tan else None
+        m["bitan"] = arr("<f4", 3 * nv).reshape(-1, 3) if has_tan else None
         m["idx"] = arr("<u4", ni)
         meshes.append(m)
     mats = {}
@@ -101,7 +103,7 @@ def read_dump(path):
     for _ in range(nmat):
         name = s()
         mats[name] = take("<5f")
-        mats[name + "#tex"] = take("<2i")
+        mats[name + "#tex"] = take("<4i")
     texs = []
     (ntex,) = take("<i")
     for _ in range(ntex):
@@ -278,7 +280,7 @@ def test_mirrored_transform_keeps_normals_unit_and_outward(model):
 def test_materials_by_name_with_forced_roughness_and_metallic(model):
     _, (meshes, mats), _ = model
     assert [m["material"] for m in meshes] == ["réd", "default", "1"]
-    assert mats["réd#tex"] == (-1, -1)
+    assert mats["réd#tex"] == (-1, -1, -1, -1)
     assert mats["réd"] == pytest.approx((0.8, 0.1, 0.1, 1.0, 0.0))
     assert mats["1"] == pytest.approx((0.2, 0.4, 0.6, 1.0, 0.0))
     assert mats["default"] == pytest.approx((1.0, 1.0, 1.0, 1.0, 0.0))
@@ -387,8 +389,8 @@ def _textured_glb(path, base_png, rough_png):
 @pytest.mark.parametrize("ctype,zmode,depth", [(2, "dynamic", 8), (6, "fixed", 8), (0, "stored", 8),
                                                (3, "dynamic", 8), (4, "dynamic", 8), (2, "dynamic", 16)])
 def test_png_textures_decoded_exactly(tmp_path, ctype, zmode, depth):
-    """Base colour (RGB) and roughness (green channel) textures from
-    embedded PNGs of every colour type, all five row filters and all three
+    """Base colour (RGB), roughness (green channel) and metalness (blue
+    channel) textures from embedded PNGs of every colour type, all five row filters and all three
     deflate block types: texels = samples / 255, rows bottom first."""
     rng = np.random.default_rng(ctype * 10 + depth)
     h, w = 13, 17
@@ -411,11 +413,12 @@ def test_png_textures_decoded_exactly(tmp_path, ctype, zmode, depth):
     r = _run("--dump-glb", path, out)
     assert r.returncode == 0, r.stderr
     meshes, mats = read_dump(out)
-    base, rough = read_dump.textures
-    assert mats["tex#tex"] == (0, 1)
+    base, rough, metal = read_dump.textures
+    assert mats["tex#tex"] == (0, 1, 2, -1)
     want = (rgb[::-1].astype(np.float32) / np.float32(255)).astype(np.float32)
     assert np.array_equal(base, want)
     assert np.array_equal(rough[..., 0], want[..., 1])
+    assert np.array_equal(metal[..., 0], want[..., 2])
 
 
 @pytest.mark.parametrize("flaw", ["adler", "interlaced", "truncated", "filter"])
@@ -430,3 +433,60 @@ def test_malformed_png_is_rejected(tmp_path, flaw):
     _textured_glb(path, png, png)
     r = _run("--dump-glb", path, str(tmp_path / "o.bin"))
     assert r.returncode == 1 and "PNG" in r.stderr, r.stderr
+
+
+def _normal_mapped_glb(path, tangent=None, translation=(0.0, 0.0, 0.0)):
+    """A quad in the xz plane (normal +y, texture u = x, v = z) whose
+    material has a normal texture; TANGENT given or not."""
+    g = Glb()
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 0, 1], [1, 0, 1]], np.float32)
+    attrs = {"POSITION": g.acc(pos, 5126, "VEC3"),
+             "NORMAL": g.acc(np.tile(np.array([0, 1, 0], np.float32), (4, 1)), 5126, "VEC3"),
+             "TEXCOORD_0": g.acc(pos[:, [0, 2]].copy(), 5126, "VEC2")}
+    if tangent is not None:
+        attrs["TANGENT"] = g.acc(np.tile(np.array(tangent, np.float32), (4, 1)), 5126, "VEC4")
+    idx = g.acc(np.array([0, 2, 1, 1, 2, 3], np.uint16), 5123, "SCALAR")
+    png = png_bytes(np.full((2, 2, 3), 128, np.uint8), 2, [0], "stored")
+    g.write(path, {"scenes": [{"nodes": [0]}], "nodes": [{"mesh": 0, "translation": list(translation)}],
+                   "meshes": [{"primitives": [{"attributes": attrs, "indices": idx, "material": 0}]}],
+                   "images": [{"bufferView": g._view(png), "mimeType": "image/png"}],
+                   "textures": [{"source": 0}], "materials": [{"name": "bumpy", "normalTexture": {"index": 0}}]})
+
+
+def test_tangents_read_from_the_file(tmp_path):
+    """TANGENT xyz and bitangent cross(NORMAL, xyz) * w, placed as the
+    reference places them: normalize(M (v, 1)), the translation included."""
+    path, out = str(tmp_path / "t.glb"), str(tmp_path / "t.bin")
+    _normal_mapped_glb(path, tangent=(0, 0, 1, -1), translation=(2, 0, 0))
+    assert _run("--dump-glb", path, out).returncode == 0
+    meshes, mats = read_dump(out)
+    assert mats["bumpy#tex"] == (-1, -1, -1, 0)
+    (me,) = meshes
+    # cross((0,1,0), (0,0,1)) * -1 = (-1, 0, 0); + (2, 0, 0) -> (1, 0, 0)
+    np.testing.assert_allclose(me["tan"], np.tile([2, 0, 1] / np.sqrt(5.0), (4, 1)), rtol=0, atol=1e-7)
+    np.testing.assert_array_equal(me["bitan"], np.tile([1, 0, 0], (4, 1)))
+
+
+def test_tangents_computed_from_texture_coordinates(tmp_path):
+    """Without TANGENT the frame follows the texture coordinates: on the
+    quad u grows along +x, so the tangent is +x; the bitangent is -dp/dv of
+    the file's v (z), -z, as assimp's step before the v flip gives it."""
+    path, out = str(tmp_path / "t.glb"), str(tmp_path / "t.bin")
+    _normal_mapped_glb(path)
+    assert _run("--dump-glb", path, out).returncode == 0
+    (me,), _ = read_dump(out)
+    np.testing.assert_array_equal(me["tan"], np.tile([1, 0, 0], (4, 1)))
+    np.testing.assert_array_equal(me["bitan"], np.tile([0, 0, -1], (4, 1)))
+    assert np.all(np.abs(np.sum(me["tan"] * me["nrm"], axis=-1)) < 1e-7)
+
+
+def test_no_tangents_without_a_normal_texture(tmp_path):
+    g = Glb()
+    p = g.acc(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), 5126, "VEC3")
+    uv = g.acc(np.array([[0, 0], [1, 0], [0, 1]], np.float32), 5126, "VEC2")
+    path, out = str(tmp_path / "t.glb"), str(tmp_path / "t.bin")
+    g.write(path, {"scenes": [{"nodes": [0]}], "nodes": [{"mesh": 0}],
+                   "meshes": [{"primitives": [{"attributes": {"POSITION": p, "TEXCOORD_0": uv}}]}]})
+    assert _run("--dump-glb", path, out).returncode == 0
+    (me,), _ = read_dump(out)
+    assert me["tan"] is None

@@ -113,6 +113,21 @@ def test_non_finite_vertices_are_rejected(gpu):
         assert e.value.status == 1 and "non-finite" in str(e.value)
 
 
+def test_texture_slots_are_validated(gpu):
+    """A normal texture needs the scene's tangent frames; each texture slot
+    takes its channel count (3 base colour / normal, 1 roughness / metalness)."""
+    sd = scenes.spheres_scene(segments=8, rings=4)
+    rgb, one = np.full((4, 4, 3), 0.5, np.float32), np.full((4, 4, 1), 0.5, np.float32)
+    for slot, tex, needs in ((13, [rgb], "tangents"), (12, [rgb], "channels"), (13, [one], "channels"),
+                             (11, [rgb], "channels")):
+        mats = sd.materials.copy()
+        mats[:, slot] = 0
+        tan = scenes.tangent_frames(sd.normals) if needs == "channels" and slot == 13 else None
+        with pytest.raises(abi.FFError) as e:
+            abi.Scene(gpu, sd.positions, sd.normals, sd.uvs, sd.material_ids, mats, tex, tangents=tan)
+        assert e.value.status == 1 and needs in str(e.value), (slot, str(e.value))
+
+
 def test_zero_area_triangles(gpu, oracle):
     """Collinear and collapsed triangles mixed into a scene: the BVH still
     equals the oracle's and they are never hit (det = 0)."""

@@ -45,25 +45,46 @@ def test_cpp_pathtracer_matches_abi_render(gpu, tmp_path):
     assert np.isfinite(img).all() and err < 0.01
 
 
-def test_cpp_pathtracer_renders_a_glb_model(gpu, tmp_path):
-    """Scene::load (binary glTF) feeds the same triangles, normals and
-    materials to the renderer as the C ABI given the loaded arrays: the
-    image matches the C ABI's render of them."""
+@pytest.mark.parametrize("textured", [False, True])
+def test_cpp_pathtracer_renders_a_glb_model(gpu, tmp_path, textured):
+    """Scene::load (binary glTF) feeds the same triangles, normals, texture
+    coordinates, materials and embedded PNG textures to the renderer as the
+    C ABI given the loaded arrays: the image matches the C ABI's render of
+    them."""
     import torch
-    from test_gltf_loader import Glb, read_dump
+    from test_gltf_loader import Glb, read_dump, png_bytes
     sd = scenes.spheres_scene(width=80, height=48, spp=6, max_depth=3)
     g = Glb()
     prims, mats = [], []
+    doc = {}
+    if textured:
+        rng = np.random.default_rng(3)
+        base = png_bytes(rng.integers(0, 256, (24, 32, 3)).astype(np.uint8), 2, [0, 1, 2, 3, 4], "dynamic")
+        mr = png_bytes(rng.integers(0, 256, (16, 16, 4)).astype(np.uint8), 6, [4], "fixed")
+        bump = np.concatenate([rng.integers(70, 186, (20, 20, 2)), rng.integers(220, 256, (20, 20, 1))], axis=-1)
+        nmap = png_bytes(bump.astype(np.uint8), 2, [2, 4], "dynamic")
+        doc["images"] = [{"bufferView": g._view(base), "mimeType": "image/png"},
+                         {"bufferView": g._view(mr), "mimeType": "image/png"},
+                         {"bufferView": g._view(nmap), "mimeType": "image/png"}]
+        doc["textures"] = [{"source": 0}, {"source": 1}, {"source": 2}]
     for m in np.unique(sd.material_ids):
         sel = sd.material_ids == m
         p = g.acc(sd.positions[sel].reshape(-1, 3).astype(np.float32), 5126, "VEC3")
         n = g.acc(sd.normals[sel].reshape(-1, 3).astype(np.float32), 5126, "VEC3")
-        prims.append({"attributes": {"POSITION": p, "NORMAL": n}, "material": len(mats)})
-        mats.append({"name": f"m{m}", "pbrMetallicRoughness": {
-            "baseColorFactor": [float(x) for x in sd.materials[m, 1:4]] + [1.0]}})
+        attrs = {"POSITION": p, "NORMAL": n}
+        pbr = {"baseColorFactor": [float(x) for x in sd.materials[m, 1:4]] + [1.0]}
+        if textured:
+            attrs["TEXCOORD_0"] = g.acc(sd.uvs[sel].reshape(-1, 2).astype(np.float32), 5126, "VEC2")
+            pbr["baseColorTexture"] = {"index": 0}
+            if m % 2 == 0:
+                pbr["metallicRoughnessTexture"] = {"index": 1}
+        prims.append({"attributes": attrs, "material": len(mats)})
+        mats.append({"name": f"m{m}", "pbrMetallicRoughness": pbr})
+        if textured and m % 3 == 1:
+            mats[-1]["normalTexture"] = {"index": 2}
     glb = str(tmp_path / "spheres.glb")
     g.write(glb, {"scenes": [{"nodes": [0]}], "nodes": [{"mesh": 0}], "meshes": [{"primitives": prims}],
-                  "materials": mats})
+                  "materials": mats, **doc})
     # the frame (camera, environment, seed) from a scene file without meshes
     empty = scenes.SceneData(sd.positions[:0], sd.normals[:0], sd.uvs[:0], sd.material_ids[:0],
                              sd.materials, sd.env, sd.inv_vp, sd.width, sd.height, sd.spp, sd.max_depth,
@@ -78,16 +99,29 @@ def test_cpp_pathtracer_renders_a_glb_model(gpu, tmp_path):
     dump = str(tmp_path / "d.bin")
     assert subprocess.run([DEMO, "--dump-glb", glb, dump], capture_output=True, timeout=60).returncode == 0
     meshes, loaded = read_dump(dump)
+    texs = read_dump.textures
+    assert len(texs) == (4 if textured else 0)
     pos = np.concatenate([m["pos"][m["idx"]] for m in meshes]).reshape(-1, 3, 3)
     nrm = np.concatenate([m["nrm"][m["idx"]] for m in meshes]).reshape(-1, 3, 3)
-    names = sorted(loaded)
+    if textured:
+        uv = np.concatenate([m["uv"][m["idx"]] for m in meshes]).reshape(-1, 3, 2)
+        def frame(m):  # meshes without a normal texture carry no tangents
+            if m["tan"] is None:
+                return np.zeros((len(m["idx"]), 6), np.float32)
+            return np.concatenate([m["tan"], m["bitan"]], axis=-1)[m["idx"]]
+        tan = np.concatenate([frame(m) for m in meshes]).reshape(-1, 3, 6)
+        assert any(m["tan"] is not None for m in meshes)
+    else:
+        uv = np.zeros((len(pos), 3, 2), np.float32)
+        tan = None
+    names = sorted(k for k in loaded if not k.endswith("#tex"))
     rec = np.zeros((len(names), abi.FF_MAT_FLOATS), np.float32)
     for k, nm in enumerate(names):
         b = loaded[nm]
         rec[k, :10] = [abi.FF_MAT_MIX, b[0], b[1], b[2], b[3], b[4], 0.04, 0.04, 0.04, 1.5]
-        rec[k, 10:12] = -1
+        rec[k, 10:14] = loaded[nm + "#tex"]
     mid = np.concatenate([np.full(len(m["idx"]) // 3, names.index(m["material"]), np.int32) for m in meshes])
-    sc = abi.Scene(gpu, pos, nrm, np.zeros((len(pos), 3, 2), np.float32), mid, rec)
+    sc = abi.Scene(gpu, pos, nrm, uv, mid, rec, texs, tangents=tan)
     env = abi.EnvMap(gpu, sd.env)
     film = torch.zeros((sd.height, sd.width, 4), dtype=torch.float32, device="cuda")
     abi.render(gpu, sc, env, abi.make_camera(sd.inv_vp), sd.width, sd.height, 6, 3, sd.seed, film)

@@ -20,7 +20,7 @@ def _scene(gpu, pos, max_prims=8, builder=abi.FF_BVH_LBVH):
     nrm = np.tile(np.array([0, 1, 0], np.float32), (n, 3, 1))
     mats = np.zeros((1, abi.FF_MAT_FLOATS), np.float32)
     mats[0, 0] = abi.FF_MAT_DIFFUSE
-    mats[0, 10:12] = -1
+    mats[0, 10:14] = -1
     return abi.Scene(gpu, pos, nrm, None, np.zeros(n, np.int32), mats, max_prims=max_prims, builder=builder)
 
 

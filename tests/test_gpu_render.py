@@ -69,6 +69,61 @@ def test_render_spheres_matches_oracle(gpu, oracle):
     assert abs(st.rays_shadow - cst.rays_shadow) <= 1e-3 * cst.rays_shadow
 
 
+def test_render_textured_materials_match_oracle(gpu, oracle):
+    """Mix materials whose base colour, roughness and metalness come from
+    textures (get_albedo / get_roughness / get_metallic with their texture
+    branches) render as the oracle does."""
+    sd = scenes.spheres_scene(width=96, height=64)
+    rng = np.random.default_rng(5)
+    sd.textures = [rng.uniform(0.1, 0.9, (32, 48, 3)).astype(np.float32),
+                   rng.uniform(0.0, 0.8, (16, 16, 1)).astype(np.float32),
+                   rng.uniform(0.0, 1.0, (24, 8, 1)).astype(np.float32)]
+    mats = sd.materials.copy()
+    mats[1:, 10] = 0
+    mats[1::2, 11] = 1
+    mats[1:, 12] = 2
+    sd.materials = mats
+    g, st, nodes, ordered = _render(gpu, sd, 8, 4, 8)
+    ps = oracle.PreparedScene(sd, nodes, ordered)
+    c, cst = ps.render(spp=8, max_depth=4, spp_batch=8)
+    assert np.all(g[..., 3] == 8) and np.isfinite(g).all()
+    err = rel_rmse(g[..., :3], c[..., :3])
+    exact = np.mean(np.all(g[..., :3] == c[..., :3], axis=-1))
+    print(f"textured materials: rel RMSE {err:.3e}; bit-identical pixels {exact:.4f}")
+    assert err < RMSE_TOL
+
+
+def test_render_normal_mapped_materials_match_oracle(gpu, oracle):
+    """Normal textures over per-vertex tangent frames (get_normal's TBN
+    product) on every material class render as the oracle does."""
+    sd = scenes.spheres_scene(width=96, height=64)
+    rng = np.random.default_rng(9)
+    nm = np.concatenate([rng.uniform(0.3, 0.7, (40, 56, 2)), rng.uniform(0.85, 1.0, (40, 56, 1))], axis=-1)
+    sd.textures = [nm.astype(np.float32)]
+    sd.tangents = scenes.tangent_frames(sd.normals)
+    mats = sd.materials.copy()
+    mats[1:, 13] = 0
+    mats[1::3, 0] = abi.FF_MAT_CONDUCTOR
+    mats[2::3, 0] = abi.FF_MAT_DIELECTRIC
+    sd.materials = mats
+    import torch
+    sc = abi.Scene(gpu, sd.positions, sd.normals, sd.uvs, sd.material_ids, sd.materials, sd.textures,
+                   tangents=sd.tangents)
+    env = abi.EnvMap(gpu, sd.env)
+    film = torch.zeros((sd.height, sd.width, 4), dtype=torch.float32, device="cuda")
+    abi.render(gpu, sc, env, abi.make_camera(sd.inv_vp), sd.width, sd.height, 8, 4, sd.seed, film, spp_batch=8)
+    g = film.cpu().numpy()
+    nodes, ordered = sc.bvh()
+    sc.close()
+    env.close()
+    c, _ = oracle.PreparedScene(sd, nodes, ordered).render(spp=8, max_depth=4, spp_batch=8)
+    assert np.all(g[..., 3] == 8) and np.isfinite(g).all()
+    err = rel_rmse(g[..., :3], c[..., :3])
+    exact = np.mean(np.all(g[..., :3] == c[..., :3], axis=-1))
+    print(f"normal-mapped materials: rel RMSE {err:.3e}; bit-identical pixels {exact:.4f}")
+    assert err < RMSE_TOL
+
+
 @pytest.mark.parametrize("mtype", [abi.FF_MAT_DIFFUSE, abi.FF_MAT_CONDUCTOR, abi.FF_MAT_DIELECTRIC])
 def test_render_material_queues_match_oracle(gpu, oracle, mtype):
     sd = scenes.spheres_scene(width=96, height=64, mtype=abi.FF_MAT_DIFFUSE)

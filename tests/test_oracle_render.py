@@ -52,3 +52,80 @@ def test_depth_one_is_direct_lighting_only(oracle):
     assert s1.rays_extension <= s1.rays_primary
     assert s3.rays_extension > s1.rays_extension
     assert np.isfinite(f1).all() and np.isfinite(f3).all()
+
+
+def test_constant_textures_equal_their_factors(oracle):
+    """A texture of one value in place of a material factor renders the
+    image of that factor (bilinear lookups of a constant give the constant
+    up to rounding): base colour, roughness and metalness each feed the
+    field they stand for."""
+    sd = scenes.spheres_scene(width=40, height=24)
+    alb, rough, metal = (0.3, 0.6, 0.45), 0.35, 0.7
+    sd.textures = [np.tile(np.array(alb, np.float32), (5, 7, 1)),
+                   np.full((4, 3, 1), rough, np.float32),
+                   np.full((6, 2, 1), metal, np.float32)]
+    fac = sd.materials.copy()
+    fac[1:, 1:4] = alb
+    fac[1:, 4] = rough
+    fac[1:, 5] = metal
+    tex = sd.materials.copy()
+    tex[1:, 1:4] = 0.9
+    tex[1:, 4] = 1.0
+    tex[1:, 5] = 0.0
+    tex[1:, 10:13] = (0, 1, 2)
+    films = []
+    for m in (fac, tex):
+        sd.materials = m
+        films.append(oracle.PreparedScene(sd).render(spp=4, max_depth=3, n_threads=4)[0])
+    np.testing.assert_allclose(films[1], films[0], rtol=1e-4, atol=1e-6)
+    # without the metalness texture the image differs: the lookup is not inert
+    tex[1:, 12] = -1
+    sd.materials = tex
+    other = oracle.PreparedScene(sd).render(spp=4, max_depth=3, n_threads=4)[0]
+    assert np.abs(other - films[0]).max() > 1e-2
+
+
+def test_normal_map_equals_the_mapped_shading_normal(oracle):
+    """On faceted triangles (one normal and one tangent frame per face) a
+    constant normal texture m gives the image of shading normals
+    normalize(T m.x + B m.y + N m.z) without a map, up to the light shadow
+    ray's offset, which keeps the interpolated normal; with T and B swapped
+    the image differs, and a texel of (0.5, 0.5, 1) leaves the normal as is."""
+    sd = scenes.spheres_scene(width=40, height=24)
+    e1 = sd.positions[:, 1] - sd.positions[:, 0]
+    e2 = sd.positions[:, 2] - sd.positions[:, 0]
+    fn = np.cross(e1, e2).astype(np.float64)
+    fn /= np.linalg.norm(fn, axis=-1, keepdims=True)
+    sd.normals = np.repeat(fn[:, None], 3, axis=1).astype(np.float32)
+    frames = scenes.tangent_frames(sd.normals)
+    sd.tangents = frames
+    m = np.array([0.35, -0.25, 0.9])
+    m /= np.linalg.norm(m)
+    tex = ((m + 1) / 2).astype(np.float32)
+    sd.textures = [np.tile(tex, (4, 4, 1)), np.tile(np.array([0.5, 0.5, 1.0], np.float32), (3, 5, 1))]
+    mats = sd.materials.copy()
+    mats[:, 13] = 0
+    sd.materials = mats
+
+    def render(s):
+        return oracle.PreparedScene(s).render(spp=4, max_depth=3, n_threads=4)[0][..., :3]
+    mapped = render(sd)
+
+    def baked(t, b):
+        nb = t * m[0] + b * m[1] + fn[:, None] * m[2]
+        nb /= np.linalg.norm(nb, axis=-1, keepdims=True)
+        s = scenes.SceneData(**{**sd.__dict__})
+        s.normals = nb.astype(np.float32)
+        s.materials = sd.materials.copy()
+        s.materials[:, 13] = -1
+        s.tangents = None
+        return render(s)
+    t, b = frames[..., :3].astype(np.float64), frames[..., 3:].astype(np.float64)
+    want = baked(t, b)
+    rmse = lambda a, c: float(np.sqrt(np.mean((a - c) ** 2)) / np.sqrt(np.mean(c ** 2)))
+    assert rmse(mapped, want) < 3e-3
+    assert rmse(baked(b, t), want) > 3e-2
+    sd.materials[:, 13] = 1
+    flat = render(sd)
+    sd.materials[:, 13] = -1
+    np.testing.assert_allclose(flat, render(sd), rtol=1e-4, atol=1e-6)
