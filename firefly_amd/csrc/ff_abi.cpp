@@ -114,8 +114,7 @@ struct ff_scene {
     int32_t* d_leaf_count = nullptr;
     float* d_root = nullptr;
     float* d_tris = nullptr;
-    float* d_nrm = nullptr;
-    float* d_uv = nullptr;
+    float4* d_attr = nullptr;        // per slot: normals, uvs, material word (64 B)
     float* d_tan = nullptr;          // per slot [3][6] tangent frames, or none
     uint32_t* d_mat = nullptr;
     float* d_mats = nullptr;
@@ -126,7 +125,7 @@ struct ff_scene {
     bool has_type[4] = {false, false, false, false};
     ff_dev_scene view() const {
         ff_dev_scene s;
-        s.nodes = d_nodes; s.nodes4 = d_nodes4; s.leaf_count = d_leaf_count; s.root = d_root; s.tris = d_tris; s.nrm = d_nrm; s.uv = d_uv; s.mat = d_mat;
+        s.nodes = d_nodes; s.nodes4 = d_nodes4; s.leaf_count = d_leaf_count; s.root = d_root; s.tris = d_tris; s.attr = d_attr; s.mat = d_mat;
         s.tan = d_tan;
         s.mats = d_mats; s.tex = d_tex_desc; s.n_nodes = (int32_t)nodes.size(); s.n_tris = n_tris;
         s.n_mats = n_mats; s.n_tex = n_tex; s.max_depth = max_depth;
@@ -479,7 +478,7 @@ ff_status ff_scene_create(ff_context* ctx, const ff_scene_desc* d, ff_scene** ou
     size_t n = (size_t)d->n_tris;
     // reorder_primitives (CRT/kernel.cu:184-191,701-712) done on the host while
     // packing the positions-only records v0 | e1 | e2.
-    std::vector<float> tris(9 * n), nrm(9 * n), uv(6 * n, 0.f), tan(d->tangents ? 18 * n : 0);
+    std::vector<float> tris(9 * n), attr(16 * n, 0.f), tan(d->tangents ? 18 * n : 0);
     std::vector<uint32_t> mat(n);
     for (size_t i = 0; i < n; i++) {
         size_t src = (size_t)s->ordered[i];
@@ -490,11 +489,12 @@ ff_status ff_scene_create(ff_context* ctx, const ff_scene_desc* d, ff_scene** ou
             t[3 + k] = p[3 + k] - p[k];
             t[6 + k] = p[6 + k] - p[k];
         }
-        std::memcpy(&nrm[i * 9], d->normals + src * 9, 36);
-        if (d->uvs) std::memcpy(&uv[i * 6], d->uvs + src * 6, 24);
+        std::memcpy(&attr[i * 16], d->normals + src * 9, 36);
+        if (d->uvs) std::memcpy(&attr[i * 16 + 9], d->uvs + src * 6, 24);
         if (d->tangents) std::memcpy(&tan[i * 18], d->tangents + src * 18, 72);
         uint32_t m = (uint32_t)d->material_ids[src];
         mat[i] = m | ((uint32_t)(int)d->materials[(size_t)m * FF_MAT_FLOATS] << MAT_TYPE_SHIFT);
+        std::memcpy(&attr[i * 16 + 15], &mat[i], 4);
     }
     for (int32_t m = 0; m < d->n_materials; m++) s->has_type[(int)d->materials[(size_t)m * FF_MAT_FLOATS]] = true;
     auto cleanup = [&](ff_status st) { ff_scene_destroy(s); return st; };
@@ -513,7 +513,7 @@ ff_status ff_scene_create(ff_context* ctx, const ff_scene_desc* d, ff_scene** ou
     if (dalloc(&s->d_nodes, wide.size()) != hipSuccess || dalloc(&s->d_nodes4, four.size()) != hipSuccess ||
         dalloc(&s->d_leaf_count, leaf_count.size()) != hipSuccess ||
         dalloc(&s->d_root, 8) != hipSuccess || dalloc(&s->d_tris, 9 * n) != hipSuccess ||
-        dalloc(&s->d_nrm, 9 * n) != hipSuccess || dalloc(&s->d_uv, 6 * n) != hipSuccess ||
+        dalloc(&s->d_attr, 4 * n) != hipSuccess ||
         dalloc(&s->d_mat, n) != hipSuccess || dalloc(&s->d_mats, (size_t)FF_MAT_FLOATS * d->n_materials) != hipSuccess ||
         (d->tangents && dalloc(&s->d_tan, 18 * n) != hipSuccess))
         return cleanup(fail(FF_ERR_ALLOC, "ff_scene_create: device allocation failed"));
@@ -525,8 +525,7 @@ ff_status ff_scene_create(ff_context* ctx, const ff_scene_desc* d, ff_scene** ou
         e = hipMemcpy(s->d_leaf_count, leaf_count.data(), leaf_count.size() * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->d_root, root, sizeof(root), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->d_tris, tris.data(), 36 * n, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(s->d_nrm, nrm.data(), 36 * n, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(s->d_uv, uv.data(), 24 * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->d_attr, attr.data(), 64 * n, hipMemcpyHostToDevice);
     if (e == hipSuccess && d->tangents) e = hipMemcpy(s->d_tan, tan.data(), 72 * n, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->d_mat, mat.data(), 4 * n, hipMemcpyHostToDevice);
     if (e == hipSuccess)
@@ -556,7 +555,7 @@ ff_status ff_scene_create(ff_context* ctx, const ff_scene_desc* d, ff_scene** ou
 ff_status ff_scene_destroy(ff_scene* s) {
     if (!s) return fail(FF_ERR_INVALID, "ff_scene_destroy: NULL scene");
     (void)hipDeviceSynchronize();
-    dfree(s->d_nodes); dfree(s->d_nodes4); dfree(s->d_leaf_count); dfree(s->d_root); dfree(s->d_tris); dfree(s->d_nrm); dfree(s->d_uv); dfree(s->d_tan); dfree(s->d_mat); dfree(s->d_mats);
+    dfree(s->d_nodes); dfree(s->d_nodes4); dfree(s->d_leaf_count); dfree(s->d_root); dfree(s->d_tris); dfree(s->d_attr); dfree(s->d_tan); dfree(s->d_mat); dfree(s->d_mats);
     for (auto& p : s->d_tex) dfree(p);
     dfree(s->d_tex_desc);
     delete s;

@@ -664,14 +664,16 @@ __device__ __forceinline__ void shade_core(const ff_paths& P, uint32_t id, Shade
     f3 o = xyz(r0), d = xyz(r1);
     uint32_t base = P.rngb[id];
     f3 Pp = vadd(o, vscl(d, hh.x));
-    const float* N = S.nrm + (size_t)slot * 9;
+    // the slot's 64-B shading record: normals n0 n1 n2, uvs t0 t1 t2, material word
+    const float4* A = S.attr + (size_t)slot * 4;
+    float4 a0 = ldg4(A), a1 = ldg4(A + 1), a2 = ldg4(A + 2), a3 = ldg4(A + 3);
     float w0 = (1.f - hh.y) - hh.z;
-    f3 nrm = vnorm(vadd(vadd(vscl(ld3(N + 3), hh.y), vscl(ld3(N + 6), hh.z)), vscl(ld3(N), w0)));
-    const float* U = S.uv + (size_t)slot * 6;
-    float tu = (ldg(U + 2) * hh.y + ldg(U + 4) * hh.z) + ldg(U) * w0;
-    float tv = (ldg(U + 3) * hh.y + ldg(U + 5) * hh.z) + ldg(U + 1) * w0;
+    f3 nrm = vnorm(vadd(vadd(vscl(V3(a0.w, a1.x, a1.y), hh.y), vscl(V3(a1.z, a1.w, a2.x), hh.z)),
+                        vscl(V3(a0.x, a0.y, a0.z), w0)));
+    float tu = (a2.w * hh.y + a3.y * hh.z) + a2.y * w0;
+    float tv = (a3.x * hh.y + a3.z * hh.z) + a2.z * w0;
     // the 64-B material record (64-B aligned) as four 16-B loads
-    const float4* M = (const float4*)(S.mats + (size_t)(ldg(S.mat + slot) & MAT_INDEX_MASK) * FF_MAT_FLOATS_DEV);
+    const float4* M = (const float4*)(S.mats + (size_t)(__float_as_uint(a3.w) & MAT_INDEX_MASK) * FF_MAT_FLOATS_DEV);
     float4 m0 = ldg4(M), m1 = ldg4(M + 1), m2 = ldg4(M + 2), m3 = ldg4(M + 3);
     f3 alb = V3(m0.y, m0.z, m0.w);
     float rough = m1.x, metal = m1.y;

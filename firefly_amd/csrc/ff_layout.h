@@ -93,8 +93,7 @@ struct ff_dev_scene {
     const int32_t* leaf_count; // per slot, for leaves of 15 or more primitives
     const float* root;         // root box lo.xyz hi.xyz, root ref
     const float* tris;
-    const float* nrm;
-    const float* uv;
+    const float4* attr;        // per slot 4 x float4: normals [0..8], uvs [9..14], material word [15]
     const float* tan;          // per slot [3][6] tangent, bitangent (null: no normal textures)
     const uint32_t* mat;       // per slot: material index | material type << MAT_TYPE_SHIFT
     const float* mats;
