@@ -145,26 +145,47 @@ EnvironmentLight::EnvironmentLight(int w, int h, std::vector<float> data) : widt
         throw Error(FF_ERR_INVALID, "EnvironmentLight: texel count does not match the size");
 }
 
-// Radiance RGBE reader: header, "-Y h +X w" resolution line, new-style RLE
-// or flat scanlines; value = mantissa * 2^(exponent - 136).
+// Radiance RGBE reader (the format of hrdi/*.hdr, which the reference loads
+// through stb_image, CRT/dTexture.cu:189-200): a "#?RADIANCE" (or "#?RGBE")
+// first line, header lines up to an empty line with
+// FORMAT=32-bit_rle_rgbe, the "-Y h +X w" resolution line, then h scanlines,
+// each either new-style run-length encoded (bytes 2, 2, w >> 8, w & 255,
+// then the four byte planes one after another, each as runs "128 + n, value"
+// and dumps "n, n bytes") or flat (w four-byte pixels; always flat when
+// w < 8 or w >= 32768).  Texel = mantissa * 2^(exponent - 136), zero for a
+// zero exponent.  Rows are stored bottom first, as the reference's loader
+// flips them (stbi_set_flip_vertically_on_load(true)).  Every byte read is
+// checked: a truncated file, a run or dump past the end of its scanline, a
+// zero-length dump or an image above 2^28 texels is an error.
 EnvironmentLight::EnvironmentLight(const std::string& path) : filepath(path) {
     std::ifstream f(path, std::ios::binary);
     if (!f) throw Error(FF_ERR_INVALID, "EnvironmentLight: cannot open " + path);
     std::string line;
-    bool fmt = false;
+    if (!std::getline(f, line) || (line.rfind("#?RADIANCE", 0) != 0 && line.rfind("#?RGBE", 0) != 0))
+        throw Error(FF_ERR_INVALID, "EnvironmentLight: not a Radiance file: " + path);
+    bool fmt = false, ended = false;
     while (std::getline(f, line)) {
-        if (line.empty()) break;
+        if (line.empty()) {
+            ended = true;
+            break;
+        }
         if (line.rfind("FORMAT=32-bit_rle_rgbe", 0) == 0) fmt = true;
     }
-    if (!fmt) throw Error(FF_ERR_INVALID, "EnvironmentLight: not a 32-bit_rle_rgbe file: " + path);
-    std::getline(f, line);
-    int w = 0, h = 0;
-    if (std::sscanf(line.c_str(), "-Y %d +X %d", &h, &w) != 2 || w <= 0 || h <= 0)
+    if (!fmt || !ended) throw Error(FF_ERR_INVALID, "EnvironmentLight: not a 32-bit_rle_rgbe file: " + path);
+    if (!std::getline(f, line)) throw Error(FF_ERR_INVALID, "EnvironmentLight: no resolution line: " + path);
+    int w = 0, h = 0, used = 0;
+    if (std::sscanf(line.c_str(), "-Y %d +X %d%n", &h, &w, &used) != 2 || (size_t)used != line.size() || w <= 0 ||
+        h <= 0 || w > 65536 || h > 65536 || (size_t)w * (size_t)h > ((size_t)1 << 28))
         throw Error(FF_ERR_INVALID, "EnvironmentLight: unsupported resolution line: " + line);
     width = w;
     height = h;
     rgb.assign((size_t)w * h * 3, 0.f);
     std::vector<unsigned char> scan((size_t)w * 4);
+    auto byte = [&f]() -> unsigned char {
+        int c = f.get();
+        if (c == EOF) throw Error(FF_ERR_INVALID, "EnvironmentLight: truncated file");
+        return (unsigned char)c;
+    };
     for (int y = 0; y < h; y++) {
         unsigned char hd[4];
         if (!f.read((char*)hd, 4)) throw Error(FF_ERR_INVALID, "EnvironmentLight: truncated file");
@@ -172,22 +193,22 @@ EnvironmentLight::EnvironmentLight(const std::string& path) : filepath(path) {
             for (int ch = 0; ch < 4; ch++) {
                 int x = 0;
                 while (x < w) {
-                    int c = f.get();
-                    if (c == EOF) throw Error(FF_ERR_INVALID, "EnvironmentLight: truncated RLE");
+                    int c = byte();
                     if (c > 128) {
-                        int cnt = c - 128, v = f.get();
+                        int cnt = c - 128;
                         if (x + cnt > w) throw Error(FF_ERR_INVALID, "EnvironmentLight: bad RLE run");
-                        for (int k = 0; k < cnt; k++) scan[(size_t)(x++) * 4 + ch] = (unsigned char)v;
+                        unsigned char v = byte();
+                        for (int k = 0; k < cnt; k++) scan[(size_t)(x++) * 4 + ch] = v;
                     } else {
                         if (c == 0 || x + c > w) throw Error(FF_ERR_INVALID, "EnvironmentLight: bad RLE dump");
-                        for (int k = 0; k < c; k++) scan[(size_t)(x++) * 4 + ch] = (unsigned char)f.get();
+                        for (int k = 0; k < c; k++) scan[(size_t)(x++) * 4 + ch] = byte();
                     }
                 }
             }
         } else {
             std::memcpy(scan.data(), hd, 4);
             if (!f.read((char*)scan.data() + 4, (std::streamsize)w * 4 - 4))
-                throw Error(FF_ERR_INVALID, "EnvironmentLight: truncated scanline");
+                throw Error(FF_ERR_INVALID, "EnvironmentLight: truncated file");
         }
         int row = h - 1 - y;  // bottom-up, as loaded with vertical flip
         for (int x = 0; x < w; x++) {

@@ -60,7 +60,30 @@ struct JVal {
         const JVal* v = get(key);
         return v && v->kind == NUM ? v->num : d;
     }
-    int int_or(const char* key, int d) const { return (int)num_or(key, d); }
+    // This value as a non-negative integer no larger than `limit` (< 2^53):
+    // fractions, negatives, non-numbers and non-finite values are rejected
+    // before any conversion, so no cast below is undefined.
+    size_t as_uint(size_t limit, const char* what) const {
+        if (kind != NUM || !(num >= 0.0) || num > (double)limit || num != std::floor(num))
+            throw Error(FF_ERR_INVALID, std::string("glTF: ") + what + " is not an integer in [0, " +
+                                            std::to_string(limit) + "]");
+        return (size_t)num;
+    }
+    // field `key` as a non-negative integer <= limit, `d` when absent
+    size_t uint_or(const char* key, size_t d, size_t limit) const {
+        const JVal* v = get(key);
+        return v ? v->as_uint(limit, key) : d;
+    }
+    // field `key` as a 32-bit signed integer, `d` when absent
+    int int_or(const char* key, int d) const {
+        const JVal* v = get(key);
+        if (!v) return d;
+        if (v->kind != NUM || !(v->num >= -2147483648.0 && v->num <= 2147483647.0) || v->num != std::floor(v->num))
+            throw Error(FF_ERR_INVALID, std::string("glTF: ") + key + " is not a 32-bit integer");
+        return (int)v->num;
+    }
+    // this value as an index into a top-level array (checked again by at())
+    int index(const char* what) const { return (int)as_uint(2147483647u, what); }
     const JVal& at(size_t i) const {
         if (kind != ARR || i >= arr.size()) throw Error(FF_ERR_INVALID, "glTF: index out of range");
         return arr[i];
@@ -296,21 +319,31 @@ Acc read_accessor(const Doc& d, int idx) {
     if (!type || type->kind != JVal::STR) throw Error(FF_ERR_INVALID, "glTF: accessor without type");
     r.comps = comps_of(type->str);
     r.ct = a.int_or("componentType", 0);
-    r.count = (size_t)a.num_or("count", 0);
     int es = bytes_of(r.ct);
     bool norm = a.get("normalized") && a.get("normalized")->b;
-    r.v.assign(r.count * (size_t)r.comps, 0.0);
     const JVal* bvi = a.get("bufferView");
-    if (!bvi) return r;  // all zeros
-    const JVal& bv = d.list("bufferViews").at((size_t)bvi->num);
+    const std::string outside = "glTF: accessor " + std::to_string(idx) + " outside its buffer";
+    if (!bvi) {  // all zeros; bounded so that a count alone cannot exhaust memory
+        r.count = a.uint_or("count", 0, (size_t)1 << 24);
+        r.v.assign(r.count * (size_t)r.comps, 0.0);
+        return r;
+    }
+    const size_t nb = d.bin.size();
+    const JVal& bv = d.list("bufferViews").at((size_t)bvi->index("bufferView"));
     if (bv.int_or("buffer", 0) != 0) throw Error(FF_ERR_UNSUPPORTED, "glTF: only the GLB binary buffer is read");
-    size_t vo = (size_t)bv.num_or("byteOffset", 0), vl = (size_t)bv.num_or("byteLength", 0);
-    size_t stride = (size_t)bv.num_or("byteStride", 0);
+    // every offset and length is at most the binary chunk's size, and each
+    // check is written as a subtraction of quantities already known to be in
+    // range, so nothing can wrap
+    size_t vo = bv.uint_or("byteOffset", 0, nb), vl = bv.uint_or("byteLength", 0, nb);
+    if (vl > nb - vo) throw Error(FF_ERR_INVALID, outside);
+    size_t stride = bv.uint_or("byteStride", 0, 255);
     size_t elem = (size_t)es * (size_t)r.comps;
     if (stride == 0) stride = elem;
-    size_t ao = (size_t)a.num_or("byteOffset", 0);
-    if (vo + vl > d.bin.size() || (r.count && ao + stride * (r.count - 1) + elem > vl))
-        throw Error(FF_ERR_INVALID, "glTF: accessor " + std::to_string(idx) + " outside its buffer");
+    size_t ao = a.uint_or("byteOffset", 0, nb);
+    r.count = a.uint_or("count", 0, nb);
+    if (r.count && (ao > vl || elem > vl - ao || r.count - 1 > (vl - ao - elem) / stride))
+        throw Error(FF_ERR_INVALID, outside);
+    r.v.assign(r.count * (size_t)r.comps, 0.0);  // count <= the chunk's size, comps <= 16
     const uint8_t* base = d.bin.data() + vo + ao;
     for (size_t i = 0; i < r.count; i++)
         for (int c = 0; c < r.comps; c++) {
@@ -383,25 +416,27 @@ struct Loader {
         if (!info) return -1;
         const JVal* ti = info->get("index");
         if (!ti) return -1;
-        const JVal& tj = d.list("textures").at((size_t)ti->num);
+        const JVal& tj = d.list("textures").at((size_t)ti->index("texture index"));
         const JVal* src = tj.get("source");
         if (!src) throw Error(FF_ERR_UNSUPPORTED, "glTF: texture without a source image");
-        long key = (long)src->num * 3 + (channel < 0 ? 0 : channel);
+        const int src_i = src->index("texture source");
+        long key = (long)src_i * 3 + (channel < 0 ? 0 : channel);
         for (const auto& kv : tex_done)
             if (kv.first == key) return kv.second;
-        const JVal& im = d.list("images").at((size_t)src->num);
+        const JVal& im = d.list("images").at((size_t)src_i);
         const JVal* bvi = im.get("bufferView");
         const JVal* mime = im.get("mimeType");
         if (!bvi || (mime && mime->str != "image/png"))
             throw Error(FF_ERR_UNSUPPORTED, "glTF: only PNG images embedded in the binary chunk are read");
-        const JVal& bv = d.list("bufferViews").at((size_t)bvi->num);
-        size_t off = (size_t)bv.num_or("byteOffset", 0), len = (size_t)bv.num_or("byteLength", 0);
-        if (off + len > d.bin.size()) throw Error(FF_ERR_INVALID, "glTF: image outside the binary chunk");
+        const JVal& bv = d.list("bufferViews").at((size_t)bvi->index("bufferView"));
+        const size_t nb = d.bin.size();
+        size_t off = bv.uint_or("byteOffset", 0, nb), len = bv.uint_or("byteLength", 0, nb);
+        if (len > nb - off) throw Error(FF_ERR_INVALID, "glTF: image outside the binary chunk");
         int w = 0, h = 0, c = 0;
         std::vector<uint8_t> px;
         decode_png(d.bin.data() + off, len, w, h, c, px);
         Texture t;
-        t.filepath = "image" + std::to_string((long)src->num);
+        t.filepath = "image" + std::to_string(src_i);
         t.width = w;
         t.height = h;
         t.nrComponents = channel < 0 ? 3 : 1;
@@ -473,7 +508,7 @@ struct Loader {
             const JVal* at = p.get("attributes");
             const JVal* pos_i = at ? at->get("POSITION") : nullptr;
             if (!pos_i) throw Error(FF_ERR_INVALID, "glTF: primitive without POSITION");
-            Acc P = read_accessor(d, (int)pos_i->num);
+            Acc P = read_accessor(d, pos_i->index("POSITION"));
             if (P.comps != 3 || P.ct != 5126) throw Error(FF_ERR_INVALID, "glTF: POSITION must be float VEC3");
             Mesh me;
             const JVal* nm = mj.get("name");
@@ -488,7 +523,7 @@ struct Loader {
                                        (float)(((m[2] * x + m[6] * y) + m[10] * z) + m[14]));
             }
             if (const JVal* ii = p.get("indices")) {
-                Acc I = read_accessor(d, (int)ii->num);
+                Acc I = read_accessor(d, ii->index("indices"));
                 if (I.comps != 1 || (I.ct != 5121 && I.ct != 5123 && I.ct != 5125))
                     throw Error(FF_ERR_INVALID, "glTF: indices must be unsigned scalars");
                 me.indices.resize(I.count);
@@ -502,7 +537,7 @@ struct Loader {
             }
             me.indices.resize(me.indices.size() / 3 * 3);
             if (const JVal* ni = at->get("NORMAL")) {
-                Acc Nn = read_accessor(d, (int)ni->num);
+                Acc Nn = read_accessor(d, ni->index("NORMAL"));
                 if (Nn.comps != 3 || Nn.count != nv) throw Error(FF_ERR_INVALID, "glTF: NORMAL must be VEC3 per vertex");
                 me.normals.resize(nv);
                 for (size_t v = 0; v < nv; v++) {
@@ -536,7 +571,7 @@ struct Loader {
                 }
             }
             if (const JVal* ti = at->get("TEXCOORD_0")) {
-                Acc T = read_accessor(d, (int)ti->num);
+                Acc T = read_accessor(d, ti->index("TEXCOORD_0"));
                 if (T.comps != 2 || T.count != nv) throw Error(FF_ERR_INVALID, "glTF: TEXCOORD_0 must be VEC2 per vertex");
                 me.texcoords.resize(nv);
                 for (size_t v = 0; v < nv; v++) {
@@ -550,8 +585,8 @@ struct Loader {
                 // TANGENT xyz and bitangent cross(NORMAL, xyz) * w (the glTF rule), both
                 // in mesh space, then placed as the reference places them:
                 // normalize(mat * vec4(v, 1)), translation included (CRT/Scene.cpp:143-153)
-                Acc T = read_accessor(d, (int)tgi->num);
-                Acc Nn = read_accessor(d, (int)nmi->num);
+                Acc T = read_accessor(d, tgi->index("TANGENT"));
+                Acc Nn = read_accessor(d, nmi->index("NORMAL"));
                 if (T.comps != 4 || T.count != nv) throw Error(FF_ERR_INVALID, "glTF: TANGENT must be VEC4 per vertex");
                 auto place = [&](double x, double y, double z) {
                     double X = ((m[0] * x + m[4] * y) + m[8] * z) + m[12], Y = ((m[1] * x + m[5] * y) + m[9] * z) + m[13],
@@ -622,9 +657,9 @@ struct Loader {
         if (depth > 256) throw Error(FF_ERR_INVALID, "glTF: node hierarchy too deep (a cycle?)");
         const JVal& nj = d.list("nodes").at((size_t)ni);
         M4 world = mul(parent, local_of(nj));
-        if (const JVal* mi = nj.get("mesh")) mesh((int)mi->num, world);
+        if (const JVal* mi = nj.get("mesh")) mesh(mi->index("mesh"), world);
         if (const JVal* ch = nj.get("children"))
-            for (const JVal& c : ch->arr) node((int)c.num, world, depth + 1);
+            for (const JVal& c : ch->arr) node(c.index("child node"), world, depth + 1);
     }
 };
 
@@ -659,7 +694,7 @@ int Scene::load(const std::string& path) {
     if (scenes && scenes->kind == JVal::ARR && !scenes->arr.empty()) {
         const JVal& sc = scenes->at((size_t)d.j.int_or("scene", 0));
         if (const JVal* roots = sc.get("nodes"))
-            for (const JVal& r : roots->arr) L.node((int)r.num, M4{}, 0);
+            for (const JVal& r : roots->arr) L.node(r.index("root node"), M4{}, 0);
     } else if (const JVal* meshes = d.j.get("meshes")) {  // no scene: every mesh untransformed
         for (size_t k = 0; k < meshes->arr.size(); k++) L.mesh((int)k, M4{});
     }

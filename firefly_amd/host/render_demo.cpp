@@ -4,6 +4,7 @@
 //
 // usage: render_demo <scene.ffs> <out.pfm|out.png> <spp> <max_path_length> [samples_per_draw] [model.glb]
 //        render_demo --dump-glb <model.glb> <out.bin>   (no GPU: the loaded meshes and materials)
+//        render_demo --dump-hdr <env.hdr> <out.bin>     (no GPU: the environment map as loaded)
 //
 // A model.glb argument adds that file's meshes (Scene::load) to the scene's.
 // The dump: int32 n_meshes; per mesh int32 name length, name, int32 material
@@ -14,6 +15,8 @@
 // colour, roughness, metallic and normal texture (-1 none);
 // then int32 n_textures; per texture int32 width, height, channels and
 // float texels[height][width][channels] (bottom row first).
+// The environment dump: int32 width, height, float texels[height][width][3]
+// (bottom row first, as EnvironmentLight(path) stores them).
 //
 // Scene file (little endian): "FFS1", int32 n_meshes, n_materials, env_w,
 // env_h, width, height, uint32 seed, float eye[3], target[3], yfov_deg,
@@ -89,12 +92,24 @@ int dump_glb(const char* glb, const char* out) {
     return 0;
 }
 
+int dump_hdr(const char* hdr, const char* out) {
+    EnvironmentLight env{std::string(hdr)};
+    FILE* o = std::fopen(out, "wb");
+    if (!o) throw Error(FF_ERR_INVALID, std::string("cannot write ") + out);
+    int32_t wh[2] = {env.get_tex_width(), env.get_tex_height()};
+    std::fwrite(wh, 4, 2, o);
+    std::fwrite(env.get_texels().data(), 4, env.get_texels().size(), o);
+    std::fclose(o);
+    std::printf("loaded %s: %dx%d\n", hdr, wh[0], wh[1]);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
-    if (argc == 4 && std::strcmp(argv[1], "--dump-glb") == 0) {
+    if (argc == 4 && (std::strcmp(argv[1], "--dump-glb") == 0 || std::strcmp(argv[1], "--dump-hdr") == 0)) {
         try {
-            return dump_glb(argv[2], argv[3]);
+            return argv[1][7] == 'g' ? dump_glb(argv[2], argv[3]) : dump_hdr(argv[2], argv[3]);
         } catch (const Error& e) {
             std::fprintf(stderr, "render_demo: %s (status %d)\n", e.what(), (int)e.status);
             return 1;

@@ -321,6 +321,58 @@ def test_malformed_files_are_rejected(tmp_path, name):
     assert r.returncode == 1 and "render_demo:" in r.stderr, (r.returncode, r.stderr)
 
 
+_HUGE_CASES = {
+    # count * comps and stride * (count - 1) wrap a 64-bit size for these
+    "count_2^62_vec4": lambda g, d: g.accs[0].update(count=2 ** 62, type="VEC4"),
+    "count_2^64-1": lambda g, d: g.accs[0].update(count=2 ** 64 - 1),
+    "accessor_offset_near_2^64": lambda g, d: g.accs[0].update(byteOffset=2 ** 64 - 8),
+    "view_offset_near_2^64": lambda g, d: g.views[0].update(byteOffset=2 ** 64 - 16),
+    "view_length_near_2^64": lambda g, d: g.views[0].update(byteLength=2 ** 64 - 4),
+    "view_offset_plus_length_wraps": lambda g, d: g.views[0].update(byteOffset=2 ** 63, byteLength=2 ** 63),
+    "negative_count": lambda g, d: g.accs[0].update(count=-3),
+    "negative_offset": lambda g, d: g.accs[0].update(byteOffset=-36),
+    "fractional_offset": lambda g, d: g.accs[0].update(byteOffset=0.5),
+    "string_count": lambda g, d: g.accs[0].update(count="3"),
+    "huge_stride": lambda g, d: g.views[0].update(byteStride=2 ** 40),
+    "negative_accessor_index": lambda g, d: d["meshes"][0]["primitives"][0]["attributes"].update(POSITION=-1),
+    "huge_node_index": lambda g, d: d["scenes"][0].update(nodes=[2 ** 63]),
+    "zero_count_huge_without_view": lambda g, d: _drop_view(g.accs[0], 2 ** 40),
+}
+
+
+def _drop_view(acc, count):
+    del acc["bufferView"]
+    acc["count"] = count
+
+
+@pytest.mark.parametrize("name", sorted(_HUGE_CASES))
+def test_out_of_range_sizes_and_offsets_are_rejected(tmp_path, name):
+    """Sizes, offsets and indices are integers in range before any cast, and
+    the bounds checks are written so that no sum or product can wrap: each
+    case here once wrapped (or was undefined) and passed the old checks."""
+    path = _bad(tmp_path, name, _HUGE_CASES[name])
+    r = _run("--dump-glb", path, str(tmp_path / "o.bin"))
+    assert r.returncode == 1 and "render_demo:" in r.stderr, (r.returncode, r.stderr)
+
+
+def test_image_view_outside_the_chunk_is_rejected(tmp_path):
+    """An image's bufferView whose offset + length wraps past 2^64."""
+    img = np.zeros((2, 2, 3), np.uint8)
+    path = str(tmp_path / "img.glb")
+    g = Glb()
+    p = g.acc(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), 5126, "VEC3")
+    uvs = g.acc(np.zeros((3, 2), np.float32), 5126, "VEC2")
+    g._view(png_bytes(img, 2, [0, 0], "stored"))
+    g.views[-1].update(byteOffset=2 ** 64 - 8, byteLength=64)
+    doc = {"scenes": [{"nodes": [0]}], "nodes": [{"mesh": 0}],
+           "meshes": [{"primitives": [{"attributes": {"POSITION": p, "TEXCOORD_0": uvs}, "material": 0}]}],
+           "materials": [{"pbrMetallicRoughness": {"baseColorTexture": {"index": 0}}}],
+           "textures": [{"source": 0}], "images": [{"bufferView": len(g.views) - 1, "mimeType": "image/png"}]}
+    g.write(path, doc)
+    r = _run("--dump-glb", path, str(tmp_path / "o.bin"))
+    assert r.returncode == 1 and ("image outside" in r.stderr or "not an integer" in r.stderr), r.stderr
+
+
 # ---- embedded PNG textures ----
 def png_bytes(img, ctype, filters, zmode, depth=8, palette=None, trns=None, interlace=0, bad_adler=False):
     """PNG writer with a chosen filter per row and deflate mode ('stored',
