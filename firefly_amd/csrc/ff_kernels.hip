@@ -1290,51 +1290,33 @@ __global__ __launch_bounds__(FF_BLOCK) void k_env_rows(const float* lum, int32_t
     if (threadIdx.x == 0) row_sum[j] = s;
 }
 
-// single workgroup: denom, marginal_p, marginal_y (blocked inclusive scan)
-__global__ __launch_bounds__(FF_BLOCK) void k_env_marginal(const float* lum, const double* row_sum, int32_t w,
-                                                           int32_t h, double* denom_out, double* marg_p,
-                                                           double* marg_y) {
+// single workgroup: denom = sum of the row sums (h values)
+__global__ __launch_bounds__(FF_BLOCK) void k_env_denom(const double* row_sum, int32_t h, double* denom_out) {
     __shared__ double red[FF_BLOCK];
     double acc = 0.0;
     for (int j = threadIdx.x; j < h; j += FF_BLOCK) acc += row_sum[j];
     double denom = block_sum(acc, red);
     if (threadIdx.x == 0) *denom_out = denom;
-    // marginal_p[j] = sum_i lum * (sin_theta_j / denom): thread-per-row, serial in i
-    for (int j = threadIdx.x; j < h; j += FF_BLOCK) {
-        float v = (float)j / (float)h;
-        double st = (double)f_sin(F_PI * v) / denom;
-        double s = 0.0;
-        for (int i = 0; i < w; i++) s += (double)lum[(size_t)j * w + i] * st;
-        marg_p[j] = s;
-    }
-    __syncthreads();
-    // blocked scan: thread t owns rows [t*c, (t+1)*c)
-    int c = (h + FF_BLOCK - 1) / FF_BLOCK;
-    int lo = threadIdx.x * c, hi = lo + c < h ? lo + c : h;
-    double local = 0.0;
-    for (int j = lo; j < hi; j++) local += marg_p[j];
-    red[threadIdx.x] = local;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double run = 0.0;
-        for (int t = 0; t < FF_BLOCK; t++) { double x = red[t]; red[t] = run; run += x; }
-    }
-    __syncthreads();
-    double run = red[threadIdx.x];
-    for (int j = lo; j < hi; j++) { run += marg_p[j]; marg_y[j] = run; }
 }
 
-// one workgroup per row: conditional CDF (blocked scan) and pdf texels
+// one workgroup per row: marginal_p[y] = sum_x lum * (sin_theta_y / denom)
+// (g_compute_marginal_dist's term), then the row's conditional CDF (blocked
+// scan of lum * sin_theta_y / (denom * marginal_p[y])) and its pdf texels
 __global__ __launch_bounds__(FF_BLOCK) void k_env_conditional(const float* lum, int32_t w, int32_t h,
-                                                              const double* denom_p, const double* marg_p,
+                                                              const double* denom_p, double* marg_p,
                                                               double* conds, float* pdf) {
     __shared__ double red[FF_BLOCK];
     int y = blockIdx.x;
     double denom = *denom_p;
     float v = (float)y / (float)h;
     double st = (double)f_sin(F_PI * v);
-    double val = st / (denom * marg_p[y]);
     const float* row = lum + (size_t)y * w;
+    double stn = st / denom;
+    double mp = 0.0;
+    for (int x = threadIdx.x; x < w; x += FF_BLOCK) mp += (double)row[x] * stn;
+    mp = block_sum(mp, red);
+    if (threadIdx.x == 0) marg_p[y] = mp;
+    double val = st / (denom * mp);
     double* crow = conds + (size_t)y * w;
     int c = (w + FF_BLOCK - 1) / FF_BLOCK;
     int lo = threadIdx.x * c, hi = lo + c < w ? lo + c : w;
@@ -1355,6 +1337,25 @@ __global__ __launch_bounds__(FF_BLOCK) void k_env_conditional(const float* lum, 
         p *= scale;
         pdf[(size_t)y * w + x] = p;
     }
+}
+
+// single workgroup: marginal_y = inclusive scan of marginal_p (thread t
+// owns rows [t*c, (t+1)*c), block offsets scanned in order)
+__global__ __launch_bounds__(FF_BLOCK) void k_env_marginal_scan(const double* marg_p, int32_t h, double* marg_y) {
+    __shared__ double red[FF_BLOCK];
+    int c = (h + FF_BLOCK - 1) / FF_BLOCK;
+    int lo = threadIdx.x * c, hi = lo + c < h ? lo + c : h;
+    double local = 0.0;
+    for (int j = lo; j < hi; j++) local += marg_p[j];
+    red[threadIdx.x] = local;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double run = 0.0;
+        for (int t = 0; t < FF_BLOCK; t++) { double x = red[t]; red[t] = run; run += x; }
+    }
+    __syncthreads();
+    double run = red[threadIdx.x];
+    for (int j = lo; j < hi; j++) { run += marg_p[j]; marg_y[j] = run; }
 }
 
 // ============================================================ film resolve
@@ -1663,8 +1664,9 @@ hipError_t launch_env_build(const float* rgb, int w, int h, float* lum, double* 
     hipLaunchKernelGGL(k_env_lum, dim3(grid_for((uint64_t)w * h, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, rgb, w, h,
                        lum);
     hipLaunchKernelGGL(k_env_rows, dim3(h), dim3(FF_BLOCK), 0, s, lum, w, h, row_sum);
-    hipLaunchKernelGGL(k_env_marginal, dim3(1), dim3(FF_BLOCK), 0, s, lum, row_sum, w, h, denom, marg_p, marg_y);
+    hipLaunchKernelGGL(k_env_denom, dim3(1), dim3(FF_BLOCK), 0, s, row_sum, h, denom);
     hipLaunchKernelGGL(k_env_conditional, dim3(h), dim3(FF_BLOCK), 0, s, lum, w, h, denom, marg_p, conds, pdf);
+    hipLaunchKernelGGL(k_env_marginal_scan, dim3(1), dim3(FF_BLOCK), 0, s, marg_p, h, marg_y);
     hipLaunchKernelGGL(k_env_guide, dim3(h + 1), dim3(FF_BLOCK), 0, s, marg_y, conds, w, h, guide_m, guide_c);
     return hipGetLastError();
 }

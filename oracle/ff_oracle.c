@@ -1166,6 +1166,9 @@ static inline void concentric(float u0, float u1, float *dx, float *dy) {
     *dx = r * f_cos(theta);
     *dy = r * f_sin(theta);
 }
+void or_concentric(const float *u, int32_t n, float *out) {
+    for (int32_t i = 0; i < n; i++) concentric(u[2 * i], u[2 * i + 1], &out[2 * i], &out[2 * i + 1]);
+}
 /* dCamera::gen_ray (CRT/dCamera.cu:10-36); AA jitter and lens draw from the
  * counter RNG at bounce 0. */
 void or_gen_ray(const or_scene *s, int32_t width, int32_t height, int32_t px, int32_t py,

@@ -69,6 +69,7 @@ def lib() -> C.CDLL:
             "or_bilinear": (None, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float, _P]),
             "or_env_pdf": (C.c_float, [_P, C.c_int32, C.c_int32, _P]),
             "or_env_L": (None, [_P, C.c_int32, C.c_int32, _P, _P]),
+            "or_concentric": (None, [_P, C.c_int32, _P]),
             "or_gen_ray": (None, [C.POINTER(OrScene), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                   C.c_uint32, _P]),
             "or_ggx_ndf": (C.c_double, [_P, _P, C.c_float]),
@@ -216,6 +217,15 @@ def env_build(rgb):
 def upper_bound(lst, val) -> int:
     a = np.ascontiguousarray(lst, dtype=np.float64)
     return lib().or_upper_bound(a.ctypes.data, len(a), float(val))
+
+
+def concentric(u):
+    """ConcentricSampleDisk of the points u [n][2] as the oracle (and the
+    GPU) compute it: [n][2]."""
+    a = f32(u).reshape(-1, 2)
+    out = np.zeros_like(a)
+    lib().or_concentric(a.ctypes.data, a.shape[0], out.ctypes.data)
+    return out
 
 
 def bilinear(tex, u, v):

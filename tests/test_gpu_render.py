@@ -69,6 +69,23 @@ def test_render_spheres_matches_oracle(gpu, oracle):
     assert abs(st.rays_shadow - cst.rays_shadow) <= 1e-3 * cst.rays_shadow
 
 
+def test_thin_lens_render_matches_oracle(gpu, oracle):
+    """The thin-lens branch of dCamera::gen_ray (CRT/dCamera.cu:28-34,
+    ConcentricSampleDisk CRT/dRandom.cu:104-126): a lens of radius 0.08
+    focused at 3.5 (spheres near the focus sharp, the ground blurred)."""
+    import dataclasses
+    sd = dataclasses.replace(scenes.spheres_scene(width=128, height=96), lens_radius=0.08, focal=3.5)
+    g, st, nodes, ordered = _render(gpu, sd, 16, 4, 16)
+    c, _ = oracle.PreparedScene(sd, nodes, ordered).render(spp=16, max_depth=4, spp_batch=16)
+    assert np.all(g[..., 3] == 16) and np.isfinite(g).all()
+    err = rel_rmse(g[..., :3], c[..., :3])
+    exact = np.mean(np.all(g[..., :3] == c[..., :3], axis=-1))
+    print(f"thin lens 128x96 16spp: rel RMSE {err:.3e}; bit-identical pixels {exact:.4f}")
+    assert err < RMSE_TOL and exact > 0.999
+    pin, _, _, _ = _render(gpu, scenes.spheres_scene(width=128, height=96), 16, 4, 16)
+    assert rel_rmse(g[..., :3], pin[..., :3]) > 0.02  # the lens changes the image
+
+
 def test_render_textured_materials_match_oracle(gpu, oracle):
     """Mix materials whose base colour, roughness and metalness come from
     textures (get_albedo / get_roughness / get_metallic with their texture

@@ -94,6 +94,26 @@ def test_camera_rays_bit_exact(cfg0, gpu, oracle):
             assert got.view(np.uint32).tolist() == ref.view(np.uint32).tolist(), f"pixel {pix}"
 
 
+def test_thin_lens_camera_rays_bit_exact(gpu, oracle):
+    """Camera rays through a lens of radius 0.1 focused at 3.0 equal the
+    oracle's bit for bit (origin offset and refocused direction)."""
+    import dataclasses
+    import torch
+    sd = dataclasses.replace(scenes.spheres_scene(width=96, height=64), lens_radius=0.1, focal=3.0)
+    ps = oracle.PreparedScene(sd)
+    W, H = sd.width, sd.height
+    rays = torch.empty((W * H, 8), dtype=torch.float32, device="cuda")
+    cam = abi.make_camera(sd.inv_vp, sd.lens_radius, sd.focal)
+    for sample in (0, 3):
+        abi.generate_rays(gpu, cam, W, H, sd.seed, sample, rays)
+        torch.cuda.synchronize()
+        g = rays.cpu().numpy()
+        for pix in range(0, W * H, 7):
+            ref = ps.gen_ray(int(pix % W), int(pix // W), sample)
+            got = np.concatenate([g[pix, 0:3], g[pix, 4:7]])
+            assert got.view(np.uint32).tolist() == ref.view(np.uint32).tolist(), f"pixel {pix}"
+
+
 def test_primary_hits_bit_exact(cfg0, gpu, oracle):
     import torch
     sd, sc, ps = cfg0
