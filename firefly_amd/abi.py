@@ -89,6 +89,7 @@ _PROTOS = {
     "ff_envmap_destroy": (C.c_int, [_P]),
     "ff_trace_closest": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_int32, _P, _P]),
     "ff_trace_any": (C.c_int, [_P, _P, C.c_uint32, _P, _P, _P]),
+    "ff_trace_closest_noalloc": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_int32, _P, _P]),
     "ff_generate_rays": (C.c_int, [_P, C.POINTER(Camera), C.c_int32, C.c_int32, C.c_uint32,
                                    C.c_int32, _P, _P]),
     "ff_render": (C.c_int, [_P, _P, _P, C.POINTER(Camera), C.POINTER(RenderReq), _P, _P]),
@@ -253,6 +254,12 @@ class Scene:
         n = rays_dev.shape[0]
         check(_lib().ff_trace_closest(self.h, _ptr(rays_dev), n, _ptr(hits_dev), int(two_sided),
                                       _ptr(counts_dev), stream))
+
+    def trace_closest_noalloc(self, rays_dev, hits_dev, two_sided=False, counts_dev=None, stream=None):
+        """ff_trace_closest_noalloc: as trace_closest, with no device allocation."""
+        n = rays_dev.shape[0]
+        check(_lib().ff_trace_closest_noalloc(self.h, _ptr(rays_dev), n, _ptr(hits_dev), int(two_sided),
+                                              _ptr(counts_dev), stream))
 
     def trace_any(self, rays_dev, occ_dev, counts_dev=None, stream=None):
         n = rays_dev.shape[0]

@@ -92,6 +92,7 @@ struct ff_context {
     uint32_t* d_qbase = nullptr;  // [req_cap + 1]
     size_t req_cap = 0;
     uint32_t* fetch = nullptr;    // [FETCH_WORDS] segmented traversal work counters
+    int32_t* scratch = nullptr;   // this device's fixed trace scratch (ff_trace_closest_noalloc)
     int trav_variant = 0;
     int trav_tail = -1;      // variant of the late bounce iterations (-1: none)
     int trav_tail_from = 0;  // first iteration that uses it
@@ -356,6 +357,12 @@ ff_status ff_context_create(int32_t device, ff_context** out) {
         dfree(c->bstate);
         delete c;
         return fail(FF_ERR_ALLOC, "ff_context_create: counter allocation failed");
+    }
+    // the fixed trace scratch's address on this device (current here)
+    hipError_t se = ffk::static_scratch(device, &c->scratch);
+    if (se != hipSuccess) {
+        ff_context_destroy(c);
+        return fail(FF_ERR_HIP, "ff_context_create: address of the fixed trace scratch: %s", hipGetErrorString(se));
     }
     *out = c;
     return FF_OK;
@@ -668,6 +675,33 @@ ff_status ff_trace_closest(ff_scene* s, const float* rays, uint32_t n, float* hi
     HIP_TRY(ffk::launch_trace_closest(s->view(), (const float4*)rays, n, (float4*)hits, two_sided, s->ctx->fetch,
                                       s->ctx->ovf,
                                       (unsigned long long*)counts, s->ctx->trav_variant, (hipStream_t)stream));
+    return FF_OK;
+}
+
+ff_status ff_trace_closest_noalloc(ff_scene* s, const float* rays, uint32_t n, float* hits, int32_t two_sided,
+                                   uint64_t* counts, void* stream) {
+    if (!s || (!rays && n) || (!hits && n)) return fail(FF_ERR_INVALID, "ff_trace_closest_noalloc: NULL argument");
+    if (n == 0) return FF_OK;
+    ff_context* c = s->ctx;
+    // stack entries a traversal can hold with the variant's record width
+    // (one per binary level; up to three per 4-wide record), beyond the
+    // smallest LDS stack of any variant
+    const int32_t d = s->max_depth;
+    const int32_t width = ffk::kTravWidth[c->trav_variant];
+    const int32_t entries = width == 4 ? std::max(d, 3 * ((d + 1) / 2)) : d;
+    const int32_t need = entries > FF_LDS_MIN ? entries - FF_LDS_MIN + 1 : 1;
+    if (need > FF_STATIC_OVF_ENTRIES)
+        return fail(FF_ERR_CAPACITY,
+                    "ff_trace_closest_noalloc: a BVH of depth %d needs %d stack overflow entries per lane with "
+                    "%d-wide records; the fixed scratch holds %d (BVH depth <= %d with two-child records, <= %d "
+                    "with 4-wide ones); use ff_trace_closest",
+                    d, need, width, FF_STATIC_OVF_ENTRIES, FF_STATIC_OVF_ENTRIES + FF_LDS_MIN - 1,
+                    2 * ((FF_STATIC_OVF_ENTRIES + FF_LDS_MIN - 1) / 3));
+    if (!c->scratch) return fail(FF_ERR_HIP, "ff_trace_closest_noalloc: no fixed scratch on device %d", c->device);
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(ffk::launch_trace_closest(s->view(), (const float4*)rays, n, (float4*)hits, two_sided,
+                                      (uint32_t*)c->scratch, c->scratch + FETCH_WORDS, (unsigned long long*)counts,
+                                      c->trav_variant, (hipStream_t)stream));
     return FF_OK;
 }
 

@@ -40,6 +40,9 @@ hipError_t launch_trace_closest(const ff_dev_scene& S, const float4* rays, uint3
 hipError_t launch_trace_any(const ff_dev_scene& S, const float4* rays, uint32_t n, uint8_t* occ,
                             uint32_t* fetch, int32_t* ovf, unsigned long long* counts, int variant,
                             hipStream_t s);
+// device address of the fixed trace scratch on `device` (FETCH_WORDS work
+// counter words, then FF_STATIC_OVF_ENTRIES * FF_TRAV_THREADS overflow entries)
+hipError_t static_scratch(int device, int32_t** out);
 hipError_t launch_gen_rays(const ff_dev_camera& cam, int W, int H, uint32_t seed,
                            uint32_t sample, float4* rays, hipStream_t s);
 hipError_t launch_env_build(const float* rgb, int w, int h, float* lum, double* row_sum,

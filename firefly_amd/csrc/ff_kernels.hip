@@ -1511,6 +1511,17 @@ __global__ void k_env_guide(const double* marg_y, const double* conds, int32_t w
 
 }  // namespace
 
+// ================================================== fixed trace scratch
+// Device memory of ff_trace_closest_noalloc, the entry point for callers
+// that cannot allocate (graph capture, serving loops): the traversal work
+// counters (FETCH_WORDS words) followed by the stack overflow area
+// (FF_STATIC_OVF_ENTRIES entries for each of the FF_TRAV_THREADS lanes a
+// persistent traversal grid holds).  Every device the library runs on holds
+// its own copy until the process ends; calls on one device share it, so such
+// calls must not overlap there.  Host code uses the address from
+// hipGetSymbolAddress, never the array's name.
+static __device__ __align__(256) int32_t scratch[FETCH_WORDS + (size_t)FF_STATIC_OVF_ENTRIES * FF_TRAV_THREADS];
+
 // ===================================================================== host
 namespace ffk {
 
@@ -1645,6 +1656,31 @@ hipError_t launch_trace_closest(const ff_dev_scene& S, const float4* rays, uint3
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(256), 0, s, nullptr, 0u, fetch, 1u << F_TRACE);
     TraceArgs T{S, rays, n, hits, nullptr, two_sided, fetch + F_TRACE * FETCH_SEGS * FETCH_STRIDE, ovf, counts};
     return trav_dispatch(3, variant, nullptr, 0, &T, s);
+}
+// Device address of `scratch` on `device` (looked up once per device, with
+// that device current, by ff_context_create).
+hipError_t static_scratch(int device, int32_t** out) {
+    static std::mutex mu;
+    static std::map<int, int32_t*> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    *out = nullptr;
+    auto it = addr.find(device);
+    if (it != addr.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess) return e;
+    if (p == nullptr) return hipErrorInvalidSymbol;
+    addr[device] = (int32_t*)p;
+    *out = (int32_t*)p;
+    return hipSuccess;
 }
 hipError_t launch_trace_any(const ff_dev_scene& S, const float4* rays, uint32_t n, uint8_t* occ, uint32_t* fetch,
                             int32_t* ovf, unsigned long long* counts, int variant, hipStream_t s) {

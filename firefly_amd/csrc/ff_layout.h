@@ -16,6 +16,9 @@
 #define FF_BLOCK 256          // threads per workgroup of the stage kernels (4 waves of 64)
 #define FF_TRAV_THREADS 524288 // threads of the persistent traversal grids (any block size)
 #define FF_LDS_MIN 8          // smallest LDS stack of a traversal variant (sizes the overflow area)
+// Stack overflow entries per traversal lane in the fixed device scratch of
+// ff_trace_closest_noalloc (with the work counters: 64 MiB + 24 KiB per device)
+#define FF_STATIC_OVF_ENTRIES 32
 #define FF_GRID_CAP 4096      // grid-stride cap of the other stage kernels
 
 // path state word (beta.w bits; after a shading pass, fext.w)

@@ -190,6 +190,18 @@ ff_status ff_trace_closest(ff_scene *scene, const float *rays, uint32_t n, float
                            int32_t two_sided, uint64_t *counts, void *stream);
 ff_status ff_trace_any(ff_scene *scene, const float *rays, uint32_t n, uint8_t *occluded,
                        uint64_t *counts, void *stream);
+/* ff_trace_closest without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as ff_trace_closest; its work counters and traversal stack
+ * overflow live in a fixed per-device scratch of 64 MiB + 24 KiB that the
+ * library holds from context creation until the process ends, so it accepts
+ * BVHs of depth <= 39 (two-child traversal variants) or <= 26 (4-wide
+ * variants) and returns FF_ERR_CAPACITY beyond that.  Calls on one device
+ * share that scratch: calls that overlap on one device (different streams,
+ * threads or contexts) race and must be serialised by the caller. */
+ff_status ff_trace_closest_noalloc(ff_scene *scene, const float *rays, uint32_t n, float *hits,
+                                   int32_t two_sided, uint64_t *counts, void *stream);
 /* camera-ray generation only (wf_generate + dCamera::gen_ray,
  * CRT/kernel.cu:457-485, CRT/dCamera.cu:10-36): rays device [h*w][8] for
  * sample index `sample`. */

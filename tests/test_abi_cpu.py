@@ -35,6 +35,7 @@ def test_null_arguments_are_rejected():
     assert L.ff_envmap_destroy(None) == 1
     assert L.ff_render(None, None, None, None, None, None, None) == 1
     assert L.ff_trace_closest(None, None, 0, None, 0, None, None) == 1
+    assert L.ff_trace_closest_noalloc(None, None, 0, None, 0, None, None) == 1
     assert L.ff_trace_any(None, None, 0, None, None, None) == 1
     assert L.ff_film_resolve(None, 0, 0, 1.0, None, None, None) == 1
     assert b"NULL" in L.ff_last_error() or b"bad" in L.ff_last_error()
