@@ -22,12 +22,25 @@
  *                           monotone CDF rows ending at 1.
  *   or_upper_bound        : hand-derived index table incl. the reference quirk.
  *   or_bilinear           : closed form (corner average, linear ramps, wrap).
- *   or_gen_ray            : closed-form pinhole unprojection.
+ *   or_gen_ray            : closed-form pinhole unprojection; thin lens: origin
+ *                           offset inside the lens disk, ray through the pinhole
+ *                           ray's focal point, uniform lens points.
+ *   or_concentric         : Shirley-Chiu closed form (radius = max(|ox|,|oy|),
+ *                           piecewise-linear angle), equal-area chi^2.
  *   BSDF pieces           : closed forms (Schlick F(0)=f0, GGX normalisation
  *                           integral = 1, pdf integrals = 1, Fresnel R+T=1, TIR).
- *   or_render             : white-furnace / constant-environment closed forms and
- *                           symmetry; no independent pin of the exact quirk
- *                           arithmetic of the full estimator beyond its parts.
+ *   or_render             : escaped camera rays return the environment; thread-
+ *                           count independence; and the composed estimator
+ *                           (wf_logic + BRDF_L: MIS pairs, cosines, pdf
+ *                           divisions, Q5 visibility gating, Q6 offset, Russian
+ *                           roulette without compensation) against expectations
+ *                           computed without sampling (enumerated light-sampler
+ *                           texel corners, quadrature of the BRDF-sampled terms,
+ *                           a closed-form dielectric-sheet chain) within 4
+ *                           standard errors of 2^20 paths per pixel
+ *                           (tests/test_oracle_estimator.py); seven planted
+ *                           composition mistakes (or_set_mutation) each fail it
+ *                           by > 50 standard errors.
  */
 #include "ff_oracle.h"
 
@@ -1380,6 +1393,19 @@ static void put3(float *dst, v3 a) { dst[0] = a.x; dst[1] = a.y; dst[2] = a.z; }
  * wf_generate / wf_extend / wf_logic / wf_mat_* / wf_shadow
  * (CRT/kernel.cu:297-651), with deferred visibility of the BRDF-sampled
  * MIS rays (same sums as BRDF_L, CRT/dMaterial.cu:363-458). */
+/* Planted mistakes for the tests of the estimator check (TEST HOOK: 0 in
+ * every other use).  Each one is a plausible slip in composing the
+ * estimator; the expectation tests must fail with it:
+ *   1 light-sample term with its MIS weights swapped (w(sp, lp) for w(lp, sp))
+ *   2 spec BRDF-sampled term without its cosine
+ *   3 diffuse BRDF-sampled term divided by its pdf twice
+ *   4 Russian roulette compensated (beta / (1 - q)), i.e. Q2 "fixed"
+ *   5 BRDF-sampled terms not gated by the light ray's visibility (Q5 "fixed")
+ *   6 mixed material's diffuse visibility ray offset along wi_d (Q6 "fixed")
+ *   7 light-sample term without its cosine */
+static int g_mutation = 0;
+void or_set_mutation(int32_t m) { g_mutation = m; }
+
 static v3 trace_path(const rctx *c, int32_t px, int32_t py, int32_t sample, pstat *ps, tape_t *tape) {
     const or_scene *s = c->s;
     const or_params *p = c->p;
@@ -1431,12 +1457,18 @@ static v3 trace_path(const rctx *c, int32_t px, int32_t py, int32_t sample, psta
                 if (visS) lb = vadd(lb, Bs);
                 if (visD) lb = vadd(lb, Cd);
                 Lp = vadd(Lp, vmul(lb, beta));
+            } else if (g_mutation == 5) {
+                v3 lb = V3(0, 0, 0);
+                if (visS) lb = vadd(lb, Bs);
+                if (visD) lb = vadd(lb, Cd);
+                Lp = vadd(Lp, vmul(lb, beta));
             }
             if (vzero(fext) || pdf_ext == 0.f) { term = 1; goto TERM; }
             beta = vmul(beta, vdiv(vscl(fext, cos_ext), pdf_ext));
             if (len > 3) {
                 float q = fmaxf(0.05f, 1.f - beta.y);
                 if (or_rng(base, len, DIM_RR) < q) term = 1;
+                else if (g_mutation == 4) beta = vscl(beta, 1.f / (1.f - q));
                 goto TERM;
             }
         }
@@ -1538,6 +1570,8 @@ static v3 trace_path(const rctx *c, int32_t px, int32_t py, int32_t sample, psta
                 /* lp is infinite where sin(acos(wi.z)) == 0; the term's limit there is 0 */
                 if (lp > 0.f && lp < INFINITY && !vzero(Li)) {
                     double dwt = power_heur(lp, diff_pdf), swt = power_heur(lp, sp);
+                    if (g_mutation == 1) { dwt = power_heur(diff_pdf, lp); swt = power_heur(sp, lp); }
+                    if (g_mutation == 7) ndwi = 1.f;
                     t_swt = swt; t_dwt = dwt; t_lp = lp;
                     if (do_spec) {
                         v3 f = vscl(spec_f(n, wi, wo, rough, metal, alb, fres), ndwi);
@@ -1560,7 +1594,7 @@ static v3 trace_path(const rctx *c, int32_t px, int32_t py, int32_t sample, psta
                     double sp = (double)spec_pdf(n, wi_s, wo, rough);
                     double lp = (double)env_pdf_c(c, wi_s);
                     double wt = power_heur(sp, lp);
-                    v3 f = vscl(spec_f(n, wi_s, wo, rough, metal, alb, fres), nd);
+                    v3 f = vscl(spec_f(n, wi_s, wo, rough, metal, alb, fres), g_mutation == 2 ? 1.f : nd);
                     v3 L2 = env_L(c, wi_s);
                     if (!vzero(f) && sp > 0.f && !vzero(L2)) {
                         Bs = vdiv(vscl(vmul(f, L2), (float)wt), (float)sp);
@@ -1580,10 +1614,11 @@ static v3 trace_path(const rctx *c, int32_t px, int32_t py, int32_t sample, psta
                     v3 L3 = env_L(c, wi_d);
                     if (!vzero(f) && diff_pdf > 0.f && !vzero(L3)) {
                         Cd = vdiv(vscl(vmul(f, L3), (float)wt), (float)diff_pdf);
+                        if (g_mutation == 3) Cd = vdiv(Cd, (float)diff_pdf);
                         needD = 1;
                         t_nd_d = nd; t_wt_d = wt;
                         /* the mixed material offsets this ray along wi_spec (CRT/dMaterial.cu:431) */
-                        dd_o = vadd(P, vscl(type == OR_MAT_MIX ? wi_s : wi_d, 0.001f));
+                        dd_o = vadd(P, vscl(type == OR_MAT_MIX && g_mutation != 6 ? wi_s : wi_d, 0.001f));
                         dd_d = wi_d;
                     }
                 }

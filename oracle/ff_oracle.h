@@ -130,6 +130,8 @@ float or_env_pdf(const float *pdf, int32_t w, int32_t h, const float *wi);
 void or_env_L(const float *rgb, int32_t w, int32_t h, const float *dir, float *out);
 
 /* ---- camera ---- */
+/* test hook: plant mistake m (1-7, 0 = none) in the path estimator */
+void or_set_mutation(int32_t m);
 /* ConcentricSampleDisk (CRT/dRandom.cu:104-126) of n points u[n][2]: out[n][2] */
 void or_concentric(const float *u, int32_t n, float *out);
 void or_gen_ray(const or_scene *s, int32_t width, int32_t height, int32_t px,

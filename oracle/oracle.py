@@ -70,6 +70,7 @@ def lib() -> C.CDLL:
             "or_env_pdf": (C.c_float, [_P, C.c_int32, C.c_int32, _P]),
             "or_env_L": (None, [_P, C.c_int32, C.c_int32, _P, _P]),
             "or_concentric": (None, [_P, C.c_int32, _P]),
+            "or_set_mutation": (None, [C.c_int32]),
             "or_gen_ray": (None, [C.POINTER(OrScene), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                   C.c_uint32, _P]),
             "or_ggx_ndf": (C.c_double, [_P, _P, C.c_float]),
