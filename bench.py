@@ -48,45 +48,84 @@ def _shape_of(variant: int, spp_batch: int) -> dict:
     return {"variant": variant, "spp_batch": spp_batch}
 
 
-def load_traffic(kernel: str, workload: str, rays_per_launch: float):
-    """HBM bytes per launch of `kernel`: the committed rocprofv3 PMC summary
-    (FETCH_SIZE + WRITE_SIZE per dispatch, copy-calibrated) gives bytes per
-    ray traced; scaled by this run's rays per launch.  None if absent."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.json")
+def load_counters(kernel: str, workload: str):
+    """The committed rocprofv3 PMC summary of `kernel` on `workload`
+    (profiles/<kernel>_<workload>_counters.json): HBM bytes per dispatch
+    (FETCH_SIZE x 1024 + 64 x TCC_EA0_RDREQ_128B + WRITE_SIZE x 1024), L2 hit
+    rate, VALU issue and lane utilisation, with the commit and commands that
+    produced them.  None if absent."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_counters.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        per_ray = json.load(f).get("traffic_bytes_per_ray")
-    return None if per_ray is None else round(per_ray * rays_per_launch)
+        return json.load(f)
 
 
-def cpu_baseline(sd, nodes, ordered, budget_s: float, spp: int):
-    """Oracle (CPU restatement) on a bounded sample of the same frame: the
-    full frame at s spp, s chosen from a 1-spp calibration run so the sample
-    costs about budget_s; all host threads.  Scaled to rays/s and spp/s."""
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _band(ps, spp, spp_batch, threads, budget_s, height):
+    """Centre band of rows whose oracle render at the frame's full spp costs
+    about budget_s on `threads` threads (threads render one row each, so the
+    calibration band has one row per thread)."""
+    mid = height // 2
+    cal = max(2, threads)
+    t0 = time.perf_counter()
+    ps.render(spp=spp, spp_batch=spp_batch, n_threads=threads, rows=(mid - cal // 2, mid - cal // 2 + cal))
+    per_row = (time.perf_counter() - t0) / cal
+    n = int(max(2, min(height, round(budget_s / max(per_row, 1e-4)))))
+    r0 = max(0, mid - n // 2)
+    return r0, min(height, r0 + n)
+
+
+def cpu_baseline(sd, nodes, ordered, budget_s: float, spp: int, spp_batch: int, gpu_film=None):
+    """Oracle (CPU restatement) on bounded samples of the timed frame: a
+    centre band of rows rendered at the frame's full spp with the frame's
+    layer grouping (samples in flight), sized to cost about budget_s on all
+    host threads, and a narrower band on one thread.  Scaled to incoherent
+    rays/s.  The all-thread band is also an image check of the timed frame:
+    the GPU film's same rows against it (relative RMSE, bit-identical
+    pixels)."""
+    import numpy as np
     from oracle import oracle as O
     ps = O.PreparedScene(sd, nodes, ordered)
     threads = len(os.sched_getaffinity(0))
-    t0 = time.perf_counter()
-    ps.render(spp=1, n_threads=threads)
-    t1 = time.perf_counter() - t0
-    s = int(max(1, min(spp, round(budget_s / max(t1, 1e-3)))))
-    t0 = time.perf_counter()
-    film, st = ps.render(spp=s, n_threads=threads)
-    t = time.perf_counter() - t0
-    inc = st.rays_extension + st.rays_shadow
-    tot = inc + st.rays_primary
-    return {
-        "value": round(inc / t / 1e6, 4),
-        "unit": "Mrays/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"full {sd.width}x{sd.height} frame at {s} spp, depth {sd.max_depth} "
-                  f"({int(st.paths)} paths, {t:.1f} s)",
-        "total_mrays_s": round(tot / t / 1e6, 4),
-        "spp_per_s_1080p": round(st.paths / t / (1920 * 1080), 5),
-        "frame_s_extrapolated": round(t / s * spp, 1),
-    }
+    out = {"unit": "Mrays/s", "kind": "port", "cores": threads, "host_nproc": os.cpu_count(),
+           "host_cpu": _cpu_model()}
+    for label, nt, budget in (("all", threads, budget_s), ("one", 1, max(2.0, budget_s / 3))):
+        r0, r1 = _band(ps, spp, spp_batch, nt, budget, sd.height)
+        t0 = time.perf_counter()
+        film, st = ps.render(spp=spp, spp_batch=spp_batch, n_threads=nt, rows=(r0, r1))
+        t = time.perf_counter() - t0
+        inc = st.rays_extension + st.rays_shadow
+        rec = {"value": round(inc / t / 1e6, 4), "threads": nt,
+               "sample": f"rows {r0}-{r1} of the {sd.width}x{sd.height} frame at {spp} spp "
+                         f"({spp_batch} in flight), depth {sd.max_depth} ({int(st.paths)} paths, {t:.1f} s)",
+               "total_mrays_s": round((inc + st.rays_primary) / t / 1e6, 4),
+               "spp_per_s_1080p": round(st.paths / t / (1920 * 1080), 5),
+               "frame_s_extrapolated": round(t * sd.height / (r1 - r0), 1)}
+        if label == "all":
+            out.update(value=rec["value"], sample=rec["sample"], total_mrays_s=rec["total_mrays_s"],
+                       spp_per_s_1080p=rec["spp_per_s_1080p"], frame_s_extrapolated=rec["frame_s_extrapolated"])
+            if gpu_film is not None:
+                g = gpu_film[r0:r1, :, :3].astype(np.float64)
+                c = film[r0:r1, :, :3].astype(np.float64)
+                out["image_check"] = {
+                    "rows": [r0, r1],
+                    "rel_rmse": float(np.sqrt(np.mean((g - c) ** 2)) / max(np.sqrt(np.mean(c ** 2)), 1e-30)),
+                    "identical_pixels": float(np.mean(np.all(gpu_film[r0:r1, :, :3] == film[r0:r1, :, :3], -1))),
+                    "counts_exact": bool(np.all(gpu_film[r0:r1, :, 3] == film[r0:r1, :, 3]))}
+        else:
+            out["one_thread"] = rec
+    return out
 
 
 def run_batch(args) -> dict:
@@ -239,6 +278,21 @@ def _timed(fn, reps: int) -> float:
     return best
 
 
+def envmap_update_ms(env, rgb, reps: int = 20) -> float:
+    """Mean wall time of ff_envmap_update from a device buffer (texel copy +
+    the FP64 CDF rebuild), the per-iteration cost of an inverse-rendering
+    loop that optimises the environment."""
+    import torch
+    dev = torch.from_numpy(rgb).cuda()
+    env.update(dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        env.update(dev)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps * 1e3
+
+
 def run_adjoint(args) -> dict:
     """Differentiable single view: 512x512, 16 spp, depth 4, the 8-sphere
     scene with the spheres' albedo from a 512x512 RGB texture, 1024x512 sky.
@@ -308,6 +362,7 @@ def run_adjoint(args) -> dict:
         "adjoint_mrays_s": round((st_a.rays_primary + st_a.rays_extension + st_a.rays_shadow) / t_adj / 1e6, 2),
         "deterministic_reproducible": reproducible,
         "spp_batch": K,
+        "envmap_update_ms": round(envmap_update_ms(env, sd.env), 4),
     }
     if not args.no_cpu:
         from oracle import oracle as O
@@ -446,10 +501,11 @@ def main() -> int:
     ap.add_argument("--subdiv", type=int, default=8)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
-    ap.add_argument("--spp-batch", type=int, default=1)
+    ap.add_argument("--spp-batch", type=int, default=0, help="samples of each pixel in flight (0: all of them)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--variant", type=int, default=-1, help="traversal kernel variant (-1: autotune)")
+    ap.add_argument("--variant", type=int, default=23,
+                    help="traversal kernel variant (default: the autotuner's usual pick for the frame)")
     ap.add_argument("--workload", choices=("all", "frame", "batch", "adjoint", "multiview", "bvh"), default="all")
     ap.add_argument("--adj-spp-batch", type=int, default=16)
     ap.add_argument("--views", type=int, default=32)
@@ -459,13 +515,17 @@ def main() -> int:
     ap.add_argument("--batch-spp-batch", type=int, default=8,
                     help="batch workload: samples of each pixel in flight (both the batched call and the "
                          "sequential renders use it, so the films stay bit-identical)")
-    ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--tail-variant", type=int, default=-1,
-                    help="with --variant: traversal variant of the bounce iterations from --tail-from on")
+    ap.add_argument("--autotune", action="store_true",
+                    help="frame: let the autotuner pick the launch shape instead of --variant/--tail-variant")
+    ap.add_argument("--no-autotune", action="store_true", help="(the default; kept for older command lines)")
+    ap.add_argument("--tail-variant", type=int, default=0,
+                    help="traversal variant of the bounce iterations from --tail-from on (-1: none)")
     ap.add_argument("--tail-from", type=int, default=4)
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run a plain device copy of this many MiB first (PMC calibration)")
     args = ap.parse_args()
+    if args.spp_batch <= 0:
+        args.spp_batch = args.spp
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build runs on a single MI355X); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
@@ -510,8 +570,11 @@ def run_bvh(args) -> dict:
 
     sd = scenes.icosphere_scene(args.width, args.height, args.spp, args.depth, seed=0, subdiv=args.subdiv)
     ctx = abi.Context(0)
-    variant = args.variant if args.variant >= 0 else getattr(args, "tuned_variant", 0)
+    variant = getattr(args, "tuned_variant", args.variant)
     abi.set_traversal_variant(ctx, variant)
+    tail = getattr(args, "tuned_tail", (args.tail_variant, args.tail_from))
+    if tail[0] >= 0:
+        abi.set_traversal_tail(ctx, tail[0], tail[1])
     abi.bvh_build_gpu(ctx, sd.positions)  # warm-up: code objects, allocator
     t0 = time.perf_counter()
     _, _, sah_depth = abi.bvh_build_host(sd.positions)
@@ -569,6 +632,8 @@ def run_frame(args) -> dict:
     cam = abi.make_camera(sd.inv_vp, sd.lens_radius, sd.focal, sd.exposure)
     log(f"upload + host BVH ({scene.n_nodes} nodes, depth {scene.max_depth}) + env CDF: "
         f"{time.perf_counter() - t0:.1f} s")
+    env_ms = envmap_update_ms(env, sd.env)
+    log(f"envmap update ({sd.env.shape[1]}x{sd.env.shape[0]}): {env_ms:.3f} ms")
     film = torch.zeros((sd.height, sd.width, 4), dtype=torch.float32, device="cuda")
     if args.copy_mb:
         src = torch.ones(args.copy_mb * 2 ** 18, dtype=torch.float32, device="cuda")
@@ -579,11 +644,11 @@ def run_frame(args) -> dict:
         del src, dst
 
     tuned = None
-    if args.variant >= 0:
+    if not args.autotune:
         abi.set_traversal_variant(ctx, args.variant)
         if args.tail_variant >= 0:
             abi.set_traversal_tail(ctx, args.tail_variant, args.tail_from)
-    elif not args.no_autotune:
+    else:
         from firefly_amd.autotune import autotune
         t0 = time.perf_counter()
         tuned = autotune(ctx, scene, env, cam, sd.width, sd.height, args.depth, sd.seed, probe_spp=4,
@@ -591,6 +656,7 @@ def run_frame(args) -> dict:
                          tail_candidates=(0, 1, 22, 25, 26), tail_from=(3, 4, 5))
         args.spp_batch = tuned["best"]["spp_batch"]
         args.tuned_variant = tuned["best"]["variant"]
+        args.tuned_tail = (tuned["best"].get("tail_variant", -1), tuned["best"].get("tail_from", 4))
         for c in tuned["candidates"]:
             log(f"  variant {c['variant']} (block {c['block']}, lds {c['lds_entries']}, refill {c['refill']}, "
                 f"ww {c['while_while']}, width {c['bvh_width']}) "
@@ -635,21 +701,39 @@ def run_frame(args) -> dict:
     }
     dom = max(kern, key=lambda k: kern[k][1])
     b, ms, nl = kern[dom]
-    achieved = b / (ms / 1e3) / 1e9
     workload = f"icosphere{args.subdiv}_{sd.width}x{sd.height}_{args.spp}spp_d{args.depth}"
-    rays_launch = ((tot["rays_primary"] + tot["rays_extension"]) if dom == "k_extend" else tot["rays_shadow"]) / nl
-    traffic = load_traffic(dom, workload, rays_launch)
+    alg = b / (ms / 1e3) / 1e9  # GB/s of algorithmic bytes (every box and triangle tested)
+    # the committed counters describe this launch shape only
+    same_shape = (not args.autotune and args.variant == 23 and args.tail_variant == 0 and args.tail_from == 4 and
+                  args.spp_batch == args.spp)
+    ctr = load_counters(dom, workload) if same_shape else None
+    traffic = round(ctr["dram_bytes_per_dispatch"]) if ctr else None
+    dram = traffic * nl / (ms / 1e3) / 1e9 if traffic else None  # GB/s of measured HBM bytes
     roofline = {
         "bound": "hbm",
         "kernel": dom,
-        "achieved": round(achieved, 2),
+        "achieved": round(dram, 2) if dram else None,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 5),
+        "frac": round(dram / HBM_PEAK_GBS, 5) if dram else None,
         "traffic": traffic,
-        "algorithmic_bytes_per_launch": round(b / nl),
+        "traffic_source": (f"{ctr['source']} at commit {ctr['commit']}: HBM bytes per dispatch "
+                           f"({ctr['dram_bytes_formula']}), mean over the frame's {ctr['dispatches']} dispatches"
+                           if ctr else None),
         "avg_launch_ms": round(ms / nl, 4),
         "launches": int(nl),
+        "algorithmic": {
+            "bytes_per_launch": round(b / nl),
+            "gbs": round(alg, 2),
+            "over_hbm_peak": round(alg / HBM_PEAK_GBS, 5),
+            "basis": "40 B per ray + 32 B per box + 36 B per triangle tested (this run's counters); "
+                     "cache-inclusive, so not bounded by the HBM peak",
+        },
+        "limiter": ({"resource": "VALU issue", "frac": round(ctr["valu_issue_per_cu_cycle"], 3),
+                     "valu_lane_utilization": round(ctr["valu_lane_utilization"], 3),
+                     "l2_hit_rate": round(ctr["l2_hit_rate"], 3),
+                     "basis": "SQ_INSTS_VALU / (256 CUs x cycles), ceiling 1; lane utilisation "
+                              "SQ_THREAD_CYCLES_VALU / (64 x SQ_ACTIVE_INST_VALU)"} if ctr else None),
     }
     stages = {k: round(tot[k] / args.steps, 2) for k in ("ms_extend", "ms_shadow", "ms_logic", "ms_shade", "ms_other")}
     per_ray = {
@@ -662,7 +746,7 @@ def run_frame(args) -> dict:
     if not args.no_cpu:
         nodes, ordered = scene.bvh()
         log("cpu baseline (oracle) ...")
-        cpu = cpu_baseline(sd, nodes, ordered, args.cpu_seconds, args.spp)
+        cpu = cpu_baseline(sd, nodes, ordered, args.cpu_seconds, args.spp, args.spp_batch, film.cpu().numpy())
         log(f"cpu baseline: {cpu}")
     out = {
         "metric": METRIC,
@@ -688,6 +772,7 @@ def run_frame(args) -> dict:
         "rays_per_step": {k: int(tot[k] / args.steps) for k in ("rays_primary", "rays_extension", "rays_shadow")},
         "stage_ms_per_step": stages,
         "traversal": per_ray,
+        "envmap_update_ms": round(env_ms, 4),
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
