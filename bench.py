@@ -61,6 +61,22 @@ def load_counters(kernel: str, workload: str):
         return json.load(f)
 
 
+def limiter_of(ctr, dram_frac):
+    """The resource the committed counters put closest to its roof: VALU
+    issue (SQ_INSTS_VALU per CU-cycle, ceiling 1) or HBM bandwidth (this
+    run's traffic rate over the 8 TB/s peak); latency when neither reaches
+    0.6 of its roof."""
+    if not ctr or dram_frac is None:
+        return None
+    v = ctr["valu_issue_per_cu_cycle"]
+    res = "latency" if max(v, dram_frac) < 0.6 else ("VALU issue" if v >= dram_frac else "HBM bandwidth")
+    return {"resource": res, "valu_issue_frac": round(v, 3), "hbm_frac": round(dram_frac, 3),
+            "valu_lane_utilization": round(ctr["valu_lane_utilization"], 3),
+            "l2_hit_rate": round(ctr["l2_hit_rate"], 3),
+            "basis": "SQ_INSTS_VALU / (256 CUs x cycles), ceiling 1; lanes SQ_THREAD_CYCLES_VALU / "
+                     "(64 x SQ_ACTIVE_INST_VALU); L2 TCC_HIT / (TCC_HIT + TCC_MISS); from " + ctr["source"]}
+
+
 def _cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -130,11 +146,13 @@ def cpu_baseline(sd, nodes, ordered, budget_s: float, spp: int, spp_batch: int, 
 
 def run_batch(args) -> dict:
     """Batched-requests configuration: 64 seeded requests of uneven
-    resolution, spp, depth and triangle count rendered (a) as one batched
-    wavefront replayed from a hipGraph, (b) as 64 plain sequential frames
-    (one sample per pixel in flight) and (c) as 64 sequential frames with the
-    batched call's samples in flight (films bit-identical to (a)).  Returns
-    the throughputs and ratios."""
+    resolution, spp, depth and triangle count, rendered (a) as one batched
+    wavefront replayed from a hipGraph and (b) as 64 sequential plain frames,
+    both with K samples of each pixel in flight for K in --batch-k (films
+    bit-identical at equal K), and (c) as 64 sequential frames with one
+    sample per pixel in flight, the reference's per-frame scheme.  Same
+    traversal launch shape as the frame.  Returns the throughputs, the
+    ratios and the batched call's stage times and traversal counts."""
     import torch
     from concurrent.futures import ThreadPoolExecutor
     from firefly_amd import abi, scenes
@@ -143,6 +161,9 @@ def run_batch(args) -> dict:
     specs = scenes.batch_request_specs(args.requests, seed=0)
     envs = [scenes.sun_env(2048, 1024, k) for k in range(4)]
     ctx = abi.Context(0)
+    abi.set_traversal_variant(ctx, args.variant)
+    if args.tail_variant >= 0:
+        abi.set_traversal_tail(ctx, args.tail_variant, args.tail_from)
 
     def make(spec):
         sd = scenes.batch_request_scene(spec, envs[spec["index"] % 4])
@@ -157,29 +178,21 @@ def run_batch(args) -> dict:
         f"(setup {time.perf_counter() - t0:.1f} s)")
     cams = [abi.make_camera(sd.inv_vp) for sd, _ in built]
     f_seq = [torch.zeros((sd.height, sd.width, 4), dtype=torch.float32, device="cuda") for sd, _ in built]
-    f_plain = [torch.zeros_like(f) for f in f_seq]
     f_bat = [torch.zeros_like(f) for f in f_seq]
     items = [(sc, env_objs[k % 4], cams[k], sd.width, sd.height, sd.spp, sd.max_depth, sd.seed, f_bat[k])
              for k, (sd, sc) in enumerate(built)]
+    ks = [int(x) for x in str(args.batch_k).split(",") if x]
 
-    kb = args.batch_spp_batch  # samples of each pixel in flight in the batched call
-
-    def sequential(films, k_in_flight):
+    def sequential(k_in_flight):
         for k, (sd, sc) in enumerate(built):
             abi.render(ctx, sc, env_objs[k % 4], cams[k], sd.width, sd.height, sd.spp, sd.max_depth, sd.seed,
-                       films[k], spp_batch=k_in_flight)
+                       f_seq[k], spp_batch=k_in_flight)
 
-    def batched(flags=abi.FF_RENDER_USE_GRAPH):
-        abi.render_batch(ctx, items, flags=flags, spp_batch=kb)
+    def batched(k_in_flight, flags=abi.FF_RENDER_USE_GRAPH):
+        abi.render_batch(ctx, items, flags=flags, spp_batch=k_in_flight)
 
-    # plain: 64 per-frame wavefronts with one sample per pixel in flight (the
-    # reference's scheme); sequential: the same frames with the batched call's
-    # samples in flight (bit-identical films); batched: one call
-    times = {}
-    runs = (("plain", lambda: sequential(f_plain, 1), f_plain),
-            ("sequential", lambda: sequential(f_seq, kb), f_seq),
-            ("batched", batched, f_bat))
-    for name, fn, films in runs:
+    def best_time(name, fn, films):
+        best = float("inf")
         for i in range(args.warmup + args.steps):
             for f in films:
                 f.zero_()
@@ -189,39 +202,42 @@ def run_batch(args) -> dict:
             torch.cuda.synchronize()
             dt = time.perf_counter() - t1
             if i >= args.warmup:
-                times.setdefault(name, []).append(dt)
+                best = min(best, dt)
             log(f"{name} run {i + 1}: {dt:.3f} s")
-    st = abi.last_render_stats(ctx)
-    same = all(torch.equal(a, b) for a, b in zip(f_seq, f_bat))
-    rel_plain = max(float((a - b).abs().max() / b.abs().max().clamp_min(1e-30)) for a, b in zip(f_bat, f_plain))
+        return best
+
+    if args.batch_only:  # profiling: the batched call alone
+        best_time(f"batched K={ks[0]}", lambda: batched(ks[0]), f_bat)
+        return {"workload": f"batch{len(built)}_uneven", "spp_batch": ks[0]}
+    sweep = {}
+    identical = True
+    for k_in in ks:
+        tb = best_time(f"batched K={k_in}", lambda: batched(k_in), f_bat)
+        ts = best_time(f"sequential K={k_in}", lambda: sequential(k_in), f_seq)
+        same = all(torch.equal(a, b) for a, b in zip(f_seq, f_bat))
+        identical = identical and same
+        sweep[k_in] = (tb, ts, same)
+    tp = best_time("sequential K=1 (plain)", lambda: sequential(1), f_seq)
+    rel_plain = max(float((a - b).abs().max() / b.abs().max().clamp_min(1e-30)) for a, b in zip(f_bat, f_seq))
+    kb = min(sweep, key=lambda k: sweep[k][0])  # the batched call's best K
+    ks_best = min(sweep, key=lambda k: sweep[k][1])
+    kp = 16 if 16 in sweep else kb  # the samples in flight of the committed counter summary
     for f in f_bat:
         f.zero_()
-    batched(abi.FF_RENDER_TIMING)  # one more call, timed per stage
-    stt = abi.last_render_stats(ctx)
-    stage_ms = {k: round(getattr(stt, k), 2) for k in ("ms_extend", "ms_shadow", "ms_logic", "ms_shade", "ms_other")}
-    mismatch = []
-    if not same:
-        # where the films differ, and whether a plain render repeats itself
-        f_rep = [torch.zeros_like(f) for f in f_seq]
-        for k, (sd, sc) in enumerate(built):
-            abi.render(ctx, sc, env_objs[k % 4], cams[k], sd.width, sd.height, sd.spp, sd.max_depth, sd.seed,
-                       f_rep[k], spp_batch=kb)
-        torch.cuda.synchronize()
-        for k, (sd, _) in enumerate(built):
-            a, b, c = f_seq[k], f_bat[k], f_rep[k]
-            if not torch.equal(a, b):
-                d = (a - b).abs()
-                mismatch.append({"request": k, "tris": sd.n_tris, "spp": sd.spp, "depth": sd.max_depth,
-                                 "side": sd.width, "pixels": int((d.amax(-1) > 0).sum()),
-                                 "max_abs": float(d.max()), "film_mean": float(a[..., :3].mean()),
-                                 "count_diff": int((a[..., 3] != b[..., 3]).sum()),
-                                 "sequential_repeats": bool(torch.equal(a, c)),
-                                 "batched_equals_repeat": bool(torch.equal(b, c))})
-        log(f"film mismatches: {len(mismatch)} requests; first: {mismatch[:6]}")
+    batched(kp, abi.FF_RENDER_TIMING)  # one more call, timed per stage
+    st = abi.last_render_stats(ctx)
+    stage_ms = {k: round(getattr(st, k), 2) for k in ("ms_extend", "ms_shadow", "ms_logic", "ms_shade", "ms_other")}
     inc = st.rays_extension + st.rays_shadow
-    tb = min(times["batched"])
-    ts = min(times["sequential"])
-    tp = min(times["plain"])
+    ext_rays = st.rays_primary + st.rays_extension
+    tb = sweep[kb][0]
+    ts_best = min(min(v[1] for v in sweep.values()), tp)
+    wl = f"batch{len(built)}_uneven"
+    ext_bytes = EXT_RAY_BYTES * ext_rays + NODE_BYTES * st.ext_nodes + TRI_BYTES * st.ext_tris
+    alg = ext_bytes / (st.ms_extend / 1e3) / 1e9
+    ctr = load_counters("k_extend", wl) if (args.variant == 23 and args.tail_variant == 0 and args.tail_from == 4
+                                            and kp == 16) else None
+    traffic = round(ctr["dram_bytes_per_dispatch"]) if ctr else None
+    dram = traffic * st.launches_extend / (st.ms_extend / 1e3) / 1e9 if traffic else None
     out = {
         "metric": METRIC,
         "value": round(inc / tb / 1e6, 3),
@@ -235,20 +251,37 @@ def run_batch(args) -> dict:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"batch{len(built)}_uneven", "requests": len(built),
+        "config": {"workload": wl, "requests": len(built),
                    "pixels": int(sum(sd.width * sd.height for sd, _ in built)),
                    "paths": int(sum(sd.width * sd.height * sd.spp for sd, _ in built)),
                    "tris": int(sum(sd.n_tris for sd, _ in built)), "spp_batch": kb},
         "batched_incoherent_mrays_s": round(inc / tb / 1e6, 3),
         "sequential_plain_mrays_s": round(inc / tp / 1e6, 3),
-        "sequential_incoherent_mrays_s": round(inc / ts / 1e6, 3),
-        "batched_over_sequential": round(tp / tb, 3),
-        "batched_over_sequential_same_spp_batch": round(ts / tb, 3),
-        "films_identical": bool(same),
+        "batched_over_plain": round(tp / tb, 3),
+        "sweep": {str(k): {"batched_ms": round(v[0] * 1e3, 2), "sequential_ms": round(v[1] * 1e3, 2),
+                           "batched_over_sequential": round(v[1] / v[0], 3), "films_identical": v[2]}
+                  for k, v in sweep.items()},
+        "batched_over_sequential_same_spp_batch": round(sweep[kb][1] / tb, 3),
+        "batched_best_over_sequential_best": round(ts_best / tb, 3),
+        "sequential_best_spp_batch": ks_best,
+        "films_identical": bool(identical),
         "film_max_rel_diff_vs_plain": rel_plain,
         "batched_stage_ms": stage_ms,
-        "film_mismatch": mismatch[:8],
+        "stage_and_roofline_spp_batch": kp,
         "rays": {"primary": st.rays_primary, "extension": st.rays_extension, "shadow": st.rays_shadow},
+        "traversal": {"ext_nodes_per_ray": round(st.ext_nodes / max(1.0, ext_rays), 3),
+                      "ext_tris_per_ray": round(st.ext_tris / max(1.0, ext_rays), 3),
+                      "shadow_nodes_per_ray": round(st.shadow_nodes / max(1.0, st.rays_shadow), 3),
+                      "shadow_tris_per_ray": round(st.shadow_tris / max(1.0, st.rays_shadow), 3)},
+        "roofline": {"bound": "hbm", "kernel": "k_extend", "achieved": round(dram, 2) if dram else None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dram / HBM_PEAK_GBS, 5) if dram else None,
+                     "traffic": traffic,
+                     "traffic_source": f"{ctr['source']} at commit {ctr['commit']}" if ctr else None,
+                     "avg_launch_ms": round(st.ms_extend / max(1.0, st.launches_extend), 4),
+                     "launches": int(st.launches_extend),
+                     "algorithmic": {"bytes_per_launch": round(ext_bytes / max(1.0, st.launches_extend)),
+                                     "gbs": round(alg, 2), "over_hbm_peak": round(alg / HBM_PEAK_GBS, 5)},
+                     "limiter": limiter_of(ctr, dram / HBM_PEAK_GBS if dram else None)},
     }
     for sc in built:
         sc[1].close()
@@ -395,6 +428,9 @@ def run_multiview(args) -> dict:
 
     sd, vps = scenes.multiview_scene(args.views, 256, 8, 6, seed=0, subdiv=args.subdiv, tex_size=1024)
     ctx = abi.Context(0)
+    abi.set_traversal_variant(ctx, args.variant)
+    if args.tail_variant >= 0:
+        abi.set_traversal_tail(ctx, args.tail_variant, args.tail_from)
     scene = abi.Scene(ctx, sd.positions, sd.normals, sd.uvs, sd.material_ids, sd.materials, sd.textures)
     env = abi.EnvMap(ctx, sd.env)
     cams = [abi.make_camera(v) for v in vps]
@@ -441,8 +477,17 @@ def run_multiview(args) -> dict:
             abi.adjoint(ctx, scene, env, cams[k], W, H, S, D, sd.seed, paths[k], ds[k], tex_grads=[g_alb, g_rough],
                         spp_batch=kb)
 
+    if args.mv_forward_only:  # profiling: the batched forward alone
+        fwd_batch()
+        torch.cuda.synchronize()
+        return {"workload": f"multiview{V}_icosphere{args.subdiv}_256x256_8spp_d6_tex1024", "spp_batch": kb}
     t_fb = _timed(fwd_batch, args.steps + 1)
     st_f = abi.last_render_stats(ctx)
+    for f in films:
+        f.zero_()
+    abi.render_batch(ctx, items, path_radiance=paths, flags=abi.FF_RENDER_USE_GRAPH | abi.FF_RENDER_TIMING,
+                     spp_batch=kb)
+    st_t = abi.last_render_stats(ctx)
     adj_batch()
     t_ab = _timed(adj_batch, args.steps + 1)
     st_a = abi.last_render_stats(ctx)
@@ -452,8 +497,29 @@ def run_multiview(args) -> dict:
     t_sep = _timed(separate, 2)
     rays_f = st_f.rays_primary + st_f.rays_extension + st_f.rays_shadow
     rays_a = st_a.rays_primary + st_a.rays_extension + st_a.rays_shadow
+    wl = f"multiview{V}_icosphere{args.subdiv}_256x256_8spp_d6_tex1024"
+    ext_rays = st_t.rays_primary + st_t.rays_extension
+    ext_bytes = EXT_RAY_BYTES * ext_rays + NODE_BYTES * st_t.ext_nodes + TRI_BYTES * st_t.ext_tris
+    alg = ext_bytes / (st_t.ms_extend / 1e3) / 1e9
+    ctr = load_counters("k_extend", wl) if (args.variant == 23 and args.tail_variant == 0 and
+                                            args.tail_from == 4 and kb == 8) else None
+    traffic = round(ctr["dram_bytes_per_dispatch"]) if ctr else None
+    dram = traffic * st_t.launches_extend / (st_t.ms_extend / 1e3) / 1e9 if traffic else None
     out = {
-        "workload": f"multiview{V}_icosphere{args.subdiv}_256x256_8spp_d6_tex1024",
+        "workload": wl,
+        "batched_forward_stage_ms": {k: round(getattr(st_t, k), 2)
+                                     for k in ("ms_extend", "ms_shadow", "ms_logic", "ms_shade", "ms_other")},
+        "traversal": {"ext_nodes_per_ray": round(st_t.ext_nodes / max(1.0, ext_rays), 3),
+                      "shadow_nodes_per_ray": round(st_t.shadow_nodes / max(1.0, st_t.rays_shadow), 3)},
+        "roofline": {"bound": "hbm", "kernel": "k_extend (batched forward)",
+                     "achieved": round(dram, 2) if dram else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(dram / HBM_PEAK_GBS, 5) if dram else None, "traffic": traffic,
+                     "traffic_source": f"{ctr['source']} at commit {ctr['commit']}" if ctr else None,
+                     "avg_launch_ms": round(st_t.ms_extend / max(1.0, st_t.launches_extend), 4),
+                     "launches": int(st_t.launches_extend),
+                     "algorithmic": {"bytes_per_launch": round(ext_bytes / max(1.0, st_t.launches_extend)),
+                                     "gbs": round(alg, 2), "over_hbm_peak": round(alg / HBM_PEAK_GBS, 5)},
+                     "limiter": limiter_of(ctr, dram / HBM_PEAK_GBS if dram else None)},
         "batched_forward_ms": round(t_fb * 1e3, 2),
         "batched_adjoint_ms": round(t_ab * 1e3, 2),
         "batched_adjoint_deterministic_ms": round(t_det * 1e3, 2),
@@ -512,9 +578,13 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=64)
     ap.add_argument("--mv-spp-batch", type=int, default=8,
                     help="multiview workload: samples of each pixel in flight (batched and separate runs)")
-    ap.add_argument("--batch-spp-batch", type=int, default=8,
-                    help="batch workload: samples of each pixel in flight (both the batched call and the "
-                         "sequential renders use it, so the films stay bit-identical)")
+    ap.add_argument("--batch-k", default="8,16,32",
+                    help="batch workload: samples of each pixel in flight to measure, batched and sequential "
+                         "(films bit-identical at equal values)")
+    ap.add_argument("--mv-forward-only", action="store_true",
+                    help="multiview workload: only the batched forward (for profiling)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="batch workload: only the batched call at the first --batch-k value (for profiling)")
     ap.add_argument("--autotune", action="store_true",
                     help="frame: let the autotuner pick the launch shape instead of --variant/--tail-variant")
     ap.add_argument("--no-autotune", action="store_true", help="(the default; kept for older command lines)")
@@ -575,25 +645,32 @@ def run_bvh(args) -> dict:
     tail = getattr(args, "tuned_tail", (args.tail_variant, args.tail_from))
     if tail[0] >= 0:
         abi.set_traversal_tail(ctx, tail[0], tail[1])
-    abi.bvh_build_gpu(ctx, sd.positions)  # warm-up: code objects, allocator
     t0 = time.perf_counter()
     _, _, sah_depth = abi.bvh_build_host(sd.positions)
     sah_s = time.perf_counter() - t0
-    best_wall, best_ms = float("inf"), float("inf")
-    for _ in range(3):
-        t0 = time.perf_counter()
-        nodes, _, lb_depth, ms = abi.bvh_build_gpu(ctx, sd.positions)
-        best_wall, best_ms = min(best_wall, time.perf_counter() - t0), min(best_ms, ms)
-    log(f"bvh: host SAH {sah_s:.3f} s (depth {sah_depth}); GPU LBVH {best_ms:.2f} ms kernels, "
-        f"{best_wall * 1e3:.1f} ms with transfers ({len(nodes)} nodes, depth {lb_depth})")
+    builds = {}
+    for name, b in (("lbvh", abi.FF_BVH_LBVH), ("ploc", abi.FF_BVH_PLOC)):
+        abi.bvh_build_gpu(ctx, sd.positions, builder=b)  # warm-up
+        best_wall, best_ms = float("inf"), float("inf")
+        for _ in range(3):
+            t0 = time.perf_counter()
+            nodes, _, depth, ms = abi.bvh_build_gpu(ctx, sd.positions, builder=b)
+            best_wall, best_ms = min(best_wall, time.perf_counter() - t0), min(best_ms, ms)
+        builds[name] = (best_ms, best_wall, depth, len(nodes))
+        log(f"bvh: GPU {name} {best_ms:.2f} ms kernels, {best_wall * 1e3:.1f} ms with transfers "
+            f"({len(nodes)} nodes, depth {depth})")
+    log(f"bvh: host SAH {sah_s:.3f} s (depth {sah_depth})")
+    best_ms, best_wall, lb_depth, _ = builds["lbvh"]
     env = abi.EnvMap(ctx, sd.env)
     cam = abi.make_camera(sd.inv_vp, sd.lens_radius, sd.focal, sd.exposure)
     film = torch.zeros((sd.height, sd.width, 4), dtype=torch.float32, device="cuda")
     out = {"tris": sd.n_tris, "host_sah_build_s": round(sah_s, 3), "sah_depth": sah_depth,
            "gpu_lbvh_build_ms": round(best_ms, 3), "gpu_lbvh_build_wall_ms": round(best_wall * 1e3, 2),
            "lbvh_depth": lb_depth, "build_speedup": round(sah_s / (best_ms / 1e3), 1),
+           "gpu_ploc_build_ms": round(builds["ploc"][0], 3), "gpu_ploc_build_wall_ms": round(builds["ploc"][1] * 1e3, 2),
+           "ploc_depth": builds["ploc"][2], "ploc_build_speedup": round(sah_s / (builds["ploc"][0] / 1e3), 1),
            "variant": variant, "spp_batch": args.spp_batch}
-    for name, b in (("sah", abi.FF_BVH_SAH), ("lbvh", abi.FF_BVH_LBVH)):
+    for name, b in (("sah", abi.FF_BVH_SAH), ("lbvh", abi.FF_BVH_LBVH), ("ploc", abi.FF_BVH_PLOC)):
         t0 = time.perf_counter()
         sc = abi.Scene(ctx, sd.positions, sd.normals, sd.uvs, sd.material_ids, sd.materials, sd.textures, builder=b)
         create_s = time.perf_counter() - t0
@@ -611,6 +688,7 @@ def run_bvh(args) -> dict:
         out[f"{name}_frame_ms"] = round(best * 1e3, 2)
         out[f"{name}_incoherent_mrays_s"] = round((st.rays_extension + st.rays_shadow) / best / 1e6, 1)
         out[f"{name}_ext_nodes_per_ray"] = round(st.ext_nodes / max(1.0, rays), 2)
+        out[f"{name}_shadow_nodes_per_ray"] = round(st.shadow_nodes / max(1.0, st.rays_shadow), 2)
         log(f"bvh {name}: scene create {create_s:.2f} s, frame {best * 1e3:.1f} ms")
         sc.close()
     env.close()
@@ -729,11 +807,7 @@ def run_frame(args) -> dict:
             "basis": "40 B per ray + 32 B per box + 36 B per triangle tested (this run's counters); "
                      "cache-inclusive, so not bounded by the HBM peak",
         },
-        "limiter": ({"resource": "VALU issue", "frac": round(ctr["valu_issue_per_cu_cycle"], 3),
-                     "valu_lane_utilization": round(ctr["valu_lane_utilization"], 3),
-                     "l2_hit_rate": round(ctr["l2_hit_rate"], 3),
-                     "basis": "SQ_INSTS_VALU / (256 CUs x cycles), ceiling 1; lane utilisation "
-                              "SQ_THREAD_CYCLES_VALU / (64 x SQ_ACTIVE_INST_VALU)"} if ctr else None),
+        "limiter": limiter_of(ctr, dram / HBM_PEAK_GBS if dram else None),
     }
     stages = {k: round(tot[k] / args.steps, 2) for k in ("ms_extend", "ms_shadow", "ms_logic", "ms_shade", "ms_other")}
     per_ray = {

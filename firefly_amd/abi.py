@@ -16,7 +16,8 @@ FF_MAT_MIX, FF_MAT_DIFFUSE, FF_MAT_CONDUCTOR, FF_MAT_DIELECTRIC = 0, 1, 2, 3
 FF_RENDER_USE_GRAPH = 1
 FF_RENDER_TIMING = 2
 FF_ADJOINT_DETERMINISTIC = 4
-FF_BVH_SAH, FF_BVH_LBVH = 0, 1  # host SAH (the reference's tree) / linear BVH built on the GPU
+# host SAH (the reference's tree) / linear BVH built on the GPU / PLOC clustering built on the GPU
+FF_BVH_SAH, FF_BVH_LBVH, FF_BVH_PLOC = 0, 1, 2
 
 NODE_DTYPE = np.dtype([("bmin", "<f4", 3), ("bmax", "<f4", 3), ("offset", "<i4"),
                        ("n_prims", "<u2"), ("axis", "u1"), ("pad", "u1")])
@@ -79,6 +80,8 @@ _PROTOS = {
                                     C.POINTER(C.c_int32)]),
     "ff_bvh_build_gpu": (C.c_int, [_P, _P, C.c_int32, C.c_int32, _P, _P, C.POINTER(C.c_int32),
                                    C.POINTER(C.c_int32), C.POINTER(C.c_float)]),
+    "ff_bvh_build_ploc": (C.c_int, [_P, _P, C.c_int32, C.c_int32, _P, _P, C.POINTER(C.c_int32),
+                                    C.POINTER(C.c_int32), C.POINTER(C.c_float)]),
     "ff_scene_create": (C.c_int, [_P, C.POINTER(SceneDesc), C.POINTER(_P)]),
     "ff_scene_destroy": (C.c_int, [_P]),
     "ff_scene_info": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
@@ -144,16 +147,18 @@ def _ptr(a) -> int:
     return a.data_ptr()
 
 
-def bvh_build_gpu(ctx: "Context", positions: np.ndarray, max_prims: int = 8):
-    """Linear BVH built on the GPU (the FF_BVH_LBVH builder): (nodes, ordered,
-    max_depth, device ms of the build kernels)."""
+def bvh_build_gpu(ctx: "Context", positions: np.ndarray, max_prims: int = 8, builder: int = FF_BVH_LBVH):
+    """BVH built on the GPU, linear (FF_BVH_LBVH, ff_bvh_build_gpu) or by
+    clustering (FF_BVH_PLOC, ff_bvh_build_ploc): (nodes, ordered, max_depth,
+    device ms of the build kernels)."""
     pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 9)
     n = pos.shape[0]
     nodes = np.zeros(max(2 * n - 1, 1), dtype=NODE_DTYPE)
     ordered = np.zeros(n, dtype=np.int32)
     nn, md, ms = C.c_int32(), C.c_int32(), C.c_float()
-    check(_lib().ff_bvh_build_gpu(ctx.h, pos.ctypes.data, n, max_prims, nodes.ctypes.data, ordered.ctypes.data,
-                                  C.byref(nn), C.byref(md), C.byref(ms)))
+    fn = _lib().ff_bvh_build_ploc if builder == FF_BVH_PLOC else _lib().ff_bvh_build_gpu
+    check(fn(ctx.h, pos.ctypes.data, n, max_prims, nodes.ctypes.data, ordered.ctypes.data,
+             C.byref(nn), C.byref(md), C.byref(ms)))
     return nodes[: nn.value].copy(), ordered, md.value, ms.value
 
 

@@ -427,13 +427,36 @@ ff_status ff_bvh_build_gpu(ff_context* ctx, const float* positions, int32_t n_tr
     return FF_OK;
 }
 
+ff_status ff_bvh_build_ploc(ff_context* ctx, const float* positions, int32_t n_tris, int32_t max_prims,
+                            ff_bvh_node* nodes_out, int32_t* ordered_out, int32_t* n_nodes_out, int32_t* max_depth_out,
+                            float* build_ms_out) {
+    if (!ctx || !positions || n_tris <= 0 || max_prims <= 0)
+        return fail(FF_ERR_INVALID, "ff_bvh_build_ploc: need a context, positions, n_tris > 0, max_prims > 0");
+    if (n_tris >= (1 << 27))
+        return fail(FF_ERR_CAPACITY, "ff_bvh_build_ploc: %d triangles (limit 2^27 - 1)", n_tris);
+    for (size_t k = 0; k < (size_t)n_tris * 9; k++)
+        if (!std::isfinite(positions[k]))
+            return fail(FF_ERR_INVALID, "ff_bvh_build_ploc: triangle %d has a non-finite vertex coordinate", (int)(k / 9));
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::vector<ffh::BvhNode> nodes;
+    std::vector<int32_t> ordered;
+    float ms = 0.f;
+    HIP_TRY(ffk::build_ploc(positions, n_tris, max_prims, nodes, ordered, &ms));
+    if (nodes_out) std::memcpy(nodes_out, nodes.data(), nodes.size() * sizeof(ffh::BvhNode));
+    if (ordered_out) std::memcpy(ordered_out, ordered.data(), ordered.size() * sizeof(int32_t));
+    if (n_nodes_out) *n_nodes_out = (int32_t)nodes.size();
+    if (max_depth_out) *max_depth_out = ffh::bvh_max_depth(nodes);
+    if (build_ms_out) *build_ms_out = ms;
+    return FF_OK;
+}
+
 ff_status ff_scene_create(ff_context* ctx, const ff_scene_desc* d, ff_scene** out) {
     if (!ctx || !d || !out) return fail(FF_ERR_INVALID, "ff_scene_create: NULL argument");
     *out = nullptr;
     if (d->n_tris <= 0 || !d->positions || !d->normals || !d->material_ids)
         return fail(FF_ERR_INVALID, "ff_scene_create: empty scene or missing positions/normals/material ids");
     if (d->n_materials <= 0 || !d->materials) return fail(FF_ERR_INVALID, "ff_scene_create: no materials");
-    if (d->builder != FF_BVH_SAH && d->builder != FF_BVH_LBVH)
+    if (d->builder != FF_BVH_SAH && d->builder != FF_BVH_LBVH && d->builder != FF_BVH_PLOC)
         return fail(FF_ERR_INVALID, "ff_scene_create: unknown BVH builder %d", d->builder);
     if ((uint32_t)d->n_materials > MAT_INDEX_MASK)
         return fail(FF_ERR_CAPACITY, "ff_scene_create: %d materials (limit 2^30 - 1)", d->n_materials);
@@ -472,8 +495,10 @@ ff_status ff_scene_create(ff_context* ctx, const ff_scene_desc* d, ff_scene** ou
     s->n_tris = d->n_tris;
     s->n_mats = d->n_materials;
     int32_t max_prims = d->max_prims_in_node > 0 ? d->max_prims_in_node : 8;
-    if (d->builder == FF_BVH_LBVH) {
-        hipError_t be = ffk::build_lbvh(d->positions, d->n_tris, max_prims, s->nodes, s->ordered);
+    if (d->builder == FF_BVH_LBVH || d->builder == FF_BVH_PLOC) {
+        hipError_t be = d->builder == FF_BVH_LBVH
+                            ? ffk::build_lbvh(d->positions, d->n_tris, max_prims, s->nodes, s->ordered)
+                            : ffk::build_ploc(d->positions, d->n_tris, max_prims, s->nodes, s->ordered);
         if (be != hipSuccess) {
             ff_scene_destroy(s);
             return fail(FF_ERR_HIP, "ff_scene_create: GPU BVH build failed: %s", hipGetErrorString(be));

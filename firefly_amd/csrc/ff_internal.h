@@ -77,4 +77,8 @@ namespace ffk {
 // (gpu_ms, if given: device time of the build kernels).
 hipError_t build_lbvh(const float* positions, int32_t n, int32_t max_prims, std::vector<ffh::BvhNode>& nodes,
                       std::vector<int32_t>& ordered, float* gpu_ms = nullptr);
+// PLOC tree on the GPU (locally-ordered clustering of the Morton-sorted
+// triangles, SAH leaf collapse), same node format and arguments.
+hipError_t build_ploc(const float* positions, int32_t n, int32_t max_prims, std::vector<ffh::BvhNode>& nodes,
+                      std::vector<int32_t>& ordered, float* gpu_ms = nullptr);
 }  // namespace ffk

@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <stdint.h>
 
@@ -302,6 +303,207 @@ __global__ __launch_bounds__(LB_BLOCK) void k_emit(Tree T, ffh::BvhNode* __restr
     out[idx] = r;
 }
 
+// ------------------------------------------------------------------ PLOC
+// Parallel locally-ordered clustering (Meister and Bittner, "Parallel
+// Locally-Ordered Clustering for Bounding Volume Hierarchy Construction",
+// IEEE TVCG 2018) over the Morton-sorted triangles: every round, each
+// cluster finds its nearest neighbour among the PLOC_R clusters on either
+// side in the current order (the smallest surface area of the two boxes'
+// union; ties to the lower position), mutual neighbours merge into a new node
+// at the lower position, and the array is compacted in order.  Node ids:
+// triangles 0..n-1 in sorted order, then internal nodes in creation order
+// (a round's merges by position), so every child precedes its parent and a
+// node's cost, leaf decision and child order are fixed when it is made.
+constexpr int PLOC_R = 16;
+
+struct PNodes {
+    float* box;       // [2n-1][6] lo xyz, hi xyz
+    int* first;       // [2n-1] children in visit order (-1: triangle)
+    int* second;
+    int* parent;      // [2n-1] (-1: root)
+    int* count;       // [2n-1] triangles in the subtree
+    float* cost;      // [2n-1] SAH cost of the subtree (half areas)
+    int* osize;       // [2n-1] output nodes of the subtree (1 for a leaf)
+    uint8_t* axis;    // [2n-1] split axis of an interior node
+    uint8_t* leaf;    // [2n-1] 1: an output leaf (a triangle or a collapsed subtree)
+    int max_prims;
+};
+
+// half the surface area of a box (the SAH's area, up to the factor 2)
+__device__ __forceinline__ float half_area(const float* b) {
+    float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+    return (dx * dy + dy * dz) + dz * dx;
+}
+
+__global__ __launch_bounds__(LB_BLOCK) void k_ploc_leaves(const uint64_t* __restrict__ keys,
+                                                         const float* __restrict__ pos, int n, PNodes N,
+                                                         int* __restrict__ C) {
+    int i = blockIdx.x * LB_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    float lo[3], hi[3];
+    tri_box(pos, (uint32_t)(keys[i] & 0xffffffffu), lo, hi);
+    float* b = N.box + (size_t)i * 6;
+    for (int c = 0; c < 3; c++) {
+        b[c] = lo[c];
+        b[3 + c] = hi[c];
+    }
+    N.first[i] = -1;
+    N.second[i] = -1;
+    N.parent[i] = -1;
+    N.count[i] = 1;
+    N.cost[i] = half_area(b);
+    N.osize[i] = 1;
+    N.axis[i] = 0;
+    N.leaf[i] = 1;
+    C[i] = i;
+}
+
+// nearest neighbour of every cluster within PLOC_R positions; the block's
+// clusters and the windows' margins are staged in LDS
+__global__ __launch_bounds__(LB_BLOCK) void k_ploc_nn(const int* __restrict__ C, int m, const float* __restrict__ box,
+                                                     int* __restrict__ nn) {
+    __shared__ float sb[(LB_BLOCK + 2 * PLOC_R) * 6];
+    int b0 = (int)blockIdx.x * (int)LB_BLOCK - PLOC_R;
+    for (int t = threadIdx.x; t < (int)LB_BLOCK + 2 * PLOC_R; t += LB_BLOCK) {
+        int q = b0 + t;
+        if (q >= 0 && q < m) {
+            const float* src = box + (size_t)C[q] * 6;
+            for (int c = 0; c < 6; c++) sb[t * 6 + c] = src[c];
+        }
+    }
+    __syncthreads();
+    int i = blockIdx.x * LB_BLOCK + threadIdx.x;
+    if (i >= m) return;
+    const float* a = &sb[(i - b0) * 6];
+    float best = 0.f;
+    int bj = -1;
+    int j0 = i - PLOC_R < 0 ? 0 : i - PLOC_R, j1 = i + PLOC_R > m - 1 ? m - 1 : i + PLOC_R;
+    for (int j = j0; j <= j1; j++) {
+        if (j == i) continue;
+        const float* b = &sb[(j - b0) * 6];
+        float u[6];
+        for (int c = 0; c < 3; c++) {
+            u[c] = mn(a[c], b[c]);
+            u[3 + c] = mx(a[3 + c], b[3 + c]);
+        }
+        float ar = half_area(u);
+        if (bj < 0 || ar < best) {
+            best = ar;
+            bj = j;
+        }
+    }
+    nn[i] = bj;
+}
+
+// per position: bit 0 keep (the cluster or its merge survives here), bit 32
+// new (a merge is created here)
+__global__ __launch_bounds__(LB_BLOCK) void k_ploc_flags(const int* __restrict__ nn, int m,
+                                                        unsigned long long* __restrict__ flags) {
+    int i = blockIdx.x * LB_BLOCK + threadIdx.x;
+    if (i >= m) return;
+    int j = nn[i];
+    bool mutual = nn[j] == i;
+    unsigned long long keep = (!mutual || i < j) ? 1ull : 0ull;
+    unsigned long long made = (mutual && i < j) ? 1ull : 0ull;
+    flags[i] = keep | (made << 32);
+}
+
+__global__ __launch_bounds__(LB_BLOCK) void k_ploc_merge(const int* __restrict__ C, const int* __restrict__ nn, int m,
+                                                        const unsigned long long* __restrict__ flags,
+                                                        const unsigned long long* __restrict__ scan, int base,
+                                                        PNodes N, int* __restrict__ Cout) {
+    int i = blockIdx.x * LB_BLOCK + threadIdx.x;
+    if (i >= m) return;
+    unsigned long long f = flags[i];
+    if (!(f & 1ull)) return;
+    int dst = (int)(scan[i] & 0xffffffffull);
+    if (!(f >> 32)) {
+        Cout[dst] = C[i];
+        return;
+    }
+    int id = base + (int)(scan[i] >> 32);
+    int a = C[i], b = C[nn[i]];  // a at the lower position
+    const float* ba = N.box + (size_t)a * 6;
+    const float* bb = N.box + (size_t)b * 6;
+    float* o = N.box + (size_t)id * 6;
+    float u[6];
+    for (int c = 0; c < 3; c++) {
+        u[c] = mn(ba[c], bb[c]);
+        u[3 + c] = mx(ba[3 + c], bb[3 + c]);
+    }
+    for (int c = 0; c < 6; c++) o[c] = u[c];
+    // split axis: the largest separation of the children's centres (lowest
+    // axis on ties); the child with the lower centre on it is visited first
+    int ax = 0;
+    float sep = 0.f, sgn = 0.f;
+    for (int c = 0; c < 3; c++) {
+        float d = (bb[c] + bb[3 + c]) - (ba[c] + ba[3 + c]);
+        float ad = d < 0.f ? -d : d;
+        if (c == 0 || ad > sep) {
+            sep = ad;
+            sgn = d;
+            ax = c;
+        }
+    }
+    int f1 = sgn < 0.f ? b : a, f2 = sgn < 0.f ? a : b;
+    int cnt = N.count[a] + N.count[b];
+    float A = half_area(u);
+    float c_split = (0.125f * A + N.cost[a]) + N.cost[b];
+    float c_leaf = (float)cnt * A;
+    bool collapse = cnt <= N.max_prims && c_leaf <= c_split;
+    N.first[id] = f1;
+    N.second[id] = f2;
+    N.parent[id] = -1;
+    N.parent[a] = id;
+    N.parent[b] = id;
+    N.count[id] = cnt;
+    N.cost[id] = collapse ? c_leaf : c_split;
+    N.leaf[id] = collapse ? 1 : 0;
+    N.osize[id] = collapse ? 1 : 1 + N.osize[f1] + N.osize[f2];
+    N.axis[id] = (uint8_t)ax;
+    Cout[dst] = id;
+}
+
+// Every node finds its depth-first output index and the depth-first index
+// of its first triangle by walking to the root; triangles place themselves
+// in the slot order, output nodes write their records.
+__global__ __launch_bounds__(LB_BLOCK) void k_ploc_emit(PNodes N, const uint64_t* __restrict__ keys, int n,
+                                                       ffh::BvhNode* __restrict__ out, int32_t* __restrict__ ordered) {
+    int v = blockIdx.x * LB_BLOCK + threadIdx.x;
+    if (v >= 2 * n - 1) return;
+    int idx = 0, poff = 0;
+    bool is_out = true;
+    for (int cur = v, p = N.parent[v]; p >= 0; cur = p, p = N.parent[p]) {
+        if (N.leaf[p]) is_out = false;
+        int f = N.first[p];
+        if (N.second[p] == cur) {
+            idx += 1 + N.osize[f];
+            poff += N.count[f];
+        } else {
+            idx += 1;
+        }
+    }
+    if (v < n) ordered[poff] = (int32_t)(keys[v] & 0xffffffffu);
+    if (!is_out) return;
+    ffh::BvhNode r;
+    const float* b = N.box + (size_t)v * 6;
+    for (int c = 0; c < 3; c++) {
+        r.bmin[c] = b[c];
+        r.bmax[c] = b[3 + c];
+    }
+    r.pad = 0;
+    if (N.leaf[v]) {
+        r.offset = poff;
+        r.n_prims = (uint16_t)N.count[v];
+        r.axis = 0;
+    } else {
+        r.offset = idx + 1 + N.osize[N.first[v]];
+        r.n_prims = 0;
+        r.axis = N.axis[v];
+    }
+    out[idx] = r;
+}
+
 template <class X>
 hipError_t alloc(X** p, size_t n) {
     return hipMalloc((void**)p, sizeof(X) * (n ? n : 1));
@@ -371,6 +573,109 @@ hipError_t build_lbvh(const float* positions, int32_t n, int32_t max_prims, std:
     for (void* p : {(void*)d_pos, (void*)d_keys, (void*)d_sorted, (void*)d_cb, (void*)d_child, (void*)d_range,
                     (void*)d_par_int, (void*)d_par_leaf, (void*)d_axis, (void*)d_ibox, (void*)d_lbox, (void*)d_isize,
                     (void*)d_flag, (void*)d_nodes, (void*)d_ord, d_tmp})
+        if (p) (void)hipFree(p);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (s) (void)hipStreamDestroy(s);
+    return e;
+}
+
+// PLOC tree (see the kernels above) in the SAH builder's node format: the
+// Morton keys of build_lbvh, then clustering rounds until one cluster is left
+// (the cluster count read back after each round), then the depth-first
+// emission.  positions on the host; gpu_ms as for build_lbvh.
+hipError_t build_ploc(const float* positions, int32_t n, int32_t max_prims, std::vector<ffh::BvhNode>& nodes,
+                      std::vector<int32_t>& ordered, float* gpu_ms) {
+    if (n <= 0) return hipErrorInvalidValue;
+    if (max_prims < 1) max_prims = 1;
+    if (max_prims > 65535) max_prims = 65535;
+    hipStream_t s = nullptr;
+    float *d_pos = nullptr, *d_box = nullptr, *d_cost = nullptr;
+    uint64_t *d_keys = nullptr, *d_sorted = nullptr;
+    unsigned long long *d_flags = nullptr, *d_scan = nullptr;
+    uint32_t* d_cb = nullptr;
+    int *d_first = nullptr, *d_second = nullptr, *d_parent = nullptr, *d_count = nullptr, *d_osize = nullptr;
+    int *d_C0 = nullptr, *d_C1 = nullptr, *d_nn = nullptr, *d_ord = nullptr;
+    uint8_t *d_axis = nullptr, *d_leaf = nullptr;
+    ffh::BvhNode* d_nodes = nullptr;
+    void *d_tmp = nullptr, *d_tmp2 = nullptr;
+    size_t m0 = (size_t)n, nn2 = 2 * m0 - 1, tmp_bytes = 0, tmp2_bytes = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipError_t e = hipStreamCreate(&s);
+    if (e == hipSuccess && gpu_ms) e = hipEventCreate(&ev0);
+    if (e == hipSuccess && gpu_ms) e = hipEventCreate(&ev1);
+    auto ok = [&](hipError_t x) { if (e == hipSuccess) e = x; return e == hipSuccess; };
+    if (ok(alloc(&d_pos, 9 * m0)) && ok(alloc(&d_keys, m0)) && ok(alloc(&d_sorted, m0)) && ok(alloc(&d_cb, 6)) &&
+        ok(alloc(&d_box, 6 * nn2)) && ok(alloc(&d_cost, nn2)) && ok(alloc(&d_first, nn2)) &&
+        ok(alloc(&d_second, nn2)) && ok(alloc(&d_parent, nn2)) && ok(alloc(&d_count, nn2)) &&
+        ok(alloc(&d_osize, nn2)) && ok(alloc(&d_axis, nn2)) && ok(alloc(&d_leaf, nn2)) && ok(alloc(&d_C0, m0)) &&
+        ok(alloc(&d_C1, m0)) && ok(alloc(&d_nn, m0)) && ok(alloc(&d_flags, m0)) && ok(alloc(&d_scan, m0)) &&
+        ok(alloc(&d_nodes, nn2)) && ok(alloc(&d_ord, m0)) &&
+        ok(hipMemcpyAsync(d_pos, positions, sizeof(float) * 9 * m0, hipMemcpyHostToDevice, s)) &&
+        ok(rocprim::radix_sort_keys(nullptr, tmp_bytes, d_keys, d_sorted, m0, 0, 62, s)) &&
+        ok(hipMalloc(&d_tmp, tmp_bytes ? tmp_bytes : 1)) &&
+        ok(rocprim::exclusive_scan(nullptr, tmp2_bytes, d_flags, d_scan, 0ull, m0, rocprim::plus<unsigned long long>(),
+                                   s)) &&
+        ok(hipMalloc(&d_tmp2, tmp2_bytes ? tmp2_bytes : 1))) {
+        uint32_t g = (uint32_t)((m0 + LB_BLOCK - 1) / LB_BLOCK);
+        uint32_t gb = g < 4096 ? g : 4096;
+        PNodes N{d_box, d_first, d_second, d_parent, d_count, d_cost, d_osize, d_axis, d_leaf, max_prims};
+        if (ev0) ok(hipEventRecord(ev0, s));
+        hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, s, d_cb);
+        hipLaunchKernelGGL(k_centroid_bounds, dim3(gb), dim3(LB_BLOCK), 0, s, d_pos, (uint32_t)n, d_cb);
+        hipLaunchKernelGGL(k_morton, dim3(gb), dim3(LB_BLOCK), 0, s, d_pos, (uint32_t)n, d_cb, d_keys);
+        if (ok(hipGetLastError()) && ok(rocprim::radix_sort_keys(d_tmp, tmp_bytes, d_keys, d_sorted, m0, 0, 62, s))) {
+            hipLaunchKernelGGL(k_ploc_leaves, dim3(g), dim3(LB_BLOCK), 0, s, d_sorted, d_pos, n, N, d_C0);
+            int m = n, base = n;
+            int *C = d_C0, *Cn = d_C1;
+            while (m > 1 && ok(hipGetLastError())) {
+                uint32_t gm = (uint32_t)((m + LB_BLOCK - 1) / LB_BLOCK);
+                hipLaunchKernelGGL(k_ploc_nn, dim3(gm), dim3(LB_BLOCK), 0, s, C, m, d_box, d_nn);
+                hipLaunchKernelGGL(k_ploc_flags, dim3(gm), dim3(LB_BLOCK), 0, s, d_nn, m, d_flags);
+                if (!ok(hipGetLastError()) ||
+                    !ok(rocprim::exclusive_scan(d_tmp2, tmp2_bytes, d_flags, d_scan, 0ull, (size_t)m,
+                                                rocprim::plus<unsigned long long>(), s)))
+                    break;
+                hipLaunchKernelGGL(k_ploc_merge, dim3(gm), dim3(LB_BLOCK), 0, s, C, d_nn, m, d_flags, d_scan, base, N,
+                                   Cn);
+                unsigned long long tail[2] = {0, 0};
+                if (!ok(hipGetLastError()) ||
+                    !ok(hipMemcpyAsync(&tail[0], d_scan + (m - 1), 8, hipMemcpyDeviceToHost, s)) ||
+                    !ok(hipMemcpyAsync(&tail[1], d_flags + (m - 1), 8, hipMemcpyDeviceToHost, s)) ||
+                    !ok(hipStreamSynchronize(s)))
+                    break;
+                unsigned long long tot = tail[0] + tail[1];
+                int made = (int)(tot >> 32), kept = (int)(tot & 0xffffffffull);
+                if (made < 1 || kept >= m) {  // a round always merges at least one pair
+                    e = hipErrorUnknown;
+                    break;
+                }
+                base += made;
+                m = kept;
+                int* t = C;
+                C = Cn;
+                Cn = t;
+            }
+            if (e == hipSuccess)
+                hipLaunchKernelGGL(k_ploc_emit, dim3((uint32_t)((nn2 + LB_BLOCK - 1) / LB_BLOCK)), dim3(LB_BLOCK), 0, s,
+                                   N, d_sorted, n, d_nodes, d_ord);
+            if (ev1) ok(hipEventRecord(ev1, s));
+            int32_t n_out = 1;
+            if (ok(hipGetLastError()))
+                ok(hipMemcpyAsync(&n_out, d_osize + (nn2 - 1), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            if (ok(hipStreamSynchronize(s))) {
+                if (ev0) ok(hipEventElapsedTime(gpu_ms, ev0, ev1));
+                nodes.resize((size_t)n_out);
+                ordered.resize(m0);
+                if (ok(hipMemcpy(nodes.data(), d_nodes, sizeof(ffh::BvhNode) * (size_t)n_out, hipMemcpyDeviceToHost)))
+                    ok(hipMemcpy(ordered.data(), d_ord, sizeof(int32_t) * m0, hipMemcpyDeviceToHost));
+            }
+        }
+    }
+    for (void* p : {(void*)d_pos, (void*)d_keys, (void*)d_sorted, (void*)d_cb, (void*)d_box, (void*)d_cost,
+                    (void*)d_first, (void*)d_second, (void*)d_parent, (void*)d_count, (void*)d_osize, (void*)d_axis,
+                    (void*)d_leaf, (void*)d_C0, (void*)d_C1, (void*)d_nn, (void*)d_flags, (void*)d_scan,
+                    (void*)d_nodes, (void*)d_ord, d_tmp, d_tmp2})
         if (p) (void)hipFree(p);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);

@@ -90,6 +90,14 @@ typedef struct {
  * the same triangles (the closest hit does not depend on the tree). */
 #define FF_BVH_SAH 0
 #define FF_BVH_LBVH 1
+/* FF_BVH_PLOC: built on the GPU by locally-ordered clustering of the
+ * Morton-sorted triangles (Meister and Bittner, IEEE TVCG 2018: each round,
+ * mutual nearest neighbours by union surface area within 16 positions
+ * merge), subtrees collapsed into leaves of at most max_prims_in_node
+ * triangles where the SAH cost (traversal 0.125 per node, 1 per triangle, as
+ * CRT/BVH.cpp:237-262 prices them) says so.  No reference counterpart; the
+ * SAH-quality alternative to FF_BVH_LBVH. */
+#define FF_BVH_PLOC 2
 
 /* dCamera (CRT/dCamera.cuh:9-22): inverse(proj * view), column-major. */
 typedef struct {
@@ -144,6 +152,13 @@ ff_status ff_context_destroy(ff_context *ctx);
 ff_status ff_bvh_build_gpu(ff_context *ctx, const float *positions, int32_t n_tris, int32_t max_prims,
                            ff_bvh_node *nodes_out, int32_t *ordered_out, int32_t *n_nodes_out,
                            int32_t *max_depth_out, float *build_ms_out);
+
+/* ---- PLOC BVH built on the GPU (the FF_BVH_PLOC scene builder) ----
+ * No reference counterpart (the reference's HLBVH, CRT/BVH.h:23, is never
+ * built).  Same arguments, node format and outputs as ff_bvh_build_gpu. */
+ff_status ff_bvh_build_ploc(ff_context *ctx, const float *positions, int32_t n_tris, int32_t max_prims,
+                            ff_bvh_node *nodes_out, int32_t *ordered_out, int32_t *n_nodes_out,
+                            int32_t *max_depth_out, float *build_ms_out);
 
 /* ---- host SAH BVH build (no GPU needed) ----
  * Replaces BVHAccel(info, SplitMethod::SAH, maxPrims) + init_BVH_triangle_info

@@ -621,6 +621,196 @@ int32_t or_lbvh_build(const float *pos, int32_t n, int32_t max_prims, or_node *n
     return n_out;
 }
 
+/* ---- PLOC tree (the product's FF_BVH_PLOC GPU builder, restated) ----
+ * No reference counterpart.  Steps as firefly_amd/csrc/ff_lbvh.hip
+ * (build_ploc): the Morton-sorted triangles of or_lbvh_build as the first
+ * clusters; each round every cluster's nearest neighbour within PL_R
+ * positions by the half surface area of the union (strictly smaller wins,
+ * so the lowest position on ties), mutual neighbours merged into a new node
+ * at the lower position (ids in position order), the survivors kept in
+ * order; a new node's split axis is the largest separation of its
+ * children's centres (lowest axis on ties), the lower child first, and it is
+ * a leaf when it holds at most max_prims triangles and its leaf cost
+ * count * A is at most 0.125 A + cost(lower-position child) + cost(other).
+ * Depth-first emission (first child at +1).  Pinned by tree invariants,
+ * traversal equal to brute force, SAH cost no worse than the linear BVH's,
+ * and node-for-node equality with the GPU tree. */
+#define PL_R 16
+static float pl_half_area(const float *b) {
+    float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+    return (dx * dy + dy * dz) + dz * dx;
+}
+static void pl_union(const float *a, const float *b, float *u) {
+    for (int c = 0; c < 3; c++) {
+        u[c] = lb_mn(a[c], b[c]);
+        u[3 + c] = lb_mx(a[3 + c], b[3 + c]);
+    }
+}
+typedef struct {
+    float *box, *cost;
+    int32_t *first, *second, *count;
+    uint8_t *axis, *leaf;
+    const uint64_t *keys;
+    or_node *out;
+    int32_t *ordered;
+} pl_ctx;
+/* emits node v at idx, its first triangle at slot *poff; returns its subtree size */
+static int pl_emit(const pl_ctx *P, int v, int idx, int32_t *poff) {
+    or_node *r = &P->out[idx];
+    const float *b = P->box + (size_t)v * 6;
+    for (int c = 0; c < 3; c++) {
+        r->bmin[c] = b[c];
+        r->bmax[c] = b[3 + c];
+    }
+    r->pad = 0;
+    if (P->leaf[v]) {
+        r->offset = *poff;
+        r->n_prims = (uint16_t)P->count[v];
+        r->axis = 0;
+        /* the subtree's triangles in depth-first order */
+        int *stack = (int *)malloc(sizeof(int) * (size_t)(P->count[v] + 1)), sp = 0;
+        stack[sp++] = v;
+        while (sp) {
+            int u = stack[--sp];
+            if (P->first[u] < 0) {
+                P->ordered[(*poff)++] = (int32_t)(P->keys[u] & 0xffffffffu);
+            } else {
+                stack[sp++] = P->second[u];
+                stack[sp++] = P->first[u];
+            }
+        }
+        free(stack);
+        return 1;
+    }
+    int s0 = pl_emit(P, P->first[v], idx + 1, poff);
+    int s1 = pl_emit(P, P->second[v], idx + 1 + s0, poff);
+    r->offset = idx + 1 + s0;
+    r->n_prims = 0;
+    r->axis = P->axis[v];
+    return 1 + s0 + s1;
+}
+int32_t or_ploc_build(const float *pos, int32_t n, int32_t max_prims, or_node *nodes_out,
+                      int32_t *ordered_out) {
+    if (n <= 0) return 0;
+    if (max_prims < 1) max_prims = 1;
+    /* Morton keys and order exactly as or_lbvh_build */
+    uint32_t kmin[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, kmax[3] = {0, 0, 0};
+    for (int32_t i = 0; i < n; i++) {
+        float lo[3], hi[3];
+        lb_tri_box(pos, i, lo, hi);
+        for (int a = 0; a < 3; a++) {
+            uint32_t k = lb_f2key(0.5f * lo[a] + 0.5f * hi[a]);
+            if (k < kmin[a]) kmin[a] = k;
+            if (k > kmax[a]) kmax[a] = k;
+        }
+    }
+    float clo[3], ext[3];
+    for (int a = 0; a < 3; a++) {
+        clo[a] = lb_key2f(kmin[a]);
+        ext[a] = lb_key2f(kmax[a]) - clo[a];
+    }
+    uint64_t *keys = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)n);
+    for (int32_t i = 0; i < n; i++) {
+        float lo[3], hi[3];
+        lb_tri_box(pos, i, lo, hi);
+        uint32_t q[3];
+        for (int a = 0; a < 3; a++) q[a] = lb_quant10(0.5f * lo[a] + 0.5f * hi[a], clo[a], ext[a]);
+        uint32_t code = (lb_spread10(q[0]) << 2) | (lb_spread10(q[1]) << 1) | lb_spread10(q[2]);
+        keys[i] = ((uint64_t)code << 32) | (uint32_t)i;
+    }
+    qsort(keys, (size_t)n, sizeof(uint64_t), lb_cmp_u64);
+    size_t nn2 = 2 * (size_t)n - 1;
+    pl_ctx P;
+    P.box = (float *)malloc(sizeof(float) * 6 * nn2);
+    P.cost = (float *)malloc(sizeof(float) * nn2);
+    P.first = (int32_t *)malloc(4 * nn2);
+    P.second = (int32_t *)malloc(4 * nn2);
+    P.count = (int32_t *)malloc(4 * nn2);
+    P.axis = (uint8_t *)malloc(nn2);
+    P.leaf = (uint8_t *)malloc(nn2);
+    P.keys = keys;
+    P.out = nodes_out;
+    P.ordered = ordered_out;
+    int32_t *C = (int32_t *)malloc(4 * (size_t)n), *Cn = (int32_t *)malloc(4 * (size_t)n);
+    int32_t *nnb = (int32_t *)malloc(4 * (size_t)n);
+    for (int32_t i = 0; i < n; i++) {
+        float *b = P.box + (size_t)i * 6;
+        lb_tri_box(pos, (int32_t)(keys[i] & 0xffffffffu), b, b + 3);
+        P.first[i] = P.second[i] = -1;
+        P.count[i] = 1;
+        P.cost[i] = pl_half_area(b);
+        P.axis[i] = 0;
+        P.leaf[i] = 1;
+        C[i] = i;
+    }
+    int32_t m = n, id = n;
+    while (m > 1) {
+        for (int32_t i = 0; i < m; i++) {
+            const float *a = P.box + (size_t)C[i] * 6;
+            float best = 0.f;
+            int32_t bj = -1;
+            int32_t j0 = i - PL_R < 0 ? 0 : i - PL_R, j1 = i + PL_R > m - 1 ? m - 1 : i + PL_R;
+            for (int32_t j = j0; j <= j1; j++) {
+                if (j == i) continue;
+                float u[6];
+                pl_union(a, P.box + (size_t)C[j] * 6, u);
+                float ar = pl_half_area(u);
+                if (bj < 0 || ar < best) {
+                    best = ar;
+                    bj = j;
+                }
+            }
+            nnb[i] = bj;
+        }
+        int32_t k = 0;
+        for (int32_t i = 0; i < m; i++) {
+            int32_t j = nnb[i];
+            int mutual = nnb[j] == i;
+            if (mutual && i > j) continue;
+            if (!mutual) {
+                Cn[k++] = C[i];
+                continue;
+            }
+            int32_t a = C[i], b = C[j];
+            const float *ba = P.box + (size_t)a * 6, *bb = P.box + (size_t)b * 6;
+            float *u = P.box + (size_t)id * 6;
+            pl_union(ba, bb, u);
+            int ax = 0;
+            float sep = 0.f, sgn = 0.f;
+            for (int c = 0; c < 3; c++) {
+                float d = (bb[c] + bb[3 + c]) - (ba[c] + ba[3 + c]);
+                float ad = d < 0.f ? -d : d;
+                if (c == 0 || ad > sep) {
+                    sep = ad;
+                    sgn = d;
+                    ax = c;
+                }
+            }
+            P.first[id] = sgn < 0.f ? b : a;
+            P.second[id] = sgn < 0.f ? a : b;
+            int32_t cnt = P.count[a] + P.count[b];
+            float A = pl_half_area(u);
+            float c_split = (0.125f * A + P.cost[a]) + P.cost[b];
+            float c_leaf = (float)cnt * A;
+            int collapse = cnt <= max_prims && c_leaf <= c_split;
+            P.count[id] = cnt;
+            P.cost[id] = collapse ? c_leaf : c_split;
+            P.leaf[id] = (uint8_t)collapse;
+            P.axis[id] = (uint8_t)ax;
+            Cn[k++] = id++;
+        }
+        int32_t *t = C;
+        C = Cn;
+        Cn = t;
+        m = k;
+    }
+    int32_t poff = 0;
+    int32_t n_out = pl_emit(&P, C[0], 0, &poff);
+    free(keys); free(P.box); free(P.cost); free(P.first); free(P.second); free(P.count);
+    free(P.axis); free(P.leaf); free(C); free(Cn); free(nnb);
+    return n_out;
+}
+
 int32_t or_bvh_max_depth(const or_node *nodes, int32_t n_nodes) {
     return n_nodes > 0 ? depth_rec(nodes, 0) : 0;
 }

@@ -89,6 +89,9 @@ int32_t or_bvh_max_depth(const or_node *nodes, int32_t n_nodes);
 /* linear BVH, as the GPU builder makes it (Morton order, Karras hierarchy) */
 int32_t or_lbvh_build(const float *pos, int32_t n, int32_t max_prims,
                       or_node *nodes_out, int32_t *ordered_out);
+/* the GPU PLOC builder restated (no reference counterpart); same outputs */
+int32_t or_ploc_build(const float *pos, int32_t n, int32_t max_prims,
+                      or_node *nodes_out, int32_t *ordered_out);
 
 /* ---- traversal: rays are [n][6] (o.xyz, d.xyz); tris are [slot][9] (v0,v1,v2) ---- */
 void or_trace_closest(const or_node *nodes, const float *tris, const float *rays,

@@ -56,6 +56,7 @@ def lib() -> C.CDLL:
             "or_bvh_build": (C.c_int32, [_P, C.c_int32, C.c_int32, _P, _P]),
             "or_bvh_max_depth": (C.c_int32, [_P, C.c_int32]),
             "or_lbvh_build": (C.c_int32, [_P, C.c_int32, C.c_int32, _P, _P]),
+            "or_ploc_build": (C.c_int32, [_P, C.c_int32, C.c_int32, _P, _P]),
             "or_trace_closest": (None, [_P, _P, _P, C.c_int32, C.c_int32, _P, _P, _P, _P]),
             "or_trace_closest_brute": (None, [C.c_int32, _P, _P, C.c_int32, C.c_int32, _P, _P]),
             "or_trace_any": (None, [_P, _P, _P, C.c_int32, _P]),
@@ -116,6 +117,16 @@ def lbvh_build(positions: np.ndarray, max_prims: int = 8):
     nodes = np.zeros(max(2 * n - 1, 1), NODE_DTYPE)
     ordered = np.zeros(n, np.int32)
     nn = lib().or_lbvh_build(pos.ctypes.data, n, max_prims, nodes.ctypes.data, ordered.ctypes.data)
+    return nodes[:nn].copy(), ordered
+
+
+def ploc_build(positions, max_prims: int = 8):
+    """The GPU PLOC builder restated (same node format as bvh_build)."""
+    pos = f32(positions).reshape(-1, 9)
+    n = pos.shape[0]
+    nodes = np.zeros(max(2 * n - 1, 1), NODE_DTYPE)
+    ordered = np.zeros(n, np.int32)
+    nn = lib().or_ploc_build(pos.ctypes.data, n, max_prims, nodes.ctypes.data, ordered.ctypes.data)
     return nodes[:nn].copy(), ordered
 
 

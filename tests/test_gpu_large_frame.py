@@ -53,6 +53,17 @@ def test_full_mesh_bvh_equals_oracle(big, oracle):
     assert np.array_equal(ordered, o_o)
 
 
+def test_full_mesh_ploc_tree_equals_oracle(big, gpu, oracle):
+    """The clustering build of the 1.31 M-triangle mesh on the GPU equals the
+    oracle's node for node."""
+    sd, _, _ = big
+    nodes, ordered, depth, ms = abi.bvh_build_gpu(gpu, sd.positions, 8, builder=abi.FF_BVH_PLOC)
+    n_o, o_o = oracle.ploc_build(sd.positions)
+    assert len(nodes) == len(n_o) and nodes.tobytes() == n_o.tobytes()
+    assert np.array_equal(ordered, o_o)
+    print(f"PLOC build of {sd.n_tris} triangles: {ms:.2f} ms of kernels, depth {depth}")
+
+
 def test_camera_and_bounce_hits_bit_exact(big, gpu, oracle):
     import torch
     sd, sc, ps = big

@@ -1,6 +1,7 @@
-"""The GPU linear-BVH builder (FF_BVH_LBVH): node for node the oracle's
-restatement of it, traced and rendered with the same results as the
-reference's SAH tree (the closest hit does not depend on the tree)."""
+"""The GPU BVH builders, linear (FF_BVH_LBVH) and clustering (FF_BVH_PLOC):
+node for node the oracle's restatements of them, traced and rendered with
+the same results as the reference's SAH tree (the closest hit does not
+depend on the tree)."""
 import numpy as np
 import pytest
 
@@ -116,3 +117,62 @@ def test_unknown_builder_is_rejected(gpu):
     with pytest.raises(abi.FFError) as e:
         _scene(gpu, _soup(10, 1), builder=7)
     assert e.value.status == 1
+
+
+# ---- the clustering builder (FF_BVH_PLOC) ----
+@pytest.mark.parametrize("name,max_prims", [("one", 8), ("two", 8), ("nine", 8), ("soup20k", 8), ("soup20k", 1),
+                                            ("soup20k", 4), ("identical", 4), ("icosphere6", 8), ("strip", 2)])
+def test_gpu_ploc_tree_equals_oracle(gpu, oracle, name, max_prims):
+    """Node for node (bytes) and slot for slot the oracle's restatement of
+    the clustering build."""
+    if name == "icosphere6":
+        pos = scenes.icosphere_scene(width=8, height=8, subdiv=6).positions
+    elif name == "identical":
+        tri = np.array([[-1, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+        pos = np.repeat(tri[None], 37, axis=0)
+    elif name == "strip":  # many rounds with few merges each
+        x = np.arange(3000, dtype=np.float32) ** 1.5
+        pos = np.zeros((3000, 3, 3), np.float32)
+        pos[:, 0, 0], pos[:, 1, 0], pos[:, 2, 0], pos[:, 2, 1] = x, x + 0.01, x, 0.01
+    else:
+        pos = _soup({"one": 1, "two": 2, "nine": 9, "soup20k": 20000}[name], 8)
+    print(f"ploc {name} max_prims {max_prims}", flush=True)
+    sc = _scene(gpu, pos, max_prims, builder=abi.FF_BVH_PLOC)
+    nodes, ordered = sc.bvh()
+    want_nodes, want_ordered = oracle.ploc_build(pos, max_prims)
+    assert np.array_equal(ordered, want_ordered)
+    assert len(nodes) == len(want_nodes)
+    assert nodes.tobytes() == want_nodes.tobytes()
+    sc.close()
+
+
+def test_standalone_ploc_build_equals_oracle(gpu, oracle):
+    """ff_bvh_build_ploc returns the tree ff_scene_create builds for
+    FF_BVH_PLOC, with the device time of its kernels."""
+    pos = _soup(50000, 12)
+    nodes, ordered, depth, ms = abi.bvh_build_gpu(gpu, pos, 8, builder=abi.FF_BVH_PLOC)
+    want_nodes, want_ordered = oracle.ploc_build(pos, 8)
+    assert nodes.tobytes() == want_nodes.tobytes() and np.array_equal(ordered, want_ordered)
+    assert depth == oracle.bvh_max_depth(want_nodes) and ms > 0.0
+
+
+def test_ploc_render_matches_sah_render(gpu):
+    """A frame rendered through the clustered tree equals the SAH tree's
+    frame up to rays whose closest hit is a tie between triangles."""
+    import torch
+    sd = scenes.spheres_scene(width=96, height=64, spp=4, max_depth=4)
+    films = []
+    for b in (abi.FF_BVH_SAH, abi.FF_BVH_PLOC):
+        sc = abi.Scene(gpu, sd.positions, sd.normals, sd.uvs, sd.material_ids, sd.materials, builder=b)
+        env = abi.EnvMap(gpu, sd.env)
+        f = torch.zeros((sd.height, sd.width, 4), dtype=torch.float32, device="cuda")
+        abi.render(gpu, sc, env, abi.make_camera(sd.inv_vp), sd.width, sd.height, sd.spp, sd.max_depth, sd.seed, f)
+        torch.cuda.synchronize()
+        films.append(f.cpu().numpy())
+        sc.close()
+        env.close()
+    a, b = films
+    same = np.mean(np.all(a == b, axis=-1))
+    err = float(np.sqrt(np.mean((a[..., :3] - b[..., :3]) ** 2)) / np.sqrt(np.mean(b[..., :3] ** 2)))
+    print(f"PLOC vs SAH film: identical pixels {same:.4f}, rel RMSE {err:.2e}")
+    assert same > 0.99 and err < 1e-3

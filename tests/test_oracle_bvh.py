@@ -171,3 +171,112 @@ def test_lbvh_traversal_equals_brute_force(oracle):
     assert np.array_equal(t[hit].view(np.uint32), tb[hit].view(np.uint32))
     assert np.mean(i[hit] == ib[hit]) > 0.999
     assert np.array_equal(oracle.trace_any(nodes, tris, rays), oracle.trace_any_brute(tris, rays))
+
+
+# ---- PLOC (locally-ordered clustering, the FF_BVH_PLOC GPU builder) ----
+def sah_cost(nodes):
+    """SAH cost of a tree in the builders' units, relative to the root's
+    area: 0.125 per interior node and 1 per leaf triangle, each weighted by
+    its box's half area (CRT/BVH.cpp:237-262 prices splits the same way)."""
+    lo, hi = nodes["bmin"].astype(np.float64), nodes["bmax"].astype(np.float64)
+    d = hi - lo
+    a = d[:, 0] * d[:, 1] + d[:, 1] * d[:, 2] + d[:, 2] * d[:, 0]
+    leaf = nodes["n_prims"] > 0
+    return float((np.where(leaf, nodes["n_prims"] * a, 0.125 * a)).sum() / a[0])
+
+
+@pytest.mark.parametrize("seed,n,max_prims", [(0, 1, 8), (1, 2, 8), (2, 5, 8), (3, 9, 8), (4, 1000, 8),
+                                              (5, 20000, 4), (6, 3000, 1), (7, 5000, 16)])
+def test_ploc_invariants(oracle, seed, n, max_prims):
+    """Every triangle once, boxes exact unions of their children, pre-order
+    layout, leaves of at most max_prims triangles."""
+    pos = _soup(n, seed)
+    nodes, ordered = oracle.ploc_build(pos, max_prims)
+    _check_invariants(nodes, ordered, pos, max_prims)
+    assert len(nodes) <= 2 * n - 1 and _leaf_sizes(nodes).max() <= max_prims
+    assert _leaf_sizes(nodes).sum() == n
+
+
+def test_ploc_traversal_equals_brute_force(oracle):
+    sd = scenes.spheres_scene(segments=16, rings=8)
+    nodes, ordered = oracle.ploc_build(sd.positions)
+    tris = oracle.ordered_tris(sd.positions, ordered)
+    rays = scenes.random_rays(4000, 22)
+    t, u, v, i = oracle.trace_closest(nodes, tris, rays)
+    tb, ib = oracle.trace_closest_brute(tris, rays)
+    hit = ib >= 0
+    assert np.array_equal(i >= 0, hit)
+    assert np.array_equal(t[hit].view(np.uint32), tb[hit].view(np.uint32))
+    assert np.mean(i[hit] == ib[hit]) > 0.999
+    assert np.array_equal(oracle.trace_any(nodes, tris, rays), oracle.trace_any_brute(tris, rays))
+
+
+@pytest.mark.parametrize("name", ["soup20k", "icosphere6", "spheres"])
+def test_ploc_tree_quality(oracle, name):
+    """The clustered tree's SAH cost is below the linear BVH's and within
+    10% of the reference's top-down SAH build on the same triangles."""
+    if name == "soup20k":
+        pos = _soup(20000, 9)
+    elif name == "icosphere6":
+        pos = scenes.icosphere_scene(width=8, height=8, subdiv=6).positions
+    else:
+        pos = scenes.spheres_scene().positions
+    c_ploc = sah_cost(oracle.ploc_build(pos)[0])
+    c_lbvh = sah_cost(oracle.lbvh_build(pos)[0])
+    c_sah = sah_cost(oracle.bvh_build(pos)[0])
+    print(f"{name}: SAH cost ploc {c_ploc:.2f}, lbvh {c_lbvh:.2f}, sah {c_sah:.2f}")
+    assert c_ploc < c_lbvh
+    assert c_ploc <= 1.10 * c_sah
+
+
+def _strip(xs, size=0.01):
+    """one small triangle centred at each x on the x axis"""
+    pos = np.zeros((len(xs), 3, 3), np.float32)
+    for k, x in enumerate(xs):
+        pos[k] = [[x - size, 0, 0], [x + size, 0, 0], [x, size, 0]]
+    return pos
+
+
+def test_ploc_merges_nearest_pairs_first(oracle):
+    """Triangles at x = 0, 1, 10, 11, 30, 31: mutual nearest neighbours pair
+    up first ((0,1), (10,11), (30,31)), then the two closest pairs; the
+    lower child on the split axis comes first, and with one triangle per
+    leaf every leaf holds one."""
+    xs = [30, 0, 11, 1, 31, 10]
+    nodes, ordered = oracle.ploc_build(_strip(xs), 1)
+
+    def leaves(i):
+        nd = nodes[i]
+        if nd["n_prims"] > 0:
+            return [xs[ordered[nd["offset"]]]]
+        return leaves(i + 1) + leaves(nd["offset"])
+
+    def shape(i):
+        nd = nodes[i]
+        if nd["n_prims"] > 0:
+            return xs[ordered[nd["offset"]]]
+        return (shape(i + 1), shape(nd["offset"]))
+    assert shape(0) == (((0, 1), (10, 11)), (30, 31))
+    assert leaves(0) == sorted(xs)
+    assert all(nodes[nodes["n_prims"] == 0]["axis"] == 0)  # separated along x
+
+
+def test_ploc_collapses_clusters_into_leaves(oracle):
+    """Eight tight clusters of four triangles far apart: each cluster is one
+    leaf (its leaf cost 4A is below any split of it), the clusters are
+    interior nodes above them."""
+    xs = [c * 10 + k * 0.001 for c in range(8) for k in range(4)]
+    nodes, ordered = oracle.ploc_build(_strip(xs, 0.05), 8)
+    sizes = sorted(_leaf_sizes(nodes).tolist())
+    assert sizes == [4] * 8
+    for nd in nodes[nodes["n_prims"] > 0]:
+        members = {round(xs[ordered[nd["offset"] + k]] // 10) for k in range(nd["n_prims"])}
+        assert len(members) == 1
+
+
+def test_ploc_identical_triangles(oracle):
+    tri = np.array([[-1, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    pos = np.repeat(tri[None], 37, axis=0)
+    nodes, ordered = oracle.ploc_build(pos, 4)
+    _check_invariants(nodes, ordered, pos, 4)
+    assert _leaf_sizes(nodes).max() <= 4 and sorted(ordered.tolist()) == list(range(37))
