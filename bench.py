@@ -753,7 +753,7 @@ def run_frame(args) -> dict:
 
     keys = ("rays_primary", "rays_extension", "rays_shadow", "paths", "ext_nodes", "ext_tris",
             "shadow_nodes", "shadow_tris", "ms_extend", "ms_shadow", "ms_logic", "ms_shade", "ms_other",
-            "launches_extend", "launches_shadow")
+            "launches_extend", "launches_shadow", "rays_shadow_gated")
     tot = dict.fromkeys(keys, 0.0)
     torch.cuda.synchronize()
     t_start = time.perf_counter()
@@ -844,6 +844,9 @@ def run_frame(args) -> dict:
         "spp_per_s_1080p": round(spp_s, 3),
         "total_mrays_s": round(total_rays / elapsed / 1e6, 3),
         "rays_per_step": {k: int(tot[k] / args.steps) for k in ("rays_primary", "rays_extension", "rays_shadow")},
+        # BRDF-sample visibility rays left untraced because their vertex's light
+        # ray was occluded (no pixel depends on them); not counted in value
+        "rays_shadow_gated_per_step": int(tot["rays_shadow_gated"] / args.steps),
         "stage_ms_per_step": stages,
         "traversal": per_ray,
         "envmap_update_ms": round(env_ms, 4),

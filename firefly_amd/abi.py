@@ -64,7 +64,7 @@ class RayStats(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "rays_primary", "rays_extension", "rays_shadow", "paths", "ext_nodes", "ext_tris",
         "shadow_nodes", "shadow_tris", "ms_extend", "ms_shadow", "ms_logic", "ms_shade", "ms_other",
-        "launches_extend", "launches_shadow")]
+        "launches_extend", "launches_shadow", "rays_shadow_gated")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -205,7 +205,7 @@ class Context:
 
 class Scene:
     """Device scene built from host arrays (positions/normals/uvs [n,3,...]); the BVH by the
-    reference's host SAH (FF_BVH_SAH) or on the GPU as a linear BVH (FF_BVH_LBVH)."""
+    reference's host SAH (FF_BVH_SAH), or on the GPU as a linear BVH (FF_BVH_LBVH) or by clustering (FF_BVH_PLOC)."""
 
     def __init__(self, ctx: Context, positions, normals, uvs, material_ids, materials,
                  textures: Sequence[np.ndarray] = (), max_prims: int = 8, builder: int = FF_BVH_SAH,

@@ -174,7 +174,7 @@ ff_status ensure_paths(ff_context* c, size_t P) {
     HIP_TRY(dalloc(&p.q_ext[0], P));
     HIP_TRY(dalloc(&p.q_ext[1], P));
     for (int m = 0; m < 4; m++) HIP_TRY(dalloc(&p.q_mat[m], P));
-    HIP_TRY(dalloc(&p.q_shadow, 3 * P));
+    HIP_TRY(dalloc(&p.q_shadow, P));
     HIP_TRY(dalloc(&p.layers, P));
     HIP_TRY(dalloc(&p.dlt, P));
     c->paths.cap = P;
@@ -1201,6 +1201,7 @@ ff_status ff_last_render_stats(ff_context* ctx, ff_ray_stats* out) {
     out->ext_tris = (double)h[S_EXT_TRIS];
     out->shadow_nodes = (double)h[S_SHD_NODES];
     out->shadow_tris = (double)h[S_SHD_TRIS];
+    out->rays_shadow_gated = (double)h[S_SHD_GATED];
     double* ms[5] = {&out->ms_extend, &out->ms_shadow, &out->ms_logic, &out->ms_shade, &out->ms_other};
     for (const StageRecord& rec : ctx->records) {
         float t = 0.f;

@@ -140,6 +140,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_generate(ff_paths P) {
 template <bool UNI>
 struct ExtSrc {  // wf_extend (CRT/kernel.cu:487-523): closest hit of every queued path
     static constexpr bool kUniform = UNI;
+    static constexpr bool kChain = false;
     ff_paths P;
     const uint32_t* queue;
     __device__ __forceinline__ void scene0(Bvh& B, const float*& tris) const {
@@ -147,8 +148,8 @@ struct ExtSrc {  // wf_extend (CRT/kernel.cu:487-523): closest hit of every queu
         B = bvh_of(S);
         tris = S.tris;
     }
-    __device__ __forceinline__ void load(uint32_t k, uint32_t& id, Bvh& B, const float*& tris, f3& o, f3& d,
-                                         bool& two) const {
+    __device__ __forceinline__ void load(uint32_t k, uint32_t& id, uint32_t&, Bvh& B, const float*& tris, f3& o,
+                                         f3& d, bool& two) const {
         id = queue[k];
         if (!UNI) {
             const ff_dev_scene& S = P.reqs[P.preq[id]].S;
@@ -164,34 +165,67 @@ struct ExtSrc {  // wf_extend (CRT/kernel.cu:487-523): closest hit of every queu
     __device__ __forceinline__ void hit(uint32_t id, const Hit& h) const {
         P.hit[id] = make_float4(h.t, h.u, h.v, __int_as_float(h.id));
     }
-    __device__ __forceinline__ void occluded(uint32_t, bool) const {}
+    __device__ __forceinline__ void occluded(uint32_t, uint32_t, bool) const {}
+    __device__ __forceinline__ bool advance(uint32_t&, bool, uint32_t&) const { return false; }
+    __device__ __forceinline__ void load_next(uint32_t, uint32_t, f3&, f3&) const {}
+    __device__ __forceinline__ void count(uint32_t, uint32_t) const {}
 };
+// A shadow-queue entry is one shading event, path << 2 | needS | needD << 1:
+// its light ray first, then -- only when the light ray is unoccluded, since
+// its visibility gates the whole NEE sum -- the BRDF-sample visibility rays
+// (spec, then diffuse) in the same lane.  aux = ray (0 light, 1 spec,
+// 2 diffuse) | rays still to trace << 2.  The reference traces the BRDF-sample
+// rays inside BRDF_L either way (CRT/dMaterial.cu:404-448); where the light
+// ray is occluded their results are never used, so no pixel changes.
 template <bool UNI>
 struct ShadowSrc {  // wf_shadow (CRT/kernel.cu:525-543): any hit of every queued shadow ray
     static constexpr bool kUniform = UNI;
+    static constexpr bool kChain = true;
     ff_paths P;
     __device__ __forceinline__ void scene0(Bvh& B, const float*& tris) const {
         const ff_dev_scene& S = P.reqs[0].S;
         B = bvh_of(S);
         tris = S.tris;
     }
-    __device__ __forceinline__ void load(uint32_t k, uint32_t& s, Bvh& B, const float*& tris, f3& o, f3& d,
-                                         bool& two) const {
-        s = P.q_shadow[k];
+    __device__ __forceinline__ void load(uint32_t k, uint32_t& id, uint32_t& aux, Bvh& B, const float*& tris, f3& o,
+                                         f3& d, bool& two) const {
+        uint32_t e = P.q_shadow[k];
+        id = e >> 2;
+        aux = (e & 3u) << 2;
         if (!UNI) {
-            const ff_dev_scene& S = P.reqs[P.preq[s / 3]].S;
+            const ff_dev_scene& S = P.reqs[P.preq[id]].S;
             B = bvh_of(S);
             tris = S.tris;
         }
-        o = xyz(P.sray[2 * s]);
-        d = xyz(P.sray[2 * s + 1]);
+        o = xyz(P.sray[6 * id]);
+        d = xyz(P.sray[6 * id + 1]);
         two = false;
     }
     __device__ __forceinline__ void hit(uint32_t, const Hit&) const {}
-    __device__ __forceinline__ void occluded(uint32_t s, bool occ) const { P.vis[s] = occ ? 0 : 1; }
+    __device__ __forceinline__ void occluded(uint32_t id, uint32_t aux, bool occ) const {
+        P.vis[3 * id + (aux & 3u)] = occ ? 0 : 1;
+    }
+    // after a ray with result occ: whether the path has another ray (aux then names it)
+    __device__ __forceinline__ bool advance(uint32_t& aux, bool occ, uint32_t& gated) const {
+        uint32_t cur = aux & 3u, rest = aux >> 2;
+        if (cur == 0u && occ) gated += (uint32_t)__popc(rest);
+        if ((cur == 0u && occ) || rest == 0u) return false;
+        uint32_t nxt = (rest & 1u) ? 1u : 2u;
+        aux = nxt | ((rest & ~nxt) << 2);
+        return true;
+    }
+    __device__ __forceinline__ void load_next(uint32_t id, uint32_t aux, f3& o, f3& d) const {
+        o = xyz(P.sray[6 * id + 2 * (aux & 3u)]);
+        d = xyz(P.sray[6 * id + 2 * (aux & 3u) + 1]);
+    }
+    __device__ __forceinline__ void count(uint32_t rays, uint32_t gated) const {
+        wave_sum_add(stats_rep(P) + S_SHADOW, rays);
+        wave_sum_add(stats_rep(P) + S_SHD_GATED, gated);
+    }
 };
 struct RaySrc {  // a plain ray array (ff_trace_closest / ff_trace_any)
     static constexpr bool kUniform = true;
+    static constexpr bool kChain = false;
     ff_dev_scene S;
     const float4* rays;
     float4* hits;
@@ -201,7 +235,7 @@ struct RaySrc {  // a plain ray array (ff_trace_closest / ff_trace_any)
         B = bvh_of(S);
         tris = S.tris;
     }
-    __device__ __forceinline__ void load(uint32_t k, uint32_t& id, Bvh&, const float*&, f3& o, f3& d,
+    __device__ __forceinline__ void load(uint32_t k, uint32_t& id, uint32_t&, Bvh&, const float*&, f3& o, f3& d,
                                          bool& two) const {
         id = k;
         o = xyz(rays[2 * k]);
@@ -211,7 +245,10 @@ struct RaySrc {  // a plain ray array (ff_trace_closest / ff_trace_any)
     __device__ __forceinline__ void hit(uint32_t id, const Hit& h) const {
         hits[id] = make_float4(h.t, h.u, h.v, __int_as_float(h.id));
     }
-    __device__ __forceinline__ void occluded(uint32_t id, bool o) const { occ[id] = o ? 1 : 0; }
+    __device__ __forceinline__ void occluded(uint32_t id, uint32_t, bool o) const { occ[id] = o ? 1 : 0; }
+    __device__ __forceinline__ bool advance(uint32_t&, bool, uint32_t&) const { return false; }
+    __device__ __forceinline__ void load_next(uint32_t, uint32_t, f3&, f3&) const {}
+    __device__ __forceinline__ void count(uint32_t, uint32_t) const {}
 };
 
 // Persistent traversal.  The grid holds only the lanes the chip keeps
@@ -252,7 +289,7 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
     st.lds_entries = LDS;
     st.ovf = (glb_int*)(ovf + gtid());
     st.ovf_stride = FF_TRAV_THREADS;
-    uint32_t cn = 0, ct = 0, id = 0;
+    uint32_t cn = 0, ct = 0, id = 0, aux = 0, c_rays = 0, c_gated = 0;
     bool busy = false, more = true, two = false;
     bool walk = false;  // WW >= 2: the lane has records or leaves left to visit
     int pend = 0;       // WW >= 2: postponed leaf (0 = none; leaf refs are negative)
@@ -263,8 +300,30 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
     if (Src::kUniform) src.scene0(B, tris);
     TravRay R;
     Hit h = no_hit();
+    // an any-hit ray is finished with result occ: the result is written and,
+    // for a source that chains rays, `nxt` marks that the lane's path has
+    // another ray, which the lane starts at the top of the next iteration
+    // instead of taking a queue entry
+    bool nxt = false;
+    auto end_any = [&](bool occ) {
+        src.occluded(id, aux, occ);
+        nxt = Src::kChain && src.advance(aux, occ, c_gated);
+    };
     for (;;) {
-        uint64_t idle = __ballot(!busy);
+        if (Src::kChain && __ballot(nxt) != 0) {
+            if (nxt) {
+                f3 o, d;
+                src.load_next(id, aux, o, d);
+                c_rays++;
+                busy = trav_begin(B, o, d, R, st, cn);
+                walk = busy;
+                pend = 0;
+                pend2 = 0;
+                nxt = false;
+                if (!busy) end_any(false);
+            }
+        }
+        uint64_t idle = __ballot(!busy && !nxt);
         uint32_t n_idle = (uint32_t)__popcll(idle);
         if (more && n_idle >= (uint32_t)REFILL) {
             uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
@@ -273,21 +332,22 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
             bool done;
             uint32_t first = F.take_chunked(n_idle, got, done);
             if (done) more = false;
-            if (!busy && rank < got) {
+            if (!busy && !nxt && rank < got) {
                 f3 o, d;
-                src.load(first + rank, id, B, tris, o, d, two);
+                src.load(first + rank, id, aux, B, tris, o, d, two);
+                c_rays++;
                 h = no_hit();
                 busy = trav_begin(B, o, d, R, st, cn);
                 walk = busy;
                 pend = 0;
                 pend2 = 0;
                 if (!busy) {
-                    if (ANY) src.occluded(id, false);
+                    if (ANY) end_any(false);
                     else src.hit(id, h);
                 }
             }
         }
-        if (__ballot(busy) == 0) {
+        if (__ballot(busy || nxt) == 0) {
             if (!more) break;
             continue;
         }
@@ -322,7 +382,7 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
                     else R.ref = st.pop();
                 }
                 if (occ || !walk) {
-                    if (ANY) src.occluded(id, occ);
+                    if (ANY) end_any(occ);
                     else src.hit(id, h);
                     busy = false;
                 }
@@ -358,7 +418,7 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
                 }
             }
             if (busy && (occ || (!walk && pend == 0))) {
-                if (ANY) src.occluded(id, occ);
+                if (ANY) end_any(occ);
                 else src.hit(id, h);
                 busy = false;
             }
@@ -368,7 +428,7 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
                 if (__ballot(at_int) == 0) break;
                 if (at_int && !visit_node<WIDE, !ANY>(B, R, prune_lim(h.t), st, cn)) {
                     if (st.sp == 0) {
-                        if (ANY) src.occluded(id, false);
+                        if (ANY) end_any(false);
                         else src.hit(id, h);
                         busy = false;
                     } else {
@@ -377,24 +437,29 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
                 }
             }
             if (busy) {  // at a leaf
-                bool done;
                 if (ANY) {
                     bool occ = leaf_any(B, tris, R, ct);
-                    done = occ || st.sp == 0;
-                    if (done) src.occluded(id, occ);
+                    if (occ || st.sp == 0) {
+                        end_any(occ);
+                        busy = false;
+                    } else {
+                        R.ref = st.pop();
+                    }
                 } else {
                     leaf_closest(B, tris, R, h, two, ct);
-                    done = st.sp == 0;
-                    if (done) src.hit(id, h);
+                    if (st.sp == 0) {
+                        src.hit(id, h);
+                        busy = false;
+                    } else {
+                        R.ref = st.pop();
+                    }
                 }
-                if (done) busy = false;
-                else R.ref = st.pop();
             }
         } else if (busy) {
             if (ANY) {
                 int r = any_step<WIDE>(B, tris, R, st, cn, ct);
                 if (r != 0) {
-                    src.occluded(id, r == 1);
+                    end_any(r == 1);
                     busy = false;
                 }
             } else if (!closest_step<WIDE>(B, tris, R, h, two, st, cn, ct)) {
@@ -407,6 +472,7 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
         wave_sum_add(&cnt[0], cn);
         wave_sum_add(&cnt[1], ct);
     }
+    src.count(c_rays, c_gated);
 }
 
 // The one-scene extend kernel of the if-if shape fits 7 waves per SIMD
@@ -887,7 +953,6 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade(ff_paths P, int next_q) {
     const uint32_t* queue = P.q_mat[TYPE];
     const uint32_t n_round = (n + FF_BLOCK - 1) / FF_BLOCK * FF_BLOCK;
     const int qidx[2] = {next_q == 0 ? C_EXT0 : C_EXT1, C_SHADOW};
-    uint32_t c_shd = 0;
     for (uint32_t i = gtid(); i < n_round; i += gstride()) {
         bool live = i < n;
         uint32_t id = live ? queue[i] : 0u;
@@ -921,17 +986,13 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade(ff_paths P, int next_q) {
             P.ray[2 * id] = f4(so.next_o, 0.f);
             P.ray[2 * id + 1] = f4(so.next_d, 0.f);
         }
-        uint32_t ns = (needL ? 1u : 0u) + (needS ? 1u : 0u) + (needD ? 1u : 0u);
-        c_shd += ns;
-        uint32_t cnt[2] = {live ? 1u : 0u, ns}, first[2];
+        // one shadow-queue entry per event with a light ray (needS / needD
+        // imply needL); the shadow stage traces and counts its rays
+        uint32_t cnt[2] = {live ? 1u : 0u, needL ? 1u : 0u}, first[2];
         block_alloc<FF_BLOCK, 2>(P.counters, qidx, cnt, first, lds);
         if (live) P.q_ext[next_q][first[0]] = id;
-        uint32_t sq = first[1];
-        if (needL) P.q_shadow[sq++] = 3 * id + 0;
-        if (needS) P.q_shadow[sq++] = 3 * id + 1;
-        if (needD) P.q_shadow[sq] = 3 * id + 2;
+        if (needL) P.q_shadow[first[1]] = (id << 2) | (needS ? 1u : 0u) | (needD ? 2u : 0u);
     }
-    wave_sum_add(stats_rep(P) + S_SHADOW, c_shd);
 }
 
 // ================================================== path-replay adjoint
@@ -1070,7 +1131,6 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade_nee(ff_paths P) {
     const uint32_t* queue = P.q_mat[TYPE];
     const uint32_t n_round = (n + FF_BLOCK - 1) / FF_BLOCK * FF_BLOCK;
     const int qidx[1] = {C_SHADOW};
-    uint32_t c_shd = 0;
     for (uint32_t i = gtid(); i < n_round; i += gstride()) {
         bool live = i < n;
         uint32_t id = live ? queue[i] : 0u;
@@ -1092,16 +1152,10 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade_nee(ff_paths P) {
                 P.sray[6 * id + 5] = f4(so.wi_d, 0.f);
             }
         }
-        uint32_t ns = (needL ? 1u : 0u) + (needS ? 1u : 0u) + (needD ? 1u : 0u);
-        c_shd += ns;
-        uint32_t cnt[1] = {ns}, first[1];
+        uint32_t cnt[1] = {needL ? 1u : 0u}, first[1];
         block_alloc<FF_BLOCK, 1>(P.counters, qidx, cnt, first, lds);
-        uint32_t sq = first[0];
-        if (needL) P.q_shadow[sq++] = 3 * id + 0;
-        if (needS) P.q_shadow[sq++] = 3 * id + 1;
-        if (needD) P.q_shadow[sq] = 3 * id + 2;
+        if (needL) P.q_shadow[first[0]] = (id << 2) | (needS ? 1u : 0u) | (needD ? 2u : 0u);
     }
-    wave_sum_add(stats_rep(P) + S_SHADOW, c_shd);
 }
 
 // Replay of a vertex once its shadow rays are traced: its visibility-gated

@@ -67,7 +67,8 @@
 #define S_EXT_TRIS 5
 #define S_SHD_NODES 6
 #define S_SHD_TRIS 7
-#define S_COUNT 8
+#define S_SHD_GATED 8  // BRDF-sample visibility rays not traced (light ray occluded)
+#define S_COUNT 9
 // ray statistics are replicated per workgroup group on separate lines
 #define S_REPLICAS 64
 #define S_STRIDE 16
@@ -172,7 +173,7 @@ struct ff_paths {
     uint32_t* preq;  // [P]     request of the slot
     uint32_t* q_ext[2];
     uint32_t* q_mat[4];
-    uint32_t* q_shadow;
+    uint32_t* q_shadow;  // [P] events with a light ray: path << 2 | needS | needD << 1
     uint32_t* counters;  // [C_COUNT]
     uint32_t* fetch;     // [FETCH_WORDS] traversal work counters
     uint32_t* qbase;     // [n_req + 1] first extension-queue slot of each request's new samples

@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define FF_ABI_VERSION 1
+#define FF_ABI_VERSION 2
 
 typedef enum {
     FF_OK = 0,
@@ -133,6 +133,10 @@ typedef struct {
     /* FF_RENDER_TIMING only: summed launch durations (ms) and launch counts */
     double ms_extend, ms_shadow, ms_logic, ms_shade, ms_other;
     double launches_extend, launches_shadow;
+    /* BRDF-sample visibility rays not traced because the light ray of their
+     * vertex was occluded (its visibility gates the whole NEE sum; the
+     * reference's BRDF_L traces them anyway, CRT/dMaterial.cu:404-448) */
+    double rays_shadow_gated;
 } ff_ray_stats;
 
 /* ---- errors / version ---- */

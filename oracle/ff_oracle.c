@@ -1543,7 +1543,7 @@ static inline float env_pdf_c(const rctx *c, v3 wi) {
     return or_env_pdf(c->pdf, c->s->env_w, c->s->env_h, w);
 }
 
-typedef struct { double prim, ext, shd; } pstat;
+typedef struct { double prim, ext, shd, gated; } pstat;
 
 /* Path tape for the differentiable render: the events of one path in the
  * order their radiance is added.  An escape adds beta * Le(d); a vertex adds
@@ -1592,7 +1592,9 @@ static void put3(float *dst, v3 a) { dst[0] = a.x; dst[1] = a.y; dst[2] = a.z; }
  *   4 Russian roulette compensated (beta / (1 - q)), i.e. Q2 "fixed"
  *   5 BRDF-sampled terms not gated by the light ray's visibility (Q5 "fixed")
  *   6 mixed material's diffuse visibility ray offset along wi_d (Q6 "fixed")
- *   7 light-sample term without its cosine */
+ *   7 light-sample term without its cosine
+ *   8 (not a mistake) BRDF-sample visibility rays traced even where the light
+ *     ray is occluded, as the reference traces them: same image, more rays */
 static int g_mutation = 0;
 void or_set_mutation(int32_t m) { g_mutation = m; }
 
@@ -1855,9 +1857,16 @@ static v3 trace_path(const rctx *c, int32_t px, int32_t py, int32_t sample, psta
         }
         /* ---------------- shadow (wf_shadow) ---------------- */
         visL = visS = visD = 0;
+        /* the BRDF-sample visibility rays only matter when the light ray is
+         * unoccluded (its visibility gates the whole NEE sum), so they are
+         * traced only then; the reference traces them inside BRDF_L either
+         * way (CRT/dMaterial.cu:404-448), which changes no pixel.  Test hook
+         * 8 traces them regardless. */
         if (needL) { visL = !any_ref(s->nodes, c->tri, lo, ld, 0, NULL, NULL); ps->shd += 1; }
-        if (needS) { visS = !any_ref(s->nodes, c->tri, so, sd, 0, NULL, NULL); ps->shd += 1; }
-        if (needD) { visD = !any_ref(s->nodes, c->tri, dd_o, dd_d, 0, NULL, NULL); ps->shd += 1; }
+        int brdf_rays = visL || g_mutation == 5 || g_mutation == 8;
+        if (needS && brdf_rays) { visS = !any_ref(s->nodes, c->tri, so, sd, 0, NULL, NULL); ps->shd += 1; }
+        if (needD && brdf_rays) { visD = !any_ref(s->nodes, c->tri, dd_o, dd_d, 0, NULL, NULL); ps->shd += 1; }
+        if (!brdf_rays) ps->gated += needS + needD;
         if (tape && type != OR_MAT_DIELECTRIC) {
             or_event *e = &tape->ev[tape->n - 1];
             e->a_spec &= visL; e->a_diff &= visL;
@@ -1879,7 +1888,7 @@ static void *render_worker(void *arg) {
     const or_params *p = c->p;
     int K = p->spp_batch > 0 ? p->spp_batch : 1;
     v3 *layer = (v3 *)malloc(sizeof(v3) * (size_t)K);
-    pstat ps = {0, 0, 0};
+    pstat ps = {0, 0, 0, 0};
     double paths = 0;
     for (;;) {
         pthread_mutex_lock(&c->mu);
@@ -1903,6 +1912,7 @@ static void *render_worker(void *arg) {
     c->st.rays_primary += ps.prim;
     c->st.rays_extension += ps.ext;
     c->st.rays_shadow += ps.shd;
+    c->st.rays_shadow_gated += ps.gated;
     c->st.paths += paths;
     pthread_mutex_unlock(&c->mu);
     free(layer);
@@ -2246,7 +2256,7 @@ static void *diff_worker(void *arg) {
                     G.g_tex[t] = (double *)calloc((size_t)s->tex_w[t] * s->tex_h[t] * s->tex_c[t], sizeof(double));
         }
     }
-    pstat ps = {0, 0, 0};
+    pstat ps = {0, 0, 0, 0};
     tape_t scratch = {NULL, 0, 0};
     for (;;) {
         pthread_mutex_lock(&c->mu);

@@ -76,6 +76,8 @@ typedef struct {
     double rays_extension; /* bounce >= 1 extension rays (closest hit)   */
     double rays_shadow;    /* any-hit rays                               */
     double paths;          /* completed samples                          */
+    double rays_shadow_gated; /* BRDF-sample visibility rays not traced: their
+                                 light ray was occluded                  */
 } or_stats;
 
 /* ---- RNG (counter based, keyed by seed/pixel/sample/bounce/dim) ---- */
@@ -133,7 +135,9 @@ float or_env_pdf(const float *pdf, int32_t w, int32_t h, const float *wi);
 void or_env_L(const float *rgb, int32_t w, int32_t h, const float *dir, float *out);
 
 /* ---- camera ---- */
-/* test hook: plant mistake m (1-7, 0 = none) in the path estimator */
+/* test hook: plant mistake m (1-7, 0 = none) in the path estimator; m = 8
+ * traces the BRDF-sample visibility rays also where the light ray is occluded
+ * (the reference's ray count, same image) */
 void or_set_mutation(int32_t m);
 /* ConcentricSampleDisk (CRT/dRandom.cu:104-126) of n points u[n][2]: out[n][2] */
 void or_concentric(const float *u, int32_t n, float *out);

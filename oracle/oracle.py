@@ -37,7 +37,7 @@ class OrParams(C.Structure):
 
 class OrStats(C.Structure):
     _fields_ = [("rays_primary", C.c_double), ("rays_extension", C.c_double),
-                ("rays_shadow", C.c_double), ("paths", C.c_double)]
+                ("rays_shadow", C.c_double), ("paths", C.c_double), ("rays_shadow_gated", C.c_double)]
 
 
 _L = None

@@ -25,7 +25,7 @@ def test_library_exports_every_declared_symbol():
     for s in syms:
         assert hasattr(L, s), s
     assert set(syms) == set(abi.EXPORTED)
-    assert L.ff_abi_version() == 1
+    assert L.ff_abi_version() == 2
 
 
 def test_null_arguments_are_rejected():

@@ -67,6 +67,8 @@ def test_render_spheres_matches_oracle(gpu, oracle):
     assert st.rays_primary == cst.rays_primary == 256 * 256 * 16
     assert abs(st.rays_extension - cst.rays_extension) <= 1e-3 * cst.rays_extension
     assert abs(st.rays_shadow - cst.rays_shadow) <= 1e-3 * cst.rays_shadow
+    assert cst.rays_shadow_gated > 0
+    assert abs(st.rays_shadow_gated - cst.rays_shadow_gated) <= 1e-3 * cst.rays_shadow_gated
 
 
 def test_thin_lens_render_matches_oracle(gpu, oracle):

@@ -129,3 +129,22 @@ def test_normal_map_equals_the_mapped_shading_normal(oracle):
     flat = render(sd)
     sd.materials[:, 13] = -1
     np.testing.assert_allclose(flat, render(sd), rtol=1e-4, atol=1e-6)
+
+
+def test_gated_brdf_visibility_rays_change_no_pixel(oracle):
+    """The BRDF-sample visibility rays are traced only where the light ray is
+    unoccluded (its visibility gates the whole NEE sum).  Tracing them
+    everywhere, as the reference's BRDF_L does (test hook 8), gives the same
+    film bit for bit with more shadow rays."""
+    sd = scenes.spheres_scene(width=48, height=32)
+    ps = oracle.PreparedScene(sd)
+    a, sa = ps.render(spp=4, max_depth=4, n_threads=4)
+    oracle.lib().or_set_mutation(8)
+    try:
+        b, sb = ps.render(spp=4, max_depth=4, n_threads=4)
+    finally:
+        oracle.lib().or_set_mutation(0)
+    assert np.array_equal(a, b)
+    assert sb.rays_shadow > sa.rays_shadow
+    assert sb.rays_shadow == sa.rays_shadow + sa.rays_shadow_gated and sb.rays_shadow_gated == 0
+    assert (sa.rays_primary, sa.rays_extension, sa.paths) == (sb.rays_primary, sb.rays_extension, sb.paths)
