@@ -783,16 +783,16 @@ __device__ __forceinline__ bool visit_interior(const Bvh& B, TravRay& R, float l
         h0 = h0 && !(tf0 < -PRUNE_BEHIND);
         h1 = h1 && !(tf1 < -PRUNE_BEHIND);
     }
-    bool neg = ((R.negm >> (uint32_t)m.z) & 1u) != 0;  // dirIsNeg[axis]
-    int nref = neg ? m.y : m.x, fref = neg ? m.x : m.y;
-    bool hn = neg ? h1 : h0, hf = neg ? h0 : h1;
-    if (hn) {
-        if (hf) st.push(fref);
-        R.ref = nref;
+    // both hit: the near child (by dirIsNeg[axis]) next, the far one pushed;
+    // one hit: that one next (no direction test needed)
+    if (h0 && h1) {
+        bool neg = ((R.negm >> (uint32_t)m.z) & 1u) != 0;  // dirIsNeg[axis]
+        st.push(neg ? m.x : m.y);
+        R.ref = neg ? m.y : m.x;
         return true;
     }
-    if (hf) {
-        R.ref = fref;
+    if (h0 || h1) {
+        R.ref = h0 ? m.x : m.y;
         return true;
     }
     return false;

@@ -300,6 +300,7 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
     if (Src::kUniform) src.scene0(B, tris);
     TravRay R;
     Hit h = no_hit();
+    float lim = prune_lim(h.t);  // pruning limit of the closest hit so far (kept beside it)
     // an any-hit ray is finished with result occ: the result is written and,
     // for a source that chains rays, `nxt` marks that the lane's path has
     // another ray, which the lane starts at the top of the next iteration
@@ -337,6 +338,7 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
                 src.load(first + rank, id, aux, B, tris, o, d, two);
                 c_rays++;
                 h = no_hit();
+                lim = prune_lim(h.t);
                 busy = trav_begin(B, o, d, R, st, cn);
                 walk = busy;
                 pend = 0;
@@ -361,7 +363,7 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
                 bool at_int = busy && walk && R.ref >= 0;
                 bool seeking = busy && walk && pend == 0;
                 if (__ballot(at_int) == 0 || __ballot(seeking) == 0) break;
-                if (at_int && !visit_node<WIDE, !ANY>(B, R, prune_lim(h.t), st, cn)) {
+                if (at_int && !visit_node<WIDE, !ANY>(B, R, lim, st, cn)) {
                     if (st.sp == 0) walk = false;
                     else R.ref = st.pop();
                 }
@@ -373,11 +375,13 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
                     L.ref = pend;
                     if (ANY) occ = leaf_any(B, tris, L, ct);
                     else leaf_closest(B, tris, L, h, two, ct);
+                    if (!ANY) lim = prune_lim(h.t);
                     pend = 0;
                 }
                 if (!occ && walk && R.ref < 0) {
                     if (ANY) occ = leaf_any(B, tris, R, ct);
                     else leaf_closest(B, tris, R, h, two, ct);
+                    if (!ANY) lim = prune_lim(h.t);
                     if (st.sp == 0) walk = false;
                     else R.ref = st.pop();
                 }
@@ -400,7 +404,7 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
                 if (st.sp == 0) walk = false;
                 else R.ref = st.pop();
             }
-            if (busy && walk && R.ref >= 0 && !visit_node<WIDE, !ANY>(B, R, prune_lim(h.t), st, cn)) {
+            if (busy && walk && R.ref >= 0 && !visit_node<WIDE, !ANY>(B, R, lim, st, cn)) {
                 if (st.sp == 0) walk = false;
                 else R.ref = st.pop();
             }
@@ -413,6 +417,7 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
                     L.ref = pend;
                     if (ANY) occ = leaf_any(B, tris, L, ct);
                     else leaf_closest(B, tris, L, h, two, ct);
+                    if (!ANY) lim = prune_lim(h.t);
                     pend = TWO ? pend2 : 0;
                     pend2 = 0;
                 }
@@ -426,7 +431,7 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
             for (;;) {  // interior records until every busy lane is at a leaf
                 bool at_int = busy && R.ref >= 0;
                 if (__ballot(at_int) == 0) break;
-                if (at_int && !visit_node<WIDE, !ANY>(B, R, prune_lim(h.t), st, cn)) {
+                if (at_int && !visit_node<WIDE, !ANY>(B, R, lim, st, cn)) {
                     if (st.sp == 0) {
                         if (ANY) end_any(false);
                         else src.hit(id, h);
@@ -447,6 +452,7 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
                     }
                 } else {
                     leaf_closest(B, tris, R, h, two, ct);
+                    lim = prune_lim(h.t);
                     if (st.sp == 0) {
                         src.hit(id, h);
                         busy = false;
