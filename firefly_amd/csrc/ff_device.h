@@ -870,6 +870,34 @@ __device__ __forceinline__ void leaf_closest(const Bvh& B, const float* __restri
         if (ok && !(t < 0.f) && t < h.t) { h.t = t; h.u = u; h.v = v; h.id = off + i; }
     }
 }
+// The closest hit of a persistent traversal lane: t in a register (every
+// test reads it), u, v and the triangle slot in three LDS words of the lane
+// (written only when a closer hit is found, read once at the end), which
+// keeps the closest-hit kernels at 8 waves per SIMD.
+typedef __attribute__((address_space(3))) float lds_float;
+struct HitSlots {
+    lds_float* u;
+    lds_float* v;
+    lds_int* id;
+};
+__device__ __forceinline__ void leaf_closest_slots(const Bvh& B, const float* __restrict__ tris, const TravRay& R,
+                                                   float& t_hit, const HitSlots& hs, bool two_sided,
+                                                   uint32_t& cnt_tris) {
+    int off, np;
+    leaf_range(B, R.ref, off, np);
+    for (int i = 0; i < np; i++) {
+        Tri T = load_tri(tris, off + i);
+        cnt_tris++;
+        float t = 0.f, u = 0.f, v = 0.f;
+        bool ok = two_sided ? mt_two(T, R.o, R.d, t, u, v) : mt_cull(T, R.o, R.d, t, u, v);
+        if (ok && !(t < 0.f) && t < t_hit) {
+            t_hit = t;
+            *hs.u = u;
+            *hs.v = v;
+            *hs.id = off + i;
+        }
+    }
+}
 // any-hit test of the leaf R.ref's triangles
 __device__ __forceinline__ bool leaf_any(const Bvh& B, const float* __restrict__ tris, const TravRay& R,
                                          uint32_t& cnt_tris) {

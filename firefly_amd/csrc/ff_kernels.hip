@@ -299,8 +299,34 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
     const float* tris = nullptr;
     if (Src::kUniform) src.scene0(B, tris);
     TravRay R;
-    Hit h = no_hit();
+    Hit h = no_hit();  // closest hit: h.t here, u, v and slot in the lane's LDS words (hs)
     float lim = prune_lim(h.t);  // pruning limit of the closest hit so far (kept beside it)
+    HitSlots hs;  // closest-hit kernels allocate LDS * BS + 3 * BS words of smem
+    hs.u = ANY ? nullptr : (lds_float*)&smem[LDS * BS + threadIdx.x];
+    hs.v = ANY ? nullptr : (lds_float*)&smem[LDS * BS + BS + threadIdx.x];
+    hs.id = ANY ? nullptr : (lds_int*)&smem[LDS * BS + 2 * BS + threadIdx.x];
+    // closest-hit sources: the lane's leaf test, the hit reset and the result
+    auto leaf_c = [&](const TravRay& L) {
+        leaf_closest_slots(B, tris, L, h.t, hs, two, ct);
+        lim = prune_lim(h.t);
+    };
+    auto reset_hit = [&]() {
+        h = no_hit();
+        lim = prune_lim(h.t);
+        if (!ANY) {
+            *hs.u = h.u;
+            *hs.v = h.v;
+            *hs.id = h.id;
+        }
+    };
+    auto hit_out = [&]() {
+        Hit r;
+        r.t = h.t;
+        r.u = *hs.u;
+        r.v = *hs.v;
+        r.id = *hs.id;
+        src.hit(id, r);
+    };
     // an any-hit ray is finished with result occ: the result is written and,
     // for a source that chains rays, `nxt` marks that the lane's path has
     // another ray, which the lane starts at the top of the next iteration
@@ -337,15 +363,14 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
                 f3 o, d;
                 src.load(first + rank, id, aux, B, tris, o, d, two);
                 c_rays++;
-                h = no_hit();
-                lim = prune_lim(h.t);
+                if (!ANY) reset_hit();
                 busy = trav_begin(B, o, d, R, st, cn);
                 walk = busy;
                 pend = 0;
                 pend2 = 0;
                 if (!busy) {
                     if (ANY) end_any(false);
-                    else src.hit(id, h);
+                    else hit_out();
                 }
             }
         }
@@ -374,20 +399,18 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
                     TravRay L = R;
                     L.ref = pend;
                     if (ANY) occ = leaf_any(B, tris, L, ct);
-                    else leaf_closest(B, tris, L, h, two, ct);
-                    if (!ANY) lim = prune_lim(h.t);
+                    else leaf_c(L);
                     pend = 0;
                 }
                 if (!occ && walk && R.ref < 0) {
                     if (ANY) occ = leaf_any(B, tris, R, ct);
-                    else leaf_closest(B, tris, R, h, two, ct);
-                    if (!ANY) lim = prune_lim(h.t);
+                    else leaf_c(R);
                     if (st.sp == 0) walk = false;
                     else R.ref = st.pop();
                 }
                 if (occ || !walk) {
                     if (ANY) end_any(occ);
-                    else src.hit(id, h);
+                    else hit_out();
                     busy = false;
                 }
             }
@@ -416,15 +439,14 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
                     TravRay L = R;
                     L.ref = pend;
                     if (ANY) occ = leaf_any(B, tris, L, ct);
-                    else leaf_closest(B, tris, L, h, two, ct);
-                    if (!ANY) lim = prune_lim(h.t);
+                    else leaf_c(L);
                     pend = TWO ? pend2 : 0;
                     pend2 = 0;
                 }
             }
             if (busy && (occ || (!walk && pend == 0))) {
                 if (ANY) end_any(occ);
-                else src.hit(id, h);
+                else hit_out();
                 busy = false;
             }
         } else if (WW == 1) {
@@ -434,7 +456,7 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
                 if (at_int && !visit_node<WIDE, !ANY>(B, R, lim, st, cn)) {
                     if (st.sp == 0) {
                         if (ANY) end_any(false);
-                        else src.hit(id, h);
+                        else hit_out();
                         busy = false;
                     } else {
                         R.ref = st.pop();
@@ -451,10 +473,9 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
                         R.ref = st.pop();
                     }
                 } else {
-                    leaf_closest(B, tris, R, h, two, ct);
-                    lim = prune_lim(h.t);
+                    leaf_c(R);
                     if (st.sp == 0) {
-                        src.hit(id, h);
+                        hit_out();
                         busy = false;
                     } else {
                         R.ref = st.pop();
@@ -468,9 +489,19 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
                     end_any(r == 1);
                     busy = false;
                 }
-            } else if (!closest_step<WIDE>(B, tris, R, h, two, st, cn, ct)) {
-                src.hit(id, h);
-                busy = false;
+            } else {
+                // closest_step with the hit's u, v and slot in LDS
+                bool cont = true;
+                if (R.ref >= 0 && visit_node<WIDE, true>(B, R, lim, st, cn)) {
+                } else {
+                    if (R.ref < 0) leaf_c(R);
+                    if (st.sp == 0) cont = false;
+                    else R.ref = st.pop();
+                }
+                if (!cont) {
+                    hit_out();
+                    busy = false;
+                }
             }
         }
     }
@@ -485,9 +516,9 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
 // (72 VGPRs) without spilling where its LDS stack allows that many; the
 // others keep the compiler's choice.
 template <int BS, int LDS, int RF, int WW, bool WIDE, bool UNI>
-__global__ __launch_bounds__(BS, (UNI && (WW == 0 || WW >= 5) && !WIDE && LDS <= 16 && BS >= 128) ? 7 : 1) void k_extend(
+__global__ __launch_bounds__(BS, (UNI && (WW == 0 || WW >= 5) && !WIDE && LDS <= 16 && BS >= 128) ? 8 : 1) void k_extend(
     ff_paths P, int q) {
-    __shared__ int smem[LDS * BS];
+    __shared__ int smem[LDS * BS + 3 * BS];  // stack, then the hit words
     ExtSrc<UNI> src{P, P.q_ext[q]};
     trav_persistent<BS, LDS, RF, false, WW, WIDE>(src, P.counters[q == 0 ? C_EXT0 : C_EXT1],
                                             P.fetch + F_EXT * FETCH_SEGS * FETCH_STRIDE, P.ovf,
@@ -1290,7 +1321,7 @@ struct TraceArgs {  // one ff_trace_closest / ff_trace_any call
 template <int BS, int LDS, int RF, int WW, bool WIDE>
 __global__ __launch_bounds__(BS, (WW == 0 && !WIDE && LDS <= 16 && BS >= 128) ? 7 : 1) void k_trace_closest(
     TraceArgs a) {
-    __shared__ int smem[LDS * BS];
+    __shared__ int smem[LDS * BS + 3 * BS];  // stack, then the hit words
     RaySrc src{a.S, a.rays, a.hits, nullptr, a.two_sided};
     trav_persistent<BS, LDS, RF, false, WW, WIDE>(src, a.n, a.fetch, a.ovf, a.counts, smem);
 }
