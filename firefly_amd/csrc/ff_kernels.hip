@@ -512,9 +512,10 @@ __device__ __forceinline__ void trav_persistent(const Src& src, uint32_t n, uint
     src.count(c_rays, c_gated);
 }
 
-// The one-scene extend kernel of the if-if shape fits 7 waves per SIMD
-// (72 VGPRs) without spilling where its LDS stack allows that many; the
-// others keep the compiler's choice.
+// The one-scene extend kernels of the one-step and leaf-batching shapes run
+// 8 waves per SIMD (64 VGPRs, a few bytes of spills; the hit's u, v and slot
+// live in LDS) where their LDS stack allows that many; the others keep the
+// compiler's choice.
 template <int BS, int LDS, int RF, int WW, bool WIDE, bool UNI>
 __global__ __launch_bounds__(BS, (UNI && (WW == 0 || WW >= 5) && !WIDE && LDS <= 16 && BS >= 128) ? 8 : 1) void k_extend(
     ff_paths P, int q) {
