@@ -974,10 +974,10 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
             HIP_TRY(timed(1, [&] { return ffk::launch_shadow(p, var_at(it), hs); }));
             uint32_t mask = (1u << (cur == 0 ? C_EXT1 : C_EXT0)) | (1u << C_SHADOW) | (0xfu << C_MAT0);
             HIP_TRY(ffk::launch_reset(p.counters, mask, p.fetch, (1u << F_EXT) | (1u << F_SHD), hs));
-            HIP_TRY(timed(2, [&] { return ffk::launch_logic(p, cur, it == 1, p.n_slots, hs); }));
+            HIP_TRY(timed(2, [&] { return ffk::launch_logic(p, cur, it, p.n_slots, hs); }));
             if (it <= max_depth) {
                 for (int m = 0; m < 4; m++)
-                    if (has_type[m]) HIP_TRY(timed(3, [&] { return ffk::launch_shade(m, p, 1 - cur, p.n_slots, hs); }));
+                    if (has_type[m]) HIP_TRY(timed(3, [&] { return ffk::launch_shade(m, p, 1 - cur, it == 1, p.n_slots, hs); }));
             }
             cur = 1 - cur;
         }

@@ -23,8 +23,8 @@ hipError_t launch_generate(const ff_paths& P, hipStream_t s);
 hipError_t launch_extend(const ff_paths& P, int q, int state, int variant, hipStream_t s);
 hipError_t launch_shadow(const ff_paths& P, int variant, hipStream_t s);
 hipError_t launch_reset(uint32_t* counters, uint32_t mask, uint32_t* fetch, uint32_t fetch_mask, hipStream_t s);
-hipError_t launch_logic(const ff_paths& P, int q, int primary, uint32_t n_max, hipStream_t s);
-hipError_t launch_shade(int type, const ff_paths& P, int next_q, uint32_t n_max, hipStream_t s);
+hipError_t launch_logic(const ff_paths& P, int q, int iter, uint32_t n_max, hipStream_t s);
+hipError_t launch_shade(int type, const ff_paths& P, int next_q, int first, uint32_t n_max, hipStream_t s);
 hipError_t launch_logic_adj(const ff_paths& P, int q, uint32_t n_max, hipStream_t s);
 hipError_t launch_shade_nee(int type, const ff_paths& P, uint32_t n_max, hipStream_t s);
 hipError_t launch_shade_adj(int type, const ff_paths& P, int next_q, uint32_t n_max, hipStream_t s);

@@ -557,8 +557,12 @@ __global__ void k_reset_counters(uint32_t* c, uint32_t mask, uint32_t* fetch, ui
 // operation.  In the first batch of a samples-in-flight request every slot is
 // written exactly once (each slot holds one path per batch and every path
 // terminates within its batch), so the layer is stored without being read.
+// A path that survives its first vertex still holds the start record
+// (beta = 1, L = 0): iteration 1 does not store it, and iteration 2 (and the
+// first shading pass) use the constants instead of reading it.
 template <bool UNI, int NP>
-__global__ __launch_bounds__(FF_BLOCK, NP == 1 ? 8 : 5) void k_logic(ff_paths P, int q, int primary) {
+__global__ __launch_bounds__(FF_BLOCK, NP == 1 ? 8 : 5) void k_logic(ff_paths P, int q, int iter) {
+    const bool primary = iter == 1;
     __shared__ uint32_t lds[4 * (FF_BLOCK / 64 + 1)];
     const uint32_t n = P.counters[q == 0 ? C_EXT0 : C_EXT1];
     const uint32_t* queue = P.q_ext[q];
@@ -583,7 +587,7 @@ __global__ __launch_bounds__(FF_BLOCK, NP == 1 ? 8 : 5) void k_logic(ff_paths P,
             // a camera path's record is the constant start state (not stored)
             bw[j] = make_float4(1.f, 1.f, 1.f, __uint_as_float(ST_PREV_DELTA));
             lp4[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (!primary) {
+            if (iter > 2) {
                 bw[j] = P.beta[id[j]];
                 lp4[j] = P.Lp[id[j]];
             }
@@ -694,8 +698,10 @@ __global__ __launch_bounds__(FF_BLOCK, NP == 1 ? 8 : 5) void k_logic(ff_paths P,
                 }
             } else {
                 uint32_t nst = (sw & (ST_INSIDE | ST_PREV_DELTA)) | ln;
-                P.beta[id[j]] = f4(beta, __uint_as_float(nst));
-                P.Lp[id[j]] = f4(Lp, 0.f);
+                if (!primary) {
+                    P.beta[id[j]] = f4(beta, __uint_as_float(nst));
+                    P.Lp[id[j]] = f4(Lp, 0.f);
+                }
                 route[j] = mtype;
             }
             c_prim += ln == 1 ? 1u : 0u;
@@ -754,11 +760,12 @@ struct ShadeOut {
 // rays of BRDF_L's BRDF-sampled MIS terms (CRT/dMaterial.cu:404-448) are
 // deferred to the shadow stage with their contributions kept per path.
 template <int TYPE, bool GRAD, bool UNI>
-__device__ __forceinline__ void shade_core(const ff_paths& P, uint32_t id, ShadeOut& so) {
+__device__ __forceinline__ void shade_core(const ff_paths& P, uint32_t id, ShadeOut& so, bool first = false) {
     const ff_dev_request& R = P.reqs[P.preq[id]];
     const ff_dev_scene& S = scene_of<UNI>(P, R);
     EnvView E = env_view(env_of<UNI>(P, R));
-    float4 bw = P.beta[id];
+    // the first vertex of a forward path: its record is the start record
+    float4 bw = first ? make_float4(1.f, 1.f, 1.f, __uint_as_float(ST_PREV_DELTA | 1u)) : P.beta[id];
     uint32_t stw = __float_as_uint(bw.w);
     uint32_t len = stw & ST_LEN_MASK;
     bool inside = (stw & ST_INSIDE) != 0;
@@ -985,7 +992,7 @@ __device__ __forceinline__ void shade_core(const ff_paths& P, uint32_t id, Shade
 }
 
 template <int TYPE, bool UNI>
-__global__ __launch_bounds__(FF_BLOCK) void k_shade(ff_paths P, int next_q) {
+__global__ __launch_bounds__(FF_BLOCK) void k_shade(ff_paths P, int next_q, int first) {
     __shared__ uint32_t lds[2 * (FF_BLOCK / 64 + 1)];
     const uint32_t n = P.counters[C_MAT0 + TYPE];
     const uint32_t* queue = P.q_mat[TYPE];
@@ -997,7 +1004,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade(ff_paths P, int next_q) {
         bool needL = false, needS = false, needD = false;
         if (live) {
             ShadeOut so;
-            shade_core<TYPE, false, UNI>(P, id, so);
+            shade_core<TYPE, false, UNI>(P, id, so, first != 0);
             needL = so.needL; needS = so.needS; needD = so.needD;
             if (needL) {
                 P.sray[6 * id + 0] = f4(so.lo, 0.f);
@@ -1661,27 +1668,27 @@ hipError_t launch_reset(uint32_t* counters, uint32_t mask, uint32_t* fetch, uint
     return hipGetLastError();
 }
 template <bool UNI>
-hipError_t logic_launch(const ff_paths& P, int q, int primary, uint32_t n_max, hipStream_t s) {
-    hipLaunchKernelGGL((k_logic<UNI, 2>), dim3(grid_for(n_max, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, P, q, primary);
+hipError_t logic_launch(const ff_paths& P, int q, int iter, uint32_t n_max, hipStream_t s) {
+    hipLaunchKernelGGL((k_logic<UNI, 2>), dim3(grid_for(n_max, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, P, q, iter);
     return hipGetLastError();
 }
-hipError_t launch_logic(const ff_paths& P, int q, int primary, uint32_t n_max, hipStream_t s) {
-    return uni_shading(P) ? logic_launch<true>(P, q, primary, n_max, s) : logic_launch<false>(P, q, primary, n_max, s);
+hipError_t launch_logic(const ff_paths& P, int q, int iter, uint32_t n_max, hipStream_t s) {
+    return uni_shading(P) ? logic_launch<true>(P, q, iter, n_max, s) : logic_launch<false>(P, q, iter, n_max, s);
 }
 template <bool UNI>
-hipError_t shade_launch(int type, const ff_paths& P, int next_q, uint32_t n_max, hipStream_t s) {
+hipError_t shade_launch(int type, const ff_paths& P, int next_q, int first, uint32_t n_max, hipStream_t s) {
     dim3 g(grid_for(n_max, FF_GRID_CAP)), b(FF_BLOCK);
     switch (type) {
-        case MAT_MIX: hipLaunchKernelGGL((k_shade<MAT_MIX, UNI>), g, b, 0, s, P, next_q); break;
-        case MAT_DIFFUSE: hipLaunchKernelGGL((k_shade<MAT_DIFFUSE, UNI>), g, b, 0, s, P, next_q); break;
-        case MAT_CONDUCTOR: hipLaunchKernelGGL((k_shade<MAT_CONDUCTOR, UNI>), g, b, 0, s, P, next_q); break;
-        default: hipLaunchKernelGGL((k_shade<MAT_DIELECTRIC, UNI>), g, b, 0, s, P, next_q); break;
+        case MAT_MIX: hipLaunchKernelGGL((k_shade<MAT_MIX, UNI>), g, b, 0, s, P, next_q, first); break;
+        case MAT_DIFFUSE: hipLaunchKernelGGL((k_shade<MAT_DIFFUSE, UNI>), g, b, 0, s, P, next_q, first); break;
+        case MAT_CONDUCTOR: hipLaunchKernelGGL((k_shade<MAT_CONDUCTOR, UNI>), g, b, 0, s, P, next_q, first); break;
+        default: hipLaunchKernelGGL((k_shade<MAT_DIELECTRIC, UNI>), g, b, 0, s, P, next_q, first); break;
     }
     return hipGetLastError();
 }
-hipError_t launch_shade(int type, const ff_paths& P, int next_q, uint32_t n_max, hipStream_t s) {
-    return uni_shading(P) ? shade_launch<true>(type, P, next_q, n_max, s)
-                          : shade_launch<false>(type, P, next_q, n_max, s);
+hipError_t launch_shade(int type, const ff_paths& P, int next_q, int first, uint32_t n_max, hipStream_t s) {
+    return uni_shading(P) ? shade_launch<true>(type, P, next_q, first, n_max, s)
+                          : shade_launch<false>(type, P, next_q, first, n_max, s);
 }
 template <bool UNI>
 hipError_t logic_adj_launch(const ff_paths& P, int q, uint32_t n_max, hipStream_t s) {
