@@ -56,6 +56,11 @@ __device__ __forceinline__ unsigned long long* stats_rep(const ff_paths& P) {
     return P.stats + (size_t)(blockIdx.x % S_REPLICAS) * S_STRIDE;
 }
 
+// request of a path: a one-request call does not store it
+__device__ __forceinline__ uint32_t req_of(const ff_paths& P, uint32_t id) {
+    return P.n_req == 1 ? 0u : P.preq[id];
+}
+
 // request owning a slot: last r with slot_base <= id
 __device__ __forceinline__ uint32_t find_req(const ff_dev_request* __restrict__ reqs, uint32_t n, uint32_t id) {
     uint32_t lo = 0, hi = n;
@@ -116,7 +121,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_generate(ff_paths P) {
         P.ray[2 * id] = f4(o, 0.f);
         P.ray[2 * id + 1] = f4(d, 0.f);
         P.rngb[id] = base;
-        P.preq[id] = r;
+        if (P.n_req > 1) P.preq[id] = r;
         if (R.mode & RQ_ADJOINT) {
             // replay: L_rem starts as the stored radiance of this path
             P.beta[id] = make_float4(1.f, 1.f, 1.f, __uint_as_float(ST_PREV_DELTA));
@@ -152,7 +157,7 @@ struct ExtSrc {  // wf_extend (CRT/kernel.cu:487-523): closest hit of every queu
                                          f3& d, bool& two) const {
         id = queue[k];
         if (!UNI) {
-            const ff_dev_scene& S = P.reqs[P.preq[id]].S;
+            const ff_dev_scene& S = P.reqs[req_of(P, id)].S;
             B = bvh_of(S);
             tris = S.tris;
         }
@@ -193,7 +198,7 @@ struct ShadowSrc {  // wf_shadow (CRT/kernel.cu:525-543): any hit of every queue
         id = e >> 2;
         aux = (e & 3u) << 2;
         if (!UNI) {
-            const ff_dev_scene& S = P.reqs[P.preq[id]].S;
+            const ff_dev_scene& S = P.reqs[req_of(P, id)].S;
             B = bvh_of(S);
             tris = S.tris;
         }
@@ -592,7 +597,7 @@ __global__ __launch_bounds__(FF_BLOCK, NP == 1 ? 8 : 5) void k_logic(ff_paths P,
                 lp4[j] = P.Lp[id[j]];
             }
             hh[j] = P.hit[id[j]];
-            rq[j] = P.n_req == 1 ? 0u : P.preq[id[j]];
+            rq[j] = req_of(P, id[j]);
             // after a shading pass the state word travels in fext.w:
             // (f * cos / pdf of the continuation, state | ST_DEAD where f or pdf is zero)
             fe[j] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -761,7 +766,7 @@ struct ShadeOut {
 // deferred to the shadow stage with their contributions kept per path.
 template <int TYPE, bool GRAD, bool UNI>
 __device__ __forceinline__ void shade_core(const ff_paths& P, uint32_t id, ShadeOut& so, bool first = false) {
-    const ff_dev_request& R = P.reqs[P.preq[id]];
+    const ff_dev_request& R = P.reqs[req_of(P, id)];
     const ff_dev_scene& S = scene_of<UNI>(P, R);
     EnvView E = env_view(env_of<UNI>(P, R));
     // the first vertex of a forward path: its record is the start record
@@ -1111,7 +1116,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_logic_adj(ff_paths P, int q) {
         bool term = false, primary = false;
         uint32_t id = live ? queue[i] : 0u;
         if (live) {
-            const ff_dev_request& R = P.reqs[P.preq[id]];
+            const ff_dev_request& R = P.reqs[req_of(P, id)];
             const ff_dev_scene& S = scene_of<UNI>(P, R);
             float4 bw = P.beta[id];
             uint32_t stw = __float_as_uint(bw.w);
@@ -1218,7 +1223,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade_adj(ff_paths P, int next_q) 
         bool live = i < n;
         uint32_t id = live ? queue[i] : 0u;
         if (live) {
-            const ff_dev_request& R = P.reqs[P.preq[id]];
+            const ff_dev_request& R = P.reqs[req_of(P, id)];
             ShadeOut so;
             shade_core<TYPE, TYPE != MAT_DIELECTRIC, UNI>(P, id, so);
             f3 beta = xyz(P.beta[id]);

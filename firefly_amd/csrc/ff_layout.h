@@ -170,7 +170,7 @@ struct ff_paths {
     float4* sray;    // [P][3][2] shadow rays
     uint8_t* vis;    // [P][3]
     uint32_t* rngb;  // [P]
-    uint32_t* preq;  // [P]     request of the slot
+    uint32_t* preq;  // [P]     request of the slot (stored only when n_req > 1)
     uint32_t* q_ext[2];
     uint32_t* q_mat[4];
     uint32_t* q_shadow;  // [P] events with a light ray: path << 2 | needS | needD << 1
