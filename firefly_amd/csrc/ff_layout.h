@@ -106,9 +106,10 @@ struct ff_dev_scene {
 };
 
 // Guide tables of the inverse-CDF searches: for bucket g of [0, 1) (width
-// 1 / ENV_GUIDE) the first and last candidate index of the upper bound.
+// 1 / ENV_GUIDE) the first and last candidate index of the upper bound (two
+// buckets per texel of a 2048-texel row, so a search is about one step).
 // Entry with bit 31 set: the list is not monotone (NaN rows), search it whole.
-#define ENV_GUIDE 256
+#define ENV_GUIDE 4096
 #define ENV_GUIDE_FULL 0x80000000u
 
 struct ff_dev_env {
