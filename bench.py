@@ -680,7 +680,7 @@ def run_bvh(args) -> dict:
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             abi.render(ctx, sc, env, cam, sd.width, sd.height, args.spp, args.depth, sd.seed, film,
-                       spp_batch=args.spp_batch, flags=abi.FF_RENDER_TIMING)
+                       spp_batch=args.spp_batch)
             st = abi.last_render_stats(ctx)
             best = min(best, time.perf_counter() - t0)
         rays = st.rays_primary + st.rays_extension
@@ -758,7 +758,7 @@ def run_frame(args) -> dict:
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for i in range(args.steps):
-        frame(abi.FF_RENDER_TIMING)
+        frame()
         st = abi.last_render_stats(ctx)  # synchronises the frame
         for k in keys:
             tot[k] += getattr(st, k)
@@ -766,16 +766,21 @@ def run_frame(args) -> dict:
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     ms_step = elapsed / args.steps * 1e3
+    # stage times and the roofline: one more frame with every launch timed by
+    # hipEvents (FF_RENDER_TIMING runs the stages one after another; the timed
+    # frames above overlap each iteration's shadow and extension traversals)
+    frame(abi.FF_RENDER_TIMING)
+    tim = abi.last_render_stats(ctx).as_dict()
 
     incoherent = tot["rays_extension"] + tot["rays_shadow"]
     total_rays = incoherent + tot["rays_primary"]
     spp_s = args.steps * args.spp / elapsed * (sd.width * sd.height) / (1920 * 1080)
-    ext_bytes = (EXT_RAY_BYTES * (tot["rays_primary"] + tot["rays_extension"]) +
-                 NODE_BYTES * tot["ext_nodes"] + TRI_BYTES * tot["ext_tris"])
-    shd_bytes = SHD_RAY_BYTES * tot["rays_shadow"] + NODE_BYTES * tot["shadow_nodes"] + TRI_BYTES * tot["shadow_tris"]
+    ext_bytes = (EXT_RAY_BYTES * (tim["rays_primary"] + tim["rays_extension"]) +
+                 NODE_BYTES * tim["ext_nodes"] + TRI_BYTES * tim["ext_tris"])
+    shd_bytes = SHD_RAY_BYTES * tim["rays_shadow"] + NODE_BYTES * tim["shadow_nodes"] + TRI_BYTES * tim["shadow_tris"]
     kern = {
-        "k_extend": (ext_bytes, tot["ms_extend"], tot["launches_extend"]),
-        "k_shadow": (shd_bytes, tot["ms_shadow"], tot["launches_shadow"]),
+        "k_extend": (ext_bytes, tim["ms_extend"], tim["launches_extend"]),
+        "k_shadow": (shd_bytes, tim["ms_shadow"], tim["launches_shadow"]),
     }
     dom = max(kern, key=lambda k: kern[k][1])
     b, ms, nl = kern[dom]
@@ -809,7 +814,7 @@ def run_frame(args) -> dict:
         },
         "limiter": limiter_of(ctr, dram / HBM_PEAK_GBS if dram else None),
     }
-    stages = {k: round(tot[k] / args.steps, 2) for k in ("ms_extend", "ms_shadow", "ms_logic", "ms_shade", "ms_other")}
+    stages = {k: round(tim[k], 2) for k in ("ms_extend", "ms_shadow", "ms_logic", "ms_shade", "ms_other")}
     per_ray = {
         "ext_nodes_per_ray": round(tot["ext_nodes"] / max(1.0, tot["rays_primary"] + tot["rays_extension"]), 3),
         "ext_tris_per_ray": round(tot["ext_tris"] / max(1.0, tot["rays_primary"] + tot["rays_extension"]), 3),
@@ -848,6 +853,8 @@ def run_frame(args) -> dict:
         # ray was occluded (no pixel depends on them); not counted in value
         "rays_shadow_gated_per_step": int(tot["rays_shadow_gated"] / args.steps),
         "stage_ms_per_step": stages,
+        "stage_ms_note": ("one more frame with every launch timed by hipEvents, which runs the stages one after "
+                          "another; the timed frames overlap each iteration's shadow and extension traversals"),
         "traversal": per_ray,
         "envmap_update_ms": round(env_ms, 4),
         "roofline": roofline,

@@ -86,6 +86,10 @@ struct ff_context {
     size_t ovf_entries = 0;  // stack entries per grid thread
     hipStream_t last_stream = nullptr;
     hipStream_t own_stream = nullptr;  // graph capture when the caller passes the null stream
+    // a second stream for the shadow traversal, which runs beside the
+    // extension traversal of the same iteration (fork / join events)
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     bool have_stats = false;
     uint32_t* bstate = nullptr;
     ff_dev_request* d_reqs = nullptr;
@@ -184,12 +188,14 @@ ff_status ensure_paths(ff_context* c, size_t P) {
 // traversal stack overflow area: entries beyond the smallest LDS stack per grid thread
 // Stack entries a traversal can hold: one per binary level for the two-child
 // records; up to three per 4-wide record (two binary levels) for the others.
+// Two areas: the extension and shadow traversals of an iteration run at the
+// same time, each with its own.
 ff_status ensure_ovf(ff_context* c, int32_t max_depth) {
     int32_t entries = std::max(max_depth, 3 * ((max_depth + 1) / 2));
     size_t need = entries > FF_LDS_MIN ? (size_t)(entries - FF_LDS_MIN + 1) : 1;
     if (c->ovf && c->ovf_entries >= need) return FF_OK;
     dfree(c->ovf);
-    HIP_TRY(dalloc(&c->ovf, need * (size_t)FF_TRAV_THREADS));
+    HIP_TRY(dalloc(&c->ovf, 2 * need * (size_t)FF_TRAV_THREADS));
     c->ovf_entries = need;
     return FF_OK;
 }
@@ -375,6 +381,9 @@ ff_status ff_context_destroy(ff_context* c) {
     c->paths.release();
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    if (c->side) (void)hipStreamDestroy(c->side);
+    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+    if (c->join_ev) (void)hipEventDestroy(c->join_ev);
     dfree(c->bstate);
     dfree(c->d_reqs);
     dfree(c->d_qbase);
@@ -817,6 +826,11 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
         if (!ctx->own_stream) HIP_TRY(hipStreamCreate(&ctx->own_stream));
         hs = ctx->own_stream;
     }
+    if (!ctx->side) {
+        HIP_TRY(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming));
+    }
     HIP_TRY(ffk::prepare_traversal(ctx->trav_variant));
     if (ctx->trav_tail >= 0) HIP_TRY(ffk::prepare_traversal(ctx->trav_tail));
     ff_paths p = ctx->paths.p;
@@ -919,6 +933,7 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
     p.qbase = ctx->d_qbase;
     p.stats = ctx->stats;
     p.ovf = ctx->ovf;
+    p.ovf_shd = ctx->ovf + ctx->ovf_entries * (size_t)FF_TRAV_THREADS;
     p.bstate = ctx->bstate;
     p.reqs = ctx->d_reqs;
     p.n_req = (uint32_t)table.size();
@@ -939,7 +954,7 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
     ctx->records.clear();
     ctx->events_used = 0;
     // Records a begin/end event pair around one stage launch when timing.
-    auto timed = [&](int stage, auto&& launch) -> hipError_t {
+    auto timed_on = [&](hipStream_t ts, int stage, auto&& launch) -> hipError_t {
         if (!timing) return launch();
         while (ctx->event_pool.size() < ctx->events_used + 2) {
             hipEvent_t e;
@@ -949,12 +964,13 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
         }
         StageRecord rec{stage, ctx->event_pool[ctx->events_used], ctx->event_pool[ctx->events_used + 1]};
         ctx->events_used += 2;
-        hipError_t e = hipEventRecord(rec.begin, hs);
+        hipError_t e = hipEventRecord(rec.begin, ts);
         if (e == hipSuccess) e = launch();
-        if (e == hipSuccess) e = hipEventRecord(rec.end, hs);
+        if (e == hipSuccess) e = hipEventRecord(rec.end, ts);
         ctx->records.push_back(rec);
         return e;
     };
+    auto timed = [&](int stage, auto&& launch) -> hipError_t { return timed_on(hs, stage, launch); };
     // the first batch stores every layer of a request whose spp covers its
     // samples in flight (wf_logic writes it without reading); only a request
     // with fewer samples than layers keeps layers that no path writes
@@ -970,8 +986,21 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
         HIP_TRY(timed(4, [&] { return ffk::launch_generate(p, hs); }));
         int cur = 0;
         for (int32_t it = 1; it <= max_depth + 1; it++) {
+            // the iteration's extension rays and its vertices' shadow rays are
+            // independent: the shadow traversal runs on the side stream, so
+            // either kernel's waves fill the other's tail (with per-launch
+            // timing the two run one after another, so each time is its own)
+            hipStream_t ss = timing ? hs : ctx->side;
+            if (!timing) {
+                HIP_TRY(hipEventRecord(ctx->fork_ev, hs));
+                HIP_TRY(hipStreamWaitEvent(ss, ctx->fork_ev, 0));
+            }
             HIP_TRY(timed(0, [&] { return ffk::launch_extend(p, cur, it == 1 ? EXT_STATE_NONE : EXT_STATE_FEXT, var_at(it), hs); }));
-            HIP_TRY(timed(1, [&] { return ffk::launch_shadow(p, var_at(it), hs); }));
+            HIP_TRY(timed_on(ss, 1, [&] { return ffk::launch_shadow(p, var_at(it), ss); }));
+            if (!timing) {
+                HIP_TRY(hipEventRecord(ctx->join_ev, ss));
+                HIP_TRY(hipStreamWaitEvent(hs, ctx->join_ev, 0));
+            }
             uint32_t mask = (1u << (cur == 0 ? C_EXT1 : C_EXT0)) | (1u << C_SHADOW) | (0xfu << C_MAT0);
             HIP_TRY(ffk::launch_reset(p.counters, mask, p.fetch, (1u << F_EXT) | (1u << F_SHD), hs));
             HIP_TRY(timed(2, [&] { return ffk::launch_logic(p, cur, it, p.n_slots, hs); }));

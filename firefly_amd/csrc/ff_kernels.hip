@@ -535,7 +535,7 @@ __global__ __launch_bounds__(BS) void k_shadow(ff_paths P) {
     __shared__ int smem[LDS * BS];
     ShadowSrc<UNI> src{P};
     trav_persistent<BS, LDS, RF, true, WW, WIDE>(src, P.counters[C_SHADOW], P.fetch + F_SHD * FETCH_SEGS * FETCH_STRIDE,
-                                           P.ovf, stats_rep(P) + S_SHD_NODES, smem);
+                                           P.ovf_shd, stats_rep(P) + S_SHD_NODES, smem);
 }
 
 // counters consumed by the previous stages are cleared by a kernel (graph-safe):

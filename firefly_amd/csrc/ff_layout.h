@@ -180,7 +180,8 @@ struct ff_paths {
     uint32_t* qbase;     // [n_req + 1] first extension-queue slot of each request's new samples
     float4* layers;      // [P] film layer accumulators (path id = layer*WH + pixel)
     unsigned long long* stats;  // [S_REPLICAS][S_STRIDE], S_COUNT used per replica
-    int32_t* ovf;        // traversal stack overflow [depth][FF_TRAV_GRID*FF_BLOCK]
+    int32_t* ovf;        // traversal stack overflow [depth][FF_TRAV_THREADS] of the extension stage
+    int32_t* ovf_shd;    // the same for the shadow stage (the two run at the same time)
     uint32_t* bstate;    // [B_COUNT]
     const ff_dev_request* reqs;
     uint32_t n_req;

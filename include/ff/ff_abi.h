@@ -121,7 +121,9 @@ typedef struct {
 } ff_render_req;
 
 #define FF_RENDER_USE_GRAPH 1 /* capture the bounce loop in a hipGraph              */
-#define FF_RENDER_TIMING 2    /* time every stage launch with hipEvents (stats)     */
+#define FF_RENDER_TIMING 2    /* time every stage launch with hipEvents (stats); the
+                                 stages then run one after another (otherwise each
+                                 iteration's shadow traversal overlaps its extension) */
 
 typedef struct {
     double rays_primary;   /* camera rays (closest hit)                     */
