@@ -206,8 +206,8 @@ def run_batch(args) -> dict:
             log(f"{name} run {i + 1}: {dt:.3f} s")
         return best
 
-    if args.batch_only:  # profiling: the batched call alone
-        best_time(f"batched K={ks[0]}", lambda: batched(ks[0]), f_bat)
+    if args.batch_only:  # profiling: the batched call alone, timed per launch as the stage times below are
+        best_time(f"batched K={ks[0]}", lambda: batched(ks[0], abi.FF_RENDER_TIMING), f_bat)
         return {"workload": f"batch{len(built)}_uneven", "spp_batch": ks[0]}
     sweep = {}
     identical = True
@@ -477,8 +477,11 @@ def run_multiview(args) -> dict:
             abi.adjoint(ctx, scene, env, cams[k], W, H, S, D, sd.seed, paths[k], ds[k], tex_grads=[g_alb, g_rough],
                         spp_batch=kb)
 
-    if args.mv_forward_only:  # profiling: the batched forward alone
-        fwd_batch()
+    if args.mv_forward_only:  # profiling: the batched forward alone, timed per launch as the stage times below are
+        for f in films:
+            f.zero_()
+        abi.render_batch(ctx, items, path_radiance=paths, flags=abi.FF_RENDER_USE_GRAPH | abi.FF_RENDER_TIMING,
+                         spp_batch=kb)
         torch.cuda.synchronize()
         return {"workload": f"multiview{V}_icosphere{args.subdiv}_256x256_8spp_d6_tex1024", "spp_batch": kb}
     t_fb = _timed(fwd_batch, args.steps + 1)
