@@ -78,6 +78,32 @@ def test_graph_render_equals_plain_render(gpu):
     assert torch.equal(a, b)
 
 
+def test_overlapped_and_serial_schedules_agree(gpu):
+    """The shadow traversal of each iteration runs on a second stream beside
+    the extension traversal; with per-launch timing the stages run one after
+    another.  Eager, graph-replayed and serial renders give the same film and
+    the same ray and traversal counts, over three batches of samples."""
+    import torch
+    sd = scenes.spheres_scene(width=96, height=64, spp=12, max_depth=5)
+    sc = abi.Scene(gpu, sd.positions, sd.normals, sd.uvs, sd.material_ids, sd.materials)
+    env = abi.EnvMap(gpu, sd.env)
+    cam = abi.make_camera(sd.inv_vp)
+    keys = ("rays_primary", "rays_extension", "rays_shadow", "rays_shadow_gated", "paths", "ext_nodes",
+            "ext_tris", "shadow_nodes", "shadow_tris")
+    out = []
+    for flags in (0, abi.FF_RENDER_USE_GRAPH, abi.FF_RENDER_TIMING):
+        f = torch.zeros((sd.height, sd.width, 4), dtype=torch.float32, device="cuda")
+        abi.render(gpu, sc, env, cam, sd.width, sd.height, 12, 5, 3, f, spp_batch=4, flags=flags)
+        st = abi.last_render_stats(gpu)
+        out.append((f.cpu(), {k: getattr(st, k) for k in keys}))
+    sc.close()
+    env.close()
+    for f, st in out[1:]:
+        assert torch.equal(f, out[0][0])
+        assert st == out[0][1]
+    assert out[0][1]["rays_shadow"] > 0 and out[0][1]["rays_shadow_gated"] > 0
+
+
 def test_traversal_variants_agree(gpu):
     """Every traversal variant (workgroup, LDS stack, refill, loop shape, BVH
     record width) gives the same image."""
