@@ -1293,6 +1293,35 @@ void or_env_build(const float *rgb, int32_t w, int32_t h, double *marginal_y, do
     free(lum);
 }
 
+/* test hook: environment tables given by the caller (the same w x h map),
+ * used by the renders in place of or_env_build's serial sums */
+static const double *g_env_my = NULL, *g_env_mp = NULL, *g_env_conds = NULL;
+static const float *g_env_pdf = NULL;
+static int32_t g_env_w = 0, g_env_h = 0;
+void or_set_env_tables(const double *marginal_y, const double *marginal_p, const double *conds,
+                       const float *pdf, int32_t w, int32_t h) {
+    g_env_my = marginal_y;
+    g_env_mp = marginal_p;
+    g_env_conds = conds;
+    g_env_pdf = pdf;
+    g_env_w = w;
+    g_env_h = h;
+}
+/* the tables of a render: the caller's (or_set_env_tables) if they are for a
+ * map of this size, else or_env_build's */
+static void env_tables(const float *rgb, int32_t w, int32_t h, double *my, double *mp, double *conds,
+                       float *pdf) {
+    if (g_env_my && g_env_w == w && g_env_h == h) {
+        size_t wh = (size_t)w * h;
+        memcpy(my, g_env_my, sizeof(double) * (size_t)h);
+        memcpy(mp, g_env_mp, sizeof(double) * (size_t)h);
+        memcpy(conds, g_env_conds, sizeof(double) * wh);
+        memcpy(pdf, g_env_pdf, sizeof(float) * wh);
+        return;
+    }
+    or_env_build(rgb, w, h, my, mp, conds, pdf, NULL);
+}
+
 /* upper_bound with its trailing adjustment list[left <= val] (CRT/Helpers.cu:16-31) */
 int32_t or_upper_bound(const double *list, int32_t size, float val) {
     int32_t left = 0, right = size;
@@ -1944,7 +1973,7 @@ void or_render(const or_scene *s, const or_params *p, float *film, or_stats *st)
         c.mp = (double *)malloc(sizeof(double) * s->env_h);
         c.conds = (double *)malloc(sizeof(double) * wh);
         c.pdf = (float *)malloc(sizeof(float) * wh);
-        or_env_build(s->env_rgb, s->env_w, s->env_h, c.my, c.mp, c.conds, c.pdf, NULL);
+        env_tables(s->env_rgb, s->env_w, s->env_h, c.my, c.mp, c.conds, c.pdf);
     }
     c.film = film;
     c.next_row = p->row_begin;
@@ -2328,7 +2357,7 @@ static void rctx_init(rctx *c, const or_scene *s, const or_params *p) {
         c->mp = (double *)malloc(sizeof(double) * s->env_h);
         c->conds = (double *)malloc(sizeof(double) * wh);
         c->pdf = (float *)malloc(sizeof(float) * wh);
-        or_env_build(s->env_rgb, s->env_w, s->env_h, c->my, c->mp, c->conds, c->pdf, NULL);
+        env_tables(s->env_rgb, s->env_w, s->env_h, c->my, c->mp, c->conds, c->pdf);
     }
     c->next_row = p->row_begin;
     pthread_mutex_init(&c->mu, NULL);

@@ -126,6 +126,11 @@ void or_trace_walk(const or_node *nodes, const float *tris, const float *rays, i
 void or_env_build(const float *rgb, int32_t w, int32_t h, double *marginal_y,
                   double *marginal_p, double *conds, float *pdf, double *denom);
 int32_t or_upper_bound(const double *list, int32_t size, float val);
+/* test hook: renders use these tables (of a w x h map, as or_env_build lays
+ * them out) instead of building their own; NULL marginal_y = build again.
+ * The pointers must stay valid while renders use them. */
+void or_set_env_tables(const double *marginal_y, const double *marginal_p, const double *conds,
+                       const float *pdf, int32_t w, int32_t h);
 void or_env_sample_dir(const double *marginal_y, const double *conds, int32_t w,
                        int32_t h, float ex, float ey, int32_t *x, int32_t *y,
                        float *dir);

@@ -72,6 +72,7 @@ def lib() -> C.CDLL:
             "or_env_L": (None, [_P, C.c_int32, C.c_int32, _P, _P]),
             "or_concentric": (None, [_P, C.c_int32, _P]),
             "or_set_mutation": (None, [C.c_int32]),
+            "or_set_env_tables": (None, [_P, _P, _P, _P, C.c_int32, C.c_int32]),
             "or_gen_ray": (None, [C.POINTER(OrScene), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                   C.c_uint32, _P]),
             "or_ggx_ndf": (C.c_double, [_P, _P, C.c_float]),
@@ -224,6 +225,27 @@ def env_build(rgb):
     lib().or_env_build(a.ctypes.data, w, h, my.ctypes.data, mp.ctypes.data, conds.ctypes.data,
                        pdf.ctypes.data, denom.ctypes.data)
     return my, mp, conds, pdf, float(denom[0])
+
+
+class env_tables:
+    """Context manager: renders inside it use the given environment tables
+    (marginal_y [h], marginal_p [h], conditional CDFs [h][w], pdf [h][w]) in
+    place of the oracle's own serial FP64 build."""
+
+    def __init__(self, marginal_y, marginal_p, conds, pdf):
+        self.a = [np.ascontiguousarray(marginal_y, np.float64), np.ascontiguousarray(marginal_p, np.float64),
+                  np.ascontiguousarray(conds, np.float64), np.ascontiguousarray(pdf, np.float32)]
+        h = self.a[0].size
+        assert self.a[1].size == h and self.a[2].size == self.a[3].size and self.a[2].size % h == 0
+        self.w, self.h = self.a[2].size // h, h
+
+    def __enter__(self):
+        lib().or_set_env_tables(*[x.ctypes.data for x in self.a], self.w, self.h)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_set_env_tables(None, None, None, None, 0, 0)
+        return False
 
 
 def upper_bound(lst, val) -> int:

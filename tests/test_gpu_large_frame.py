@@ -94,6 +94,25 @@ def test_camera_and_bounce_hits_bit_exact(big, gpu, oracle):
     assert np.array_equal(occ.cpu().numpy(), oracle.trace_any(ps.nodes, ps.tris_ordered(), b))
 
 
+def _assert_exact_with_tables(ps, tables, g, st, **kw):
+    """The oracle's render (of kw's rows) with the GPU's environment tables
+    (FP64 sums in tree order; the oracle's own are the reference's serial
+    sums, so a light-sample uniform within a few ulps of a CDF entry can pick
+    the neighbouring texel) equals the GPU film in every bit."""
+    import sys
+    from oracle import oracle as O
+    with O.env_tables(*tables):
+        c, cst = ps.render(**kw)
+    r0, r1 = kw.get("rows", (0, g.shape[0]))
+    same = float(np.mean(np.all(g[r0:r1] == c[r0:r1], axis=-1)))
+    print(f"rows {r0}-{r1} with the GPU's CDF tables: {same:.6f} of pixels bit-identical", file=sys.stderr,
+          flush=True)
+    assert np.array_equal(g[r0:r1], c[r0:r1])
+    if st is not None:
+        for k in ("rays_primary", "rays_extension", "rays_shadow", "rays_shadow_gated", "paths"):
+            assert getattr(st, k) == getattr(cst, k), k
+
+
 def test_full_frame_one_sample_matches_oracle(big, gpu):
     """All 2.07 M paths of one sample per pixel at depth 8, every material
     queue and the importance-sampled sun: relative RMSE <= 1% of the oracle,
@@ -104,8 +123,11 @@ def test_full_frame_one_sample_matches_oracle(big, gpu):
     film = torch.zeros((sd.height, sd.width, 4), dtype=torch.float32, device="cuda")
     abi.render(gpu, sc, env, abi.make_camera(sd.inv_vp), sd.width, sd.height, 1, 8, sd.seed, film)
     g = film.cpu().numpy()
+    st = abi.last_render_stats(gpu)
+    tables = env.read()[:4]
     env.close()
     c, _ = ps.render(spp=1, max_depth=8, n_threads=THREADS)
+    _assert_exact_with_tables(ps, tables, g, st, spp=1, max_depth=8, n_threads=THREADS)
     assert np.isfinite(g).all() and np.all(g[..., 3] == 1)
     err = float(np.sqrt(np.mean((g[..., :3] - c[..., :3]) ** 2)) / np.sqrt(np.mean(c[..., :3] ** 2)))
     exact = float(np.mean(np.all(g[..., :3] == c[..., :3], axis=-1)))
@@ -137,8 +159,11 @@ def test_benchmark_frame_matches_oracle(big, gpu):
         abi.set_traversal_variant(gpu, 0)
     g = film.cpu().numpy()
     del film
+    tables = env.read()[:4]
     env.close()
     c, _ = ps.render(spp=64, max_depth=8, spp_batch=64, n_threads=THREADS)
+    _assert_exact_with_tables(ps, tables, g, None, spp=64, max_depth=8, spp_batch=64, n_threads=THREADS,
+                              rows=(500, 540))
     assert np.isfinite(g).all() and np.all(g[..., 3] == 64)
     err = float(np.sqrt(np.mean((g[..., :3] - c[..., :3]) ** 2)) / np.sqrt(np.mean(c[..., :3] ** 2)))
     exact = float(np.mean(np.all(g[..., :3] == c[..., :3], axis=-1)))

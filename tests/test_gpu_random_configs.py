@@ -6,7 +6,10 @@ against the oracle's render of the same scene with the same samples in flight
 (wf_logic / wf_mat_* / wf_extend / wf_shadow, CRT/kernel.cu:297-651): sample
 counts exact, at least 99 % of pixels bit-identical, relative RMSE under
 1e-3, and the ray counts (camera, extension, traced and gated shadow rays)
-within 1e-3 of the oracle's."""
+within 1e-3 of the oracle's.  Rendered with the GPU's environment tables
+(FP64 sums in tree order, not the reference's serial order) the oracle's film
+and ray counts equal the GPU's exactly: the pixels that differ are the ones
+with a light sample whose uniform lies within a few ulps of a CDF entry."""
 import numpy as np
 import pytest
 
@@ -57,13 +60,21 @@ def test_random_configuration_matches_oracle(gpu, oracle, case):
         st = abi.last_render_stats(gpu)
         g = f.cpu().numpy()
         nodes, ordered = sc.bvh()
+        tables = env.read()[:4]
     finally:
         abi.set_traversal_tail(gpu, -1, 1)
         abi.set_traversal_variant(gpu, 0)
         sc.close()
         env.close()
-    ref, cst = oracle.PreparedScene(sd, nodes, ordered).render(spp=c["spp"], max_depth=c["depth"], seed=c["seed"],
-                                                               spp_batch=c["K"])
+    ps = oracle.PreparedScene(sd, nodes, ordered)
+    kw = {"spp": c["spp"], "max_depth": c["depth"], "seed": c["seed"], "spp_batch": c["K"]}
+    ref, cst = ps.render(**kw)
+    with oracle.env_tables(*tables):
+        ref_t, cst_t = ps.render(**kw)
+    # with the GPU's environment tables every pixel and every ray count is equal
+    assert np.array_equal(g, ref_t), f"case {case}: film differs from the oracle's with the same CDF tables"
+    for k in ("rays_primary", "rays_extension", "rays_shadow", "rays_shadow_gated", "paths"):
+        assert getattr(st, k) == getattr(cst_t, k), f"case {case}: {k}"
     assert np.all(g[..., 3] == c["spp"]) and np.array_equal(g[..., 3], ref[..., 3])
     same = float(np.mean(np.all(g[..., :3] == ref[..., :3], axis=-1)))
     err = float(np.sqrt(np.mean((g[..., :3] - ref[..., :3]) ** 2)) / max(np.sqrt(np.mean(ref[..., :3] ** 2)), 1e-30))

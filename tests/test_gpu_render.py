@@ -45,16 +45,33 @@ def _render(gpu, sd, spp, max_depth, spp_batch, seed=None):
     st = abi.last_render_stats(gpu)
     out = film.cpu().numpy()
     nodes, ordered = sc.bvh()
+    tables = env.read()[:4]
     sc.close()
     env.close()
-    return out, st, nodes, ordered
+    return out, st, nodes, ordered, tables
+
+
+def assert_exact_with_tables(oracle, ps, tables, g, st=None, **kw):
+    """The oracle's render with the GPU's environment tables (its FP64 sums
+    in tree order; the oracle's own are serial, so a light-sample uniform
+    within a few ulps of a CDF entry can pick the neighbouring texel) equals
+    the GPU film in every bit, and the ray counts exactly."""
+    with oracle.env_tables(*tables):
+        c, cst = ps.render(**kw)
+    r0, r1 = kw.get("rows", (0, g.shape[0]))
+    same = float(np.mean(np.all(g[r0:r1] == c[r0:r1], axis=-1)))
+    assert np.array_equal(g[r0:r1], c[r0:r1]), f"GPU tables: {same:.5f} of pixels bit-identical"
+    if st is not None:
+        for k in ("rays_primary", "rays_extension", "rays_shadow", "rays_shadow_gated", "paths"):
+            assert getattr(st, k) == getattr(cst, k), k
 
 
 def test_render_spheres_matches_oracle(gpu, oracle):
     sd = scenes.spheres_scene()
-    g, st, nodes, ordered = _render(gpu, sd, 16, 4, 16)
+    g, st, nodes, ordered, tables = _render(gpu, sd, 16, 4, 16)
     ps = oracle.PreparedScene(sd, nodes, ordered)
     c, cst = ps.render(spp=16, max_depth=4, spp_batch=16)
+    assert_exact_with_tables(oracle, ps, tables, g, st, spp=16, max_depth=4, spp_batch=16)
     assert np.all(g[..., 3] == 16)
     assert np.isfinite(g).all()
     gi, ci = g[..., :3] / 16, c[..., :3] / 16
@@ -77,14 +94,16 @@ def test_thin_lens_render_matches_oracle(gpu, oracle):
     focused at 3.5 (spheres near the focus sharp, the ground blurred)."""
     import dataclasses
     sd = dataclasses.replace(scenes.spheres_scene(width=128, height=96), lens_radius=0.08, focal=3.5)
-    g, st, nodes, ordered = _render(gpu, sd, 16, 4, 16)
-    c, _ = oracle.PreparedScene(sd, nodes, ordered).render(spp=16, max_depth=4, spp_batch=16)
+    g, st, nodes, ordered, tables = _render(gpu, sd, 16, 4, 16)
+    ps = oracle.PreparedScene(sd, nodes, ordered)
+    c, _ = ps.render(spp=16, max_depth=4, spp_batch=16)
+    assert_exact_with_tables(oracle, ps, tables, g, st, spp=16, max_depth=4, spp_batch=16)
     assert np.all(g[..., 3] == 16) and np.isfinite(g).all()
     err = rel_rmse(g[..., :3], c[..., :3])
     exact = np.mean(np.all(g[..., :3] == c[..., :3], axis=-1))
     print(f"thin lens 128x96 16spp: rel RMSE {err:.3e}; bit-identical pixels {exact:.4f}")
     assert err < RMSE_TOL and exact > 0.999
-    pin, _, _, _ = _render(gpu, scenes.spheres_scene(width=128, height=96), 16, 4, 16)
+    pin = _render(gpu, scenes.spheres_scene(width=128, height=96), 16, 4, 16)[0]
     assert rel_rmse(g[..., :3], pin[..., :3]) > 0.02  # the lens changes the image
 
 
@@ -102,9 +121,10 @@ def test_render_textured_materials_match_oracle(gpu, oracle):
     mats[1::2, 11] = 1
     mats[1:, 12] = 2
     sd.materials = mats
-    g, st, nodes, ordered = _render(gpu, sd, 8, 4, 8)
+    g, st, nodes, ordered, tables = _render(gpu, sd, 8, 4, 8)
     ps = oracle.PreparedScene(sd, nodes, ordered)
     c, cst = ps.render(spp=8, max_depth=4, spp_batch=8)
+    assert_exact_with_tables(oracle, ps, tables, g, st, spp=8, max_depth=4, spp_batch=8)
     assert np.all(g[..., 3] == 8) and np.isfinite(g).all()
     err = rel_rmse(g[..., :3], c[..., :3])
     exact = np.mean(np.all(g[..., :3] == c[..., :3], axis=-1))
@@ -132,10 +152,14 @@ def test_render_normal_mapped_materials_match_oracle(gpu, oracle):
     film = torch.zeros((sd.height, sd.width, 4), dtype=torch.float32, device="cuda")
     abi.render(gpu, sc, env, abi.make_camera(sd.inv_vp), sd.width, sd.height, 8, 4, sd.seed, film, spp_batch=8)
     g = film.cpu().numpy()
+    st = abi.last_render_stats(gpu)
     nodes, ordered = sc.bvh()
+    tables = env.read()[:4]
     sc.close()
     env.close()
-    c, _ = oracle.PreparedScene(sd, nodes, ordered).render(spp=8, max_depth=4, spp_batch=8)
+    ps = oracle.PreparedScene(sd, nodes, ordered)
+    c, _ = ps.render(spp=8, max_depth=4, spp_batch=8)
+    assert_exact_with_tables(oracle, ps, tables, g, st, spp=8, max_depth=4, spp_batch=8)
     assert np.all(g[..., 3] == 8) and np.isfinite(g).all()
     err = rel_rmse(g[..., :3], c[..., :3])
     exact = np.mean(np.all(g[..., :3] == c[..., :3], axis=-1))
@@ -152,9 +176,10 @@ def test_render_material_queues_match_oracle(gpu, oracle, mtype):
         mats[1:, 4] = np.linspace(0.05, 0.5, len(mats) - 1)
         mats[1:, 5] = 1.0
     sd.materials = mats
-    g, st, nodes, ordered = _render(gpu, sd, 32, 6, 8)
+    g, st, nodes, ordered, tables = _render(gpu, sd, 32, 6, 8)
     ps = oracle.PreparedScene(sd, nodes, ordered)
     c, _ = ps.render(spp=32, max_depth=6, spp_batch=8)
+    assert_exact_with_tables(oracle, ps, tables, g, st, spp=32, max_depth=6, spp_batch=8)
     err = rel_rmse(g[..., :3], c[..., :3])
     print(f"material {mtype}: rel RMSE {err:.3e}")
     assert np.isfinite(g).all()

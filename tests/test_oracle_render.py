@@ -148,3 +148,25 @@ def test_gated_brdf_visibility_rays_change_no_pixel(oracle):
     assert sb.rays_shadow > sa.rays_shadow
     assert sb.rays_shadow == sa.rays_shadow + sa.rays_shadow_gated and sb.rays_shadow_gated == 0
     assert (sa.rays_primary, sa.rays_extension, sa.paths) == (sb.rays_primary, sb.rays_extension, sb.paths)
+
+
+def test_environment_tables_hook(oracle):
+    """Renders inside oracle.env_tables use the given CDF tables: the oracle's
+    own tables give its film bit for bit, conditional CDFs scaled by 1 + 1e-3
+    move light samples to other texels, and leaving the block restores the
+    oracle's own build."""
+    sd = scenes.spheres_scene(width=40, height=30, spp=2, max_depth=3)
+    ps = oracle.PreparedScene(sd)
+    a, sa = ps.render(n_threads=4)
+    my, mp, conds, pdf, _ = oracle.env_build(sd.env)
+    with oracle.env_tables(my, mp, conds, pdf):
+        b, sb = ps.render(n_threads=4)
+    assert np.array_equal(a, b) and sa.rays_shadow == sb.rays_shadow
+    with oracle.env_tables(my, mp, conds * (1 + 1e-3), pdf):
+        c, _ = ps.render(n_threads=4)
+    assert np.mean(np.all(a == c, axis=-1)) < 0.95
+    with oracle.env_tables(my[:-1], mp[:-1], conds[:-1], pdf[:-1]):  # another map size: not used
+        d, _ = ps.render(n_threads=4)
+    assert np.array_equal(a, d)
+    e, _ = ps.render(n_threads=4)
+    assert np.array_equal(a, e)
