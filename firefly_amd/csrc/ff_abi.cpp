@@ -622,6 +622,8 @@ ff_status ff_envmap_create(ff_context* ctx, const float* rgb, int32_t w, int32_t
     if (!ctx || !rgb || !out) return fail(FF_ERR_INVALID, "ff_envmap_create: NULL argument");
     *out = nullptr;
     if (w < 2 || h < 2) return fail(FF_ERR_INVALID, "ff_envmap_create: map must be at least 2x2 (got %dx%d)", w, h);
+    if ((uint32_t)w > ENV_GUIDE_INDEX || (uint32_t)h > ENV_GUIDE_INDEX)
+        return fail(FF_ERR_INVALID, "ff_envmap_create: map side above %u texels", ENV_GUIDE_INDEX);
     double total = 0.0;
     for (size_t i = 0; i < (size_t)w * h * 3; i++) {
         if (!(rgb[i] >= 0.f) || rgb[i] > 3.0e38f)

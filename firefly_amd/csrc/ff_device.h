@@ -303,27 +303,28 @@ struct EnvView {
 // upper_bound_ref over a non-decreasing list, starting from its guide table:
 // the guide brackets the first element greater than val (the bracket of
 // val's bucket), so the loop below finds the same index as a search of the
-// whole list, and the trailing adjustment is applied to it unchanged.  Lists
-// that are not monotone (guide entries flagged) and values outside [0, 1)
-// are searched whole, exactly as upper_bound_ref does.
+// whole list.  The trailing adjustment tests list[(float)left <= val], which
+// for val in [0, 1) is list[1] when left == 0 and list[0] otherwise; the
+// guide entries carry both tests.  Lists that are not monotone (guide entries
+// flagged) are searched whole, and values outside [0, 1) are searched whole
+// with the adjustment read from the list, exactly as upper_bound_ref does.
 __device__ __forceinline__ int upper_bound_guided(const double* __restrict__ list, int size,
                                                   const uint32_t* __restrict__ guide, float val) {
     int left = 0, right = size;
-    if (val >= 0.f && val < 1.f) {
-        uint32_t g = (uint32_t)(val * (float)ENV_GUIDE);  // exact: ENV_GUIDE is a power of two
-        if (g > ENV_GUIDE - 1) g = ENV_GUIDE - 1;
-        uint32_t a = ldg(guide + g), b = ldg(guide + g + 1);
-        if (((a | b) & ENV_GUIDE_FULL) == 0) {
-            left = (int)a;
-            right = (int)b;
-        }
+    if (!(val >= 0.f && val < 1.f)) return upper_bound_ref(list, size, val);
+    uint32_t g = (uint32_t)(val * (float)ENV_GUIDE);  // exact: ENV_GUIDE is a power of two
+    if (g > ENV_GUIDE - 1) g = ENV_GUIDE - 1;
+    uint32_t a = ldg(guide + g), b = ldg(guide + g + 1);
+    if (((a | b) & ENV_GUIDE_FULL) == 0) {
+        left = (int)(a & ENV_GUIDE_INDEX);
+        right = (int)(b & ENV_GUIDE_INDEX);
     }
     while (left < right) {
         int middle = (right - left) / 2 + left;
         if ((double)val >= ldg(list + middle)) left = middle + 1;
         else right = middle;
     }
-    if (left < size && ldg(list + (((float)left <= val) ? 1 : 0)) != 0.0) left++;
+    if (left < size && (a & (left == 0 ? ENV_GUIDE_NZ1 : ENV_GUIDE_NZ0)) != 0u) left++;
     return left;
 }
 // dEnvironmentLight::L (CRT/dEnvironmentLight.cu:66-76)

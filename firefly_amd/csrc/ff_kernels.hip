@@ -1588,7 +1588,8 @@ hipError_t trav_dispatch(int which, int variant, const ff_paths* P, int q, const
 // bound of g / ENV_GUIDE (first element greater, compared in double as the
 // sampler compares), so the upper bound of any val in [g, g+1) / ENV_GUIDE
 // lies in [entry g, entry g+1].  A list with a NaN or a decreasing step gets
-// flagged entries (searched whole).
+// flagged entries (searched whole).  Every entry also carries list[0] != 0
+// and list[1] != 0 (ENV_GUIDE_NZ0 / ENV_GUIDE_NZ1).
 __global__ void k_env_guide(const double* marg_y, const double* conds, int32_t w, int32_t h, uint32_t* guide_m,
                             uint32_t* guide_c) {
     const bool marg = (int)blockIdx.x == h;
@@ -1601,6 +1602,7 @@ __global__ void k_env_guide(const double* marg_y, const double* conds, int32_t w
         if (v != v || (i > 0 && v < list[i - 1])) mine = 1;
     }
     const int bad = __syncthreads_or(mine);
+    const uint32_t nz = (list[0] != 0.0 ? ENV_GUIDE_NZ0 : 0u) | (list[1] != 0.0 ? ENV_GUIDE_NZ1 : 0u);
     for (int g = threadIdx.x; g <= ENV_GUIDE; g += blockDim.x) {
         float val = (float)g / (float)ENV_GUIDE;
         int left = 0, right = size;
@@ -1609,7 +1611,7 @@ __global__ void k_env_guide(const double* marg_y, const double* conds, int32_t w
             if ((double)val >= list[middle]) left = middle + 1;
             else right = middle;
         }
-        out[g] = bad ? ENV_GUIDE_FULL : (uint32_t)left;
+        out[g] = (bad ? ENV_GUIDE_FULL : (uint32_t)left) | nz;
     }
 }
 

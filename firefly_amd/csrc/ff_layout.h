@@ -109,8 +109,13 @@ struct ff_dev_scene {
 // 1 / ENV_GUIDE) the first and last candidate index of the upper bound (two
 // buckets per texel of a 2048-texel row, so a search is about one step).
 // Entry with bit 31 set: the list is not monotone (NaN rows), search it whole.
+// Bits 29 and 30 of every entry of a list: list[0] != 0 and list[1] != 0, the
+// two values the search's trailing adjustment tests (no second fetch there).
 #define ENV_GUIDE 4096
 #define ENV_GUIDE_FULL 0x80000000u
+#define ENV_GUIDE_NZ0 0x20000000u
+#define ENV_GUIDE_NZ1 0x40000000u
+#define ENV_GUIDE_INDEX 0x1fffffffu
 
 struct ff_dev_env {
     const float* rgb;
