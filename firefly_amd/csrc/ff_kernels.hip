@@ -1023,9 +1023,11 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade(ff_paths P, int next_q, int 
                 P.sray[6 * id + 4] = f4(so.dd_o, 0.f);
                 P.sray[6 * id + 5] = f4(so.wi_d, 0.f);
             }
-            P.nee[3 * id] = f4(so.A, 0.f);
-            P.nee[3 * id + 1] = f4(so.Bs, 0.f);
-            P.nee[3 * id + 2] = f4(so.Cd, 0.f);
+            // wf_logic reads a term only where the state word asks for it
+            // (needS and needD imply needL)
+            if (needL) P.nee[3 * id] = f4(so.A, 0.f);
+            if (needS) P.nee[3 * id + 1] = f4(so.Bs, 0.f);
+            if (needD) P.nee[3 * id + 2] = f4(so.Cd, 0.f);
             // the throughput factor wf_logic applies, f * cos / pdf, formed here
             // with the same operations (logic then reads 16 B instead of 32)
             // and the continuation's state word rides in its w (the path
