@@ -151,9 +151,10 @@ struct ff_envmap {
     float* pdf = nullptr;
     uint32_t* guide_m = nullptr;
     uint32_t* guide_c = nullptr;
+    float4* rgbp = nullptr;
     ff_dev_env view() const {
         ff_dev_env e;
-        e.rgb = rgb; e.marg_y = marg_y; e.conds = conds; e.pdf = pdf; e.w = w; e.h = h;
+        e.rgb = rgb; e.rgbp = rgbp; e.marg_y = marg_y; e.conds = conds; e.pdf = pdf; e.w = w; e.h = h;
         e.guide_m = guide_m; e.guide_c = guide_c;
         return e;
     }
@@ -642,14 +643,14 @@ ff_status ff_envmap_create(ff_context* ctx, const float* rgb, int32_t w, int32_t
         dalloc(&e->marg_p, h) != hipSuccess || dalloc(&e->marg_y, h) != hipSuccess ||
         dalloc(&e->conds, wh) != hipSuccess || dalloc(&e->pdf, wh) != hipSuccess ||
         dalloc(&e->guide_m, ENV_GUIDE + 1) != hipSuccess ||
-        dalloc(&e->guide_c, (size_t)h * (ENV_GUIDE + 1)) != hipSuccess) {
+        dalloc(&e->guide_c, (size_t)h * (ENV_GUIDE + 1)) != hipSuccess || dalloc(&e->rgbp, wh) != hipSuccess) {
         ff_envmap_destroy(e);
         return fail(FF_ERR_ALLOC, "ff_envmap_create: device allocation failed");
     }
     hipError_t err = hipMemcpy(e->rgb, rgb, 12 * wh, hipMemcpyHostToDevice);
     if (err == hipSuccess)
         err = ffk::launch_env_build(e->rgb, w, h, e->lum, e->row_sum, e->denom, e->marg_p, e->marg_y, e->conds,
-                                    e->pdf, e->guide_m, e->guide_c, nullptr);
+                                    e->pdf, e->guide_m, e->guide_c, e->rgbp, nullptr);
     if (err == hipSuccess) err = hipDeviceSynchronize();
     if (err != hipSuccess) {
         ff_envmap_destroy(e);
@@ -666,7 +667,7 @@ ff_status ff_envmap_update(ff_envmap* e, const float* rgb, void* stream) {
     hipStream_t hs = (hipStream_t)stream;
     HIP_TRY(hipMemcpyAsync(e->rgb, rgb, 12 * wh, hipMemcpyDefault, hs));
     HIP_TRY(ffk::launch_env_build(e->rgb, e->w, e->h, e->lum, e->row_sum, e->denom, e->marg_p, e->marg_y, e->conds,
-                                  e->pdf, e->guide_m, e->guide_c, hs));
+                                  e->pdf, e->guide_m, e->guide_c, e->rgbp, hs));
     return FF_OK;
 }
 
@@ -696,7 +697,7 @@ ff_status ff_envmap_destroy(ff_envmap* e) {
     if (!e) return fail(FF_ERR_INVALID, "ff_envmap_destroy: NULL env");
     (void)hipDeviceSynchronize();
     dfree(e->rgb); dfree(e->lum); dfree(e->row_sum); dfree(e->denom); dfree(e->marg_p); dfree(e->marg_y);
-    dfree(e->conds); dfree(e->pdf); dfree(e->guide_m); dfree(e->guide_c);
+    dfree(e->conds); dfree(e->pdf); dfree(e->guide_m); dfree(e->guide_c); dfree(e->rgbp);
     delete e;
     return FF_OK;
 }

@@ -293,10 +293,9 @@ __device__ __forceinline__ int upper_bound_ref(const double* __restrict__ list, 
 }
 
 struct EnvView {
-    const float* rgb;      // [h][w][3]
+    const float4* rgbp;    // [h][w] (r, g, b, pdf)
     const double* marg_y;  // [h]
     const double* conds;   // [h][w]
-    const float* pdf;      // [h][w]
     const uint32_t* guide_m, *guide_c;
     int w, h;
 };
@@ -327,12 +326,23 @@ __device__ __forceinline__ int upper_bound_guided(const double* __restrict__ lis
     if (left < size && (a & (left == 0 ? ENV_GUIDE_NZ1 : ENV_GUIDE_NZ0)) != 0u) left++;
     return left;
 }
-// dEnvironmentLight::L (CRT/dEnvironmentLight.cu:66-76)
+// dEnvironmentLight::L (CRT/dEnvironmentLight.cu:66-76): bilinear() over the
+// (r, g, b) of the interleaved texels, one 16-B load per texel
 __device__ __forceinline__ f3 env_L(const EnvView& e, f3 wi) {
-    float u, v, o[3];
+    float u, v;
     sph_map(wi, u, v);
-    bilinear(e.rgb, e.w, e.h, 3, u, v, o);
-    return ld3(o);
+    float uu = u - floorf(u), vv = v - floorf(v);
+    float x = uu * (float)e.w - 0.5f, y = vv * (float)e.h - 0.5f;
+    float fx = floorf(x), fy = floorf(y);
+    float a = x - fx, b = y - fy;
+    int i0, j0, i1, j1;
+    wrap_texels(fx, fy, e.w, e.h, i0, j0, i1, j1);
+    float w00 = (1.f - a) * (1.f - b), w10 = a * (1.f - b), w01 = (1.f - a) * b, w11 = a * b;
+    float4 t00 = ldg4(e.rgbp + (size_t)j0 * e.w + i0), t10 = ldg4(e.rgbp + (size_t)j0 * e.w + i1);
+    float4 t01 = ldg4(e.rgbp + (size_t)j1 * e.w + i0), t11 = ldg4(e.rgbp + (size_t)j1 * e.w + i1);
+    return V3(((w00 * t00.x + w10 * t10.x) + w01 * t01.x) + w11 * t11.x,
+              ((w00 * t00.y + w10 * t10.y) + w01 * t01.y) + w11 * t11.y,
+              ((w00 * t00.z + w10 * t10.z) + w01 * t01.z) + w11 * t11.z);
 }
 // dEnvironmentLight::get_pdf(isect, wi) (CRT/dEnvironmentLight.cu:83-103)
 __device__ __forceinline__ float env_pdf(const EnvView& e, f3 wi) {
@@ -342,7 +352,7 @@ __device__ __forceinline__ float env_pdf(const EnvView& e, f3 wi) {
     px = px < 0 ? 0 : (px > e.w - 1 ? e.w - 1 : px);
     py = py < 0 ? 0 : (py > e.h - 1 ? e.h - 1 : py);
     float z = wi.z < -1.f ? -1.f : (wi.z > 1.f ? 1.f : wi.z);
-    return ldg(e.pdf + (size_t)py * e.w + px) / f_sin(f_acos(z));
+    return ldg4(e.rgbp + (size_t)py * e.w + px).w / f_sin(f_acos(z));
 }
 // dEnvironmentLight::get_direction (CRT/dEnvironmentLight.cu:41-58);
 // indices clamped at 0 where the reference would index out of range.

@@ -47,7 +47,8 @@ hipError_t launch_gen_rays(const ff_dev_camera& cam, int W, int H, uint32_t seed
                            uint32_t sample, float4* rays, hipStream_t s);
 hipError_t launch_env_build(const float* rgb, int w, int h, float* lum, double* row_sum,
                             double* denom, double* marg_p, double* marg_y, double* conds,
-                            float* pdf, uint32_t* guide_m, uint32_t* guide_c, hipStream_t s);
+                            float* pdf, uint32_t* guide_m, uint32_t* guide_c, float4* rgbp,
+                            hipStream_t s);
 hipError_t launch_film_resolve(const float4* film, uint32_t n, float exposure, float* avg,
                                uint8_t* rgba, hipStream_t s);
 }  // namespace ffk

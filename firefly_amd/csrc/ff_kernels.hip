@@ -46,7 +46,7 @@ __device__ __forceinline__ const ff_dev_env& env_of(const ff_paths& P, const ff_
 
 __device__ __forceinline__ EnvView env_view(const ff_dev_env& e) {
     EnvView v;
-    v.rgb = e.rgb; v.marg_y = e.marg_y; v.conds = e.conds; v.pdf = e.pdf; v.w = e.w; v.h = e.h;
+    v.rgbp = e.rgbp; v.marg_y = e.marg_y; v.conds = e.conds; v.w = e.w; v.h = e.h;
     v.guide_m = e.guide_m; v.guide_c = e.guide_c;
     return v;
 }
@@ -1445,6 +1445,12 @@ __global__ __launch_bounds__(FF_BLOCK) void k_env_conditional(const float* lum, 
     }
 }
 
+// the render-time texel array: (r, g, b, pdf) per texel
+__global__ void k_env_pack(const float* rgb, const float* pdf, uint32_t n, float4* rgbp) {
+    for (uint32_t k = gtid(); k < n; k += gstride())
+        rgbp[k] = make_float4(rgb[3 * (size_t)k], rgb[3 * (size_t)k + 1], rgb[3 * (size_t)k + 2], pdf[k]);
+}
+
 // single workgroup: marginal_y = inclusive scan of marginal_p (thread t
 // owns rows [t*c, (t+1)*c), block offsets scanned in order)
 __global__ __launch_bounds__(FF_BLOCK) void k_env_marginal_scan(const double* marg_p, int32_t h, double* marg_y) {
@@ -1804,7 +1810,7 @@ hipError_t launch_gen_rays(const ff_dev_camera& cam, int W, int H, uint32_t seed
 }
 hipError_t launch_env_build(const float* rgb, int w, int h, float* lum, double* row_sum, double* denom,
                             double* marg_p, double* marg_y, double* conds, float* pdf, uint32_t* guide_m,
-                            uint32_t* guide_c, hipStream_t s) {
+                            uint32_t* guide_c, float4* rgbp, hipStream_t s) {
     hipLaunchKernelGGL(k_env_lum, dim3(grid_for((uint64_t)w * h, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, rgb, w, h,
                        lum);
     hipLaunchKernelGGL(k_env_rows, dim3(h), dim3(FF_BLOCK), 0, s, lum, w, h, row_sum);
@@ -1812,6 +1818,8 @@ hipError_t launch_env_build(const float* rgb, int w, int h, float* lum, double* 
     hipLaunchKernelGGL(k_env_conditional, dim3(h), dim3(FF_BLOCK), 0, s, lum, w, h, denom, marg_p, conds, pdf);
     hipLaunchKernelGGL(k_env_marginal_scan, dim3(1), dim3(FF_BLOCK), 0, s, marg_p, h, marg_y);
     hipLaunchKernelGGL(k_env_guide, dim3(h + 1), dim3(FF_BLOCK), 0, s, marg_y, conds, w, h, guide_m, guide_c);
+    hipLaunchKernelGGL(k_env_pack, dim3(grid_for((uint64_t)w * h, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, rgb, pdf,
+                       (uint32_t)((size_t)w * h), rgbp);
     return hipGetLastError();
 }
 hipError_t launch_film_resolve(const float4* film, uint32_t n, float exposure, float* avg, uint8_t* rgba,

@@ -119,6 +119,7 @@ struct ff_dev_scene {
 
 struct ff_dev_env {
     const float* rgb;
+    const float4* rgbp;       // [h][w] (r, g, b, pdf): the texel and pdf lookups of rendering
     const double* marg_y;
     const double* conds;
     const float* pdf;
