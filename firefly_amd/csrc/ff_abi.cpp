@@ -60,7 +60,7 @@ struct PathStore {
     ff_paths p{};
     void release() {
         dfree(p.ray); dfree(p.hit); dfree(p.beta); dfree(p.Lp); dfree(p.fext); dfree(p.nee);
-        dfree(p.sray); dfree(p.vis); dfree(p.rngb); dfree(p.preq); dfree(p.q_ext[0]); dfree(p.q_ext[1]);
+        dfree(p.sray); dfree(p.vis); dfree(p.preq); dfree(p.q_ext[0]); dfree(p.q_ext[1]);
         for (int m = 0; m < 4; m++) dfree(p.q_mat[m]);
         dfree(p.q_shadow); dfree(p.layers); dfree(p.dlt);
         cap = 0;
@@ -174,7 +174,6 @@ ff_status ensure_paths(ff_context* c, size_t P) {
     HIP_TRY(dalloc(&p.nee, 3 * P));
     HIP_TRY(dalloc(&p.sray, 6 * P));
     HIP_TRY(dalloc(&p.vis, 3 * P));
-    HIP_TRY(dalloc(&p.rngb, P));
     HIP_TRY(dalloc(&p.preq, P));
     HIP_TRY(dalloc(&p.q_ext[0], P));
     HIP_TRY(dalloc(&p.q_ext[1], P));
@@ -998,7 +997,7 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
                 HIP_TRY(hipEventRecord(ctx->fork_ev, hs));
                 HIP_TRY(hipStreamWaitEvent(ss, ctx->fork_ev, 0));
             }
-            HIP_TRY(timed(0, [&] { return ffk::launch_extend(p, cur, it == 1 ? EXT_STATE_NONE : EXT_STATE_FEXT, var_at(it), hs); }));
+            HIP_TRY(timed(0, [&] { return ffk::launch_extend(p, cur, var_at(it), hs); }));
             HIP_TRY(timed_on(ss, 1, [&] { return ffk::launch_shadow(p, var_at(it), ss); }));
             if (!timing) {
                 HIP_TRY(hipEventRecord(ctx->join_ev, ss));
@@ -1022,7 +1021,7 @@ ff_status render_requests(ff_context* ctx, const std::vector<HostReq>& reqs, int
         HIP_TRY(timed(4, [&] { return ffk::launch_generate(p, hs); }));
         int cur = 0;
         for (int32_t it = 1; it <= max_depth + 1; it++) {
-            HIP_TRY(timed(0, [&] { return ffk::launch_extend(p, cur, it == 1 ? EXT_STATE_NONE : EXT_STATE_BETA, var_at(it), hs); }));
+            HIP_TRY(timed(0, [&] { return ffk::launch_extend(p, cur, var_at(it), hs); }));
             uint32_t mask = (1u << (cur == 0 ? C_EXT1 : C_EXT0)) | (1u << C_SHADOW) | (0xfu << C_MAT0);
             HIP_TRY(ffk::launch_reset(p.counters, mask, p.fetch, (1u << F_EXT) | (1u << F_SHD), hs));
             HIP_TRY(timed(2, [&] { return ffk::launch_logic_adj(p, cur, p.n_slots, hs); }));

@@ -20,7 +20,7 @@ extern const int kTravWidth[];  // BVH record width of the variant (2 or 4)
 hipError_t prepare_traversal(int variant);
 hipError_t launch_batch_begin(const ff_paths& P, int set, uint32_t value, hipStream_t s);
 hipError_t launch_generate(const ff_paths& P, hipStream_t s);
-hipError_t launch_extend(const ff_paths& P, int q, int state, int variant, hipStream_t s);
+hipError_t launch_extend(const ff_paths& P, int q, int variant, hipStream_t s);
 hipError_t launch_shadow(const ff_paths& P, int variant, hipStream_t s);
 hipError_t launch_reset(uint32_t* counters, uint32_t mask, uint32_t* fetch, uint32_t fetch_mask, hipStream_t s);
 hipError_t launch_logic(const ff_paths& P, int q, int iter, uint32_t n_max, hipStream_t s);

@@ -118,9 +118,8 @@ __global__ __launch_bounds__(FF_BLOCK) void k_generate(ff_paths P) {
         uint32_t base = rng_base(R.seed, pix, sample);
         f3 o, d;
         gen_ray(R.C, (int)R.W, (int)R.H, (int)(pix % R.W), (int)(pix / R.W), base, o, d);
-        P.ray[2 * id] = f4(o, 0.f);
-        P.ray[2 * id + 1] = f4(d, 0.f);
-        P.rngb[id] = base;
+        P.ray[2 * id] = f4(o, __uint_as_float(base));
+        P.ray[2 * id + 1] = f4(d, 0.f);  // state word 0: outside every surface
         if (P.n_req > 1) P.preq[id] = r;
         if (R.mode & RQ_ADJOINT) {
             // replay: L_rem starts as the stored radiance of this path
@@ -162,10 +161,9 @@ struct ExtSrc {  // wf_extend (CRT/kernel.cu:487-523): closest hit of every queu
             tris = S.tris;
         }
         o = xyz(P.ray[2 * id]);
-        d = xyz(P.ray[2 * id + 1]);
-        two = false;
-        if (P.ext_state == EXT_STATE_FEXT) two = (__float_as_uint(P.fext[id].w) & ST_INSIDE) != 0;
-        else if (P.ext_state == EXT_STATE_BETA) two = (__float_as_uint(P.beta[id].w) & ST_INSIDE) != 0;
+        float4 r1 = P.ray[2 * id + 1];
+        d = xyz(r1);
+        two = (__float_as_uint(r1.w) & ST_INSIDE) != 0;
     }
     __device__ __forceinline__ void hit(uint32_t id, const Hit& h) const {
         P.hit[id] = make_float4(h.t, h.u, h.v, __int_as_float(h.id));
@@ -638,7 +636,7 @@ __global__ __launch_bounds__(FF_BLOCK, NP == 1 ? 8 : 5) void k_logic(ff_paths P,
                 nD[j] = xyz(P.nee[3 * id[j] + 2]);
             }
             rb[j] = 0u;
-            if (ext && found[j] && len[j] > 3) rb[j] = P.rngb[id[j]];
+            if (ext && found[j] && len[j] > 3) rb[j] = __float_as_uint(ldg(&P.ray[2 * id[j]].w));
         }
         // the NEE sum of the previous vertex, gated by its visibilities
         f3 lb[NP];
@@ -750,6 +748,7 @@ struct ShadeOut {
     float pdf_ext, cos_ext;
     f3 next_o, next_d;
     uint32_t nst;       // state of the continuation
+    uint32_t base;      // the path's rng base (rides with the continuation ray)
     // GRAD only
     bool a_on, rough_live;
     int at, rt;
@@ -778,7 +777,8 @@ __device__ __forceinline__ void shade_core(const ff_paths& P, uint32_t id, Shade
     int slot = __float_as_int(hh.w);
     float4 r0 = P.ray[2 * id], r1 = P.ray[2 * id + 1];
     f3 o = xyz(r0), d = xyz(r1);
-    uint32_t base = P.rngb[id];
+    uint32_t base = __float_as_uint(r0.w);
+    so.base = base;
     f3 Pp = vadd(o, vscl(d, hh.x));
     // the slot's 64-B shading record: normals n0 n1 n2, uvs t0 t1 t2, material word
     const float4* A = S.attr + (size_t)slot * 4;
@@ -1033,10 +1033,11 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade(ff_paths P, int next_q, int 
             // and the continuation's state word rides in its w (the path
             // record's beta is left to wf_logic)
             bool dead = vzero(so.fext) || so.pdf_ext == 0.f;
-            P.fext[id] = dead ? make_float4(0.f, 0.f, 0.f, __uint_as_float(so.nst | ST_DEAD))
-                              : f4(vdiv(vscl(so.fext, so.cos_ext), so.pdf_ext), __uint_as_float(so.nst));
-            P.ray[2 * id] = f4(so.next_o, 0.f);
-            P.ray[2 * id + 1] = f4(so.next_d, 0.f);
+            uint32_t sw = dead ? so.nst | ST_DEAD : so.nst;
+            P.fext[id] = dead ? make_float4(0.f, 0.f, 0.f, __uint_as_float(sw))
+                              : f4(vdiv(vscl(so.fext, so.cos_ext), so.pdf_ext), __uint_as_float(sw));
+            P.ray[2 * id] = f4(so.next_o, __uint_as_float(so.base));
+            P.ray[2 * id + 1] = f4(so.next_d, __uint_as_float(sw));
         }
         // one shadow-queue entry per event with a light ray (needS / needD
         // imply needL); the shadow stage traces and counts its rays
@@ -1147,7 +1148,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_logic_adj(ff_paths P, int q) {
             }
             if (!(stw & ST_DEAD) && len > 3) {
                 float qrr = fmaxf(0.05f, 1.f - beta.y);
-                if (rng(P.rngb[id], len, DIM_RR) < qrr) term = true;
+                if (rng(__float_as_uint(ldg(&P.ray[2 * id].w)), len, DIM_RR) < qrr) term = true;
             }
             if (!term) {
                 uint32_t nst = (stw & (ST_INSIDE | ST_PREV_DELTA)) | len;
@@ -1278,8 +1279,8 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade_adj(ff_paths P, int next_q) 
                 P.Lp[id] = f4(Lrem, 0.f);
             }
             P.beta[id] = f4(beta, __uint_as_float(nst));
-            P.ray[2 * id] = f4(so.next_o, 0.f);
-            P.ray[2 * id + 1] = f4(so.next_d, 0.f);
+            P.ray[2 * id] = f4(so.next_o, __uint_as_float(so.base));
+            P.ray[2 * id + 1] = f4(so.next_d, __uint_as_float(nst));
         }
         uint32_t cnt[1] = {live ? 1u : 0u}, first[1];
         block_alloc<FF_BLOCK, 1>(P.counters, qidx, cnt, first, lds);
@@ -1669,10 +1670,8 @@ hipError_t launch_generate(const ff_paths& P, hipStream_t s) {
     hipLaunchKernelGGL(k_generate, dim3(grid_for(P.n_slots, FF_GRID_CAP)), dim3(FF_BLOCK), 0, s, P);
     return hipGetLastError();
 }
-hipError_t launch_extend(const ff_paths& P, int q, int state, int variant, hipStream_t s) {
-    ff_paths E = P;
-    E.ext_state = state;
-    return trav_dispatch(0, variant, &E, q, nullptr, s);
+hipError_t launch_extend(const ff_paths& P, int q, int variant, hipStream_t s) {
+    return trav_dispatch(0, variant, &P, q, nullptr, s);
 }
 hipError_t launch_shadow(const ff_paths& P, int variant, hipStream_t s) {
     return trav_dispatch(1, variant, &P, 0, nullptr, s);

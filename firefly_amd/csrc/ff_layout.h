@@ -6,8 +6,9 @@
 //             (both child boxes + child refs; 32 B per box tested)
 //   tris    : 9 floats per slot, v0 | e1 | e2 (36 B, positions only)
 //   nrm/uv  : shading attributes per slot, read once per closest hit
-//   paths   : ray (2 x float4), hit, beta+state, Lp, fext, nee[3], sray[3][2],
-//             vis[3], rng base; queues of u32 path ids with device counters.
+//   paths   : ray (2 x float4, carrying the rng base and the state word in
+//             their w), hit, beta+state, Lp, fext, nee[3], sray[3][2], vis[3];
+//             queues of u32 path ids with device counters.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -29,12 +30,6 @@
 #define ST_NEED_S (1u << 11)
 #define ST_NEED_D (1u << 12)
 #define ST_DEAD (1u << 13)   // the continuation had zero f or pdf
-// Where the extension stage finds a path's state word (its inside flag makes
-// the closest-hit test two-sided): none on camera rays, fext.w after a
-// forward shading pass, beta.w after an adjoint replay pass.
-#define EXT_STATE_NONE 0
-#define EXT_STATE_FEXT 1
-#define EXT_STATE_BETA 2
 // Per-triangle material word: the material's type (0-3) in the top two bits,
 // so wf_logic routes a hit with one load.
 #define MAT_TYPE_SHIFT 30
@@ -168,7 +163,8 @@ struct ff_dev_request {
 #define B_COUNT 4
 
 struct ff_paths {
-    float4* ray;     // [P][2]  (o, -) (d, -)
+    float4* ray;     // [P][2]  (o, rng base bits) (d, state word bits: 0 on camera rays; its
+                     //         inside flag makes the closest-hit test two-sided)
     float4* hit;     // [P]     (t, u, v, slot)
     float4* beta;    // [P]     (beta.xyz, state)
     float4* Lp;      // [P]     (path radiance, -)
@@ -176,7 +172,6 @@ struct ff_paths {
     float4* nee;     // [P][3]  light term, spec term, diffuse term
     float4* sray;    // [P][3][2] shadow rays
     uint8_t* vis;    // [P][3]
-    uint32_t* rngb;  // [P]
     uint32_t* preq;  // [P]     request of the slot (stored only when n_req > 1)
     uint32_t* q_ext[2];
     uint32_t* q_mat[4];
@@ -194,7 +189,6 @@ struct ff_paths {
     uint32_t n_slots;    // total slots over all requests
     int32_t uni_scene;   // every request of the call renders the same scene
     int32_t uni_env;     // every request of the call uses the same environment
-    int32_t ext_state;   // where an extension ray's state word is: EXT_STATE_*
     // adjoint replay
     float4* dlt;         // [P]     d loss / d path radiance (xyz)
     uint32_t* g_max;     // arena: max |term| bits per gradient element (GM_BOUND)
