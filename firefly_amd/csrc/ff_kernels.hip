@@ -763,20 +763,32 @@ struct ShadeOut {
 // sample of wf_logic (CRT/kernel.cu:435-439) is drawn here; the visibility
 // rays of BRDF_L's BRDF-sampled MIS terms (CRT/dMaterial.cu:404-448) are
 // deferred to the shadow stage with their contributions kept per path.
+// Where a vertex's state word comes from (length, inside, previous delta):
+#define SW_RECORD 0   // the path record's beta.w (adjoint replay)
+#define SW_START 1    // the first vertex of a forward path: the start record's
+#define SW_RAY 2      // a later forward vertex: the ray's .w holds the previous
+                      // shading word, which wf_logic turns into this vertex's
+                      // word by adding one to the length and keeping the flags
 template <int TYPE, bool GRAD, bool UNI>
-__device__ __forceinline__ void shade_core(const ff_paths& P, uint32_t id, ShadeOut& so, bool first = false) {
+__device__ __forceinline__ void shade_core(const ff_paths& P, uint32_t id, ShadeOut& so, int sw_src = SW_RECORD) {
     const ff_dev_request& R = P.reqs[req_of(P, id)];
     const ff_dev_scene& S = scene_of<UNI>(P, R);
     EnvView E = env_view(env_of<UNI>(P, R));
-    // the first vertex of a forward path: its record is the start record
-    float4 bw = first ? make_float4(1.f, 1.f, 1.f, __uint_as_float(ST_PREV_DELTA | 1u)) : P.beta[id];
-    uint32_t stw = __float_as_uint(bw.w);
-    uint32_t len = stw & ST_LEN_MASK;
-    bool inside = (stw & ST_INSIDE) != 0;
     float4 hh = P.hit[id];
     int slot = __float_as_int(hh.w);
     float4 r0 = P.ray[2 * id], r1 = P.ray[2 * id + 1];
     f3 o = xyz(r0), d = xyz(r1);
+    uint32_t stw;
+    if (sw_src == SW_START) {
+        stw = ST_PREV_DELTA | 1u;
+    } else if (sw_src == SW_RAY) {
+        uint32_t w = __float_as_uint(r1.w);
+        stw = (w & (ST_INSIDE | ST_PREV_DELTA)) | ((w & ST_LEN_MASK) + 1u);
+    } else {
+        stw = __float_as_uint(P.beta[id].w);
+    }
+    uint32_t len = stw & ST_LEN_MASK;
+    bool inside = (stw & ST_INSIDE) != 0;
     uint32_t base = __float_as_uint(r0.w);
     so.base = base;
     f3 Pp = vadd(o, vscl(d, hh.x));
@@ -1009,7 +1021,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade(ff_paths P, int next_q, int 
         bool needL = false, needS = false, needD = false;
         if (live) {
             ShadeOut so;
-            shade_core<TYPE, false, UNI>(P, id, so, first != 0);
+            shade_core<TYPE, false, UNI>(P, id, so, first ? SW_START : SW_RAY);
             needL = so.needL; needS = so.needS; needD = so.needD;
             if (needL) {
                 P.sray[6 * id + 0] = f4(so.lo, 0.f);
