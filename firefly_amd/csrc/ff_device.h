@@ -1049,6 +1049,104 @@ __device__ __forceinline__ void block_alloc(uint32_t* counters, const int* qidx,
     __syncthreads();
 }
 
+// Queue appends staged in LDS.  A device-scope atomic on one address costs
+// about 11 ns at the memory side however many workgroups issue it, so one
+// atomic per queue per 256-path tile (a pass over 133 M paths: 520 k of them
+// on each counter) bounds a kernel at several milliseconds.  A workgroup
+// instead collects the entries of several tiles in LDS and appends each
+// queue's run with one atomic when fewer than BS free places remain (and at
+// the end): CAP / BS times fewer atomics, and the runs are written coalesced.
+// Each lane adds at most one entry per add(), so the staged count grows by at
+// most BS per add(): flush_if_full(..., adds) makes room for the next `adds`
+// calls of add() (a workgroup that stages two entries per thread per tile
+// passes 2).  Every thread of the workgroup calls add(), flush_if_full() and
+// flush() the same number of times.
+template <int BS, int NQ, int CAP>
+struct QStage {
+    static constexpr int NW = BS / 64;
+    static_assert(CAP >= 4 * BS && NQ < BS && NQ <= 16, "staging room for several adds");
+    uint32_t val[CAP];       // entry values
+    uint32_t pos[CAP];       // queue << 28 | rank in the queue's run
+    uint32_t wtot[NQ][NW];   // this add's entries per queue and wave
+    uint32_t fill[NQ];       // entries staged per queue
+    uint32_t base[NQ];       // first queue place of each run (flush)
+    uint32_t total;          // entries staged
+
+    __device__ __forceinline__ void init() {
+        if (threadIdx.x < NQ) fill[threadIdx.x] = 0u;
+        if (threadIdx.x == 0) total = 0u;
+        __syncthreads();
+    }
+    // q: the lane's queue (-1: no entry), v: the entry
+    __device__ __forceinline__ void add(int q, uint32_t v) {
+        const uint32_t w = threadIdx.x >> 6;
+        uint32_t rank_q = 0u;
+#pragma unroll
+        for (int k = 0; k < NQ; k++) {
+            uint64_t b = __ballot(q == k);
+            uint32_t r = __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+            if (q == k) rank_q = r;
+            if (__lane_id() == 0) wtot[k][w] = (uint32_t)__popcll(b);
+        }
+        uint64_t ball = __ballot(q >= 0);
+        uint32_t rank_all = __builtin_amdgcn_mbcnt_hi((uint32_t)(ball >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)ball, 0u));
+        __syncthreads();
+        if (q >= 0) {
+            uint32_t off_all = 0u, off_q = 0u;
+            for (uint32_t ww = 0; ww < w; ww++) {
+#pragma unroll
+                for (int k = 0; k < NQ; k++) off_all += wtot[k][ww];
+                off_q += wtot[q][ww];
+            }
+            uint32_t slot = total + off_all + rank_all;
+            val[slot] = v;
+            pos[slot] = ((uint32_t)q << 28) | (fill[q] + off_q + rank_q);
+        }
+        __syncthreads();
+        if (threadIdx.x <= (unsigned)NQ) {
+            uint32_t t = 0u;
+            if (threadIdx.x < (unsigned)NQ) {
+                for (int ww = 0; ww < NW; ww++) t += wtot[threadIdx.x][ww];
+                fill[threadIdx.x] += t;
+            } else {
+                for (int k = 0; k < NQ; k++)
+                    for (int ww = 0; ww < NW; ww++) t += wtot[k][ww];
+                total += t;
+            }
+        }
+        __syncthreads();
+    }
+    // appends every staged run: queue q's entries go to queues[q] from the
+    // place an atomic on counters[qidx[q]] returns
+    __device__ __forceinline__ void flush(uint32_t* counters, const int* qidx, uint32_t* const* queues) {
+        if (threadIdx.x < (unsigned)NQ) {
+            uint32_t f = fill[threadIdx.x];
+            base[threadIdx.x] = f ? atomicAdd(&counters[qidx[threadIdx.x]], f) : 0u;
+        }
+        __syncthreads();
+        const uint32_t T = total;
+        for (uint32_t k = threadIdx.x; k < T; k += BS) {
+            uint32_t p = pos[k];
+            uint32_t q = p >> 28;
+            uint32_t* dst = queues[0];
+#pragma unroll
+            for (int j = 1; j < NQ; j++)
+                if (q == (uint32_t)j) dst = queues[j];
+            dst[base[q] + (p & 0x0fffffffu)] = val[k];
+        }
+        __syncthreads();
+        if (threadIdx.x < (unsigned)NQ) fill[threadIdx.x] = 0u;
+        if (threadIdx.x == 0) total = 0u;
+        __syncthreads();
+    }
+    __device__ __forceinline__ void flush_if_full(uint32_t* counters, const int* qidx, uint32_t* const* queues,
+                                                  uint32_t adds = 1u) {
+        // total is the same for every thread; afterwards total + adds * BS <= CAP
+        if (total + adds * BS > CAP) flush(counters, qidx, queues);
+    }
+};
+
 // Segmented work fetch of the persistent traversal: queue [0, n) is split
 // into FETCH_SEGS segments with a counter each; a wave draws from its current
 // segment and, once that is exhausted, finds the next segment with work left

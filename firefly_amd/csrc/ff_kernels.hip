@@ -566,7 +566,10 @@ __global__ void k_reset_counters(uint32_t* c, uint32_t mask, uint32_t* fetch, ui
 template <bool UNI, int NP>
 __global__ __launch_bounds__(FF_BLOCK, NP == 1 ? 8 : 5) void k_logic(ff_paths P, int q, int iter) {
     const bool primary = iter == 1;
-    __shared__ uint32_t lds[4 * (FF_BLOCK / 64 + 1)];
+    __shared__ QStage<FF_BLOCK, 4, QSTAGE_CAP> st;
+    const int qidx[4] = {C_MAT0, C_MAT0 + 1, C_MAT0 + 2, C_MAT0 + 3};
+    uint32_t* const qs[4] = {P.q_mat[0], P.q_mat[1], P.q_mat[2], P.q_mat[3]};
+    st.init();
     const uint32_t n = P.counters[q == 0 ? C_EXT0 : C_EXT1];
     const uint32_t* queue = P.q_ext[q];
     const bool first_batch = P.bstate[B_BATCH] == 0;
@@ -711,25 +714,11 @@ __global__ __launch_bounds__(FF_BLOCK, NP == 1 ? 8 : 5) void k_logic(ff_paths P,
             c_ext += ln == 1 ? 0u : 1u;
             c_paths += tm ? 1u : 0u;
         }
-        const int qidx[4] = {C_MAT0, C_MAT0 + 1, C_MAT0 + 2, C_MAT0 + 3};
-        uint32_t cnt[4], first[4];
 #pragma unroll
-        for (int m = 0; m < 4; m++) {
-            cnt[m] = 0u;
-#pragma unroll
-            for (int j = 0; j < NP; j++) cnt[m] += route[j] == m ? 1u : 0u;
-        }
-        block_alloc<FF_BLOCK, 4>(P.counters, qidx, cnt, first, lds);
-#pragma unroll
-        for (int j = 0; j < NP; j++) {
-            if (route[j] >= 0) {
-                uint32_t f = route[j] == 0 ? first[0] : (route[j] == 1 ? first[1] : (route[j] == 2 ? first[2] : first[3]));
-#pragma unroll
-                for (int k = 0; k < j; k++) f += route[k] == route[j] ? 1u : 0u;
-                P.q_mat[route[j]][f] = id[j];
-            }
-        }
+        for (int j = 0; j < NP; j++) st.add(route[j], id[j]);
+        st.flush_if_full(P.counters, qidx, qs, NP);  // room for the next tile's NP adds
     }
+    st.flush(P.counters, qidx, qs);
     unsigned long long* sr = stats_rep(P);
     wave_sum_add(sr + S_PRIMARY, c_prim);
     wave_sum_add(sr + S_EXTENSION, c_ext);
@@ -737,6 +726,19 @@ __global__ __launch_bounds__(FF_BLOCK, NP == 1 ? 8 : 5) void k_logic(ff_paths P,
 }
 
 // ======================================================= material queues
+// Every shaded event continues to the next extension stage, so the next
+// extension queue is the material queues one after another: material queue
+// TYPE starts after the entries of the types below it.  The first workgroup
+// adds the queue's length to the extension counter (one atomic per launch).
+template <int TYPE>
+__device__ __forceinline__ uint32_t ext_base(const ff_paths& P, int next_q, uint32_t n) {
+    uint32_t b = 0u;
+#pragma unroll
+    for (int t = 0; t < TYPE; t++) b += P.counters[C_MAT0 + t];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(&P.counters[next_q == 0 ? C_EXT0 : C_EXT1], n);
+    return b;
+}
+
 // Everything the shading of one vertex produces.  With GRAD the partial
 // derivatives of its terms are produced too (path-replay adjoint).
 struct ShadeOut {
@@ -1010,11 +1012,14 @@ __device__ __forceinline__ void shade_core(const ff_paths& P, uint32_t id, Shade
 
 template <int TYPE, bool UNI>
 __global__ __launch_bounds__(FF_BLOCK) void k_shade(ff_paths P, int next_q, int first) {
-    __shared__ uint32_t lds[2 * (FF_BLOCK / 64 + 1)];
+    __shared__ QStage<FF_BLOCK, 1, QSTAGE_CAP> st;
     const uint32_t n = P.counters[C_MAT0 + TYPE];
     const uint32_t* queue = P.q_mat[TYPE];
     const uint32_t n_round = (n + FF_BLOCK - 1) / FF_BLOCK * FF_BLOCK;
-    const int qidx[2] = {next_q == 0 ? C_EXT0 : C_EXT1, C_SHADOW};
+    const int qidx[1] = {C_SHADOW};
+    uint32_t* const qs[1] = {P.q_shadow};
+    const uint32_t ebase = ext_base<TYPE>(P, next_q, n);
+    st.init();
     for (uint32_t i = gtid(); i < n_round; i += gstride()) {
         bool live = i < n;
         uint32_t id = live ? queue[i] : 0u;
@@ -1051,13 +1056,15 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade(ff_paths P, int next_q, int 
             P.ray[2 * id] = f4(so.next_o, __uint_as_float(so.base));
             P.ray[2 * id + 1] = f4(so.next_d, __uint_as_float(sw));
         }
-        // one shadow-queue entry per event with a light ray (needS / needD
-        // imply needL); the shadow stage traces and counts its rays
-        uint32_t cnt[2] = {live ? 1u : 0u, needL ? 1u : 0u}, first[2];
-        block_alloc<FF_BLOCK, 2>(P.counters, qidx, cnt, first, lds);
-        if (live) P.q_ext[next_q][first[0]] = id;
-        if (needL) P.q_shadow[first[1]] = (id << 2) | (needS ? 1u : 0u) | (needD ? 2u : 0u);
+        // every event continues (wf_logic decides its fate): the material
+        // queues follow one another in the next extension queue.  One
+        // shadow-queue entry per event with a light ray (needS / needD imply
+        // needL); the shadow stage traces and counts its rays.
+        if (live) P.q_ext[next_q][ebase + i] = id;
+        st.add(needL ? 0 : -1, (id << 2) | (needS ? 1u : 0u) | (needD ? 2u : 0u));
+        st.flush_if_full(P.counters, qidx, qs);
     }
+    st.flush(P.counters, qidx, qs);
 }
 
 // ================================================== path-replay adjoint
@@ -1120,7 +1127,10 @@ __device__ __forceinline__ void grad_tex(const ff_paths& P, const ff_dev_request
 // NEE terms and throughput update were replayed by k_shade_adj.
 template <bool UNI>
 __global__ __launch_bounds__(FF_BLOCK) void k_logic_adj(ff_paths P, int q) {
-    __shared__ uint32_t lds[4 * (FF_BLOCK / 64 + 1)];
+    __shared__ QStage<FF_BLOCK, 4, QSTAGE_CAP> st;
+    const int qidx[4] = {C_MAT0, C_MAT0 + 1, C_MAT0 + 2, C_MAT0 + 3};
+    uint32_t* const qs[4] = {P.q_mat[0], P.q_mat[1], P.q_mat[2], P.q_mat[3]};
+    st.init();
     const uint32_t n = P.counters[q == 0 ? C_EXT0 : C_EXT1];
     const uint32_t* queue = P.q_ext[q];
     const uint32_t n_round = (n + FF_BLOCK - 1) / FF_BLOCK * FF_BLOCK;  // uniform trip count per workgroup
@@ -1172,16 +1182,10 @@ __global__ __launch_bounds__(FF_BLOCK) void k_logic_adj(ff_paths P, int q) {
         c_prim += (live && primary) ? 1u : 0u;
         c_ext += (live && !primary) ? 1u : 0u;
         c_paths += (live && term) ? 1u : 0u;
-        const int qidx[4] = {C_MAT0, C_MAT0 + 1, C_MAT0 + 2, C_MAT0 + 3};
-        uint32_t cnt[4], first[4];
-#pragma unroll
-        for (int m = 0; m < 4; m++) cnt[m] = route == m ? 1u : 0u;
-        block_alloc<FF_BLOCK, 4>(P.counters, qidx, cnt, first, lds);
-        if (route >= 0) {
-            uint32_t f = route == 0 ? first[0] : (route == 1 ? first[1] : (route == 2 ? first[2] : first[3]));
-            P.q_mat[route][f] = id;
-        }
+        st.add(route, id);
+        st.flush_if_full(P.counters, qidx, qs);
     }
+    st.flush(P.counters, qidx, qs);
     unsigned long long* sr = stats_rep(P);
     wave_sum_add(sr + S_PRIMARY, c_prim);
     wave_sum_add(sr + S_EXTENSION, c_ext);
@@ -1191,11 +1195,13 @@ __global__ __launch_bounds__(FF_BLOCK) void k_logic_adj(ff_paths P, int q) {
 // Replay of a vertex's shadow rays: the same rays k_shade queues.
 template <int TYPE, bool UNI>
 __global__ __launch_bounds__(FF_BLOCK) void k_shade_nee(ff_paths P) {
-    __shared__ uint32_t lds[FF_BLOCK / 64 + 1];
+    __shared__ QStage<FF_BLOCK, 1, QSTAGE_CAP> st;
     const uint32_t n = P.counters[C_MAT0 + TYPE];
     const uint32_t* queue = P.q_mat[TYPE];
     const uint32_t n_round = (n + FF_BLOCK - 1) / FF_BLOCK * FF_BLOCK;
     const int qidx[1] = {C_SHADOW};
+    uint32_t* const qs[1] = {P.q_shadow};
+    st.init();
     for (uint32_t i = gtid(); i < n_round; i += gstride()) {
         bool live = i < n;
         uint32_t id = live ? queue[i] : 0u;
@@ -1217,10 +1223,10 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade_nee(ff_paths P) {
                 P.sray[6 * id + 5] = f4(so.wi_d, 0.f);
             }
         }
-        uint32_t cnt[1] = {needL ? 1u : 0u}, first[1];
-        block_alloc<FF_BLOCK, 1>(P.counters, qidx, cnt, first, lds);
-        if (needL) P.q_shadow[first[0]] = (id << 2) | (needS ? 1u : 0u) | (needD ? 2u : 0u);
+        st.add(needL ? 0 : -1, (id << 2) | (needS ? 1u : 0u) | (needD ? 2u : 0u));
+        st.flush_if_full(P.counters, qidx, qs);
     }
+    st.flush(P.counters, qidx, qs);
 }
 
 // Replay of a vertex once its shadow rays are traced: its visibility-gated
@@ -1229,11 +1235,10 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade_nee(ff_paths P) {
 // next extension ray.
 template <int TYPE, bool UNI>
 __global__ __launch_bounds__(FF_BLOCK) void k_shade_adj(ff_paths P, int next_q) {
-    __shared__ uint32_t lds[FF_BLOCK / 64 + 1];
     const uint32_t n = P.counters[C_MAT0 + TYPE];
     const uint32_t* queue = P.q_mat[TYPE];
     const uint32_t n_round = (n + FF_BLOCK - 1) / FF_BLOCK * FF_BLOCK;
-    const int qidx[1] = {next_q == 0 ? C_EXT0 : C_EXT1};
+    const uint32_t ebase = ext_base<TYPE>(P, next_q, n);
     for (uint32_t i = gtid(); i < n_round; i += gstride()) {
         bool live = i < n;
         uint32_t id = live ? queue[i] : 0u;
@@ -1294,9 +1299,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_shade_adj(ff_paths P, int next_q) 
             P.ray[2 * id] = f4(so.next_o, __uint_as_float(so.base));
             P.ray[2 * id + 1] = f4(so.next_d, __uint_as_float(nst));
         }
-        uint32_t cnt[1] = {live ? 1u : 0u}, first[1];
-        block_alloc<FF_BLOCK, 1>(P.counters, qidx, cnt, first, lds);
-        if (live) P.q_ext[next_q][first[0]] = id;
+        if (live) P.q_ext[next_q][ebase + i] = id;
     }
 }
 

@@ -21,6 +21,7 @@
 // ff_trace_closest_noalloc (with the work counters: 64 MiB + 24 KiB per device)
 #define FF_STATIC_OVF_ENTRIES 32
 #define FF_GRID_CAP 4096      // grid-stride cap of the other stage kernels
+#define QSTAGE_CAP 2048       // queue entries a workgroup stages in LDS before appending them
 
 // path state word (beta.w bits; after a shading pass, fext.w)
 #define ST_LEN_MASK 0xffu
